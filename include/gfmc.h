@@ -1,0 +1,216 @@
+/*
+ * gfmc.h -- C-ABI of libgfmc_hip.so, the MI355X (gfx950) GFMC engine.
+ *
+ * Plain C types only: every buffer is a device pointer owned by the caller
+ * (torch tensors' data_ptr() from the Python side), every size is explicit,
+ * and every entry point is stream-ordered on the caller's hipStream_t (passed
+ * as void*; NULL is the default stream).  Entry points return 0 on success
+ * and a negative GFMC_E* code otherwise; gfmc_last_error() then holds a
+ * message (per host thread).
+ *
+ * The reference (PyGFMC, JAX) has no FFI: its seam is Python.  Each entry
+ * point below names the reference function whose job it takes over, and the
+ * Python package pygfmc_amd binds them with ctypes behind the reference's own
+ * API (run / run_gfmc, SpinHalf.create, JastrowSpinHalf, prepare_walkers_sz,
+ * Walker).
+ *
+ * Spin layout in HBM: bit-packed, one walker = n_words = ceil(N/32) uint32
+ * words (walker-major, contiguous), bit (k & 31) of word (k >> 5) set <=> spin
+ * k is +1 (sigma_k = +1, i.e. S^z = +1/2).
+ */
+#ifndef GFMC_H
+#define GFMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFMC_ABI_VERSION 1
+
+/* status codes */
+#define GFMC_OK 0
+#define GFMC_EINVAL -1   /* bad argument (shape, size, pointer)          */
+#define GFMC_EHIP -2     /* a HIP runtime call failed                     */
+#define GFMC_ENOMEM -3   /* the library's own table allocation failed     */
+#define GFMC_EUNSUP -4   /* a combination this build does not implement   */
+
+/* arithmetic of the rates/ratios (weights, times and accumulators are FP64) */
+#define GFMC_FP64 0
+#define GFMC_FP32 1
+
+/* guide kinds */
+#define GFMC_GUIDE_NONE 0
+#define GFMC_GUIDE_JASTROW 1
+#define GFMC_GUIDE_RBM 2
+
+/* Philox stream tags (counter word 3): disjoint streams per purpose */
+#define GFMC_TAG_PROPAGATE 0u
+#define GFMC_TAG_RECONFIG 1u
+#define GFMC_TAG_INIT 2u
+#define GFMC_TAG_WARMUP_PROPAGATE 3u
+#define GFMC_TAG_WARMUP_RECONFIG 4u
+
+/*
+ * Problem descriptor: a SpinHalf Hamiltonian (codes/pygfmc/hamiltonian/
+ * SpinHalf.py:70-142, stored-coefficient convention of create():144-288) and
+ * its guide, lowered to flat host arrays.  The library copies what it needs
+ * to the device at gfmc_problem_create; the caller may free these after.
+ *
+ * Off-diagonal slot layout = SpinHalf.get_rates (SpinHalf.py:316-369) minus
+ * the diagonal: slots [0, n_hx) flip site s, slots [n_hx, n_hx + n_jpm) flip
+ * both ends of edge s - n_hx.
+ */
+typedef struct gfmc_problem_desc {
+    int32_t n_sites;          /* N                                             */
+    int32_t n_edges;          /* Hamiltonian edges                             */
+    int32_t n_hx;             /* 0 or N  (SpinHalf.n_hx_static)                */
+    int32_t n_jpm;            /* 0 or n_edges (SpinHalf.n_jpm_static)          */
+    int32_t has_jz;           /* SpinHalf.has_jz                               */
+    int32_t uniform_jz;       /* SpinHalf.uniform_jz                           */
+    int32_t has_hz;           /* SpinHalf.has_hz                               */
+    int32_t guide;            /* GFMC_GUIDE_*                                  */
+    const int32_t *edge_i;    /* [n_edges]                                     */
+    const int32_t *edge_j;    /* [n_edges]                                     */
+    const double *jz;         /* [n_edges] (uniform: all equal)                */
+    const double *jpm;        /* [n_edges]  J_pm   = (J_x+J_y)/4               */
+    const double *jpmpm;      /* [n_edges]  J_pmpm = (J_x-J_y)/4               */
+    const double *hz;         /* [N] stored -h_z, or NULL when !has_hz         */
+    const double *hx_half;    /* [N] stored -h_x/2, or NULL when n_hx == 0     */
+    /* Jastrow guide (jastrow_spin_half.py:79-157, build_cache:400-459) */
+    int32_t jas_max_deg;
+    const int32_t *jas_site_other; /* [N*max_deg]  partner sites (clipped >= 0)  */
+    const double *jas_site_J;      /* [N*max_deg]  J = -beta, 0 on padding      */
+    const double *jas_h;           /* [N]          one-body field               */
+    const int32_t *jas_edge_other; /* [n_edges*2*max_deg] JastrowOpCache.edge_other */
+    const double *jas_edge_J;      /* [n_edges*2*max_deg] JastrowOpCache.edge_J    */
+    /* RBM guide: log psi = sum_k a_k x_k + sum_h lncosh(b_h + sum_k W_hk x_k) */
+    int32_t rbm_hidden;            /* H                                         */
+    const float *rbm_W;            /* [H*N] row-major (hidden-major)            */
+    const float *rbm_a;            /* [N]                                       */
+    const float *rbm_b;            /* [H]                                       */
+    /* incremental update lists (built by the host, see pygfmc_amd/problem.py):
+     * after move s the slots dep_slots[dep_off[s] : dep_off[s+1]] are
+     * re-evaluated and the edges dep_edges[edge_off[s] : edge_off[s+1]]
+     * change their parallel/anti-parallel state.  dep_off == NULL selects a
+     * full re-evaluation of every slot after every move.                  */
+    const int32_t *dep_off;        /* [n_off+1] or NULL                         */
+    const int32_t *dep_slots;
+    const int32_t *edge_off;       /* [n_off+1] (required when has_jz)          */
+    const int32_t *dep_edges;
+} gfmc_problem_desc;
+
+typedef struct gfmc_problem gfmc_problem;   /* opaque, device tables */
+
+typedef struct gfmc_problem_info {
+    int32_t n_sites, n_words, n_off, m_total;
+    int32_t precision, guide, block_k, full_recompute;
+    int64_t lds_bytes;                          /* per walker-wavefront      */
+} gfmc_problem_info;
+
+const char *gfmc_version(void);
+const char *gfmc_last_error(void);
+
+/* Lower a SpinHalf (+ guide) to device tables.  Replaces the per-call
+ * guide_cache / build_cache step (evolution.py:193-202,
+ * jastrow_spin_half.py:400-459).  precision: GFMC_FP64 / GFMC_FP32. */
+int gfmc_problem_create(const gfmc_problem_desc *desc, int32_t precision,
+                        gfmc_problem **out);
+int gfmc_problem_destroy(gfmc_problem *p);
+int gfmc_problem_get_info(const gfmc_problem *p, gfmc_problem_info *out);
+
+/* Walker initialisers (walker.py:143-306).  _sz: every walker a Philox-driven
+ * Fisher-Yates permutation of n_up up-spins among N sites
+ * (prepare_walkers_sz:257-306).  _random: every site an independent fair bit
+ * (prepare_walkers_random:207-254).  spins: [n_walkers * n_words]. */
+int gfmc_init_walkers_sz(int32_t n_sites, int64_t n_walkers, int32_t n_up,
+                         uint64_t seed, uint32_t *spins, void *stream);
+int gfmc_init_walkers_random(int32_t n_sites, int64_t n_walkers,
+                             uint64_t seed, uint32_t *spins, void *stream);
+
+/* +-1 doubles <-> bit-packed spins (device buffers). */
+int gfmc_pack(int32_t n_sites, int64_t n_walkers, const double *config,
+              uint32_t *spins, void *stream);
+int gfmc_unpack(int32_t n_sites, int64_t n_walkers, const uint32_t *spins,
+                double *config, void *stream);
+
+/* Guide-transformed rates of every walker in the SpinHalf layout, FP64 out
+ * (transition_rates, evolution.py:205-261 with SpinHalf.get_rates and
+ * JastrowSpinHalf.log_diff_from_operator).  rates: [n_walkers * m_total] or
+ * NULL; e_loc: [n_walkers] = diag - sum of off-diagonal weights, or NULL;
+ * log_diff: [n_walkers * m_total] guide log-ratios (0 in slot 0) or NULL. */
+int gfmc_rates(const gfmc_problem *p, int64_t n_walkers, const uint32_t *spins,
+               double *rates, double *e_loc, double *log_diff, void *stream);
+
+/* log psi_T of every walker (JastrowSpinHalf.log_psi,
+ * jastrow_spin_half.py:299-324; RBM: the network's log-amplitude). */
+int gfmc_log_psi(const gfmc_problem *p, int64_t n_walkers, const uint32_t *spins,
+                 double *out, void *stream);
+
+/* One continuous-time branching step of every walker, in place
+ * (evolve_walker_cont_time, evolution.py:559-721, vmapped at :817-843):
+ * hold times ~ Exp(R_x), moves by inverse-CDF over the off-diagonal weights
+ * (sample_index, :30-59), log_w -= e_L * tau per hold, then -= alw * beta.
+ * The uniforms of hop h of walker w are Philox4x32-10 with counter
+ * (h, w, step, tag) and key (seed lo, seed hi): a 53-bit U for the move
+ * and a 53-bit psi for the hold (tau = -log1p(-psi) / R_x).
+ * evals: [n_walkers], or NULL: each walker's loop iterations = rate
+ * evaluations (iter_count of the reference's while_loop) are ADDED to it. */
+int gfmc_propagate(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
+                   double *log_w, double *e_loc, double beta, double alw,
+                   uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals,
+                   void *stream);
+
+/* Systematic-resampling reconfiguration (reconfiguration.py:9-62): FP64
+ * log-sum-exp of log_w, normalised weights, E_n = sum w e_loc, the prefix
+ * sum, parents[i] = min(searchsorted_right(cdf, (i+psi)/n), last positive).
+ * psi is Philox (0, 0, step, tag) unless psi_opt is non-NULL (a host double).
+ * out2 (device, 2 doubles) <- {log G_n - log n, E_n}.  work: device scratch
+ * of n_walkers doubles (the prefix sum), caller-owned. */
+int gfmc_reconfigure(int64_t n_walkers, const double *log_w, const double *e_loc,
+                     uint64_t seed, uint32_t step, uint32_t tag,
+                     const double *psi_opt, int32_t *parents, double *out2,
+                     double *work, void *stream);
+
+/* The gather that follows reconfiguration: spins_out[i] = spins_in[parents[i]],
+ * e_loc_out[i] = e_loc_in[parents[i]], log_w_out[i] = 0 (NULL skips). */
+int gfmc_gather(int32_t n_sites, int64_t n_walkers, const uint32_t *spins_in,
+                const double *e_loc_in, const int32_t *parents,
+                uint32_t *spins_out, double *e_loc_out, double *log_w_out,
+                void *stream);
+
+/* n_steps production (or warm-up) branching steps in one call: per step t,
+ * gfmc_propagate at counter step0 + t with tag_prop, gfmc_reconfigure with
+ * tag_reconf, gfmc_gather (_make_prod_step_fn, driver.py:194-342, without
+ * operators).  The walkers (spins, log_w = 0 after each step, e_loc) are
+ * updated in place; spins_b / e_loc_b are same-sized scratch; parents
+ * [n_walkers] int32 and work [n_walkers] doubles are scratch.
+ * series [2*n_steps] doubles <- {log G_t - log n, E_t} per step.  evals, when
+ * non-NULL, [n_walkers] accumulates the rate evaluations of every step (of
+ * the walker slot, before resampling).  Launches are enqueued on stream;
+ * nothing synchronises. */
+int gfmc_run_steps(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
+                   double *log_w, double *e_loc, uint32_t *spins_b, double *e_loc_b,
+                   int32_t *parents, double *work, int32_t n_steps, double beta,
+                   double alw, uint64_t seed, uint32_t step0, uint32_t tag_prop,
+                   uint32_t tag_reconf, double *series, int32_t *evals, void *stream);
+
+/* Device test hook for the move sampler the propagator uses: for each of
+ * n_draws uniforms u_d, the slot drawn from the non-negative weights
+ * w[0:n] with the propagator's blocked CDF (sample_index, evolution.py:30-59).
+ * weights/uniforms/out are device buffers; precision as in the problem. */
+int gfmc_sample_index(int32_t precision, int32_t n, const double *weights,
+                      int32_t n_draws, const double *uniforms, int32_t *out,
+                      void *stream);
+
+/* Philox4x32-10 block (Salmon et al., SC'11) for the known-answer test:
+ * out[4*i..] = philox(ctr[4*i..], key[0..1]); host pointers. */
+int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key,
+                     uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GFMC_H */

@@ -1,0 +1,237 @@
+"""CPU oracle of the GFMC hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg use this package, and only as the checker / baseline; the product
+(pygfmc_amd) never imports it.  ``liboracle.so`` is built from
+``gfmc_oracle.c`` by ``python -m oracle.build``.
+
+Functions take the descriptor arrays of a lowered problem (a mapping of
+numpy arrays and scalars, as produced by ``pygfmc_amd.problem.lower``) and
+host numpy buffers.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class Desc(ctypes.Structure):
+    """Layout of gfmc_problem_desc (include/gfmc.h)."""
+    _fields_ = [
+        ("n_sites", ctypes.c_int32), ("n_edges", ctypes.c_int32),
+        ("n_hx", ctypes.c_int32), ("n_jpm", ctypes.c_int32),
+        ("has_jz", ctypes.c_int32), ("uniform_jz", ctypes.c_int32),
+        ("has_hz", ctypes.c_int32), ("guide", ctypes.c_int32),
+        ("edge_i", _i32p), ("edge_j", _i32p),
+        ("jz", _f64p), ("jpm", _f64p), ("jpmpm", _f64p), ("hz", _f64p), ("hx_half", _f64p),
+        ("jas_max_deg", ctypes.c_int32),
+        ("jas_site_other", _i32p), ("jas_site_J", _f64p), ("jas_h", _f64p),
+        ("jas_edge_other", _i32p), ("jas_edge_J", _f64p),
+        ("rbm_hidden", ctypes.c_int32),
+        ("rbm_W", _f32p), ("rbm_a", _f32p), ("rbm_b", _f32p),
+        ("dep_off", _i32p), ("dep_slots", _i32p), ("edge_off", _i32p), ("dep_edges", _i32p),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build
+            build.build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        D = ctypes.POINTER(Desc)
+        L.orc_philox.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_u53.restype = ctypes.c_double
+        L.orc_u53.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_blocked_scan.restype = ctypes.c_double
+        L.orc_blocked_scan.argtypes = [_f64p, ctypes.c_int, _f64p]
+        L.orc_sample_many.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _i32p]
+        L.orc_rates_batch.argtypes = [D, ctypes.c_int64, _u32p, _f64p, _f64p, _f64p]
+        L.orc_log_psi_batch.argtypes = [D, ctypes.c_int64, _u32p, _f64p]
+        L.orc_propagate.restype = ctypes.c_int
+        L.orc_propagate.argtypes = [D, ctypes.c_int64, _u32p, _f64p, _f64p, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
+                                    ctypes.c_uint32, _i32p, ctypes.c_int]
+        L.orc_reconfigure.restype = ctypes.c_int
+        L.orc_reconfigure.argtypes = [ctypes.c_int64, _f64p, _f64p, ctypes.c_double, _i32p,
+                                      _f64p, _f64p]
+        L.orc_reconfig_psi.restype = ctypes.c_double
+        L.orc_reconfig_psi.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_gather.argtypes = [ctypes.c_int, ctypes.c_int64, _u32p, _f64p, _i32p, _u32p, _f64p]
+        L.orc_init_sz.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_uint64,
+                                  _u32p]
+        L.orc_init_random.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, _u32p]
+        L.orc_pack.argtypes = [ctypes.c_int, ctypes.c_int64, _f64p, _u32p]
+        L.orc_unpack.argtypes = [ctypes.c_int, ctypes.c_int64, _u32p, _f64p]
+        del vp
+        _LIB = L
+    return _LIB
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+class Problem:
+    """A descriptor built from lowered arrays (keeps them alive)."""
+
+    def __init__(self, lowered):
+        self.lowered = lowered
+        self.arrays = {k: (None if v is None else np.ascontiguousarray(v))
+                       for k, v in lowered.arrays.items()}
+        d = Desc()
+        for k, v in lowered.scalars.items():
+            setattr(d, k, int(v))
+        for k, v in self.arrays.items():
+            if v is None:
+                continue
+            ct = ctypes.c_int32 if v.dtype == np.int32 else ctypes.c_double
+            setattr(d, k, _p(v, ct))
+        self.desc = d
+        self.N = int(lowered.scalars["n_sites"])
+        self.n_off = int(lowered.scalars["n_hx"]) + int(lowered.scalars["n_jpm"])
+        self.W = (self.N + 31) // 32
+
+    def rates(self, spins):
+        spins = np.ascontiguousarray(spins, dtype=np.uint32)
+        nw = spins.shape[0]
+        M = self.n_off + 1
+        r = np.empty((nw, M))
+        e = np.empty(nw)
+        ld = np.empty((nw, M))
+        lib().orc_rates_batch(ctypes.byref(self.desc), nw, _p(spins, ctypes.c_uint32),
+                              _p(r, ctypes.c_double), _p(e, ctypes.c_double),
+                              _p(ld, ctypes.c_double))
+        return r, e, ld
+
+    def log_psi(self, spins):
+        spins = np.ascontiguousarray(spins, dtype=np.uint32)
+        out = np.empty(spins.shape[0])
+        lib().orc_log_psi_batch(ctypes.byref(self.desc), spins.shape[0],
+                                _p(spins, ctypes.c_uint32), _p(out, ctypes.c_double))
+        return out
+
+    def propagate(self, spins, log_w, e_loc, beta, alw, seed, step, tag, n_threads=0):
+        """In place on copies; returns (spins, log_w, e_loc, evals)."""
+        spins = np.array(spins, dtype=np.uint32, copy=True)
+        log_w = np.array(log_w, dtype=np.float64, copy=True)
+        e_loc = np.array(e_loc, dtype=np.float64, copy=True)
+        nw = spins.shape[0]
+        ev = np.zeros(nw, dtype=np.int32)
+        lib().orc_propagate(ctypes.byref(self.desc), nw, _p(spins, ctypes.c_uint32),
+                            _p(log_w, ctypes.c_double), _p(e_loc, ctypes.c_double),
+                            float(beta), float(alw), int(seed), int(step) & 0xFFFFFFFF,
+                            int(tag), _p(ev, ctypes.c_int32), int(n_threads))
+        return spins, log_w, e_loc, ev
+
+
+def philox(ctr, key):
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32)
+    key = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.empty(4, dtype=np.uint32)
+    lib().orc_philox(_p(ctr, ctypes.c_uint32), _p(key, ctypes.c_uint32),
+                     _p(out, ctypes.c_uint32))
+    return out
+
+
+def blocked_scan(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    incl = np.empty(64)
+    tot = lib().orc_blocked_scan(_p(v, ctypes.c_double), len(v), _p(incl, ctypes.c_double))
+    return tot, incl
+
+
+def sample_many(weights, uniforms):
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    u = np.ascontiguousarray(np.atleast_1d(uniforms), dtype=np.float64)
+    out = np.empty(len(u), dtype=np.int32)
+    lib().orc_sample_many(_p(w, ctypes.c_double), len(w), len(u), _p(u, ctypes.c_double),
+                          _p(out, ctypes.c_int32))
+    return out
+
+
+def reconfigure(log_w, e_loc, psi):
+    lw = np.ascontiguousarray(log_w, dtype=np.float64)
+    el = np.ascontiguousarray(e_loc, dtype=np.float64)
+    n = len(lw)
+    parents = np.empty(n, dtype=np.int32)
+    out2 = np.empty(2)
+    cdf = np.empty(n)
+    lib().orc_reconfigure(n, _p(lw, ctypes.c_double), _p(el, ctypes.c_double), float(psi),
+                          _p(parents, ctypes.c_int32), _p(out2, ctypes.c_double),
+                          _p(cdf, ctypes.c_double))
+    return parents, out2[0], out2[1], cdf
+
+
+def reconfig_psi(seed, step, tag=1):
+    return lib().orc_reconfig_psi(int(seed), int(step) & 0xFFFFFFFF, int(tag))
+
+
+def gather(N, spins, e_loc, parents):
+    spins = np.ascontiguousarray(spins, dtype=np.uint32)
+    e = np.ascontiguousarray(e_loc, dtype=np.float64)
+    par = np.ascontiguousarray(parents, dtype=np.int32)
+    so = np.empty_like(spins)
+    eo = np.empty_like(e)
+    lib().orc_gather(int(N), len(par), _p(spins, ctypes.c_uint32), _p(e, ctypes.c_double),
+                     _p(par, ctypes.c_int32), _p(so, ctypes.c_uint32), _p(eo, ctypes.c_double))
+    return so, eo
+
+
+def init_sz(N, nw, n_up, seed):
+    out = np.empty((nw, (N + 31) // 32), dtype=np.uint32)
+    lib().orc_init_sz(int(N), int(nw), int(n_up), int(seed), _p(out, ctypes.c_uint32))
+    return out
+
+
+def init_random(N, nw, seed):
+    out = np.empty((nw, (N + 31) // 32), dtype=np.uint32)
+    lib().orc_init_random(int(N), int(nw), int(seed), _p(out, ctypes.c_uint32))
+    return out
+
+
+def pack(config):
+    c = np.ascontiguousarray(config, dtype=np.float64)
+    nw, N = c.shape
+    out = np.empty((nw, (N + 31) // 32), dtype=np.uint32)
+    lib().orc_pack(N, nw, _p(c, ctypes.c_double), _p(out, ctypes.c_uint32))
+    return out
+
+
+def unpack(spins, N):
+    s = np.ascontiguousarray(spins, dtype=np.uint32)
+    out = np.empty((s.shape[0], N))
+    lib().orc_unpack(int(N), s.shape[0], _p(s, ctypes.c_uint32), _p(out, ctypes.c_double))
+    return out
+
+
+def run_chain(prob, spins, n_steps, beta, alw, seed, step0, warmup=False, n_threads=0):
+    """n_steps branching steps (propagate, reconfigure, gather) with the
+    same counters as the GPU step loop; returns (spins, e_loc, series)."""
+    tp, tr = (3, 4) if warmup else (0, 1)
+    nw = spins.shape[0]
+    lw = np.zeros(nw)
+    e = np.zeros(nw)
+    series = np.empty((n_steps, 2))
+    for t in range(n_steps):
+        spins, lw, e, _ = prob.propagate(spins, lw, e, beta, alw, seed, step0 + t, tp,
+                                         n_threads=n_threads)
+        psi = reconfig_psi(seed, step0 + t, tr)
+        par, lg, En, _ = reconfigure(lw, e, psi)
+        series[t] = (lg, En)
+        spins, e = gather(prob.N, spins, e, par)
+        lw = np.zeros(nw)
+    return spins, e, series

@@ -1,0 +1,23 @@
+"""pygfmc_amd -- continuous-time Green's-function Monte Carlo on AMD MI355X.
+
+The user API follows PyGFMC (``run``/``run_gfmc``, ``SpinHalf.create``,
+``JastrowSpinHalf``, ``prepare_walkers_sz``, ``Walker``); the hot path runs
+in hand-written HIP kernels (``libgfmc_hip.so``, built by
+``python -m pygfmc_amd.build``) with PyTorch tensors as device buffers.
+"""
+from .engine import (DEFAULT_WARMUP_STEPS, evolve_walkers, estimate_average_log_branch_weight,
+                     reconfiguration, run, run_gfmc, sample_index, systematic_parents)
+from .estimators import estimate_energy, evaluate_observables
+from .guides import JastrowSpinHalf
+from .hamiltonian import SpinHalf
+from .lattice import Lattice, chain, kagome, square
+from .walkers import (PhiloxKey, Walker, prepare_walkers_from_configs, prepare_walkers_random,
+                      prepare_walkers_sz)
+
+__version__ = "0.1.0"
+
+__all__ = ["run", "run_gfmc", "evolve_walkers", "reconfiguration", "systematic_parents",
+           "sample_index", "estimate_average_log_branch_weight", "estimate_energy",
+           "evaluate_observables", "SpinHalf", "JastrowSpinHalf", "Lattice", "chain", "square",
+           "kagome", "Walker", "PhiloxKey", "prepare_walkers_sz", "prepare_walkers_random",
+           "prepare_walkers_from_configs", "DEFAULT_WARMUP_STEPS"]

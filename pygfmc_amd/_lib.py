@@ -1,0 +1,144 @@
+"""ctypes binding of ``libgfmc_hip.so`` (the C-ABI of ``include/gfmc.h``).
+
+The library is built in-tree by ``python -m pygfmc_amd.build``.  There is no
+fallback: when the library is missing, or no ROCm GPU is visible, every
+device entry point raises.
+"""
+import ctypes
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgfmc_hip.so")
+
+GFMC_FP64 = 0
+GFMC_FP32 = 1
+GUIDE_NONE = 0
+GUIDE_JASTROW = 1
+GUIDE_RBM = 2
+TAG_PROPAGATE = 0
+TAG_RECONFIG = 1
+TAG_INIT = 2
+TAG_WARMUP_PROPAGATE = 3
+TAG_WARMUP_RECONFIG = 4
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+
+class ProblemDesc(ctypes.Structure):
+    """Mirror of ``gfmc_problem_desc``."""
+    _fields_ = [
+        ("n_sites", ctypes.c_int32), ("n_edges", ctypes.c_int32),
+        ("n_hx", ctypes.c_int32), ("n_jpm", ctypes.c_int32),
+        ("has_jz", ctypes.c_int32), ("uniform_jz", ctypes.c_int32),
+        ("has_hz", ctypes.c_int32), ("guide", ctypes.c_int32),
+        ("edge_i", _i32p), ("edge_j", _i32p),
+        ("jz", _f64p), ("jpm", _f64p), ("jpmpm", _f64p), ("hz", _f64p), ("hx_half", _f64p),
+        ("jas_max_deg", ctypes.c_int32),
+        ("jas_site_other", _i32p), ("jas_site_J", _f64p), ("jas_h", _f64p),
+        ("jas_edge_other", _i32p), ("jas_edge_J", _f64p),
+        ("rbm_hidden", ctypes.c_int32),
+        ("rbm_W", _f32p), ("rbm_a", _f32p), ("rbm_b", _f32p),
+        ("dep_off", _i32p), ("dep_slots", _i32p), ("edge_off", _i32p), ("dep_edges", _i32p),
+    ]
+
+
+class ProblemInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_sites", ctypes.c_int32), ("n_words", ctypes.c_int32),
+        ("n_off", ctypes.c_int32), ("m_total", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("guide", ctypes.c_int32),
+        ("block_k", ctypes.c_int32), ("full_recompute", ctypes.c_int32),
+        ("lds_bytes", ctypes.c_int64),
+    ]
+
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ("gfmc_version", ctypes.c_char_p, []),
+    ("gfmc_last_error", ctypes.c_char_p, []),
+    ("gfmc_problem_create", ctypes.c_int, [ctypes.POINTER(ProblemDesc), ctypes.c_int32,
+                                           ctypes.POINTER(_vp)]),
+    ("gfmc_problem_destroy", ctypes.c_int, [_vp]),
+    ("gfmc_problem_get_info", ctypes.c_int, [_vp, ctypes.POINTER(ProblemInfo)]),
+    ("gfmc_init_walkers_sz", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                            ctypes.c_uint64, _vp, _vp]),
+    ("gfmc_init_walkers_random", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64,
+                                                ctypes.c_uint64, _vp, _vp]),
+    ("gfmc_pack", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp]),
+    ("gfmc_unpack", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp]),
+    ("gfmc_rates", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
+    ("gfmc_log_psi", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
+    ("gfmc_propagate", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
+                                      ctypes.c_uint32, _vp, _vp]),
+    ("gfmc_reconfigure", ctypes.c_int, [ctypes.c_int64, _vp, _vp, ctypes.c_uint64,
+                                        ctypes.c_uint32, ctypes.c_uint32, _f64p, _vp, _vp,
+                                        _vp, _vp]),
+    ("gfmc_gather", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp,
+                                   _vp, _vp]),
+    ("gfmc_run_steps", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, _vp, _vp, _vp]),
+    ("gfmc_sample_index", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32,
+                                         _vp, _vp, _vp]),
+    ("gfmc_philox_host", ctypes.c_int, [ctypes.c_int32, _u32p, _u32p, _u32p]),
+]
+
+_LIB = None
+
+
+def load():
+    """The loaded library (raises when it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m pygfmc_amd.build` "
+                "(hipcc, gfx950). pygfmc_amd has no CPU fallback.")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+class GfmcError(RuntimeError):
+    """A C-ABI call returned an error status."""
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().gfmc_last_error().decode(errors="replace")
+        raise GfmcError(f"{what} failed ({rc}): {msg}")
+
+
+def device():
+    """The GPU the engine runs on (raises when no ROCm GPU is visible)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "pygfmc_amd runs on an AMD GPU (ROCm) and none is visible; there is no "
+            "CPU path.")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream():
+    """Handle of torch's current HIP stream, as the void* the C-ABI takes."""
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a contiguous tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("device buffers must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())

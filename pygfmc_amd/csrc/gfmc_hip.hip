@@ -1,0 +1,1126 @@
+// gfmc_hip.hip -- continuous-time GFMC on MI355X (gfx950): walker
+// propagation, reconfiguration and their helpers behind the C-ABI of
+// include/gfmc.h.
+//
+// Execution model: one 64-lane wavefront per walker.  The walker's spins
+// (one byte per site), its off-diagonal weights and the edge states live in
+// LDS for the whole branching step; HBM holds only the bit-packed spins and
+// the FP64 per-walker scalars.  After a move only the slots whose rate
+// depends on a flipped spin are re-evaluated (host-built dependency lists);
+// every re-evaluated slot is computed from scratch, so it is bit-identical to
+// a full re-enumeration (the reference evaluates every slot at every hop,
+// evolution.py:666-674).
+//
+// Summation order of the off-diagonal weights ("blocked wave order"): slot s
+// belongs to lane s / K with K = ceil(n_off / 64); each lane adds its block
+// sequentially from 0, and the lane sums are combined by a Hillis-Steele
+// inclusive scan.  LDS stores slot s at (s % K) * 64 + s / K so the lane
+// loops are bank-conflict free.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gfmc.h"
+#include "philox.h"
+
+#define WAVE 64
+#define RECONF_THREADS 1024
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return fail(GFMC_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device problem
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Tables {
+    const T *jpm, *jpmpm, *hx_half, *jas_h, *jas_site_J, *jas_edge_J;
+};
+
+struct DevProblem {
+    int N, n_edges, n_hx, n_jpm, n_off;
+    int has_jz, uniform_jz, has_hz, guide;
+    int words, K, Ke, Ks, max_deg, full;
+    int n_pad, e_pad;
+    double jz0;
+    const int *edge_i, *edge_j;
+    const double *jz, *hz;
+    const int *jas_site_other, *jas_edge_other;
+    const int *dep_off, *dep_slots, *edge_off, *dep_edges;
+};
+
+struct gfmc_problem {
+    DevProblem dp;
+    Tables<double> t64;
+    Tables<float> t32;
+    int precision;
+    std::vector<void *> allocs;
+    int64_t lds_bytes;
+};
+
+static int block_k_of(int n) { return n > 0 ? (n + WAVE - 1) / WAVE : 1; }
+static int round_up(int n, int m) { return (n + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        T o = __shfl_up(v, d, WAVE);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ double dev_exp(double v) { return exp(v); }
+__device__ __forceinline__ float dev_exp(float v) { return expf(v); }
+
+// next representable value towards 0 of a positive finite number
+__device__ __forceinline__ double below(double r) {
+    return __longlong_as_double(__double_as_longlong(r) - 1);
+}
+__device__ __forceinline__ float below(float r) { return __int_as_float(__float_as_int(r) - 1); }
+
+template <typename T>
+__device__ __forceinline__ T uniform_for(gfmc_u4 r);
+template <>
+__device__ __forceinline__ double uniform_for<double>(gfmc_u4 r) {
+    return gfmc_u53(r.x, r.y);
+}
+template <>
+__device__ __forceinline__ float uniform_for<float>(gfmc_u4 r) {
+    return gfmc_u24(r.x);
+}
+
+// ---------------------------------------------------------------------------
+// rates of one slot (SpinHalf.get_rates + JastrowSpinHalf ratios)
+// ---------------------------------------------------------------------------
+// Guide log-ratio of slot s (jastrow_spin_half.py:290-297, 462-472).
+template <typename T>
+__device__ __forceinline__ T slot_log_diff(const DevProblem &P, const Tables<T> &tb,
+                                           const signed char *x, int s) {
+    const int D = P.max_deg;
+    if (s < P.n_hx) {
+        T mf = (T)0;
+        for (int q = 0; q < D; ++q)
+            mf += tb.jas_site_J[s * D + q] * (T)x[P.jas_site_other[s * D + q]];
+        T xk = (T)x[s];
+        return ((T)-2 * xk) * mf - ((T)2 * tb.jas_h[s]) * xk;
+    }
+    const int e = s - P.n_hx;
+    const int i = P.edge_i[e], j = P.edge_j[e];
+    const int *oi = P.jas_edge_other + (size_t)e * 2 * D;
+    const T *Ji = tb.jas_edge_J + (size_t)e * 2 * D;
+    T mfi = (T)0, mfj = (T)0;
+    for (int q = 0; q < D; ++q) mfi += Ji[q] * (T)x[oi[q]];
+    for (int q = 0; q < D; ++q) mfj += Ji[D + q] * (T)x[oi[D + q]];
+    T xi = (T)x[i], xj = (T)x[j];
+    T pair = xi * mfi + xj * mfj;
+    T hterm = tb.jas_h[i] * xi + tb.jas_h[j] * xj;
+    return (T)-2 * pair - (T)2 * hterm;
+}
+
+template <typename T>
+__device__ __forceinline__ T slot_bare(const DevProblem &P, const Tables<T> &tb,
+                                       const signed char *x, int s) {
+    if (s < P.n_hx) return tb.hx_half[s];
+    const int e = s - P.n_hx;
+    return (x[P.edge_i[e]] != x[P.edge_j[e]]) ? tb.jpm[e] : tb.jpmpm[e];
+}
+
+// weight = -rate = -(bare * exp(where(bare == 0, 0, log_diff)))
+// (transition_rates, evolution.py:253-261)
+template <typename T>
+__device__ __forceinline__ T slot_weight(const DevProblem &P, const Tables<T> &tb,
+                                         const signed char *x, int s) {
+    T bare = slot_bare(P, tb, x, s);
+    if (bare == (T)0 || P.guide != GFMC_GUIDE_JASTROW) return -bare;
+    return -(bare * dev_exp(slot_log_diff(P, tb, x, s)));
+}
+
+// Diagonal element (SpinHalf.diagonal_weight, SpinHalf.py:291-314).
+__device__ double diag_value(const DevProblem &P, const signed char *x,
+                             const unsigned char *anti, int n_anti, int lane) {
+    double d = 0.0;
+    if (P.has_jz) {
+        if (P.uniform_jz) {
+            d = (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti);
+        } else {
+            double acc = 0.0;
+            for (int k = 0; k < P.Ke; ++k) {
+                int e = lane * P.Ke + k;
+                if (e < P.n_edges) acc += anti[e] ? -P.jz[e] / 4.0 : P.jz[e] / 4.0;
+            }
+            acc = wave_incl_scan(acc, lane);
+            d = __shfl(acc, WAVE - 1, WAVE);
+        }
+    }
+    if (P.has_hz) {
+        double acc = 0.0;
+        for (int k = 0; k < P.Ks; ++k) {
+            int s = lane * P.Ks + k;
+            if (s < P.N) acc += P.hz[s] * (double)x[s];
+        }
+        acc = wave_incl_scan(acc, lane);
+        d = d + __shfl(acc, WAVE - 1, WAVE) / 2;
+    }
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// per-walker LDS state
+// ---------------------------------------------------------------------------
+template <typename T>
+struct WalkerLds {
+    T *w;                 // [K*64], slot s at (s % K) * 64 + s / K
+    signed char *x;       // [n_pad]
+    unsigned char *anti;  // [e_pad]
+};
+
+template <typename T>
+__device__ __forceinline__ WalkerLds<T> carve(const DevProblem &P, unsigned char *lds) {
+    WalkerLds<T> L;
+    L.w = (T *)lds;
+    L.x = (signed char *)(lds + (size_t)P.K * WAVE * sizeof(T));
+    L.anti = (unsigned char *)(L.x + P.n_pad);
+    return L;
+}
+
+__device__ __forceinline__ int wpos(int s, int K) { return (s % K) * WAVE + s / K; }
+
+__device__ __forceinline__ void load_spins(const DevProblem &P, const uint32_t *sp,
+                                           signed char *x, int lane) {
+    for (int k = lane; k < P.N; k += WAVE) x[k] = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1 : -1;
+}
+
+__device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
+                                            const signed char *x, int lane) {
+    for (int q = lane; q < P.words; q += WAVE) {
+        uint32_t v = 0;
+        for (int b = 0; b < 32; ++b) {
+            int k = q * 32 + b;
+            if (k < P.N && x[k] > 0) v |= 1u << b;
+        }
+        sp[q] = v;
+    }
+}
+
+// every slot weight, every edge state; returns the anti-parallel count
+template <typename T>
+__device__ int full_eval(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L, int lane) {
+    for (int k = 0; k < P.K; ++k) {
+        int s = lane * P.K + k;
+        L.w[k * WAVE + lane] = s < P.n_off ? slot_weight(P, tb, L.x, s) : (T)0;
+    }
+    int cnt = 0;
+    if (P.has_jz) {
+        for (int e = lane; e < P.n_edges; e += WAVE) {
+            unsigned char a = L.x[P.edge_i[e]] != L.x[P.edge_j[e]];
+            L.anti[e] = a;
+            cnt += a;
+        }
+    }
+    return wave_sum_int(cnt);
+}
+
+// lane sums (sequential) + Hillis-Steele: returns this lane's inclusive sum
+template <typename T>
+__device__ __forceinline__ T lane_scan(const DevProblem &P, const WalkerLds<T> &L, int lane,
+                                       T *lane_sum) {
+    T acc = (T)0;
+    for (int k = 0; k < P.K; ++k) acc += L.w[k * WAVE + lane];
+    *lane_sum = acc;
+    return wave_incl_scan(acc, lane);
+}
+
+// Inverse-CDF draw (sample_index, evolution.py:30-59) in blocked wave order;
+// u in [0, R).  Mirrors orc_sample_blocked.
+template <typename T>
+__device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T lane_sum, T u,
+                           int lane) {
+    unsigned long long hit = __ballot(incl > u);
+    int j = hit ? __ffsll((unsigned long long)hit) - 1 : WAVE - 1;
+    T prev = __shfl(incl, j > 0 ? j - 1 : 0, WAVE);
+    if (j == 0) prev = (T)0;
+    int idx = -1;
+    if (lane == j) {
+        T c = prev;
+        for (int k = 0; k < P.K; ++k) {
+            int s = j * P.K + k;
+            if (s >= P.n_off) break;
+            c += L.w[k * WAVE + j];
+            if (c > u) {
+                idx = s;
+                break;
+            }
+        }
+    }
+    idx = __shfl(idx, j, WAVE);
+    if (idx < 0) {
+        // rounding left u past lane j: the last positive slot at or before it
+        unsigned long long pos = __ballot(lane_sum > (T)0 && lane <= j);
+        int jj = pos ? 63 - __clzll((long long)pos) : 0;
+        int found = 0;
+        if (lane == jj) {
+            for (int k = P.K - 1; k >= 0; --k) {
+                int s = jj * P.K + k;
+                if (s < P.n_off && L.w[k * WAVE + jj] > (T)0) {
+                    found = s;
+                    break;
+                }
+            }
+        }
+        idx = __shfl(found, jj, WAVE);
+    }
+    return idx;
+}
+
+// flip the sites of move idx and re-evaluate what depends on them; returns
+// the new anti-parallel count
+template <typename T>
+__device__ int apply_move(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L, int idx,
+                          int n_anti, int lane) {
+    if (lane == 0) {
+        if (idx < P.n_hx) {
+            L.x[idx] = (signed char)-L.x[idx];
+        } else {
+            int e = idx - P.n_hx;
+            int a = P.edge_i[e], b = P.edge_j[e];
+            L.x[a] = (signed char)-L.x[a];
+            L.x[b] = (signed char)-L.x[b];
+        }
+    }
+    __syncthreads();
+    if (P.full) {
+        int cnt = full_eval(P, tb, L, lane);
+        __syncthreads();
+        return cnt;
+    }
+    const int o0 = P.dep_off[idx], o1 = P.dep_off[idx + 1];
+    for (int t = o0 + lane; t < o1; t += WAVE) {
+        int s = P.dep_slots[t];
+        L.w[wpos(s, P.K)] = slot_weight(P, tb, L.x, s);
+    }
+    if (P.has_jz) {
+        const int e0 = P.edge_off[idx], e1 = P.edge_off[idx + 1];
+        int delta = 0;
+        for (int t = e0 + lane; t < e1; t += WAVE) {
+            int e = P.dep_edges[t];
+            unsigned char a = L.x[P.edge_i[e]] != L.x[P.edge_j[e]];
+            delta += (int)a - (int)L.anti[e];
+            L.anti[e] = a;
+        }
+        n_anti += wave_sum_int(delta);
+    }
+    __syncthreads();
+    return n_anti;
+}
+
+// ---------------------------------------------------------------------------
+// propagation kernel (evolve_walker_cont_time, evolution.py:559-721)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
+                                                    uint32_t *spins, double *log_w,
+                                                    double *e_loc, double beta, double alw,
+                                                    uint32_t key0, uint32_t key1,
+                                                    uint32_t step, uint32_t tag,
+                                                    int32_t *evals) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x;
+    const int64_t wk = blockIdx.x;
+    if (wk >= nw) return;
+    WalkerLds<T> L = carve<T>(P, lds);
+    uint32_t *sp = spins + wk * P.words;
+    load_spins(P, sp, L.x, lane);
+    __syncthreads();
+    int n_anti = full_eval(P, tb, L, lane);
+    __syncthreads();
+    double diag = diag_value(P, L.x, L.anti, n_anti, lane);
+    double lw = log_w[wk];
+    double t = 0.0, eL = 0.0;
+    uint32_t it = 0;
+    while (t < beta) {
+        T lane_sum;
+        T incl = lane_scan(P, L, lane, &lane_sum);
+        T R = __shfl(incl, WAVE - 1, WAVE);
+        eL = diag - (double)R;
+        gfmc_u4 r = gfmc_philox(it, (uint32_t)wk, step, tag, key0, key1);
+        double psi = gfmc_u53(r.z, r.w);
+        int idx = -1;
+        double tau_raw = INFINITY;
+        if (R > (T)0) {
+            T u = uniform_for<T>(r) * R;
+            T cap = below(R);
+            if (u > cap) u = cap;
+            idx = sample_slot(P, L, incl, lane_sum, u, lane);
+            tau_raw = -log1p(-psi) / (double)R;
+        }
+        double rem = beta - t;
+        double tau = tau_raw < rem ? tau_raw : rem;
+        lw = lw - eL * tau;
+        t = t + tau;
+        ++it;
+        if (tau_raw < rem && idx >= 0) {
+            n_anti = apply_move(P, tb, L, idx, n_anti, lane);
+            diag = diag_value(P, L.x, L.anti, n_anti, lane);
+        }
+    }
+    lw = lw - alw * beta;
+    store_spins(P, sp, L.x, lane);
+    if (lane == 0) {
+        log_w[wk] = lw;
+        e_loc[wk] = eL;
+        if (evals) evals[wk] += (int32_t)it;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// rates / log psi kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(WAVE) void k_rates(DevProblem P, Tables<T> tb, int64_t nw,
+                                                const uint32_t *spins, double *rates,
+                                                double *e_loc, double *log_diff) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x;
+    const int64_t wk = blockIdx.x;
+    if (wk >= nw) return;
+    WalkerLds<T> L = carve<T>(P, lds);
+    load_spins(P, spins + wk * P.words, L.x, lane);
+    __syncthreads();
+    int n_anti = full_eval(P, tb, L, lane);
+    __syncthreads();
+    double diag = diag_value(P, L.x, L.anti, n_anti, lane);
+    T lane_sum;
+    T incl = lane_scan(P, L, lane, &lane_sum);
+    T R = __shfl(incl, WAVE - 1, WAVE);
+    const int64_t M = P.n_off + 1;
+    if (rates) {
+        if (lane == 0) rates[wk * M] = diag;
+        for (int s = lane; s < P.n_off; s += WAVE)
+            rates[wk * M + 1 + s] = -(double)L.w[wpos(s, P.K)];
+    }
+    if (log_diff) {
+        if (lane == 0) log_diff[wk * M] = 0.0;
+        for (int s = lane; s < P.n_off; s += WAVE)
+            log_diff[wk * M + 1 + s] = P.guide == GFMC_GUIDE_JASTROW
+                                           ? (double)slot_log_diff(P, tb, L.x, s)
+                                           : 0.0;
+    }
+    if (e_loc && lane == 0) e_loc[wk] = diag - (double)R;
+}
+
+// log psi = 0.5 * sum_k sum_q J_kq x_k x_other + sum_k h_k x_k (sequential)
+__global__ void k_log_psi(DevProblem P, Tables<double> tb, int64_t nw, const uint32_t *spins,
+                          double *out) {
+    int64_t wk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (wk >= nw) return;
+    const uint32_t *sp = spins + wk * P.words;
+    if (P.guide != GFMC_GUIDE_JASTROW) {
+        out[wk] = 0.0;
+        return;
+    }
+    const int D = P.max_deg;
+    double pair = 0.0, field = 0.0;
+    for (int k = 0; k < P.N; ++k) {
+        double xk = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1.0 : -1.0;
+        for (int q = 0; q < D; ++q) {
+            int o = P.jas_site_other[k * D + q];
+            double xo = ((sp[o >> 5] >> (o & 31)) & 1u) ? 1.0 : -1.0;
+            pair += tb.jas_site_J[k * D + q] * xk * xo;
+        }
+    }
+    for (int k = 0; k < P.N; ++k) {
+        double xk = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1.0 : -1.0;
+        field += tb.jas_h[k] * xk;
+    }
+    out[wk] = 0.5 * pair + field;
+}
+
+// ---------------------------------------------------------------------------
+// initialisers, packing
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WAVE) void k_init_sz(int N, int words, int64_t nw, int n_up,
+                                                  uint32_t key0, uint32_t key1,
+                                                  uint32_t *spins) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    signed char *x = (signed char *)lds;
+    const int lane = threadIdx.x;
+    const int64_t wk = blockIdx.x;
+    if (wk >= nw) return;
+    for (int k = lane; k < N; k += WAVE) x[k] = k < N - n_up ? -1 : 1;
+    __syncthreads();
+    if (lane == 0) {
+        gfmc_u4 r = {0, 0, 0, 0};
+        for (int i = N - 1; i >= 1; --i) {
+            if (i == N - 1 || (i & 3) == 3)
+                r = gfmc_philox((uint32_t)(i >> 2), (uint32_t)wk, 0u, GFMC_TAG_INIT, key0, key1);
+            uint32_t rv = (i & 3) == 0 ? r.x : (i & 3) == 1 ? r.y : (i & 3) == 2 ? r.z : r.w;
+            uint32_t j = (uint32_t)(((uint64_t)rv * (uint64_t)(i + 1)) >> 32);
+            signed char tmp = x[i];
+            x[i] = x[j];
+            x[j] = tmp;
+        }
+    }
+    __syncthreads();
+    DevProblem P;
+    P.N = N;
+    P.words = words;
+    store_spins(P, spins + wk * words, x, lane);
+}
+
+__global__ void k_init_random(int N, int words, int64_t nw, uint32_t key0, uint32_t key1,
+                              uint32_t *spins) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nw * words) return;
+    int64_t wk = g / words;
+    int q = (int)(g % words);
+    gfmc_u4 r = gfmc_philox((uint32_t)(q >> 2), (uint32_t)wk, 1u, GFMC_TAG_INIT, key0, key1);
+    uint32_t v = (q & 3) == 0 ? r.x : (q & 3) == 1 ? r.y : (q & 3) == 2 ? r.z : r.w;
+    int rem = N - 32 * q;
+    if (rem < 32) v &= (1u << rem) - 1u;
+    spins[g] = v;
+}
+
+__global__ void k_pack(int N, int words, int64_t nw, const double *config, uint32_t *spins) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nw * words) return;
+    int64_t wk = g / words;
+    int q = (int)(g % words);
+    uint32_t v = 0;
+    for (int b = 0; b < 32; ++b) {
+        int k = q * 32 + b;
+        if (k < N && config[wk * N + k] > 0) v |= 1u << b;
+    }
+    spins[g] = v;
+}
+
+__global__ void k_unpack(int N, int words, int64_t nw, const uint32_t *spins, double *config) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nw * N) return;
+    int64_t wk = g / N;
+    int k = (int)(g % N);
+    config[g] = ((spins[wk * words + (k >> 5)] >> (k & 31)) & 1u) ? 1.0 : -1.0;
+}
+
+// ---------------------------------------------------------------------------
+// reconfiguration (reconfiguration.py:9-62), one 1024-thread block
+// ---------------------------------------------------------------------------
+// inclusive scan of per-thread values in block order (mirrors block_incl of
+// the oracle): Hillis-Steele per wave, sequential over the 16 wave totals
+__device__ double block_incl(double v, double *sh_tot) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double incl = wave_incl_scan(v, lane);
+    if (lane == 63) sh_tot[wv] = incl;
+    __syncthreads();
+    double P = 0.0;
+    for (int q = 0; q < wv; ++q) P = P + sh_tot[q];
+    __syncthreads();
+    return P + incl;
+}
+
+__device__ double block_max(double v, double *sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        double o = __shfl_xor(v, d, WAVE);
+        v = o > v ? o : v;
+    }
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double m = sh[0];
+    for (int q = 1; q < RECONF_THREADS / WAVE; ++q) m = sh[q] > m ? sh[q] : m;
+    __syncthreads();
+    return m;
+}
+
+// exclusive running max over threads in block order (max is exact, so any
+// tree gives the oracle's sequential running max)
+__device__ double block_excl_max(double v, double *sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double incl = v;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        double o = __shfl_up(incl, d, WAVE);
+        if (lane >= d) incl = o > incl ? o : incl;
+    }
+    double excl = __shfl_up(incl, 1, WAVE);
+    if (lane == 0) excl = -INFINITY;
+    __syncthreads();
+    if (lane == 63) sh[wv] = incl;
+    __syncthreads();
+    double P = -INFINITY;
+    for (int q = 0; q < wv; ++q) P = sh[q] > P ? sh[q] : P;
+    __syncthreads();
+    return P > excl ? P : excl;
+}
+
+__device__ long long block_max_ll(long long v, long long *sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        long long o = __shfl_xor(v, d, WAVE);
+        v = o > v ? o : v;
+    }
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    long long m = sh[0];
+    for (int q = 1; q < RECONF_THREADS / WAVE; ++q) m = sh[q] > m ? sh[q] : m;
+    __syncthreads();
+    return m;
+}
+
+__global__ __launch_bounds__(RECONF_THREADS) void k_reconfigure(
+    int64_t n, const double *log_w, const double *e_loc, uint32_t key0, uint32_t key1,
+    uint32_t step, uint32_t tag, int use_psi, double psi_in, int32_t *parents, double *out2,
+    double *cdf) {
+    __shared__ double sh_tot[RECONF_THREADS / WAVE];
+    __shared__ double sh_m[RECONF_THREADS / WAVE];
+    __shared__ long long sh_l[RECONF_THREADS / WAVE];
+    __shared__ double sh_incl[RECONF_THREADS];
+    const int tid = threadIdx.x;
+    int64_t C = (n + RECONF_THREADS - 1) / RECONF_THREADS;
+    if (C < 1) C = 1;
+    const int64_t k0 = tid * C, k1 = (k0 + C < n) ? k0 + C : n;
+    // max (exact); jax logsumexp replaces a non-finite max by 0
+    double m = -INFINITY;
+    for (int64_t k = k0; k < k1; ++k) m = log_w[k] > m ? log_w[k] : m;
+    m = block_max(m, sh_m);
+    const double m0 = isfinite(m) ? m : 0.0;
+    double acc = 0.0;
+    for (int64_t k = k0; k < k1; ++k) acc += exp(log_w[k] - m0);
+    double S = block_incl(acc, sh_tot);
+    if (tid == RECONF_THREADS - 1) sh_m[0] = S;
+    __syncthreads();
+    const double logG = log(sh_m[0]) + m0;
+    __syncthreads();
+    // E_n = sum w e_loc
+    acc = 0.0;
+    for (int64_t k = k0; k < k1; ++k) acc += e_loc[k] * exp(log_w[k] - logG);
+    double Et = block_incl(acc, sh_tot);
+    if (tid == RECONF_THREADS - 1) sh_m[1] = Et;
+    __syncthreads();
+    const double En = sh_m[1];
+    __syncthreads();
+    // prefix sum of the normalised weights
+    acc = 0.0;
+    long long lastpos = 0;
+    for (int64_t k = k0; k < k1; ++k) {
+        double wk = exp(log_w[k] - logG);
+        acc += wk;
+        if (wk > 0) lastpos = k;
+    }
+    double incl = block_incl(acc, sh_tot);
+    sh_incl[tid] = incl;
+    __syncthreads();
+    double c = tid == 0 ? 0.0 : sh_incl[tid - 1];
+    for (int64_t k = k0; k < k1; ++k) {
+        c = c + exp(log_w[k] - logG);
+        cdf[k] = c;
+    }
+    // running max over chunks (exact): chunk maxima are their last values
+    double cmax = (k1 > k0) ? c : -INFINITY;
+    double prevmax = block_excl_max(cmax, sh_m);
+    for (int64_t k = k0; k < k1; ++k)
+        if (cdf[k] < prevmax) cdf[k] = prevmax;
+    lastpos = block_max_ll(lastpos, sh_l);
+    double psi;
+    if (use_psi) {
+        psi = psi_in;
+    } else {
+        gfmc_u4 r = gfmc_philox(0u, 0u, step, tag, key0, key1);
+        psi = gfmc_u53(r.x, r.y);
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int64_t i = tid; i < n; i += RECONF_THREADS) {
+        double u = ((double)i + psi) / (double)n;
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (cdf[mid] > u) hi = mid;
+            else lo = mid + 1;
+        }
+        parents[i] = (int32_t)(lo < lastpos ? lo : lastpos);
+    }
+    if (tid == 0) {
+        out2[0] = logG - log((double)n);
+        out2[1] = En;
+    }
+}
+
+__global__ void k_gather(int words, int64_t nw, const uint32_t *spins_in, const double *e_in,
+                         const int32_t *parents, uint32_t *spins_out, double *e_out,
+                         double *lw_out) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nw * words) return;
+    int64_t wk = g / words;
+    int q = (int)(g % words);
+    int64_t p = parents[wk];
+    spins_out[g] = spins_in[p * words + q];
+    if (q == 0) {
+        if (e_out) e_out[wk] = e_in[p];
+        if (lw_out) lw_out[wk] = 0.0;
+    }
+}
+
+// sampler test hook: one wave, weights in LDS in the propagator's layout
+template <typename T>
+__global__ __launch_bounds__(WAVE) void k_sample(int n, const double *weights, int n_draws,
+                                                 const double *uniforms, int32_t *out) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    DevProblem P;
+    P.n_off = n;
+    P.K = n > 0 ? (n + WAVE - 1) / WAVE : 1;
+    P.n_hx = 0;
+    P.n_pad = 0;
+    WalkerLds<T> L = carve<T>(P, lds);
+    const int lane = threadIdx.x;
+    for (int k = 0; k < P.K; ++k) {
+        int s = lane * P.K + k;
+        L.w[k * WAVE + lane] = s < n ? (T)weights[s] : (T)0;
+    }
+    __syncthreads();
+    T lane_sum;
+    T incl = lane_scan(P, L, lane, &lane_sum);
+    T R = __shfl(incl, WAVE - 1, WAVE);
+    for (int d = 0; d < n_draws; ++d) {
+        T u = (T)uniforms[d] * R;
+        T cap = below(R);
+        if (u > cap) u = cap;
+        int idx = sample_slot(P, L, incl, lane_sum, u, lane);
+        if (lane == 0) out[d] = idx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+template <typename V>
+static int upload(gfmc_problem *p, const V *src, size_t n, const V **dst) {
+    if (n == 0 || src == nullptr) {
+        *dst = nullptr;
+        return 0;
+    }
+    void *d = nullptr;
+    HIP_TRY(hipMalloc(&d, n * sizeof(V)));
+    p->allocs.push_back(d);
+    HIP_TRY(hipMemcpy(d, src, n * sizeof(V), hipMemcpyHostToDevice));
+    *dst = (const V *)d;
+    return 0;
+}
+
+static int upload_f32(gfmc_problem *p, const double *src, size_t n, const float **dst) {
+    if (n == 0 || src == nullptr) {
+        *dst = nullptr;
+        return 0;
+    }
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+    return upload(p, tmp.data(), n, dst);
+}
+
+static void free_problem(gfmc_problem *p) {
+    for (void *a : p->allocs) (void)hipFree(a);
+    delete p;
+}
+
+extern "C" {
+
+const char *gfmc_version(void) { return "pygfmc_amd gfmc_hip 0.1 (gfx950)"; }
+const char *gfmc_last_error(void) { return g_err.c_str(); }
+
+int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_problem **out) {
+    if (!d || !out) return fail(GFMC_EINVAL, "gfmc_problem_create: NULL argument");
+    if (precision != GFMC_FP64 && precision != GFMC_FP32)
+        return fail(GFMC_EINVAL, "gfmc_problem_create: precision must be GFMC_FP64 or GFMC_FP32");
+    if (d->n_sites <= 0) return fail(GFMC_EINVAL, "gfmc_problem_create: n_sites must be > 0");
+    if (d->n_edges < 0 || (d->n_edges > 0 && (!d->edge_i || !d->edge_j)))
+        return fail(GFMC_EINVAL, "gfmc_problem_create: bad edge list");
+    if (!(d->n_hx == 0 || d->n_hx == d->n_sites))
+        return fail(GFMC_EINVAL, "gfmc_problem_create: n_hx must be 0 or n_sites");
+    if (!(d->n_jpm == 0 || d->n_jpm == d->n_edges))
+        return fail(GFMC_EINVAL, "gfmc_problem_create: n_jpm must be 0 or n_edges");
+    if (d->guide == GFMC_GUIDE_RBM)
+        return fail(GFMC_EUNSUP, "gfmc_problem_create: the RBM guide uses gfmc_rbm_* entry points");
+    if (d->guide != GFMC_GUIDE_NONE && d->guide != GFMC_GUIDE_JASTROW)
+        return fail(GFMC_EINVAL, "gfmc_problem_create: unknown guide kind");
+    const int N = d->n_sites, E = d->n_edges, n_off = d->n_hx + d->n_jpm;
+    if (d->guide == GFMC_GUIDE_JASTROW &&
+        (d->jas_max_deg <= 0 || !d->jas_site_other || !d->jas_site_J || !d->jas_h ||
+         (E > 0 && (!d->jas_edge_other || !d->jas_edge_J))))
+        return fail(GFMC_EINVAL, "gfmc_problem_create: incomplete Jastrow tables");
+    if (d->has_jz && (!d->jz || (!d->dep_off ? false : !d->edge_off)))
+        return fail(GFMC_EINVAL, "gfmc_problem_create: J_z tables missing");
+    for (int e = 0; e < E; ++e)
+        if (d->edge_i[e] < 0 || d->edge_i[e] >= N || d->edge_j[e] < 0 || d->edge_j[e] >= N)
+            return fail(GFMC_EINVAL, "gfmc_problem_create: edge index out of range");
+    if (d->guide == GFMC_GUIDE_JASTROW) {
+        const int D = d->jas_max_deg;
+        for (int i = 0; i < N * D; ++i)
+            if (d->jas_site_other[i] < 0 || d->jas_site_other[i] >= N)
+                return fail(GFMC_EINVAL, "gfmc_problem_create: Jastrow partner out of range");
+        for (int i = 0; i < E * 2 * D; ++i)
+            if (d->jas_edge_other[i] < 0 || d->jas_edge_other[i] >= N)
+                return fail(GFMC_EINVAL, "gfmc_problem_create: Jastrow edge partner out of range");
+    }
+    if (d->dep_off) {
+        if (d->dep_off[0] != 0) return fail(GFMC_EINVAL, "gfmc_problem_create: dep_off[0] != 0");
+        for (int s = 0; s < n_off; ++s) {
+            if (d->dep_off[s + 1] < d->dep_off[s])
+                return fail(GFMC_EINVAL, "gfmc_problem_create: dep_off not monotone");
+            for (int t = d->dep_off[s]; t < d->dep_off[s + 1]; ++t)
+                if (d->dep_slots[t] < 0 || d->dep_slots[t] >= n_off)
+                    return fail(GFMC_EINVAL, "gfmc_problem_create: dependent slot out of range");
+            if (d->has_jz) {
+                if (d->edge_off[s + 1] < d->edge_off[s])
+                    return fail(GFMC_EINVAL, "gfmc_problem_create: edge_off not monotone");
+                for (int t = d->edge_off[s]; t < d->edge_off[s + 1]; ++t)
+                    if (d->dep_edges[t] < 0 || d->dep_edges[t] >= E)
+                        return fail(GFMC_EINVAL, "gfmc_problem_create: dependent edge out of range");
+            }
+        }
+    }
+
+    gfmc_problem *p = new gfmc_problem();
+    p->precision = precision;
+    DevProblem &P = p->dp;
+    memset(&P, 0, sizeof P);
+    P.N = N;
+    P.n_edges = E;
+    P.n_hx = d->n_hx;
+    P.n_jpm = d->n_jpm;
+    P.n_off = n_off;
+    P.has_jz = d->has_jz;
+    P.uniform_jz = d->uniform_jz;
+    P.has_hz = d->has_hz;
+    P.guide = d->guide;
+    P.words = (N + 31) / 32;
+    P.K = block_k_of(n_off);
+    P.Ke = block_k_of(E);
+    P.Ks = block_k_of(N);
+    P.max_deg = d->guide == GFMC_GUIDE_JASTROW ? d->jas_max_deg : 0;
+    P.full = d->dep_off == nullptr;
+    P.n_pad = round_up(N, 16);
+    P.e_pad = round_up(E > 0 ? E : 1, 16);
+    P.jz0 = (d->has_jz && E > 0) ? d->jz[0] : 0.0;
+    const size_t D = (size_t)P.max_deg;
+    int rc = 0;
+#define UP(src, n, dst)                              \
+    do {                                             \
+        if ((rc = upload(p, src, n, dst)) != 0) {    \
+            free_problem(p);                         \
+            return rc;                               \
+        }                                            \
+    } while (0)
+#define UP32(src, n, dst)                            \
+    do {                                             \
+        if ((rc = upload_f32(p, src, n, dst)) != 0) { \
+            free_problem(p);                         \
+            return rc;                               \
+        }                                            \
+    } while (0)
+    UP(d->edge_i, (size_t)E, &P.edge_i);
+    UP(d->edge_j, (size_t)E, &P.edge_j);
+    if (d->has_jz) UP(d->jz, (size_t)E, &P.jz);
+    if (d->has_hz) UP(d->hz, (size_t)N, &P.hz);
+    if (d->guide == GFMC_GUIDE_JASTROW) {
+        UP(d->jas_site_other, (size_t)N * D, &P.jas_site_other);
+        UP(d->jas_edge_other, (size_t)E * 2 * D, &P.jas_edge_other);
+    }
+    if (d->dep_off) {
+        UP(d->dep_off, (size_t)n_off + 1, &P.dep_off);
+        UP(d->dep_slots, (size_t)d->dep_off[n_off], &P.dep_slots);
+        if (d->has_jz) {
+            UP(d->edge_off, (size_t)n_off + 1, &P.edge_off);
+            UP(d->dep_edges, (size_t)d->edge_off[n_off], &P.dep_edges);
+        }
+    }
+    // coupling tables in both precisions
+    UP(d->jpm, (size_t)E, &p->t64.jpm);
+    UP(d->jpmpm, (size_t)E, &p->t64.jpmpm);
+    UP32(d->jpm, (size_t)E, &p->t32.jpm);
+    UP32(d->jpmpm, (size_t)E, &p->t32.jpmpm);
+    if (d->n_hx) {
+        UP(d->hx_half, (size_t)N, &p->t64.hx_half);
+        UP32(d->hx_half, (size_t)N, &p->t32.hx_half);
+    }
+    if (d->guide == GFMC_GUIDE_JASTROW) {
+        UP(d->jas_h, (size_t)N, &p->t64.jas_h);
+        UP32(d->jas_h, (size_t)N, &p->t32.jas_h);
+        UP(d->jas_site_J, (size_t)N * D, &p->t64.jas_site_J);
+        UP32(d->jas_site_J, (size_t)N * D, &p->t32.jas_site_J);
+        UP(d->jas_edge_J, (size_t)E * 2 * D, &p->t64.jas_edge_J);
+        UP32(d->jas_edge_J, (size_t)E * 2 * D, &p->t32.jas_edge_J);
+    }
+#undef UP
+#undef UP32
+    const size_t wbytes = (size_t)P.K * WAVE * (precision == GFMC_FP64 ? 8 : 4);
+    p->lds_bytes = (int64_t)(wbytes + P.n_pad + P.e_pad);
+    if (p->lds_bytes > 64 * 1024) {
+        free_problem(p);
+        return fail(GFMC_EUNSUP,
+                    "gfmc_problem_create: walker state exceeds 64 KiB of LDS per wavefront");
+    }
+    *out = p;
+    return 0;
+}
+
+int gfmc_problem_destroy(gfmc_problem *p) {
+    if (p) free_problem(p);
+    return 0;
+}
+
+int gfmc_problem_get_info(const gfmc_problem *p, gfmc_problem_info *out) {
+    if (!p || !out) return fail(GFMC_EINVAL, "gfmc_problem_get_info: NULL argument");
+    out->n_sites = p->dp.N;
+    out->n_words = p->dp.words;
+    out->n_off = p->dp.n_off;
+    out->m_total = p->dp.n_off + 1;
+    out->precision = p->precision;
+    out->guide = p->dp.guide;
+    out->block_k = p->dp.K;
+    out->full_recompute = p->dp.full;
+    out->lds_bytes = p->lds_bytes;
+    return 0;
+}
+
+static unsigned grid1d(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+int gfmc_init_walkers_sz(int32_t n_sites, int64_t nw, int32_t n_up, uint64_t seed,
+                         uint32_t *spins, void *stream) {
+    if (n_sites <= 0 || nw < 0 || n_up < 0 || n_up > n_sites || (nw > 0 && !spins))
+        return fail(GFMC_EINVAL, "gfmc_init_walkers_sz: bad argument");
+    if (nw == 0) return 0;
+    if (nw > 0x7fffffffLL) return fail(GFMC_EINVAL, "gfmc_init_walkers_sz: too many walkers");
+    const int words = (n_sites + 31) / 32;
+    size_t lds = (size_t)round_up(n_sites, 16);
+    hipLaunchKernelGGL(k_init_sz, dim3((unsigned)nw), dim3(WAVE), lds, (hipStream_t)stream,
+                       (int)n_sites, words, nw, (int)n_up, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), spins);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_init_walkers_random(int32_t n_sites, int64_t nw, uint64_t seed, uint32_t *spins,
+                             void *stream) {
+    if (n_sites <= 0 || nw < 0 || (nw > 0 && !spins))
+        return fail(GFMC_EINVAL, "gfmc_init_walkers_random: bad argument");
+    if (nw == 0) return 0;
+    const int words = (n_sites + 31) / 32;
+    hipLaunchKernelGGL(k_init_random, dim3(grid1d(nw * words, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (int)n_sites, words, nw, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), spins);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_pack(int32_t n_sites, int64_t nw, const double *config, uint32_t *spins, void *stream) {
+    if (n_sites <= 0 || nw < 0 || (nw > 0 && (!config || !spins)))
+        return fail(GFMC_EINVAL, "gfmc_pack: bad argument");
+    if (nw == 0) return 0;
+    const int words = (n_sites + 31) / 32;
+    hipLaunchKernelGGL(k_pack, dim3(grid1d(nw * words, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (int)n_sites, words, nw, config, spins);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_unpack(int32_t n_sites, int64_t nw, const uint32_t *spins, double *config,
+                void *stream) {
+    if (n_sites <= 0 || nw < 0 || (nw > 0 && (!config || !spins)))
+        return fail(GFMC_EINVAL, "gfmc_unpack: bad argument");
+    if (nw == 0) return 0;
+    const int words = (n_sites + 31) / 32;
+    hipLaunchKernelGGL(k_unpack, dim3(grid1d(nw * n_sites, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (int)n_sites, words, nw, spins, config);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int check_walkers(const char *fn, const gfmc_problem *p, int64_t nw) {
+    if (!p) return fail(GFMC_EINVAL, std::string(fn) + ": NULL problem");
+    if (nw < 0 || nw > 0x7fffffffLL)
+        return fail(GFMC_EINVAL, std::string(fn) + ": n_walkers out of range");
+    return 0;
+}
+
+int gfmc_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double *rates,
+               double *e_loc, double *log_diff, void *stream) {
+    int rc = check_walkers("gfmc_rates", p, nw);
+    if (rc) return rc;
+    if (nw == 0) return 0;
+    if (!spins) return fail(GFMC_EINVAL, "gfmc_rates: NULL spins");
+    if (p->precision == GFMC_FP64)
+        hipLaunchKernelGGL(k_rates<double>, dim3((unsigned)nw), dim3(WAVE), (size_t)p->lds_bytes,
+                           (hipStream_t)stream, p->dp, p->t64, nw, spins, rates, e_loc, log_diff);
+    else
+        hipLaunchKernelGGL(k_rates<float>, dim3((unsigned)nw), dim3(WAVE), (size_t)p->lds_bytes,
+                           (hipStream_t)stream, p->dp, p->t32, nw, spins, rates, e_loc, log_diff);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_log_psi(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double *out,
+                 void *stream) {
+    int rc = check_walkers("gfmc_log_psi", p, nw);
+    if (rc) return rc;
+    if (nw == 0) return 0;
+    if (!spins || !out) return fail(GFMC_EINVAL, "gfmc_log_psi: NULL buffer");
+    hipLaunchKernelGGL(k_log_psi, dim3(grid1d(nw, 64)), dim3(64), 0, (hipStream_t)stream, p->dp,
+                       p->t64, nw, spins, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                   double *e_loc, double beta, double alw, uint64_t seed, uint32_t step,
+                   uint32_t tag, int32_t *evals, void *stream) {
+    int rc = check_walkers("gfmc_propagate", p, nw);
+    if (rc) return rc;
+    if (nw == 0) return 0;
+    if (!spins || !log_w || !e_loc) return fail(GFMC_EINVAL, "gfmc_propagate: NULL buffer");
+    if (!(beta > 0) || !std::isfinite(beta))
+        return fail(GFMC_EINVAL, "gfmc_propagate: beta must be finite and > 0");
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    if (p->precision == GFMC_FP64)
+        hipLaunchKernelGGL(k_propagate<double>, dim3((unsigned)nw), dim3(WAVE),
+                           (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t64, nw, spins,
+                           log_w, e_loc, beta, alw, k0, k1, step, tag, evals);
+    else
+        hipLaunchKernelGGL(k_propagate<float>, dim3((unsigned)nw), dim3(WAVE),
+                           (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
+                           log_w, e_loc, beta, alw, k0, k1, step, tag, evals);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_reconfigure(int64_t nw, const double *log_w, const double *e_loc, uint64_t seed,
+                     uint32_t step, uint32_t tag, const double *psi_opt, int32_t *parents,
+                     double *out2, double *work, void *stream) {
+    if (nw <= 0 || nw > 0x7fffffffLL)
+        return fail(GFMC_EINVAL, "gfmc_reconfigure: n_walkers must be in [1, 2^31)");
+    if (!log_w || !e_loc || !parents || !out2 || !work)
+        return fail(GFMC_EINVAL, "gfmc_reconfigure: NULL buffer");
+    double psi = psi_opt ? *psi_opt : 0.0;
+    if (psi_opt && !(psi >= 0.0 && psi < 1.0))
+        return fail(GFMC_EINVAL, "gfmc_reconfigure: psi must lie in [0, 1)");
+    hipLaunchKernelGGL(k_reconfigure, dim3(1), dim3(RECONF_THREADS), 0, (hipStream_t)stream, nw,
+                       log_w, e_loc, (uint32_t)seed, (uint32_t)(seed >> 32), step, tag,
+                       psi_opt ? 1 : 0, psi, parents, out2, work);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_gather(int32_t n_sites, int64_t nw, const uint32_t *spins_in, const double *e_loc_in,
+                const int32_t *parents, uint32_t *spins_out, double *e_loc_out,
+                double *log_w_out, void *stream) {
+    if (n_sites <= 0 || nw < 0) return fail(GFMC_EINVAL, "gfmc_gather: bad size");
+    if (nw == 0) return 0;
+    if (!spins_in || !parents || !spins_out || (e_loc_out && !e_loc_in))
+        return fail(GFMC_EINVAL, "gfmc_gather: NULL buffer");
+    const int words = (n_sites + 31) / 32;
+    hipLaunchKernelGGL(k_gather, dim3(grid1d(nw * words, 256)), dim3(256), 0,
+                       (hipStream_t)stream, words, nw, spins_in, e_loc_in, parents, spins_out,
+                       e_loc_out, log_w_out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_run_steps(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                   double *e_loc, uint32_t *spins_b, double *e_loc_b, int32_t *parents,
+                   double *work, int32_t n_steps, double beta, double alw, uint64_t seed,
+                   uint32_t step0, uint32_t tag_prop, uint32_t tag_reconf, double *series,
+                   int32_t *evals, void *stream) {
+    int rc = check_walkers("gfmc_run_steps", p, nw);
+    if (rc) return rc;
+    if (nw == 0 || n_steps < 0) return fail(GFMC_EINVAL, "gfmc_run_steps: bad size");
+    if (!spins || !log_w || !e_loc || !spins_b || !e_loc_b || !parents || !work ||
+        (n_steps > 0 && !series))
+        return fail(GFMC_EINVAL, "gfmc_run_steps: NULL buffer");
+    const int words = p->dp.words;
+    uint32_t *sa = spins, *sb = spins_b;
+    double *ea = e_loc, *eb = e_loc_b;
+    for (int32_t t = 0; t < n_steps; ++t) {
+        rc = gfmc_propagate(p, nw, sa, log_w, ea, beta, alw, seed, step0 + (uint32_t)t,
+                            tag_prop, evals, stream);
+        if (rc) return rc;
+        rc = gfmc_reconfigure(nw, log_w, ea, seed, step0 + (uint32_t)t, tag_reconf, nullptr,
+                              parents, series + 2 * (int64_t)t, work, stream);
+        if (rc) return rc;
+        rc = gfmc_gather(p->dp.N, nw, sa, ea, parents, sb, eb, log_w, stream);
+        if (rc) return rc;
+        uint32_t *ts = sa;
+        sa = sb;
+        sb = ts;
+        double *te = ea;
+        ea = eb;
+        eb = te;
+    }
+    if (sa != spins) {
+        HIP_TRY(hipMemcpyAsync(spins, sa, (size_t)nw * words * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, (hipStream_t)stream));
+        HIP_TRY(hipMemcpyAsync(e_loc, ea, (size_t)nw * sizeof(double), hipMemcpyDeviceToDevice,
+                               (hipStream_t)stream));
+    }
+    return 0;
+}
+
+int gfmc_sample_index(int32_t precision, int32_t n, const double *weights, int32_t n_draws,
+                      const double *uniforms, int32_t *out, void *stream) {
+    if (n <= 0 || n_draws < 0 || !weights || (n_draws > 0 && (!uniforms || !out)))
+        return fail(GFMC_EINVAL, "gfmc_sample_index: bad argument");
+    if (n_draws == 0) return 0;
+    const int K = (n + WAVE - 1) / WAVE;
+    if (precision == GFMC_FP64)
+        hipLaunchKernelGGL(k_sample<double>, dim3(1), dim3(WAVE), (size_t)K * WAVE * 8,
+                           (hipStream_t)stream, n, weights, n_draws, uniforms, out);
+    else
+        hipLaunchKernelGGL(k_sample<float>, dim3(1), dim3(WAVE), (size_t)K * WAVE * 4,
+                           (hipStream_t)stream, n, weights, n_draws, uniforms, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+    if (n < 0 || (n > 0 && (!ctr || !key || !out)))
+        return fail(GFMC_EINVAL, "gfmc_philox_host: bad argument");
+    for (int i = 0; i < n; ++i) {
+        gfmc_u4 r = gfmc_philox(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3],
+                                key[0], key[1]);
+        out[4 * i] = r.x;
+        out[4 * i + 1] = r.y;
+        out[4 * i + 2] = r.z;
+        out[4 * i + 3] = r.w;
+    }
+    return 0;
+}
+
+}  // extern "C"

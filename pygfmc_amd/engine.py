@@ -1,0 +1,369 @@
+"""The GFMC driver: ``run`` / ``run_gfmc``, ``evolve_walkers``,
+``reconfiguration`` (codes/pygfmc/engine/driver.py, evolution.py,
+walkers/reconfiguration.py), driving the HIP kernels.
+
+Method: continuous-time GFMC only (``method="cont_time"``, evolution.py:
+559-721).  The discrete ``single_step`` / ``multi_step`` propagators and the
+diagonal / off-diagonal forward-walking observables are not implemented and
+raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .estimators import (check_population, estimate_energy, validated_int,
+                         validated_p_array)
+from .problem import get_problem
+from .walkers import PhiloxKey, Walker, as_key
+
+__all__ = ["run", "run_gfmc", "evolve_walkers", "reconfiguration", "systematic_parents",
+           "sample_index", "estimate_average_log_branch_weight", "DEFAULT_WARMUP_STEPS",
+           "Chain"]
+
+DEFAULT_WARMUP_STEPS = 1000
+
+
+def require_step_size(method, step_size):
+    """evolution.py:744-758."""
+    if not math.isfinite(float(step_size)):
+        raise ValueError(f"step_size must be finite; got step_size={step_size!r}")
+    if not float(step_size) > 0:
+        raise ValueError(f"step_size must be > 0; got step_size={step_size!r}")
+    if method != "cont_time" and float(int(step_size)) != float(step_size):
+        raise ValueError(f"method={method!r} requires an integer step_size; "
+                         f"got step_size={step_size!r}")
+
+
+def require_method(method):
+    if method == "cont_time":
+        return
+    if method in ("single_step", "multi_step"):
+        raise NotImplementedError(
+            f"method={method!r}: pygfmc_amd implements the continuous-time propagator "
+            "(method='cont_time') only.")
+    raise ValueError(f"Unknown method: {method!r}")
+
+
+def warmup_schedule(step_f, warmup_steps):
+    """(step size, count) of the four warm-up ramp stages beta/8, beta/4,
+    beta/2, beta with counts divmod(warmup_steps, 4) (driver.py:567-584)."""
+    if warmup_steps <= 0:
+        return []
+    base, extra = divmod(warmup_steps, 4)
+    counts = [base + (i < extra) for i in range(4)]
+    sizes = [step_f / 8.0, step_f / 4.0, step_f / 2.0, step_f]
+    return [(s, c) for s, c in zip(sizes, counts) if c > 0]
+
+
+class Chain:
+    """Device buffers of a walker population for repeated branching steps."""
+
+    def __init__(self, problem, walkers):
+        if walkers.n_sites != problem.n_sites:
+            raise ValueError(f"walkers have {walkers.n_sites} sites, the Hamiltonian "
+                             f"{problem.n_sites}")
+        self.p = problem
+        self.nw = walkers.n_walkers
+        dev = walkers.spins.device
+        self.spins = walkers.spins.clone()
+        self.log_w = walkers.log_weight_dev.clone()
+        self.e_loc = walkers.e_loc_dev.clone()
+        self.spins_b = torch.empty_like(self.spins)
+        self.e_loc_b = torch.empty_like(self.e_loc)
+        self.parents = torch.empty(self.nw, dtype=torch.int32, device=dev)
+        self.work = torch.empty(self.nw, dtype=torch.float64, device=dev)
+        self.evals = torch.zeros(self.nw, dtype=torch.int32, device=dev)
+
+    def steps(self, n, beta, alw, seed, step0, warmup=False, series=None, evals=False):
+        """Enqueue ``n`` branching steps; returns the (n, 2) device series."""
+        if series is None:
+            series = torch.empty((n, 2), dtype=torch.float64, device=self.spins.device)
+        tp = _lib.TAG_WARMUP_PROPAGATE if warmup else _lib.TAG_PROPAGATE
+        tr = _lib.TAG_WARMUP_RECONFIG if warmup else _lib.TAG_RECONFIG
+        _lib.check(_lib.load().gfmc_run_steps(
+            self.p.handle, self.nw, _lib.ptr(self.spins), _lib.ptr(self.log_w),
+            _lib.ptr(self.e_loc), _lib.ptr(self.spins_b), _lib.ptr(self.e_loc_b),
+            _lib.ptr(self.parents), _lib.ptr(self.work), int(n), float(beta), float(alw),
+            int(seed) & 0xFFFFFFFFFFFFFFFF, int(step0) & 0xFFFFFFFF, tp, tr,
+            _lib.ptr(series), _lib.ptr(self.evals) if evals else None, _lib.stream()),
+            "gfmc_run_steps")
+        return series
+
+    def step(self, beta, alw, seed, step, out2, events=None, evals=True):
+        """One production step as three separate launches (propagate,
+        reconfigure, gather) on torch's current stream; ``events`` (start,
+        end) bracket the propagation kernel.  ``out2``: 2-double device row."""
+        lib = _lib.load()
+        st = _lib.stream()
+        if events is not None:
+            events[0].record()
+        self.p.propagate(self.spins, self.log_w, self.e_loc, beta, alw, seed, step,
+                         _lib.TAG_PROPAGATE, self.evals if evals else None)
+        if events is not None:
+            events[1].record()
+        _lib.check(lib.gfmc_reconfigure(self.nw, _lib.ptr(self.log_w), _lib.ptr(self.e_loc),
+                                        int(seed) & 0xFFFFFFFFFFFFFFFF, int(step) & 0xFFFFFFFF,
+                                        _lib.TAG_RECONFIG, None, _lib.ptr(self.parents),
+                                        _lib.ptr(out2), _lib.ptr(self.work), st),
+                   "gfmc_reconfigure")
+        _lib.check(lib.gfmc_gather(self.p.n_sites, self.nw, _lib.ptr(self.spins),
+                                   _lib.ptr(self.e_loc), _lib.ptr(self.parents),
+                                   _lib.ptr(self.spins_b), _lib.ptr(self.e_loc_b),
+                                   _lib.ptr(self.log_w), st), "gfmc_gather")
+        self.spins, self.spins_b = self.spins_b, self.spins
+        self.e_loc, self.e_loc_b = self.e_loc_b, self.e_loc
+
+    def walkers(self):
+        return Walker.from_device(self.spins.clone(), self.log_w.clone(), self.e_loc.clone(),
+                                  self.p.n_sites)
+
+
+# -- single-call API -----------------------------------------------------------
+
+def evolve_walkers(walkers, hamiltonian, step_size, guide, method, rng_key,
+                   avg_log_weight, lam=None, precision=None, return_evals=False):
+    """One branching step of every walker without reconfiguration
+    (evolve_walkers, evolution.py:761-814).  ``rng_key`` (seed, position)
+    selects the Philox stream: counter = (hop, walker, position, PROPAGATE)."""
+    require_method(method)
+    require_step_size(method, step_size)
+    key = as_key(rng_key)
+    p = get_problem(hamiltonian, guide, precision)
+    w = walkers.clone()
+    evals = torch.zeros(w.n_walkers, dtype=torch.int32, device=w.spins.device)
+    p.propagate(w.spins, w.log_weight_dev, w.e_loc_dev, float(step_size),
+                float(avg_log_weight), key.seed, key.position, _lib.TAG_PROPAGATE, evals)
+    return (w, evals) if return_evals else w
+
+
+def reconfiguration(walkers, rng_key, psi=None):
+    """Systematic resampling (reconfiguration.py:27-62).  Returns
+    ``(walkers, log_G_n - log n, E_n, indices)``; ``psi`` overrides the
+    Philox offset (counter (0, 0, position, RECONFIG))."""
+    key = as_key(rng_key)
+    lib = _lib.load()
+    nw = walkers.n_walkers
+    dev = walkers.spins.device
+    parents = torch.empty(nw, dtype=torch.int32, device=dev)
+    out2 = torch.empty(2, dtype=torch.float64, device=dev)
+    work = torch.empty(nw, dtype=torch.float64, device=dev)
+    psi_c = ctypes.c_double(float(psi)) if psi is not None else None
+    _lib.check(lib.gfmc_reconfigure(
+        nw, _lib.ptr(walkers.log_weight_dev), _lib.ptr(walkers.e_loc_dev), key.seed,
+        key.position & 0xFFFFFFFF, _lib.TAG_RECONFIG,
+        ctypes.byref(psi_c) if psi_c is not None else None,
+        _lib.ptr(parents), _lib.ptr(out2), _lib.ptr(work), _lib.stream()), "gfmc_reconfigure")
+    spins = torch.empty_like(walkers.spins)
+    e = torch.empty_like(walkers.e_loc_dev)
+    lw = torch.empty_like(walkers.log_weight_dev)
+    _lib.check(lib.gfmc_gather(walkers.n_sites, nw, _lib.ptr(walkers.spins),
+                               _lib.ptr(walkers.e_loc_dev), _lib.ptr(parents), _lib.ptr(spins),
+                               _lib.ptr(e), _lib.ptr(lw), _lib.stream()), "gfmc_gather")
+    o = out2.cpu().numpy()
+    return (Walker.from_device(spins, lw, e, walkers.n_sites), float(o[0]), float(o[1]),
+            parents.cpu().numpy())
+
+
+def systematic_parents(w, psi):
+    """Parents of systematic resampling for normalised weights ``w`` and
+    offset ``psi`` (reconfiguration.py:9-24), on the device."""
+    w = np.asarray(w, dtype=np.float64)
+    with np.errstate(divide="ignore"):
+        lw = np.log(w)
+    dev = _lib.device()
+    lwt = torch.as_tensor(lw, device=dev)
+    e = torch.zeros_like(lwt)
+    parents = torch.empty(len(w), dtype=torch.int32, device=dev)
+    out2 = torch.empty(2, dtype=torch.float64, device=dev)
+    work = torch.empty(len(w), dtype=torch.float64, device=dev)
+    psi_c = ctypes.c_double(float(psi))
+    _lib.check(_lib.load().gfmc_reconfigure(len(w), _lib.ptr(lwt), _lib.ptr(e), 0, 0,
+                                            _lib.TAG_RECONFIG, ctypes.byref(psi_c),
+                                            _lib.ptr(parents), _lib.ptr(out2), _lib.ptr(work),
+                                            _lib.stream()), "gfmc_reconfigure")
+    return parents.cpu().numpy()
+
+
+def sample_index(weights, uniforms, precision="fp64"):
+    """Slots drawn from non-negative ``weights`` at each uniform in [0, 1)
+    with the propagator's sampler (sample_index, evolution.py:30-59)."""
+    w = torch.as_tensor(np.asarray(weights, dtype=np.float64), device=_lib.device())
+    u = torch.as_tensor(np.atleast_1d(np.asarray(uniforms, dtype=np.float64)), device=w.device)
+    out = torch.empty(u.shape[0], dtype=torch.int32, device=w.device)
+    prec = _lib.GFMC_FP64 if precision in ("fp64", _lib.GFMC_FP64) else _lib.GFMC_FP32
+    _lib.check(_lib.load().gfmc_sample_index(prec, w.shape[0], _lib.ptr(w), u.shape[0],
+                                             _lib.ptr(u), _lib.ptr(out), _lib.stream()),
+               "gfmc_sample_index")
+    return out.cpu().numpy()
+
+
+def estimate_average_log_branch_weight(walkers, hamiltonian, guide, method="cont_time",
+                                       lam=None, precision=None):
+    """mean(-sum rates) over the walkers for cont_time (estimators.py:722-751)."""
+    require_method(method)
+    p = get_problem(hamiltonian, guide, precision)
+    _, e, _ = p.rates(walkers.spins, want_rates=False)
+    return float(torch.mean(-e).item())
+
+
+# -- the driver ----------------------------------------------------------------
+
+def _warmup(chain, key, step_f, alw, warmup_steps, verbosity):
+    """Four-stage ramp; the shift is updated per stage unless pinned
+    (driver.py:713-788): alw <- alw + (LSE(log G_stage) - log n) / step."""
+    pinned = alw is not None
+    if not pinned:
+        _, e, _ = chain.p.rates(chain.spins, want_rates=False)
+        alw = float(torch.mean(-e).item())
+    counter = key.position
+    for i, (step_, n_w) in enumerate(warmup_schedule(step_f, warmup_steps)):
+        t0 = time.time()
+        series = chain.steps(n_w, step_, alw, key.seed, counter, warmup=True)
+        counter += n_w
+        lg = series[:, 0].cpu().numpy()
+        check_population(lg, 0)
+        m = np.max(lg)
+        lse = m + np.log(np.sum(np.exp(lg - m)))
+        measured = alw + (lse - np.log(n_w)) / step_
+        if verbosity >= 8:
+            print(f"[gfmc] - [run_warmup {i + 1}/4] - step={step_}, {n_w} steps @ "
+                  f"{n_w / max(time.time() - t0, 1e-9):.1f} it/s, avg_log_weight = "
+                  f"{measured:.6f}", flush=True)
+        if not pinned:
+            alw = float(measured)
+    return alw
+
+
+def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
+             method="cont_time", rng_key=None, lam=None, diag_operators=None,
+             off_diag_operators=None, return_raw=True, avg_log_weight=None,
+             warmup_steps=None, verbosity=7, number_of_outputs=100, precision=None,
+             **_unused):
+    """Single-GPU GFMC run (run_gfmc, driver.py:1148-1547), energy only.
+
+    Returns ``(walkers, energy, obs, stats, raw)`` (or the first four with
+    ``return_raw=False``): ``energy`` {p: E_p}, ``obs`` {} (no operators),
+    ``stats`` with ``avg_log_weight``, ``it_per_sec``, ``runtime``,
+    ``total_wall_time``, ``number_of_devices`` and ``parameters/*``, ``raw``
+    with the per-step ``log_weights`` and ``energy`` series.  ``rng_key``:
+    an int seed or (seed, position); ``stats['parameters/rng_key_next']``
+    continues the chain.  ``precision``: 'fp64' / 'fp32' (default from the
+    Hamiltonian's dtype).
+    """
+    wall_start = time.time()
+    if diag_operators or off_diag_operators:
+        raise NotImplementedError("pygfmc_amd measures the energy only; diagonal and "
+                                  "off-diagonal operators are not implemented.")
+    if rng_key is None:
+        import secrets
+        rng_key = secrets.randbits(32)
+        if verbosity >= 1:
+            print(f"[gfmc] - [run_gfmc] - rng_key not given; using seed {rng_key}", flush=True)
+    key = as_key(rng_key)
+    if walkers.n_walkers == 0:
+        raise ValueError("At least one walker is required.")
+    n_steps = validated_int("n_steps", n_steps, 1)
+    require_method(method)
+    require_step_size(method, step_size)
+    if warmup_steps is None:
+        warmup_steps = DEFAULT_WARMUP_STEPS if avg_log_weight is None else 0
+    warmup_steps = validated_int("warmup_steps", warmup_steps, 0)
+    number_of_outputs = validated_int("number_of_outputs", number_of_outputs, 1)
+    p_sorted = validated_p_array(p_array)
+    if p_sorted[-1] >= n_steps:
+        raise ValueError(f"every p value must be smaller than n_steps: got n_steps={n_steps} "
+                         f"and max(p)={p_sorted[-1]}")
+    if avg_log_weight is None and warmup_steps == 0:
+        raise ValueError("warmup_steps=0 needs an avg_log_weight: without a warm-up nothing "
+                         "measures it.")
+    step_f = float(step_size)
+    p = get_problem(hamiltonian, guide, precision)
+    chain = Chain(p, walkers)
+    alw_given = avg_log_weight
+    alw = _warmup(chain, key, step_f, avg_log_weight, warmup_steps, verbosity)
+
+    print_every = max(1, n_steps // max(1, number_of_outputs))
+    dev = chain.spins.device
+    series = torch.empty((n_steps, 2), dtype=torch.float64, device=dev)
+    evals_sum, evals_n = 0.0, 0
+    torch.cuda.synchronize()
+    t_start = time.time()
+    pending = None
+    step0 = 0
+    host_chunks = []
+    while step0 < n_steps or pending is not None:
+        if step0 < n_steps:
+            n = min(print_every, n_steps - step0)
+            chain.evals.zero_()
+            chain.steps(n, step_f, alw, key.seed, key.position + step0, series=series[step0:step0 + n],
+                        evals=True)
+            ev = chain.evals.to(torch.float64).mean() / n
+            nxt = (step0, n, ev)
+            step0 += n
+        else:
+            nxt = None
+        if pending is not None:
+            s0, n, ev = pending
+            chunk = series[s0:s0 + n].cpu().numpy()
+            check_population(chunk[:, 0], s0, chunk[:, 1])
+            host_chunks.append(chunk)
+            evals_sum += float(ev.item())
+            evals_n += 1
+            if verbosity >= 3 and len(host_chunks) % max(1, number_of_outputs // 10) == 0:
+                print(f"[gfmc] - [run_gfmc] - step {s0 + n}/{n_steps}", flush=True)
+        pending = nxt
+    torch.cuda.synchronize()
+    runtime = time.time() - t_start
+    arr = np.concatenate(host_chunks, axis=0)
+    log_weights, E = arr[:, 0].copy(), arr[:, 1].copy()
+    energy = estimate_energy(p_sorted, log_weights, E)
+    key_next = key.advance(n_steps)
+    stats = {
+        "avg_log_weight": float(alw if alw_given is None else alw_given),
+        "it_per_sec": n_steps / runtime,
+        "runtime": runtime,
+        "total_wall_time": time.time() - wall_start,
+        "number_of_devices": 1,
+        "evals_per_step_mean": evals_sum / max(1, evals_n),
+        "parameters/n_steps": n_steps,
+        "parameters/n_walkers": walkers.n_walkers,
+        "parameters/n_sites": walkers.n_sites,
+        "parameters/step_size": step_f,
+        "parameters/lam": float("nan") if lam is None else float(lam),
+        "parameters/p_array": np.asarray(p_sorted),
+        "parameters/m_array": np.asarray([q // 2 for q in p_sorted]),
+        "parameters/warmup_steps": warmup_steps,
+        "parameters/method": method,
+        "parameters/hamiltonian": type(hamiltonian).__name__,
+        "parameters/guide": type(guide).__name__ if guide is not None else "None",
+        "parameters/has_guide": int(guide is not None),
+        "parameters/precision": "fp64" if p.precision == _lib.GFMC_FP64 else "fp32",
+        "parameters/rng_key": np.asarray([key.seed, key.position], dtype=np.uint64),
+        "parameters/rng_key_next": np.asarray([key_next.seed, key_next.position],
+                                              dtype=np.uint64),
+    }
+    out_walkers = chain.walkers()
+    if verbosity >= 1:
+        print(f"[gfmc] - [run_gfmc] - {n_steps} steps @ {stats['it_per_sec']:.1f} it/s; "
+              f"E = {energy}", flush=True)
+    if return_raw:
+        raw = dict(log_weights=log_weights, energy=E, diag=None, off_diag=None,
+                   off_diag_norm=None, off_diag_steps=None, additional_diag=None,
+                   p_array=list(p_sorted), m_array=[q // 2 for q in p_sorted],
+                   diag_operators=None, off_diag_operators=None)
+        return out_walkers, energy, {}, stats, raw
+    return out_walkers, energy, {}, stats
+
+
+def run(walkers, n_steps, hamiltonian, step_size, guide, p_array, *, method="cont_time",
+        rng_key=None, lam=None, **kw):
+    """Entry point (multi_gpu.run, multi_gpu.py:822-868): one GPU, so it is
+    :func:`run_gfmc`."""
+    return run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, method=method,
+                    rng_key=rng_key, lam=lam, **kw)

@@ -1,0 +1,144 @@
+"""Windowed energy estimator and its blocked jackknife (host, FP64 NumPy).
+
+The per-step series (log G_n, E_n) come off the GPU once per chunk; the
+estimator and the jackknife are O(n_steps) host work, as in the reference
+(estimators.py:268-305, io.py:921-1157).
+"""
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+
+__all__ = ["validated_p_array", "estimate_energy", "evaluate_observables",
+           "validated_int", "check_population"]
+
+
+def validated_int(name, value, minimum):
+    """Integer-valued control parameter >= minimum (_validation.py:7-19)."""
+    if isinstance(value, (bool, np.bool_)):
+        raise ValueError(f"{name} must be an integer >= {minimum}, got {value!r}.")
+    try:
+        result = int(value)
+        ok = float(result) == float(value)
+    except (TypeError, ValueError, OverflowError):
+        ok, result = False, minimum - 1
+    if not ok or result < minimum:
+        raise ValueError(f"{name} must be an integer >= {minimum}, got {value!r}.")
+    return result
+
+
+def validated_p_array(p_array):
+    """Sorted unique positive integer windows (estimators.py:30-42)."""
+    if p_array is None or len(p_array) == 0:
+        raise ValueError("p_array must contain at least one positive window size.")
+    out = []
+    for p in p_array:
+        p_int = int(p)
+        if float(p_int) != float(p):
+            raise ValueError(f"p_array values must be integers, got {p!r}.")
+        if p_int <= 0:
+            raise ValueError(f"p_array values must be > 0, got {p_int}.")
+        out.append(p_int)
+    return sorted(set(out))
+
+
+def estimate_energy(p_array, log_weights, E):
+    """Mixed estimator <E>_p = sum_i G_p(i) E(i+p) / sum_i G_p(i) with
+    log G_p(i) = C[i+p] - C[i], C the cumulative sum of log G_n, shifted by
+    its max before the exponential (estimators.py:268-305, 595)."""
+    w = np.asarray(log_weights, dtype=np.float64)
+    E = np.asarray(E, dtype=np.float64)
+    if w.ndim != 1 or E.ndim != 1:
+        raise ValueError(f"log_weights and E must be one-dimensional; got {w.shape} and {E.shape}.")
+    if len(w) != len(E):
+        raise ValueError(f"log_weights and E must have equal lengths; got {len(w)} and {len(E)}.")
+    if len(w) == 0:
+        raise ValueError("log_weights and E must contain at least one sample.")
+    sorted_p = validated_p_array(p_array)
+    n = len(w)
+    if not np.all(np.isfinite(E)):
+        raise ValueError("Energy samples contain nonfinite values.")
+    too_large = [p for p in sorted_p if p >= n]
+    if too_large:
+        raise ValueError(f"Each p value must be smaller than the number of available "
+                         f"samples ({n}); invalid values: {too_large}.")
+    if not np.all(np.isfinite(w)):
+        raise ValueError("Branch weights contain nonfinite values.")
+    C = np.cumsum(w)
+    out = {}
+    for p in sorted_p:
+        log_G_p = C[p:] - C[:-p]
+        shift = np.max(log_G_p)
+        G_p = np.exp(log_G_p - shift)
+        out[p] = float(np.sum(G_p * E[p:])) / float(np.sum(G_p))
+    return out
+
+
+def evaluate_observables(raw, p_array=None, n_blocks=None):
+    """Re-estimate the energy from ``raw`` (``log_weights``, ``energy``,
+    ``p_array``), optionally with the blocked jackknife error: delete each of
+    ``n_blocks`` contiguous blocks, recompute the whole estimator on the rest,
+    err = sqrt((nb-1)/nb * sum (x_b - mean)^2) (io.py:921-1157).  Trailing
+    samples that do not fill a block are dropped from both the estimate and
+    the error.  Returns ``(energy, obs)`` or ``(energy, obs, energy_err,
+    obs_err)``; ``obs`` is empty (no operator series in this engine)."""
+    if raw.get("energy") is None:
+        raise ValueError("raw holds no 'energy' series, so nothing can be re-estimated.")
+    if raw.get("log_weights") is None:
+        raise ValueError("raw holds no 'log_weights' series; the windowed estimator "
+                         "cannot be rebuilt without the per-step branch weights.")
+    E = np.asarray(raw["energy"], dtype=np.float64)
+    lw = np.asarray(raw["log_weights"], dtype=np.float64)
+    grid = p_array if p_array is not None else raw.get("p_array")
+    if grid is None:
+        raise ValueError("no p grid: pass p_array or a raw dict that carries one")
+    grid = validated_p_array(grid)
+    if n_blocks is None:
+        return estimate_energy(grid, lw, E), {}
+    n_blocks = validated_int("n_blocks", n_blocks, 1)
+    n = E.shape[0]
+    blen = n // n_blocks
+    if n_blocks < 2 or blen < 1:
+        raise ValueError(f"n_blocks={n_blocks} over {n} steps leaves {blen} steps per block; "
+                         "a jackknife needs at least 2 usable blocks.")
+    usable = blen * n_blocks
+    if usable != n:
+        warnings.warn(f"Discarding {n - usable} trailing samples for equal jackknife blocks; "
+                      "point estimates and errors both use the retained samples.", stacklevel=2)
+        E, lw = E[:usable], lw[:usable]
+        n = usable
+    if n_blocks < 20:
+        warnings.warn(f"n_blocks={n_blocks} is few: the error on the error is then "
+                      "comparable to the error itself. ~20 or more is advisable.", stacklevel=2)
+    est = estimate_energy(grid, lw, E)
+    reps = []
+    for b in range(n_blocks):
+        m = np.ones(n, dtype=bool)
+        m[b * blen:(b + 1) * blen] = False
+        e = estimate_energy(grid, lw[m], E[m])
+        reps.append([e[p] for p in grid])
+    arr = np.asarray(reps, dtype=float)
+    centred = arr - arr.mean(axis=0, keepdims=True)
+    err = np.sqrt((n_blocks - 1) / n_blocks * np.sum(centred ** 2, axis=0))
+    return est, {}, dict(zip(grid, (float(x) for x in err))), {}
+
+
+def check_population(log_G_chunk, step0, E_chunk=None):
+    """Raise on a non-finite log G_n (extinct population) or E_n
+    (driver.py:913-937)."""
+    bad = np.flatnonzero(~np.isfinite(log_G_chunk))
+    if bad.size:
+        raise ValueError(
+            f"[gfmc] population extinct at step {step0 + int(bad[0])}: the mean branch "
+            "weight of the step is not finite, i.e. every walker's weight is zero (or one "
+            "overflowed).  The ensemble cannot be renormalised; check the step size and "
+            "the guide.")
+    if E_chunk is not None:
+        bad = np.flatnonzero(~np.isfinite(E_chunk))
+        if bad.size:
+            raise ValueError(
+                f"[gfmc] non-finite energy at step {step0 + int(bad[0])}: the weighted mean "
+                "of the local energies is not finite while the branch weights are, i.e. a "
+                "walker's local energy overflowed; check the guide and the Hamiltonian on "
+                "the configurations the walk reaches.")

@@ -1,0 +1,142 @@
+"""Guiding wavefunctions.
+
+``JastrowSpinHalf`` (jastrow_spin_half.py:42-499): psi_T(x) =
+exp(sum_pairs J_ij x_i x_j + sum_i h_i x_i) with J = -beta.  The host builds
+the per-site partner tables (__init__:79-157) and the per-edge
+external-partner cache (build_cache:400-459); every evaluation runs on the
+GPU through the rate / log-psi kernels.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .hamiltonian import edges_array
+
+__all__ = ["JastrowSpinHalf", "JastrowOpCache"]
+
+
+class JastrowOpCache:
+    """External-partner tables of a (Jastrow, SpinHalf) pair
+    (JastrowOpCache, jastrow_spin_half.py:31-39)."""
+
+    def __init__(self, edge_other, edge_J):
+        self.edge_other = edge_other    # (n_edges, 2, max_deg) int32
+        self.edge_J = edge_J            # (n_edges, 2, max_deg) float64, 0 on excluded/padding
+
+
+class JastrowSpinHalf:
+    """Two-body Jastrow guide with sparse couplings.
+
+    ``JastrowSpinHalf(N, pairs, beta, h=None, dtype=np.float32)``: ``beta``
+    scalar (tied) or one per pair, ``h`` scalar, ``(N,)`` or None.  Pairs are
+    canonicalised to (min, max); each site lists its partners in pair order
+    and both endpoints read the same coupling.
+    """
+
+    def __init__(self, N, pairs, beta, h=None, dtype=np.float32):
+        N = int(N)
+        if pairs is None:
+            raise ValueError("pairs must be provided (n_pairs, 2) int array)")
+        pairs_np, _, n_min = edges_array(pairs)
+        if n_min > N:
+            raise ValueError(f"pair indices must lie in [0, {N}).")
+        pairs_np = np.sort(pairs_np.astype(np.int32), axis=1)
+        n_pairs = pairs_np.shape[0]
+        beta_arr = np.asarray(beta, dtype=float)
+        if beta_arr.ndim > 0 and beta_arr.shape != (n_pairs,):
+            raise ValueError(f"beta must be a scalar or have shape ({n_pairs},), "
+                             f"got {beta_arr.shape}")
+        J_np = -np.broadcast_to(beta_arr, (n_pairs,)).astype(float)
+        if h is None:
+            h_np = np.zeros(N)
+        else:
+            h_in = np.asarray(h, dtype=float)
+            if h_in.ndim > 1 or (h_in.ndim == 1 and h_in.shape != (N,)):
+                raise ValueError(f"h must be a scalar or have shape ({N},), got {h_in.shape}")
+            h_np = np.broadcast_to(h_in, (N,)).copy()
+        site_partners = [[] for _ in range(N)]
+        for p_idx, (a, b) in enumerate(pairs_np):
+            site_partners[a].append((int(b), p_idx))
+            site_partners[b].append((int(a), p_idx))
+        max_deg = max((len(s) for s in site_partners), default=0) or 1
+        site_other = -np.ones((N, max_deg), dtype=np.int32)
+        site_pair = np.zeros((N, max_deg), dtype=np.int32)
+        for k in range(N):
+            for slot, (other, p_idx) in enumerate(site_partners[k]):
+                site_other[k, slot] = other
+                site_pair[k, slot] = p_idx
+        self.dtype = np.dtype(dtype)
+        self.N = N
+        self.pairs = pairs_np
+        self.site_valid_mask = site_other >= 0
+        self.site_other = np.clip(site_other, 0, None).astype(np.int32)
+        self._site_pair = site_pair
+        self._J = J_np
+        self._tied = beta_arr.ndim == 0
+        self._h = None if h is None else h_np
+        self._cache = {}
+
+    # -- parameters --------------------------------------------------------
+    @property
+    def site_J(self):
+        """J per (site, partner slot), 0 on padding (site_J, :159-163)."""
+        return np.where(self.site_valid_mask, self._J[self._site_pair], 0.0)
+
+    @property
+    def h(self):
+        return np.zeros(self.N) if self._h is None else self._h.copy()
+
+    @property
+    def beta(self):
+        return -self._J.copy()
+
+    @property
+    def tied_beta(self):
+        return self._tied
+
+    @property
+    def max_deg(self):
+        return self.site_other.shape[1]
+
+    def fingerprint(self):
+        return b"jastrow|" + np.int64(self.N).tobytes() + self.pairs.tobytes() + \
+            self._J.tobytes() + self.h.tobytes()
+
+    # -- operator cache (build_cache, :400-459) -----------------------------
+    def build_cache(self, op):
+        if not hasattr(op, "n_hx_static") or not hasattr(op, "edge_i"):
+            raise TypeError(f"{type(self).__name__} needs an operator with the SpinHalf "
+                            f"transition layout; got {type(op).__name__}.")
+        ei = np.asarray(op.edge_i, dtype=np.int64)
+        ej = np.asarray(op.edge_j, dtype=np.int64)
+        key = (ei.tobytes(), ej.tobytes(), self._J.tobytes())
+        hit = self._cache.get(key)
+        if hit is not None:
+            return hit
+        site_J = self.site_J
+        i_part, j_part = self.site_other[ei], self.site_other[ej]
+        i_J, j_J = site_J[ei], site_J[ej]
+        i_mask = self.site_valid_mask[ei] & ~(i_part == ej[:, None])
+        j_mask = self.site_valid_mask[ej] & ~(j_part == ei[:, None])
+        edge_other = np.stack([i_part, j_part], axis=1).astype(np.int32)
+        edge_J = np.where(np.stack([i_mask, j_mask], axis=1), np.stack([i_J, j_J], axis=1), 0.0)
+        out = JastrowOpCache(edge_other, edge_J)
+        self._cache[key] = out
+        return out
+
+    # -- evaluation on the device --------------------------------------------
+    def log_psi(self, config, cache=None):
+        """log psi_T of one ``(N,)`` or a batch ``(B, N)`` of configurations
+        (log_psi, :299-324), by the log-psi kernel."""
+        from .problem import device_log_psi
+        return device_log_psi(self, config)
+
+    def log_diff_from_operator(self, config, op, cache=None):
+        """log psi(x'_k) - log psi(x) in the SpinHalf layout of ``op``
+        (log_diff_from_operator, :369-398), by the rate kernel."""
+        from .problem import device_log_diff
+        return device_log_diff(op, self, config)
+
+    def log_diff_from_configs(self, config, new_configs, cache=None):
+        new = np.asarray(new_configs, dtype=float)
+        return self.log_psi(new) - self.log_psi(np.asarray(config, dtype=float)[None])[0]

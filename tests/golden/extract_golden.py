@@ -1,0 +1,46 @@
+"""Write tests/golden/golden_spinhalf.npz from the reference's committed
+fixture tests/core/reference_data/golden_cpu.npz (PyGFMC, Apache-2.0,
+Copyright 2026 Robin Schaefer; jax 0.9.2, cpu, float64).
+
+Kept (data only):
+* ``<case>/config`` (64, 8) and ``<case>/e_loc`` (64,) of the SpinHalf runs
+  without a guide and with the Jastrow guide, all three methods: the
+  final walkers of a 300-step run on chain(8), J_x = J_y = -1, J_z = 1,
+  h_x = 0.5, Jastrow beta = 0.3, h = 0.05 (test_golden_outputs.py:191-215);
+  e_loc is the local energy of config after the last reconfiguration.
+* ``<case>/raw/log_weights``, ``<case>/raw/energy`` (300,) and
+  ``<case>/energy/<p>`` of every case that has them: the windowed energy
+  estimator's inputs and outputs.
+
+Run in the build container only (the reference is not on the GPU box):
+    python tests/golden/extract_golden.py
+"""
+import os
+
+import numpy as np
+
+SRC = "/root/reference/tests/core/reference_data/golden_cpu.npz"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_spinhalf.npz")
+
+
+def main():
+    g = np.load(SRC, allow_pickle=False)
+    out = {}
+    for guide in ("none", "jastrow"):
+        for method in ("cont_time", "multi_step", "single_step"):
+            case = f"spinhalf_{guide}_{method}"
+            out[f"{case}/config"] = g[f"{case}/config"]
+            out[f"{case}/e_loc"] = g[f"{case}/e_loc"]
+    cases = sorted({k.split("/")[0] for k in g.files if "/raw/log_weights" in k})
+    for case in cases:
+        out[f"{case}/raw/log_weights"] = g[f"{case}/raw/log_weights"]
+        out[f"{case}/raw/energy"] = g[f"{case}/raw/energy"]
+        for k in g.files:
+            if k.startswith(f"{case}/energy/"):
+                out[k] = g[k]
+    np.savez_compressed(OUT, **out)
+    print(f"wrote {OUT}: {len(out)} arrays from {len(cases)} cases")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,273 @@
+"""Pin the CPU oracle: the reference's golden vectors, its known-answer
+tests, brute force on tiny systems, and closed forms.  CPU only."""
+import numpy as np
+import pytest
+
+from helpers import (basis, golden_system, index_of, jastrow_log_psi, random_model,
+                     spinhalf_matrix)
+from pygfmc_amd.problem import dependency_lists, lower
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                    "golden_spinhalf.npz")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(GOLDEN, allow_pickle=False)
+
+
+# -- golden local energies (test_golden_outputs.py, tolerance rtol 1e-10) --------
+
+@pytest.mark.parametrize("guide", ["none", "jastrow"])
+@pytest.mark.parametrize("method", ["cont_time", "multi_step", "single_step"])
+def test_oracle_local_energy_matches_golden(orc, golden, guide, method):
+    H, G = golden_system(guide=guide == "jastrow")
+    prob = orc.Problem(lower(H, G))
+    c = golden[f"spinhalf_{guide}_{method}/config"]
+    e_ref = golden[f"spinhalf_{guide}_{method}/e_loc"]
+    rates, e_loc, _ = prob.rates(orc.pack(c))
+    np.testing.assert_allclose(e_loc, e_ref, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(rates.sum(axis=1), e_ref, rtol=1e-12, atol=1e-13)
+
+
+# -- brute force: Kronecker-product matrix and explicit log psi ------------------
+
+@pytest.mark.parametrize("seed,lattice,pairs", [(0, "chain", "edges"), (1, "chain", "nnn"),
+                                                (2, "kagome", "edges")])
+def test_oracle_rates_equal_matrix_elements_times_guide_ratios(orc, seed, lattice, pairs):
+    N = 8 if lattice == "chain" else 12
+    H, G = random_model(N=N, seed=seed, lattice=lattice, guide_pairs=pairs)
+    M = spinhalf_matrix(H).toarray()
+    rng = np.random.default_rng(seed)
+    configs = basis(H.N)[rng.choice(2 ** H.N, size=64, replace=False)]
+    prob = orc.Problem(lower(H, G))
+    rates, e_loc, log_diff = prob.rates(orc.pack(configs))
+    lp = jastrow_log_psi(G, configs)
+    for w, x in enumerate(configs):
+        col = index_of(x)[0]
+        succ = H.successors(x)
+        rows = index_of(succ)
+        ld_ref = jastrow_log_psi(G, succ) - lp[w]
+        # diagonal and off-diagonal matrix elements (the matrix is symmetric)
+        assert rates[w, 0] == pytest.approx(M[col, col], abs=1e-12)
+        bare = np.array([M[r, col] for r in rows[1:]])
+        np.testing.assert_allclose(log_diff[w, 1:], ld_ref[1:], atol=1e-12)
+        np.testing.assert_allclose(rates[w, 1:], bare * np.exp(ld_ref[1:]), rtol=1e-12,
+                                   atol=1e-14)
+        # local energy = (H psi)(x) / psi(x) over the full column
+        psi_all = np.exp(jastrow_log_psi(G, basis(H.N)) - lp[w])
+        assert e_loc[w] == pytest.approx(M[:, col] @ psi_all, rel=1e-11, abs=1e-11)
+
+
+def test_oracle_log_psi_is_the_pair_sum(orc):
+    H, G = random_model(N=8, seed=4, guide_pairs="nnn")
+    configs = basis(8)[::7]
+    prob = orc.Problem(lower(H, G))
+    np.testing.assert_allclose(prob.log_psi(orc.pack(configs)), jastrow_log_psi(G, configs),
+                               atol=1e-12)
+
+
+def test_dependency_lists_cover_every_changed_slot(orc):
+    """After move s only the slots listed for s change value, and only the
+    listed edges change their parallel state."""
+    for seed, pairs in ((0, "edges"), (3, "nnn")):
+        H, G = random_model(N=8, seed=seed, guide_pairs=pairs)
+        dep_off, dep_slots, edge_off, dep_edges = dependency_lists(H, G, G.build_cache(H))
+        prob = orc.Problem(lower(H, G))
+        rng = np.random.default_rng(seed)
+        for x in basis(8)[rng.choice(256, 12, replace=False)]:
+            r0, _, _ = prob.rates(orc.pack(x[None]))
+            anti0 = x[H.edge_i] != x[H.edge_j]
+            for s in range(H.n_off):
+                y = H.generate_state(x, s + 1)
+                r1, _, _ = prob.rates(orc.pack(y[None]))
+                changed = np.flatnonzero(r1[0, 1:] != r0[0, 1:])
+                listed = set(dep_slots[dep_off[s]:dep_off[s + 1]].tolist())
+                assert set(changed.tolist()) <= listed
+                anti1 = y[H.edge_i] != y[H.edge_j]
+                listed_e = set(dep_edges[edge_off[s]:edge_off[s + 1]].tolist())
+                assert set(np.flatnonzero(anti0 != anti1).tolist()) <= listed_e
+
+
+# -- the windowed estimator (estimators.py:268-305) -------------------------------
+
+def test_energy_estimator_reproduces_golden(golden):
+    from pygfmc_amd.estimators import estimate_energy
+    cases = sorted({k.split("/")[0] for k in golden.files if "/raw/" in k})
+    assert len(cases) == 45
+    exact = 0
+    for c in cases:
+        E = estimate_energy([2, 4], golden[f"{c}/raw/log_weights"], golden[f"{c}/raw/energy"])
+        for p in (2, 4):
+            ref = float(golden[f"{c}/energy/{p}"])
+            assert E[p] == pytest.approx(ref, rel=1e-14, abs=1e-14)
+            exact += E[p] == ref
+    assert exact >= 80        # the rest differ in the last bit (NumPy's sum order)
+
+
+def test_jackknife_two_blocks_is_half_chain_split(golden):
+    """n_blocks = 2: err = |x(first half) - x(second half)| / 2
+    (test_io_hdf5.py:724 restated for the energy)."""
+    from pygfmc_amd.estimators import evaluate_observables
+    raw = dict(log_weights=golden["spinhalf_jastrow_cont_time/raw/log_weights"],
+               energy=golden["spinhalf_jastrow_cont_time/raw/energy"], p_array=[2, 4])
+    n = len(raw["energy"])
+    half = lambda a, b: dict(log_weights=raw["log_weights"][a:b], energy=raw["energy"][a:b],  # noqa
+                             p_array=[2, 4])
+    x0, _ = evaluate_observables(half(n // 2, n))
+    x1, _ = evaluate_observables(half(0, n // 2))
+    with pytest.warns(UserWarning):
+        _, _, err, _ = evaluate_observables(raw, n_blocks=2)
+    for p in (2, 4):
+        assert err[p] == pytest.approx(abs(x0[p] - x1[p]) / 2, rel=1e-12)
+
+
+def test_jackknife_error_of_iid_series_is_the_standard_error():
+    """For p = 1 and constant weights the estimator is the mean, and the
+    jackknife over single-sample blocks is exactly the standard error."""
+    from pygfmc_amd.estimators import evaluate_observables
+    rng = np.random.default_rng(0)
+    E = rng.normal(size=400)
+    raw = dict(log_weights=np.zeros(400), energy=E, p_array=[1])
+    est, _, err, _ = evaluate_observables(raw, n_blocks=400)
+    assert est[1] == pytest.approx(E[1:].mean(), rel=1e-12)
+    # the estimator uses E[1:] (window p = 1); the replicas keep 398 or 399 of them
+    assert err[1] == pytest.approx(E[1:].std(ddof=1) / np.sqrt(399), rel=0.02)
+
+
+# -- Philox4x32-10 known-answer vectors (Random123 kat_vectors) -------------------
+
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_known_answers(orc, ctr, key, want):
+    assert tuple(int(v) for v in orc.philox(ctr, key)) == want
+
+
+def test_product_philox_matches_known_answers():
+    """The HIP library's Philox (its host entry; no GPU needed)."""
+    import ctypes
+    from pygfmc_amd import _lib
+    L = _lib.load()
+    ctr = np.array([c for c, _, _ in PHILOX_KAT], dtype=np.uint32)
+    out = np.zeros((3, 4), dtype=np.uint32)
+    for i, (c, k, w) in enumerate(PHILOX_KAT):
+        cc = np.array(c, dtype=np.uint32)
+        kk = np.array(k, dtype=np.uint32)
+        oo = np.zeros(4, dtype=np.uint32)
+        P = ctypes.POINTER(ctypes.c_uint32)
+        assert L.gfmc_philox_host(1, cc.ctypes.data_as(P), kk.ctypes.data_as(P),
+                                  oo.ctypes.data_as(P)) == 0
+        out[i] = oo
+    assert [tuple(int(v) for v in r) for r in out] == [w for _, _, w in PHILOX_KAT]
+    del ctr
+
+
+# -- sampler KATs (test_sampler.py:20-44) -------------------------------------------
+
+def test_sampler_never_draws_a_zero_weight_slot(orc):
+    u = np.linspace(0, 1, 4097)[:-1]
+    idx = orc.sample_many([1.0, 0.0, 1.0], u)
+    assert set(np.unique(idx)) <= {0, 2}
+    idx = orc.sample_many([0.0, 2.0, 0.0, 0.0], u)
+    assert set(np.unique(idx)) == {1}
+    # leading, interior and trailing zeros across several 64-lane blocks
+    w = np.zeros(300)
+    w[[5, 130, 131, 299]] = [1.0, 0.5, 0.25, 2.0]
+    idx = orc.sample_many(w, u)
+    assert set(np.unique(idx)) <= {5, 130, 131, 299}
+
+
+def test_sampler_draws_proportionally(orc):
+    rng = np.random.default_rng(1)
+    w = rng.uniform(0, 1, 200)
+    w[rng.choice(200, 50, replace=False)] = 0
+    u = rng.uniform(size=200000)
+    idx = orc.sample_many(w, u)
+    freq = np.bincount(idx, minlength=200) / len(u)
+    assert np.all(freq[w == 0] == 0)
+    np.testing.assert_allclose(freq, w / w.sum(), atol=4e-3)
+
+
+def test_sampler_matches_exact_searchsorted_on_exact_sums(orc):
+    """With dyadic weights every partial sum is exact, so the blocked CDF is
+    the exact CDF and the draw is searchsorted(cdf, u, 'right')."""
+    rng = np.random.default_rng(2)
+    w = rng.integers(0, 8, 257) / 8.0
+    u = rng.uniform(size=5000)
+    idx = orc.sample_many(w, u)
+    cdf = np.cumsum(w)
+    ref = np.searchsorted(cdf, u * cdf[-1], side="right")
+    np.testing.assert_array_equal(idx, ref)
+
+
+# -- systematic resampling KATs (test_walker_io.py:60-77) --------------------------
+
+def _parents_from_w(orc, w, psi):
+    with np.errstate(divide="ignore"):
+        lw = np.log(np.asarray(w, dtype=float))
+    par, _, _, _ = orc.reconfigure(lw, np.zeros(len(w)), psi)
+    return par.tolist()
+
+
+def test_systematic_parents_known_answers(orc):
+    assert _parents_from_w(orc, [0.0, 1.0], 0.0) == [1, 1]
+    assert _parents_from_w(orc, [0.0, 0.5, 0.5, 0.0], 0.0) == [1, 1, 2, 2]
+    short = [0.25, 0.25, 0.4999999, 0.0]
+    par = _parents_from_w(orc, short, 0.9999999)
+    assert par[-1] == 2 and all(short[k] > 0 for k in par)
+    # dead walkers: -inf log weights are never chosen
+    for psi in np.linspace(0, 0.999, 20):
+        par, _, _, _ = orc.reconfigure(np.array([-np.inf, 0.0, -np.inf]), np.zeros(3), psi)
+        assert par.tolist() == [1, 1, 1]
+
+
+def test_reconfiguration_matches_the_direct_formulas(orc):
+    rng = np.random.default_rng(3)
+    n = 5000
+    lw = rng.normal(scale=2.0, size=n)
+    e = rng.normal(size=n)
+    psi = 0.37
+    par, lg, En, _ = orc.reconfigure(lw, e, psi)
+    m = lw.max()
+    logG = m + np.log(np.sum(np.exp(lw - m)))
+    w = np.exp(lw - logG)
+    assert lg == pytest.approx(logG - np.log(n), rel=1e-13)
+    assert En == pytest.approx(np.sum(w * e), rel=1e-12, abs=1e-14)
+    ref = np.minimum(np.searchsorted(np.cumsum(w), (np.arange(n) + psi) / n, side="right"),
+                     n - 1)
+    # rounding of the two prefix sums may move a parent across a boundary only
+    # where u lies within a few ulps of a cdf value
+    assert np.mean(par == ref) > 0.999
+    # offspring counts are floor or ceil of n * w
+    cnt = np.bincount(par, minlength=n)
+    assert np.all(np.abs(cnt - n * w) < 1.0 + 1e-9)
+
+
+def test_init_sz_has_exact_magnetisation(orc):
+    spins = orc.init_sz(37, 500, 20, seed=11)
+    c = orc.unpack(spins, 37)
+    assert np.all(c.sum(axis=1) == 2 * 20 - 37)
+    # every site is up with probability 20/37
+    assert np.abs(((c + 1) / 2).mean(axis=0) - 20 / 37).max() < 0.08
+    # walkers differ
+    assert len({row.tobytes() for row in spins}) > 490
+
+
+def test_init_random_is_fair(orc):
+    spins = orc.init_random(45, 2000, seed=5)
+    c = orc.unpack(spins, 45)
+    assert abs(c.mean()) < 0.02
+    assert np.all((spins[:, 1] >> 13) == 0)      # bits past N stay clear
+
+
+def test_pack_unpack_roundtrip(orc):
+    rng = np.random.default_rng(0)
+    c = rng.choice([-1.0, 1.0], size=(17, 70))
+    np.testing.assert_array_equal(orc.unpack(orc.pack(c), 70), c)
