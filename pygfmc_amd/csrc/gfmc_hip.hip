@@ -18,6 +18,7 @@
 // loops are bank-conflict free.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,7 +54,22 @@ static int fail(int code, const std::string &msg) {
 template <typename T>
 struct Tables {
     const T *jpm, *jpmpm, *hx_half, *jas_h, *jas_site_J, *jas_edge_J;
+    // uniform values used by the record path
+    T jpm0, jpmpm0, hx0, J0, h0;
 };
+
+// Record path ("fast"): usable when the couplings are uniform, the Jastrow
+// coupling is tied (every J is J0 or a masked 0), the field h is uniform,
+// every site has at most REC_D partners and N, n_off < 32768.  Each slot is
+// described by one 32-byte record of 16 int16 fields:
+//   [0] slot  [1] site i  [2] site j (-1 for a site flip)  [3] partner mask
+//   [4 .. 4+2D) partners: D of site i (excluding j), then D of site j
+// and the slots a move s re-evaluates are stored as whole records at
+// mv_rec[s * R + t] (R records per move, slot -1 = padding), so a move needs
+// one round trip to HBM/L2.  For uniform J_z the anti-parallel count is
+// updated from mv_nbr[s * RN + t] = neighbour p | (flipped site f << 16) of
+// the flipped sites (edges between two flipped sites excluded).
+#define REC_D 6
 
 struct DevProblem {
     int N, n_edges, n_hx, n_jpm, n_off;
@@ -65,6 +81,14 @@ struct DevProblem {
     const double *jz, *hz;
     const int *jas_site_other, *jas_edge_other;
     const int *dep_off, *dep_slots, *edge_off, *dep_edges;
+    // record path
+    int fast, R, RN, RE, flags;
+    const uint4 *slot_rec;   // [n_off * 2]
+    const uint4 *mv_rec;     // [n_off * R * 2]
+    const int *mv_nbr;       // [n_off * RN]
+    const int *mv_sites;     // [n_off]  i | (j << 16), j = 0xffff for a site flip
+    const int *mv_edges;     // [n_off * RE] edges whose state a move may change (-1 pad)
+    const int *edge_pack;    // [n_edges] i | (j << 16)
 };
 
 struct gfmc_problem {
@@ -164,6 +188,68 @@ __device__ __forceinline__ T slot_weight(const DevProblem &P, const Tables<T> &t
     return -(bare * dev_exp(slot_log_diff(P, tb, x, s)));
 }
 
+// ---------------------------------------------------------------------------
+// record path: the same slot arithmetic from a pre-joined 32-byte record
+// ---------------------------------------------------------------------------
+struct Rec {
+    uint32_t w[8];
+};
+
+__device__ __forceinline__ Rec load_rec(const uint4 *base, size_t r) {
+    const uint4 a = base[2 * r], b = base[2 * r + 1];
+    Rec o;
+    o.w[0] = a.x; o.w[1] = a.y; o.w[2] = a.z; o.w[3] = a.w;
+    o.w[4] = b.x; o.w[5] = b.y; o.w[6] = b.z; o.w[7] = b.w;
+    return o;
+}
+
+__device__ __forceinline__ Rec empty_rec() {
+    Rec o;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o.w[q] = 0xffffffffu;
+    return o;
+}
+
+// int16 field k (sign-extended); k must be a compile-time constant
+__device__ __forceinline__ int rec_f(const Rec &r, int k) {
+    int v = (int)((r.w[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+    return (v ^ 0x8000) - 0x8000;
+}
+
+// slot weight from a record: identical operations, in the same order, as
+// slot_weight on the generic tables
+template <typename T>
+__device__ __forceinline__ T eval_rec(const DevProblem &P, const Tables<T> &tb, const Rec &r,
+                                      const signed char *x) {
+    const int i = rec_f(r, 1), j = rec_f(r, 2), mask = rec_f(r, 3) & 0xffff;
+    const int D = P.max_deg;
+    const bool jas = P.guide == GFMC_GUIDE_JASTROW;
+    if (j < 0) {
+        const T bare = tb.hx0;
+        if (bare == (T)0 || !jas) return -bare;
+        T mf = (T)0;
+#pragma unroll
+        for (int q = 0; q < REC_D; ++q)
+            if (q < D) mf += (((mask >> q) & 1) ? tb.J0 : (T)0) * (T)x[rec_f(r, 4 + q)];
+        const T xk = (T)x[i];
+        return -(bare * dev_exp(((T)-2 * xk) * mf - ((T)2 * tb.h0) * xk));
+    }
+    const T xi = (T)x[i], xj = (T)x[j];
+    const T bare = (x[i] != x[j]) ? tb.jpm0 : tb.jpmpm0;
+    if (bare == (T)0 || !jas) return -bare;
+    T mfi = (T)0, mfj = (T)0;
+#pragma unroll
+    for (int q = 0; q < REC_D; ++q)
+        if (q < D) mfi += (((mask >> q) & 1) ? tb.J0 : (T)0) * (T)x[rec_f(r, 4 + q)];
+#pragma unroll
+    for (int q = 0; q < REC_D; ++q)
+        if (q < D)
+            mfj += (((mask >> (REC_D + q)) & 1) ? tb.J0 : (T)0) * (T)x[rec_f(r, 4 + REC_D + q)];
+    const T pair = xi * mfi + xj * mfj;
+    const T hterm = tb.h0 * xi + tb.h0 * xj;
+    return -(bare * dev_exp((T)-2 * pair - (T)2 * hterm));
+}
+
 // Diagonal element (SpinHalf.diagonal_weight, SpinHalf.py:291-314).
 __device__ double diag_value(const DevProblem &P, const signed char *x,
                              const unsigned char *anti, int n_anti, int lane) {
@@ -232,17 +318,37 @@ __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
 }
 
 // every slot weight, every edge state; returns the anti-parallel count
-template <typename T>
+template <typename T, bool FAST>
 __device__ int full_eval(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L, int lane) {
-    for (int k = 0; k < P.K; ++k) {
-        int s = lane * P.K + k;
-        L.w[k * WAVE + lane] = s < P.n_off ? slot_weight(P, tb, L.x, s) : (T)0;
+    if constexpr (FAST) {
+        // two records in flight per lane
+        for (int k0 = 0; k0 < P.K; k0 += 2) {
+            Rec r[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int s = lane * P.K + k0 + q;
+                r[q] = (k0 + q < P.K && s < P.n_off) ? load_rec(P.slot_rec, s) : empty_rec();
+            }
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (k0 + q < P.K)
+                    L.w[(k0 + q) * WAVE + lane] =
+                        rec_f(r[q], 0) >= 0 ? eval_rec(P, tb, r[q], L.x) : (T)0;
+            }
+        }
+    } else {
+        for (int k = 0; k < P.K; ++k) {
+            int s = lane * P.K + k;
+            L.w[k * WAVE + lane] = s < P.n_off ? slot_weight(P, tb, L.x, s) : (T)0;
+        }
     }
     int cnt = 0;
     if (P.has_jz) {
+        const bool flags = P.flags;
         for (int e = lane; e < P.n_edges; e += WAVE) {
-            unsigned char a = L.x[P.edge_i[e]] != L.x[P.edge_j[e]];
-            L.anti[e] = a;
+            const int pk = P.edge_pack[e];
+            unsigned char a = L.x[pk & 0xffff] != L.x[(pk >> 16) & 0xffff];
+            if (flags) L.anti[e] = a;
             cnt += a;
         }
     }
@@ -270,14 +376,19 @@ __device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T
     if (j == 0) prev = (T)0;
     int idx = -1;
     if (lane == j) {
+        // sequential running sum of lane j's block, eight LDS reads in flight
         T c = prev;
-        for (int k = 0; k < P.K; ++k) {
-            int s = j * P.K + k;
-            if (s >= P.n_off) break;
-            c += L.w[k * WAVE + j];
-            if (c > u) {
-                idx = s;
-                break;
+        for (int k0 = 0; k0 < P.K && idx < 0; k0 += 8) {
+            T v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = (k0 + q < P.K) ? L.w[(k0 + q) * WAVE + j] : (T)0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int s = j * P.K + k0 + q;
+                if (idx < 0 && k0 + q < P.K && s < P.n_off) {
+                    c += v[q];
+                    if (c > u) idx = s;
+                }
             }
         }
     }
@@ -301,6 +412,57 @@ __device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T
     return idx;
 }
 
+// Data of move idx that the record path prefetches before the hold
+// arithmetic: this lane's record, this lane's neighbour entry, the sites.
+struct MovePrefetch {
+    Rec rec;
+    int nbr;
+    int sites;
+};
+
+__device__ __forceinline__ MovePrefetch prefetch_move(const DevProblem &P, int idx, int lane) {
+    MovePrefetch m;
+    m.rec = lane < P.R ? load_rec(P.mv_rec, (size_t)idx * P.R + lane) : empty_rec();
+    m.nbr = (P.has_jz && P.uniform_jz && lane < P.RN) ? P.mv_nbr[(size_t)idx * P.RN + lane] : -1;
+    m.sites = P.mv_sites[idx];
+    return m;
+}
+
+// record path of apply_move: count the edges that turn (anti-)parallel,
+// flip, re-evaluate the prefetched records
+template <typename T>
+__device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L,
+                              const MovePrefetch &m, int idx, int n_anti, int lane) {
+    if (P.has_jz && P.uniform_jz) {
+        int d = 0;
+        if (m.nbr >= 0) {
+            const int p = m.nbr & 0xffff, f = (m.nbr >> 16) & 0xffff;
+            d = (L.x[f] == L.x[p]) ? 1 : -1;
+        }
+        n_anti += wave_sum_int(d);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        const int a = m.sites & 0xffff, b = (m.sites >> 16) & 0xffff;
+        L.x[a] = (signed char)-L.x[a];
+        if (b != 0xffff) L.x[b] = (signed char)-L.x[b];
+    }
+    __syncthreads();
+    const int s = rec_f(m.rec, 0);
+    if (s >= 0) L.w[wpos(s, P.K)] = eval_rec(P, tb, m.rec, L.x);
+    if (P.has_jz && !P.uniform_jz) {
+        for (int t = lane; t < P.RE; t += WAVE) {
+            const int e = P.mv_edges[(size_t)idx * P.RE + t];
+            if (e >= 0) {
+                const int pk = P.edge_pack[e];
+                L.anti[e] = L.x[pk & 0xffff] != L.x[(pk >> 16) & 0xffff];
+            }
+        }
+    }
+    __syncthreads();
+    return n_anti;
+}
+
 // flip the sites of move idx and re-evaluate what depends on them; returns
 // the new anti-parallel count
 template <typename T>
@@ -318,7 +480,7 @@ __device__ int apply_move(const DevProblem &P, const Tables<T> &tb, WalkerLds<T>
     }
     __syncthreads();
     if (P.full) {
-        int cnt = full_eval(P, tb, L, lane);
+        int cnt = full_eval<T, false>(P, tb, L, lane);
         __syncthreads();
         return cnt;
     }
@@ -345,7 +507,7 @@ __device__ int apply_move(const DevProblem &P, const Tables<T> &tb, WalkerLds<T>
 // ---------------------------------------------------------------------------
 // propagation kernel (evolve_walker_cont_time, evolution.py:559-721)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool FAST>
 __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
                                                     uint32_t *spins, double *log_w,
                                                     double *e_loc, double beta, double alw,
@@ -360,7 +522,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
     uint32_t *sp = spins + wk * P.words;
     load_spins(P, sp, L.x, lane);
     __syncthreads();
-    int n_anti = full_eval(P, tb, L, lane);
+    int n_anti = full_eval<T, FAST>(P, tb, L, lane);
     __syncthreads();
     double diag = diag_value(P, L.x, L.anti, n_anti, lane);
     double lw = log_w[wk];
@@ -375,11 +537,13 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
         double psi = gfmc_u53(r.z, r.w);
         int idx = -1;
         double tau_raw = INFINITY;
+        MovePrefetch m;
         if (R > (T)0) {
             T u = uniform_for<T>(r) * R;
             T cap = below(R);
             if (u > cap) u = cap;
             idx = sample_slot(P, L, incl, lane_sum, u, lane);
+            if constexpr (FAST) m = prefetch_move(P, idx, lane);  // in flight during the hold math
             tau_raw = -log1p(-psi) / (double)R;
         }
         double rem = beta - t;
@@ -388,8 +552,14 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
         t = t + tau;
         ++it;
         if (tau_raw < rem && idx >= 0) {
-            n_anti = apply_move(P, tb, L, idx, n_anti, lane);
-            diag = diag_value(P, L.x, L.anti, n_anti, lane);
+            if constexpr (FAST)
+                n_anti = apply_move_rec(P, tb, L, m, idx, n_anti, lane);
+            else
+                n_anti = apply_move(P, tb, L, idx, n_anti, lane);
+            if (!(P.has_jz && P.uniform_jz) || P.has_hz)
+                diag = diag_value(P, L.x, L.anti, n_anti, lane);
+            else
+                diag = (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti);
         }
     }
     lw = lw - alw * beta;
@@ -404,7 +574,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
 // ---------------------------------------------------------------------------
 // rates / log psi kernels
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool FAST>
 __global__ __launch_bounds__(WAVE) void k_rates(DevProblem P, Tables<T> tb, int64_t nw,
                                                 const uint32_t *spins, double *rates,
                                                 double *e_loc, double *log_diff) {
@@ -415,7 +585,7 @@ __global__ __launch_bounds__(WAVE) void k_rates(DevProblem P, Tables<T> tb, int6
     WalkerLds<T> L = carve<T>(P, lds);
     load_spins(P, spins + wk * P.words, L.x, lane);
     __syncthreads();
-    int n_anti = full_eval(P, tb, L, lane);
+    int n_anti = full_eval<T, FAST>(P, tb, L, lane);
     __syncthreads();
     double diag = diag_value(P, L.x, L.anti, n_anti, lane);
     T lane_sum;
@@ -747,6 +917,167 @@ static int upload_f32(gfmc_problem *p, const double *src, size_t n, const float 
     return upload(p, tmp.data(), n, dst);
 }
 
+// Host-built tables of the record path (see DevProblem).
+struct HostRecords {
+    std::vector<uint32_t> slot_rec, mv_rec;   // 8 words = 16 int16 per record
+    std::vector<int> mv_nbr, mv_sites, mv_edges, edge_pack;
+    int R = 0, RN = 0, RE = 0;
+    double J0 = 0.0, h0 = 0.0;
+};
+
+static void put16(uint32_t *rec, int k, int v) {
+    uint32_t &w = rec[k >> 1];
+    const int sh = (k & 1) * 16;
+    w = (w & ~(0xffffu << sh)) | ((uint32_t)(v & 0xffff) << sh);
+}
+
+// Decide whether the record path applies and build its tables; the packed
+// edge list is always built.  Returns true for the record path.
+static bool build_records(const gfmc_problem_desc *d, HostRecords &hr) {
+    const int N = d->n_sites, E = d->n_edges, n_hx = d->n_hx, n_off = d->n_hx + d->n_jpm;
+    hr.edge_pack.resize(E > 0 ? E : 1, 0);
+    for (int e = 0; e < E; ++e) hr.edge_pack[e] = d->edge_i[e] | (d->edge_j[e] << 16);
+    if (!d->dep_off || N >= 32768 || n_off >= 32768 || n_off == 0) return false;
+    for (int e = 0; e < E && d->n_jpm; ++e)
+        if (d->jpm[e] != d->jpm[0] || d->jpmpm[e] != d->jpmpm[0]) return false;
+    for (int k = 0; k < N && n_hx; ++k)
+        if (d->hx_half[k] != d->hx_half[0]) return false;
+    const bool jas = d->guide == GFMC_GUIDE_JASTROW;
+    const int D = jas ? d->jas_max_deg : 0;
+    if (D > REC_D) return false;
+    if (jas) {
+        bool have = false;
+        auto tied = [&](double v) {
+            if (v == 0.0) return true;
+            if (!have) {
+                hr.J0 = v;
+                have = true;
+                return true;
+            }
+            return v == hr.J0;
+        };
+        for (int i = 0; i < N * D; ++i)
+            if (!tied(d->jas_site_J[i])) return false;
+        for (int i = 0; i < E * 2 * D; ++i)
+            if (!tied(d->jas_edge_J[i])) return false;
+        hr.h0 = d->jas_h[0];
+        for (int k = 0; k < N; ++k)
+            if (d->jas_h[k] != hr.h0) return false;
+    }
+    // one record per slot
+    hr.slot_rec.assign((size_t)n_off * 8, 0u);
+    for (int s = 0; s < n_off; ++s) {
+        uint32_t *r = &hr.slot_rec[(size_t)s * 8];
+        int i, j, mask = 0;
+        if (s < n_hx) {
+            i = s;
+            j = -1;
+        } else {
+            i = d->edge_i[s - n_hx];
+            j = d->edge_j[s - n_hx];
+        }
+        put16(r, 0, s);
+        put16(r, 1, i);
+        put16(r, 2, j);
+        for (int q = 0; q < 2 * REC_D; ++q) put16(r, 4 + q, i);
+        if (jas) {
+            if (s < n_hx) {
+                for (int q = 0; q < D; ++q) {
+                    put16(r, 4 + q, d->jas_site_other[s * D + q]);
+                    if (d->jas_site_J[s * D + q] != 0.0) mask |= 1 << q;
+                }
+            } else {
+                const int e = s - n_hx;
+                for (int side = 0; side < 2; ++side)
+                    for (int q = 0; q < D; ++q) {
+                        const size_t o = ((size_t)e * 2 + side) * D + q;
+                        put16(r, 4 + side * REC_D + q, d->jas_edge_other[o]);
+                        if (d->jas_edge_J[o] != 0.0) mask |= 1 << (side * REC_D + q);
+                    }
+            }
+        }
+        put16(r, 3, mask);
+    }
+    // per-move record blocks
+    for (int s = 0; s < n_off; ++s) hr.R = std::max(hr.R, d->dep_off[s + 1] - d->dep_off[s]);
+    if (hr.R > WAVE) return false;
+    hr.mv_rec.assign((size_t)n_off * hr.R * 8, 0xffffffffu);
+    for (int s = 0; s < n_off; ++s)
+        for (int t = d->dep_off[s]; t < d->dep_off[s + 1]; ++t)
+            memcpy(&hr.mv_rec[((size_t)s * hr.R + (t - d->dep_off[s])) * 8],
+                   &hr.slot_rec[(size_t)d->dep_slots[t] * 8], 8 * sizeof(uint32_t));
+    // flipped sites and the neighbours whose bond state flips
+    std::vector<std::vector<int>> adj(N);
+    for (int e = 0; e < E; ++e) {
+        adj[d->edge_i[e]].push_back(d->edge_j[e]);
+        adj[d->edge_j[e]].push_back(d->edge_i[e]);
+    }
+    hr.mv_sites.resize(n_off);
+    std::vector<std::vector<int>> nb(n_off);
+    for (int s = 0; s < n_off; ++s) {
+        int a, b;
+        if (s < n_hx) {
+            a = s;
+            b = -1;
+        } else {
+            a = d->edge_i[s - n_hx];
+            b = d->edge_j[s - n_hx];
+        }
+        hr.mv_sites[s] = a | ((b < 0 ? 0xffff : b) << 16);
+        for (int f : {a, b}) {
+            if (f < 0) continue;
+            for (int p : adj[f])
+                if (p != a && p != b) nb[s].push_back(p | (f << 16));
+        }
+        hr.RN = std::max(hr.RN, (int)nb[s].size());
+    }
+    if (d->has_jz && d->uniform_jz && hr.RN > WAVE) return false;
+    hr.RN = std::max(hr.RN, 1);
+    hr.mv_nbr.assign((size_t)n_off * hr.RN, -1);
+    for (int s = 0; s < n_off; ++s)
+        for (size_t t = 0; t < nb[s].size(); ++t) hr.mv_nbr[(size_t)s * hr.RN + t] = nb[s][t];
+    // edges whose state a move may change (non-uniform J_z keeps flags)
+    if (d->has_jz) {
+        for (int s = 0; s < n_off; ++s)
+            hr.RE = std::max(hr.RE, d->edge_off[s + 1] - d->edge_off[s]);
+        hr.RE = std::max(hr.RE, 1);
+        hr.mv_edges.assign((size_t)n_off * hr.RE, -1);
+        for (int s = 0; s < n_off; ++s)
+            for (int t = d->edge_off[s]; t < d->edge_off[s + 1]; ++t)
+                hr.mv_edges[(size_t)s * hr.RE + (t - d->edge_off[s])] = d->dep_edges[t];
+    } else {
+        hr.mv_edges.assign(1, -1);
+    }
+    return true;
+}
+
+template <typename T>
+static void launch_rates(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
+                         const uint32_t *spins, double *rates, double *e_loc, double *log_diff,
+                         hipStream_t st) {
+    if (p->dp.fast)
+        hipLaunchKernelGGL((k_rates<T, true>), dim3((unsigned)nw), dim3(WAVE),
+                           (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, rates, e_loc, log_diff);
+    else
+        hipLaunchKernelGGL((k_rates<T, false>), dim3((unsigned)nw), dim3(WAVE),
+                           (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, rates, e_loc, log_diff);
+}
+
+template <typename T>
+static void launch_propagate(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
+                             uint32_t *spins, double *log_w, double *e_loc, double beta,
+                             double alw, uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
+                             int32_t *evals, hipStream_t st) {
+    if (p->dp.fast)
+        hipLaunchKernelGGL((k_propagate<T, true>), dim3((unsigned)nw), dim3(WAVE),
+                           (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, log_w, e_loc, beta,
+                           alw, k0, k1, step, tag, evals);
+    else
+        hipLaunchKernelGGL((k_propagate<T, false>), dim3((unsigned)nw), dim3(WAVE),
+                           (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, log_w, e_loc, beta,
+                           alw, k0, k1, step, tag, evals);
+}
+
 static void free_problem(gfmc_problem *p) {
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;
@@ -762,6 +1093,8 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     if (precision != GFMC_FP64 && precision != GFMC_FP32)
         return fail(GFMC_EINVAL, "gfmc_problem_create: precision must be GFMC_FP64 or GFMC_FP32");
     if (d->n_sites <= 0) return fail(GFMC_EINVAL, "gfmc_problem_create: n_sites must be > 0");
+    if (d->n_sites > 65535)
+        return fail(GFMC_EUNSUP, "gfmc_problem_create: at most 65535 sites per problem");
     if (d->n_edges < 0 || (d->n_edges > 0 && (!d->edge_i || !d->edge_j)))
         return fail(GFMC_EINVAL, "gfmc_problem_create: bad edge list");
     if (!(d->n_hx == 0 || d->n_hx == d->n_sites))
@@ -828,10 +1161,30 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     P.Ks = block_k_of(N);
     P.max_deg = d->guide == GFMC_GUIDE_JASTROW ? d->jas_max_deg : 0;
     P.full = d->dep_off == nullptr;
-    P.n_pad = round_up(N, 16);
-    P.e_pad = round_up(E > 0 ? E : 1, 16);
     P.jz0 = (d->has_jz && E > 0) ? d->jz[0] : 0.0;
     const size_t D = (size_t)P.max_deg;
+    HostRecords hr;
+    P.fast = build_records(d, hr) ? 1 : 0;
+    P.R = hr.R;
+    P.RN = hr.RN;
+    P.RE = hr.RE;
+    P.flags = d->has_jz && (!d->uniform_jz || !P.fast);
+    P.n_pad = round_up(N, 16);
+    P.e_pad = P.flags ? round_up(E > 0 ? E : 1, 16) : 0;
+    {
+        const double jpm0 = E ? d->jpm[0] : 0.0, jpmpm0 = E ? d->jpmpm[0] : 0.0;
+        const double hx0 = d->n_hx ? d->hx_half[0] : 0.0;
+        p->t64.jpm0 = jpm0;
+        p->t64.jpmpm0 = jpmpm0;
+        p->t64.hx0 = hx0;
+        p->t64.J0 = hr.J0;
+        p->t64.h0 = hr.h0;
+        p->t32.jpm0 = (float)jpm0;
+        p->t32.jpmpm0 = (float)jpmpm0;
+        p->t32.hx0 = (float)hx0;
+        p->t32.J0 = (float)hr.J0;
+        p->t32.h0 = (float)hr.h0;
+    }
     int rc = 0;
 #define UP(src, n, dst)                              \
     do {                                             \
@@ -849,6 +1202,17 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     } while (0)
     UP(d->edge_i, (size_t)E, &P.edge_i);
     UP(d->edge_j, (size_t)E, &P.edge_j);
+    UP(hr.edge_pack.data(), hr.edge_pack.size(), &P.edge_pack);
+    if (P.fast) {
+        const uint32_t *sr = nullptr, *mr = nullptr;
+        UP(hr.slot_rec.data(), hr.slot_rec.size(), &sr);
+        UP(hr.mv_rec.data(), hr.mv_rec.size(), &mr);
+        P.slot_rec = (const uint4 *)sr;
+        P.mv_rec = (const uint4 *)mr;
+        UP(hr.mv_nbr.data(), hr.mv_nbr.size(), &P.mv_nbr);
+        UP(hr.mv_sites.data(), hr.mv_sites.size(), &P.mv_sites);
+        UP(hr.mv_edges.data(), hr.mv_edges.size(), &P.mv_edges);
+    }
     if (d->has_jz) UP(d->jz, (size_t)E, &P.jz);
     if (d->has_hz) UP(d->hz, (size_t)N, &P.hz);
     if (d->guide == GFMC_GUIDE_JASTROW) {
@@ -979,11 +1343,9 @@ int gfmc_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double 
     if (nw == 0) return 0;
     if (!spins) return fail(GFMC_EINVAL, "gfmc_rates: NULL spins");
     if (p->precision == GFMC_FP64)
-        hipLaunchKernelGGL(k_rates<double>, dim3((unsigned)nw), dim3(WAVE), (size_t)p->lds_bytes,
-                           (hipStream_t)stream, p->dp, p->t64, nw, spins, rates, e_loc, log_diff);
+        launch_rates(p, p->t64, nw, spins, rates, e_loc, log_diff, (hipStream_t)stream);
     else
-        hipLaunchKernelGGL(k_rates<float>, dim3((unsigned)nw), dim3(WAVE), (size_t)p->lds_bytes,
-                           (hipStream_t)stream, p->dp, p->t32, nw, spins, rates, e_loc, log_diff);
+        launch_rates(p, p->t32, nw, spins, rates, e_loc, log_diff, (hipStream_t)stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1011,13 +1373,11 @@ int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
         return fail(GFMC_EINVAL, "gfmc_propagate: beta must be finite and > 0");
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     if (p->precision == GFMC_FP64)
-        hipLaunchKernelGGL(k_propagate<double>, dim3((unsigned)nw), dim3(WAVE),
-                           (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t64, nw, spins,
-                           log_w, e_loc, beta, alw, k0, k1, step, tag, evals);
+        launch_propagate(p, p->t64, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
+                         evals, (hipStream_t)stream);
     else
-        hipLaunchKernelGGL(k_propagate<float>, dim3((unsigned)nw), dim3(WAVE),
-                           (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
-                           log_w, e_loc, beta, alw, k0, k1, step, tag, evals);
+        launch_propagate(p, p->t32, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
+                         evals, (hipStream_t)stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
