@@ -205,6 +205,12 @@ int gfmc_sample_index(int32_t precision, int32_t n, const double *weights,
                       int32_t n_draws, const double *uniforms, int32_t *out,
                       void *stream);
 
+/* Device test hook for the 64-lane inclusive scan every kernel reduction
+ * uses (the "wave tree": Hillis-Steele within 16-lane rows, then the row
+ * totals; no reference counterpart -- it fixes the summation order that the
+ * oracle mirrors).  in/out: 64 device doubles; precision as above. */
+int gfmc_wave_scan(int32_t precision, const double *in, double *out, void *stream);
+
 /* Philox4x32-10 block (Salmon et al., SC'11) for the known-answer test:
  * out[4*i..] = philox(ctr[4*i..], key[0..1]); host pointers. */
 int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key,

@@ -56,6 +56,7 @@ def lib():
         L.orc_philox.argtypes = [_u32p, _u32p, _u32p]
         L.orc_u53.restype = ctypes.c_double
         L.orc_u53.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_wave_tree.argtypes = [_f64p]
         L.orc_blocked_scan.restype = ctypes.c_double
         L.orc_blocked_scan.argtypes = [_f64p, ctypes.c_int, _f64p]
         L.orc_sample_many.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _i32p]
@@ -145,6 +146,14 @@ def philox(ctr, key):
     lib().orc_philox(_p(ctr, ctypes.c_uint32), _p(key, ctypes.c_uint32),
                      _p(out, ctypes.c_uint32))
     return out
+
+
+def wave_tree(v):
+    """Inclusive scan of 64 values in the kernels' wave-tree order."""
+    s = np.array(v, dtype=np.float64, copy=True)
+    assert s.shape == (64,)
+    lib().orc_wave_tree(_p(s, ctypes.c_double))
+    return s
 
 
 def blocked_scan(v):

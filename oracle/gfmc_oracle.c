@@ -10,7 +10,7 @@
  * uniforms -- is the reduction order of the off-diagonal weight sum and its
  * CDF: "blocked wave order" (blocked_scan below), where slot s belongs to
  * lane s / K (K = ceil(n / 64)), each lane sums its block sequentially and
- * the 64 lane sums are combined by a Hillis-Steele inclusive scan.  The
+ * the 64 lane sums are combined by the wave-tree inclusive scan.  The
  * reference's own order is XLA's (cumsum, evolution.py:633); both are the
  * same sum to within a few ulps.  Reconfiguration uses the same idea at
  * 1024-thread block width (block_prefix below).
@@ -67,17 +67,31 @@ static void draw2(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t step, uint32
 
 /* ------------------------------------------------------------------------ */
 /* Blocked wave order: lane l owns [l*K, (l+1)*K); lane sums sequential from */
-/* 0.0; Hillis-Steele inclusive scan over the 64 lane sums.                  */
+/* 0.0; wave-tree inclusive scan (wave_tree) over the 64 lane sums.         */
 /* ------------------------------------------------------------------------ */
 static int block_k(int n) { return n > 0 ? (n + WAVE - 1) / WAVE : 1; }
 
-static void hillis_steele(double *s, int width) {
-    double t[RECONF_THREADS];
-    for (int d = 1; d < width; d <<= 1) {
-        for (int l = 0; l < width; ++l) t[l] = (l >= d) ? s[l] + s[l - d] : s[l];
-        memcpy(s, t, sizeof(double) * (size_t)width);
+/* Inclusive scan of 64 lane values in "wave tree" order: Hillis-Steele
+ * inside each 16-lane row (offsets 1, 2, 4, 8), then rows 1 and 3 add the
+ * last lane of the row before, then rows 2 and 3 add lane 31 -- the order in
+ * which the kernels' DPP scan (row_shr, row_bcast:15, row_bcast:31) adds. */
+static void wave_tree(double *s) {
+    double t[WAVE];
+    for (int d = 1; d < 16; d <<= 1) {
+        memcpy(t, s, sizeof t);
+        for (int l = 0; l < WAVE; ++l)
+            if ((l & 15) >= d) s[l] = t[l] + t[l - d];
     }
+    memcpy(t, s, sizeof t);
+    for (int l = 0; l < WAVE; ++l) {
+        int row = l >> 4;
+        if (row == 1 || row == 3) s[l] = t[l] + t[row * 16 - 1];
+    }
+    memcpy(t, s, sizeof t);
+    for (int l = 32; l < WAVE; ++l) s[l] = t[l] + t[31];
 }
+
+void orc_wave_tree(double *s) { wave_tree(s); }
 
 /* lane_incl[64] <- inclusive scan of the lane sums of v[0:n]; returns total. */
 double orc_blocked_scan(const double *v, int n, double *lane_incl) {
@@ -91,7 +105,7 @@ double orc_blocked_scan(const double *v, int n, double *lane_incl) {
         }
         s[l] = acc;
     }
-    hillis_steele(s, WAVE);
+    wave_tree(s);
     if (lane_incl) memcpy(lane_incl, s, sizeof s);
     return s[WAVE - 1];
 }
@@ -389,7 +403,7 @@ int orc_propagate(const gfmc_problem_desc *d, int64_t nw, uint32_t *spins, doubl
 /* ------------------------------------------------------------------------ */
 /* Reconfiguration (reconfiguration.py:9-62) in 1024-thread block order:     */
 /* thread t owns the contiguous chunk [t*C, (t+1)*C), C = ceil(n/1024); the  */
-/* chunk sums are combined by a Hillis-Steele scan inside each 64-wide wave  */
+/* chunk sums are combined by the wave-tree scan inside each 64-wide wave    */
 /* and a sequential scan over the 16 wave totals.                            */
 /* ------------------------------------------------------------------------ */
 static void block_incl(const double *v, int64_t n, double *incl /*[1024]*/) {
@@ -400,7 +414,7 @@ static void block_incl(const double *v, int64_t n, double *incl /*[1024]*/) {
         for (int64_t k = t * C; k < (t + 1) * C && k < n; ++k) acc += v[k];
         incl[t] = acc;
     }
-    for (int wv = 0; wv < RECONF_THREADS / WAVE; ++wv) hillis_steele(incl + wv * WAVE, WAVE);
+    for (int wv = 0; wv < RECONF_THREADS / WAVE; ++wv) wave_tree(incl + wv * WAVE);
     double P = 0.0;
     for (int wv = 0; wv < RECONF_THREADS / WAVE; ++wv) {
         double tot = incl[wv * WAVE + WAVE - 1];

@@ -88,6 +88,7 @@ SIGNATURES = [
                                       ctypes.c_uint32, _vp, _vp, _vp]),
     ("gfmc_sample_index", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32,
                                          _vp, _vp, _vp]),
+    ("gfmc_wave_scan", ctypes.c_int, [ctypes.c_int32, _vp, _vp, _vp]),
     ("gfmc_philox_host", ctypes.c_int, [ctypes.c_int32, _u32p, _u32p, _u32p]),
 ]
 

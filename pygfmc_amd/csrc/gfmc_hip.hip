@@ -106,14 +106,46 @@ static int round_up(int n, int m) { return (n + m - 1) / m * m; }
 // ---------------------------------------------------------------------------
 // wave helpers
 // ---------------------------------------------------------------------------
+// DPP lane moves: value of the lane the control selects, 0 where that lane
+// is outside the row (bound_ctrl) or this lane's row is masked off.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_get(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, true));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_get(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Inclusive scan over the 64 lanes in "wave tree" order: Hillis-Steele
+// inside each 16-lane row (row_shr 1, 2, 4, 8), then rows 1 and 3 add the
+// last lane of the row before (row_bcast:15), then rows 2 and 3 add lane 31
+// (row_bcast:31).  All data movement is DPP; no LDS round trips.
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v, int lane) {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        T o = __shfl_up(v, d, WAVE);
-        if (lane >= d) v += o;
-    }
+    (void)lane;
+    v += dpp_get<0x111, 0xf>(v);
+    v += dpp_get<0x112, 0xf>(v);
+    v += dpp_get<0x114, 0xf>(v);
+    v += dpp_get<0x118, 0xf>(v);
+    v += dpp_get<0x142, 0xa>(v);
+    v += dpp_get<0x143, 0xc>(v);
     return v;
+}
+
+__device__ __forceinline__ float lane_bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ int lane_bcast(int v, int src) {
+    return __builtin_amdgcn_readlane(v, src);
 }
 
 __device__ __forceinline__ int wave_sum_int(int v) {
@@ -264,7 +296,7 @@ __device__ double diag_value(const DevProblem &P, const signed char *x,
                 if (e < P.n_edges) acc += anti[e] ? -P.jz[e] / 4.0 : P.jz[e] / 4.0;
             }
             acc = wave_incl_scan(acc, lane);
-            d = __shfl(acc, WAVE - 1, WAVE);
+            d = lane_bcast(acc, WAVE - 1);
         }
     }
     if (P.has_hz) {
@@ -274,7 +306,7 @@ __device__ double diag_value(const DevProblem &P, const signed char *x,
             if (s < P.N) acc += P.hz[s] * (double)x[s];
         }
         acc = wave_incl_scan(acc, lane);
-        d = d + __shfl(acc, WAVE - 1, WAVE) / 2;
+        d = d + lane_bcast(acc, WAVE - 1) / 2;
     }
     return d;
 }
@@ -321,21 +353,21 @@ __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
 template <typename T, bool FAST>
 __device__ int full_eval(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L, int lane) {
     if constexpr (FAST) {
-        // two records in flight per lane
-        for (int k0 = 0; k0 < P.K; k0 += 2) {
-            Rec r[2];
+        // coalesced: slot s0 + q*64 + lane, four records in flight per lane
+        for (int s0 = 0; s0 < P.n_off; s0 += 4 * WAVE) {
+            Rec r[4];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int s = lane * P.K + k0 + q;
-                r[q] = (k0 + q < P.K && s < P.n_off) ? load_rec(P.slot_rec, s) : empty_rec();
+            for (int q = 0; q < 4; ++q) {
+                const int s = s0 + q * WAVE + lane;
+                r[q] = s < P.n_off ? load_rec(P.slot_rec, s) : empty_rec();
             }
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                if (k0 + q < P.K)
-                    L.w[(k0 + q) * WAVE + lane] =
-                        rec_f(r[q], 0) >= 0 ? eval_rec(P, tb, r[q], L.x) : (T)0;
+            for (int q = 0; q < 4; ++q) {
+                const int s = rec_f(r[q], 0);
+                if (s >= 0) L.w[wpos(s, P.K)] = eval_rec(P, tb, r[q], L.x);
             }
         }
+        for (int s = P.n_off + lane; s < P.K * WAVE; s += WAVE) L.w[wpos(s, P.K)] = (T)0;
     } else {
         for (int k = 0; k < P.K; ++k) {
             int s = lane * P.K + k;
@@ -372,7 +404,7 @@ __device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T
                            int lane) {
     unsigned long long hit = __ballot(incl > u);
     int j = hit ? __ffsll((unsigned long long)hit) - 1 : WAVE - 1;
-    T prev = __shfl(incl, j > 0 ? j - 1 : 0, WAVE);
+    T prev = lane_bcast(incl, j > 0 ? j - 1 : 0);
     if (j == 0) prev = (T)0;
     int idx = -1;
     if (lane == j) {
@@ -392,7 +424,7 @@ __device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T
             }
         }
     }
-    idx = __shfl(idx, j, WAVE);
+    idx = lane_bcast(idx, j);
     if (idx < 0) {
         // rounding left u past lane j: the last positive slot at or before it
         unsigned long long pos = __ballot(lane_sum > (T)0 && lane <= j);
@@ -407,7 +439,7 @@ __device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T
                 }
             }
         }
-        idx = __shfl(found, jj, WAVE);
+        idx = lane_bcast(found, jj);
     }
     return idx;
 }
@@ -434,12 +466,13 @@ template <typename T>
 __device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L,
                               const MovePrefetch &m, int idx, int n_anti, int lane) {
     if (P.has_jz && P.uniform_jz) {
-        int d = 0;
+        bool plus = false, minus = false;
         if (m.nbr >= 0) {
             const int p = m.nbr & 0xffff, f = (m.nbr >> 16) & 0xffff;
-            d = (L.x[f] == L.x[p]) ? 1 : -1;
+            plus = L.x[f] == L.x[p];
+            minus = !plus;
         }
-        n_anti += wave_sum_int(d);
+        n_anti += __popcll(__ballot(plus)) - __popcll(__ballot(minus));
     }
     __syncthreads();
     if (lane == 0) {
@@ -531,7 +564,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
     while (t < beta) {
         T lane_sum;
         T incl = lane_scan(P, L, lane, &lane_sum);
-        T R = __shfl(incl, WAVE - 1, WAVE);
+        T R = lane_bcast(incl, WAVE - 1);
         eL = diag - (double)R;
         gfmc_u4 r = gfmc_philox(it, (uint32_t)wk, step, tag, key0, key1);
         double psi = gfmc_u53(r.z, r.w);
@@ -590,7 +623,7 @@ __global__ __launch_bounds__(WAVE) void k_rates(DevProblem P, Tables<T> tb, int6
     double diag = diag_value(P, L.x, L.anti, n_anti, lane);
     T lane_sum;
     T incl = lane_scan(P, L, lane, &lane_sum);
-    T R = __shfl(incl, WAVE - 1, WAVE);
+    T R = lane_bcast(incl, WAVE - 1);
     const int64_t M = P.n_off + 1;
     if (rates) {
         if (lane == 0) rates[wk * M] = diag;
@@ -861,6 +894,14 @@ __global__ void k_gather(int words, int64_t nw, const uint32_t *spins_in, const 
     }
 }
 
+// wave-tree test hook: one wave scans 64 values
+template <typename T>
+__global__ __launch_bounds__(WAVE) void k_wave_scan(const double *in, double *out) {
+    const int lane = threadIdx.x;
+    T v = (T)in[lane];
+    out[lane] = (double)wave_incl_scan(v, lane);
+}
+
 // sampler test hook: one wave, weights in LDS in the propagator's layout
 template <typename T>
 __global__ __launch_bounds__(WAVE) void k_sample(int n, const double *weights, int n_draws,
@@ -880,7 +921,7 @@ __global__ __launch_bounds__(WAVE) void k_sample(int n, const double *weights, i
     __syncthreads();
     T lane_sum;
     T incl = lane_scan(P, L, lane, &lane_sum);
-    T R = __shfl(incl, WAVE - 1, WAVE);
+    T R = lane_bcast(incl, WAVE - 1);
     for (int d = 0; d < n_draws; ++d) {
         T u = (T)uniforms[d] * R;
         T cap = below(R);
@@ -1465,6 +1506,18 @@ int gfmc_sample_index(int32_t precision, int32_t n, const double *weights, int32
     else
         hipLaunchKernelGGL(k_sample<float>, dim3(1), dim3(WAVE), (size_t)K * WAVE * 4,
                            (hipStream_t)stream, n, weights, n_draws, uniforms, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_wave_scan(int32_t precision, const double *in, double *out, void *stream) {
+    if (!in || !out) return fail(GFMC_EINVAL, "gfmc_wave_scan: NULL buffer");
+    if (precision == GFMC_FP64)
+        hipLaunchKernelGGL(k_wave_scan<double>, dim3(1), dim3(WAVE), 0, (hipStream_t)stream, in,
+                           out);
+    else
+        hipLaunchKernelGGL(k_wave_scan<float>, dim3(1), dim3(WAVE), 0, (hipStream_t)stream, in,
+                           out);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -71,6 +71,58 @@ def random_spins(orc, N, nw, seed, sz=None):
     return orc.init_sz(N, nw, (N + sz) // 2, seed)
 
 
+def wave_tree_inputs(seed):
+    """64 values of mixed magnitude: reassociating their sum changes the
+    rounded result, so only the exact tree reproduces the oracle bitwise."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0, 1, 64) * 10.0 ** rng.integers(-8, 8, 64)
+
+
+def test_wave_tree_inputs_distinguish_summation_orders(orc):
+    v = wave_tree_inputs(0)
+    tree = orc.wave_tree(v)
+    seq = np.cumsum(v)
+    assert np.any(tree != seq)                    # a sequential scan would be caught
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_wave_scan_matches_oracle_tree_bitwise(orc, prec):
+    say("wave tree", prec)
+    L = _lib.load()
+    for seed in range(5):
+        v = wave_tree_inputs(seed)
+        if prec == "fp32":
+            v = v.astype(np.float32).astype(np.float64)
+        src = torch.as_tensor(v, device=dev())
+        out = torch.empty(64, dtype=torch.float64, device=dev())
+        _lib.check(L.gfmc_wave_scan(0 if prec == "fp64" else 1, _lib.ptr(src), _lib.ptr(out),
+                                    _lib.stream()), "gfmc_wave_scan")
+        got = out.cpu().numpy()
+        if prec == "fp64":
+            np.testing.assert_array_equal(got, orc.wave_tree(v))
+        else:
+            ref = wave_tree_f32(v)
+            np.testing.assert_array_equal(got.astype(np.float32), ref)
+
+
+def wave_tree_f32(v):
+    """The wave tree evaluated in float32 (numpy), for the FP32 kernels."""
+    s = np.asarray(v, dtype=np.float32).copy()
+    for d in (1, 2, 4, 8):
+        t = s.copy()
+        for l in range(64):
+            if (l & 15) >= d:
+                s[l] = np.float32(t[l] + t[l - d])
+    t = s.copy()
+    for l in range(64):
+        if (l >> 4) in (1, 3):
+            s[l] = np.float32(t[l] + t[(l >> 4) * 16 - 1])
+    t = s.copy()
+    for l in range(32, 64):
+        s[l] = np.float32(t[l] + t[31])
+    return s
+
+
 @pytest.mark.parametrize("name,H,G", SYSTEMS, ids=IDS)
 def test_rates_fp64_match_oracle(orc, name, H, G):
     say("rates fp64", name)

@@ -135,6 +135,21 @@ def test_jackknife_error_of_iid_series_is_the_standard_error():
     assert err[1] == pytest.approx(E[1:].std(ddof=1) / np.sqrt(399), rel=0.02)
 
 
+# -- the wave-tree summation order ------------------------------------------------
+
+def test_wave_tree_is_an_inclusive_scan(orc):
+    """Exact on integers (no rounding); within a few ulps of the exact prefix
+    sum otherwise; and really a different association than sequential."""
+    rng = np.random.default_rng(0)
+    ints = rng.integers(-1000, 1000, 64).astype(float)
+    np.testing.assert_array_equal(orc.wave_tree(ints), np.cumsum(ints))
+    v = rng.uniform(0, 1, 64) * 10.0 ** rng.integers(-8, 8, 64)
+    from fractions import Fraction
+    exact = np.array([float(sum(Fraction(x) for x in v[:k + 1])) for k in range(64)])
+    np.testing.assert_allclose(orc.wave_tree(v), exact, rtol=4e-16 * 8)
+    assert np.any(orc.wave_tree(v) != np.cumsum(v))
+
+
 # -- Philox4x32-10 known-answer vectors (Random123 kat_vectors) -------------------
 
 PHILOX_KAT = [
