@@ -28,13 +28,16 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--delta", type=float, default=1.0)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--walkers", type=int, default=65536)
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="before the timed steps, copy a tensor of this many MiB once "
+                         "(a plain copy kernel of known size, to scale HBM counters)")
     return ap.parse_args()
 
 
@@ -77,6 +80,14 @@ def main():
             lg = series[t - 4: t + 1, 0].cpu().numpy()
             alw += float(np.mean(lg)) / beta
     torch.cuda.synchronize()
+
+    if args.calib_copy_mib > 0:
+        src = torch.ones(args.calib_copy_mib * (1 << 18), dtype=torch.float32,
+                         device=chain.spins.device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     chain.evals.zero_()
     events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))

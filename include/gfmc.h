@@ -167,12 +167,16 @@ int gfmc_propagate(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
  * log-sum-exp of log_w, normalised weights, E_n = sum w e_loc, the prefix
  * sum, parents[i] = min(searchsorted_right(cdf, (i+psi)/n), last positive).
  * psi is Philox (0, 0, step, tag) unless psi_opt is non-NULL (a host double).
- * out2 (device, 2 doubles) <- {log G_n - log n, E_n}.  work: device scratch
- * of n_walkers doubles (the prefix sum), caller-owned. */
+ * out2 (device, 2 doubles) <- {log G_n - log n, E_n}.  work: caller-owned
+ * device scratch of gfmc_reconfigure_work_doubles(n_walkers) doubles (the
+ * prefix sum and per-block partials).  Summation order: blocks of 1024
+ * walkers, 4 per thread, the wave tree inside a block, block totals in
+ * block order. */
 int gfmc_reconfigure(int64_t n_walkers, const double *log_w, const double *e_loc,
                      uint64_t seed, uint32_t step, uint32_t tag,
                      const double *psi_opt, int32_t *parents, double *out2,
                      double *work, void *stream);
+int64_t gfmc_reconfigure_work_doubles(int64_t n_walkers);
 
 /* The gather that follows reconfiguration: spins_out[i] = spins_in[parents[i]],
  * e_loc_out[i] = e_loc_in[parents[i]], log_w_out[i] = 0 (NULL skips). */
@@ -186,7 +190,8 @@ int gfmc_gather(int32_t n_sites, int64_t n_walkers, const uint32_t *spins_in,
  * tag_reconf, gfmc_gather (_make_prod_step_fn, driver.py:194-342, without
  * operators).  The walkers (spins, log_w = 0 after each step, e_loc) are
  * updated in place; spins_b / e_loc_b are same-sized scratch; parents
- * [n_walkers] int32 and work [n_walkers] doubles are scratch.
+ * [n_walkers] int32 and work [gfmc_reconfigure_work_doubles(n_walkers)] doubles
+ * are scratch.
  * series [2*n_steps] doubles <- {log G_t - log n, E_t} per step.  evals, when
  * non-NULL, [n_walkers] accumulates the rate evaluations of every step (of
  * the walker slot, before resampling).  Launches are enqueued on stream;

@@ -12,8 +12,8 @@
  * lane s / K (K = ceil(n / 64)), each lane sums its block sequentially and
  * the 64 lane sums are combined by the wave-tree inclusive scan.  The
  * reference's own order is XLA's (cumsum, evolution.py:633); both are the
- * same sum to within a few ulps.  Reconfiguration uses the same idea at
- * 1024-thread block width (block_prefix below).
+ * same sum to within a few ulps.  Reconfiguration uses the same idea with
+ * 1024-walker blocks (rc_block_incl below).
  *
  * Parity status of each function (how it is pinned) is listed in oracle/README.md.
  */
@@ -28,7 +28,6 @@
 #include "../include/gfmc.h"
 
 #define WAVE 64
-#define RECONF_THREADS 1024
 
 /* ------------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as  */
@@ -401,61 +400,80 @@ int orc_propagate(const gfmc_problem_desc *d, int64_t nw, uint32_t *spins, doubl
 }
 
 /* ------------------------------------------------------------------------ */
-/* Reconfiguration (reconfiguration.py:9-62) in 1024-thread block order:     */
-/* thread t owns the contiguous chunk [t*C, (t+1)*C), C = ceil(n/1024); the  */
-/* chunk sums are combined by the wave-tree scan inside each 64-wide wave    */
-/* and a sequential scan over the 16 wave totals.                            */
+/* Reconfiguration (reconfiguration.py:9-62) in block order: blocks of RB = */
+/* 1024 walkers, 256 threads x 4 contiguous walkers; each thread adds its 4 */
+/* walkers sequentially from 0, a block combines its 256 thread values with */
+/* the wave tree inside each 64-wide wave and sequentially over the 4 wave  */
+/* totals, and block totals are combined sequentially in block order.       */
 /* ------------------------------------------------------------------------ */
-static void block_incl(const double *v, int64_t n, double *incl /*[1024]*/) {
-    int64_t C = (n + RECONF_THREADS - 1) / RECONF_THREADS;
-    if (C < 1) C = 1;
-    for (int t = 0; t < RECONF_THREADS; ++t) {
+#define RC_T 256
+#define RC_PER 4
+#define RB (RC_T * RC_PER)
+
+static void rc_block_incl(const double *v, int64_t n, int64_t b, double *incl /*[RC_T]*/) {
+    for (int t = 0; t < RC_T; ++t) {
         double acc = 0.0;
-        for (int64_t k = t * C; k < (t + 1) * C && k < n; ++k) acc += v[k];
+        for (int c = 0; c < RC_PER; ++c) {
+            int64_t k = b * RB + (int64_t)t * RC_PER + c;
+            if (k < n) acc += v[k];
+        }
         incl[t] = acc;
     }
-    for (int wv = 0; wv < RECONF_THREADS / WAVE; ++wv) wave_tree(incl + wv * WAVE);
+    for (int wv = 0; wv < RC_T / WAVE; ++wv) wave_tree(incl + wv * WAVE);
     double P = 0.0;
-    for (int wv = 0; wv < RECONF_THREADS / WAVE; ++wv) {
+    for (int wv = 0; wv < RC_T / WAVE; ++wv) {
         double tot = incl[wv * WAVE + WAVE - 1];
         for (int l = 0; l < WAVE; ++l) incl[wv * WAVE + l] = P + incl[wv * WAVE + l];
         P = P + tot;
     }
 }
 
-static double block_total(const double *v, int64_t n) {
-    double incl[RECONF_THREADS];
-    block_incl(v, n, incl);
-    return incl[RECONF_THREADS - 1];
+/* block totals in block order, summed sequentially from 0 */
+static double rc_total(const double *v, int64_t n, double *btot /* [B] or NULL */) {
+    int64_t B = (n + RB - 1) / RB;
+    double incl[RC_T], S = 0.0;
+    for (int64_t b = 0; b < B; ++b) {
+        rc_block_incl(v, n, b, incl);
+        if (btot) btot[b] = incl[RC_T - 1];
+        S = S + incl[RC_T - 1];
+    }
+    return S;
 }
 
 /* psi: the shared offset in [0, 1).  out2 <- {log G_n - log n, E_n}. */
 int orc_reconfigure(int64_t n, const double *log_w, const double *e_loc, double psi,
                     int32_t *parents, double *out2, double *cdf_out) {
+    int64_t B = (n + RB - 1) / RB;
     double *v = (double *)malloc(sizeof(double) * (size_t)n);
     double *wn = (double *)malloc(sizeof(double) * (size_t)n);
     double *cdf = (double *)malloc(sizeof(double) * (size_t)n);
+    double *bw = (double *)malloc(sizeof(double) * (size_t)B);
     /* logsumexp with a non-finite max replaced by 0 (jax.scipy.special.logsumexp) */
     double m = -INFINITY;
     for (int64_t k = 0; k < n; ++k) m = log_w[k] > m ? log_w[k] : m;
     double m0 = isfinite(m) ? m : 0.0;
     for (int64_t k = 0; k < n; ++k) v[k] = exp(log_w[k] - m0);
-    double logG = log(block_total(v, n)) + m0;
+    double logG = log(rc_total(v, n, NULL)) + m0;
     for (int64_t k = 0; k < n; ++k) wn[k] = exp(log_w[k] - logG);
     for (int64_t k = 0; k < n; ++k) v[k] = e_loc[k] * wn[k];
-    double En = block_total(v, n);
-    /* prefix sum: chunk-sequential from the previous thread's inclusive sum,
-     * then a running max so the CDF is non-decreasing despite rounding */
-    double incl[RECONF_THREADS];
-    block_incl(wn, n, incl);
-    int64_t C = (n + RECONF_THREADS - 1) / RECONF_THREADS;
-    if (C < 1) C = 1;
-    for (int t = 0; t < RECONF_THREADS; ++t) {
-        double c = t == 0 ? 0.0 : incl[t - 1];
-        for (int64_t k = t * C; k < (t + 1) * C && k < n; ++k) {
-            c = c + wn[k];
-            cdf[k] = c;
+    double En = rc_total(v, n, NULL);
+    /* prefix sum: block prefix + thread prefix, then sequential within the
+     * thread's 4 walkers; then a running max so the CDF is non-decreasing */
+    rc_total(wn, n, bw);
+    double incl[RC_T], Pb = 0.0;
+    for (int64_t b = 0; b < B; ++b) {
+        rc_block_incl(wn, n, b, incl);
+        for (int t = 0; t < RC_T; ++t) {
+            double c = Pb + (t == 0 ? 0.0 : incl[t - 1]);
+            for (int q = 0; q < RC_PER; ++q) {
+                int64_t k = b * RB + (int64_t)t * RC_PER + q;
+                if (k < n) {
+                    c = c + wn[k];
+                    cdf[k] = c;
+                }
+            }
         }
+        Pb = Pb + bw[b];
     }
     for (int64_t k = 1; k < n; ++k)
         if (cdf[k] < cdf[k - 1]) cdf[k] = cdf[k - 1];
@@ -477,6 +495,7 @@ int orc_reconfigure(int64_t n, const double *log_w, const double *e_loc, double 
     free(v);
     free(wn);
     free(cdf);
+    free(bw);
     return 0;
 }
 

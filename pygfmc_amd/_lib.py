@@ -80,6 +80,7 @@ SIGNATURES = [
     ("gfmc_reconfigure", ctypes.c_int, [ctypes.c_int64, _vp, _vp, ctypes.c_uint64,
                                         ctypes.c_uint32, ctypes.c_uint32, _f64p, _vp, _vp,
                                         _vp, _vp]),
+    ("gfmc_reconfigure_work_doubles", ctypes.c_int64, [ctypes.c_int64]),
     ("gfmc_gather", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp,
                                    _vp, _vp]),
     ("gfmc_run_steps", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
@@ -134,6 +135,11 @@ def device():
 def stream():
     """Handle of torch's current HIP stream, as the void* the C-ABI takes."""
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def work_doubles(n_walkers):
+    """Doubles of device scratch gfmc_reconfigure needs for n_walkers."""
+    return int(load().gfmc_reconfigure_work_doubles(int(n_walkers)))
 
 
 def ptr(t):

@@ -734,39 +734,64 @@ __global__ void k_unpack(int N, int words, int64_t nw, const uint32_t *spins, do
 }
 
 // ---------------------------------------------------------------------------
-// reconfiguration (reconfiguration.py:9-62), one 1024-thread block
+// reconfiguration (reconfiguration.py:9-62): six short kernels over blocks
+// of RB = 1024 walkers (256 threads x 4 contiguous walkers).  Summation
+// order: each thread adds its 4 walkers sequentially from 0; the 256 thread
+// values of a block are combined by the wave tree inside each wave and
+// sequentially over the 4 wave totals; block totals are combined
+// sequentially in block order.  Scratch layout (doubles) after the n-entry
+// CDF: bm[B] bs[B] bw[B] be[B] bml[B] blp[B] logG[1].
 // ---------------------------------------------------------------------------
-// inclusive scan of per-thread values in block order (mirrors block_incl of
-// the oracle): Hillis-Steele per wave, sequential over the 16 wave totals
-__device__ double block_incl(double v, double *sh_tot) {
+#define RC_T 256
+#define RC_PER 4
+#define RB (RC_T * RC_PER)
+
+struct RcScratch {
+    double *cdf, *bm, *bs, *bw, *be, *bml, *blp, *logG;
+};
+
+__device__ __forceinline__ RcScratch rc_scratch(double *work, int64_t n, int B) {
+    RcScratch s;
+    s.cdf = work;
+    s.bm = work + n;
+    s.bs = s.bm + B;
+    s.bw = s.bs + B;
+    s.be = s.bw + B;
+    s.bml = s.be + B;
+    s.blp = s.bml + B;
+    s.logG = s.blp + B;
+    return s;
+}
+
+// inclusive scan of per-thread values in block order
+__device__ double rc_block_incl(double v, double *sh4) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double incl = wave_incl_scan(v, lane);
-    if (lane == 63) sh_tot[wv] = incl;
+    if (lane == 63) sh4[wv] = incl;
     __syncthreads();
     double P = 0.0;
-    for (int q = 0; q < wv; ++q) P = P + sh_tot[q];
+    for (int q = 0; q < wv; ++q) P = P + sh4[q];
     __syncthreads();
     return P + incl;
 }
 
-__device__ double block_max(double v, double *sh) {
+__device__ double rc_block_max(double v, double *sh4) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         double o = __shfl_xor(v, d, WAVE);
         v = o > v ? o : v;
     }
-    if (lane == 0) sh[wv] = v;
+    if (lane == 0) sh4[wv] = v;
     __syncthreads();
-    double m = sh[0];
-    for (int q = 1; q < RECONF_THREADS / WAVE; ++q) m = sh[q] > m ? sh[q] : m;
+    double m = sh4[0];
+    for (int q = 1; q < RC_T / WAVE; ++q) m = sh4[q] > m ? sh4[q] : m;
     __syncthreads();
     return m;
 }
 
-// exclusive running max over threads in block order (max is exact, so any
-// tree gives the oracle's sequential running max)
-__device__ double block_excl_max(double v, double *sh) {
+// exclusive running max over threads in block order (exact in any tree)
+__device__ double rc_block_excl_max(double v, double *sh4) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double incl = v;
 #pragma unroll
@@ -776,84 +801,154 @@ __device__ double block_excl_max(double v, double *sh) {
     }
     double excl = __shfl_up(incl, 1, WAVE);
     if (lane == 0) excl = -INFINITY;
-    __syncthreads();
-    if (lane == 63) sh[wv] = incl;
+    if (lane == 63) sh4[wv] = incl;
     __syncthreads();
     double P = -INFINITY;
-    for (int q = 0; q < wv; ++q) P = sh[q] > P ? sh[q] : P;
+    for (int q = 0; q < wv; ++q) P = sh4[q] > P ? sh4[q] : P;
     __syncthreads();
     return P > excl ? P : excl;
 }
 
-__device__ long long block_max_ll(long long v, long long *sh) {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        long long o = __shfl_xor(v, d, WAVE);
-        v = o > v ? o : v;
-    }
-    if (lane == 0) sh[wv] = v;
-    __syncthreads();
-    long long m = sh[0];
-    for (int q = 1; q < RECONF_THREADS / WAVE; ++q) m = sh[q] > m ? sh[q] : m;
-    __syncthreads();
+__device__ __forceinline__ double rc_max_of(const double *a, int B) {
+    double m = -INFINITY;
+    for (int b = 0; b < B; ++b) m = a[b] > m ? a[b] : m;
     return m;
 }
 
-__global__ __launch_bounds__(RECONF_THREADS) void k_reconfigure(
-    int64_t n, const double *log_w, const double *e_loc, uint32_t key0, uint32_t key1,
-    uint32_t step, uint32_t tag, int use_psi, double psi_in, int32_t *parents, double *out2,
-    double *cdf) {
-    __shared__ double sh_tot[RECONF_THREADS / WAVE];
-    __shared__ double sh_m[RECONF_THREADS / WAVE];
-    __shared__ long long sh_l[RECONF_THREADS / WAVE];
-    __shared__ double sh_incl[RECONF_THREADS];
-    const int tid = threadIdx.x;
-    int64_t C = (n + RECONF_THREADS - 1) / RECONF_THREADS;
-    if (C < 1) C = 1;
-    const int64_t k0 = tid * C, k1 = (k0 + C < n) ? k0 + C : n;
-    // max (exact); jax logsumexp replaces a non-finite max by 0
+__device__ __forceinline__ double rc_m0(const RcScratch &s, int B) {
+    const double m = rc_max_of(s.bm, B);
+    return isfinite(m) ? m : 0.0;   // jax logsumexp replaces a non-finite max by 0
+}
+
+// 1: block maxima of log_w
+__global__ __launch_bounds__(RC_T) void k_rc_max(int64_t n, const double *lw, double *work) {
+    __shared__ double sh4[RC_T / WAVE];
+    const int B = (int)((n + RB - 1) / RB);
+    RcScratch s = rc_scratch(work, n, B);
+    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
     double m = -INFINITY;
-    for (int64_t k = k0; k < k1; ++k) m = log_w[k] > m ? log_w[k] : m;
-    m = block_max(m, sh_m);
-    const double m0 = isfinite(m) ? m : 0.0;
+#pragma unroll
+    for (int c = 0; c < RC_PER; ++c)
+        if (k0 + c < n) m = lw[k0 + c] > m ? lw[k0 + c] : m;
+    m = rc_block_max(m, sh4);
+    if (threadIdx.x == 0) s.bm[blockIdx.x] = m;
+}
+
+// 2: block sums of exp(log_w - m0)
+__global__ __launch_bounds__(RC_T) void k_rc_lse(int64_t n, const double *lw, double *work) {
+    __shared__ double sh4[RC_T / WAVE];
+    const int B = (int)((n + RB - 1) / RB);
+    RcScratch s = rc_scratch(work, n, B);
+    const double m0 = rc_m0(s, B);
+    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
     double acc = 0.0;
-    for (int64_t k = k0; k < k1; ++k) acc += exp(log_w[k] - m0);
-    double S = block_incl(acc, sh_tot);
-    if (tid == RECONF_THREADS - 1) sh_m[0] = S;
-    __syncthreads();
-    const double logG = log(sh_m[0]) + m0;
-    __syncthreads();
-    // E_n = sum w e_loc
-    acc = 0.0;
-    for (int64_t k = k0; k < k1; ++k) acc += e_loc[k] * exp(log_w[k] - logG);
-    double Et = block_incl(acc, sh_tot);
-    if (tid == RECONF_THREADS - 1) sh_m[1] = Et;
-    __syncthreads();
-    const double En = sh_m[1];
-    __syncthreads();
-    // prefix sum of the normalised weights
-    acc = 0.0;
-    long long lastpos = 0;
-    for (int64_t k = k0; k < k1; ++k) {
-        double wk = exp(log_w[k] - logG);
-        acc += wk;
-        if (wk > 0) lastpos = k;
+#pragma unroll
+    for (int c = 0; c < RC_PER; ++c)
+        if (k0 + c < n) acc += exp(lw[k0 + c] - m0);
+    const double incl = rc_block_incl(acc, sh4);
+    if (threadIdx.x == RC_T - 1) s.bs[blockIdx.x] = incl;
+}
+
+__device__ __forceinline__ double rc_logG(const RcScratch &s, int B) {
+    double S = 0.0;
+    for (int b = 0; b < B; ++b) S = S + s.bs[b];
+    return log(S) + rc_m0(s, B);
+}
+
+// 3: block sums of w and e w, last positive walker per block
+__global__ __launch_bounds__(RC_T) void k_rc_weights(int64_t n, const double *lw,
+                                                    const double *el, double *work) {
+    __shared__ double sh4[RC_T / WAVE];
+    const int B = (int)((n + RB - 1) / RB);
+    RcScratch s = rc_scratch(work, n, B);
+    const double logG = rc_logG(s, B);
+    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
+    double aw = 0.0, ae = 0.0, lp = -1.0;
+#pragma unroll
+    for (int c = 0; c < RC_PER; ++c) {
+        if (k0 + c < n) {
+            const double w = exp(lw[k0 + c] - logG);
+            aw += w;
+            ae += el[k0 + c] * w;
+            if (w > 0) lp = (double)(k0 + c);
+        }
     }
-    double incl = block_incl(acc, sh_tot);
-    sh_incl[tid] = incl;
-    __syncthreads();
-    double c = tid == 0 ? 0.0 : sh_incl[tid - 1];
-    for (int64_t k = k0; k < k1; ++k) {
-        c = c + exp(log_w[k] - logG);
-        cdf[k] = c;
+    const double iw = rc_block_incl(aw, sh4);
+    const double ie = rc_block_incl(ae, sh4);
+    lp = rc_block_max(lp, sh4);
+    if (threadIdx.x == RC_T - 1) {
+        s.bw[blockIdx.x] = iw;
+        s.be[blockIdx.x] = ie;
     }
-    // running max over chunks (exact): chunk maxima are their last values
-    double cmax = (k1 > k0) ? c : -INFINITY;
-    double prevmax = block_excl_max(cmax, sh_m);
-    for (int64_t k = k0; k < k1; ++k)
-        if (cdf[k] < prevmax) cdf[k] = prevmax;
-    lastpos = block_max_ll(lastpos, sh_l);
+    if (threadIdx.x == 0) {
+        s.blp[blockIdx.x] = lp;
+        if (blockIdx.x == 0) s.logG[0] = logG;
+    }
+}
+
+// 4: raw prefix sums of the normalised weights, block maxima of them
+__global__ __launch_bounds__(RC_T) void k_rc_cdf(int64_t n, const double *lw, double *work) {
+    __shared__ double sh4[RC_T / WAVE];
+    __shared__ double sh_incl[RC_T];
+    const int B = (int)((n + RB - 1) / RB);
+    RcScratch s = rc_scratch(work, n, B);
+    const double logG = s.logG[0];
+    double Pb = 0.0;
+    for (int b = 0; b < (int)blockIdx.x; ++b) Pb = Pb + s.bw[b];
+    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
+    double w[RC_PER], aw = 0.0;
+#pragma unroll
+    for (int c = 0; c < RC_PER; ++c) {
+        w[c] = (k0 + c < n) ? exp(lw[k0 + c] - logG) : 0.0;
+        if (k0 + c < n) aw += w[c];
+    }
+    const double incl = rc_block_incl(aw, sh4);
+    sh_incl[threadIdx.x] = incl;
+    __syncthreads();
+    const double Et = threadIdx.x == 0 ? 0.0 : sh_incl[threadIdx.x - 1];
+    double cv = Pb + Et;
+    double last = -INFINITY;
+#pragma unroll
+    for (int c = 0; c < RC_PER; ++c) {
+        if (k0 + c < n) {
+            cv = cv + w[c];
+            s.cdf[k0 + c] = cv;
+            last = cv;
+        }
+    }
+    last = rc_block_max(last, sh4);
+    if (threadIdx.x == 0) s.bml[blockIdx.x] = last;
+}
+
+// 5: running max, so the CDF is non-decreasing despite rounding
+__global__ __launch_bounds__(RC_T) void k_rc_runmax(int64_t n, double *work) {
+    __shared__ double sh4[RC_T / WAVE];
+    const int B = (int)((n + RB - 1) / RB);
+    RcScratch s = rc_scratch(work, n, B);
+    double Mb = -INFINITY;
+    for (int b = 0; b < (int)blockIdx.x; ++b) Mb = s.bml[b] > Mb ? s.bml[b] : Mb;
+    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
+    double v[RC_PER], last = -INFINITY;
+#pragma unroll
+    for (int c = 0; c < RC_PER; ++c) {
+        v[c] = (k0 + c < n) ? s.cdf[k0 + c] : -INFINITY;
+        if (k0 + c < n) last = v[c];
+    }
+    double prev = rc_block_excl_max(last, sh4);
+    prev = prev > Mb ? prev : Mb;
+#pragma unroll
+    for (int c = 0; c < RC_PER; ++c)
+        if (k0 + c < n && v[c] < prev) s.cdf[k0 + c] = prev;
+}
+
+// 6: parents[i] = min(first k with cdf[k] > (i + psi) / n, last positive)
+__global__ __launch_bounds__(RC_T) void k_rc_parents(int64_t n, const double *work_c,
+                                                     uint32_t key0, uint32_t key1,
+                                                     uint32_t step, uint32_t tag, int use_psi,
+                                                     double psi_in, int32_t *parents,
+                                                     double *out2) {
+    const int B = (int)((n + RB - 1) / RB);
+    RcScratch s = rc_scratch(const_cast<double *>(work_c), n, B);
     double psi;
     if (use_psi) {
         psi = psi_in;
@@ -861,20 +956,23 @@ __global__ __launch_bounds__(RECONF_THREADS) void k_reconfigure(
         gfmc_u4 r = gfmc_philox(0u, 0u, step, tag, key0, key1);
         psi = gfmc_u53(r.x, r.y);
     }
-    __threadfence_block();
-    __syncthreads();
-    for (int64_t i = tid; i < n; i += RECONF_THREADS) {
-        double u = ((double)i + psi) / (double)n;
+    const double lpd = rc_max_of(s.blp, B);
+    const int64_t lastpos = lpd < 0 ? 0 : (int64_t)lpd;
+    const int64_t i = (int64_t)blockIdx.x * RC_T + threadIdx.x;
+    if (i < n) {
+        const double u = ((double)i + psi) / (double)n;
         int64_t lo = 0, hi = n;
         while (lo < hi) {
-            int64_t mid = (lo + hi) >> 1;
-            if (cdf[mid] > u) hi = mid;
+            const int64_t mid = (lo + hi) >> 1;
+            if (s.cdf[mid] > u) hi = mid;
             else lo = mid + 1;
         }
         parents[i] = (int32_t)(lo < lastpos ? lo : lastpos);
     }
-    if (tid == 0) {
-        out2[0] = logG - log((double)n);
+    if (i == 0) {
+        double En = 0.0;
+        for (int b = 0; b < B; ++b) En = En + s.be[b];
+        out2[0] = s.logG[0] - log((double)n);
         out2[1] = En;
     }
 }
@@ -1433,11 +1531,23 @@ int gfmc_reconfigure(int64_t nw, const double *log_w, const double *e_loc, uint6
     double psi = psi_opt ? *psi_opt : 0.0;
     if (psi_opt && !(psi >= 0.0 && psi < 1.0))
         return fail(GFMC_EINVAL, "gfmc_reconfigure: psi must lie in [0, 1)");
-    hipLaunchKernelGGL(k_reconfigure, dim3(1), dim3(RECONF_THREADS), 0, (hipStream_t)stream, nw,
-                       log_w, e_loc, (uint32_t)seed, (uint32_t)(seed >> 32), step, tag,
-                       psi_opt ? 1 : 0, psi, parents, out2, work);
+    const unsigned B = (unsigned)((nw + RB - 1) / RB);
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_rc_max, dim3(B), dim3(RC_T), 0, st, nw, log_w, work);
+    hipLaunchKernelGGL(k_rc_lse, dim3(B), dim3(RC_T), 0, st, nw, log_w, work);
+    hipLaunchKernelGGL(k_rc_weights, dim3(B), dim3(RC_T), 0, st, nw, log_w, e_loc, work);
+    hipLaunchKernelGGL(k_rc_cdf, dim3(B), dim3(RC_T), 0, st, nw, log_w, work);
+    hipLaunchKernelGGL(k_rc_runmax, dim3(B), dim3(RC_T), 0, st, nw, work);
+    hipLaunchKernelGGL(k_rc_parents, dim3(grid1d(nw, RC_T)), dim3(RC_T), 0, st, nw,
+                       (const double *)work, (uint32_t)seed, (uint32_t)(seed >> 32), step, tag,
+                       psi_opt ? 1 : 0, psi, parents, out2);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+int64_t gfmc_reconfigure_work_doubles(int64_t n_walkers) {
+    if (n_walkers <= 0) return 0;
+    return n_walkers + 7 * ((n_walkers + RB - 1) / RB) + 1;
 }
 
 int gfmc_gather(int32_t n_sites, int64_t nw, const uint32_t *spins_in, const double *e_loc_in,

@@ -77,7 +77,7 @@ class Chain:
         self.spins_b = torch.empty_like(self.spins)
         self.e_loc_b = torch.empty_like(self.e_loc)
         self.parents = torch.empty(self.nw, dtype=torch.int32, device=dev)
-        self.work = torch.empty(self.nw, dtype=torch.float64, device=dev)
+        self.work = torch.empty(_lib.work_doubles(self.nw), dtype=torch.float64, device=dev)
         self.evals = torch.zeros(self.nw, dtype=torch.int32, device=dev)
 
     def steps(self, n, beta, alw, seed, step0, warmup=False, series=None, evals=False):
@@ -152,7 +152,7 @@ def reconfiguration(walkers, rng_key, psi=None):
     dev = walkers.spins.device
     parents = torch.empty(nw, dtype=torch.int32, device=dev)
     out2 = torch.empty(2, dtype=torch.float64, device=dev)
-    work = torch.empty(nw, dtype=torch.float64, device=dev)
+    work = torch.empty(_lib.work_doubles(nw), dtype=torch.float64, device=dev)
     psi_c = ctypes.c_double(float(psi)) if psi is not None else None
     _lib.check(lib.gfmc_reconfigure(
         nw, _lib.ptr(walkers.log_weight_dev), _lib.ptr(walkers.e_loc_dev), key.seed,
@@ -181,7 +181,7 @@ def systematic_parents(w, psi):
     e = torch.zeros_like(lwt)
     parents = torch.empty(len(w), dtype=torch.int32, device=dev)
     out2 = torch.empty(2, dtype=torch.float64, device=dev)
-    work = torch.empty(len(w), dtype=torch.float64, device=dev)
+    work = torch.empty(_lib.work_doubles(len(w)), dtype=torch.float64, device=dev)
     psi_c = ctypes.c_double(float(psi))
     _lib.check(_lib.load().gfmc_reconfigure(len(w), _lib.ptr(lwt), _lib.ptr(e), 0, 0,
                                             _lib.TAG_RECONFIG, ctypes.byref(psi_c),
