@@ -44,8 +44,9 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part "
-                                   "of this build (walker sharding is described in DESIGN.md)",
+        print(json.dumps({"error": "this engine runs on one GPU; runs on more than one GPU "
+                                   "are not implemented (walkers would shard by contiguous "
+                                   "ranges with a scalar scan of the weight sums)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch

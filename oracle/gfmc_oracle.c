@@ -15,7 +15,10 @@
  * same sum to within a few ulps.  Reconfiguration uses the same idea with
  * 1024-walker blocks (rc_block_incl below).
  *
- * Parity status of each function (how it is pinned) is listed in oracle/README.md.
+ * Pinning: the rates against the reference's golden local energies and a
+ * Kronecker-product matrix, the sampler and resampling against the
+ * reference's known-answer tests, Philox against the Random123 vectors
+ * (tests/test_oracle.py).
  */
 #include <math.h>
 #include <stdint.h>
