@@ -99,7 +99,8 @@ class Problem:
         for k, v in self.arrays.items():
             if v is None:
                 continue
-            ct = ctypes.c_int32 if v.dtype == np.int32 else ctypes.c_double
+            ct = {np.dtype(np.int32): ctypes.c_int32, np.dtype(np.float32): ctypes.c_float,
+                  np.dtype(np.float64): ctypes.c_double}[v.dtype]
             setattr(d, k, _p(v, ct))
         self.desc = d
         self.N = int(lowered.scalars["n_sites"])

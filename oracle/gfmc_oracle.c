@@ -174,9 +174,52 @@ static double edge_mf(const gfmc_problem_desc *d, const int8_t *x, int e, int si
     return acc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* RBM guide (no reference counterpart; the reference's network guide is     */
+/* ClusterMLP, wavefunction/cluster_mlp.py:133-439): log psi = sum_k a_k x_k  */
+/* + sum_h lncosh(theta_h), theta_h = b_h + sum_k W_hk x_k, evaluated in FP64 */
+/* from the (BF16-valued) float parameters.                                   */
+/* ------------------------------------------------------------------------ */
+static double lncosh(double y) {
+    double a = fabs(y);
+    return a + log1p(exp(-2.0 * a)) - log(2.0);
+}
+
+/* theta[h] = b_h + sum_k W_hk x_k, k ascending */
+void orc_rbm_theta(const gfmc_problem_desc *d, const int8_t *x, double *theta) {
+    int H = d->rbm_hidden, N = d->n_sites;
+    for (int h = 0; h < H; ++h) {
+        double acc = (double)d->rbm_b[h];
+        for (int k = 0; k < N; ++k) acc += (double)d->rbm_W[(size_t)h * N + k] * (double)x[k];
+        theta[h] = acc;
+    }
+}
+
+/* log psi(x') - log psi(x) for flipping site i (and j >= 0) */
+static double rbm_log_ratio(const gfmc_problem_desc *d, const int8_t *x, const double *theta,
+                            int i, int j) {
+    int H = d->rbm_hidden, N = d->n_sites;
+    double xi = (double)x[i], xj = j >= 0 ? (double)x[j] : 0.0;
+    double lr = -2.0 * (double)d->rbm_a[i] * xi;
+    if (j >= 0) lr += -2.0 * (double)d->rbm_a[j] * xj;
+    double acc = 0.0;
+    for (int h = 0; h < H; ++h) {
+        double delta = -2.0 * (double)d->rbm_W[(size_t)h * N + i] * xi;
+        if (j >= 0) delta += -2.0 * (double)d->rbm_W[(size_t)h * N + j] * xj;
+        acc += lncosh(theta[h] + delta) - lncosh(theta[h]);
+    }
+    return lr + acc;
+}
+
 /* Guide log-ratio of off-diagonal slot s (log_diff_from_operator,
- * jastrow_spin_half.py:369-398). */
-static double slot_log_diff(const gfmc_problem_desc *d, const int8_t *x, int s) {
+ * jastrow_spin_half.py:369-398); theta: the RBM's theta of x (RBM only). */
+static double slot_log_diff(const gfmc_problem_desc *d, const int8_t *x, int s,
+                            const double *theta) {
+    if (d->guide == GFMC_GUIDE_RBM) {
+        if (s < d->n_hx) return rbm_log_ratio(d, x, theta, s, -1);
+        int e = s - d->n_hx;
+        return rbm_log_ratio(d, x, theta, d->edge_i[e], d->edge_j[e]);
+    }
     if (d->guide != GFMC_GUIDE_JASTROW) return 0.0;
     if (s < d->n_hx) {
         double xk = (double)x[s];
@@ -201,12 +244,21 @@ static double slot_bare(const gfmc_problem_desc *d, const int8_t *x, int s) {
 
 /* Guide-transformed rate of slot s: bare * exp(where(bare == 0, 0, ld))
  * (transition_rates, evolution.py:253-261). */
-static double slot_rate(const gfmc_problem_desc *d, const int8_t *x, int s, double *ld_out) {
+static double slot_rate(const gfmc_problem_desc *d, const int8_t *x, int s, double *ld_out,
+                        const double *theta) {
     double bare = slot_bare(d, x, s);
-    double ld = slot_log_diff(d, x, s);
+    if (bare == 0.0 && !ld_out) return bare;
+    double ld = slot_log_diff(d, x, s, theta);
     if (ld_out) *ld_out = ld;
     if (bare == 0.0) return bare;
     return bare * exp(ld);
+}
+
+static double *theta_of(const gfmc_problem_desc *d, const int8_t *x) {
+    if (d->guide != GFMC_GUIDE_RBM) return NULL;
+    double *t = (double *)malloc(sizeof(double) * (size_t)d->rbm_hidden);
+    orc_rbm_theta(d, x, t);
+    return t;
 }
 
 /* Diagonal element H_xx (SpinHalf.diagonal_weight:291-314).  The non-uniform
@@ -244,24 +296,34 @@ double orc_diag(const gfmc_problem_desc *d, const int8_t *x) {
 double orc_rates(const gfmc_problem_desc *d, const int8_t *x, double *rates, double *log_diff) {
     int n_off = n_off_of(d);
     double *w = (double *)malloc(sizeof(double) * (size_t)(n_off > 0 ? n_off : 1));
+    double *theta = theta_of(d, x);
     double diag = orc_diag(d, x);
     if (rates) rates[0] = diag;
     if (log_diff) log_diff[0] = 0.0;
     for (int s = 0; s < n_off; ++s) {
         double ld;
-        double r = slot_rate(d, x, s, &ld);
+        double r = slot_rate(d, x, s, &ld, theta);
         w[s] = -r;
         if (rates) rates[1 + s] = r;
         if (log_diff) log_diff[1 + s] = ld;
     }
     double R = orc_blocked_scan(w, n_off, NULL);
     free(w);
+    free(theta);
     return diag - R;
 }
 
 /* log psi_T (JastrowSpinHalf.log_psi, jastrow_spin_half.py:299-324):
  * 0.5 * sum_k sum_q J_kq x_k x_other + sum_k h_k x_k, sequential order. */
 double orc_log_psi(const gfmc_problem_desc *d, const int8_t *x) {
+    if (d->guide == GFMC_GUIDE_RBM) {
+        double *t = theta_of(d, x);
+        double field = 0.0, hid = 0.0;
+        for (int k = 0; k < d->n_sites; ++k) field += (double)d->rbm_a[k] * (double)x[k];
+        for (int h = 0; h < d->rbm_hidden; ++h) hid += lncosh(t[h]);
+        free(t);
+        return field + hid;
+    }
     if (d->guide != GFMC_GUIDE_JASTROW) return 0.0;
     int D = d->jas_max_deg;
     double pair = 0.0, field = 0.0;
@@ -343,7 +405,9 @@ static void propagate_one(const gfmc_problem_desc *d, int64_t widx, uint32_t *sp
     uint32_t it = 0;
     double incl[WAVE];
     while (t < beta) {
-        for (int s = 0; s < n_off; ++s) w[s] = -slot_rate(d, x, s, NULL);
+        double *theta = theta_of(d, x);
+        for (int s = 0; s < n_off; ++s) w[s] = -slot_rate(d, x, s, NULL, theta);
+        free(theta);
         double diag = orc_diag(d, x);
         double R = orc_blocked_scan(w, n_off, incl);
         eL = diag - R;

@@ -83,12 +83,16 @@ struct DevProblem {
     const int *dep_off, *dep_slots, *edge_off, *dep_edges;
     // record path
     int fast, R, RN, RE, flags;
-    const uint4 *slot_rec;   // [n_off * 2]
+    const uint4 *slot_rec;   // [K * 64 * 2], in LDS order (record p -> LDS position p)
     const uint4 *mv_rec;     // [n_off * R * 2]
     const int *mv_nbr;       // [n_off * RN]
     const int *mv_sites;     // [n_off]  i | (j << 16), j = 0xffff for a site flip
     const int *mv_edges;     // [n_off * RE] edges whose state a move may change (-1 pad)
     const int *edge_pack;    // [n_edges] i | (j << 16)
+    // RBM guide (FP32 arithmetic on BF16-valued parameters)
+    int H, w_words, rbm_wave_bytes;
+    const uint32_t *rbm_W;   // [H][N] BF16, packed two per word (N even-padded)
+    const float *rbm_a, *rbm_b, *rbm_cs;   // [N], [H], column sums sum_h W_hk [N]
 };
 
 struct gfmc_problem {
@@ -98,6 +102,7 @@ struct gfmc_problem {
     int precision;
     std::vector<void *> allocs;
     int64_t lds_bytes;
+    int n_cu;
 };
 
 static int block_k_of(int n) { return n > 0 ? (n + WAVE - 1) / WAVE : 1; }
@@ -353,21 +358,23 @@ __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
 template <typename T, bool FAST>
 __device__ int full_eval(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L, int lane) {
     if constexpr (FAST) {
-        // coalesced: slot s0 + q*64 + lane, four records in flight per lane
-        for (int s0 = 0; s0 < P.n_off; s0 += 4 * WAVE) {
+        // slot_rec is stored in LDS order (record p describes the slot kept at
+        // LDS position p, slot -1 = padding): coalesced reads, conflict-free
+        // writes, four records in flight per lane
+        for (int p0 = 0; p0 < P.K * WAVE; p0 += 4 * WAVE) {
             Rec r[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int s = s0 + q * WAVE + lane;
-                r[q] = s < P.n_off ? load_rec(P.slot_rec, s) : empty_rec();
+                const int p = p0 + q * WAVE + lane;
+                r[q] = p < P.K * WAVE ? load_rec(P.slot_rec, p) : empty_rec();
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int s = rec_f(r[q], 0);
-                if (s >= 0) L.w[wpos(s, P.K)] = eval_rec(P, tb, r[q], L.x);
+                const int p = p0 + q * WAVE + lane;
+                if (p < P.K * WAVE)
+                    L.w[p] = rec_f(r[q], 0) >= 0 ? eval_rec(P, tb, r[q], L.x) : (T)0;
             }
         }
-        for (int s = P.n_off + lane; s < P.K * WAVE; s += WAVE) L.w[wpos(s, P.K)] = (T)0;
     } else {
         for (int k = 0; k < P.K; ++k) {
             int s = lane * P.K + k;
@@ -665,6 +672,313 @@ __global__ void k_log_psi(DevProblem P, Tables<double> tb, int64_t nw, const uin
         field += tb.jas_h[k] * xk;
     }
     out[wk] = 0.5 * pair + field;
+}
+
+// ---------------------------------------------------------------------------
+// RBM guide: log psi = sum_k a_k x_k + sum_h lncosh(theta_h),
+// theta_h = b_h + sum_k W_hk x_k, W in BF16 staged in LDS once per workgroup
+// of RBM_WAVES walker-wavefronts.  A slot's log-ratio is
+//   -2 sum_{k in S} a_k x_k - sum_h Delta_h + sum_h log(p_h e^{2 Delta_h} + q_h)
+// with Delta_h = -2 sum_{k in S} W_hk x_k, p = (1 + tanh theta)/2,
+// q = (1 - tanh theta)/2 (cosh(theta + Delta)/cosh theta = e^{-Delta}
+// (p e^{2 Delta} + q)); sum_h Delta_h comes from the column sums of W, and the
+// logarithm is taken once per four hidden units of a running product.
+// Every slot is re-evaluated after every move (theta changes everywhere).
+// ---------------------------------------------------------------------------
+#define RBM_WAVES 8
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float bf16_at(const uint32_t *W, int idx) {
+    const uint32_t w = W[idx >> 1];
+    return __uint_as_float((idx & 1) ? (w & 0xffff0000u) : (w << 16));
+}
+
+struct RbmLds {
+    signed char *x;
+    float *theta, *p, *q, *w;
+    int *list;
+    unsigned char *anti;
+};
+
+__device__ __forceinline__ RbmLds rbm_carve(const DevProblem &P, unsigned char *lds, int wave) {
+    unsigned char *base = lds + (size_t)P.w_words * 4 + (size_t)wave * P.rbm_wave_bytes;
+    RbmLds L;
+    L.w = (float *)base;
+    L.theta = L.w + P.K * WAVE;
+    L.p = L.theta + P.H;
+    L.q = L.p + P.H;
+    L.list = (int *)(L.q + P.H);
+    L.x = (signed char *)(L.list + P.n_off);
+    L.anti = (unsigned char *)(L.x + P.n_pad);
+    return L;
+}
+
+__device__ __forceinline__ void rbm_pq(float th, float *p, float *q) {
+    const float e = __expf(-2.0f * fabsf(th));
+    const float inv = 1.0f / (1.0f + e);
+    const float big = inv, small = e * inv;
+    *p = th >= 0.0f ? big : small;
+    *q = th >= 0.0f ? small : big;
+}
+
+// theta from scratch (k ascending), lanes over hidden units
+__device__ void rbm_theta_init(const DevProblem &P, const uint32_t *Wl, RbmLds &L, int lane) {
+    for (int h = lane; h < P.H; h += WAVE) {
+        float acc = P.rbm_b[h];
+        const int row = h * (P.w_words * 2 / P.H);
+        for (int k = 0; k < P.N; ++k) acc += bf16_at(Wl, row + k) * (float)L.x[k];
+        L.theta[h] = acc;
+        rbm_pq(acc, &L.p[h], &L.q[h]);
+    }
+}
+
+// theta += Delta of flipping site i (and j >= 0) at the current (pre-flip) x
+__device__ void rbm_theta_move(const DevProblem &P, const uint32_t *Wl, RbmLds &L, int i, int j,
+                               int lane) {
+    const int Np = P.w_words * 2 / P.H;
+    const float xi = (float)L.x[i], xj = j >= 0 ? (float)L.x[j] : 0.0f;
+    for (int h = lane; h < P.H; h += WAVE) {
+        float d = -2.0f * xi * bf16_at(Wl, h * Np + i);
+        if (j >= 0) d += -2.0f * xj * bf16_at(Wl, h * Np + j);
+        const float th = L.theta[h] + d;
+        L.theta[h] = th;
+        rbm_pq(th, &L.p[h], &L.q[h]);
+    }
+}
+
+// log psi(x') - log psi(x) of flipping i (and j >= 0)
+__device__ float rbm_log_ratio(const DevProblem &P, const uint32_t *Wl, const RbmLds &L, int i,
+                               int j) {
+    const int Np = P.w_words * 2 / P.H;
+    const float L2E = 1.4426950408889634f;
+    const float xi = (float)L.x[i], xj = j >= 0 ? (float)L.x[j] : 0.0f;
+    const float ci = -4.0f * L2E * xi, cj = -4.0f * L2E * xj;
+    float acc = 0.0f;
+    for (int h0 = 0; h0 < P.H; h0 += 4) {
+        float prod = 1.0f;
+        const int hn = h0 + 4 < P.H ? 4 : P.H - h0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u < hn) {
+                const int h = h0 + u;
+                float a = ci * bf16_at(Wl, h * Np + i);
+                if (j >= 0) a += cj * bf16_at(Wl, h * Np + j);
+                prod *= L.p[h] * __builtin_amdgcn_exp2f(a) + L.q[h];
+            }
+        }
+        if (prod > 1e-30f && prod < 1e30f) {
+            acc += __builtin_amdgcn_logf(prod);
+        } else {
+            for (int u = 0; u < hn; ++u) {
+                const int h = h0 + u;
+                float a = ci * bf16_at(Wl, h * Np + i);
+                if (j >= 0) a += cj * bf16_at(Wl, h * Np + j);
+                acc += __builtin_amdgcn_logf(L.p[h] * __builtin_amdgcn_exp2f(a) + L.q[h]);
+            }
+        }
+    }
+    float sum_delta = -2.0f * xi * P.rbm_cs[i];
+    float afield = -2.0f * P.rbm_a[i] * xi;
+    if (j >= 0) {
+        sum_delta += -2.0f * xj * P.rbm_cs[j];
+        afield += -2.0f * P.rbm_a[j] * xj;
+    }
+    return afield - sum_delta + 0.69314718055994531f * acc;
+}
+
+__device__ __forceinline__ void slot_sites(const DevProblem &P, int s, int *i, int *j) {
+    if (s < P.n_hx) {
+        *i = s;
+        *j = -1;
+    } else {
+        const int pk = P.edge_pack[s - P.n_hx];
+        *i = pk & 0xffff;
+        *j = (pk >> 16) & 0xffff;
+    }
+}
+
+// every slot weight (active slots compacted first), returns the anti count
+__device__ int rbm_eval_all(const DevProblem &P, const Tables<float> &tb, const uint32_t *Wl,
+                            RbmLds &L, int lane) {
+    int base = 0;
+    for (int c = 0; c < P.n_off; c += WAVE) {
+        const int s = c + lane;
+        float bare = 0.0f;
+        if (s < P.n_off) {
+            if (s < P.n_hx) {
+                bare = tb.hx_half[s];
+            } else {
+                const int pk = P.edge_pack[s - P.n_hx];
+                bare = (L.x[pk & 0xffff] != L.x[(pk >> 16) & 0xffff]) ? tb.jpm[s - P.n_hx]
+                                                                      : tb.jpmpm[s - P.n_hx];
+            }
+        }
+        const bool act = s < P.n_off && bare != 0.0f;
+        if (s < P.n_off && !act) L.w[wpos(s, P.K)] = -bare;
+        const unsigned long long m = __ballot(act);
+        const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (act) L.list[pos] = s;
+        base += __popcll(m);
+    }
+    wave_sync();
+    for (int t = lane; t < base; t += WAVE) {
+        const int s = L.list[t];
+        int i, j;
+        slot_sites(P, s, &i, &j);
+        const float bare = s < P.n_hx ? tb.hx_half[s]
+                                      : ((L.x[i] != L.x[j]) ? tb.jpm[s - P.n_hx]
+                                                            : tb.jpmpm[s - P.n_hx]);
+        L.w[wpos(s, P.K)] = -(bare * __expf(rbm_log_ratio(P, Wl, L, i, j)));
+    }
+    int cnt = 0;
+    if (P.has_jz) {
+        for (int e = lane; e < P.n_edges; e += WAVE) {
+            const int pk = P.edge_pack[e];
+            const unsigned char a = L.x[pk & 0xffff] != L.x[(pk >> 16) & 0xffff];
+            if (P.flags) L.anti[e] = a;
+            cnt += a;
+        }
+    }
+    wave_sync();
+    return wave_sum_int(cnt);
+}
+
+__device__ void rbm_stage_W(const DevProblem &P, uint32_t *Wl) {
+    for (int t = threadIdx.x; t < P.w_words; t += blockDim.x) Wl[t] = P.rbm_W[t];
+    __syncthreads();
+}
+
+__device__ void rbm_walker_start(const DevProblem &P, const Tables<float> &tb,
+                                 const uint32_t *Wl, RbmLds &L, const uint32_t *sp, int lane) {
+    for (int k = lane; k < P.N; k += WAVE) L.x[k] = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1 : -1;
+    for (int s = P.n_off + lane; s < P.K * WAVE; s += WAVE) L.w[wpos(s, P.K)] = 0.0f;
+    wave_sync();
+    rbm_theta_init(P, Wl, L, lane);
+    wave_sync();
+}
+
+__global__ __launch_bounds__(RBM_WAVES * WAVE) void k_propagate_rbm(
+    DevProblem P, Tables<float> tb, int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
+    double beta, double alw, uint32_t key0, uint32_t key1, uint32_t step, uint32_t tag,
+    int32_t *evals) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    uint32_t *Wl = (uint32_t *)lds;
+    rbm_stage_W(P, Wl);
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    RbmLds L = rbm_carve(P, lds, wave);
+    WalkerLds<float> WL;
+    WL.w = L.w;
+    WL.x = L.x;
+    WL.anti = L.anti;
+    for (int64_t wk = (int64_t)blockIdx.x * RBM_WAVES + wave; wk < nw;
+         wk += (int64_t)gridDim.x * RBM_WAVES) {
+        uint32_t *sp = spins + wk * P.words;
+        rbm_walker_start(P, tb, Wl, L, sp, lane);
+        int n_anti = rbm_eval_all(P, tb, Wl, L, lane);
+        double diag = diag_value(P, L.x, L.anti, n_anti, lane);
+        double lw = log_w[wk];
+        double t = 0.0, eL = 0.0;
+        uint32_t it = 0;
+        while (t < beta) {
+            float lane_sum;
+            const float incl = lane_scan(P, WL, lane, &lane_sum);
+            const float R = lane_bcast(incl, WAVE - 1);
+            eL = diag - (double)R;
+            const gfmc_u4 r = gfmc_philox(it, (uint32_t)wk, step, tag, key0, key1);
+            const double psi = gfmc_u53(r.z, r.w);
+            int idx = -1;
+            double tau_raw = INFINITY;
+            if (R > 0.0f) {
+                float u = gfmc_u24(r.x) * R;
+                const float cap = below(R);
+                if (u > cap) u = cap;
+                idx = sample_slot(P, WL, incl, lane_sum, u, lane);
+                tau_raw = -log1p(-psi) / (double)R;
+            }
+            const double rem = beta - t;
+            const double tau = tau_raw < rem ? tau_raw : rem;
+            lw = lw - eL * tau;
+            t = t + tau;
+            ++it;
+            if (tau_raw < rem && idx >= 0) {
+                int i, j;
+                slot_sites(P, idx, &i, &j);
+                rbm_theta_move(P, Wl, L, i, j, lane);
+                wave_sync();
+                if (lane == 0) {
+                    L.x[i] = (signed char)-L.x[i];
+                    if (j >= 0) L.x[j] = (signed char)-L.x[j];
+                }
+                wave_sync();
+                n_anti = rbm_eval_all(P, tb, Wl, L, lane);
+                diag = diag_value(P, L.x, L.anti, n_anti, lane);
+            }
+        }
+        lw = lw - alw * beta;
+        store_spins(P, sp, L.x, lane);
+        if (lane == 0) {
+            log_w[wk] = lw;
+            e_loc[wk] = eL;
+            if (evals) evals[wk] += (int32_t)it;
+        }
+        wave_sync();
+    }
+}
+
+__global__ __launch_bounds__(RBM_WAVES * WAVE) void k_rates_rbm(
+    DevProblem P, Tables<float> tb, int64_t nw, const uint32_t *spins, double *rates,
+    double *e_loc, double *log_diff, double *log_psi) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    uint32_t *Wl = (uint32_t *)lds;
+    rbm_stage_W(P, Wl);
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    RbmLds L = rbm_carve(P, lds, wave);
+    WalkerLds<float> WL;
+    WL.w = L.w;
+    WL.x = L.x;
+    WL.anti = L.anti;
+    const int64_t M = P.n_off + 1;
+    for (int64_t wk = (int64_t)blockIdx.x * RBM_WAVES + wave; wk < nw;
+         wk += (int64_t)gridDim.x * RBM_WAVES) {
+        rbm_walker_start(P, tb, Wl, L, spins + wk * P.words, lane);
+        if (log_psi) {
+            float acc = 0.0f;
+            for (int h = lane; h < P.H; h += WAVE) {
+                const float a = fabsf(L.theta[h]);
+                acc += a + log1pf(__expf(-2.0f * a)) - 0.69314718055994531f;
+            }
+            for (int k = lane; k < P.N; k += WAVE) acc += P.rbm_a[k] * (float)L.x[k];
+            const float tot = lane_bcast(wave_incl_scan(acc, lane), WAVE - 1);
+            if (lane == 0) log_psi[wk] = (double)tot;
+        }
+        if (rates || e_loc || log_diff) {
+            const int n_anti = rbm_eval_all(P, tb, Wl, L, lane);
+            const double diag = diag_value(P, L.x, L.anti, n_anti, lane);
+            float lane_sum;
+            const float incl = lane_scan(P, WL, lane, &lane_sum);
+            const float R = lane_bcast(incl, WAVE - 1);
+            if (rates) {
+                if (lane == 0) rates[wk * M] = diag;
+                for (int s = lane; s < P.n_off; s += WAVE)
+                    rates[wk * M + 1 + s] = -(double)L.w[wpos(s, P.K)];
+            }
+            if (log_diff) {
+                if (lane == 0) log_diff[wk * M] = 0.0;
+                for (int s = lane; s < P.n_off; s += WAVE) {
+                    int i, j;
+                    slot_sites(P, s, &i, &j);
+                    log_diff[wk * M + 1 + s] = (double)rbm_log_ratio(P, Wl, L, i, j);
+                }
+            }
+            if (e_loc && lane == 0) e_loc[wk] = diag - (double)R;
+        }
+        wave_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,7 +1372,7 @@ static int upload_f32(gfmc_problem *p, const double *src, size_t n, const float 
 
 // Host-built tables of the record path (see DevProblem).
 struct HostRecords {
-    std::vector<uint32_t> slot_rec, mv_rec;   // 8 words = 16 int16 per record
+    std::vector<uint32_t> slot_rec, slot_rec_lds, mv_rec;   // 8 words = 16 int16 per record
     std::vector<int> mv_nbr, mv_sites, mv_edges, edge_pack;
     int R = 0, RN = 0, RE = 0;
     double J0 = 0.0, h0 = 0.0;
@@ -1136,6 +1450,14 @@ static bool build_records(const gfmc_problem_desc *d, HostRecords &hr) {
             }
         }
         put16(r, 3, mask);
+    }
+    // the same records in LDS order: position (s % K) * 64 + s / K
+    {
+        const int K = block_k_of(n_off);
+        hr.slot_rec_lds.assign((size_t)K * WAVE * 8, 0xffffffffu);
+        for (int s = 0; s < n_off; ++s)
+            memcpy(&hr.slot_rec_lds[(size_t)((s % K) * WAVE + s / K) * 8],
+                   &hr.slot_rec[(size_t)s * 8], 8 * sizeof(uint32_t));
     }
     // per-move record blocks
     for (int s = 0; s < n_off; ++s) hr.R = std::max(hr.R, d->dep_off[s + 1] - d->dep_off[s]);
@@ -1217,6 +1539,14 @@ static void launch_propagate(const gfmc_problem *p, const Tables<T> &tb, int64_t
                            alw, k0, k1, step, tag, evals);
 }
 
+// persistent RBM grid: as many workgroups as fit (one per CU at full LDS)
+static unsigned rbm_grid(const gfmc_problem *p, int64_t nw) {
+    const int64_t need = (nw + RBM_WAVES - 1) / RBM_WAVES;
+    const int64_t per_cu = (160 * 1024) / (p->lds_bytes > 0 ? p->lds_bytes : 1);
+    const int64_t cap = (int64_t)p->n_cu * (per_cu > 0 ? per_cu : 1);
+    return (unsigned)(need < cap ? need : cap);
+}
+
 static void free_problem(gfmc_problem *p) {
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;
@@ -1240,9 +1570,15 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         return fail(GFMC_EINVAL, "gfmc_problem_create: n_hx must be 0 or n_sites");
     if (!(d->n_jpm == 0 || d->n_jpm == d->n_edges))
         return fail(GFMC_EINVAL, "gfmc_problem_create: n_jpm must be 0 or n_edges");
-    if (d->guide == GFMC_GUIDE_RBM)
-        return fail(GFMC_EUNSUP, "gfmc_problem_create: the RBM guide uses gfmc_rbm_* entry points");
-    if (d->guide != GFMC_GUIDE_NONE && d->guide != GFMC_GUIDE_JASTROW)
+    if (d->guide == GFMC_GUIDE_RBM) {
+        if (precision != GFMC_FP32)
+            return fail(GFMC_EINVAL, "gfmc_problem_create: the RBM guide runs in FP32 "
+                                     "(BF16 weights); pass GFMC_FP32");
+        if (d->rbm_hidden <= 0 || !d->rbm_W || !d->rbm_a || !d->rbm_b)
+            return fail(GFMC_EINVAL, "gfmc_problem_create: incomplete RBM parameters");
+    }
+    if (d->guide != GFMC_GUIDE_NONE && d->guide != GFMC_GUIDE_JASTROW &&
+        d->guide != GFMC_GUIDE_RBM)
         return fail(GFMC_EINVAL, "gfmc_problem_create: unknown guide kind");
     const int N = d->n_sites, E = d->n_edges, n_off = d->n_hx + d->n_jpm;
     if (d->guide == GFMC_GUIDE_JASTROW &&
@@ -1283,6 +1619,14 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
 
     gfmc_problem *p = new gfmc_problem();
     p->precision = precision;
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            ncu <= 0)
+            ncu = 256;
+        p->n_cu = ncu;
+    }
     DevProblem &P = p->dp;
     memset(&P, 0, sizeof P);
     P.N = N;
@@ -1344,7 +1688,7 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     UP(hr.edge_pack.data(), hr.edge_pack.size(), &P.edge_pack);
     if (P.fast) {
         const uint32_t *sr = nullptr, *mr = nullptr;
-        UP(hr.slot_rec.data(), hr.slot_rec.size(), &sr);
+        UP(hr.slot_rec_lds.data(), hr.slot_rec_lds.size(), &sr);
         UP(hr.mv_rec.data(), hr.mv_rec.size(), &mr);
         P.slot_rec = (const uint4 *)sr;
         P.mv_rec = (const uint4 *)mr;
@@ -1383,11 +1727,53 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         UP(d->jas_edge_J, (size_t)E * 2 * D, &p->t64.jas_edge_J);
         UP32(d->jas_edge_J, (size_t)E * 2 * D, &p->t32.jas_edge_J);
     }
+    if (d->guide == GFMC_GUIDE_RBM) {
+        const int H = d->rbm_hidden, Np = N + (N & 1);
+        P.H = H;
+        P.w_words = H * Np / 2;
+        // W rounded to BF16 (round to nearest even), two per word, rows of Np
+        std::vector<uint32_t> wp((size_t)P.w_words, 0u);
+        std::vector<float> cs((size_t)N, 0.0f);
+        std::vector<double> csd((size_t)N, 0.0);
+        for (int h = 0; h < H; ++h)
+            for (int k = 0; k < N; ++k) {
+                uint32_t bits;
+                memcpy(&bits, &d->rbm_W[(size_t)h * N + k], 4);
+                const uint32_t b16 = (bits + 0x7fffu + ((bits >> 16) & 1u)) >> 16;
+                const size_t idx = (size_t)h * Np + k;
+                wp[idx >> 1] |= (idx & 1) ? (b16 << 16) : b16;
+                const uint32_t back = b16 << 16;
+                float wv;
+                memcpy(&wv, &back, 4);
+                csd[k] += (double)wv;
+            }
+        for (int k = 0; k < N; ++k) cs[k] = (float)csd[k];
+        const uint32_t *wdev = nullptr;
+        UP(wp.data(), wp.size(), &wdev);
+        P.rbm_W = wdev;
+        UP(d->rbm_a, (size_t)N, &P.rbm_a);
+        UP(d->rbm_b, (size_t)H, &P.rbm_b);
+        UP(cs.data(), cs.size(), &P.rbm_cs);
+        P.rbm_wave_bytes = round_up(P.K * WAVE * 4 + 3 * H * 4 + n_off * 4 + P.n_pad + P.e_pad, 16);
+    }
 #undef UP
 #undef UP32
     const size_t wbytes = (size_t)P.K * WAVE * (precision == GFMC_FP64 ? 8 : 4);
     p->lds_bytes = (int64_t)(wbytes + P.n_pad + P.e_pad);
-    if (p->lds_bytes > 64 * 1024) {
+    if (d->guide == GFMC_GUIDE_RBM) {
+        p->lds_bytes = (int64_t)P.w_words * 4 + (int64_t)RBM_WAVES * P.rbm_wave_bytes;
+        if (p->lds_bytes > 160 * 1024) {
+            free_problem(p);
+            return fail(GFMC_EUNSUP, "gfmc_problem_create: RBM weights plus walker state "
+                                     "exceed the 160 KiB of LDS per workgroup");
+        }
+        HIP_TRY(hipFuncSetAttribute((const void *)k_propagate_rbm,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)p->lds_bytes));
+        HIP_TRY(hipFuncSetAttribute((const void *)k_rates_rbm,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)p->lds_bytes));
+    } else if (p->lds_bytes > 64 * 1024) {
         free_problem(p);
         return fail(GFMC_EUNSUP,
                     "gfmc_problem_create: walker state exceeds 64 KiB of LDS per wavefront");
@@ -1481,7 +1867,11 @@ int gfmc_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double 
     if (rc) return rc;
     if (nw == 0) return 0;
     if (!spins) return fail(GFMC_EINVAL, "gfmc_rates: NULL spins");
-    if (p->precision == GFMC_FP64)
+    if (p->dp.guide == GFMC_GUIDE_RBM)
+        hipLaunchKernelGGL(k_rates_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
+                           (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
+                           rates, e_loc, log_diff, (double *)nullptr);
+    else if (p->precision == GFMC_FP64)
         launch_rates(p, p->t64, nw, spins, rates, e_loc, log_diff, (hipStream_t)stream);
     else
         launch_rates(p, p->t32, nw, spins, rates, e_loc, log_diff, (hipStream_t)stream);
@@ -1495,6 +1885,11 @@ int gfmc_log_psi(const gfmc_problem *p, int64_t nw, const uint32_t *spins, doubl
     if (rc) return rc;
     if (nw == 0) return 0;
     if (!spins || !out) return fail(GFMC_EINVAL, "gfmc_log_psi: NULL buffer");
+    if (p->dp.guide == GFMC_GUIDE_RBM)
+        hipLaunchKernelGGL(k_rates_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
+                           (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
+                           (double *)nullptr, (double *)nullptr, (double *)nullptr, out);
+    else
     hipLaunchKernelGGL(k_log_psi, dim3(grid1d(nw, 64)), dim3(64), 0, (hipStream_t)stream, p->dp,
                        p->t64, nw, spins, out);
     HIP_TRY(hipGetLastError());
@@ -1511,7 +1906,11 @@ int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
     if (!(beta > 0) || !std::isfinite(beta))
         return fail(GFMC_EINVAL, "gfmc_propagate: beta must be finite and > 0");
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-    if (p->precision == GFMC_FP64)
+    if (p->dp.guide == GFMC_GUIDE_RBM)
+        hipLaunchKernelGGL(k_propagate_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
+                           (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
+                           log_w, e_loc, beta, alw, k0, k1, step, tag, evals);
+    else if (p->precision == GFMC_FP64)
         launch_propagate(p, p->t64, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
                          evals, (hipStream_t)stream);
     else

@@ -140,3 +140,64 @@ class JastrowSpinHalf:
     def log_diff_from_configs(self, config, new_configs, cache=None):
         new = np.asarray(new_configs, dtype=float)
         return self.log_psi(new) - self.log_psi(np.asarray(config, dtype=float)[None])[0]
+
+
+def bf16_round(a):
+    """float32 values rounded to the nearest BF16 value (ties to even),
+    returned as float32."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    bits = f.view(np.uint32).astype(np.uint64)
+    r = ((bits + 0x7FFF + ((bits >> 16) & 1)) >> 16) << 16
+    return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32).reshape(f.shape)
+
+
+class RBMSpinHalf:
+    """Restricted-Boltzmann-machine guide (no counterpart in the reference,
+    whose network guide is ClusterMLP, wavefunction/cluster_mlp.py:133-439):
+
+        log psi(x) = sum_k a_k x_k + sum_h lncosh(b_h + sum_k W_hk x_k)
+
+    with ``hidden = alpha * N`` units.  The weights W are held on the BF16
+    grid (the kernels read them as BF16 and compute in FP32); a and b are
+    FP32.  Trained by :func:`pygfmc_amd.training.train_variance`.
+    """
+
+    def __init__(self, N, alpha=4, hidden=None, W=None, a=None, b=None, seed=0, scale=0.01,
+                 dtype=np.float32):
+        self.N = int(N)
+        H = int(hidden) if hidden is not None else int(alpha) * self.N
+        rng = np.random.default_rng(seed)
+        if W is None:
+            W = rng.normal(scale=scale, size=(H, self.N))
+        W = np.asarray(W, dtype=np.float32)
+        if W.shape != (H, self.N):
+            raise ValueError(f"W must have shape ({H}, {self.N}), got {W.shape}")
+        self.hidden = H
+        self.dtype = np.dtype(dtype)
+        self.W = bf16_round(W)
+        self.a = np.zeros(self.N, np.float32) if a is None else np.asarray(a, np.float32).copy()
+        self.b = np.zeros(H, np.float32) if b is None else np.asarray(b, np.float32).copy()
+        if self.a.shape != (self.N,) or self.b.shape != (H,):
+            raise ValueError("a must have shape (N,) and b shape (hidden,)")
+
+    def parameters(self):
+        return dict(W=self.W.copy(), a=self.a.copy(), b=self.b.copy())
+
+    def with_parameters(self, W, a, b):
+        return RBMSpinHalf(self.N, hidden=self.hidden, W=W, a=a, b=b, dtype=self.dtype)
+
+    def fingerprint(self):
+        return (b"rbm|" + np.int64(self.N).tobytes() + np.int64(self.hidden).tobytes()
+                + self.W.tobytes() + self.a.tobytes() + self.b.tobytes())
+
+    def log_psi(self, config, cache=None):
+        from .problem import device_log_psi
+        return device_log_psi(self, config)
+
+    def log_diff_from_operator(self, config, op, cache=None):
+        from .problem import device_log_diff
+        return device_log_diff(op, self, config)
+
+    def log_diff_from_configs(self, config, new_configs, cache=None):
+        new = np.asarray(new_configs, dtype=float)
+        return self.log_psi(new) - self.log_psi(np.asarray(config, dtype=float)[None])[0]

@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .guides import JastrowSpinHalf
+from .guides import JastrowSpinHalf, RBMSpinHalf
 from .hamiltonian import SpinHalf
 
 __all__ = ["lower", "Problem", "get_problem"]
@@ -93,8 +93,9 @@ class Lowered:
         if not isinstance(H, SpinHalf):
             raise TypeError(f"pygfmc_amd propagates SpinHalf Hamiltonians; got "
                             f"{type(H).__name__}.")
-        if guide is not None and not isinstance(guide, JastrowSpinHalf):
-            raise TypeError(f"guide must be None or JastrowSpinHalf, got {type(guide).__name__}")
+        if guide is not None and not isinstance(guide, (JastrowSpinHalf, RBMSpinHalf)):
+            raise TypeError(f"guide must be None, JastrowSpinHalf or RBMSpinHalf, got "
+                            f"{type(guide).__name__}")
         if guide is not None and guide.N != H.N:
             raise ValueError(f"guide has N={guide.N} sites, the Hamiltonian {H.N}")
         self.H, self.guide = H, guide
@@ -108,25 +109,35 @@ class Lowered:
         a["hz"] = _f64(H.per_site(H.h_z)) if H.has_hz else None
         a["hx_half"] = _f64(H.per_site(H.h_x_half)) if H.n_hx else None
         cache = None
-        if guide is not None:
+        rbm = isinstance(guide, RBMSpinHalf)
+        if rbm:
+            a["rbm_W"] = np.ascontiguousarray(guide.W.reshape(-1), dtype=np.float32)
+            a["rbm_a"] = np.ascontiguousarray(guide.a, dtype=np.float32)
+            a["rbm_b"] = np.ascontiguousarray(guide.b, dtype=np.float32)
+        elif guide is not None:
             cache = guide.build_cache(H)
             a["jas_site_other"] = _i32(guide.site_other.reshape(-1))
             a["jas_site_J"] = _f64(guide.site_J.reshape(-1))
             a["jas_h"] = _f64(guide.h)
             a["jas_edge_other"] = _i32(cache.edge_other.reshape(-1))
             a["jas_edge_J"] = _f64(cache.edge_J.reshape(-1))
-        dep = dependency_lists(H, guide, cache)
         n_off = H.n_off
-        if full is None:
-            full = n_off > 0 and dep[0][-1] > n_off * max(_DEP_LIMIT, n_off // 4)
-        if not full:
-            a["dep_off"], a["dep_slots"], a["edge_off"], a["dep_edges"] = dep
+        if rbm:
+            full = True         # theta changes everywhere: every slot after every move
+        else:
+            dep = dependency_lists(H, guide, cache)
+            if full is None:
+                full = n_off > 0 and dep[0][-1] > n_off * max(_DEP_LIMIT, n_off // 4)
+            if not full:
+                a["dep_off"], a["dep_slots"], a["edge_off"], a["dep_edges"] = dep
         self.arrays = a
         self.scalars = dict(n_sites=H.N, n_edges=E, n_hx=H.n_hx, n_jpm=H.n_jpm,
                             has_jz=int(H.has_jz), uniform_jz=int(H.uniform_jz),
                             has_hz=int(H.has_hz),
-                            guide=_lib.GUIDE_JASTROW if guide is not None else _lib.GUIDE_NONE,
-                            jas_max_deg=guide.max_deg if guide is not None else 0)
+                            guide=(_lib.GUIDE_RBM if rbm else _lib.GUIDE_JASTROW
+                                   if guide is not None else _lib.GUIDE_NONE),
+                            jas_max_deg=guide.max_deg if (guide is not None and not rbm) else 0,
+                            rbm_hidden=guide.hidden if rbm else 0)
         self.full = bool(full)
 
     def desc(self):
@@ -136,7 +147,8 @@ class Lowered:
         for k, v in self.arrays.items():
             if v is None:
                 continue
-            ct = ctypes.c_int32 if v.dtype == np.int32 else ctypes.c_double
+            ct = {np.dtype(np.int32): ctypes.c_int32, np.dtype(np.float32): ctypes.c_float,
+                  np.dtype(np.float64): ctypes.c_double}[v.dtype]
             setattr(d, k, v.ctypes.data_as(ctypes.POINTER(ct)))
         return d
 
@@ -215,8 +227,11 @@ def precision_of(H, precision=None):
 
 
 def get_problem(H, guide=None, precision=None, full=None):
-    """Cached :class:`Problem` for (H, guide, precision)."""
+    """Cached :class:`Problem` for (H, guide, precision).  The RBM guide runs
+    in FP32 on BF16 weights whatever the Hamiltonian's dtype."""
     prec = precision_of(H, precision)
+    if precision is None and isinstance(guide, RBMSpinHalf):
+        prec = _lib.GFMC_FP32
     h = hashlib.sha1(H.fingerprint())
     h.update(b"none" if guide is None else guide.fingerprint())
     h.update(str((prec, full)).encode())
@@ -261,7 +276,8 @@ def device_log_diff(H, guide, config):
 def device_log_psi(guide, config):
     # a field-free Hamiltonian on no edges carries the guide's tables
     H = SpinHalf.create([], N=guide.N, dtype=np.float64)
-    p = get_problem(H, guide, precision=_lib.GFMC_FP64)
+    p = get_problem(H, guide, precision=(_lib.GFMC_FP32 if isinstance(guide, RBMSpinHalf)
+                                         else _lib.GFMC_FP64))
     spins, single = _configs_to_spins(config, guide.N)
     out = p.log_psi(spins).cpu().numpy()
     return out[0] if single else out

@@ -116,3 +116,20 @@ def random_model(N=8, seed=0, dtype=np.float64, lattice="chain", fields=True,
     G = JastrowSpinHalf(lat.N, pairs, beta=rng.uniform(0.05, 0.4, len(pairs)),
                         h=rng.uniform(-0.2, 0.2, lat.N), dtype=dtype)
     return H, G
+
+
+def random_rbm(N, alpha=2, seed=0, scale=0.4):
+    """An RBM guide with non-trivial random parameters (BF16 weights)."""
+    from pygfmc_amd.guides import RBMSpinHalf
+    rng = np.random.default_rng(seed)
+    H = alpha * N
+    return RBMSpinHalf(N, hidden=H, W=rng.normal(scale=scale, size=(H, N)),
+                       a=rng.normal(scale=0.2, size=N), b=rng.normal(scale=0.2, size=H))
+
+
+def rbm_log_psi(G, configs):
+    """a.x + sum_h log cosh(b_h + W_h . x), explicitly, in float64."""
+    c = np.atleast_2d(np.asarray(configs, dtype=np.float64))
+    th = c @ G.W.astype(np.float64).T + G.b.astype(np.float64)
+    out = c @ G.a.astype(np.float64) + np.sum(np.log(np.cosh(th)), axis=1)
+    return out if np.ndim(configs) > 1 else out[0]

@@ -1,0 +1,120 @@
+"""RBM guide on the GPU: FP32 kernels on BF16 weights against the FP64
+oracle, ratio consistency, and a GFMC run against exact diagonalisation."""
+import sys
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import ground_energy, random_rbm
+from pygfmc_amd import SpinHalf, chain, evaluate_observables, prepare_walkers_sz, run, square
+from pygfmc_amd import _lib
+from pygfmc_amd.problem import Problem, lower
+
+pytestmark = pytest.mark.gpu
+
+# FP32 arithmetic over H hidden units: errors of a few 1e-7 per unit add up
+# to ~1e-5 relative in a rate; the log-ratios are compared absolutely.
+RTOL = 2e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(spins):
+    return torch.as_tensor(spins.view(np.int32), device="cuda")
+
+
+def rbm_systems():
+    out = []
+    L = chain(12)
+    H = SpinHalf.create(L.edges, -1.0, -0.6, 0.7, h_x=0.4, dtype=np.float64)
+    out.append(("chain12_hx", H, random_rbm(12, alpha=2, seed=1)))
+    L = square(4, 4)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    out.append(("square4_heis", H, random_rbm(16, alpha=4, seed=2, scale=0.2)))
+    L = square(10, 10)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    out.append(("square10_alpha4", H, random_rbm(100, alpha=4, seed=3, scale=0.05)))
+    return out
+
+
+SYSTEMS = rbm_systems()
+IDS = [s[0] for s in SYSTEMS]
+
+
+@pytest.mark.parametrize("name,H,G", SYSTEMS, ids=IDS)
+def test_rbm_rates_match_oracle(orc, name, H, G):
+    say("rbm rates", name)
+    prob = Problem(H, G, _lib.GFMC_FP32)
+    spins = orc.init_sz(H.N, 96, H.N // 2, 7) if H.n_hx == 0 else orc.init_random(H.N, 96, 7)
+    r_ref, e_ref, ld_ref = orc.Problem(lower(H, G)).rates(spins)
+    r, e, ld = prob.rates(to_dev(spins), want_logdiff=True)
+    torch.cuda.synchronize()
+    r, e, ld = r.cpu().numpy(), e.cpu().numpy(), ld.cpu().numpy()
+    np.testing.assert_allclose(ld, ld_ref, atol=3e-5)
+    np.testing.assert_allclose(r, r_ref, rtol=RTOL, atol=RTOL * np.abs(r_ref).max())
+    np.testing.assert_allclose(e, e_ref, rtol=RTOL, atol=RTOL * np.abs(e_ref).max())
+
+
+@pytest.mark.parametrize("name,H,G", SYSTEMS, ids=IDS)
+def test_rbm_log_psi_matches_oracle_and_ratios(orc, name, H, G):
+    say("rbm log psi", name)
+    prob = Problem(H, G, _lib.GFMC_FP32)
+    spins = orc.init_random(H.N, 64, 9)
+    ref = orc.Problem(lower(H, G)).log_psi(spins)
+    lp = prob.log_psi(to_dev(spins)).cpu().numpy()
+    np.testing.assert_allclose(lp, ref, rtol=1e-5, atol=1e-4)
+    # the kernel's ratios against its own log psi of the successors
+    x = orc.unpack(spins[:1], H.N)[0]
+    succ = H.successors(x)
+    lp_s = prob.log_psi(to_dev(orc.pack(succ))).cpu().numpy()
+    _, _, ld = prob.rates(to_dev(spins[:1]), want_rates=False, want_eloc=False,
+                          want_logdiff=True)
+    np.testing.assert_allclose(ld.cpu().numpy()[0, 1:], lp_s[1:] - lp_s[0], atol=2e-4)
+
+
+def test_rbm_propagation_statistics_match_fp64_oracle(orc):
+    """One step from the same walkers: the FP32 RBM kernel and the FP64
+    oracle take the same path for most walkers (same uniforms), with the
+    same local energies to FP32 accuracy where they do."""
+    name, H, G = SYSTEMS[1]
+    prob = Problem(H, G, _lib.GFMC_FP32)
+    nw = 512
+    spins = orc.init_sz(16, nw, 8, 11)
+    s_ref, lw_ref, e_ref, ev_ref = orc.Problem(lower(H, G)).propagate(
+        spins, np.zeros(nw), np.zeros(nw), 0.2, 0.3, 77, 3, 0)
+    s = to_dev(spins)
+    lw = torch.zeros(nw, dtype=torch.float64, device="cuda")
+    e = torch.zeros(nw, dtype=torch.float64, device="cuda")
+    ev = torch.zeros(nw, dtype=torch.int32, device="cuda")
+    prob.propagate(s, lw, e, 0.2, 0.3, 77, 3, 0, ev)
+    torch.cuda.synchronize()
+    got = s.cpu().numpy().view(np.uint32)
+    same = np.all(got == s_ref, axis=1)
+    say("  same path:", same.mean(), "evals", ev_ref.mean())
+    assert same.mean() > 0.95
+    np.testing.assert_array_equal(ev.cpu().numpy()[same], ev_ref[same])
+    np.testing.assert_allclose(e.cpu().numpy()[same], e_ref[same], rtol=RTOL,
+                               atol=RTOL * np.abs(e_ref).max())
+
+
+def test_rbm_guided_gfmc_energy_matches_ed():
+    """Any positive guide gives the exact ground energy: 4x4 Heisenberg with
+    a random small RBM guide, within 4 sigma + 2e-3 of ED."""
+    say("rbm ED square4")
+    L = square(4, 4)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    e0 = ground_energy(H)
+    G = random_rbm(16, alpha=4, seed=5, scale=0.05)
+    w = prepare_walkers_sz(1024, 16, 1)
+    _, E, _, stats, raw = run(w, 2000, H, 0.2, G, [4, 32], rng_key=4, warmup_steps=400,
+                              verbosity=0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        E, _, err, _ = evaluate_observables(raw, n_blocks=20)
+    say("  E", E[32], "+-", err[32], "exact", e0, "it/s", stats["it_per_sec"])
+    assert abs(E[32] - e0) < 4 * err[32] + 2e-3

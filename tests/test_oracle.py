@@ -286,3 +286,28 @@ def test_pack_unpack_roundtrip(orc):
     rng = np.random.default_rng(0)
     c = rng.choice([-1.0, 1.0], size=(17, 70))
     np.testing.assert_array_equal(orc.unpack(orc.pack(c), 70), c)
+
+
+# -- RBM guide ---------------------------------------------------------------------
+
+def test_oracle_rbm_ratios_are_log_cosh_differences(orc):
+    """The oracle's RBM log-ratios and log psi against the explicit
+    a.x + sum_h log cosh(b + W x) on every successor."""
+    from helpers import rbm_log_psi, random_rbm
+    from pygfmc_amd import SpinHalf, chain
+    L = chain(8)
+    H = SpinHalf.create(L.edges, -1.0, -0.6, 0.7, h_x=0.4, dtype=np.float64)
+    G = random_rbm(8, alpha=2, seed=1)
+    prob = orc.Problem(lower(H, G))
+    X = basis(8)[::17]
+    rates, e_loc, ld = prob.rates(orc.pack(X))
+    for w, x in enumerate(X):
+        succ = H.successors(x)
+        ref = rbm_log_psi(G, succ) - rbm_log_psi(G, x)
+        np.testing.assert_allclose(ld[w, 1:], ref[1:], atol=1e-13)
+    np.testing.assert_allclose(prob.log_psi(orc.pack(X)), rbm_log_psi(G, X), atol=1e-12)
+    M = spinhalf_matrix(H).toarray()
+    lp_all = rbm_log_psi(G, basis(8))
+    for w, x in enumerate(X):
+        col = index_of(x)[0]
+        assert e_loc[w] == pytest.approx(M[:, col] @ np.exp(lp_all - lp_all[col]), rel=1e-11)
