@@ -221,6 +221,47 @@ int gfmc_wave_scan(int32_t precision, const double *in, double *out, void *strea
 int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key,
                      uint32_t *out);
 
+/* ---- RBM guide training ------------------------------------------------
+ * No counterpart in the reference, which trains guides by NetKet VMC energy
+ * minimisation (train_with_netket, generic_netket_training.py:331-473):
+ * here the RBM is trained by minimising the variance of the local energy
+ * over configurations GFMC generated.  Parameters travel as one flat FP32
+ * vector [W (hidden x n_sites, row-major), a (n_sites), b (hidden)] of
+ * gfmc_rbm_n_params elements; the problem itself holds W in BF16.  All
+ * calls are stream-ordered; a problem whose parameters are being set must
+ * not run on another stream at the same time. */
+
+/* H*N + N + H of an RBM problem, -1 otherwise. */
+int64_t gfmc_rbm_n_params(const gfmc_problem *p);
+
+/* The problem's parameters from a flat device vector: W rounded to BF16
+ * (nearest even), a and b copied, W's column sums recomputed. */
+int gfmc_rbm_set_params(gfmc_problem *p, const float *flat, void *stream);
+
+/* theta[i][h] = b_h + sum_k W_hk x_ik for n packed configurations (BF16
+ * MFMA, FP32 accumulation); theta: [n][hidden] FP32 device buffer. */
+int gfmc_rbm_theta(const gfmc_problem *p, int64_t n, const uint32_t *spins,
+                   float *theta, void *stream);
+
+/* Bytes of device workspace gfmc_rbm_vmin_grad needs for n samples. */
+int64_t gfmc_rbm_vmin_work_bytes(const gfmc_problem *p, int64_t n);
+
+/* Gradient of L = (1/n) sum_i (E_i - mean E)^2 over n packed samples,
+ * E_i = sum_k rates_k(x_i) with the problem's current parameters:
+ * dL/dp = (2/n) sum_i d_i sum_{k>=1} rates_k (O_p(x_i^k) - O_p(x_i)),
+ * O = d log psi / dp, d_i = E_i - mean E clamped to +-clip*sqrt(L) when
+ * clip > 0.  grad: flat FP32 [n_params]; out2 (device) <- (mean E, L).
+ * Deterministic (fixed-order reductions). */
+int gfmc_rbm_vmin_grad(const gfmc_problem *p, int64_t n, const uint32_t *spins,
+                       float clip, float *grad, double *out2, void *work,
+                       void *stream);
+
+/* One Adam step (Kingma & Ba 2015, in torch.optim.Adam's form) on n FP32
+ * device parameters with moment buffers m, v; step counts from 1. */
+int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *v,
+                   int64_t step, float lr, float beta1, float beta2, float eps,
+                   void *stream);
+
 #ifdef __cplusplus
 }
 #endif

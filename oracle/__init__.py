@@ -77,6 +77,9 @@ def lib():
         L.orc_init_random.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, _u32p]
         L.orc_pack.argtypes = [ctypes.c_int, ctypes.c_int64, _f64p, _u32p]
         L.orc_unpack.argtypes = [ctypes.c_int, ctypes.c_int64, _u32p, _f64p]
+        L.orc_rbm_vmin_grad.argtypes = [D, ctypes.c_int64, _u32p, ctypes.c_double, _f64p,
+                                        _f64p]
+        L.orc_rbm_theta_batch.argtypes = [D, ctypes.c_int64, _u32p, _f64p]
         del vp
         _LIB = L
     return _LIB
@@ -124,6 +127,26 @@ class Problem:
         out = np.empty(spins.shape[0])
         lib().orc_log_psi_batch(ctypes.byref(self.desc), spins.shape[0],
                                 _p(spins, ctypes.c_uint32), _p(out, ctypes.c_double))
+        return out
+
+    def rbm_vmin_grad(self, spins, clip=0.0):
+        """(flat gradient [W, a, b], mean E, variance) of the local-energy
+        variance over the samples ``spins`` (RBM guide)."""
+        spins = np.ascontiguousarray(spins, dtype=np.uint32)
+        Hh = int(self.lowered.scalars["rbm_hidden"])
+        g = np.empty(Hh * self.N + self.N + Hh)
+        out2 = np.empty(2)
+        lib().orc_rbm_vmin_grad(ctypes.byref(self.desc), spins.shape[0],
+                                _p(spins, ctypes.c_uint32), float(clip),
+                                _p(g, ctypes.c_double), _p(out2, ctypes.c_double))
+        return g, out2[0], out2[1]
+
+    def rbm_theta(self, spins):
+        spins = np.ascontiguousarray(spins, dtype=np.uint32)
+        Hh = int(self.lowered.scalars["rbm_hidden"])
+        out = np.empty((spins.shape[0], Hh))
+        lib().orc_rbm_theta_batch(ctypes.byref(self.desc), spins.shape[0],
+                                  _p(spins, ctypes.c_uint32), _p(out, ctypes.c_double))
         return out
 
     def propagate(self, spins, log_w, e_loc, beta, alw, seed, step, tag, n_threads=0):

@@ -623,3 +623,94 @@ void orc_init_random(int N, int64_t nw, uint64_t seed, uint32_t *spins) {
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* RBM training: local-energy variance minimisation (no reference           */
+/* counterpart; the reference trains guides by NetKet VMC energy            */
+/* minimisation, generic_netket_training.py:331-473).  Over n samples x_i:  */
+/*   L = (1/n) sum_i (E_i - Ebar)^2,   E_i = sum_k rates_k(x_i)             */
+/* and, by the definition of the rates (rates_k = H_k psi(x^k)/psi(x)),     */
+/*   dL/dp = (2/n) sum_i d_i sum_{k>=1} rates_k(x_i) (O_p(x_i^k) - O_p(x_i)) */
+/* with d_i = E_i - Ebar (clamped to +-clip*sqrt(L) when clip > 0), x^k the */
+/* successor of slot k and O_p = d log psi / dp:                            */
+/*   O_{W_hm} = tanh(theta_h) x_m,  O_{a_m} = x_m,  O_{b_h} = tanh(theta_h). */
+/* Flat parameter order [W (H x N row-major), a (N), b (H)].  Every O is    */
+/* computed from scratch for every successor, in FP64.                      */
+/* ------------------------------------------------------------------------ */
+static void rbm_O(const gfmc_problem_desc *d, const int8_t *x, double *theta, double *O) {
+    int H = d->rbm_hidden, N = d->n_sites;
+    orc_rbm_theta(d, x, theta);
+    for (int h = 0; h < H; ++h) {
+        double t = tanh(theta[h]);
+        for (int m = 0; m < N; ++m) O[(size_t)h * N + m] = t * (double)x[m];
+        O[(size_t)H * N + N + h] = t;
+    }
+    for (int m = 0; m < N; ++m) O[(size_t)H * N + m] = (double)x[m];
+}
+
+/* grad: H*N + N + H doubles; out2 = (mean E, variance L) */
+void orc_rbm_vmin_grad(const gfmc_problem_desc *d, int64_t n, const uint32_t *spins, double clip,
+                       double *grad, double *out2) {
+    int N = d->n_sites, W = n_words_of(N), H = d->rbm_hidden, n_off = n_off_of(d);
+    int M = 1 + n_off;
+    size_t P = (size_t)H * N + N + H;
+    double *rates = (double *)malloc(sizeof(double) * (size_t)n * M);
+    double *E = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    int8_t *x = (int8_t *)malloc((size_t)N), *xs = (int8_t *)malloc((size_t)N);
+    double *theta = (double *)malloc(sizeof(double) * (size_t)H);
+    double *O = (double *)malloc(sizeof(double) * P), *Os = (double *)malloc(sizeof(double) * P);
+    for (size_t q = 0; q < P; ++q) grad[q] = 0.0;
+    double mean = 0.0, var = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        unpack1(spins + i * W, N, x);
+        E[i] = orc_rates(d, x, rates + i * M, NULL);
+        mean += E[i];
+    }
+    mean /= (double)n;
+    for (int64_t i = 0; i < n; ++i) var += (E[i] - mean) * (E[i] - mean);
+    var /= (double)n;
+    double lim = clip > 0.0 ? clip * sqrt(var) : INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        double dev = E[i] - mean;
+        if (dev > lim) dev = lim;
+        if (dev < -lim) dev = -lim;
+        double g = 2.0 * dev / (double)n;
+        unpack1(spins + i * W, N, x);
+        rbm_O(d, x, theta, O);
+        for (int s = 0; s < n_off; ++s) {
+            double r = rates[i * M + 1 + s];
+            if (r == 0.0) continue;
+            memcpy(xs, x, (size_t)N);
+            if (s < d->n_hx) {
+                xs[s] = (int8_t)-xs[s];
+            } else {
+                int e = s - d->n_hx;
+                xs[d->edge_i[e]] = (int8_t)-xs[d->edge_i[e]];
+                xs[d->edge_j[e]] = (int8_t)-xs[d->edge_j[e]];
+            }
+            rbm_O(d, xs, theta, Os);
+            for (size_t q = 0; q < P; ++q) grad[q] += g * r * (Os[q] - O[q]);
+        }
+    }
+    out2[0] = mean;
+    out2[1] = var;
+    free(rates);
+    free(E);
+    free(x);
+    free(xs);
+    free(theta);
+    free(O);
+    free(Os);
+}
+
+/* theta (n x H) of a batch of packed configurations */
+void orc_rbm_theta_batch(const gfmc_problem_desc *d, int64_t n, const uint32_t *spins,
+                         double *theta) {
+    int N = d->n_sites, W = n_words_of(N), H = d->rbm_hidden;
+    int8_t *x = (int8_t *)malloc((size_t)N);
+    for (int64_t i = 0; i < n; ++i) {
+        unpack1(spins + i * W, N, x);
+        orc_rbm_theta(d, x, theta + i * H);
+    }
+    free(x);
+}

@@ -8,9 +8,10 @@ in hand-written HIP kernels (``libgfmc_hip.so``, built by
 from .engine import (DEFAULT_WARMUP_STEPS, evolve_walkers, estimate_average_log_branch_weight,
                      reconfiguration, run, run_gfmc, sample_index, systematic_parents)
 from .estimators import estimate_energy, evaluate_observables
-from .guides import JastrowSpinHalf
+from .guides import JastrowSpinHalf, RBMSpinHalf
 from .hamiltonian import SpinHalf
 from .lattice import Lattice, chain, kagome, square
+from .training import train_variance, vmin_gradient
 from .walkers import (PhiloxKey, Walker, prepare_walkers_from_configs, prepare_walkers_random,
                       prepare_walkers_sz)
 
@@ -18,6 +19,7 @@ __version__ = "0.1.0"
 
 __all__ = ["run", "run_gfmc", "evolve_walkers", "reconfiguration", "systematic_parents",
            "sample_index", "estimate_average_log_branch_weight", "estimate_energy",
-           "evaluate_observables", "SpinHalf", "JastrowSpinHalf", "Lattice", "chain", "square",
+           "evaluate_observables", "SpinHalf", "JastrowSpinHalf", "RBMSpinHalf", "train_variance",
+           "vmin_gradient", "Lattice", "chain", "square",
            "kagome", "Walker", "PhiloxKey", "prepare_walkers_sz", "prepare_walkers_random",
            "prepare_walkers_from_configs", "DEFAULT_WARMUP_STEPS"]

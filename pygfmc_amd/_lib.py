@@ -91,6 +91,15 @@ SIGNATURES = [
                                          _vp, _vp, _vp]),
     ("gfmc_wave_scan", ctypes.c_int, [ctypes.c_int32, _vp, _vp, _vp]),
     ("gfmc_philox_host", ctypes.c_int, [ctypes.c_int32, _u32p, _u32p, _u32p]),
+    ("gfmc_rbm_n_params", ctypes.c_int64, [_vp]),
+    ("gfmc_rbm_set_params", ctypes.c_int, [_vp, _vp, _vp]),
+    ("gfmc_rbm_theta", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
+    ("gfmc_rbm_vmin_work_bytes", ctypes.c_int64, [_vp, ctypes.c_int64]),
+    ("gfmc_rbm_vmin_grad", ctypes.c_int, [_vp, ctypes.c_int64, _vp, ctypes.c_float, _vp, _vp,
+                                          _vp, _vp]),
+    ("gfmc_adam_step", ctypes.c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int64,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, _vp]),
 ]
 
 _LIB = None

@@ -2046,3 +2046,5 @@ int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32
 }
 
 }  // extern "C"
+
+#include "gfmc_train.inl"

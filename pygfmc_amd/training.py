@@ -1,0 +1,154 @@
+"""Training of the RBM guide on configurations GFMC generates.
+
+The loss is the variance of the local energy over the current walker
+population, L = (1/n) sum_i (E_i - mean E)^2; it vanishes for an exact
+eigenstate on the walkers' support.  The reference has no counterpart: its
+guides are trained by NetKet VMC energy minimisation (train_with_netket,
+generic_netket_training.py:331-473).  Every step runs on the GPU through
+the C-ABI: GFMC branching steps with the current guide (gfmc_run_steps), the
+gradient (gfmc_rbm_vmin_grad: rates, theta by BF16 MFMA, the per-move
+backward pass, the B^T X product by MFMA), Adam (gfmc_adam_step) and the
+re-packing of the weights into the problem (gfmc_rbm_set_params).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .guides import RBMSpinHalf
+from .problem import Problem
+from .walkers import as_key
+
+__all__ = ["RBMTrainer", "vmin_gradient", "train_variance"]
+
+
+def flat_params(rbm):
+    """[W (hidden x N row-major), a, b] as one float32 vector."""
+    return np.concatenate([rbm.W.reshape(-1), rbm.a, rbm.b]).astype(np.float32)
+
+
+def rbm_from_flat(rbm, flat):
+    flat = np.asarray(flat, dtype=np.float32)
+    H, N = rbm.hidden, rbm.N
+    return rbm.with_parameters(flat[:H * N].reshape(H, N), flat[H * N:H * N + N],
+                               flat[H * N + N:])
+
+
+class RBMTrainer:
+    """Device state of variance minimisation for one RBM problem: FP32 master
+    parameters, Adam moments, the gradient and its workspace."""
+
+    def __init__(self, problem, rbm, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0):
+        if not isinstance(rbm, RBMSpinHalf):
+            raise TypeError(f"RBMTrainer trains an RBMSpinHalf, got {type(rbm).__name__}")
+        if problem.info.guide != _lib.GUIDE_RBM:
+            raise ValueError("the problem's guide is not an RBM")
+        self.p = problem
+        self.rbm = rbm
+        lib = _lib.load()
+        self.n_params = int(lib.gfmc_rbm_n_params(problem.handle))
+        dev = _lib.device()
+        self.params = torch.as_tensor(flat_params(rbm), device=dev)
+        if self.params.numel() != self.n_params:
+            raise ValueError("parameter count differs from the problem's")
+        self.grad = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.t = 0
+        self.lr, self.betas, self.eps, self.clip = float(lr), betas, float(eps), float(clip)
+        self._work = None
+        self._work_n = -1
+
+    def _workspace(self, n):
+        if n != self._work_n:
+            nb = int(_lib.load().gfmc_rbm_vmin_work_bytes(self.p.handle, n))
+            if nb < 0:
+                raise _lib.GfmcError(_lib.load().gfmc_last_error().decode())
+            self._work = torch.empty(nb, dtype=torch.uint8, device=self.params.device)
+            self._work_n = n
+        return self._work
+
+    def gradient(self, spins, out2):
+        """Enqueue the variance gradient over ``spins`` into ``self.grad``;
+        ``out2`` (2 device doubles) receives (mean E, variance)."""
+        n = spins.shape[0]
+        _lib.check(_lib.load().gfmc_rbm_vmin_grad(
+            self.p.handle, n, _lib.ptr(spins), self.clip, _lib.ptr(self.grad), _lib.ptr(out2),
+            _lib.ptr(self._workspace(n)), _lib.stream()), "gfmc_rbm_vmin_grad")
+
+    def update(self):
+        """One Adam step on the master parameters, then the problem's BF16
+        weights from them."""
+        self.t += 1
+        lib = _lib.load()
+        _lib.check(lib.gfmc_adam_step(self.n_params, _lib.ptr(self.params), _lib.ptr(self.grad),
+                                      _lib.ptr(self.m), _lib.ptr(self.v), self.t, self.lr,
+                                      float(self.betas[0]), float(self.betas[1]), self.eps,
+                                      _lib.stream()), "gfmc_adam_step")
+        self.push()
+
+    def push(self):
+        _lib.check(_lib.load().gfmc_rbm_set_params(self.p.handle, _lib.ptr(self.params),
+                                                   _lib.stream()), "gfmc_rbm_set_params")
+
+    def guide(self):
+        """The trained guide (host copy; W rounded to BF16)."""
+        return rbm_from_flat(self.rbm, self.params.cpu().numpy())
+
+
+def vmin_gradient(hamiltonian, rbm, spins, clip=0.0):
+    """(flat gradient, mean E, variance) of the local-energy variance of
+    ``rbm`` over the packed device configurations ``spins``."""
+    p = Problem(hamiltonian, rbm, _lib.GFMC_FP32)
+    tr = RBMTrainer(p, rbm, clip=clip)
+    out2 = torch.empty(2, dtype=torch.float64, device=spins.device)
+    tr.gradient(spins, out2)
+    o = out2.cpu().numpy()
+    return tr.grad.cpu().numpy(), float(o[0]), float(o[1])
+
+
+def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, step_size=0.2,
+                   lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0, rng_key=0,
+                   warmup_steps=40, verbosity=0):
+    """Variance minimisation of ``rbm`` on GFMC samples.
+
+    The walkers evolve under GFMC with the current guide: ``warmup_steps``
+    branching steps, then per iteration ``steps_per_iter`` steps followed by
+    one gradient + Adam update on the resampled population.  Returns
+    ``(trained RBMSpinHalf, history)`` with history arrays ``mean_e`` and
+    ``variance`` (before each update) and ``seconds`` (GPU wall time).
+    """
+    key = as_key(rng_key)
+    from .engine import Chain
+    p = Problem(hamiltonian, rbm, _lib.GFMC_FP32)
+    tr = RBMTrainer(p, rbm, lr=lr, betas=betas, eps=eps, clip=clip)
+    chain = Chain(p, walkers)
+    dev = chain.spins.device
+    hist = torch.zeros((max(n_iter, 1), 2), dtype=torch.float64, device=dev)
+    _, e, _ = p.rates(chain.spins, want_rates=False)
+    alw = float(torch.mean(-e).item())
+    torch.cuda.synchronize()
+    t0 = time.time()
+    counter = key.position
+    if warmup_steps:
+        chain.steps(warmup_steps, step_size, alw, key.seed, counter, warmup=True)
+        counter += warmup_steps
+    for it in range(n_iter):
+        chain.steps(steps_per_iter, step_size, alw, key.seed, counter)
+        counter += steps_per_iter
+        tr.gradient(chain.spins, hist[it])
+        tr.update()
+        if verbosity and (it + 1) % max(1, n_iter // 10) == 0:
+            h = hist[it].cpu().numpy()
+            alw = -float(h[0])
+            print(f"[train] iter {it + 1}/{n_iter}: E = {h[0]:.6f}, var = {h[1]:.6f}",
+                  flush=True)
+    torch.cuda.synchronize()
+    secs = time.time() - t0
+    h = hist[:n_iter].cpu().numpy()
+    history = dict(mean_e=h[:, 0].copy(), variance=h[:, 1].copy(), seconds=secs,
+                   walker_steps=walkers.n_walkers * (warmup_steps + n_iter * steps_per_iter))
+    return tr.guide(), history

@@ -311,3 +311,89 @@ def test_oracle_rbm_ratios_are_log_cosh_differences(orc):
     for w, x in enumerate(X):
         col = index_of(x)[0]
         assert e_loc[w] == pytest.approx(M[:, col] @ np.exp(lp_all - lp_all[col]), rel=1e-11)
+
+
+# -- RBM training: variance-loss gradient -----------------------------------------
+
+def _torch_variance_grad(H, G, X, clip):
+    """Gradient of the local-energy variance by torch FP64 autograd, with E_L
+    from the Kronecker matrix (E_L(x) = sum_y M_yx psi(y) / psi(x)) and the
+    explicit log psi = a.x + sum_h log cosh(b + W x); clip > 0 differentiates
+    the clamped surrogate (2/n) sum_i clamp(d_i) E_i."""
+    import torch
+    M = torch.tensor(spinhalf_matrix(H).toarray())
+    B = torch.tensor(basis(H.N))
+    W = torch.tensor(G.W.astype(np.float64), requires_grad=True)
+    a = torch.tensor(G.a.astype(np.float64), requires_grad=True)
+    b = torch.tensor(G.b.astype(np.float64), requires_grad=True)
+    lp = B @ a + torch.log(torch.cosh(B @ W.T + b)).sum(1)
+    idx = torch.tensor(index_of(X))
+    E = (M[:, idx] * torch.exp(lp[:, None] - lp[idx][None, :])).sum(0)
+    n = len(idx)
+    dev = E - E.mean()
+    var = (dev ** 2).mean()
+    if clip > 0:
+        lim = clip * torch.sqrt(var.detach())
+        loss = (2.0 / n) * (torch.clamp(dev.detach(), -lim, lim) * E).sum()
+    else:
+        loss = var
+    loss.backward()
+    g = np.concatenate([W.grad.numpy().ravel(), a.grad.numpy(), b.grad.numpy()])
+    return g, float(E.detach().mean()), float(var.detach())
+
+
+@pytest.mark.parametrize("clip", [0.0, 1.0])
+@pytest.mark.parametrize("hx", [0.0, 0.4])
+def test_oracle_rbm_variance_gradient_matches_autograd(orc, clip, hx):
+    from helpers import random_rbm
+    from pygfmc_amd import SpinHalf, chain
+    L = chain(8)
+    H = SpinHalf.create(L.edges, -1.0, -0.6, 0.7, h_x=hx, dtype=np.float64)
+    G = random_rbm(8, alpha=2, seed=3, scale=0.3)
+    rng = np.random.default_rng(5)
+    X = basis(8)[rng.choice(256, 40, replace=False)]
+    if hx == 0.0:      # stay in a fixed-magnetisation sector as GFMC does
+        X = X[X.sum(1) == 0] if np.any(X.sum(1) == 0) else X
+    g, mean, var = orc.Problem(lower(H, G)).rbm_vmin_grad(orc.pack(X), clip)
+    g_ref, mean_ref, var_ref = _torch_variance_grad(H, G, X, clip)
+    assert mean == pytest.approx(mean_ref, rel=1e-12)
+    assert var == pytest.approx(var_ref, rel=1e-10)
+    np.testing.assert_allclose(g, g_ref, rtol=1e-9, atol=1e-12 * np.abs(g_ref).max())
+
+
+def test_oracle_rbm_variance_gradient_matches_finite_differences(orc):
+    """Central differences of the oracle's own variance on a few parameters
+    (a and b entries, exact in float32 steps)."""
+    from helpers import random_rbm
+    from pygfmc_amd import SpinHalf, chain
+    from pygfmc_amd.guides import RBMSpinHalf
+    L = chain(8)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    G = random_rbm(8, alpha=2, seed=4, scale=0.3)
+    X = basis(8)[[3, 5, 6, 9, 10, 12, 15, 60, 90, 99, 165, 195]]
+    X = X[X.sum(1) == 0]
+    spins = orc.pack(X)
+    g, _, _ = orc.Problem(lower(H, G)).rbm_vmin_grad(spins)
+    P = G.parameters()
+    nW = G.W.size
+    h = 2.0 ** -10
+    for which, k, off in (("a", 2, nW + 2), ("b", 5, nW + 8 + 5), ("b", 11, nW + 8 + 11)):
+        vals = []
+        for sgn in (1, -1):
+            q = {key: v.copy() for key, v in P.items()}
+            q[which][k] += sgn * h
+            Gq = RBMSpinHalf(8, hidden=G.hidden, W=q["W"], a=q["a"], b=q["b"])
+            vals.append(orc.Problem(lower(H, Gq)).rbm_vmin_grad(spins)[2])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert g[off] == pytest.approx(fd, rel=2e-5, abs=1e-9)
+
+
+def test_oracle_rbm_theta_batch(orc):
+    from helpers import random_rbm
+    from pygfmc_amd import SpinHalf, chain
+    G = random_rbm(10, alpha=3, seed=6)
+    H = SpinHalf.create(chain(10).edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    X = basis(10)[::37]
+    th = orc.Problem(lower(H, G)).rbm_theta(orc.pack(X))
+    ref = X @ G.W.astype(np.float64).T + G.b.astype(np.float64)
+    np.testing.assert_allclose(th, ref, rtol=0, atol=1e-12)
