@@ -35,6 +35,11 @@ def parse():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-race", action="store_true",
+                    help="skip the second metric (GPU-seconds to a fixed energy error bar, "
+                         "learned RBM guide vs Jastrow, 10x10 Heisenberg)")
+    ap.add_argument("--race-train-iters", type=int, default=600)
+    ap.add_argument("--race-rbm-steps", type=int, default=300)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before the timed steps, copy a tensor of this many MiB once "
                          "(a plain copy kernel of known size, to scale HBM counters)")
@@ -148,7 +153,95 @@ def main():
                      "flop_per_eval": flop_per_eval},
         "cpu_baseline": cpu,
     }
+    if not args.no_race:
+        out["sigma_race"] = sigma_race(args)
     print(json.dumps(out), flush=True)
+
+
+def timed_run(walkers, n_steps, H, G, p_array, seed, warmup):
+    """run() bracketed by device synchronisation: (GPU seconds, E, sigma at
+    max p, stats)."""
+    import warnings
+
+    import torch
+
+    from pygfmc_amd import evaluate_observables, run
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, _, st, raw = run(walkers, n_steps, H, 0.2, G, p_array, rng_key=seed,
+                           warmup_steps=warmup, verbosity=0)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        E, _, err, _ = evaluate_observables(raw, n_blocks=20)
+    pm = max(p_array)
+    return secs, float(E[pm]), float(err[pm]), st
+
+
+def sigma_race(args):
+    """Second metric on the 10 x 10 periodic Heisenberg lattice (J_x = J_y =
+    -1, J_z = 1, Sz = 0), p = [4, 16, 64, 128], 16 384 walkers, step 0.2:
+    GPU-seconds until the jackknife error bar (20 blocks) of E at p = 128
+    reaches that of the Jastrow-FP64 run (1000 production steps after 400
+    warm-up steps), for the learned BF16 RBM guide (alpha = 4, trained on the
+    GPU by local-energy variance minimisation on GFMC samples of the beta =
+    0.15 Jastrow guide) with and without its training time.  The RBM's time
+    to the target error bar is extrapolated from its own run by sigma^2 ~
+    1 / production steps.  Two Jastrow baselines: beta = 0.3 (the reference
+    configurations' guide) and beta = 0.15 (the best of a scan)."""
+    import torch
+
+    from pygfmc_amd import (JastrowSpinHalf, RBMSpinHalf, SpinHalf, prepare_walkers_sz, square,
+                            train_variance)
+    lat = square(10, 10)
+    P = [4, 16, 64, 128]
+    nw = 16384
+    H64 = SpinHalf.create(lat.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    H32 = SpinHalf.create(lat.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    out = {"workload": "heisenberg_square10x10_w16384_step0.2_p4-128",
+           "exact_E": -67.1549, "exact_source": "Sandvik, PRB 56, 11678 (1997): E/N = -0.671549 "
+                                                 "at L = 10"}
+    jas = {}
+    for beta in (0.3, 0.15):
+        G = JastrowSpinHalf(100, lat.edges, beta=beta, dtype=np.float64)
+        w = prepare_walkers_sz(nw, 100, 0)
+        secs, E, err, st = timed_run(w, 1000, H64, G, P, 1, 400)
+        jas[beta] = dict(beta=beta, gpu_s=secs, E=E, sigma=err, steps=1000, warmup=400,
+                         walkers=nw, evals_per_step=st["evals_per_step_mean"])
+    out["jastrow_fp64_beta0.3"] = jas[0.3]
+    out["jastrow_fp64_beta0.15"] = jas[0.15]
+    # training
+    sampler = JastrowSpinHalf(100, lat.edges, beta=0.15, dtype=np.float32)
+    rbm0 = RBMSpinHalf(100, alpha=4, seed=0, scale=0.01)
+    w = prepare_walkers_sz(4096, 100, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    G, hist = train_variance(H32, rbm0, w, n_iter=args.race_train_iters, steps_per_iter=2,
+                             lr=1e-3, clip=3.0, rng_key=3, sampler=sampler)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    v = hist["variance"]
+    # production with the trained guide
+    w = prepare_walkers_sz(nw, 100, 0)
+    warm = 100
+    secs, E, err, st = timed_run(w, args.race_rbm_steps, H32, G, P, 1, warm)
+    frac_prod = args.race_rbm_steps / (args.race_rbm_steps + warm)
+    out["rbm_bf16"] = dict(alpha=4, hidden=400, train_gpu_s=train_s,
+                           train_iters=args.race_train_iters, train_walkers=4096,
+                           variance_start=float(v[:5].mean()), variance_end=float(v[-20:].mean()),
+                           gpu_s=secs, E=E, sigma=err, steps=args.race_rbm_steps, warmup=warm,
+                           walkers=nw, evals_per_step=st["evals_per_step_mean"])
+    for beta, J in jas.items():
+        t_warm = secs * (1 - frac_prod)
+        t_need = t_warm + secs * frac_prod * (err / J["sigma"]) ** 2
+        z = (E - J["E"]) / np.hypot(err, J["sigma"])
+        out[f"vs_jastrow_beta{beta:g}"] = dict(
+            target_sigma=J["sigma"], jastrow_gpu_s=J["gpu_s"], rbm_gpu_s=t_need,
+            rbm_gpu_s_with_training=t_need + train_s,
+            speedup=J["gpu_s"] / t_need, speedup_with_training=J["gpu_s"] / (t_need + train_s),
+            energy_z=float(z))
+    return out
 
 
 def cpu_baseline(H, G, chain, N, beta, alw, seed, seconds):

@@ -112,23 +112,38 @@ def vmin_gradient(hamiltonian, rbm, spins, clip=0.0):
 
 def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, step_size=0.2,
                    lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0, rng_key=0,
-                   warmup_steps=40, verbosity=0):
+                   warmup_steps=40, sampler="self", verbosity=0, trainer=None):
     """Variance minimisation of ``rbm`` on GFMC samples.
 
-    The walkers evolve under GFMC with the current guide: ``warmup_steps``
-    branching steps, then per iteration ``steps_per_iter`` steps followed by
-    one gradient + Adam update on the resampled population.  Returns
-    ``(trained RBMSpinHalf, history)`` with history arrays ``mean_e`` and
-    ``variance`` (before each update) and ``seconds`` (GPU wall time).
+    The walkers evolve under GFMC guided by ``sampler``: ``"self"`` (the RBM
+    being trained), ``None`` (no guide) or any guide ``run`` accepts, e.g. a
+    Jastrow guide.  A fixed sampler draws from psi_S phi_0, which does not
+    move with the RBM; sampling with the RBM itself draws from
+    psi_RBM phi_0 but can freeze when the RBM suppresses every move.
+    ``warmup_steps`` branching steps, then per iteration ``steps_per_iter``
+    steps followed by one gradient + Adam update on the resampled
+    population.  ``trainer`` continues an earlier run (its parameters and
+    Adam moments).  Returns ``(trained RBMSpinHalf, history)``: arrays
+    ``mean_e`` and ``variance`` (of the RBM's local energies, before each
+    update), ``seconds`` (GPU wall time), ``walker_steps`` and ``trainer``.
     """
     key = as_key(rng_key)
     from .engine import Chain
-    p = Problem(hamiltonian, rbm, _lib.GFMC_FP32)
-    tr = RBMTrainer(p, rbm, lr=lr, betas=betas, eps=eps, clip=clip)
-    chain = Chain(p, walkers)
+    from .problem import get_problem
+    if trainer is None:
+        p = Problem(hamiltonian, rbm, _lib.GFMC_FP32)
+        trainer = RBMTrainer(p, rbm, lr=lr, betas=betas, eps=eps, clip=clip)
+    else:
+        p = trainer.p
+        trainer.lr, trainer.betas, trainer.eps, trainer.clip = float(lr), betas, float(eps), \
+            float(clip)
+    tr = trainer
+    ps = p if (isinstance(sampler, str) and sampler == "self") else get_problem(hamiltonian,
+                                                                                sampler)
+    chain = Chain(ps, walkers)
     dev = chain.spins.device
     hist = torch.zeros((max(n_iter, 1), 2), dtype=torch.float64, device=dev)
-    _, e, _ = p.rates(chain.spins, want_rates=False)
+    _, e, _ = ps.rates(chain.spins, want_rates=False)
     alw = float(torch.mean(-e).item())
     torch.cuda.synchronize()
     t0 = time.time()
@@ -143,12 +158,12 @@ def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, s
         tr.update()
         if verbosity and (it + 1) % max(1, n_iter // 10) == 0:
             h = hist[it].cpu().numpy()
-            alw = -float(h[0])
             print(f"[train] iter {it + 1}/{n_iter}: E = {h[0]:.6f}, var = {h[1]:.6f}",
                   flush=True)
     torch.cuda.synchronize()
     secs = time.time() - t0
     h = hist[:n_iter].cpu().numpy()
     history = dict(mean_e=h[:, 0].copy(), variance=h[:, 1].copy(), seconds=secs,
-                   walker_steps=walkers.n_walkers * (warmup_steps + n_iter * steps_per_iter))
+                   walker_steps=walkers.n_walkers * (warmup_steps + n_iter * steps_per_iter),
+                   trainer=tr)
     return tr.guide(), history
