@@ -221,6 +221,45 @@ int gfmc_wave_scan(int32_t precision, const double *in, double *out, void *strea
 int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key,
                      uint32_t *out);
 
+/* ---- Batches of independent lattices -------------------------------------
+ * Many lattices of uneven size in one launch (no reference counterpart: the
+ * reference runs one Hamiltonian per run_gfmc call, driver.py:1148-1547;
+ * batching replaces a loop of runs).  Lattice l owns walkers [woff[l],
+ * woff[l+1]) in the order given; spins use a common stride of the largest
+ * word count (gfmc_batch_get_info), padding words zero.  Lattice l of a
+ * batch reproduces bit for bit the single-lattice calls with the seed
+ * seed + l; parents are global walker indices; out2 / series rows hold
+ * (log G - log n_l, E_l) per lattice. */
+typedef struct gfmc_batch gfmc_batch;
+
+/* All problems need one precision, no RBM guide and the same kernel path
+ * (gfmc_problem_info.full_recompute aside); n_walkers[l] >= 1.  The batch
+ * copies the problems' device descriptors: they must outlive it. */
+int gfmc_batch_create(const gfmc_problem *const *problems, int32_t n_lattices,
+                      const int64_t *n_walkers, gfmc_batch **out);
+int gfmc_batch_destroy(gfmc_batch *b);
+/* total walkers, spin stride (words) and reconfiguration scratch (doubles) */
+int gfmc_batch_get_info(const gfmc_batch *b, int64_t *n_total, int32_t *stride,
+                        int64_t *work_doubles);
+
+/* gfmc_propagate for every lattice; alw: device [n_lattices] shifts. */
+int gfmc_batch_propagate(const gfmc_batch *b, uint32_t *spins, double *log_w,
+                         double *e_loc, double beta, const double *alw, uint64_t seed,
+                         uint32_t step, uint32_t tag, int32_t *evals, void *stream);
+
+/* gfmc_reconfigure for every lattice's population (segmented, one set of
+ * launches); out2: device [n_lattices][2]. */
+int gfmc_batch_reconfigure(const gfmc_batch *b, const double *log_w, const double *e_loc,
+                           uint64_t seed, uint32_t step, uint32_t tag, int32_t *parents,
+                           double *out2, double *work, void *stream);
+
+/* gfmc_run_steps for the batch; series: device [n_steps][n_lattices][2]. */
+int gfmc_batch_run_steps(const gfmc_batch *b, uint32_t *spins, double *log_w, double *e_loc,
+                         uint32_t *spins_b, double *e_loc_b, int32_t *parents, double *work,
+                         int32_t n_steps, double beta, const double *alw, uint64_t seed,
+                         uint32_t step0, uint32_t tag_prop, uint32_t tag_reconf,
+                         double *series, int32_t *evals, void *stream);
+
 /* ---- RBM guide training ------------------------------------------------
  * No counterpart in the reference, which trains guides by NetKet VMC energy
  * minimisation (train_with_netket, generic_netket_training.py:331-473):

@@ -9,7 +9,8 @@ from .engine import (DEFAULT_WARMUP_STEPS, evolve_walkers, estimate_average_log_
                      reconfiguration, run, run_gfmc, sample_index, systematic_parents)
 from .estimators import estimate_energy, evaluate_observables
 from .guides import JastrowSpinHalf, RBMSpinHalf
-from .hamiltonian import SpinHalf
+from .batch import Batch, run_batch
+from .hamiltonian import SpinHalf, tfim_chain_ground_energy, transverse_field_ising
 from .lattice import Lattice, chain, kagome, square
 from .training import train_variance, vmin_gradient
 from .walkers import (PhiloxKey, Walker, prepare_walkers_from_configs, prepare_walkers_random,
@@ -20,6 +21,7 @@ __version__ = "0.1.0"
 __all__ = ["run", "run_gfmc", "evolve_walkers", "reconfiguration", "systematic_parents",
            "sample_index", "estimate_average_log_branch_weight", "estimate_energy",
            "evaluate_observables", "SpinHalf", "JastrowSpinHalf", "RBMSpinHalf", "train_variance",
-           "vmin_gradient", "Lattice", "chain", "square",
+           "vmin_gradient", "run_batch", "Batch", "transverse_field_ising",
+           "tfim_chain_ground_energy", "Lattice", "chain", "square",
            "kagome", "Walker", "PhiloxKey", "prepare_walkers_sz", "prepare_walkers_random",
            "prepare_walkers_from_configs", "DEFAULT_WARMUP_STEPS"]

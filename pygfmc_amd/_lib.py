@@ -91,6 +91,21 @@ SIGNATURES = [
                                          _vp, _vp, _vp]),
     ("gfmc_wave_scan", ctypes.c_int, [ctypes.c_int32, _vp, _vp, _vp]),
     ("gfmc_philox_host", ctypes.c_int, [ctypes.c_int32, _u32p, _u32p, _u32p]),
+    ("gfmc_batch_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_vp)]),
+    ("gfmc_batch_destroy", ctypes.c_int, [_vp]),
+    ("gfmc_batch_get_info", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int32),
+                                           ctypes.POINTER(ctypes.c_int64)]),
+    ("gfmc_batch_propagate", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, _vp,
+                                            ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                            _vp, _vp]),
+    ("gfmc_batch_reconfigure", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32,
+                                              ctypes.c_uint32, _vp, _vp, _vp, _vp]),
+    ("gfmc_batch_run_steps", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            ctypes.c_int32, ctypes.c_double, _vp,
+                                            ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint32, _vp, _vp, _vp]),
     ("gfmc_rbm_n_params", ctypes.c_int64, [_vp]),
     ("gfmc_rbm_set_params", ctypes.c_int, [_vp, _vp, _vp]),
     ("gfmc_rbm_theta", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),

@@ -547,25 +547,23 @@ __device__ int apply_move(const DevProblem &P, const Tables<T> &tb, WalkerLds<T>
 // ---------------------------------------------------------------------------
 // propagation kernel (evolve_walker_cont_time, evolution.py:559-721)
 // ---------------------------------------------------------------------------
+// one walker's branching step by the workgroup's wavefront: spins at sp,
+// log_w / e_loc / evals at lw_p / el_p / ev_p; wk is the walker index in the
+// Philox counter (hop, wk, step, tag)
 template <typename T, bool FAST>
-__global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
-                                                    uint32_t *spins, double *log_w,
-                                                    double *e_loc, double beta, double alw,
-                                                    uint32_t key0, uint32_t key1,
-                                                    uint32_t step, uint32_t tag,
-                                                    int32_t *evals) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    const int lane = threadIdx.x;
-    const int64_t wk = blockIdx.x;
-    if (wk >= nw) return;
+__device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tables<T> &tb,
+                                                 unsigned char *lds, int lane, uint32_t wk,
+                                                 uint32_t *sp, double *lw_p, double *el_p,
+                                                 int32_t *ev_p, double beta, double alw,
+                                                 uint32_t key0, uint32_t key1, uint32_t step,
+                                                 uint32_t tag) {
     WalkerLds<T> L = carve<T>(P, lds);
-    uint32_t *sp = spins + wk * P.words;
     load_spins(P, sp, L.x, lane);
     __syncthreads();
     int n_anti = full_eval<T, FAST>(P, tb, L, lane);
     __syncthreads();
     double diag = diag_value(P, L.x, L.anti, n_anti, lane);
-    double lw = log_w[wk];
+    double lw = *lw_p;
     double t = 0.0, eL = 0.0;
     uint32_t it = 0;
     while (t < beta) {
@@ -573,7 +571,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
         T incl = lane_scan(P, L, lane, &lane_sum);
         T R = lane_bcast(incl, WAVE - 1);
         eL = diag - (double)R;
-        gfmc_u4 r = gfmc_philox(it, (uint32_t)wk, step, tag, key0, key1);
+        gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
         double psi = gfmc_u53(r.z, r.w);
         int idx = -1;
         double tau_raw = INFINITY;
@@ -605,10 +603,50 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
     lw = lw - alw * beta;
     store_spins(P, sp, L.x, lane);
     if (lane == 0) {
-        log_w[wk] = lw;
-        e_loc[wk] = eL;
-        if (evals) evals[wk] += (int32_t)it;
+        *lw_p = lw;
+        *el_p = eL;
+        if (ev_p) *ev_p += (int32_t)it;
     }
+}
+
+template <typename T, bool FAST>
+__global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
+                                                    uint32_t *spins, double *log_w,
+                                                    double *e_loc, double beta, double alw,
+                                                    uint32_t key0, uint32_t key1,
+                                                    uint32_t step, uint32_t tag,
+                                                    int32_t *evals) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t wk = blockIdx.x;
+    if (wk >= nw) return;
+    propagate_walker<T, FAST>(P, tb, lds, threadIdx.x, (uint32_t)wk, spins + wk * P.words,
+                              log_w + wk, e_loc + wk, evals ? evals + wk : nullptr, beta, alw,
+                              key0, key1, step, tag);
+}
+
+// A batch of independent lattices in one launch: walker g belongs to lattice
+// l with woff[l] <= g < woff[l + 1] and runs exactly as walker g - woff[l] of
+// a single-lattice launch with the Philox key seed + l and shift alw[l];
+// spins of every lattice use the common stride `stride` words.
+template <typename T, bool FAST>
+__global__ __launch_bounds__(WAVE) void k_propagate_batch(
+    const DevProblem *Ps, const Tables<T> *Ts, int n_lat, const int64_t *woff, int64_t n_total,
+    int stride, uint32_t *spins, double *log_w, double *e_loc, double beta, const double *alw,
+    uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t g = blockIdx.x;
+    if (g >= n_total) return;
+    int lo = 0, hi = n_lat;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (woff[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const uint64_t key = seed + (uint64_t)lo;
+    propagate_walker<T, FAST>(Ps[lo], Ts[lo], lds, threadIdx.x, (uint32_t)(g - woff[lo]),
+                              spins + g * stride, log_w + g, e_loc + g,
+                              evals ? evals + g : nullptr, beta, alw[lo], (uint32_t)key,
+                              (uint32_t)(key >> 32), step, tag);
 }
 
 // ---------------------------------------------------------------------------
@@ -1077,6 +1115,51 @@ __device__ __forceinline__ RcScratch rc_scratch(double *work, int64_t n, int B) 
     return s;
 }
 
+// Segments: independent populations reconfigured in one launch (lattices of
+// a batch).  nseg == 0: one population of n walkers, no tables.  Otherwise
+// segment g holds walkers [w[g], w[g+1]) and blocks [b[g], b[g+1]) (each
+// segment starts a new block), and its scratch of n_g + 7 B_g + 1 doubles at
+// work + w[g] + 7 b[g] + g.  Every segment is reconfigured exactly as a
+// population of its own.
+struct RcSegs {
+    int nseg;
+    int64_t n;
+    const int64_t *w;
+    const int32_t *b;
+};
+
+struct RcLoc {
+    int seg, lb, B;
+    int64_t w0, n;
+    double *work;
+};
+
+__device__ __forceinline__ RcLoc rc_locate(const RcSegs &S, double *work, int bid) {
+    RcLoc r;
+    if (S.nseg == 0) {
+        r.seg = 0;
+        r.w0 = 0;
+        r.n = S.n;
+        r.lb = bid;
+        r.B = (int)((S.n + RB - 1) / RB);
+        r.work = work;
+        return r;
+    }
+    int lo = 0, hi = S.nseg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (S.b[mid] <= bid) lo = mid;
+        else hi = mid;
+    }
+    r.seg = lo;
+    r.w0 = S.w[lo];
+    r.n = S.w[lo + 1] - r.w0;
+    r.lb = bid - S.b[lo];
+    r.B = S.b[lo + 1] - S.b[lo];
+    r.work = work + r.w0 + 7 * (int64_t)S.b[lo] + lo;
+    return r;
+}
+
 // inclusive scan of per-thread values in block order
 __device__ double rc_block_incl(double v, double *sh4) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1135,32 +1218,34 @@ __device__ __forceinline__ double rc_m0(const RcScratch &s, int B) {
 }
 
 // 1: block maxima of log_w
-__global__ __launch_bounds__(RC_T) void k_rc_max(int64_t n, const double *lw, double *work) {
+__global__ __launch_bounds__(RC_T) void k_rc_max(RcSegs S, const double *lw, double *work) {
     __shared__ double sh4[RC_T / WAVE];
-    const int B = (int)((n + RB - 1) / RB);
-    RcScratch s = rc_scratch(work, n, B);
-    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
+    const RcLoc g = rc_locate(S, work, blockIdx.x);
+    RcScratch s = rc_scratch(g.work, g.n, g.B);
+    lw += g.w0;
+    const int64_t k0 = (int64_t)g.lb * RB + threadIdx.x * RC_PER;
     double m = -INFINITY;
 #pragma unroll
     for (int c = 0; c < RC_PER; ++c)
-        if (k0 + c < n) m = lw[k0 + c] > m ? lw[k0 + c] : m;
+        if (k0 + c < g.n) m = lw[k0 + c] > m ? lw[k0 + c] : m;
     m = rc_block_max(m, sh4);
-    if (threadIdx.x == 0) s.bm[blockIdx.x] = m;
+    if (threadIdx.x == 0) s.bm[g.lb] = m;
 }
 
 // 2: block sums of exp(log_w - m0)
-__global__ __launch_bounds__(RC_T) void k_rc_lse(int64_t n, const double *lw, double *work) {
+__global__ __launch_bounds__(RC_T) void k_rc_lse(RcSegs S, const double *lw, double *work) {
     __shared__ double sh4[RC_T / WAVE];
-    const int B = (int)((n + RB - 1) / RB);
-    RcScratch s = rc_scratch(work, n, B);
-    const double m0 = rc_m0(s, B);
-    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
+    const RcLoc g = rc_locate(S, work, blockIdx.x);
+    RcScratch s = rc_scratch(g.work, g.n, g.B);
+    lw += g.w0;
+    const double m0 = rc_m0(s, g.B);
+    const int64_t k0 = (int64_t)g.lb * RB + threadIdx.x * RC_PER;
     double acc = 0.0;
 #pragma unroll
     for (int c = 0; c < RC_PER; ++c)
-        if (k0 + c < n) acc += exp(lw[k0 + c] - m0);
+        if (k0 + c < g.n) acc += exp(lw[k0 + c] - m0);
     const double incl = rc_block_incl(acc, sh4);
-    if (threadIdx.x == RC_T - 1) s.bs[blockIdx.x] = incl;
+    if (threadIdx.x == RC_T - 1) s.bs[g.lb] = incl;
 }
 
 __device__ __forceinline__ double rc_logG(const RcScratch &s, int B) {
@@ -1170,17 +1255,19 @@ __device__ __forceinline__ double rc_logG(const RcScratch &s, int B) {
 }
 
 // 3: block sums of w and e w, last positive walker per block
-__global__ __launch_bounds__(RC_T) void k_rc_weights(int64_t n, const double *lw,
+__global__ __launch_bounds__(RC_T) void k_rc_weights(RcSegs S, const double *lw,
                                                     const double *el, double *work) {
     __shared__ double sh4[RC_T / WAVE];
-    const int B = (int)((n + RB - 1) / RB);
-    RcScratch s = rc_scratch(work, n, B);
-    const double logG = rc_logG(s, B);
-    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
+    const RcLoc g = rc_locate(S, work, blockIdx.x);
+    RcScratch s = rc_scratch(g.work, g.n, g.B);
+    lw += g.w0;
+    el += g.w0;
+    const double logG = rc_logG(s, g.B);
+    const int64_t k0 = (int64_t)g.lb * RB + threadIdx.x * RC_PER;
     double aw = 0.0, ae = 0.0, lp = -1.0;
 #pragma unroll
     for (int c = 0; c < RC_PER; ++c) {
-        if (k0 + c < n) {
+        if (k0 + c < g.n) {
             const double w = exp(lw[k0 + c] - logG);
             aw += w;
             ae += el[k0 + c] * w;
@@ -1191,30 +1278,31 @@ __global__ __launch_bounds__(RC_T) void k_rc_weights(int64_t n, const double *lw
     const double ie = rc_block_incl(ae, sh4);
     lp = rc_block_max(lp, sh4);
     if (threadIdx.x == RC_T - 1) {
-        s.bw[blockIdx.x] = iw;
-        s.be[blockIdx.x] = ie;
+        s.bw[g.lb] = iw;
+        s.be[g.lb] = ie;
     }
     if (threadIdx.x == 0) {
-        s.blp[blockIdx.x] = lp;
-        if (blockIdx.x == 0) s.logG[0] = logG;
+        s.blp[g.lb] = lp;
+        if (g.lb == 0) s.logG[0] = logG;
     }
 }
 
 // 4: raw prefix sums of the normalised weights, block maxima of them
-__global__ __launch_bounds__(RC_T) void k_rc_cdf(int64_t n, const double *lw, double *work) {
+__global__ __launch_bounds__(RC_T) void k_rc_cdf(RcSegs S, const double *lw, double *work) {
     __shared__ double sh4[RC_T / WAVE];
     __shared__ double sh_incl[RC_T];
-    const int B = (int)((n + RB - 1) / RB);
-    RcScratch s = rc_scratch(work, n, B);
+    const RcLoc g = rc_locate(S, work, blockIdx.x);
+    RcScratch s = rc_scratch(g.work, g.n, g.B);
+    lw += g.w0;
     const double logG = s.logG[0];
     double Pb = 0.0;
-    for (int b = 0; b < (int)blockIdx.x; ++b) Pb = Pb + s.bw[b];
-    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
+    for (int b = 0; b < g.lb; ++b) Pb = Pb + s.bw[b];
+    const int64_t k0 = (int64_t)g.lb * RB + threadIdx.x * RC_PER;
     double w[RC_PER], aw = 0.0;
 #pragma unroll
     for (int c = 0; c < RC_PER; ++c) {
-        w[c] = (k0 + c < n) ? exp(lw[k0 + c] - logG) : 0.0;
-        if (k0 + c < n) aw += w[c];
+        w[c] = (k0 + c < g.n) ? exp(lw[k0 + c] - logG) : 0.0;
+        if (k0 + c < g.n) aw += w[c];
     }
     const double incl = rc_block_incl(aw, sh4);
     sh_incl[threadIdx.x] = incl;
@@ -1224,70 +1312,91 @@ __global__ __launch_bounds__(RC_T) void k_rc_cdf(int64_t n, const double *lw, do
     double last = -INFINITY;
 #pragma unroll
     for (int c = 0; c < RC_PER; ++c) {
-        if (k0 + c < n) {
+        if (k0 + c < g.n) {
             cv = cv + w[c];
             s.cdf[k0 + c] = cv;
             last = cv;
         }
     }
     last = rc_block_max(last, sh4);
-    if (threadIdx.x == 0) s.bml[blockIdx.x] = last;
+    if (threadIdx.x == 0) s.bml[g.lb] = last;
 }
 
 // 5: running max, so the CDF is non-decreasing despite rounding
-__global__ __launch_bounds__(RC_T) void k_rc_runmax(int64_t n, double *work) {
+__global__ __launch_bounds__(RC_T) void k_rc_runmax(RcSegs S, double *work) {
     __shared__ double sh4[RC_T / WAVE];
-    const int B = (int)((n + RB - 1) / RB);
-    RcScratch s = rc_scratch(work, n, B);
+    const RcLoc g = rc_locate(S, work, blockIdx.x);
+    RcScratch s = rc_scratch(g.work, g.n, g.B);
     double Mb = -INFINITY;
-    for (int b = 0; b < (int)blockIdx.x; ++b) Mb = s.bml[b] > Mb ? s.bml[b] : Mb;
-    const int64_t k0 = (int64_t)blockIdx.x * RB + threadIdx.x * RC_PER;
+    for (int b = 0; b < g.lb; ++b) Mb = s.bml[b] > Mb ? s.bml[b] : Mb;
+    const int64_t k0 = (int64_t)g.lb * RB + threadIdx.x * RC_PER;
     double v[RC_PER], last = -INFINITY;
 #pragma unroll
     for (int c = 0; c < RC_PER; ++c) {
-        v[c] = (k0 + c < n) ? s.cdf[k0 + c] : -INFINITY;
-        if (k0 + c < n) last = v[c];
+        v[c] = (k0 + c < g.n) ? s.cdf[k0 + c] : -INFINITY;
+        if (k0 + c < g.n) last = v[c];
     }
     double prev = rc_block_excl_max(last, sh4);
     prev = prev > Mb ? prev : Mb;
 #pragma unroll
     for (int c = 0; c < RC_PER; ++c)
-        if (k0 + c < n && v[c] < prev) s.cdf[k0 + c] = prev;
+        if (k0 + c < g.n && v[c] < prev) s.cdf[k0 + c] = prev;
 }
 
-// 6: parents[i] = min(first k with cdf[k] > (i + psi) / n, last positive)
-__global__ __launch_bounds__(RC_T) void k_rc_parents(int64_t n, const double *work_c,
-                                                     uint32_t key0, uint32_t key1,
+// 6: parents[i] = min(first k with cdf[k] > (i + psi) / n, last positive),
+// thread per walker; segment g draws psi with the key seed + g and writes
+// global indices w[g] + k and out2[2 g .. 2 g + 1]
+__global__ __launch_bounds__(RC_T) void k_rc_parents(RcSegs S, int64_t n_total,
+                                                     const double *work_c, uint64_t seed,
                                                      uint32_t step, uint32_t tag, int use_psi,
                                                      double psi_in, int32_t *parents,
                                                      double *out2) {
-    const int B = (int)((n + RB - 1) / RB);
-    RcScratch s = rc_scratch(const_cast<double *>(work_c), n, B);
+    const int64_t gi = (int64_t)blockIdx.x * RC_T + threadIdx.x;
+    if (gi >= n_total) return;
+    int seg = 0;
+    int64_t w0 = 0, n = S.n;
+    int B, b0 = 0;
+    if (S.nseg == 0) {
+        B = (int)((n + RB - 1) / RB);
+    } else {
+        int lo = 0, hi = S.nseg;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (S.w[mid] <= gi) lo = mid;
+            else hi = mid;
+        }
+        seg = lo;
+        w0 = S.w[lo];
+        n = S.w[lo + 1] - w0;
+        b0 = S.b[lo];
+        B = S.b[lo + 1] - b0;
+    }
+    double *work = const_cast<double *>(work_c) + (S.nseg ? w0 + 7 * (int64_t)b0 + seg : 0);
+    RcScratch s = rc_scratch(work, n, B);
     double psi;
     if (use_psi) {
         psi = psi_in;
     } else {
-        gfmc_u4 r = gfmc_philox(0u, 0u, step, tag, key0, key1);
+        const uint64_t key = seed + (uint64_t)seg;
+        gfmc_u4 r = gfmc_philox(0u, 0u, step, tag, (uint32_t)key, (uint32_t)(key >> 32));
         psi = gfmc_u53(r.x, r.y);
     }
     const double lpd = rc_max_of(s.blp, B);
     const int64_t lastpos = lpd < 0 ? 0 : (int64_t)lpd;
-    const int64_t i = (int64_t)blockIdx.x * RC_T + threadIdx.x;
-    if (i < n) {
-        const double u = ((double)i + psi) / (double)n;
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (s.cdf[mid] > u) hi = mid;
-            else lo = mid + 1;
-        }
-        parents[i] = (int32_t)(lo < lastpos ? lo : lastpos);
+    const int64_t i = gi - w0;
+    const double u = ((double)i + psi) / (double)n;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (s.cdf[mid] > u) hi = mid;
+        else lo = mid + 1;
     }
+    parents[gi] = (int32_t)(w0 + (lo < lastpos ? lo : lastpos));
     if (i == 0) {
         double En = 0.0;
         for (int b = 0; b < B; ++b) En = En + s.be[b];
-        out2[0] = s.logG[0] - log((double)n);
-        out2[1] = En;
+        out2[2 * seg] = s.logG[0] - log((double)n);
+        out2[2 * seg + 1] = En;
     }
 }
 
@@ -1545,6 +1654,20 @@ static unsigned rbm_grid(const gfmc_problem *p, int64_t nw) {
     const int64_t per_cu = (160 * 1024) / (p->lds_bytes > 0 ? p->lds_bytes : 1);
     const int64_t cap = (int64_t)p->n_cu * (per_cu > 0 ? per_cu : 1);
     return (unsigned)(need < cap ? need : cap);
+}
+
+static void launch_reconfigure(const RcSegs &S, int64_t n_total, unsigned blocks,
+                               const double *log_w, const double *e_loc, uint64_t seed,
+                               uint32_t step, uint32_t tag, int use_psi, double psi,
+                               int32_t *parents, double *out2, double *work, hipStream_t st) {
+    hipLaunchKernelGGL(k_rc_max, dim3(blocks), dim3(RC_T), 0, st, S, log_w, work);
+    hipLaunchKernelGGL(k_rc_lse, dim3(blocks), dim3(RC_T), 0, st, S, log_w, work);
+    hipLaunchKernelGGL(k_rc_weights, dim3(blocks), dim3(RC_T), 0, st, S, log_w, e_loc, work);
+    hipLaunchKernelGGL(k_rc_cdf, dim3(blocks), dim3(RC_T), 0, st, S, log_w, work);
+    hipLaunchKernelGGL(k_rc_runmax, dim3(blocks), dim3(RC_T), 0, st, S, work);
+    hipLaunchKernelGGL(k_rc_parents, dim3((unsigned)((n_total + RC_T - 1) / RC_T)), dim3(RC_T),
+                       0, st, S, n_total, (const double *)work, seed, step, tag, use_psi, psi,
+                       parents, out2);
 }
 
 static void free_problem(gfmc_problem *p) {
@@ -1930,16 +2053,13 @@ int gfmc_reconfigure(int64_t nw, const double *log_w, const double *e_loc, uint6
     double psi = psi_opt ? *psi_opt : 0.0;
     if (psi_opt && !(psi >= 0.0 && psi < 1.0))
         return fail(GFMC_EINVAL, "gfmc_reconfigure: psi must lie in [0, 1)");
-    const unsigned B = (unsigned)((nw + RB - 1) / RB);
-    const hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_rc_max, dim3(B), dim3(RC_T), 0, st, nw, log_w, work);
-    hipLaunchKernelGGL(k_rc_lse, dim3(B), dim3(RC_T), 0, st, nw, log_w, work);
-    hipLaunchKernelGGL(k_rc_weights, dim3(B), dim3(RC_T), 0, st, nw, log_w, e_loc, work);
-    hipLaunchKernelGGL(k_rc_cdf, dim3(B), dim3(RC_T), 0, st, nw, log_w, work);
-    hipLaunchKernelGGL(k_rc_runmax, dim3(B), dim3(RC_T), 0, st, nw, work);
-    hipLaunchKernelGGL(k_rc_parents, dim3(grid1d(nw, RC_T)), dim3(RC_T), 0, st, nw,
-                       (const double *)work, (uint32_t)seed, (uint32_t)(seed >> 32), step, tag,
-                       psi_opt ? 1 : 0, psi, parents, out2);
+    RcSegs S;
+    S.nseg = 0;
+    S.n = nw;
+    S.w = nullptr;
+    S.b = nullptr;
+    launch_reconfigure(S, nw, (unsigned)((nw + RB - 1) / RB), log_w, e_loc, seed, step, tag,
+                       psi_opt ? 1 : 0, psi, parents, out2, work, (hipStream_t)stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -2048,3 +2168,4 @@ int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32
 }  // extern "C"
 
 #include "gfmc_train.inl"
+#include "gfmc_batch.inl"

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["SpinHalf", "edges_array"]
+__all__ = ["SpinHalf", "edges_array", "transverse_field_ising", "tfim_chain_ground_energy"]
 
 
 def edges_array(edges):
@@ -240,3 +240,20 @@ class SpinHalf:
         """(M_total, N) successor configurations of one configuration."""
         c = np.asarray(config, dtype=float)
         return np.stack([self.generate_state(c, k) for k in range(self.M_total)])
+
+
+def transverse_field_ising(edges, J=1.0, h=1.0, N=None, *, dtype=np.float64):
+    """H = -J sum_<ij> sigma^z_i sigma^z_j - h sum_i sigma^x_i in Pauli
+    matrices, as the SpinHalf operator with J_z = -4 J and h_x = 2 h
+    (sigma = 2 S); sign-free for h >= 0."""
+    return SpinHalf.create(edges, J_x=0.0, J_y=0.0, J_z=-4.0 * np.asarray(J, dtype=float),
+                           h_x=2.0 * np.asarray(h, dtype=float), N=N, dtype=dtype)
+
+
+def tfim_chain_ground_energy(L, J=1.0, h=1.0):
+    """Ground energy of the periodic transverse-field Ising chain of L sites
+    (Pauli convention above) from its free-fermion solution (Pfeuty, Ann.
+    Phys. 57, 79 (1970)): E_0 = -sum_k sqrt(J^2 + h^2 - 2 J h cos k) over the
+    antiperiodic momenta k = (2n + 1) pi / L, n = 0..L-1."""
+    k = (2.0 * np.arange(int(L)) + 1.0) * np.pi / int(L)
+    return float(-np.sum(np.sqrt(J * J + h * h - 2.0 * J * h * np.cos(k))))

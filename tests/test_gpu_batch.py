@@ -1,0 +1,135 @@
+"""Batches of independent lattices of uneven size: one launch per kernel for
+the whole batch, each lattice bit-identical to its single-lattice run with
+the seed seed + l, segmented reconfiguration against the oracle, and the
+free-fermion energies of transverse-field Ising chains."""
+import sys
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+from pygfmc_amd import (JastrowSpinHalf, SpinHalf, chain, evaluate_observables,
+                        prepare_walkers_random, prepare_walkers_sz, run_batch,
+                        tfim_chain_ground_energy, transverse_field_ising)
+from pygfmc_amd import _lib
+from pygfmc_amd.batch import Batch, BatchChain
+from pygfmc_amd.engine import Chain
+from pygfmc_amd.problem import Problem
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def tfim_lattices(Ls, hs, dtype):
+    Hs, Gs = [], []
+    for L, h in zip(Ls, hs):
+        e = chain(L).edges
+        Hs.append(transverse_field_ising(e, 1.0, h, dtype=dtype))
+        Gs.append(JastrowSpinHalf(L, e, beta=-0.1, dtype=dtype))
+    return Hs, Gs
+
+
+def xxz_lattices(Ls, dtype):
+    Hs, Gs = [], []
+    for L in Ls:
+        e = chain(L).edges
+        Hs.append(SpinHalf.create(e, -1.0, -1.0, 0.8, dtype=dtype))
+        Gs.append(JastrowSpinHalf(L, e, beta=0.2, dtype=dtype))
+    return Hs, Gs
+
+
+CASES = [
+    ("tfim_fp64", "tfim", [16, 23, 40, 70, 130], [64, 100, 256, 37, 1100], np.float64),
+    ("tfim_fp32", "tfim", [16, 23, 40, 70, 130], [64, 100, 256, 37, 1100], np.float32),
+    ("xxz_fp64", "xxz", [10, 36, 64, 100], [200, 1030, 77, 512], np.float64),
+]
+
+
+def build(kind, Ls, dtype):
+    if kind == "tfim":
+        return tfim_lattices(Ls, np.linspace(0.6, 1.45, len(Ls)), dtype)
+    return xxz_lattices(Ls, dtype)
+
+
+def populations(kind, Ls, nws):
+    if kind == "tfim":
+        return [prepare_walkers_random(nw, L, 10 + l) for l, (L, nw) in enumerate(zip(Ls, nws))]
+    return [prepare_walkers_sz(nw, L, 10 + l) for l, (L, nw) in enumerate(zip(Ls, nws))]
+
+
+@pytest.mark.parametrize("name,kind,Ls,nws,dtype", CASES, ids=[c[0] for c in CASES])
+def test_batch_steps_equal_single_lattice_steps(name, kind, Ls, nws, dtype):
+    say("batch vs single", name)
+    Hs, Gs = build(kind, Ls, dtype)
+    walkers = populations(kind, Ls, nws)
+    b = Batch(Hs, Gs, nws)
+    ch = BatchChain(b, walkers)
+    alw_h = np.array([0.5 + 0.3 * l for l in range(len(Ls))])
+    alw = torch.as_tensor(alw_h, device="cuda")
+    seed, step0, n = 99, 5, 6
+    series = ch.steps(n, 0.2, alw, seed, step0).cpu().numpy()
+    for l in range(len(Ls)):
+        p = Problem(Hs[l], Gs[l], b.problems[l].precision)
+        c = Chain(p, walkers[l])
+        s1 = c.steps(n, 0.2, alw_h[l], seed + l, step0).cpu().numpy()
+        wb = ch.walkers(l)
+        np.testing.assert_array_equal(wb.spins.cpu().numpy(), c.spins.cpu().numpy())
+        np.testing.assert_array_equal(wb.e_loc, c.e_loc.cpu().numpy())
+        np.testing.assert_array_equal(wb.log_weight, c.log_w.cpu().numpy())
+        np.testing.assert_array_equal(series[:, l], s1)
+        # padding words stay zero
+        a, z = b.offsets[l], b.offsets[l + 1]
+        assert not ch.spins[a:z, p.n_words:].any()
+
+
+def test_segmented_reconfiguration_matches_oracle(orc):
+    say("segmented reconfiguration")
+    Hs, Gs = tfim_lattices([16, 20, 30, 24], [0.7, 0.8, 1.1, 1.3], np.float64)
+    nws = [1, 1024, 1025, 3000]
+    b = Batch(Hs, Gs, nws)
+    rng = np.random.default_rng(3)
+    lw = rng.normal(scale=2.0, size=b.n_total)
+    lw[5] = -np.inf
+    el = rng.normal(size=b.n_total)
+    lw_d = torch.as_tensor(lw, device="cuda")
+    el_d = torch.as_tensor(el, device="cuda")
+    par = torch.empty(b.n_total, dtype=torch.int32, device="cuda")
+    out2 = torch.empty((len(nws), 2), dtype=torch.float64, device="cuda")
+    work = torch.empty(b.work_doubles, dtype=torch.float64, device="cuda")
+    seed, step = 1234, 77
+    _lib.check(_lib.load().gfmc_batch_reconfigure(
+        b.handle, _lib.ptr(lw_d), _lib.ptr(el_d), seed, step, _lib.TAG_RECONFIG,
+        _lib.ptr(par), _lib.ptr(out2), _lib.ptr(work), _lib.stream()), "gfmc_batch_reconfigure")
+    par = par.cpu().numpy()
+    out2 = out2.cpu().numpy()
+    for l in range(len(nws)):
+        a, z = b.offsets[l], b.offsets[l + 1]
+        psi = orc.reconfig_psi(seed + l, step, _lib.TAG_RECONFIG)
+        p_ref, lg, En, _ = orc.reconfigure(lw[a:z], el[a:z], psi)
+        np.testing.assert_array_equal(par[a:z] - a, p_ref)
+        assert out2[l, 0] == lg and out2[l, 1] == En
+
+
+def test_batch_tfim_energies_match_free_fermions():
+    """Eight transverse-field Ising chains of uneven length in one batch:
+    every lattice's E(p = 64) at its free-fermion ground energy."""
+    say("batch tfim energies")
+    Ls = [16, 20, 32, 48, 64, 96, 128, 200]
+    hs = np.random.default_rng(0).uniform(0.5, 1.5, len(Ls))
+    Hs, Gs = tfim_lattices(Ls, hs, np.float32)
+    walkers = [prepare_walkers_random(1024, L, l) for l, L in enumerate(Ls)]
+    energies, raws, stats = run_batch(walkers, 1500, Hs, 0.2, Gs, [4, 16, 64], rng_key=7,
+                                      warmup_steps=200)
+    say(f"  {stats['walker_steps_per_sec']:.3g} walker-steps/s")
+    for l, (L, h) in enumerate(zip(Ls, hs)):
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            E, _, err, _ = evaluate_observables(raws[l], n_blocks=20)
+        e0 = tfim_chain_ground_energy(L, 1.0, h)
+        say(f"  L={L} h={h:.3f}: E {E[64]:.5f} +- {err[64]:.5f} exact {e0:.5f}")
+        assert abs(E[64] - e0) < 4 * err[64] + 1e-4 * abs(e0)

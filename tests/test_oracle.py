@@ -397,3 +397,13 @@ def test_oracle_rbm_theta_batch(orc):
     th = orc.Problem(lower(H, G)).rbm_theta(orc.pack(X))
     ref = X @ G.W.astype(np.float64).T + G.b.astype(np.float64)
     np.testing.assert_allclose(th, ref, rtol=0, atol=1e-12)
+
+
+# -- transverse-field Ising chains: free fermions against ED -------------------------
+
+@pytest.mark.parametrize("L,h", [(6, 0.5), (8, 1.0), (10, 1.37)])
+def test_tfim_free_fermion_energy_matches_ed(L, h):
+    from helpers import ground_energy
+    from pygfmc_amd import chain, tfim_chain_ground_energy, transverse_field_ising
+    H = transverse_field_ising(chain(L).edges, 1.0, h)
+    assert tfim_chain_ground_energy(L, 1.0, h) == pytest.approx(ground_energy(H), rel=1e-12)
