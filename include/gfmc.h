@@ -260,6 +260,32 @@ int gfmc_batch_run_steps(const gfmc_batch *b, uint32_t *spins, double *log_w, do
                          uint32_t step0, uint32_t tag_prop, uint32_t tag_reconf,
                          double *series, int32_t *evals, void *stream);
 
+/* ---- Step loops captured into HIP graphs ----------------------------------
+ * The production loop of run_gfmc (the chunked lax.scan of _production,
+ * driver.py:940-1143) as one HIP graph of n_steps x (propagate, six
+ * reconfiguration kernels, gather) launched per chunk.  Every captured
+ * kernel takes its Philox position as step + *counter; the graph's last
+ * node adds n_steps to the device counter, so consecutive launches continue
+ * the chain exactly as eager calls with step0 = 0, n, 2n, ... plus the
+ * counter's initial value.  Buffers, beta, shifts, seed and tags are fixed
+ * at capture; series receives the n_steps rows of each launch. */
+typedef struct gfmc_graph gfmc_graph;
+
+int gfmc_graph_capture_steps(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
+                             double *log_w, double *e_loc, uint32_t *spins_b,
+                             double *e_loc_b, int32_t *parents, double *work,
+                             int32_t n_steps, double beta, double alw, uint64_t seed,
+                             uint32_t tag_prop, uint32_t tag_reconf, double *series,
+                             int32_t *evals, uint32_t *counter, gfmc_graph **out);
+int gfmc_graph_capture_batch_steps(const gfmc_batch *b, uint32_t *spins, double *log_w,
+                                   double *e_loc, uint32_t *spins_b, double *e_loc_b,
+                                   int32_t *parents, double *work, int32_t n_steps,
+                                   double beta, const double *alw, uint64_t seed,
+                                   uint32_t tag_prop, uint32_t tag_reconf, double *series,
+                                   int32_t *evals, uint32_t *counter, gfmc_graph **out);
+int gfmc_graph_launch(gfmc_graph *g, void *stream);
+int gfmc_graph_destroy(gfmc_graph *g);
+
 /* ---- RBM guide training ------------------------------------------------
  * No counterpart in the reference, which trains guides by NetKet VMC energy
  * minimisation (train_with_netket, generic_netket_training.py:331-473):

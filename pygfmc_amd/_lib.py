@@ -106,6 +106,18 @@ SIGNATURES = [
                                             ctypes.c_int32, ctypes.c_double, _vp,
                                             ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                             ctypes.c_uint32, _vp, _vp, _vp]),
+    ("gfmc_graph_capture_steps", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp,
+                                                _vp, _vp, ctypes.c_int32, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_uint64,
+                                                ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
+                                                ctypes.POINTER(_vp)]),
+    ("gfmc_graph_capture_batch_steps", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                      ctypes.c_int32, ctypes.c_double, _vp,
+                                                      ctypes.c_uint64, ctypes.c_uint32,
+                                                      ctypes.c_uint32, _vp, _vp, _vp,
+                                                      ctypes.POINTER(_vp)]),
+    ("gfmc_graph_launch", ctypes.c_int, [_vp, _vp]),
+    ("gfmc_graph_destroy", ctypes.c_int, [_vp]),
     ("gfmc_rbm_n_params", ctypes.c_int64, [_vp]),
     ("gfmc_rbm_set_params", ctypes.c_int, [_vp, _vp, _vp]),
     ("gfmc_rbm_theta", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),

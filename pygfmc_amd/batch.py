@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import require_method, require_step_size, warmup_schedule
+from .engine import StepGraph, require_method, require_step_size, warmup_schedule
 from .estimators import check_population, estimate_energy, validated_int, validated_p_array
 from .problem import Problem, precision_of
 from .walkers import Walker, as_key
@@ -117,7 +117,7 @@ def _lse(v):
 
 def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *,
               method="cont_time", rng_key=0, warmup_steps=1000, precision=None, verbosity=0,
-              number_of_outputs=10):
+              number_of_outputs=10, use_graph=True):
     """GFMC on every lattice of the batch.  Returns ``(energies, raws,
     stats)``: ``energies[l]`` {p: E_p}, ``raws[l]`` with ``log_weights`` and
     ``energy`` series, ``stats`` with the batch throughput."""
@@ -155,10 +155,16 @@ def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *
     torch.cuda.synchronize()
     t0 = time.time()
     s0 = 0
+    graph = None
     while s0 < n_steps:
         n = min(chunk, n_steps - s0)
-        chain.steps(n, step_f, alw_dev, key.seed, key.position + s0, series=series[s0:s0 + n],
-                    evals=True)
+        if use_graph and n == chunk and n_steps >= 2 * chunk:
+            if graph is None:
+                graph = StepGraph(chain, n, step_f, alw_dev, key.seed, evals=True)
+            graph.launch(key.position + s0, out=series[s0:s0 + n])
+        else:
+            chain.steps(n, step_f, alw_dev, key.seed, key.position + s0,
+                        series=series[s0:s0 + n], evals=True)
         s0 += n
     torch.cuda.synchronize()
     runtime = time.time() - t0
@@ -174,7 +180,8 @@ def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *
     stats = dict(runtime=runtime, it_per_sec=n_steps / runtime,
                  walker_steps_per_sec=b.n_total * n_steps / runtime,
                  evals_per_step_mean=float(evals.mean().item()) / n_steps,
-                 avg_log_weight=alw.copy(), n_walkers_total=b.n_total, n_lattices=b.n_lat)
+                 avg_log_weight=alw.copy(), n_walkers_total=b.n_total, n_lattices=b.n_lat,
+                 hip_graph=graph is not None)
     if verbosity:
         print(f"[gfmc] - [run_batch] - {b.n_lat} lattices, {n_steps} steps @ "
               f"{stats['it_per_sec']:.1f} it/s", flush=True)

@@ -56,12 +56,12 @@ static void launch_propagate_batch(const gfmc_batch *b, const Tables<T> *Ts, uin
         hipLaunchKernelGGL((k_propagate_batch<T, true>), dim3((unsigned)b->n_total), dim3(WAVE),
                            (size_t)b->lds_bytes, st, b->d_P, Ts, b->n_lat, b->d_woff,
                            b->n_total, b->stride, spins, log_w, e_loc, beta, alw, seed, step,
-                           tag, evals);
+                           tag, evals, g_step_base);
     else
         hipLaunchKernelGGL((k_propagate_batch<T, false>), dim3((unsigned)b->n_total),
                            dim3(WAVE), (size_t)b->lds_bytes, st, b->d_P, Ts, b->n_lat,
                            b->d_woff, b->n_total, b->stride, spins, log_w, e_loc, beta, alw,
-                           seed, step, tag, evals);
+                           seed, step, tag, evals, g_step_base);
 }
 
 extern "C" {

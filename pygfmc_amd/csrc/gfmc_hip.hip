@@ -35,6 +35,9 @@
 // error handling
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
+// device counter added to the step of every launch this thread enqueues
+// while a step loop is captured into a graph (null otherwise)
+static thread_local const uint32_t *g_step_base = nullptr;
 
 static int fail(int code, const std::string &msg) {
     g_err = msg;
@@ -615,13 +618,14 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
                                                     double *e_loc, double beta, double alw,
                                                     uint32_t key0, uint32_t key1,
                                                     uint32_t step, uint32_t tag,
-                                                    int32_t *evals) {
+                                                    int32_t *evals, const uint32_t *step_base) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t wk = blockIdx.x;
     if (wk >= nw) return;
+    const uint32_t st = step + (step_base ? *step_base : 0u);
     propagate_walker<T, FAST>(P, tb, lds, threadIdx.x, (uint32_t)wk, spins + wk * P.words,
                               log_w + wk, e_loc + wk, evals ? evals + wk : nullptr, beta, alw,
-                              key0, key1, step, tag);
+                              key0, key1, st, tag);
 }
 
 // A batch of independent lattices in one launch: walker g belongs to lattice
@@ -632,10 +636,11 @@ template <typename T, bool FAST>
 __global__ __launch_bounds__(WAVE) void k_propagate_batch(
     const DevProblem *Ps, const Tables<T> *Ts, int n_lat, const int64_t *woff, int64_t n_total,
     int stride, uint32_t *spins, double *log_w, double *e_loc, double beta, const double *alw,
-    uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals) {
+    uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals, const uint32_t *step_base) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t g = blockIdx.x;
     if (g >= n_total) return;
+    step += step_base ? *step_base : 0u;
     int lo = 0, hi = n_lat;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -903,7 +908,8 @@ __device__ void rbm_walker_start(const DevProblem &P, const Tables<float> &tb,
 __global__ __launch_bounds__(RBM_WAVES * WAVE) void k_propagate_rbm(
     DevProblem P, Tables<float> tb, int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
     double beta, double alw, uint32_t key0, uint32_t key1, uint32_t step, uint32_t tag,
-    int32_t *evals) {
+    int32_t *evals, const uint32_t *step_base) {
+    step += step_base ? *step_base : 0u;
     extern __shared__ __align__(16) unsigned char lds[];
     uint32_t *Wl = (uint32_t *)lds;
     rbm_stage_W(P, Wl);
@@ -1350,9 +1356,10 @@ __global__ __launch_bounds__(RC_T) void k_rc_parents(RcSegs S, int64_t n_total,
                                                      const double *work_c, uint64_t seed,
                                                      uint32_t step, uint32_t tag, int use_psi,
                                                      double psi_in, int32_t *parents,
-                                                     double *out2) {
+                                                     double *out2, const uint32_t *step_base) {
     const int64_t gi = (int64_t)blockIdx.x * RC_T + threadIdx.x;
     if (gi >= n_total) return;
+    step += step_base ? *step_base : 0u;
     int seg = 0;
     int64_t w0 = 0, n = S.n;
     int B, b0 = 0;
@@ -1641,11 +1648,11 @@ static void launch_propagate(const gfmc_problem *p, const Tables<T> &tb, int64_t
     if (p->dp.fast)
         hipLaunchKernelGGL((k_propagate<T, true>), dim3((unsigned)nw), dim3(WAVE),
                            (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, log_w, e_loc, beta,
-                           alw, k0, k1, step, tag, evals);
+                           alw, k0, k1, step, tag, evals, g_step_base);
     else
         hipLaunchKernelGGL((k_propagate<T, false>), dim3((unsigned)nw), dim3(WAVE),
                            (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, log_w, e_loc, beta,
-                           alw, k0, k1, step, tag, evals);
+                           alw, k0, k1, step, tag, evals, g_step_base);
 }
 
 // persistent RBM grid: as many workgroups as fit (one per CU at full LDS)
@@ -1667,7 +1674,7 @@ static void launch_reconfigure(const RcSegs &S, int64_t n_total, unsigned blocks
     hipLaunchKernelGGL(k_rc_runmax, dim3(blocks), dim3(RC_T), 0, st, S, work);
     hipLaunchKernelGGL(k_rc_parents, dim3((unsigned)((n_total + RC_T - 1) / RC_T)), dim3(RC_T),
                        0, st, S, n_total, (const double *)work, seed, step, tag, use_psi, psi,
-                       parents, out2);
+                       parents, out2, g_step_base);
 }
 
 static void free_problem(gfmc_problem *p) {
@@ -2032,7 +2039,7 @@ int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
     if (p->dp.guide == GFMC_GUIDE_RBM)
         hipLaunchKernelGGL(k_propagate_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
                            (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
-                           log_w, e_loc, beta, alw, k0, k1, step, tag, evals);
+                           log_w, e_loc, beta, alw, k0, k1, step, tag, evals, g_step_base);
     else if (p->precision == GFMC_FP64)
         launch_propagate(p, p->t64, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
                          evals, (hipStream_t)stream);
@@ -2169,3 +2176,4 @@ int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32
 
 #include "gfmc_train.inl"
 #include "gfmc_batch.inl"
+#include "gfmc_graph.inl"

@@ -24,7 +24,7 @@ from .walkers import PhiloxKey, Walker, as_key
 
 __all__ = ["run", "run_gfmc", "evolve_walkers", "reconfiguration", "systematic_parents",
            "sample_index", "estimate_average_log_branch_weight", "DEFAULT_WARMUP_STEPS",
-           "Chain"]
+           "Chain", "StepGraph"]
 
 DEFAULT_WARMUP_STEPS = 1000
 
@@ -122,6 +122,59 @@ class Chain:
     def walkers(self):
         return Walker.from_device(self.spins.clone(), self.log_w.clone(), self.e_loc.clone(),
                                   self.p.n_sites)
+
+
+class StepGraph:
+    """``n`` branching steps of a chain (or batch chain) captured once into
+    a HIP graph (gfmc_graph_capture_steps) and replayed by :meth:`launch`;
+    each launch runs positions ``step0 .. step0 + n - 1`` into ``series``."""
+
+    def __init__(self, chain, n, beta, alw, seed, evals=False, warmup=False):
+        lib = _lib.load()
+        dev = chain.spins.device
+        self.n = int(n)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        batch = hasattr(chain, "b")
+        shape = (self.n, chain.b.n_lat, 2) if batch else (self.n, 2)
+        self.series = torch.empty(shape, dtype=torch.float64, device=dev)
+        tp = _lib.TAG_WARMUP_PROPAGATE if warmup else _lib.TAG_PROPAGATE
+        tr = _lib.TAG_WARMUP_RECONFIG if warmup else _lib.TAG_RECONFIG
+        seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        ev = _lib.ptr(chain.evals) if evals else None
+        h = ctypes.c_void_p()
+        common = (_lib.ptr(chain.spins), _lib.ptr(chain.log_w), _lib.ptr(chain.e_loc),
+                  _lib.ptr(chain.spins_b), _lib.ptr(chain.e_loc_b), _lib.ptr(chain.parents),
+                  _lib.ptr(chain.work), self.n, float(beta))
+        if batch:
+            self._alw = alw
+            rc = lib.gfmc_graph_capture_batch_steps(chain.b.handle, *common, _lib.ptr(alw), seed,
+                                                    tp, tr, _lib.ptr(self.series), ev,
+                                                    _lib.ptr(self.counter), ctypes.byref(h))
+        else:
+            rc = lib.gfmc_graph_capture_steps(chain.p.handle, chain.nw, *common, float(alw),
+                                              seed, tp, tr, _lib.ptr(self.series), ev,
+                                              _lib.ptr(self.counter), ctypes.byref(h))
+        _lib.check(rc, "gfmc_graph_capture")
+        self.handle = h
+        self.chain = chain
+
+    def launch(self, step0, out=None):
+        """Run the ``n`` steps from position ``step0``; ``out`` (optional)
+        receives a copy of the series rows."""
+        v = int(step0) & 0xFFFFFFFF
+        self.counter.fill_(v - 2 ** 32 if v >= 2 ** 31 else v)
+        _lib.check(_lib.load().gfmc_graph_launch(self.handle, _lib.stream()),
+                   "gfmc_graph_launch")
+        if out is not None:
+            out.copy_(self.series)
+        return self.series
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                _lib.load().gfmc_graph_destroy(self.handle)
+        except Exception:
+            pass
 
 
 # -- single-call API -----------------------------------------------------------
@@ -244,7 +297,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
              method="cont_time", rng_key=None, lam=None, diag_operators=None,
              off_diag_operators=None, return_raw=True, avg_log_weight=None,
              warmup_steps=None, verbosity=7, number_of_outputs=100, precision=None,
-             **_unused):
+             use_graph=True, **_unused):
     """Single-GPU GFMC run (run_gfmc, driver.py:1148-1547), energy only.
 
     Returns ``(walkers, energy, obs, stats, raw)`` (or the first four with
@@ -254,7 +307,8 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     with the per-step ``log_weights`` and ``energy`` series.  ``rng_key``:
     an int seed or (seed, position); ``stats['parameters/rng_key_next']``
     continues the chain.  ``precision``: 'fp64' / 'fp32' (default from the
-    Hamiltonian's dtype).
+    Hamiltonian's dtype).  ``use_graph``: production chunks replay one
+    captured HIP graph (identical results to eager launches).
     """
     wall_start = time.time()
     if diag_operators or off_diag_operators:
@@ -297,12 +351,18 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     pending = None
     step0 = 0
     host_chunks = []
+    graph = None
     while step0 < n_steps or pending is not None:
         if step0 < n_steps:
             n = min(print_every, n_steps - step0)
             chain.evals.zero_()
-            chain.steps(n, step_f, alw, key.seed, key.position + step0, series=series[step0:step0 + n],
-                        evals=True)
+            if use_graph and n == print_every and n_steps >= 2 * print_every:
+                if graph is None:
+                    graph = StepGraph(chain, n, step_f, alw, key.seed, evals=True)
+                graph.launch(key.position + step0, out=series[step0:step0 + n])
+            else:
+                chain.steps(n, step_f, alw, key.seed, key.position + step0,
+                            series=series[step0:step0 + n], evals=True)
             ev = chain.evals.to(torch.float64).mean() / n
             nxt = (step0, n, ev)
             step0 += n
@@ -330,6 +390,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         "runtime": runtime,
         "total_wall_time": time.time() - wall_start,
         "number_of_devices": 1,
+        "hip_graph": graph is not None,
         "evals_per_step_mean": evals_sum / max(1, evals_n),
         "parameters/n_steps": n_steps,
         "parameters/n_walkers": walkers.n_walkers,
