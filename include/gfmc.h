@@ -108,6 +108,8 @@ typedef struct gfmc_problem_info {
     int32_t n_sites, n_words, n_off, m_total;
     int32_t precision, guide, block_k, full_recompute;
     int64_t lds_bytes;                          /* per walker-wavefront      */
+    int32_t lanes;                              /* lanes per walker (64/32/16) */
+    int32_t lanes_ok;                           /* bit mask of usable lanes  */
 } gfmc_problem_info;
 
 const char *gfmc_version(void);
@@ -119,6 +121,16 @@ const char *gfmc_last_error(void);
 int gfmc_problem_create(const gfmc_problem_desc *desc, int32_t precision,
                         gfmc_problem **out);
 int gfmc_problem_destroy(gfmc_problem *p);
+
+/* Lanes per walker of gfmc_propagate / gfmc_run_steps (no reference
+ * counterpart; the tuning knob of the propagation kernel): 64 (one walker
+ * per wavefront, the default), 32 or 16 (two or four walkers per wavefront,
+ * record path only, every move's dependent slots within the group).  The
+ * off-diagonal sums then run in blocked order over the group's lanes
+ * (K = ceil(n_off / lanes)), so results are bit-identical to the oracle
+ * run with the same lane count, and differ from another lane count in the
+ * last bits.  gfmc_problem_info.lanes_ok lists the usable values. */
+int gfmc_problem_set_lanes(gfmc_problem *p, int32_t lanes);
 int gfmc_problem_get_info(const gfmc_problem *p, gfmc_problem_info *out);
 
 /* Walker initialisers (walker.py:143-306).  _sz: every walker a Philox-driven
@@ -238,6 +250,11 @@ typedef struct gfmc_batch gfmc_batch;
 int gfmc_batch_create(const gfmc_problem *const *problems, int32_t n_lattices,
                       const int64_t *n_walkers, gfmc_batch **out);
 int gfmc_batch_destroy(gfmc_batch *b);
+/* Lanes per walker for the whole batch (gfmc_problem_set_lanes); 32 and 16
+ * also need every lattice's walker count to be a multiple of 64 / lanes.
+ * gfmc_batch_lanes_ok returns the bit mask of usable values. */
+int gfmc_batch_set_lanes(gfmc_batch *b, int32_t lanes);
+int gfmc_batch_lanes_ok(const gfmc_batch *b);
 /* total walkers, spin stride (words) and reconfiguration scratch (doubles) */
 int gfmc_batch_get_info(const gfmc_batch *b, int64_t *n_total, int32_t *stride,
                         int64_t *work_doubles);

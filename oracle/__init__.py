@@ -57,6 +57,7 @@ def lib():
         L.orc_u53.restype = ctypes.c_double
         L.orc_u53.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.orc_wave_tree.argtypes = [_f64p]
+        L.orc_set_lanes.argtypes = [ctypes.c_int]
         L.orc_blocked_scan.restype = ctypes.c_double
         L.orc_blocked_scan.argtypes = [_f64p, ctypes.c_int, _f64p]
         L.orc_sample_many.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _i32p]
@@ -149,17 +150,23 @@ class Problem:
                                   _p(spins, ctypes.c_uint32), _p(out, ctypes.c_double))
         return out
 
-    def propagate(self, spins, log_w, e_loc, beta, alw, seed, step, tag, n_threads=0):
-        """In place on copies; returns (spins, log_w, e_loc, evals)."""
+    def propagate(self, spins, log_w, e_loc, beta, alw, seed, step, tag, n_threads=0,
+                  lanes=64):
+        """In place on copies; returns (spins, log_w, e_loc, evals).  lanes:
+        lanes per walker of the kernel being mirrored (blocked order)."""
         spins = np.array(spins, dtype=np.uint32, copy=True)
         log_w = np.array(log_w, dtype=np.float64, copy=True)
         e_loc = np.array(e_loc, dtype=np.float64, copy=True)
         nw = spins.shape[0]
         ev = np.zeros(nw, dtype=np.int32)
-        lib().orc_propagate(ctypes.byref(self.desc), nw, _p(spins, ctypes.c_uint32),
-                            _p(log_w, ctypes.c_double), _p(e_loc, ctypes.c_double),
-                            float(beta), float(alw), int(seed), int(step) & 0xFFFFFFFF,
-                            int(tag), _p(ev, ctypes.c_int32), int(n_threads))
+        lib().orc_set_lanes(int(lanes))
+        try:
+            lib().orc_propagate(ctypes.byref(self.desc), nw, _p(spins, ctypes.c_uint32),
+                                _p(log_w, ctypes.c_double), _p(e_loc, ctypes.c_double),
+                                float(beta), float(alw), int(seed), int(step) & 0xFFFFFFFF,
+                                int(tag), _p(ev, ctypes.c_int32), int(n_threads))
+        finally:
+            lib().orc_set_lanes(64)
         return spins, log_w, e_loc, ev
 
 
@@ -251,7 +258,8 @@ def unpack(spins, N):
     return out
 
 
-def run_chain(prob, spins, n_steps, beta, alw, seed, step0, warmup=False, n_threads=0):
+def run_chain(prob, spins, n_steps, beta, alw, seed, step0, warmup=False, n_threads=0,
+              lanes=64):
     """n_steps branching steps (propagate, reconfigure, gather) with the
     same counters as the GPU step loop; returns (spins, e_loc, series)."""
     tp, tr = (3, 4) if warmup else (0, 1)
@@ -261,10 +269,25 @@ def run_chain(prob, spins, n_steps, beta, alw, seed, step0, warmup=False, n_thre
     series = np.empty((n_steps, 2))
     for t in range(n_steps):
         spins, lw, e, _ = prob.propagate(spins, lw, e, beta, alw, seed, step0 + t, tp,
-                                         n_threads=n_threads)
+                                         n_threads=n_threads, lanes=lanes)
         psi = reconfig_psi(seed, step0 + t, tr)
         par, lg, En, _ = reconfigure(lw, e, psi)
         series[t] = (lg, En)
         spins, e = gather(prob.N, spins, e, par)
         lw = np.zeros(nw)
     return spins, e, series
+
+
+class lanes:
+    """Context: the oracle's blocked order over ``G`` lanes per walker."""
+
+    def __init__(self, G):
+        self.G = int(G)
+
+    def __enter__(self):
+        lib().orc_set_lanes(self.G)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_lanes(64)
+        return False

@@ -71,35 +71,48 @@ static void draw2(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t step, uint32
 /* Blocked wave order: lane l owns [l*K, (l+1)*K); lane sums sequential from */
 /* 0.0; wave-tree inclusive scan (wave_tree) over the 64 lane sums.         */
 /* ------------------------------------------------------------------------ */
-static int block_k(int n) { return n > 0 ? (n + WAVE - 1) / WAVE : 1; }
+/* Lanes per walker G of the propagation kernel (64, 32 or 16): the blocked
+ * order runs over G lanes, K = ceil(n / G), and the scan is the first stages
+ * of the wave tree (orc_set_lanes; the rates and reconfiguration use 64). */
+static int g_lanes = WAVE;
+
+void orc_set_lanes(int lanes) { g_lanes = (lanes == 32 || lanes == 16) ? lanes : WAVE; }
+
+static int block_k(int n) { return n > 0 ? (n + g_lanes - 1) / g_lanes : 1; }
 
 /* Inclusive scan of 64 lane values in "wave tree" order: Hillis-Steele
  * inside each 16-lane row (offsets 1, 2, 4, 8), then rows 1 and 3 add the
  * last lane of the row before, then rows 2 and 3 add lane 31 -- the order in
  * which the kernels' DPP scan (row_shr, row_bcast:15, row_bcast:31) adds. */
-static void wave_tree(double *s) {
+static void wave_tree_g(double *s, int G) {
     double t[WAVE];
     for (int d = 1; d < 16; d <<= 1) {
-        memcpy(t, s, sizeof t);
-        for (int l = 0; l < WAVE; ++l)
+        memcpy(t, s, sizeof(double) * (size_t)G);
+        for (int l = 0; l < G; ++l)
             if ((l & 15) >= d) s[l] = t[l] + t[l - d];
     }
-    memcpy(t, s, sizeof t);
-    for (int l = 0; l < WAVE; ++l) {
-        int row = l >> 4;
-        if (row == 1 || row == 3) s[l] = t[l] + t[row * 16 - 1];
+    if (G >= 32) {
+        memcpy(t, s, sizeof(double) * (size_t)G);
+        for (int l = 0; l < G; ++l) {
+            int row = l >> 4;
+            if (row == 1 || row == 3) s[l] = t[l] + t[row * 16 - 1];
+        }
     }
-    memcpy(t, s, sizeof t);
-    for (int l = 32; l < WAVE; ++l) s[l] = t[l] + t[31];
+    if (G == WAVE) {
+        memcpy(t, s, sizeof t);
+        for (int l = 32; l < WAVE; ++l) s[l] = t[l] + t[31];
+    }
 }
+
+static void wave_tree(double *s) { wave_tree_g(s, WAVE); }
 
 void orc_wave_tree(double *s) { wave_tree(s); }
 
 /* lane_incl[64] <- inclusive scan of the lane sums of v[0:n]; returns total. */
 double orc_blocked_scan(const double *v, int n, double *lane_incl) {
-    int K = block_k(n);
+    int K = block_k(n), G = g_lanes;
     double s[WAVE];
-    for (int l = 0; l < WAVE; ++l) {
+    for (int l = 0; l < G; ++l) {
         double acc = 0.0;
         for (int k = 0; k < K; ++k) {
             int idx = l * K + k;
@@ -107,9 +120,9 @@ double orc_blocked_scan(const double *v, int n, double *lane_incl) {
         }
         s[l] = acc;
     }
-    wave_tree(s);
-    if (lane_incl) memcpy(lane_incl, s, sizeof s);
-    return s[WAVE - 1];
+    wave_tree_g(s, G);
+    if (lane_incl) memcpy(lane_incl, s, sizeof(double) * (size_t)G);
+    return s[G - 1];
 }
 
 /* Inverse-CDF draw of a slot of w[0:n] (all >= 0, total > 0) at u in
@@ -121,7 +134,7 @@ double orc_blocked_scan(const double *v, int n, double *lane_incl) {
 int orc_sample_blocked(const double *w, int n, const double *lane_incl, double u) {
     int K = block_k(n);
     int j = 0;
-    while (j < WAVE - 1 && !(lane_incl[j] > u)) ++j;
+    while (j < g_lanes - 1 && !(lane_incl[j] > u)) ++j;
     double c = (j == 0) ? 0.0 : lane_incl[j - 1];
     int lo = j * K, hi = lo + K < n ? lo + K : n;
     for (int k = lo; k < hi; ++k) {

@@ -55,6 +55,7 @@ class ProblemInfo(ctypes.Structure):
         ("precision", ctypes.c_int32), ("guide", ctypes.c_int32),
         ("block_k", ctypes.c_int32), ("full_recompute", ctypes.c_int32),
         ("lds_bytes", ctypes.c_int64),
+        ("lanes", ctypes.c_int32), ("lanes_ok", ctypes.c_int32),
     ]
 
 
@@ -66,6 +67,7 @@ SIGNATURES = [
                                            ctypes.POINTER(_vp)]),
     ("gfmc_problem_destroy", ctypes.c_int, [_vp]),
     ("gfmc_problem_get_info", ctypes.c_int, [_vp, ctypes.POINTER(ProblemInfo)]),
+    ("gfmc_problem_set_lanes", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("gfmc_init_walkers_sz", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                             ctypes.c_uint64, _vp, _vp]),
     ("gfmc_init_walkers_random", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64,
@@ -94,6 +96,8 @@ SIGNATURES = [
     ("gfmc_batch_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32,
                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_vp)]),
     ("gfmc_batch_destroy", ctypes.c_int, [_vp]),
+    ("gfmc_batch_set_lanes", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("gfmc_batch_lanes_ok", ctypes.c_int, [_vp]),
     ("gfmc_batch_get_info", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int32),
                                            ctypes.POINTER(ctypes.c_int64)]),

@@ -50,6 +50,7 @@ class Batch:
                                            ctypes.byref(wd)), "gfmc_batch_get_info")
         self.n_total, self.stride, self.work_doubles = nt.value, st.value, wd.value
         self.offsets = np.concatenate([[0], np.cumsum(self.n_walkers)]).astype(np.int64)
+        self.lanes = 64
 
     def __del__(self):
         try:
@@ -57,6 +58,16 @@ class Batch:
                 _lib.load().gfmc_batch_destroy(self.handle)
         except Exception:
             pass
+
+    @property
+    def lanes_ok(self):
+        m = int(_lib.load().gfmc_batch_lanes_ok(self.handle))
+        return [g for g in (64, 32, 16) if m & g]
+
+    def set_lanes(self, lanes):
+        _lib.check(_lib.load().gfmc_batch_set_lanes(self.handle, int(lanes)),
+                   "gfmc_batch_set_lanes")
+        self.lanes = int(lanes)
 
 
 class BatchChain:
@@ -117,10 +128,11 @@ def _lse(v):
 
 def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *,
               method="cont_time", rng_key=0, warmup_steps=1000, precision=None, verbosity=0,
-              number_of_outputs=10, use_graph=True):
+              number_of_outputs=10, use_graph=True, lanes="auto"):
     """GFMC on every lattice of the batch.  Returns ``(energies, raws,
     stats)``: ``energies[l]`` {p: E_p}, ``raws[l]`` with ``log_weights`` and
-    ``energy`` series, ``stats`` with the batch throughput."""
+    ``energy`` series, ``stats`` with the batch throughput.  ``lanes``: lanes
+    per walker (64, 32, 16) or "auto" (pygfmc_amd.autotune)."""
     require_method(method)
     require_step_size(method, step_size)
     n_steps = validated_int("n_steps", n_steps, 1)
@@ -133,6 +145,13 @@ def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *
     b = Batch(hamiltonians, guides, [w.n_walkers for w in walkers_list], precision)
     chain = BatchChain(b, walkers_list)
     dev = chain.spins.device
+    tune_times = None
+    if lanes == "auto":
+        from .autotune import tune_batch
+        _, tune_times = tune_batch(chain, float(step_size),
+                                   torch.zeros(b.n_lat, dtype=torch.float64, device=dev))
+    else:
+        b.set_lanes(int(lanes))
     alw = np.empty(b.n_lat)
     for l in range(b.n_lat):
         a, z = b.offsets[l], b.offsets[l + 1]
@@ -181,7 +200,8 @@ def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *
                  walker_steps_per_sec=b.n_total * n_steps / runtime,
                  evals_per_step_mean=float(evals.mean().item()) / n_steps,
                  avg_log_weight=alw.copy(), n_walkers_total=b.n_total, n_lattices=b.n_lat,
-                 hip_graph=graph is not None)
+                 hip_graph=graph is not None, lanes_per_walker=b.lanes,
+                 autotune_seconds=tune_times)
     if verbosity:
         print(f"[gfmc] - [run_batch] - {b.n_lat} lattices, {n_steps} steps @ "
               f"{stats['it_per_sec']:.1f} it/s", flush=True)

@@ -11,8 +11,10 @@
 // seed seed + l.
 
 struct gfmc_batch {
-    int n_lat = 0, precision = 0, fast = 0, stride = 0;
+    int n_lat = 0, precision = 0, fast = 0, stride = 0, lanes = WAVE;
     int64_t n_total = 0, lds_bytes = 0, work_doubles = 0;
+    int64_t lds_g[3] = {0, 0, 0};   // per wavefront with 64 / 32 / 16 lanes per walker
+    int g_ok[3] = {1, 0, 0};
     unsigned rc_blocks = 0;
     DevProblem *d_P = nullptr;
     Tables<double> *d_T64 = nullptr;
@@ -52,16 +54,27 @@ static void launch_propagate_batch(const gfmc_batch *b, const Tables<T> *Ts, uin
                                    double *log_w, double *e_loc, double beta, const double *alw,
                                    uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals,
                                    hipStream_t st) {
-    if (b->fast)
-        hipLaunchKernelGGL((k_propagate_batch<T, true>), dim3((unsigned)b->n_total), dim3(WAVE),
+    const unsigned n = (unsigned)b->n_total;
+    if (!b->fast)
+        hipLaunchKernelGGL((k_propagate_batch<T, false, 64>), dim3(n), dim3(WAVE),
                            (size_t)b->lds_bytes, st, b->d_P, Ts, b->n_lat, b->d_woff,
                            b->n_total, b->stride, spins, log_w, e_loc, beta, alw, seed, step,
                            tag, evals, g_step_base);
+    else if (b->lanes == 32)
+        hipLaunchKernelGGL((k_propagate_batch<T, true, 32>), dim3((n + 1) / 2), dim3(WAVE),
+                           (size_t)b->lds_g[1], st, b->d_P, Ts, b->n_lat, b->d_woff, b->n_total,
+                           b->stride, spins, log_w, e_loc, beta, alw, seed, step, tag, evals,
+                           g_step_base);
+    else if (b->lanes == 16)
+        hipLaunchKernelGGL((k_propagate_batch<T, true, 16>), dim3((n + 3) / 4), dim3(WAVE),
+                           (size_t)b->lds_g[2], st, b->d_P, Ts, b->n_lat, b->d_woff, b->n_total,
+                           b->stride, spins, log_w, e_loc, beta, alw, seed, step, tag, evals,
+                           g_step_base);
     else
-        hipLaunchKernelGGL((k_propagate_batch<T, false>), dim3((unsigned)b->n_total),
-                           dim3(WAVE), (size_t)b->lds_bytes, st, b->d_P, Ts, b->n_lat,
-                           b->d_woff, b->n_total, b->stride, spins, log_w, e_loc, beta, alw,
-                           seed, step, tag, evals, g_step_base);
+        hipLaunchKernelGGL((k_propagate_batch<T, true, 64>), dim3(n), dim3(WAVE),
+                           (size_t)b->lds_bytes, st, b->d_P, Ts, b->n_lat, b->d_woff,
+                           b->n_total, b->stride, spins, log_w, e_loc, beta, alw, seed, step,
+                           tag, evals, g_step_base);
 }
 
 extern "C" {
@@ -98,6 +111,21 @@ int gfmc_batch_create(const gfmc_problem *const *ps, int32_t n_lat, const int64_
         stride = std::max(stride, p->dp.words);
         lds = std::max(lds, p->lds_bytes);
     }
+    int64_t lds_g[3] = {lds, 0, 0};
+    int g_ok[3] = {1, fast, fast};
+    for (int gi = 1; gi < 3; ++gi) {
+        const int G = WAVE >> gi, per_wave = WAVE / G;
+        for (int l = 0; l < n_lat; ++l) {
+            const gfmc_problem *p = ps[l];
+            if (!p->dp.g_ok[gi] || n_walkers[l] % per_wave) g_ok[gi] = 0;
+            int64_t b1 = gi == 1 ? (prec == GFMC_FP64 ? walker_lds_bytes<double, 32>(p->dp)
+                                                      : walker_lds_bytes<float, 32>(p->dp))
+                                 : (prec == GFMC_FP64 ? walker_lds_bytes<double, 16>(p->dp)
+                                                      : walker_lds_bytes<float, 16>(p->dp));
+            lds_g[gi] = std::max(lds_g[gi], per_wave * b1);
+        }
+        if (lds_g[gi] > 64 * 1024) g_ok[gi] = 0;
+    }
     woff[n_lat] = total;
     boff[n_lat] = (int32_t)blocks;
     if (total > 0x7fffffffLL || blocks > 0x7fffffffLL)
@@ -112,6 +140,10 @@ int gfmc_batch_create(const gfmc_problem *const *ps, int32_t n_lat, const int64_
     b->work_doubles = work;
     b->rc_blocks = (unsigned)blocks;
     b->woff = woff;
+    for (int gi = 0; gi < 3; ++gi) {
+        b->lds_g[gi] = lds_g[gi];
+        b->g_ok[gi] = g_ok[gi];
+    }
     std::vector<DevProblem> P(n_lat);
     std::vector<Tables<double>> T64(n_lat);
     std::vector<Tables<float>> T32(n_lat);
@@ -144,6 +176,26 @@ int gfmc_batch_get_info(const gfmc_batch *b, int64_t *n_total, int32_t *stride,
     if (n_total) *n_total = b->n_total;
     if (stride) *stride = b->stride;
     if (work_doubles) *work_doubles = b->work_doubles;
+    return 0;
+}
+
+int gfmc_batch_lanes_ok(const gfmc_batch *b) {
+    if (!b) return 0;
+    int m = 0;
+    for (int gi = 0; gi < 3; ++gi)
+        if (b->g_ok[gi]) m |= WAVE >> gi;
+    return m;
+}
+
+int gfmc_batch_set_lanes(gfmc_batch *b, int32_t lanes) {
+    if (!b) return fail(GFMC_EINVAL, "gfmc_batch_set_lanes: NULL batch");
+    if (lanes != 64 && lanes != 32 && lanes != 16)
+        return fail(GFMC_EINVAL, "gfmc_batch_set_lanes: lanes must be 64, 32 or 16");
+    if (!b->g_ok[gidx_of(lanes)])
+        return fail(GFMC_EUNSUP, "gfmc_batch_set_lanes: some lattice cannot run with this many "
+                                 "lanes per walker (record path, moves within the group, "
+                                 "walker counts a multiple of 64 / lanes, LDS)");
+    b->lanes = lanes;
     return 0;
 }
 

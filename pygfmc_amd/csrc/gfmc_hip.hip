@@ -87,6 +87,11 @@ struct DevProblem {
     // record path
     int fast, R, RN, RE, flags;
     const uint4 *slot_rec;   // [K * 64 * 2], in LDS order (record p -> LDS position p)
+    // the same per walker-group size G = 64, 32, 16: K_G = ceil(n_off / G),
+    // records in that group's LDS order, and whether the record path fits G
+    // lanes (every move's records and neighbours within G)
+    const uint4 *slot_rec_g[3];
+    int Kg[3], g_ok[3];
     const uint4 *mv_rec;     // [n_off * R * 2]
     const int *mv_nbr;       // [n_off * RN]
     const int *mv_sites;     // [n_off]  i | (j << 16), j = 0xffff for a site flip
@@ -106,7 +111,10 @@ struct gfmc_problem {
     std::vector<void *> allocs;
     int64_t lds_bytes;
     int n_cu;
+    int lanes = WAVE;   // lanes per walker of the propagation kernel (64, 32, 16)
 };
+
+static int gidx_of(int lanes) { return lanes == 64 ? 0 : (lanes == 32 ? 1 : 2); }
 
 static int block_k_of(int n) { return n > 0 ? (n + WAVE - 1) / WAVE : 1; }
 static int round_up(int n, int m) { return (n + m - 1) / m * m; }
@@ -161,6 +169,57 @@ __device__ __forceinline__ int wave_sum_int(int v) {
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
     return v;
 }
+
+// Walker groups: G = 64, 32 or 16 lanes per walker (64 / G walkers per
+// wavefront).  A group is G consecutive lanes; its scan is the first stages
+// of the wave tree (the 16-lane rows, then row_bcast:15 for G >= 32, then
+// row_bcast:31 for G = 64), so G = 64 is the wave tree itself.
+template <int G, typename T>
+__device__ __forceinline__ T grp_incl_scan(T v) {
+    v += dpp_get<0x111, 0xf>(v);
+    v += dpp_get<0x112, 0xf>(v);
+    v += dpp_get<0x114, 0xf>(v);
+    v += dpp_get<0x118, 0xf>(v);
+    if constexpr (G >= 32) v += dpp_get<0x142, 0xa>(v);
+    if constexpr (G == 64) v += dpp_get<0x143, 0xc>(v);
+    return v;
+}
+
+// value of group lane src (group-uniform src)
+template <int G, typename T>
+__device__ __forceinline__ T grp_bcast(T v, int src) {
+    if constexpr (G == WAVE) return lane_bcast(v, src);
+    else return __shfl(v, src, G);
+}
+
+template <int G>
+__device__ __forceinline__ int grp_sum_int(int v) {
+#pragma unroll
+    for (int d = G / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, G);
+    return v;
+}
+
+// ballot of the group, bit g = group lane g
+template <int G>
+__device__ __forceinline__ unsigned long long grp_ballot(bool pred, int lane) {
+    const unsigned long long b = __ballot(pred);
+    if constexpr (G == WAVE) {
+        (void)lane;
+        return b;
+    } else {
+        return (b >> (lane & ~(G - 1))) & ((1ull << G) - 1ull);
+    }
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// index of G in the per-group tables (64 -> 0, 32 -> 1, 16 -> 2)
+template <int G>
+constexpr int gidx() { return G == 64 ? 0 : (G == 32 ? 1 : 2); }
 
 __device__ __forceinline__ double dev_exp(double v) { return exp(v); }
 __device__ __forceinline__ float dev_exp(float v) { return expf(v); }
@@ -290,31 +349,35 @@ __device__ __forceinline__ T eval_rec(const DevProblem &P, const Tables<T> &tb, 
     return -(bare * dev_exp((T)-2 * pair - (T)2 * hterm));
 }
 
-// Diagonal element (SpinHalf.diagonal_weight, SpinHalf.py:291-314).
+// Diagonal element (SpinHalf.diagonal_weight, SpinHalf.py:291-314); gl is
+// the lane in the walker's group of G lanes (blocked order over G lanes).
+template <int G = WAVE>
 __device__ double diag_value(const DevProblem &P, const signed char *x,
-                             const unsigned char *anti, int n_anti, int lane) {
+                             const unsigned char *anti, int n_anti, int gl) {
     double d = 0.0;
     if (P.has_jz) {
         if (P.uniform_jz) {
             d = (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti);
         } else {
+            const int Ke = (P.n_edges + G - 1) / G;
             double acc = 0.0;
-            for (int k = 0; k < P.Ke; ++k) {
-                int e = lane * P.Ke + k;
+            for (int k = 0; k < Ke; ++k) {
+                int e = gl * Ke + k;
                 if (e < P.n_edges) acc += anti[e] ? -P.jz[e] / 4.0 : P.jz[e] / 4.0;
             }
-            acc = wave_incl_scan(acc, lane);
-            d = lane_bcast(acc, WAVE - 1);
+            acc = grp_incl_scan<G>(acc);
+            d = grp_bcast<G>(acc, G - 1);
         }
     }
     if (P.has_hz) {
+        const int Ks = (P.N + G - 1) / G;
         double acc = 0.0;
-        for (int k = 0; k < P.Ks; ++k) {
-            int s = lane * P.Ks + k;
+        for (int k = 0; k < Ks; ++k) {
+            int s = gl * Ks + k;
             if (s < P.N) acc += P.hz[s] * (double)x[s];
         }
-        acc = wave_incl_scan(acc, lane);
-        d = d + lane_bcast(acc, WAVE - 1) / 2;
+        acc = grp_incl_scan<G>(acc);
+        d = d + grp_bcast<G>(acc, G - 1) / 2;
     }
     return d;
 }
@@ -338,16 +401,37 @@ __device__ __forceinline__ WalkerLds<T> carve(const DevProblem &P, unsigned char
     return L;
 }
 
-__device__ __forceinline__ int wpos(int s, int K) { return (s % K) * WAVE + s / K; }
-
-__device__ __forceinline__ void load_spins(const DevProblem &P, const uint32_t *sp,
-                                           signed char *x, int lane) {
-    for (int k = lane; k < P.N; k += WAVE) x[k] = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1 : -1;
+// LDS bytes of one walker in a group of G lanes, and its carve
+template <typename T, int G>
+__host__ __device__ inline int walker_lds_bytes(const DevProblem &P) {
+    const int K = (P.n_off > 0 ? P.n_off + G - 1 : G) / G;
+    return (int)(((size_t)K * G * sizeof(T) + P.n_pad + P.e_pad + 15) / 16 * 16);
 }
 
+template <typename T, int G>
+__device__ __forceinline__ WalkerLds<T> carve_g(const DevProblem &P, unsigned char *lds) {
+    WalkerLds<T> L;
+    L.w = (T *)lds;
+    L.x = (signed char *)(lds + (size_t)P.Kg[gidx<G>()] * G * sizeof(T));
+    L.anti = (unsigned char *)(L.x + P.n_pad);
+    return L;
+}
+
+__device__ __forceinline__ int wpos(int s, int K) { return (s % K) * WAVE + s / K; }
+
+template <int G>
+__device__ __forceinline__ int wpos_g(int s, int K) { return (s % K) * G + s / K; }
+
+template <int G = WAVE>
+__device__ __forceinline__ void load_spins(const DevProblem &P, const uint32_t *sp,
+                                           signed char *x, int gl) {
+    for (int k = gl; k < P.N; k += G) x[k] = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1 : -1;
+}
+
+template <int G = WAVE>
 __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
-                                            const signed char *x, int lane) {
-    for (int q = lane; q < P.words; q += WAVE) {
+                                            const signed char *x, int gl) {
+    for (int q = gl; q < P.words; q += G) {
         uint32_t v = 0;
         for (int b = 0; b < 32; ++b) {
             int k = q * 32 + b;
@@ -357,99 +441,105 @@ __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
     }
 }
 
-// every slot weight, every edge state; returns the anti-parallel count
-template <typename T, bool FAST>
-__device__ int full_eval(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L, int lane) {
+// every slot weight, every edge state; returns the anti-parallel count.
+// gl: lane in the walker's group of G lanes (the generic path: G = 64).
+template <typename T, bool FAST, int G = WAVE>
+__device__ int full_eval(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L, int gl) {
     if constexpr (FAST) {
-        // slot_rec is stored in LDS order (record p describes the slot kept at
-        // LDS position p, slot -1 = padding): coalesced reads, conflict-free
-        // writes, four records in flight per lane
-        for (int p0 = 0; p0 < P.K * WAVE; p0 += 4 * WAVE) {
+        // slot records are stored in LDS order (record p describes the slot
+        // kept at LDS position p, slot -1 = padding): coalesced reads,
+        // conflict-free writes, four records in flight per lane
+        const int n = P.Kg[gidx<G>()] * G;
+        const uint4 *recs = P.slot_rec_g[gidx<G>()];
+        for (int p0 = 0; p0 < n; p0 += 4 * G) {
             Rec r[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int p = p0 + q * WAVE + lane;
-                r[q] = p < P.K * WAVE ? load_rec(P.slot_rec, p) : empty_rec();
+                const int p = p0 + q * G + gl;
+                r[q] = p < n ? load_rec(recs, p) : empty_rec();
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int p = p0 + q * WAVE + lane;
-                if (p < P.K * WAVE)
-                    L.w[p] = rec_f(r[q], 0) >= 0 ? eval_rec(P, tb, r[q], L.x) : (T)0;
+                const int p = p0 + q * G + gl;
+                if (p < n) L.w[p] = rec_f(r[q], 0) >= 0 ? eval_rec(P, tb, r[q], L.x) : (T)0;
             }
         }
     } else {
+        static_assert(FAST || G == WAVE, "the generic path runs one walker per wavefront");
         for (int k = 0; k < P.K; ++k) {
-            int s = lane * P.K + k;
-            L.w[k * WAVE + lane] = s < P.n_off ? slot_weight(P, tb, L.x, s) : (T)0;
+            int s = gl * P.K + k;
+            L.w[k * WAVE + gl] = s < P.n_off ? slot_weight(P, tb, L.x, s) : (T)0;
         }
     }
     int cnt = 0;
     if (P.has_jz) {
         const bool flags = P.flags;
-        for (int e = lane; e < P.n_edges; e += WAVE) {
+        for (int e = gl; e < P.n_edges; e += G) {
             const int pk = P.edge_pack[e];
             unsigned char a = L.x[pk & 0xffff] != L.x[(pk >> 16) & 0xffff];
             if (flags) L.anti[e] = a;
             cnt += a;
         }
     }
-    return wave_sum_int(cnt);
+    return grp_sum_int<G>(cnt);
 }
 
-// lane sums (sequential) + Hillis-Steele: returns this lane's inclusive sum
-template <typename T>
-__device__ __forceinline__ T lane_scan(const DevProblem &P, const WalkerLds<T> &L, int lane,
+// lane sums (sequential) + group scan: returns this lane's inclusive sum
+template <typename T, int G = WAVE>
+__device__ __forceinline__ T lane_scan(const DevProblem &P, const WalkerLds<T> &L, int gl,
                                        T *lane_sum) {
+    const int K = G == WAVE ? P.K : P.Kg[gidx<G>()];
     T acc = (T)0;
-    for (int k = 0; k < P.K; ++k) acc += L.w[k * WAVE + lane];
+    for (int k = 0; k < K; ++k) acc += L.w[k * G + gl];
     *lane_sum = acc;
-    return wave_incl_scan(acc, lane);
+    return grp_incl_scan<G>(acc);
 }
 
-// Inverse-CDF draw (sample_index, evolution.py:30-59) in blocked wave order;
-// u in [0, R).  Mirrors orc_sample_blocked.
-template <typename T>
+// Inverse-CDF draw (sample_index, evolution.py:30-59) in blocked order over
+// the group's G lanes; u in [0, R).  Mirrors orc_sample_blocked.
+template <typename T, int G = WAVE>
 __device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T lane_sum, T u,
                            int lane) {
-    unsigned long long hit = __ballot(incl > u);
-    int j = hit ? __ffsll((unsigned long long)hit) - 1 : WAVE - 1;
-    T prev = lane_bcast(incl, j > 0 ? j - 1 : 0);
+    const int gl = lane & (G - 1);
+    const int K = G == WAVE ? P.K : P.Kg[gidx<G>()];
+    unsigned long long hit = grp_ballot<G>(incl > u, lane);
+    int j = hit ? __ffsll((unsigned long long)hit) - 1 : G - 1;
+    T prev = grp_bcast<G>(incl, j > 0 ? j - 1 : 0);
     if (j == 0) prev = (T)0;
     int idx = -1;
-    if (lane == j) {
+    if (gl == j) {
         // sequential running sum of lane j's block, eight LDS reads in flight
         T c = prev;
-        for (int k0 = 0; k0 < P.K && idx < 0; k0 += 8) {
+        for (int k0 = 0; k0 < K && idx < 0; k0 += 8) {
             T v[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = (k0 + q < P.K) ? L.w[(k0 + q) * WAVE + j] : (T)0;
+            for (int q = 0; q < 8; ++q) v[q] = (k0 + q < K) ? L.w[(k0 + q) * G + j] : (T)0;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int s = j * P.K + k0 + q;
-                if (idx < 0 && k0 + q < P.K && s < P.n_off) {
+                const int s = j * K + k0 + q;
+                if (idx < 0 && k0 + q < K && s < P.n_off) {
                     c += v[q];
                     if (c > u) idx = s;
                 }
             }
         }
     }
-    idx = lane_bcast(idx, j);
+    idx = grp_bcast<G>(idx, j);
     if (idx < 0) {
         // rounding left u past lane j: the last positive slot at or before it
-        unsigned long long pos = __ballot(lane_sum > (T)0 && lane <= j);
+        unsigned long long pos = grp_ballot<G>(lane_sum > (T)0 && gl <= j, lane);
         int jj = pos ? 63 - __clzll((long long)pos) : 0;
         int found = 0;
-        if (lane == jj) {
-            for (int k = P.K - 1; k >= 0; --k) {
-                int s = jj * P.K + k;
-                if (s < P.n_off && L.w[k * WAVE + jj] > (T)0) {
+        if (gl == jj) {
+            for (int k = K - 1; k >= 0; --k) {
+                int s = jj * K + k;
+                if (s < P.n_off && L.w[k * G + jj] > (T)0) {
                     found = s;
                     break;
                 }
             }
         }
-        idx = lane_bcast(found, jj);
+        idx = grp_bcast<G>(found, jj);
     }
     return idx;
 }
@@ -462,19 +552,20 @@ struct MovePrefetch {
     int sites;
 };
 
-__device__ __forceinline__ MovePrefetch prefetch_move(const DevProblem &P, int idx, int lane) {
+__device__ __forceinline__ MovePrefetch prefetch_move(const DevProblem &P, int idx, int gl) {
     MovePrefetch m;
-    m.rec = lane < P.R ? load_rec(P.mv_rec, (size_t)idx * P.R + lane) : empty_rec();
-    m.nbr = (P.has_jz && P.uniform_jz && lane < P.RN) ? P.mv_nbr[(size_t)idx * P.RN + lane] : -1;
+    m.rec = gl < P.R ? load_rec(P.mv_rec, (size_t)idx * P.R + gl) : empty_rec();
+    m.nbr = (P.has_jz && P.uniform_jz && gl < P.RN) ? P.mv_nbr[(size_t)idx * P.RN + gl] : -1;
     m.sites = P.mv_sites[idx];
     return m;
 }
 
 // record path of apply_move: count the edges that turn (anti-)parallel,
 // flip, re-evaluate the prefetched records
-template <typename T>
+template <typename T, int G = WAVE>
 __device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L,
                               const MovePrefetch &m, int idx, int n_anti, int lane) {
+    const int gl = lane & (G - 1);
     if (P.has_jz && P.uniform_jz) {
         bool plus = false, minus = false;
         if (m.nbr >= 0) {
@@ -482,19 +573,19 @@ __device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLd
             plus = L.x[f] == L.x[p];
             minus = !plus;
         }
-        n_anti += __popcll(__ballot(plus)) - __popcll(__ballot(minus));
+        n_anti += __popcll(grp_ballot<G>(plus, lane)) - __popcll(grp_ballot<G>(minus, lane));
     }
-    __syncthreads();
-    if (lane == 0) {
+    wave_sync();
+    if (gl == 0) {
         const int a = m.sites & 0xffff, b = (m.sites >> 16) & 0xffff;
         L.x[a] = (signed char)-L.x[a];
         if (b != 0xffff) L.x[b] = (signed char)-L.x[b];
     }
-    __syncthreads();
+    wave_sync();
     const int s = rec_f(m.rec, 0);
-    if (s >= 0) L.w[wpos(s, P.K)] = eval_rec(P, tb, m.rec, L.x);
+    if (s >= 0) L.w[wpos_g<G>(s, G == WAVE ? P.K : P.Kg[gidx<G>()])] = eval_rec(P, tb, m.rec, L.x);
     if (P.has_jz && !P.uniform_jz) {
-        for (int t = lane; t < P.RE; t += WAVE) {
+        for (int t = gl; t < P.RE; t += G) {
             const int e = P.mv_edges[(size_t)idx * P.RE + t];
             if (e >= 0) {
                 const int pk = P.edge_pack[e];
@@ -502,7 +593,7 @@ __device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLd
             }
         }
     }
-    __syncthreads();
+    wave_sync();
     return n_anti;
 }
 
@@ -550,29 +641,32 @@ __device__ int apply_move(const DevProblem &P, const Tables<T> &tb, WalkerLds<T>
 // ---------------------------------------------------------------------------
 // propagation kernel (evolve_walker_cont_time, evolution.py:559-721)
 // ---------------------------------------------------------------------------
-// one walker's branching step by the workgroup's wavefront: spins at sp,
-// log_w / e_loc / evals at lw_p / el_p / ev_p; wk is the walker index in the
-// Philox counter (hop, wk, step, tag)
-template <typename T, bool FAST>
+// One walker's branching step by a group of G lanes (G = 64: the whole
+// wavefront; the generic path only runs with G = 64): spins at sp, log_w /
+// e_loc / evals at lw_p / el_p / ev_p, LDS state at lds; wk is the walker
+// index in the Philox counter (hop, wk, step, tag).  Groups of one
+// wavefront run independently (no workgroup barriers on this path).
+template <typename T, bool FAST, int G>
 __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tables<T> &tb,
                                                  unsigned char *lds, int lane, uint32_t wk,
                                                  uint32_t *sp, double *lw_p, double *el_p,
                                                  int32_t *ev_p, double beta, double alw,
                                                  uint32_t key0, uint32_t key1, uint32_t step,
                                                  uint32_t tag) {
-    WalkerLds<T> L = carve<T>(P, lds);
-    load_spins(P, sp, L.x, lane);
-    __syncthreads();
-    int n_anti = full_eval<T, FAST>(P, tb, L, lane);
-    __syncthreads();
-    double diag = diag_value(P, L.x, L.anti, n_anti, lane);
+    const int gl = lane & (G - 1);
+    WalkerLds<T> L = FAST ? carve_g<T, G>(P, lds) : carve<T>(P, lds);
+    load_spins<G>(P, sp, L.x, gl);
+    wave_sync();
+    int n_anti = full_eval<T, FAST, G>(P, tb, L, gl);
+    wave_sync();
+    double diag = diag_value<G>(P, L.x, L.anti, n_anti, gl);
     double lw = *lw_p;
     double t = 0.0, eL = 0.0;
     uint32_t it = 0;
     while (t < beta) {
         T lane_sum;
-        T incl = lane_scan(P, L, lane, &lane_sum);
-        T R = lane_bcast(incl, WAVE - 1);
+        T incl = lane_scan<T, G>(P, L, gl, &lane_sum);
+        T R = grp_bcast<G>(incl, G - 1);
         eL = diag - (double)R;
         gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
         double psi = gfmc_u53(r.z, r.w);
@@ -583,8 +677,8 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
             T u = uniform_for<T>(r) * R;
             T cap = below(R);
             if (u > cap) u = cap;
-            idx = sample_slot(P, L, incl, lane_sum, u, lane);
-            if constexpr (FAST) m = prefetch_move(P, idx, lane);  // in flight during the hold math
+            idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
+            if constexpr (FAST) m = prefetch_move(P, idx, gl);  // in flight during the hold math
             tau_raw = -log1p(-psi) / (double)R;
         }
         double rem = beta - t;
@@ -594,25 +688,26 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
         ++it;
         if (tau_raw < rem && idx >= 0) {
             if constexpr (FAST)
-                n_anti = apply_move_rec(P, tb, L, m, idx, n_anti, lane);
+                n_anti = apply_move_rec<T, G>(P, tb, L, m, idx, n_anti, lane);
             else
                 n_anti = apply_move(P, tb, L, idx, n_anti, lane);
             if (!(P.has_jz && P.uniform_jz) || P.has_hz)
-                diag = diag_value(P, L.x, L.anti, n_anti, lane);
+                diag = diag_value<G>(P, L.x, L.anti, n_anti, gl);
             else
                 diag = (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti);
         }
     }
     lw = lw - alw * beta;
-    store_spins(P, sp, L.x, lane);
-    if (lane == 0) {
+    store_spins<G>(P, sp, L.x, gl);
+    if (gl == 0) {
         *lw_p = lw;
         *el_p = eL;
         if (ev_p) *ev_p += (int32_t)it;
     }
 }
 
-template <typename T, bool FAST>
+// one wavefront per workgroup, 64 / G walkers per wavefront
+template <typename T, bool FAST, int G>
 __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
                                                     uint32_t *spins, double *log_w,
                                                     double *e_loc, double beta, double alw,
@@ -620,38 +715,46 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
                                                     uint32_t step, uint32_t tag,
                                                     int32_t *evals, const uint32_t *step_base) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int64_t wk = blockIdx.x;
+    const int grp = threadIdx.x / G;
+    const int64_t wk = (int64_t)blockIdx.x * (WAVE / G) + grp;
     if (wk >= nw) return;
     const uint32_t st = step + (step_base ? *step_base : 0u);
-    propagate_walker<T, FAST>(P, tb, lds, threadIdx.x, (uint32_t)wk, spins + wk * P.words,
-                              log_w + wk, e_loc + wk, evals ? evals + wk : nullptr, beta, alw,
-                              key0, key1, st, tag);
+    unsigned char *base = G == WAVE ? lds : lds + (size_t)grp * walker_lds_bytes<T, G>(P);
+    propagate_walker<T, FAST, G>(P, tb, base, threadIdx.x, (uint32_t)wk, spins + wk * P.words,
+                                 log_w + wk, e_loc + wk, evals ? evals + wk : nullptr, beta,
+                                 alw, key0, key1, st, tag);
 }
 
 // A batch of independent lattices in one launch: walker g belongs to lattice
 // l with woff[l] <= g < woff[l + 1] and runs exactly as walker g - woff[l] of
 // a single-lattice launch with the Philox key seed + l and shift alw[l];
-// spins of every lattice use the common stride `stride` words.
-template <typename T, bool FAST>
+// spins of every lattice use the common stride `stride` words.  With G < 64
+// every lattice's walker count is a multiple of 64 / G, so the lattice is
+// uniform over a wavefront.
+template <typename T, bool FAST, int G>
 __global__ __launch_bounds__(WAVE) void k_propagate_batch(
     const DevProblem *Ps, const Tables<T> *Ts, int n_lat, const int64_t *woff, int64_t n_total,
     int stride, uint32_t *spins, double *log_w, double *e_loc, double beta, const double *alw,
     uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals, const uint32_t *step_base) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int64_t g = blockIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * (WAVE / G);
+    const int grp = threadIdx.x / G;
+    const int64_t g = g0 + grp;
     if (g >= n_total) return;
     step += step_base ? *step_base : 0u;
     int lo = 0, hi = n_lat;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (woff[mid] <= g) lo = mid;
+        if (woff[mid] <= g0) lo = mid;
         else hi = mid;
     }
     const uint64_t key = seed + (uint64_t)lo;
-    propagate_walker<T, FAST>(Ps[lo], Ts[lo], lds, threadIdx.x, (uint32_t)(g - woff[lo]),
-                              spins + g * stride, log_w + g, e_loc + g,
-                              evals ? evals + g : nullptr, beta, alw[lo], (uint32_t)key,
-                              (uint32_t)(key >> 32), step, tag);
+    const DevProblem &P = Ps[lo];
+    unsigned char *base = G == WAVE ? lds : lds + (size_t)grp * walker_lds_bytes<T, G>(P);
+    propagate_walker<T, FAST, G>(P, Ts[lo], base, threadIdx.x, (uint32_t)(g - woff[lo]),
+                                 spins + g * stride, log_w + g, e_loc + g,
+                                 evals ? evals + g : nullptr, beta, alw[lo], (uint32_t)key,
+                                 (uint32_t)(key >> 32), step, tag);
 }
 
 // ---------------------------------------------------------------------------
@@ -729,12 +832,6 @@ __global__ void k_log_psi(DevProblem P, Tables<double> tb, int64_t nw, const uin
 // Every slot is re-evaluated after every move (theta changes everywhere).
 // ---------------------------------------------------------------------------
 #define RBM_WAVES 8
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __device__ __forceinline__ float bf16_at(const uint32_t *W, int idx) {
     const uint32_t w = W[idx >> 1];
@@ -1489,6 +1586,7 @@ static int upload_f32(gfmc_problem *p, const double *src, size_t n, const float 
 // Host-built tables of the record path (see DevProblem).
 struct HostRecords {
     std::vector<uint32_t> slot_rec, slot_rec_lds, mv_rec;   // 8 words = 16 int16 per record
+    std::vector<uint32_t> slot_rec_g[3];                    // LDS order of G = 64, 32, 16
     std::vector<int> mv_nbr, mv_sites, mv_edges, edge_pack;
     int R = 0, RN = 0, RE = 0;
     double J0 = 0.0, h0 = 0.0;
@@ -1567,14 +1665,17 @@ static bool build_records(const gfmc_problem_desc *d, HostRecords &hr) {
         }
         put16(r, 3, mask);
     }
-    // the same records in LDS order: position (s % K) * 64 + s / K
-    {
-        const int K = block_k_of(n_off);
-        hr.slot_rec_lds.assign((size_t)K * WAVE * 8, 0xffffffffu);
+    // the same records in LDS order: position (s % K) * G + s / K with
+    // K = ceil(n_off / G), for walker groups of G = 64, 32, 16 lanes
+    for (int gi = 0; gi < 3; ++gi) {
+        const int G = WAVE >> gi;
+        const int K = (n_off + G - 1) / G;
+        hr.slot_rec_g[gi].assign((size_t)K * G * 8, 0xffffffffu);
         for (int s = 0; s < n_off; ++s)
-            memcpy(&hr.slot_rec_lds[(size_t)((s % K) * WAVE + s / K) * 8],
+            memcpy(&hr.slot_rec_g[gi][(size_t)((s % K) * G + s / K) * 8],
                    &hr.slot_rec[(size_t)s * 8], 8 * sizeof(uint32_t));
     }
+    hr.slot_rec_lds = hr.slot_rec_g[0];
     // per-move record blocks
     for (int s = 0; s < n_off; ++s) hr.R = std::max(hr.R, d->dep_off[s + 1] - d->dep_off[s]);
     if (hr.R > WAVE) return false;
@@ -1645,12 +1746,22 @@ static void launch_propagate(const gfmc_problem *p, const Tables<T> &tb, int64_t
                              uint32_t *spins, double *log_w, double *e_loc, double beta,
                              double alw, uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
                              int32_t *evals, hipStream_t st) {
-    if (p->dp.fast)
-        hipLaunchKernelGGL((k_propagate<T, true>), dim3((unsigned)nw), dim3(WAVE),
+    if (!p->dp.fast)
+        hipLaunchKernelGGL((k_propagate<T, false, 64>), dim3((unsigned)nw), dim3(WAVE),
                            (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, log_w, e_loc, beta,
                            alw, k0, k1, step, tag, evals, g_step_base);
+    else if (p->lanes == 32)
+        hipLaunchKernelGGL((k_propagate<T, true, 32>), dim3((unsigned)((nw + 1) / 2)),
+                           dim3(WAVE), (size_t)(2 * walker_lds_bytes<T, 32>(p->dp)), st, p->dp,
+                           tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
+                           g_step_base);
+    else if (p->lanes == 16)
+        hipLaunchKernelGGL((k_propagate<T, true, 16>), dim3((unsigned)((nw + 3) / 4)),
+                           dim3(WAVE), (size_t)(4 * walker_lds_bytes<T, 16>(p->dp)), st, p->dp,
+                           tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
+                           g_step_base);
     else
-        hipLaunchKernelGGL((k_propagate<T, false>), dim3((unsigned)nw), dim3(WAVE),
+        hipLaunchKernelGGL((k_propagate<T, true, 64>), dim3((unsigned)nw), dim3(WAVE),
                            (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, log_w, e_loc, beta,
                            alw, k0, k1, step, tag, evals, g_step_base);
 }
@@ -1675,6 +1786,20 @@ static void launch_reconfigure(const RcSegs &S, int64_t n_total, unsigned blocks
     hipLaunchKernelGGL(k_rc_parents, dim3((unsigned)((n_total + RC_T - 1) / RC_T)), dim3(RC_T),
                        0, st, S, n_total, (const double *)work, seed, step, tag, use_psi, psi,
                        parents, out2, g_step_base);
+}
+
+// the walker groups of one wavefront fit 64 KiB of LDS
+static bool lanes_lds_fits(const gfmc_problem *p, int lanes) {
+    int64_t b = 0;
+    if (lanes == 32)
+        b = 2 * (int64_t)(p->precision == GFMC_FP64 ? walker_lds_bytes<double, 32>(p->dp)
+                                                     : walker_lds_bytes<float, 32>(p->dp));
+    else if (lanes == 16)
+        b = 4 * (int64_t)(p->precision == GFMC_FP64 ? walker_lds_bytes<double, 16>(p->dp)
+                                                     : walker_lds_bytes<float, 16>(p->dp));
+    else
+        b = p->lds_bytes;
+    return b <= 64 * 1024;
 }
 
 static void free_problem(gfmc_problem *p) {
@@ -1822,6 +1947,18 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         UP(hr.mv_rec.data(), hr.mv_rec.size(), &mr);
         P.slot_rec = (const uint4 *)sr;
         P.mv_rec = (const uint4 *)mr;
+        P.slot_rec_g[0] = P.slot_rec;
+        P.Kg[0] = P.K;
+        P.g_ok[0] = 1;
+        for (int gi = 1; gi < 3; ++gi) {
+            const int G = WAVE >> gi;
+            const uint32_t *sg = nullptr;
+            UP(hr.slot_rec_g[gi].data(), hr.slot_rec_g[gi].size(), &sg);
+            P.slot_rec_g[gi] = (const uint4 *)sg;
+            P.Kg[gi] = (n_off + G - 1) / G;
+            // every move's records and bond-state neighbours fit the group
+            P.g_ok[gi] = hr.R <= G && (!(d->has_jz && d->uniform_jz) || hr.RN <= G);
+        }
         UP(hr.mv_nbr.data(), hr.mv_nbr.size(), &P.mv_nbr);
         UP(hr.mv_sites.data(), hr.mv_sites.size(), &P.mv_sites);
         UP(hr.mv_edges.data(), hr.mv_edges.size(), &P.mv_edges);
@@ -1917,6 +2054,18 @@ int gfmc_problem_destroy(gfmc_problem *p) {
     return 0;
 }
 
+int gfmc_problem_set_lanes(gfmc_problem *p, int32_t lanes) {
+    if (!p) return fail(GFMC_EINVAL, "gfmc_problem_set_lanes: NULL problem");
+    if (lanes != 64 && lanes != 32 && lanes != 16)
+        return fail(GFMC_EINVAL, "gfmc_problem_set_lanes: lanes must be 64, 32 or 16");
+    if (lanes != 64 && (p->dp.guide == GFMC_GUIDE_RBM || !p->dp.fast ||
+                        !p->dp.g_ok[gidx_of(lanes)] || !lanes_lds_fits(p, lanes)))
+        return fail(GFMC_EUNSUP, "gfmc_problem_set_lanes: this problem runs one walker per "
+                                 "wavefront only (record path, moves within the group, LDS)");
+    p->lanes = lanes;
+    return 0;
+}
+
 int gfmc_problem_get_info(const gfmc_problem *p, gfmc_problem_info *out) {
     if (!p || !out) return fail(GFMC_EINVAL, "gfmc_problem_get_info: NULL argument");
     out->n_sites = p->dp.N;
@@ -1928,6 +2077,11 @@ int gfmc_problem_get_info(const gfmc_problem *p, gfmc_problem_info *out) {
     out->block_k = p->dp.K;
     out->full_recompute = p->dp.full;
     out->lds_bytes = p->lds_bytes;
+    out->lanes = p->lanes;
+    out->lanes_ok = 0;
+    for (int gi = 0; gi < 3; ++gi)
+        if (gi == 0 || (p->dp.fast && p->dp.g_ok[gi] && lanes_lds_fits(p, WAVE >> gi)))
+            out->lanes_ok |= WAVE >> gi;
     return 0;
 }
 

@@ -297,7 +297,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
              method="cont_time", rng_key=None, lam=None, diag_operators=None,
              off_diag_operators=None, return_raw=True, avg_log_weight=None,
              warmup_steps=None, verbosity=7, number_of_outputs=100, precision=None,
-             use_graph=True, **_unused):
+             use_graph=True, lanes="auto", **_unused):
     """Single-GPU GFMC run (run_gfmc, driver.py:1148-1547), energy only.
 
     Returns ``(walkers, energy, obs, stats, raw)`` (or the first four with
@@ -308,7 +308,9 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     an int seed or (seed, position); ``stats['parameters/rng_key_next']``
     continues the chain.  ``precision``: 'fp64' / 'fp32' (default from the
     Hamiltonian's dtype).  ``use_graph``: production chunks replay one
-    captured HIP graph (identical results to eager launches).
+    captured HIP graph (identical results to eager launches).  ``lanes``:
+    lanes per walker of the propagation kernel (64, 32, 16) or "auto" for
+    the autotuner (pygfmc_amd.autotune); pin it for bit-reproducible runs.
     """
     wall_start = time.time()
     if diag_operators or off_diag_operators:
@@ -339,6 +341,12 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     step_f = float(step_size)
     p = get_problem(hamiltonian, guide, precision)
     chain = Chain(p, walkers)
+    tune_times = None
+    if lanes == "auto":
+        from .autotune import tune_problem
+        _, tune_times = tune_problem(p, chain.spins, step_f, 0.0)
+    else:
+        p.set_lanes(int(lanes))
     alw_given = avg_log_weight
     alw = _warmup(chain, key, step_f, avg_log_weight, warmup_steps, verbosity)
 
@@ -391,6 +399,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         "total_wall_time": time.time() - wall_start,
         "number_of_devices": 1,
         "hip_graph": graph is not None,
+        "autotune_seconds": tune_times,
         "evals_per_step_mean": evals_sum / max(1, evals_n),
         "parameters/n_steps": n_steps,
         "parameters/n_walkers": walkers.n_walkers,
@@ -405,6 +414,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         "parameters/guide": type(guide).__name__ if guide is not None else "None",
         "parameters/has_guide": int(guide is not None),
         "parameters/precision": "fp64" if p.precision == _lib.GFMC_FP64 else "fp32",
+        "parameters/lanes_per_walker": p.lanes,
         "parameters/rng_key": np.asarray([key.seed, key.position], dtype=np.uint64),
         "parameters/rng_key_next": np.asarray([key_next.seed, key_next.position],
                                               dtype=np.uint64),

@@ -186,6 +186,29 @@ class Problem:
         except Exception:
             pass
 
+    # -- lanes per walker of the propagation kernel ----------------------------
+    @property
+    def lanes_ok(self):
+        """Usable lanes-per-walker values (64 always; 32 / 16 on the record
+        path when every move's dependent slots fit the group)."""
+        return [g for g in (64, 32, 16) if self.info.lanes_ok & g]
+
+    @property
+    def lanes(self):
+        return int(self.info.lanes)
+
+    def lowered_fingerprint(self):
+        h = hashlib.sha1(self.H.fingerprint())
+        h.update(b"none" if self.guide is None else self.guide.fingerprint())
+        h.update(str(self.precision).encode())
+        return h.digest()
+
+    def set_lanes(self, lanes):
+        _lib.check(_lib.load().gfmc_problem_set_lanes(self.handle, int(lanes)),
+                   "gfmc_problem_set_lanes")
+        _lib.check(_lib.load().gfmc_problem_get_info(self.handle, ctypes.byref(self.info)),
+                   "gfmc_problem_get_info")
+
     # -- calls ---------------------------------------------------------------
     def rates(self, spins, want_rates=True, want_eloc=True, want_logdiff=False):
         nw = spins.shape[0]

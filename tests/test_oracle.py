@@ -407,3 +407,37 @@ def test_tfim_free_fermion_energy_matches_ed(L, h):
     from pygfmc_amd import chain, tfim_chain_ground_energy, transverse_field_ising
     H = transverse_field_ising(chain(L).edges, 1.0, h)
     assert tfim_chain_ground_energy(L, 1.0, h) == pytest.approx(ground_energy(H), rel=1e-12)
+
+
+# -- blocked order over 32 / 16 lanes per walker --------------------------------------
+
+@pytest.mark.parametrize("G", [16, 32, 64])
+def test_group_scan_is_exact_prefix_sum_on_integers(orc, G):
+    """With integer-valued weights every order is exact: the G-lane blocked
+    scan returns the prefix sums of the lane blocks and the sampler the
+    first slot whose cumulative sum exceeds u."""
+    rng = np.random.default_rng(G)
+    for n in (1, 7, G, 3 * G + 5, 200):
+        v = rng.integers(0, 5, n).astype(float)
+        v[-1] += 1.0
+        K = (n + G - 1) // G
+        with orc.lanes(G):
+            tot, incl = orc.blocked_scan(v)
+            blocks = [v[l * K:(l + 1) * K].sum() for l in range(G)]
+            np.testing.assert_array_equal(incl[:G], np.cumsum(blocks))
+            assert tot == v.sum()
+            us = rng.uniform(size=50)
+            got = orc.sample_many(v, us)
+        ref = np.searchsorted(np.cumsum(v), us * v.sum(), side="right")
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_group_tree_differs_from_wave_tree_on_mixed_magnitudes(orc):
+    """The 16- and 32-lane trees are different summation orders from the
+    64-lane tree (so a kernel using the wrong one fails bit-exact tests)."""
+    v = np.random.default_rng(2).uniform(size=200)
+    res = {}
+    for G in (16, 32, 64):
+        with orc.lanes(G):
+            res[G] = orc.blocked_scan(v)[0]
+    assert len({res[16], res[32], res[64]}) == 3
