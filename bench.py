@@ -35,19 +35,75 @@ def parse():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", default="auto", help="lanes per walker: auto, 64, 32, 16")
+    ap.add_argument("--kernel-steps", type=int, default=10)
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the chain (L = 100) and batched Ising-chain sections")
     ap.add_argument("--no-race", action="store_true",
                     help="skip the second metric (GPU-seconds to a fixed energy error bar, "
                          "learned RBM guide vs Jastrow, 10x10 Heisenberg)")
     ap.add_argument("--race-train-iters", type=int, default=600)
     ap.add_argument("--race-rbm-steps", type=int, default=300)
+    ap.add_argument("--summarize-profile", default=None,
+                    help="print a text summary of a rocprofv3 database and exit")
+    ap.add_argument("--summarize-kernel", default="k_propagate<float, true")
+    ap.add_argument("--summarize-tail", default=None,
+                    help="kernel-pass,graph launch counts of the propagation kernel")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before the timed steps, copy a tensor of this many MiB once "
                          "(a plain copy kernel of known size, to scale HBM counters)")
     return ap.parse_args()
 
 
+def summarize_profile(db, prefix="k_propagate<float, true", tail=None):
+    """Text summary of a rocprofv3 database (rocpd SQLite) of this script:
+    per-kernel totals, and for the propagation kernel the average duration
+    of its last launches (``tail`` = the bench's kernel pass + graph region),
+    split into the kernel pass (events-timed) and the graph replays; for a
+    --pmc database the per-kernel averages of each counter."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    lines = [f"database: {os.path.basename(db)}"]
+    rows = cur.execute("select name, total_calls, average, percentage from top_kernels "
+                       "order by total_duration desc limit 14").fetchall()
+    lines.append("| kernel | calls | avg us | % |")
+    lines.append("|---|---|---|---|")
+    for n, c, a, pc in rows:
+        short = n.split("(")[0]
+        lines.append(f"| {short} | {c} | {a:.1f} | {pc:.2f} |")
+    pmc = cur.execute("select kernel_name, counter_name, avg(value), count(*) from "
+                      "counters_collection group by kernel_name, counter_name").fetchall()
+    if pmc:
+        lines.append("")
+        lines.append("| kernel | counter | avg per dispatch | dispatches |")
+        lines.append("|---|---|---|---|")
+        for kn, cn, v, c in sorted(pmc, key=lambda r: -r[2])[:12]:
+            lines.append(f"| {kn.split('(')[0][:60]} | {cn} | {v:.1f} | {c} |")
+    if tail:
+        n_k, n_graph = tail
+        d = [r[0] for r in cur.execute("select duration from kernels where name like ? "
+                                       "order by start", ("%" + prefix + "%",)).fetchall()]
+        if len(d) >= n_k + n_graph:
+            kp = d[len(d) - n_graph - n_k:len(d) - n_graph]
+            gr = d[len(d) - n_graph:]
+            lines.append("")
+            lines.append(f"{prefix}...>: {len(d)} launches; kernel pass (last {n_k} before "
+                         f"the graph): avg {np.mean(kp) / 1e6:.4f} ms; graph replays "
+                         f"({n_graph}): avg {np.mean(gr) / 1e6:.4f} ms")
+    return "\n".join(lines)
+
+
 def main():
     args = parse()
+    if args.summarize_profile:
+        tail = None
+        if args.summarize_tail:
+            a, b = args.summarize_tail.split(",")
+            tail = (int(a), int(b))
+        print(summarize_profile(args.summarize_profile, args.summarize_kernel, tail))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; runs on more than one GPU "
                                    "are not implemented (walkers would shard by contiguous "
@@ -86,6 +142,13 @@ def main():
             lg = series[t - 4: t + 1, 0].cpu().numpy()
             alw += float(np.mean(lg)) / beta
     torch.cuda.synchronize()
+    # lanes per walker of the propagation kernel: the autotuner, or pinned
+    if args.lanes == "auto":
+        from pygfmc_amd.autotune import tune_problem
+        lanes, tune_times = tune_problem(p, chain.spins, beta, alw)
+    else:
+        p.set_lanes(int(args.lanes))
+        lanes, tune_times = int(args.lanes), None
 
     if args.calib_copy_mib > 0:
         src = torch.ones(args.calib_copy_mib * (1 << 18), dtype=torch.float32,
@@ -95,30 +158,50 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    chain.evals.zero_()
+    # kernel pass: per-step launches, HIP events around the propagation
+    # kernel on its launch stream (the roofline's kernel time and work)
+    n_k = max(1, args.kernel_steps)
     events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
+              for _ in range(n_k)]
+    chain.evals.zero_()
+    for k in range(n_k):
         one_step(args.warmup + k, k, events[k])
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    lg = series[: args.steps, 0].cpu().numpy()
-    if not np.all(np.isfinite(lg)):
-        raise SystemExit("population extinct during the bench")
     prop_ms = [a.elapsed_time(b) for a, b in events]
     prop_avg_s = float(np.mean(prop_ms)) / 1e3
+    evals_kernel = float(chain.evals.to(torch.float64).sum().item())
+
+    # timed region: the production loop as replays of one captured HIP
+    # graph of `chunk` steps, inputs resident in HBM
+    from pygfmc_amd.engine import StepGraph
+    chunk = max(1, min(args.graph_steps, args.steps))
+    n_rep = max(1, args.steps // chunk)
+    steps_timed = n_rep * chunk
+    graph = StepGraph(chain, chunk, beta, alw, seed, evals=True)
+    pos = args.warmup + n_k
+    graph.launch(pos)          # untimed: first replay
+    pos += chunk
+    chain.evals.zero_()
+    out_series = torch.empty((steps_timed, 2), dtype=torch.float64, device=chain.spins.device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(n_rep):
+        graph.launch(pos + r * chunk, out=out_series[r * chunk:(r + 1) * chunk])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lg = out_series[:, 0].cpu().numpy()
+    if not np.all(np.isfinite(lg)):
+        raise SystemExit("population extinct during the bench")
     evals_total = float(chain.evals.to(torch.float64).sum().item())
-    evals_per_ws = evals_total / (nw * args.steps)
-    ms_per_step = elapsed / args.steps * 1e3
-    value = nw * args.steps / elapsed
+    evals_per_ws = evals_total / (nw * steps_timed)
+    ms_per_step = elapsed / steps_timed * 1e3
+    value = nw * steps_timed / elapsed
 
     # algorithmic FP32 work of one launch (reference algorithm: every slot of
     # every rate evaluation): n_edges * (4 (d - 1) + 6) per evaluation
     d = G.max_deg
     flop_per_eval = E * (4 * (d - 1) + 6)
-    flops_launch = evals_total / args.steps * flop_per_eval
+    flops_launch = evals_kernel / n_k * flop_per_eval
     achieved_tf = flops_launch / prop_avg_s / 1e12
     peak_tf = 157.3
 
@@ -131,7 +214,7 @@ def main():
         "value": value,
         "unit": "walker-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps_timed,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
@@ -144,18 +227,81 @@ def main():
                    "lattice": f"square {L}x{L} periodic", "n_sites": N, "n_edges": E,
                    "n_walkers": nw, "delta": args.delta, "step_size": beta,
                    "guide": "Jastrow beta=0.3", "global_batch": nw, "seq_len": N,
-                   "parallelism": "none (1 GPU)"},
+                   "parallelism": "none (1 GPU)", "hip_graph_steps": chunk,
+                   "lanes_per_walker": lanes, "autotune_seconds": tune_times,
+                   "evals_per_walker_step": evals_per_ws},
         "roofline": {"bound": "mfma", "compute_unit": "VALU FP32 (no GEMM on this path)",
                      "achieved": achieved_tf, "peak": peak_tf, "unit": "TFLOP/s",
                      "frac": achieved_tf / peak_tf, "traffic": None,
-                     "kernel": "k_propagate<float>", "kernel_avg_ms": prop_avg_s * 1e3,
-                     "evals_per_walker_step": evals_per_ws,
+                     "kernel": f"k_propagate<float, record path, {lanes} lanes>",
+                     "kernel_avg_ms": prop_avg_s * 1e3, "kernel_steps": n_k,
+                     "evals_per_walker_step": evals_kernel / (nw * n_k),
                      "flop_per_eval": flop_per_eval},
         "cpu_baseline": cpu,
     }
+    if not args.no_configs:
+        out["heisenberg_chain100"] = chain100_section()
+        out["batched_tfim_chains"] = tfim_batch_section()
     if not args.no_race:
         out["sigma_race"] = sigma_race(args)
     print(json.dumps(out), flush=True)
+
+
+def chain100_section():
+    """The reference's plain method: Heisenberg ring L = 100 (sign-free frame),
+    Jastrow beta 0.3, 1024 walkers Sz = 0, step 0.2, p = [4, 16, 64, 128],
+    2000 warm-up + 1000 production steps, FP64; E/L against the Bethe-ansatz
+    value -0.4432296."""
+    from pygfmc_amd import JastrowSpinHalf, SpinHalf, chain, prepare_walkers_sz
+    lat = chain(100)
+    H = SpinHalf.create(lat.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    G = JastrowSpinHalf(100, lat.edges, beta=0.3, dtype=np.float64)
+    w = prepare_walkers_sz(1024, 100, 0)
+    secs, E, err, st = timed_run(w, 1000, H, G, [4, 16, 64, 128], 0, 2000)
+    return {"workload": "heisenberg_chain100_jastrow0.3_w1024_step0.2_fp64",
+            "walker_steps_per_s": 1024 * 1000 / st["runtime"], "it_per_sec": st["it_per_sec"],
+            "gpu_s_total": secs, "E_per_site": E / 100, "sigma_per_site": err / 100,
+            "exact_per_site": -0.4432296,
+            "z": (E / 100 + 0.4432296) / (err / 100) if err > 0 else None,
+            "lanes_per_walker": st["parameters/lanes_per_walker"],
+            "evals_per_step": st["evals_per_step_mean"]}
+
+
+def tfim_batch_section():
+    """64 transverse-field Ising chains in one launch (J = 1, h ~ U[0.5, 1.5]
+    and L ~ U{16..256} from seed 0), Jastrow beta = -0.25, 2048 walkers each,
+    step 0.2, p = [4, 16, 64], 400 warm-up + 1500 steps, FP32 ratios; z-scores
+    of E(p = 64) against the free-fermion ground energies."""
+    import warnings
+
+    from pygfmc_amd import (JastrowSpinHalf, chain, evaluate_observables,
+                            prepare_walkers_random, run_batch, tfim_chain_ground_energy,
+                            transverse_field_ising)
+    rng = np.random.default_rng(0)
+    Ls = rng.integers(16, 257, 64)
+    hs = rng.uniform(0.5, 1.5, 64)
+    Hs = [transverse_field_ising(chain(int(L)).edges, 1.0, h, dtype=np.float32)
+          for L, h in zip(Ls, hs)]
+    Gs = [JastrowSpinHalf(int(L), chain(int(L)).edges, beta=-0.25, dtype=np.float32)
+          for L in Ls]
+    ws = [prepare_walkers_random(2048, int(L), l) for l, L in enumerate(Ls)]
+    _, raws, st = run_batch(ws, 1500, Hs, 0.2, Gs, [4, 16, 64], rng_key=7, warmup_steps=400)
+    zs = []
+    for l in range(64):
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            E, _, err, _ = evaluate_observables(raws[l], n_blocks=20)
+        zs.append((E[64] - tfim_chain_ground_energy(int(Ls[l]), 1.0, hs[l])) / err[64])
+    zs = np.asarray(zs)
+    small = Ls <= 64
+    return {"workload": "tfim_64chains_L16-256_w2048_step0.2_fp32",
+            "walker_steps_per_s": st["walker_steps_per_sec"], "it_per_sec": st["it_per_sec"],
+            "n_walkers_total": st["n_walkers_total"], "lanes_per_walker": st["lanes_per_walker"],
+            "autotune_seconds": st["autotune_seconds"], "hip_graph": st["hip_graph"],
+            "evals_per_step": st["evals_per_step_mean"],
+            "z_rms_all": float(np.sqrt(np.mean(zs ** 2))),
+            "z_rms_L_le_64": float(np.sqrt(np.mean(zs[small] ** 2))),
+            "frac_abs_z_lt_3": float(np.mean(np.abs(zs) < 3))}
 
 
 def timed_run(walkers, n_steps, H, G, p_array, seed, warmup):
