@@ -46,6 +46,12 @@ extern "C" {
 #define GFMC_GUIDE_JASTROW 1
 #define GFMC_GUIDE_RBM 2
 
+/* propagators (evolution.py): continuous time (evolve_walker_cont_time,
+ * :559-721), single_step (:322-424) and multi_step (:427-556) */
+#define GFMC_METHOD_CONT 0
+#define GFMC_METHOD_SINGLE 1
+#define GFMC_METHOD_MULTI 2
+
 /* Philox stream tags (counter word 3): disjoint streams per purpose */
 #define GFMC_TAG_PROPAGATE 0u
 #define GFMC_TAG_RECONFIG 1u
@@ -131,6 +137,18 @@ int gfmc_problem_destroy(gfmc_problem *p);
  * run with the same lane count, and differ from another lane count in the
  * last bits.  gfmc_problem_info.lanes_ok lists the usable values. */
 int gfmc_problem_set_lanes(gfmc_problem *p, int32_t lanes);
+
+/* Propagator of gfmc_propagate / gfmc_run_steps / the graphs and batches
+ * built from this problem (require_lam, evolution.py:724-741): method
+ * GFMC_METHOD_CONT (default; the step argument is beta) or
+ * GFMC_METHOD_SINGLE / GFMC_METHOD_MULTI (the step argument is the integer
+ * sub-step count n_bra, lam the finite shift, lam >= max H_xx over the
+ * reachable configurations).  Discrete methods: every sub-step draws
+ * between staying (weight lam - H_xx) and the off-diagonal slots, log_w
+ * gains log(lam - H_xx + R) per sub-step, alw * n_bra is subtracted at the
+ * end, and e_loc is the local energy of the final configuration (one more
+ * evaluation).  The RBM guide propagates in continuous time only. */
+int gfmc_problem_set_method(gfmc_problem *p, int32_t method, double lam);
 int gfmc_problem_get_info(const gfmc_problem *p, gfmc_problem_info *out);
 
 /* Walker initialisers (walker.py:143-306).  _sz: every walker a Philox-driven

@@ -66,7 +66,8 @@ def lib():
         L.orc_propagate.restype = ctypes.c_int
         L.orc_propagate.argtypes = [D, ctypes.c_int64, _u32p, _f64p, _f64p, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
-                                    ctypes.c_uint32, _i32p, ctypes.c_int]
+                                    ctypes.c_uint32, _i32p, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double]
         L.orc_reconfigure.restype = ctypes.c_int
         L.orc_reconfigure.argtypes = [ctypes.c_int64, _f64p, _f64p, ctypes.c_double, _i32p,
                                       _f64p, _f64p]
@@ -151,9 +152,11 @@ class Problem:
         return out
 
     def propagate(self, spins, log_w, e_loc, beta, alw, seed, step, tag, n_threads=0,
-                  lanes=64):
+                  lanes=64, method=0, lam=0.0):
         """In place on copies; returns (spins, log_w, e_loc, evals).  lanes:
-        lanes per walker of the kernel being mirrored (blocked order)."""
+        lanes per walker of the kernel being mirrored (blocked order);
+        method 0 / 1 / 2 = cont_time / single_step / multi_step (beta is then
+        the sub-step count, lam the shift)."""
         spins = np.array(spins, dtype=np.uint32, copy=True)
         log_w = np.array(log_w, dtype=np.float64, copy=True)
         e_loc = np.array(e_loc, dtype=np.float64, copy=True)
@@ -164,7 +167,8 @@ class Problem:
             lib().orc_propagate(ctypes.byref(self.desc), nw, _p(spins, ctypes.c_uint32),
                                 _p(log_w, ctypes.c_double), _p(e_loc, ctypes.c_double),
                                 float(beta), float(alw), int(seed), int(step) & 0xFFFFFFFF,
-                                int(tag), _p(ev, ctypes.c_int32), int(n_threads))
+                                int(tag), _p(ev, ctypes.c_int32), int(n_threads), int(method),
+                                float(lam))
         finally:
             lib().orc_set_lanes(64)
         return spins, log_w, e_loc, ev
@@ -259,7 +263,7 @@ def unpack(spins, N):
 
 
 def run_chain(prob, spins, n_steps, beta, alw, seed, step0, warmup=False, n_threads=0,
-              lanes=64):
+              lanes=64, method=0, lam=0.0):
     """n_steps branching steps (propagate, reconfigure, gather) with the
     same counters as the GPU step loop; returns (spins, e_loc, series)."""
     tp, tr = (3, 4) if warmup else (0, 1)
@@ -269,7 +273,8 @@ def run_chain(prob, spins, n_steps, beta, alw, seed, step0, warmup=False, n_thre
     series = np.empty((n_steps, 2))
     for t in range(n_steps):
         spins, lw, e, _ = prob.propagate(spins, lw, e, beta, alw, seed, step0 + t, tp,
-                                         n_threads=n_threads, lanes=lanes)
+                                         n_threads=n_threads, lanes=lanes, method=method,
+                                         lam=lam)
         psi = reconfig_psi(seed, step0 + t, tr)
         par, lg, En, _ = reconfigure(lw, e, psi)
         series[t] = (lg, En)

@@ -408,48 +408,119 @@ void orc_log_psi_batch(const gfmc_problem_desc *d, int64_t nw, const uint32_t *s
 /* loop runs while time_used < beta (:697); the final e_loc is the e_L of    */
 /* the last evaluation; log_w -= avg_log_weight * beta at the end (:718).    */
 /* ------------------------------------------------------------------------ */
+static void flip_slot(const gfmc_problem_desc *d, int8_t *x, int idx) {
+    if (idx < d->n_hx) {
+        x[idx] = (int8_t)-x[idx];
+    } else {
+        int e = idx - d->n_hx;
+        x[d->edge_i[e]] = (int8_t)-x[d->edge_i[e]];
+        x[d->edge_j[e]] = (int8_t)-x[d->edge_j[e]];
+    }
+}
+
+/* every off-diagonal weight w[s] = -rate_s of x; returns the diagonal */
+static double weights_of(const gfmc_problem_desc *d, const int8_t *x, double *w) {
+    int n_off = n_off_of(d);
+    double *theta = theta_of(d, x);
+    for (int s = 0; s < n_off; ++s) w[s] = -slot_rate(d, x, s, NULL, theta);
+    free(theta);
+    return orc_diag(d, x);
+}
+
+/* method 0: continuous time (evolve_walker_cont_time, evolution.py:559-721);
+ * 1: n_bra sub-steps of (lam - H) (evolve_walker_single_step, :322-424):
+ *    stay with weight w0 = lam - H_xx, hop to slot s with weight w_s; the
+ *    draw u = min(U (w0 + R), nextafter(w0 + R, 0)) stays when u < w0, else
+ *    u - w0 is drawn from the blocked CDF of the slots; log_w += log(w0 + R);
+ * 2: the same with geometric holds (evolve_walker_multi_step, :427-556):
+ *    k = min(floor(log1p(-psi) / log1p(-R / b)) + 1, remaining + 1), the hop
+ *    applied when k <= remaining, log_w += log(b) k, b = lam - H_xx + R.
+ * The discrete methods end with one more evaluation for e_loc. */
 static void propagate_one(const gfmc_problem_desc *d, int64_t widx, uint32_t *sp,
                           double *log_w, double *e_loc, double beta, double alw,
                           uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals,
-                          int8_t *x, double *w) {
+                          int8_t *x, double *w, int method, double lam) {
     int N = d->n_sites, n_off = n_off_of(d);
     unpack1(sp, N, x);
-    double lw = *log_w, t = 0.0, eL = 0.0;
+    double lw = *log_w, eL = 0.0;
     uint32_t it = 0;
     double incl[WAVE];
-    while (t < beta) {
-        double *theta = theta_of(d, x);
-        for (int s = 0; s < n_off; ++s) w[s] = -slot_rate(d, x, s, NULL, theta);
-        free(theta);
-        double diag = orc_diag(d, x);
-        double R = orc_blocked_scan(w, n_off, incl);
-        eL = diag - R;
-        double U1, psi;
-        draw2(seed, it, (uint32_t)widx, step, tag, &U1, &psi);
-        int idx = -1;
-        double tau_raw = INFINITY;
-        if (R > 0) {
-            double u = U1 * R, cap = nextafter(R, 0.0);
-            if (u > cap) u = cap;
-            idx = orc_sample_blocked(w, n_off, incl, u);
-            tau_raw = -log1p(-psi) / R;
+    if (method == 0) {
+        double t = 0.0;
+        while (t < beta) {
+            double diag = weights_of(d, x, w);
+            double R = orc_blocked_scan(w, n_off, incl);
+            eL = diag - R;
+            double U1, psi;
+            draw2(seed, it, (uint32_t)widx, step, tag, &U1, &psi);
+            int idx = -1;
+            double tau_raw = INFINITY;
+            if (R > 0) {
+                double u = U1 * R, cap = nextafter(R, 0.0);
+                if (u > cap) u = cap;
+                idx = orc_sample_blocked(w, n_off, incl, u);
+                tau_raw = -log1p(-psi) / R;
+            }
+            double rem = beta - t;
+            double tau = tau_raw < rem ? tau_raw : rem;
+            lw = lw - eL * tau;
+            t = t + tau;
+            ++it;
+            if (tau_raw < rem && idx >= 0) flip_slot(d, x, idx);
         }
-        double rem = beta - t;
-        double tau = tau_raw < rem ? tau_raw : rem;
-        lw = lw - eL * tau;
-        t = t + tau;
-        ++it;
-        if (tau_raw < rem && idx >= 0) {
-            if (idx < d->n_hx) {
-                x[idx] = (int8_t)-x[idx];
-            } else {
-                int e = idx - d->n_hx;
-                x[d->edge_i[e]] = (int8_t)-x[d->edge_i[e]];
-                x[d->edge_j[e]] = (int8_t)-x[d->edge_j[e]];
+        lw = lw - alw * beta;
+    } else {
+        int n_bra = (int)beta;
+        if (method == 1) {
+            for (; it < (uint32_t)n_bra;) {
+                double diag = weights_of(d, x, w);
+                double R = orc_blocked_scan(w, n_off, incl);
+                double w0 = lam - diag, W = w0 + R;
+                double U1, psi;
+                draw2(seed, it, (uint32_t)widx, step, tag, &U1, &psi);
+                double u = U1 * W, cap = nextafter(W, 0.0);
+                if (u > cap) u = cap;
+                int idx = -1;
+                if (!(u < w0)) idx = orc_sample_blocked(w, n_off, incl, u - w0);
+                lw = lw + log(w0 + R);
+                ++it;
+                if (idx >= 0) flip_slot(d, x, idx);
+            }
+        } else {
+            int used = 0;
+            while (used < n_bra) {
+                double diag = weights_of(d, x, w);
+                double R = orc_blocked_scan(w, n_off, incl);
+                double bx = (lam - diag) + R;
+                double U1, psi;
+                draw2(seed, it, (uint32_t)widx, step, tag, &U1, &psi);
+                int idx = -1;
+                if (R > 0) {
+                    double u = U1 * R, cap = nextafter(R, 0.0);
+                    if (u > cap) u = cap;
+                    idx = orc_sample_blocked(w, n_off, incl, u);
+                }
+                int remaining = n_bra - used;
+                double p_hop = R / bx;
+                double k_raw = (double)remaining + 1.0;
+                if (p_hop > 0.0) {
+                    double hold = floor(log1p(-psi) / log1p(-p_hop)) + 1.0;
+                    k_raw = hold < k_raw ? hold : k_raw;
+                }
+                int kr = (int)k_raw;
+                int k = kr < remaining ? kr : remaining;
+                lw = lw + log(bx) * (double)k;
+                used += k;
+                ++it;
+                if (kr <= remaining && idx >= 0) flip_slot(d, x, idx);
             }
         }
+        double diag = weights_of(d, x, w);
+        double R = orc_blocked_scan(w, n_off, incl);
+        eL = diag - R;
+        ++it;
+        lw = lw - alw * (double)n_bra;
     }
-    lw = lw - alw * beta;
     *log_w = lw;
     *e_loc = eL;
     if (evals) *evals = (int32_t)it;
@@ -458,7 +529,7 @@ static void propagate_one(const gfmc_problem_desc *d, int64_t widx, uint32_t *sp
 
 int orc_propagate(const gfmc_problem_desc *d, int64_t nw, uint32_t *spins, double *log_w,
                   double *e_loc, double beta, double alw, uint64_t seed, uint32_t step,
-                  uint32_t tag, int32_t *evals, int n_threads) {
+                  uint32_t tag, int32_t *evals, int n_threads, int method, double lam) {
     int N = d->n_sites, W = n_words_of(N), n_off = n_off_of(d);
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
@@ -472,7 +543,7 @@ int orc_propagate(const gfmc_problem_desc *d, int64_t nw, uint32_t *spins, doubl
 #pragma omp for schedule(dynamic, 4)
         for (int64_t i = 0; i < nw; ++i)
             propagate_one(d, i, spins + i * W, log_w + i, e_loc + i, beta, alw, seed, step, tag,
-                          evals ? evals + i : NULL, x, w);
+                          evals ? evals + i : NULL, x, w, method, lam);
         free(x);
         free(w);
     }

@@ -18,7 +18,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import StepGraph, require_method, require_step_size, warmup_schedule
+from .engine import (METHODS, StepGraph, require_lam, require_method, require_step_size,
+                     warmup_schedule)
 from .estimators import check_population, estimate_energy, validated_int, validated_p_array
 from .problem import Problem, precision_of
 from .walkers import Walker, as_key
@@ -29,13 +30,21 @@ __all__ = ["Batch", "BatchChain", "run_batch"]
 class Batch:
     """Device batch of lattices; keeps its problems alive."""
 
-    def __init__(self, hamiltonians, guides, n_walkers, precision=None):
+    def __init__(self, hamiltonians, guides, n_walkers, precision=None, method="cont_time",
+                 lam=None):
         if len(hamiltonians) != len(guides) or len(hamiltonians) != len(n_walkers):
             raise ValueError("hamiltonians, guides and n_walkers must have equal lengths")
         if len(hamiltonians) == 0:
             raise ValueError("a batch needs at least one lattice")
+        require_method(method)
         prec = precision_of(hamiltonians[0], precision)
         self.problems = [Problem(H, G, prec) for H, G in zip(hamiltonians, guides)]
+        lams = lam if np.ndim(lam) else [lam] * len(self.problems)
+        for p, lv in zip(self.problems, lams):
+            lv = require_lam(method, lv)
+            _lib.check(_lib.load().gfmc_problem_set_method(p.handle, METHODS[method],
+                                                           0.0 if lv is None else lv),
+                       "gfmc_problem_set_method")
         self.n_lat = len(self.problems)
         self.n_walkers = [int(n) for n in n_walkers]
         lib = _lib.load()
@@ -128,7 +137,7 @@ def _lse(v):
 
 def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *,
               method="cont_time", rng_key=0, warmup_steps=1000, precision=None, verbosity=0,
-              number_of_outputs=10, use_graph=True, lanes="auto"):
+              number_of_outputs=10, use_graph=True, lanes="auto", lam=None):
     """GFMC on every lattice of the batch.  Returns ``(energies, raws,
     stats)``: ``energies[l]`` {p: E_p}, ``raws[l]`` with ``log_weights`` and
     ``energy`` series, ``stats`` with the batch throughput.  ``lanes``: lanes
@@ -142,7 +151,9 @@ def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *
         raise ValueError(f"every p value must be smaller than n_steps: got n_steps={n_steps} "
                          f"and max(p)={p_sorted[-1]}")
     key = as_key(rng_key)
-    b = Batch(hamiltonians, guides, [w.n_walkers for w in walkers_list], precision)
+    b = Batch(hamiltonians, guides, [w.n_walkers for w in walkers_list], precision, method,
+              lam)
+    lams = lam if np.ndim(lam) else [lam] * b.n_lat
     chain = BatchChain(b, walkers_list)
     dev = chain.spins.device
     tune_times = None
@@ -157,10 +168,11 @@ def run_batch(walkers_list, n_steps, hamiltonians, step_size, guides, p_array, *
         a, z = b.offsets[l], b.offsets[l + 1]
         p = b.problems[l]
         _, e, _ = p.rates(chain.spins[a:z, :p.n_words].contiguous(), want_rates=False)
-        alw[l] = float(torch.mean(-e).item())
+        alw[l] = float(torch.mean(-e).item() if method == "cont_time" else
+                       torch.mean(torch.log(float(lams[l]) - e)).item())
     step_f = float(step_size)
     counter = key.position
-    for step_, n_w in warmup_schedule(step_f, warmup_steps):
+    for step_, n_w in warmup_schedule(step_f, warmup_steps, method):
         series = chain.steps(n_w, step_, torch.as_tensor(alw, device=dev), key.seed, counter,
                              warmup=True)
         counter += n_w

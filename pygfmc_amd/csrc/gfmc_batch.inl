@@ -11,7 +11,7 @@
 // seed seed + l.
 
 struct gfmc_batch {
-    int n_lat = 0, precision = 0, fast = 0, stride = 0, lanes = WAVE;
+    int n_lat = 0, precision = 0, fast = 0, stride = 0, lanes = WAVE, method = 0;
     int64_t n_total = 0, lds_bytes = 0, work_doubles = 0;
     int64_t lds_g[3] = {0, 0, 0};   // per wavefront with 64 / 32 / 16 lanes per walker
     int g_ok[3] = {1, 0, 0};
@@ -49,6 +49,28 @@ static RcSegs batch_segs(const gfmc_batch *b) {
     return S;
 }
 
+template <typename T, bool FAST, int G>
+static void launch_batch_g(const gfmc_batch *b, const Tables<T> *Ts, unsigned grid, size_t lds,
+                           uint32_t *spins, double *log_w, double *e_loc, double beta,
+                           const double *alw, uint64_t seed, uint32_t step, uint32_t tag,
+                           int32_t *evals, hipStream_t st) {
+    if (b->method == GFMC_METHOD_SINGLE)
+        hipLaunchKernelGGL((k_propagate_batch<T, FAST, G, GFMC_METHOD_SINGLE>), dim3(grid),
+                           dim3(WAVE), lds, st, b->d_P, Ts, b->n_lat, b->d_woff, b->n_total,
+                           b->stride, spins, log_w, e_loc, beta, alw, seed, step, tag, evals,
+                           g_step_base);
+    else if (b->method == GFMC_METHOD_MULTI)
+        hipLaunchKernelGGL((k_propagate_batch<T, FAST, G, GFMC_METHOD_MULTI>), dim3(grid),
+                           dim3(WAVE), lds, st, b->d_P, Ts, b->n_lat, b->d_woff, b->n_total,
+                           b->stride, spins, log_w, e_loc, beta, alw, seed, step, tag, evals,
+                           g_step_base);
+    else
+        hipLaunchKernelGGL((k_propagate_batch<T, FAST, G, GFMC_METHOD_CONT>), dim3(grid),
+                           dim3(WAVE), lds, st, b->d_P, Ts, b->n_lat, b->d_woff, b->n_total,
+                           b->stride, spins, log_w, e_loc, beta, alw, seed, step, tag, evals,
+                           g_step_base);
+}
+
 template <typename T>
 static void launch_propagate_batch(const gfmc_batch *b, const Tables<T> *Ts, uint32_t *spins,
                                    double *log_w, double *e_loc, double beta, const double *alw,
@@ -56,25 +78,17 @@ static void launch_propagate_batch(const gfmc_batch *b, const Tables<T> *Ts, uin
                                    hipStream_t st) {
     const unsigned n = (unsigned)b->n_total;
     if (!b->fast)
-        hipLaunchKernelGGL((k_propagate_batch<T, false, 64>), dim3(n), dim3(WAVE),
-                           (size_t)b->lds_bytes, st, b->d_P, Ts, b->n_lat, b->d_woff,
-                           b->n_total, b->stride, spins, log_w, e_loc, beta, alw, seed, step,
-                           tag, evals, g_step_base);
+        launch_batch_g<T, false, 64>(b, Ts, n, (size_t)b->lds_bytes, spins, log_w, e_loc, beta,
+                                     alw, seed, step, tag, evals, st);
     else if (b->lanes == 32)
-        hipLaunchKernelGGL((k_propagate_batch<T, true, 32>), dim3((n + 1) / 2), dim3(WAVE),
-                           (size_t)b->lds_g[1], st, b->d_P, Ts, b->n_lat, b->d_woff, b->n_total,
-                           b->stride, spins, log_w, e_loc, beta, alw, seed, step, tag, evals,
-                           g_step_base);
+        launch_batch_g<T, true, 32>(b, Ts, (n + 1) / 2, (size_t)b->lds_g[1], spins, log_w,
+                                    e_loc, beta, alw, seed, step, tag, evals, st);
     else if (b->lanes == 16)
-        hipLaunchKernelGGL((k_propagate_batch<T, true, 16>), dim3((n + 3) / 4), dim3(WAVE),
-                           (size_t)b->lds_g[2], st, b->d_P, Ts, b->n_lat, b->d_woff, b->n_total,
-                           b->stride, spins, log_w, e_loc, beta, alw, seed, step, tag, evals,
-                           g_step_base);
+        launch_batch_g<T, true, 16>(b, Ts, (n + 3) / 4, (size_t)b->lds_g[2], spins, log_w,
+                                    e_loc, beta, alw, seed, step, tag, evals, st);
     else
-        hipLaunchKernelGGL((k_propagate_batch<T, true, 64>), dim3(n), dim3(WAVE),
-                           (size_t)b->lds_bytes, st, b->d_P, Ts, b->n_lat, b->d_woff,
-                           b->n_total, b->stride, spins, log_w, e_loc, beta, alw, seed, step,
-                           tag, evals, g_step_base);
+        launch_batch_g<T, true, 64>(b, Ts, n, (size_t)b->lds_bytes, spins, log_w, e_loc, beta,
+                                    alw, seed, step, tag, evals, st);
 }
 
 extern "C" {
@@ -101,6 +115,8 @@ int gfmc_batch_create(const gfmc_problem *const *ps, int32_t n_lat, const int64_
         if (p->dp.fast != fast)
             return fail(GFMC_EUNSUP, "gfmc_batch_create: every lattice must take the same "
                                      "kernel path (record path or generic)");
+        if (p->dp.method != ps[0]->dp.method)
+            return fail(GFMC_EINVAL, "gfmc_batch_create: every lattice needs one method");
         if (n_walkers[l] < 1)
             return fail(GFMC_EINVAL, "gfmc_batch_create: every lattice needs >= 1 walker");
         woff[l] = total;
@@ -134,6 +150,7 @@ int gfmc_batch_create(const gfmc_problem *const *ps, int32_t n_lat, const int64_
     b->n_lat = n_lat;
     b->precision = prec;
     b->fast = fast;
+    b->method = ps[0]->dp.method;
     b->stride = stride;
     b->n_total = total;
     b->lds_bytes = lds;

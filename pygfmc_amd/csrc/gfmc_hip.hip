@@ -86,6 +86,8 @@ struct DevProblem {
     const int *dep_off, *dep_slots, *edge_off, *dep_edges;
     // record path
     int fast, R, RN, RE, flags;
+    int method;       // GFMC_METHOD_* of the propagation kernels
+    double lam;       // the shift lambda of the discrete methods
     const uint4 *slot_rec;   // [K * 64 * 2], in LDS order (record p -> LDS position p)
     // the same per walker-group size G = 64, 32, 16: K_G = ceil(n_off / G),
     // records in that group's LDS order, and whether the record path fits G
@@ -641,16 +643,23 @@ __device__ int apply_move(const DevProblem &P, const Tables<T> &tb, WalkerLds<T>
 // ---------------------------------------------------------------------------
 // propagation kernel (evolve_walker_cont_time, evolution.py:559-721)
 // ---------------------------------------------------------------------------
+// Propagators (evolution.py, GFMC_METHOD_* of gfmc.h): continuous time
+// (evolve_walker_cont_time, :559-721), n_bra elementary sub-steps of
+// (lambda - H) (single_step, :322-424) and the same with the diagonal stays
+// of a hold drawn as one geometric variate (multi_step, :427-556).
+
 // One walker's branching step by a group of G lanes (G = 64: the whole
 // wavefront; the generic path only runs with G = 64): spins at sp, log_w /
 // e_loc / evals at lw_p / el_p / ev_p, LDS state at lds; wk is the walker
-// index in the Philox counter (hop, wk, step, tag).  Groups of one
-// wavefront run independently (no workgroup barriers on this path).
-template <typename T, bool FAST, int G>
+// index in the Philox counter (hop, wk, step, tag).  step_size is beta
+// (continuous time) or the sub-step count n_bra (discrete methods, shift
+// P.lam).  Groups of one wavefront run independently (no workgroup
+// barriers on this path).
+template <typename T, bool FAST, int G, int METHOD>
 __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tables<T> &tb,
                                                  unsigned char *lds, int lane, uint32_t wk,
                                                  uint32_t *sp, double *lw_p, double *el_p,
-                                                 int32_t *ev_p, double beta, double alw,
+                                                 int32_t *ev_p, double step_size, double alw,
                                                  uint32_t key0, uint32_t key1, uint32_t step,
                                                  uint32_t tag) {
     const int gl = lane & (G - 1);
@@ -661,43 +670,119 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
     wave_sync();
     double diag = diag_value<G>(P, L.x, L.anti, n_anti, gl);
     double lw = *lw_p;
-    double t = 0.0, eL = 0.0;
+    double eL = 0.0;
     uint32_t it = 0;
-    while (t < beta) {
+    // flip the sites of slot idx, re-evaluate what depends on them, new diag
+    auto move = [&](int idx, const MovePrefetch &m) {
+        if constexpr (FAST)
+            n_anti = apply_move_rec<T, G>(P, tb, L, m, idx, n_anti, lane);
+        else
+            n_anti = apply_move(P, tb, L, idx, n_anti, lane);
+        if (!(P.has_jz && P.uniform_jz) || P.has_hz)
+            diag = diag_value<G>(P, L.x, L.anti, n_anti, gl);
+        else
+            diag = (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti);
+    };
+    if constexpr (METHOD == GFMC_METHOD_CONT) {
+        const double beta = step_size;
+        double t = 0.0;
+        while (t < beta) {
+            T lane_sum;
+            T incl = lane_scan<T, G>(P, L, gl, &lane_sum);
+            T R = grp_bcast<G>(incl, G - 1);
+            eL = diag - (double)R;
+            gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
+            double psi = gfmc_u53(r.z, r.w);
+            int idx = -1;
+            double tau_raw = INFINITY;
+            MovePrefetch m;
+            if (R > (T)0) {
+                T u = uniform_for<T>(r) * R;
+                T cap = below(R);
+                if (u > cap) u = cap;
+                idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
+                if constexpr (FAST) m = prefetch_move(P, idx, gl);  // in flight during the hold math
+                tau_raw = -log1p(-psi) / (double)R;
+            }
+            double rem = beta - t;
+            double tau = tau_raw < rem ? tau_raw : rem;
+            lw = lw - eL * tau;
+            t = t + tau;
+            ++it;
+            if (tau_raw < rem && idx >= 0) move(idx, m);
+        }
+        lw = lw - alw * beta;
+    } else {
+        const int n_bra = (int)step_size;
+        if constexpr (METHOD == GFMC_METHOD_SINGLE) {
+            // sub-step: stay with weight w0 = lam - H_xx or hop to slot s
+            // with weight -rate_s; log_w += log(b_x), b_x = w0 + R.  The draw
+            // u in [0, b_x) is tested against w0 first, then u - w0 against
+            // the blocked CDF of the slots.
+            for (; it < (uint32_t)n_bra;) {
+                T lane_sum;
+                T incl = lane_scan<T, G>(P, L, gl, &lane_sum);
+                T R = grp_bcast<G>(incl, G - 1);
+                const double w0d = P.lam - diag;
+                const T w0 = (T)w0d;
+                const T W = w0 + R;
+                gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
+                T u = uniform_for<T>(r) * W;
+                const T cap = below(W);
+                if (u > cap) u = cap;
+                int idx = -1;
+                MovePrefetch m;
+                if (!(u < w0)) {
+                    idx = sample_slot<T, G>(P, L, incl, lane_sum, u - w0, lane);
+                    if constexpr (FAST) m = prefetch_move(P, idx, gl);
+                }
+                lw = lw + log(w0d + (double)R);
+                ++it;
+                if (idx >= 0) move(idx, m);
+            }
+        } else {
+            // hold: k sub-steps at once, P(stay) = 1 - R / b_x per sub-step;
+            // k = min(floor(log1p(-psi) / log1p(-R / b_x)) + 1, remaining + 1)
+            int used = 0;
+            while (used < n_bra) {
+                T lane_sum;
+                T incl = lane_scan<T, G>(P, L, gl, &lane_sum);
+                T R = grp_bcast<G>(incl, G - 1);
+                const double bx = (P.lam - diag) + (double)R;
+                gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
+                const double psi = gfmc_u53(r.z, r.w);
+                int idx = -1;
+                MovePrefetch m;
+                if (R > (T)0) {
+                    T u = uniform_for<T>(r) * R;
+                    T cap = below(R);
+                    if (u > cap) u = cap;
+                    idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
+                    if constexpr (FAST) m = prefetch_move(P, idx, gl);
+                }
+                const int remaining = n_bra - used;
+                const double p_hop = (double)R / bx;
+                double k_raw = (double)remaining + 1.0;
+                if (p_hop > 0.0) {
+                    const double hold = floor(log1p(-psi) / log1p(-p_hop)) + 1.0;
+                    k_raw = hold < k_raw ? hold : k_raw;
+                }
+                const int kr = (int)k_raw;
+                const int k = kr < remaining ? kr : remaining;
+                lw = lw + log(bx) * (double)k;
+                used += k;
+                ++it;
+                if (kr <= remaining && idx >= 0) move(idx, m);
+            }
+        }
+        // e_L of the final configuration: one more evaluation
         T lane_sum;
         T incl = lane_scan<T, G>(P, L, gl, &lane_sum);
         T R = grp_bcast<G>(incl, G - 1);
         eL = diag - (double)R;
-        gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
-        double psi = gfmc_u53(r.z, r.w);
-        int idx = -1;
-        double tau_raw = INFINITY;
-        MovePrefetch m;
-        if (R > (T)0) {
-            T u = uniform_for<T>(r) * R;
-            T cap = below(R);
-            if (u > cap) u = cap;
-            idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
-            if constexpr (FAST) m = prefetch_move(P, idx, gl);  // in flight during the hold math
-            tau_raw = -log1p(-psi) / (double)R;
-        }
-        double rem = beta - t;
-        double tau = tau_raw < rem ? tau_raw : rem;
-        lw = lw - eL * tau;
-        t = t + tau;
         ++it;
-        if (tau_raw < rem && idx >= 0) {
-            if constexpr (FAST)
-                n_anti = apply_move_rec<T, G>(P, tb, L, m, idx, n_anti, lane);
-            else
-                n_anti = apply_move(P, tb, L, idx, n_anti, lane);
-            if (!(P.has_jz && P.uniform_jz) || P.has_hz)
-                diag = diag_value<G>(P, L.x, L.anti, n_anti, gl);
-            else
-                diag = (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti);
-        }
+        lw = lw - alw * (double)n_bra;
     }
-    lw = lw - alw * beta;
     store_spins<G>(P, sp, L.x, gl);
     if (gl == 0) {
         *lw_p = lw;
@@ -707,7 +792,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
 }
 
 // one wavefront per workgroup, 64 / G walkers per wavefront
-template <typename T, bool FAST, int G>
+template <typename T, bool FAST, int G, int METHOD>
 __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
                                                     uint32_t *spins, double *log_w,
                                                     double *e_loc, double beta, double alw,
@@ -720,7 +805,8 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
     if (wk >= nw) return;
     const uint32_t st = step + (step_base ? *step_base : 0u);
     unsigned char *base = G == WAVE ? lds : lds + (size_t)grp * walker_lds_bytes<T, G>(P);
-    propagate_walker<T, FAST, G>(P, tb, base, threadIdx.x, (uint32_t)wk, spins + wk * P.words,
+    propagate_walker<T, FAST, G, METHOD>(P, tb, base, threadIdx.x, (uint32_t)wk,
+                                         spins + wk * P.words,
                                  log_w + wk, e_loc + wk, evals ? evals + wk : nullptr, beta,
                                  alw, key0, key1, st, tag);
 }
@@ -731,7 +817,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
 // spins of every lattice use the common stride `stride` words.  With G < 64
 // every lattice's walker count is a multiple of 64 / G, so the lattice is
 // uniform over a wavefront.
-template <typename T, bool FAST, int G>
+template <typename T, bool FAST, int G, int METHOD>
 __global__ __launch_bounds__(WAVE) void k_propagate_batch(
     const DevProblem *Ps, const Tables<T> *Ts, int n_lat, const int64_t *woff, int64_t n_total,
     int stride, uint32_t *spins, double *log_w, double *e_loc, double beta, const double *alw,
@@ -751,7 +837,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate_batch(
     const uint64_t key = seed + (uint64_t)lo;
     const DevProblem &P = Ps[lo];
     unsigned char *base = G == WAVE ? lds : lds + (size_t)grp * walker_lds_bytes<T, G>(P);
-    propagate_walker<T, FAST, G>(P, Ts[lo], base, threadIdx.x, (uint32_t)(g - woff[lo]),
+    propagate_walker<T, FAST, G, METHOD>(P, Ts[lo], base, threadIdx.x, (uint32_t)(g - woff[lo]),
                                  spins + g * stride, log_w + g, e_loc + g,
                                  evals ? evals + g : nullptr, beta, alw[lo], (uint32_t)key,
                                  (uint32_t)(key >> 32), step, tag);
@@ -1741,29 +1827,44 @@ static void launch_rates(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
                            (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, rates, e_loc, log_diff);
 }
 
+template <typename T, bool FAST, int G>
+static void launch_prop_g(const gfmc_problem *p, const Tables<T> &tb, unsigned grid, size_t lds,
+                          int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
+                          double beta, double alw, uint32_t k0, uint32_t k1, uint32_t step,
+                          uint32_t tag, int32_t *evals, hipStream_t st) {
+    if (p->dp.method == GFMC_METHOD_SINGLE)
+        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_SINGLE>), dim3(grid), dim3(WAVE),
+                           lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
+                           tag, evals, g_step_base);
+    else if (p->dp.method == GFMC_METHOD_MULTI)
+        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_MULTI>), dim3(grid), dim3(WAVE),
+                           lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
+                           tag, evals, g_step_base);
+    else
+        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_CONT>), dim3(grid), dim3(WAVE),
+                           lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
+                           tag, evals, g_step_base);
+}
+
 template <typename T>
 static void launch_propagate(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
                              uint32_t *spins, double *log_w, double *e_loc, double beta,
                              double alw, uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
                              int32_t *evals, hipStream_t st) {
     if (!p->dp.fast)
-        hipLaunchKernelGGL((k_propagate<T, false, 64>), dim3((unsigned)nw), dim3(WAVE),
-                           (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, log_w, e_loc, beta,
-                           alw, k0, k1, step, tag, evals, g_step_base);
+        launch_prop_g<T, false, 64>(p, tb, (unsigned)nw, (size_t)p->lds_bytes, nw, spins, log_w,
+                                    e_loc, beta, alw, k0, k1, step, tag, evals, st);
     else if (p->lanes == 32)
-        hipLaunchKernelGGL((k_propagate<T, true, 32>), dim3((unsigned)((nw + 1) / 2)),
-                           dim3(WAVE), (size_t)(2 * walker_lds_bytes<T, 32>(p->dp)), st, p->dp,
-                           tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
-                           g_step_base);
+        launch_prop_g<T, true, 32>(p, tb, (unsigned)((nw + 1) / 2),
+                                   (size_t)(2 * walker_lds_bytes<T, 32>(p->dp)), nw, spins,
+                                   log_w, e_loc, beta, alw, k0, k1, step, tag, evals, st);
     else if (p->lanes == 16)
-        hipLaunchKernelGGL((k_propagate<T, true, 16>), dim3((unsigned)((nw + 3) / 4)),
-                           dim3(WAVE), (size_t)(4 * walker_lds_bytes<T, 16>(p->dp)), st, p->dp,
-                           tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
-                           g_step_base);
+        launch_prop_g<T, true, 16>(p, tb, (unsigned)((nw + 3) / 4),
+                                   (size_t)(4 * walker_lds_bytes<T, 16>(p->dp)), nw, spins,
+                                   log_w, e_loc, beta, alw, k0, k1, step, tag, evals, st);
     else
-        hipLaunchKernelGGL((k_propagate<T, true, 64>), dim3((unsigned)nw), dim3(WAVE),
-                           (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, log_w, e_loc, beta,
-                           alw, k0, k1, step, tag, evals, g_step_base);
+        launch_prop_g<T, true, 64>(p, tb, (unsigned)nw, (size_t)p->lds_bytes, nw, spins, log_w,
+                                   e_loc, beta, alw, k0, k1, step, tag, evals, st);
 }
 
 // persistent RBM grid: as many workgroups as fit (one per CU at full LDS)
@@ -2066,6 +2167,23 @@ int gfmc_problem_set_lanes(gfmc_problem *p, int32_t lanes) {
     return 0;
 }
 
+int gfmc_problem_set_method(gfmc_problem *p, int32_t method, double lam) {
+    if (!p) return fail(GFMC_EINVAL, "gfmc_problem_set_method: NULL problem");
+    if (method != GFMC_METHOD_CONT && method != GFMC_METHOD_SINGLE &&
+        method != GFMC_METHOD_MULTI)
+        return fail(GFMC_EINVAL, "gfmc_problem_set_method: unknown method");
+    if (method != GFMC_METHOD_CONT) {
+        if (!std::isfinite(lam))
+            return fail(GFMC_EINVAL, "gfmc_problem_set_method: lam must be finite");
+        if (p->dp.guide == GFMC_GUIDE_RBM)
+            return fail(GFMC_EUNSUP, "gfmc_problem_set_method: the RBM guide propagates in "
+                                     "continuous time only");
+    }
+    p->dp.method = method;
+    p->dp.lam = method == GFMC_METHOD_CONT ? 0.0 : lam;
+    return 0;
+}
+
 int gfmc_problem_get_info(const gfmc_problem *p, gfmc_problem_info *out) {
     if (!p || !out) return fail(GFMC_EINVAL, "gfmc_problem_get_info: NULL argument");
     out->n_sites = p->dp.N;
@@ -2189,6 +2307,9 @@ int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
     if (!spins || !log_w || !e_loc) return fail(GFMC_EINVAL, "gfmc_propagate: NULL buffer");
     if (!(beta > 0) || !std::isfinite(beta))
         return fail(GFMC_EINVAL, "gfmc_propagate: beta must be finite and > 0");
+    if (p->dp.method != GFMC_METHOD_CONT && (beta != std::floor(beta) || beta > 1e9))
+        return fail(GFMC_EINVAL, "gfmc_propagate: the discrete methods take an integer "
+                                 "number of sub-steps");
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     if (p->dp.guide == GFMC_GUIDE_RBM)
         hipLaunchKernelGGL(k_propagate_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),

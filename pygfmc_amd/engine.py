@@ -2,10 +2,10 @@
 ``reconfiguration`` (codes/pygfmc/engine/driver.py, evolution.py,
 walkers/reconfiguration.py), driving the HIP kernels.
 
-Method: continuous-time GFMC only (``method="cont_time"``, evolution.py:
-559-721).  The discrete ``single_step`` / ``multi_step`` propagators and the
-diagonal / off-diagonal forward-walking observables are not implemented and
-raise.
+Methods: continuous time (``"cont_time"``, evolution.py:559-721) and the
+discrete ``"single_step"`` / ``"multi_step"`` propagators with the shift
+``lam`` (evolution.py:322-556).  The diagonal / off-diagonal forward-walking
+observables are not implemented and raise.
 """
 from __future__ import annotations
 
@@ -24,7 +24,7 @@ from .walkers import PhiloxKey, Walker, as_key
 
 __all__ = ["run", "run_gfmc", "evolve_walkers", "reconfiguration", "systematic_parents",
            "sample_index", "estimate_average_log_branch_weight", "DEFAULT_WARMUP_STEPS",
-           "Chain", "StepGraph"]
+           "Chain", "StepGraph", "set_method"]
 
 DEFAULT_WARMUP_STEPS = 1000
 
@@ -40,24 +40,53 @@ def require_step_size(method, step_size):
                          f"got step_size={step_size!r}")
 
 
+METHODS = {"cont_time": 0, "single_step": 1, "multi_step": 2}
+
+
 def require_method(method):
+    if method not in METHODS:
+        raise ValueError(f"Unknown method: {method!r}")
+
+
+def require_lam(method, lam):
+    """evolution.py:724-741: a finite float for the discrete methods, None
+    for cont_time."""
     if method == "cont_time":
-        return
-    if method in ("single_step", "multi_step"):
-        raise NotImplementedError(
-            f"method={method!r}: pygfmc_amd implements the continuous-time propagator "
-            "(method='cont_time') only.")
-    raise ValueError(f"Unknown method: {method!r}")
+        return None
+    if lam is None:
+        raise ValueError(
+            f"method={method!r} needs the GFMC shift: pass lam=<float> with lam >= max_x "
+            f"H_xx over the reachable configurations (cont_time is the only method that "
+            f"does without it).")
+    value = float(lam)
+    if not math.isfinite(value):
+        raise ValueError(f"lam must be finite; got lam={lam!r}")
+    return value
 
 
-def warmup_schedule(step_f, warmup_steps):
-    """(step size, count) of the four warm-up ramp stages beta/8, beta/4,
-    beta/2, beta with counts divmod(warmup_steps, 4) (driver.py:567-584)."""
+def set_method(problem, method, lam):
+    """Select the propagator of ``problem``'s kernels (and its shift)."""
+    lam = require_lam(method, lam)
+    _lib.check(_lib.load().gfmc_problem_set_method(problem.handle, METHODS[method],
+                                                   0.0 if lam is None else lam),
+               "gfmc_problem_set_method")
+    return lam
+
+
+def warmup_schedule(step_f, warmup_steps, method="cont_time"):
+    """(step size, count) of the four warm-up ramp stages with counts
+    divmod(warmup_steps, 4): beta/8, beta/4, beta/2, beta, or for the
+    discrete methods max(1, n_bra // 8), ... , n_bra (driver.py:567-584)."""
     if warmup_steps <= 0:
         return []
     base, extra = divmod(warmup_steps, 4)
     counts = [base + (i < extra) for i in range(4)]
-    sizes = [step_f / 8.0, step_f / 4.0, step_f / 2.0, step_f]
+    if method == "cont_time":
+        sizes = [step_f / 8.0, step_f / 4.0, step_f / 2.0, step_f]
+    else:
+        n = int(step_f)
+        sizes = [float(max(1, n // 8)), float(max(1, n // 4)), float(max(1, n // 2)),
+                 float(n)]
     return [(s, c) for s, c in zip(sizes, counts) if c > 0]
 
 
@@ -188,6 +217,7 @@ def evolve_walkers(walkers, hamiltonian, step_size, guide, method, rng_key,
     require_step_size(method, step_size)
     key = as_key(rng_key)
     p = get_problem(hamiltonian, guide, precision)
+    set_method(p, method, lam)
     w = walkers.clone()
     evals = torch.zeros(w.n_walkers, dtype=torch.int32, device=w.spins.device)
     p.propagate(w.spins, w.log_weight_dev, w.e_loc_dev, float(step_size),
@@ -258,24 +288,29 @@ def sample_index(weights, uniforms, precision="fp64"):
 
 def estimate_average_log_branch_weight(walkers, hamiltonian, guide, method="cont_time",
                                        lam=None, precision=None):
-    """mean(-sum rates) over the walkers for cont_time (estimators.py:722-751)."""
+    """mean(-sum rates) for cont_time, mean(log(lam - sum rates)) for the
+    discrete methods (estimators.py:722-751)."""
     require_method(method)
+    lam = require_lam(method, lam)
     p = get_problem(hamiltonian, guide, precision)
     _, e, _ = p.rates(walkers.spins, want_rates=False)
-    return float(torch.mean(-e).item())
+    if lam is None:
+        return float(torch.mean(-e).item())
+    return float(torch.mean(torch.log(lam - e)).item())
 
 
 # -- the driver ----------------------------------------------------------------
 
-def _warmup(chain, key, step_f, alw, warmup_steps, verbosity):
+def _warmup(chain, key, step_f, alw, warmup_steps, verbosity, method="cont_time", lam=None):
     """Four-stage ramp; the shift is updated per stage unless pinned
     (driver.py:713-788): alw <- alw + (LSE(log G_stage) - log n) / step."""
     pinned = alw is not None
     if not pinned:
         _, e, _ = chain.p.rates(chain.spins, want_rates=False)
-        alw = float(torch.mean(-e).item())
+        alw = float(torch.mean(-e).item() if lam is None else
+                    torch.mean(torch.log(lam - e)).item())
     counter = key.position
-    for i, (step_, n_w) in enumerate(warmup_schedule(step_f, warmup_steps)):
+    for i, (step_, n_w) in enumerate(warmup_schedule(step_f, warmup_steps, method)):
         t0 = time.time()
         series = chain.steps(n_w, step_, alw, key.seed, counter, warmup=True)
         counter += n_w
@@ -340,6 +375,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
                          "measures it.")
     step_f = float(step_size)
     p = get_problem(hamiltonian, guide, precision)
+    lam_v = set_method(p, method, lam)
     chain = Chain(p, walkers)
     tune_times = None
     if lanes == "auto":
@@ -348,7 +384,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     else:
         p.set_lanes(int(lanes))
     alw_given = avg_log_weight
-    alw = _warmup(chain, key, step_f, avg_log_weight, warmup_steps, verbosity)
+    alw = _warmup(chain, key, step_f, avg_log_weight, warmup_steps, verbosity, method, lam_v)
 
     print_every = max(1, n_steps // max(1, number_of_outputs))
     dev = chain.spins.device

@@ -128,7 +128,7 @@ def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, s
     update), ``seconds`` (GPU wall time), ``walker_steps`` and ``trainer``.
     """
     key = as_key(rng_key)
-    from .engine import Chain
+    from .engine import Chain, set_method
     from .problem import get_problem
     if trainer is None:
         p = Problem(hamiltonian, rbm, _lib.GFMC_FP32)
@@ -140,6 +140,7 @@ def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, s
     tr = trainer
     ps = p if (isinstance(sampler, str) and sampler == "self") else get_problem(hamiltonian,
                                                                                 sampler)
+    set_method(ps, "cont_time", None)
     chain = Chain(ps, walkers)
     dev = chain.spins.device
     hist = torch.zeros((max(n_iter, 1), 2), dtype=torch.float64, device=dev)

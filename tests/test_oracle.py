@@ -441,3 +441,88 @@ def test_group_tree_differs_from_wave_tree_on_mixed_magnitudes(orc):
         with orc.lanes(G):
             res[G] = orc.blocked_scan(v)[0]
     assert len({res[16], res[32], res[64]}) == 3
+
+
+# -- discrete propagators (single_step / multi_step) ----------------------------------
+
+def _discrete_system():
+    from pygfmc_amd import JastrowSpinHalf, SpinHalf, chain
+    L = chain(6)
+    H = SpinHalf.create(L.edges, -1.0, -0.7, 0.8, h_x=0.3, dtype=np.float64)
+    G = JastrowSpinHalf(6, L.edges, beta=0.2, h=0.05, dtype=np.float64)
+    return H, G
+
+
+def test_oracle_single_step_draws_the_column_of_lambda_minus_h(orc):
+    """One sub-step from a fixed configuration: the successor frequencies
+    match (lam delta_yx - H_yx psi(y)/psi(x)) / b_x from the Kronecker
+    matrix, and log_w = log(b_x) exactly, b_x = lam - E_L(x)."""
+    H, G = _discrete_system()
+    M = spinhalf_matrix(H).toarray()
+    B = basis(6)
+    lp = jastrow_log_psi(G, B)
+    lam = 4.0
+    x = np.array([[1, -1, -1, 1, 1, -1]], dtype=float)
+    col = index_of(x)[0]
+    P = (lam * (np.arange(64) == col) - M[:, col]) * np.exp(lp - lp[col])
+    bx = P.sum()
+    n = 40000
+    spins = np.repeat(orc.pack(x), n, axis=0)
+    s, lw, e, ev = orc.Problem(lower(H, G)).propagate(spins, np.zeros(n), np.zeros(n), 1, 0.0,
+                                                      77, 3, 0, method=1, lam=lam)
+    np.testing.assert_allclose(lw, np.log(bx), rtol=1e-13)
+    assert np.all(ev == 2)
+    idx = index_of(orc.unpack(s, 6))
+    freq = np.bincount(idx, minlength=64) / n
+    p = P / bx
+    assert np.all(np.abs(freq - p) <= 5 * np.sqrt(p * (1 - p) / n) + 1e-12)
+    # e_loc is the local energy of the successor
+    e_ref = (M[:, idx] * np.exp(lp[:, None] - lp[idx][None, :])).sum(0)
+    np.testing.assert_allclose(e, e_ref, rtol=1e-12)
+
+
+def test_oracle_multi_step_matches_single_step_in_distribution(orc):
+    """Batched geometric holds are the same Markov chain: after n_bra = 4
+    sub-steps from a fixed configuration, the successor distribution and the
+    weight sum per successor agree between the two discrete methods."""
+    H, G = _discrete_system()
+    prob = orc.Problem(lower(H, G))
+    x = np.array([[1, 1, -1, -1, 1, -1]], dtype=float)
+    n, lam = 60000, 3.5
+    spins = np.repeat(orc.pack(x), n, axis=0)
+    out = {}
+    for method in (1, 2):
+        s, lw, _, ev = prob.propagate(spins, np.zeros(n), np.zeros(n), 4, 0.0, 5, 1, 0,
+                                      method=method, lam=lam)
+        idx = index_of(orc.unpack(s, 6))
+        out[method] = (np.bincount(idx, minlength=64) / n,
+                       np.bincount(idx, weights=np.exp(lw), minlength=64) / n, ev)
+    f1, w1, ev1 = out[1]
+    f2, w2, ev2 = out[2]
+    assert np.all(np.abs(f1 - f2) <= 6 * np.sqrt(f1 * (1 - f1) / n) + 1e-12)
+    np.testing.assert_allclose(w1.sum(), w2.sum(), rtol=0.01)
+    assert ev2.mean() < ev1.mean()      # the holds skip the stays
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_oracle_discrete_gfmc_energy_matches_ed(orc, method):
+    """chain(8) Heisenberg without a guide: the oracle's branching chain with
+    the discrete propagators reaches the exact ground energy."""
+    import warnings
+    from helpers import ground_energy
+    from pygfmc_amd import SpinHalf, chain
+    from pygfmc_amd.estimators import evaluate_observables
+    H = SpinHalf.create(chain(8).edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    e0 = ground_energy(H)
+    prob = orc.Problem(lower(H, None))
+    spins = orc.init_sz(8, 256, 4, 3)
+    lam = 2.0
+    s, _, _ = orc.run_chain(prob, spins, 60, 6, np.log(lam - e0), 9, 0, warmup=True,
+                            method=method, lam=lam)
+    _, _, ser = orc.run_chain(prob, s, 600, 6, np.log(lam - e0), 9, 100, method=method,
+                              lam=lam)
+    raw = dict(log_weights=ser[:, 0], energy=ser[:, 1], p_array=[4, 16], m_array=[2, 8])
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        E, _, err, _ = evaluate_observables(raw, n_blocks=20)
+    assert abs(E[16] - e0) < 4 * err[16] + 1e-3
