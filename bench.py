@@ -242,6 +242,7 @@ def main():
     if not args.no_configs:
         out["heisenberg_chain100"] = chain100_section()
         out["batched_tfim_chains"] = tfim_batch_section()
+        out["published_throughput_config"] = fig3_section()
     if not args.no_race:
         out["sigma_race"] = sigma_race(args)
     print(json.dumps(out), flush=True)
@@ -265,6 +266,42 @@ def chain100_section():
             "z": (E / 100 + 0.4432296) / (err / 100) if err > 0 else None,
             "lanes_per_walker": st["parameters/lanes_per_walker"],
             "evals_per_step": st["evals_per_step_mean"]}
+
+
+# The reference's published throughput (paper Fig. 3 data, it/s x walkers):
+# L = 100 Heisenberg ring, no guide, Neel start, single_step, n_bra = 6,
+# lam = 25; walker-steps/s on one GPU.
+FIG3_PUBLISHED = {65536: {"A100": 356.04 * 65536, "H200": 554.18 * 65536},
+                  2097152: {"A100": 14.234 * 2097152, "H200": 27.647 * 2097152}}
+
+
+def fig3_section():
+    """The reference's own throughput benchmark configuration (Heisenberg
+    ring L = 100, J_xy = -1, J_z = 1, no guide, Neel initial state,
+    method single_step, n_bra = 6, lam = 25, FP32 rates) at 65 536 and
+    2 097 152 walkers, against its published one-GPU A100 / H200 numbers.
+    Shorter than the published protocol (10 000 + 10 000 steps): 2000 warm-up
+    + 3000 timed steps at 65 536 walkers, 200 + 300 at 2 097 152."""
+    from pygfmc_amd import SpinHalf, Walker, chain, prepare_walkers_from_configs, run
+    lat = chain(100)
+    H = SpinHalf.create(lat.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    neel = np.tile([1.0, -1.0], 50)
+    out = {"workload": "heisenberg_chain100_noguide_neel_single_step_nbra6_lam25_fp32"}
+    for nw, warm, steps in ((65536, 2000, 3000), (2097152, 200, 300)):
+        one = prepare_walkers_from_configs(neel[None])
+        w = Walker.from_device(one.spins.repeat(nw, 1).contiguous(),
+                               one.log_weight_dev.repeat(nw), one.e_loc_dev.repeat(nw), 100)
+        _, E, _, st, _ = run(w, steps, H, 6, None, [4], method="single_step", lam=25.0,
+                             rng_key=1, warmup_steps=warm, verbosity=0, number_of_outputs=10)
+        v = nw * st["it_per_sec"]
+        pub = FIG3_PUBLISHED[nw]
+        out[f"w{nw}"] = {"walker_steps_per_s": v, "it_per_sec": st["it_per_sec"],
+                         "steps": steps, "warmup": warm, "E_per_site_p4": E[4] / 100,
+                         "lanes_per_walker": st["parameters/lanes_per_walker"],
+                         "published_A100": pub["A100"], "published_H200": pub["H200"],
+                         "vs_A100": v / pub["A100"], "vs_H200": v / pub["H200"]}
+        del w
+    return out
 
 
 def tfim_batch_section():
