@@ -321,6 +321,20 @@ int gfmc_graph_capture_batch_steps(const gfmc_batch *b, uint32_t *spins, double 
 int gfmc_graph_launch(gfmc_graph *g, void *stream);
 int gfmc_graph_destroy(gfmc_graph *g);
 
+/* ---- The p_array energy estimator on the device -------------------------
+ * _energy_and_diagonal_estimates (estimators.py:268-305): E_p = sum_i
+ * G_p(i) E(i+p) / sum_i G_p(i), log G_p(i) = C[i+p] - C[i], C the cumulative
+ * sum of the steps' log G.  Streamed over the series rows a step loop
+ * writes: a ring buffer of the last p_max + 1 values of C and, per p, a
+ * running maximum with rescaled sums.  state: gfmc_fw_state_doubles device
+ * doubles, [0] steps consumed, [2] 1 once a non-finite value was seen; read
+ * E_p = num_p / den_p from the last 2 n_p entries ([len - 2 n_p + j] and
+ * [len - n_p + j]).  p_list: n_p (1..64) device int32 values in 1..p_max. */
+int64_t gfmc_fw_state_doubles(int32_t n_p, int32_t p_max);
+int gfmc_fw_init(double *state, int32_t n_p, int32_t p_max, void *stream);
+int gfmc_fw_update(const int32_t *p_list, int32_t n_p, int32_t p_max, const double *rows,
+                   int32_t n_rows, double *state, void *stream);
+
 /* ---- RBM guide training ------------------------------------------------
  * No counterpart in the reference, which trains guides by NetKet VMC energy
  * minimisation (train_with_netket, generic_netket_training.py:331-473):

@@ -123,6 +123,10 @@ SIGNATURES = [
                                                       ctypes.POINTER(_vp)]),
     ("gfmc_graph_launch", ctypes.c_int, [_vp, _vp]),
     ("gfmc_graph_destroy", ctypes.c_int, [_vp]),
+    ("gfmc_fw_state_doubles", ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    ("gfmc_fw_init", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    ("gfmc_fw_update", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32,
+                                      _vp, _vp]),
     ("gfmc_rbm_n_params", ctypes.c_int64, [_vp]),
     ("gfmc_rbm_set_params", ctypes.c_int, [_vp, _vp, _vp]),
     ("gfmc_rbm_theta", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),

@@ -2452,3 +2452,4 @@ int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32
 #include "gfmc_train.inl"
 #include "gfmc_batch.inl"
 #include "gfmc_graph.inl"
+#include "gfmc_estim.inl"

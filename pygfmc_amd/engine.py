@@ -396,6 +396,8 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     step0 = 0
     host_chunks = []
     graph = None
+    from .estimators import DeviceEnergyEstimator
+    fw = DeviceEnergyEstimator(p_sorted, dev)
     while step0 < n_steps or pending is not None:
         if step0 < n_steps:
             n = min(print_every, n_steps - step0)
@@ -407,6 +409,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
             else:
                 chain.steps(n, step_f, alw, key.seed, key.position + step0,
                             series=series[step0:step0 + n], evals=True)
+            fw.update(series[step0:step0 + n])
             ev = chain.evals.to(torch.float64).mean() / n
             nxt = (step0, n, ev)
             step0 += n
@@ -427,6 +430,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     arr = np.concatenate(host_chunks, axis=0)
     log_weights, E = arr[:, 0].copy(), arr[:, 1].copy()
     energy = estimate_energy(p_sorted, log_weights, E)
+    energy_device = fw.result()
     key_next = key.advance(n_steps)
     stats = {
         "avg_log_weight": float(alw if alw_given is None else alw_given),
@@ -435,6 +439,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         "total_wall_time": time.time() - wall_start,
         "number_of_devices": 1,
         "hip_graph": graph is not None,
+        "energy_device": energy_device,
         "autotune_seconds": tune_times,
         "evals_per_step_mean": evals_sum / max(1, evals_n),
         "parameters/n_steps": n_steps,

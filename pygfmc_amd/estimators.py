@@ -10,7 +10,7 @@ import warnings
 
 import numpy as np
 
-__all__ = ["validated_p_array", "estimate_energy", "evaluate_observables",
+__all__ = ["validated_p_array", "estimate_energy", "evaluate_observables", "DeviceEnergyEstimator",
            "validated_int", "check_population"]
 
 
@@ -142,3 +142,46 @@ def check_population(log_G_chunk, step0, E_chunk=None):
                 "of the local energies is not finite while the branch weights are, i.e. a "
                 "walker's local energy overflowed; check the guide and the Hamiltonian on "
                 "the configurations the walk reaches.")
+
+
+class DeviceEnergyEstimator:
+    """The windowed energy estimator of :func:`estimate_energy`, streamed on
+    the GPU over the device series (gfmc_fw_update): a ring buffer of the
+    last max(p) + 1 cumulative log normalisation factors and, per p, a
+    running maximum with rescaled sums.  Agrees with the host formula to
+    rounding (the maximum is running instead of global)."""
+
+    def __init__(self, p_array, device):
+        import torch
+
+        from . import _lib
+        self.p = validated_p_array(p_array)
+        if len(self.p) > 64:
+            raise ValueError("at most 64 p values")
+        self.p_max = int(self.p[-1])
+        n = int(_lib.load().gfmc_fw_state_doubles(len(self.p), self.p_max))
+        self.state = torch.empty(n, dtype=torch.float64, device=device)
+        self.p_dev = torch.as_tensor(np.asarray(self.p, dtype=np.int32), device=device)
+        _lib.check(_lib.load().gfmc_fw_init(_lib.ptr(self.state), len(self.p), self.p_max,
+                                            _lib.stream()), "gfmc_fw_init")
+
+    def update(self, rows):
+        """Consume (n, 2) device rows (log G - log n, E) of consecutive steps."""
+        from . import _lib
+        if rows.shape[0] == 0:
+            return
+        _lib.check(_lib.load().gfmc_fw_update(_lib.ptr(self.p_dev), len(self.p), self.p_max,
+                                              _lib.ptr(rows), int(rows.shape[0]),
+                                              _lib.ptr(self.state), _lib.stream()),
+                   "gfmc_fw_update")
+
+    def result(self):
+        """{p: E_p} (host); raises ValueError when a non-finite value was
+        consumed."""
+        st = self.state.cpu().numpy()
+        if st[2] != 0.0:
+            raise ValueError("Nonfinite log weights or energies in the streamed series.")
+        n_p = len(self.p)
+        num, den = st[-2 * n_p:-n_p], st[-n_p:]
+        return {p: float(num[j] / den[j]) for j, p in enumerate(self.p)}
+
