@@ -10,6 +10,7 @@ from .engine import (DEFAULT_WARMUP_STEPS, evolve_walkers, estimate_average_log_
 from .estimators import estimate_energy, evaluate_observables
 from .guides import JastrowSpinHalf, RBMSpinHalf
 from .batch import Batch, run_batch
+from .checkpoint import load_raw, read_checkpoint, run_from_checkpoint, write_checkpoint
 from .hamiltonian import SpinHalf, tfim_chain_ground_energy, transverse_field_ising
 from .lattice import Lattice, chain, kagome, square
 from .training import train_variance, vmin_gradient
@@ -21,7 +22,8 @@ __version__ = "0.1.0"
 __all__ = ["run", "run_gfmc", "evolve_walkers", "reconfiguration", "systematic_parents",
            "sample_index", "estimate_average_log_branch_weight", "estimate_energy",
            "evaluate_observables", "SpinHalf", "JastrowSpinHalf", "RBMSpinHalf", "train_variance",
-           "vmin_gradient", "run_batch", "Batch", "transverse_field_ising",
+           "vmin_gradient", "run_batch", "Batch", "write_checkpoint", "read_checkpoint",
+           "load_raw", "run_from_checkpoint", "transverse_field_ising",
            "tfim_chain_ground_energy", "Lattice", "chain", "square",
            "kagome", "Walker", "PhiloxKey", "prepare_walkers_sz", "prepare_walkers_random",
            "prepare_walkers_from_configs", "DEFAULT_WARMUP_STEPS"]

@@ -1,0 +1,153 @@
+"""Checkpoint / resume of a GFMC chain and its raw series.
+
+The reference stores runs in HDF5 (write_to_HDF5 / run_from_HDF5 /
+load_raw_from_HDF5, engine/io.py:370-1473).  h5py is not part of this
+engine's environment, so the same information goes to one NumPy ``.npz``
+archive (no pickled objects; ``numpy.load(allow_pickle=False)`` reads it):
+
+* ``parameters/*``: what must match for a continuation to be the same
+  calculation (step size, p_array, method, lam, precision, lanes per
+  walker, sites, walkers, and SHA-1 fingerprints of the Hamiltonian and the
+  guide);
+* ``current_walkers/{spins,log_weight,e_loc}``: the ensemble after the
+  newest run (bit-packed spins);
+* ``run_<n>/{log_weights,energy,rng_key,rng_key_next,avg_log_weight}`` per
+  appended run.
+
+``run_from_checkpoint`` continues the stored chain without a warm-up, with
+the stored shift, walkers and Philox position: the series of a run split
+into a run plus a continuation is bit-identical to the uninterrupted run.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+
+import numpy as np
+import torch
+
+from .walkers import Walker
+
+__all__ = ["write_checkpoint", "read_checkpoint", "load_raw", "run_from_checkpoint"]
+
+MUST_MATCH = ("step_size", "p_array", "method", "lam", "precision", "n_sites", "n_walkers",
+              "hamiltonian_sha1", "guide_sha1")
+
+
+def _sha1(obj):
+    if obj is None:
+        return "none"
+    return hashlib.sha1(obj.fingerprint()).hexdigest()
+
+
+def _read(filename):
+    if not os.path.exists(filename):
+        return {}
+    with np.load(filename, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _runs(data):
+    return sorted({int(k.split("/")[0][4:]) for k in data if k.startswith("run_")})
+
+
+def write_checkpoint(filename, stats, walkers, hamiltonian, guide=None, raw=None):
+    """Append the run described by ``stats`` / ``raw`` (as returned by
+    ``run``) and replace the stored ensemble by ``walkers``.  The first write
+    records the parameters; later writes must match them."""
+    params = {
+        "step_size": np.float64(stats["parameters/step_size"]),
+        "p_array": np.asarray(stats["parameters/p_array"], dtype=np.int64),
+        "method": np.asarray(stats["parameters/method"]),
+        "lam": np.float64(stats["parameters/lam"]),
+        "precision": np.asarray(stats["parameters/precision"]),
+        "lanes_per_walker": np.int64(stats["parameters/lanes_per_walker"]),
+        "n_sites": np.int64(walkers.n_sites),
+        "n_walkers": np.int64(walkers.n_walkers),
+        "hamiltonian_sha1": np.asarray(_sha1(hamiltonian)),
+        "guide_sha1": np.asarray(_sha1(guide)),
+    }
+    data = _read(filename)
+    if data:
+        for k in MUST_MATCH:
+            old, new = data[f"parameters/{k}"], params[k]
+            same = (np.array_equal(old, new) or
+                    (np.asarray(old).dtype.kind == "f" and np.isnan(old) and np.isnan(new)))
+            if not same:
+                raise ValueError(f"{filename}: parameter {k!r} differs from the stored run "
+                                 f"({old!r} vs {new!r}); a continuation must be the same "
+                                 f"calculation")
+    for k, v in params.items():
+        data[f"parameters/{k}"] = v
+    n = (max(_runs(data)) + 1) if _runs(data) else 0
+    if raw is not None:
+        data[f"run_{n}/log_weights"] = np.asarray(raw["log_weights"], dtype=np.float64)
+        data[f"run_{n}/energy"] = np.asarray(raw["energy"], dtype=np.float64)
+    data[f"run_{n}/rng_key"] = np.asarray(stats["parameters/rng_key"], dtype=np.uint64)
+    data[f"run_{n}/rng_key_next"] = np.asarray(stats["parameters/rng_key_next"],
+                                               dtype=np.uint64)
+    data[f"run_{n}/avg_log_weight"] = np.float64(stats["avg_log_weight"])
+    data[f"run_{n}/n_steps"] = np.int64(stats["parameters/n_steps"])
+    data["current_walkers/spins"] = walkers.spins.cpu().numpy().view(np.uint32)
+    data["current_walkers/log_weight"] = walkers.log_weight
+    data["current_walkers/e_loc"] = walkers.e_loc
+    tmp = filename + ".tmp.npz"
+    np.savez(tmp, **data)
+    os.replace(tmp, filename)
+    return n
+
+
+def read_checkpoint(filename):
+    """The stored arrays as a flat dict (``parameters/*``,
+    ``current_walkers/*``, ``run_<n>/*``)."""
+    data = _read(filename)
+    if not data:
+        raise FileNotFoundError(filename)
+    return data
+
+
+def load_raw(filename):
+    """The raw series of every stored run, concatenated in run order
+    (load_raw_from_HDF5, engine/io.py:742)."""
+    data = read_checkpoint(filename)
+    runs = [n for n in _runs(data) if f"run_{n}/log_weights" in data]
+    if not runs:
+        raise ValueError(f"{filename} holds no raw series")
+    lw = np.concatenate([data[f"run_{n}/log_weights"] for n in runs])
+    E = np.concatenate([data[f"run_{n}/energy"] for n in runs])
+    p = [int(v) for v in data["parameters/p_array"]]
+    return dict(log_weights=lw, energy=E, p_array=p, m_array=[q // 2 for q in p])
+
+
+def run_from_checkpoint(filename, hamiltonian, guide=None, n_steps=None, **run_kwargs):
+    """Continue the stored chain (run_from_HDF5, engine/io.py:1303-1473):
+    stored walkers, shift and Philox position, no warm-up; step size,
+    p_array, method, lam and lanes per walker come from the file.  The
+    Hamiltonian and guide are checked against the stored fingerprints."""
+    from .engine import run_gfmc
+    data = read_checkpoint(filename)
+    if str(data["parameters/hamiltonian_sha1"]) != _sha1(hamiltonian):
+        raise ValueError(f"{filename}: the Hamiltonian differs from the stored one")
+    if str(data["parameters/guide_sha1"]) != _sha1(guide):
+        raise ValueError(f"{filename}: the guide differs from the stored one")
+    for k in ("step_size", "p_array", "method", "lam", "lanes", "precision", "rng_key",
+              "avg_log_weight", "warmup_steps"):
+        if k in run_kwargs:
+            raise ValueError(f"{k!r} comes from the checkpoint and cannot be overridden")
+    last = max(_runs(data))
+    if n_steps is None:
+        n_steps = int(data[f"run_{last}/n_steps"])
+    N = int(data["parameters/n_sites"])
+    spins = torch.as_tensor(data["current_walkers/spins"].view(np.int32), device="cuda")
+    w = Walker.from_device(spins,
+                           torch.as_tensor(data["current_walkers/log_weight"], device="cuda"),
+                           torch.as_tensor(data["current_walkers/e_loc"], device="cuda"), N)
+    method = str(data["parameters/method"])
+    lam = float(data["parameters/lam"])
+    key = tuple(int(v) for v in data[f"run_{last}/rng_key_next"])
+    return run_gfmc(w, n_steps, hamiltonian, float(data["parameters/step_size"]), guide,
+                    [int(v) for v in data["parameters/p_array"]], method=method,
+                    lam=None if np.isnan(lam) else lam, rng_key=key,
+                    avg_log_weight=float(data[f"run_{last}/avg_log_weight"]), warmup_steps=0,
+                    lanes=int(data["parameters/lanes_per_walker"]),
+                    precision=str(data["parameters/precision"]), **run_kwargs)
