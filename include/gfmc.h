@@ -376,6 +376,95 @@ int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *
                    int64_t step, float lr, float beta1, float beta2, float eps,
                    void *stream);
 
+/* ---- Diagonal operators measured by forward walking ---------------------
+ * The diagonal half of the production step (driver.py:240-288,
+ * _diag_ring_buffers :810-838): after each step's resampling, every diagonal
+ * operator is evaluated on the resampled walkers and stored, with the
+ * step's parent indices, in a ring of m_max + 1 slots; for each window p
+ * (m = p / 2) every walker's ancestor m steps back is found by following
+ * the stored parents, and the walker mean of the ancestors' stored values
+ * becomes the step's row.  Slots not yet written hold identity parents and
+ * zero values, so a row whose lag reaches before the ring's first step is
+ * zero there, as in the reference.
+ *
+ * An operator is lowered to value columns (a scalar operator is one column,
+ * a structure factor one column per wavevector):
+ *   GFMC_DIAG_PAIR: sum_e c_e s_i(e) s_j(e) + sum_k a_k s_k + c
+ *     (SpinHalf.diagonal_weight, SpinHalf.py:291-314: c_e = J_z/4, a_k =
+ *      -h_z/2; when every c_e of a column is equal the value is evaluated as
+ *      c_e * (n_parallel - n_antiparallel), the reference's uniform form);
+ *   GFMC_DIAG_SF: ((sum_k a_k s_k)^2 + (sum_k b_k s_k)^2) / 4 - c
+ *     (SzSzStructureFactor.diagonal_weight, observables.py:162-165: a = cos,
+ *      b = sin of k.r, c = N/4);
+ * with s = +-1.  Values and means are FP64. */
+#define GFMC_DIAG_PAIR 0
+#define GFMC_DIAG_SF 1
+
+typedef struct {
+    int32_t n_sites;
+    int32_t n_cols;            /* 1..256 value columns                       */
+    const int32_t *kind;       /* [n_cols] GFMC_DIAG_PAIR / GFMC_DIAG_SF      */
+    const int32_t *edge_off;   /* [n_cols + 1] column c owns edges            */
+                               /* edge_off[c] .. edge_off[c+1]-1 (PAIR only)  */
+    const int32_t *edge_i;     /* [edge_off[n_cols]] sites, 0..n_sites-1      */
+    const int32_t *edge_j;
+    const double *edge_c;      /* [edge_off[n_cols]]                          */
+    const double *a;           /* [n_cols][n_sites]                           */
+    const double *b;           /* [n_cols][n_sites] (read for SF columns)     */
+    const double *c;           /* [n_cols]                                    */
+    int32_t n_p;               /* 1..64 windows                               */
+    const int32_t *m;          /* [n_p] lags p / 2, non-decreasing, >= 0      */
+} gfmc_diag_desc;
+
+typedef struct gfmc_diag gfmc_diag;
+
+/* Copy the (host) tables to the device.  The handle is read-only after. */
+int gfmc_diag_create(const gfmc_diag_desc *d, gfmc_diag **out);
+int gfmc_diag_destroy(gfmc_diag *d);
+
+/* Bytes of the caller-owned ring for n_walkers (the step counter, the
+ * (m_max + 1) x n_walkers parents, the (m_max + 1) x n_walkers x n_cols
+ * values) and doubles of the per-step workspace. */
+int64_t gfmc_diag_ring_bytes(const gfmc_diag *d, int64_t n_walkers);
+int64_t gfmc_diag_work_doubles(const gfmc_diag *d, int64_t n_walkers);
+
+/* Reset the ring: counter 0, identity parents, zero values. */
+int gfmc_diag_ring_init(const gfmc_diag *d, int64_t n_walkers, void *ring, void *stream);
+
+/* values [n_walkers][n_cols] <- every column on every packed walker (the
+ * operators' diagonal_weight). */
+int gfmc_diag_values(const gfmc_diag *d, int64_t n_walkers, const uint32_t *spins,
+                     double *values, void *stream);
+
+/* One step's measurement after resampling: store the values of the
+ * resampled walkers spins and their parents, walk the ancestry, write the
+ * walker means to row [n_cols][n_p], advance the ring's counter.  work:
+ * gfmc_diag_work_doubles doubles.  Deterministic (fixed-order sums). */
+int gfmc_diag_record(const gfmc_diag *d, int64_t n_walkers, const uint32_t *spins,
+                     const int32_t *parents, void *ring, double *row, double *work,
+                     void *stream);
+
+/* gfmc_run_steps with gfmc_diag_record after every step's resampling:
+ * diag_series [n_steps][n_cols][n_p]; the ring continues across calls. */
+int gfmc_run_steps_diag(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
+                        double *log_w, double *e_loc, uint32_t *spins_b, double *e_loc_b,
+                        int32_t *parents, double *work, int32_t n_steps, double beta,
+                        double alw, uint64_t seed, uint32_t step0, uint32_t tag_prop,
+                        uint32_t tag_reconf, double *series, int32_t *evals,
+                        const gfmc_diag *d, void *ring, double *diag_series,
+                        double *diag_work, void *stream);
+
+/* gfmc_graph_capture_steps over gfmc_run_steps_diag (the ring's counter is
+ * device state, so replays continue it). */
+int gfmc_graph_capture_steps_diag(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
+                                  double *log_w, double *e_loc, uint32_t *spins_b,
+                                  double *e_loc_b, int32_t *parents, double *work,
+                                  int32_t n_steps, double beta, double alw, uint64_t seed,
+                                  uint32_t tag_prop, uint32_t tag_reconf, double *series,
+                                  int32_t *evals, const gfmc_diag *d, void *ring,
+                                  double *diag_series, double *diag_work, uint32_t *counter,
+                                  gfmc_graph **out);
+
 #ifdef __cplusplus
 }
 #endif

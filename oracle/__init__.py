@@ -41,6 +41,16 @@ class Desc(ctypes.Structure):
     ]
 
 
+class DiagDesc(ctypes.Structure):
+    """Layout of gfmc_diag_desc (include/gfmc.h)."""
+    _fields_ = [
+        ("n_sites", ctypes.c_int32), ("n_cols", ctypes.c_int32),
+        ("kind", _i32p), ("edge_off", _i32p), ("edge_i", _i32p), ("edge_j", _i32p),
+        ("edge_c", _f64p), ("a", _f64p), ("b", _f64p), ("c", _f64p),
+        ("n_p", ctypes.c_int32), ("m", _i32p),
+    ]
+
+
 _LIB = None
 
 
@@ -82,6 +92,7 @@ def lib():
         L.orc_rbm_vmin_grad.argtypes = [D, ctypes.c_int64, _u32p, ctypes.c_double, _f64p,
                                         _f64p]
         L.orc_rbm_theta_batch.argtypes = [D, ctypes.c_int64, _u32p, _f64p]
+        L.orc_diag_values.argtypes = [ctypes.POINTER(DiagDesc), ctypes.c_int64, _u32p, _f64p]
         del vp
         _LIB = L
     return _LIB
@@ -281,6 +292,85 @@ def run_chain(prob, spins, n_steps, beta, alw, seed, step0, warmup=False, n_thre
         spins, e = gather(prob.N, spins, e, par)
         lw = np.zeros(nw)
     return spins, e, series
+
+
+class Diag:
+    """Diagonal operator columns (the arrays of a gfmc_diag_desc: kind,
+    edge_off, edge_i, edge_j, edge_c, a, b, c, N), as a descriptor."""
+
+    def __init__(self, cols):
+        self.cols = cols
+        self.keep = [np.ascontiguousarray(getattr(cols, k), dtype=dt) for k, dt in
+                     (("kind", np.int32), ("edge_off", np.int32), ("edge_i", np.int32),
+                      ("edge_j", np.int32), ("edge_c", np.float64), ("a", np.float64),
+                      ("b", np.float64), ("c", np.float64))]
+        kind, eoff, ei, ej, ec, a, b, c = self.keep
+        d = DiagDesc()
+        d.n_sites = int(cols.N)
+        d.n_cols = len(kind)
+        d.kind, d.edge_off = _p(kind, ctypes.c_int32), _p(eoff, ctypes.c_int32)
+        d.edge_i, d.edge_j = _p(ei, ctypes.c_int32), _p(ej, ctypes.c_int32)
+        d.edge_c, d.a = _p(ec, ctypes.c_double), _p(a, ctypes.c_double)
+        d.b, d.c = _p(b, ctypes.c_double), _p(c, ctypes.c_double)
+        self.desc = d
+        self.N = int(cols.N)
+        self.V = len(kind)
+
+    def values(self, spins):
+        """(nw, n_cols) column values of packed walkers."""
+        s = np.ascontiguousarray(spins, dtype=np.uint32)
+        out = np.empty((s.shape[0], self.V))
+        lib().orc_diag_values(ctypes.byref(self.desc), s.shape[0], _p(s, ctypes.c_uint32),
+                              _p(out, ctypes.c_double))
+        return out
+
+
+def diag_chain(prob, diag, m_list, spins, n_steps, beta, alw, seed, step0, lanes=64, method=0,
+               lam=0.0, history=False):
+    """run_chain with the forward-walking record of driver.py:240-288 after
+    every resampling: a ring of R = max(m) + 1 slots of parents (initially
+    the identity) and column values (initially zero); per step the ring slot
+    buf gets the parents and the values of the resampled walkers, each
+    walker's ancestor m_j steps back is followed through slots buf, buf-1,
+    ..., and the walker mean of the ancestors' values at slot buf - m_j is
+    row[:, j].  Returns (spins, e_loc, series, rows[n_steps, n_cols, n_p])
+    and, with ``history``, the per-step parents and values."""
+    m_list = [int(m) for m in m_list]
+    R = max(m_list) + 1
+    nw = spins.shape[0]
+    ring_par = np.tile(np.arange(nw, dtype=np.int64), (R, 1))
+    ring_val = np.zeros((R, nw, diag.V))
+    buf = 0
+    lw = np.zeros(nw)
+    e = np.zeros(nw)
+    series = np.empty((n_steps, 2))
+    rows = np.empty((n_steps, diag.V, len(m_list)))
+    pars, vals = [], []
+    for t in range(n_steps):
+        spins, lw, e, _ = prob.propagate(spins, lw, e, beta, alw, seed, step0 + t, 0,
+                                         lanes=lanes, method=method, lam=lam)
+        psi = reconfig_psi(seed, step0 + t, 1)
+        par, lg, En, _ = reconfigure(lw, e, psi)
+        series[t] = (lg, En)
+        spins, e = gather(prob.N, spins, e, par)
+        lw = np.zeros(nw)
+        v = diag.values(spins)
+        if history:
+            pars.append(par.copy())
+            vals.append(v)
+        ring_par[buf] = par
+        ring_val[buf] = v
+        anc = np.arange(nw)
+        k = 0
+        for j, m in enumerate(m_list):
+            while k < m:
+                anc = ring_par[(buf - k) % R][anc]
+                k += 1
+            rows[t, :, j] = ring_val[(buf - m) % R][anc].mean(axis=0)
+        buf = (buf + 1) % R
+    if history:
+        return spins, e, series, rows, pars, vals
+    return spins, e, series, rows
 
 
 class lanes:

@@ -798,3 +798,58 @@ void orc_rbm_theta_batch(const gfmc_problem_desc *d, int64_t n, const uint32_t *
     }
     free(x);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Diagonal operators of forward walking (driver.py:240-288): the value of  */
+/* every column of a gfmc_diag_desc on packed walkers.  PAIR columns are    */
+/* SpinHalf.diagonal_weight (SpinHalf.py:291-314): c0 * (n_par - n_anti)    */
+/* when every edge coefficient is c0 (the reference's uniform form), else   */
+/* the edge terms in edge order; then the site terms in site order.  SF     */
+/* columns are SzSzStructureFactor.diagonal_weight (observables.py:162-165):*/
+/* ((sum a s)^2 + (sum b s)^2) / 4 - c, sums in site order.                 */
+/* ------------------------------------------------------------------------ */
+static double diag_column(const gfmc_diag_desc *d, const int8_t *x, int v) {
+    const int N = d->n_sites;
+    const double *a = d->a + (int64_t)v * N;
+    if (d->kind[v] == GFMC_DIAG_SF) {
+        const double *b = d->b + (int64_t)v * N;
+        double C = 0.0, S = 0.0;
+        for (int k = 0; k < N; ++k) {
+            C += x[k] > 0 ? a[k] : -a[k];
+            S += x[k] > 0 ? b[k] : -b[k];
+        }
+        return 0.25 * (C * C + S * S) - d->c[v];
+    }
+    const int e0 = d->edge_off[v], e1 = d->edge_off[v + 1];
+    int uniform = e1 > e0;
+    for (int e = e0; e < e1; ++e)
+        if (d->edge_c[e] != d->edge_c[e0]) uniform = 0;
+    double val = 0.0;
+    if (uniform) {
+        int cnt = 0;
+        for (int e = e0; e < e1; ++e) cnt += x[d->edge_i[e]] == x[d->edge_j[e]] ? 1 : -1;
+        val = d->edge_c[e0] * (double)cnt;
+    } else {
+        for (int e = e0; e < e1; ++e)
+            val += x[d->edge_i[e]] == x[d->edge_j[e]] ? d->edge_c[e] : -d->edge_c[e];
+    }
+    int linear = 0;
+    for (int k = 0; k < N; ++k)
+        if (a[k] != 0.0) linear = 1;
+    if (linear) {
+        double s = 0.0;
+        for (int k = 0; k < N; ++k) s += x[k] > 0 ? a[k] : -a[k];
+        val += s;
+    }
+    return val + d->c[v];
+}
+
+void orc_diag_values(const gfmc_diag_desc *d, int64_t nw, const uint32_t *spins, double *out) {
+    const int N = d->n_sites, W = n_words_of(N), V = d->n_cols;
+    int8_t *x = (int8_t *)malloc((size_t)N);
+    for (int64_t i = 0; i < nw; ++i) {
+        unpack1(spins + i * W, N, x);
+        for (int v = 0; v < V; ++v) out[i * V + v] = diag_column(d, x, v);
+    }
+    free(x);
+}

@@ -59,6 +59,20 @@ class ProblemInfo(ctypes.Structure):
     ]
 
 
+DIAG_PAIR = 0
+DIAG_SF = 1
+
+
+class DiagDesc(ctypes.Structure):
+    """Mirror of ``gfmc_diag_desc``."""
+    _fields_ = [
+        ("n_sites", ctypes.c_int32), ("n_cols", ctypes.c_int32),
+        ("kind", _i32p), ("edge_off", _i32p), ("edge_i", _i32p), ("edge_j", _i32p),
+        ("edge_c", _f64p), ("a", _f64p), ("b", _f64p), ("c", _f64p),
+        ("n_p", ctypes.c_int32), ("m", _i32p),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("gfmc_version", ctypes.c_char_p, []),
@@ -136,6 +150,24 @@ SIGNATURES = [
     ("gfmc_adam_step", ctypes.c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int64,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, _vp]),
+    ("gfmc_diag_create", ctypes.c_int, [ctypes.POINTER(DiagDesc), ctypes.POINTER(_vp)]),
+    ("gfmc_diag_destroy", ctypes.c_int, [_vp]),
+    ("gfmc_diag_ring_bytes", ctypes.c_int64, [_vp, ctypes.c_int64]),
+    ("gfmc_diag_work_doubles", ctypes.c_int64, [_vp, ctypes.c_int64]),
+    ("gfmc_diag_ring_init", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp]),
+    ("gfmc_diag_values", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
+    ("gfmc_diag_record", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("gfmc_run_steps_diag", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           _vp, ctypes.c_int32, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
+                                           _vp, _vp, _vp, _vp]),
+    ("gfmc_graph_capture_steps_diag", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp,
+                                                     _vp, _vp, _vp, ctypes.c_int32,
+                                                     ctypes.c_double, ctypes.c_double,
+                                                     ctypes.c_uint64, ctypes.c_uint32,
+                                                     ctypes.c_uint32, _vp, _vp, _vp, _vp,
+                                                     _vp, _vp, _vp, ctypes.POINTER(_vp)]),
 ]
 
 _LIB = None

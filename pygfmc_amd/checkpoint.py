@@ -12,11 +12,18 @@ archive (no pickled objects; ``numpy.load(allow_pickle=False)`` reads it):
 * ``current_walkers/{spins,log_weight,e_loc}``: the ensemble after the
   newest run (bit-packed spins);
 * ``run_<n>/{log_weights,energy,rng_key,rng_key_next,avg_log_weight}`` per
-  appended run.
+  appended run, and with diagonal operators ``run_<n>/diag/<name>`` and the
+  lead-in ``run_<n>/additional_diag/<name>`` (the diagonal rows of the
+  max(m) steps after the run, traced with its ancestry).
 
 ``run_from_checkpoint`` continues the stored chain without a warm-up, with
 the stored shift, walkers and Philox position: the series of a run split
 into a run plus a continuation is bit-identical to the uninterrupted run.
+The continuation's ancestry ring starts empty, so its first max(m)
+diagonal rows in ``raw['diag']`` are replaced by the stored lead-in
+(io.py:1428-1470), which makes those series identical to the uninterrupted
+run's as well (the returned ``obs`` are the call's own estimates, as in the
+reference; re-estimate the spliced series with ``evaluate_observables``).
 """
 from __future__ import annotations
 
@@ -31,7 +38,7 @@ from .walkers import Walker
 __all__ = ["write_checkpoint", "read_checkpoint", "load_raw", "run_from_checkpoint"]
 
 MUST_MATCH = ("step_size", "p_array", "method", "lam", "precision", "n_sites", "n_walkers",
-              "hamiltonian_sha1", "guide_sha1")
+              "hamiltonian_sha1", "guide_sha1", "diag_operators", "diag_operator_hashes")
 
 
 def _sha1(obj):
@@ -66,6 +73,12 @@ def write_checkpoint(filename, stats, walkers, hamiltonian, guide=None, raw=None
         "n_walkers": np.int64(walkers.n_walkers),
         "hamiltonian_sha1": np.asarray(_sha1(hamiltonian)),
         "guide_sha1": np.asarray(_sha1(guide)),
+        "diag_operators": np.asarray(",".join(
+            x.decode() if isinstance(x, bytes) else str(x)
+            for x in stats.get("parameters/diag_operators", []))),
+        "diag_operator_hashes": np.asarray(",".join(
+            x.decode() if isinstance(x, bytes) else str(x)
+            for x in stats.get("parameters/diag_operator_hashes", []))),
     }
     data = _read(filename)
     if data:
@@ -83,6 +96,11 @@ def write_checkpoint(filename, stats, walkers, hamiltonian, guide=None, raw=None
     if raw is not None:
         data[f"run_{n}/log_weights"] = np.asarray(raw["log_weights"], dtype=np.float64)
         data[f"run_{n}/energy"] = np.asarray(raw["energy"], dtype=np.float64)
+        for group in ("diag", "additional_diag"):
+            for name, arr in (raw.get(group) or {}).items():
+                if "/" in name:
+                    raise ValueError(f"operator name {name!r} contains '/'")
+                data[f"run_{n}/{group}/{name}"] = np.asarray(arr, dtype=np.float64)
     data[f"run_{n}/rng_key"] = np.asarray(stats["parameters/rng_key"], dtype=np.uint64)
     data[f"run_{n}/rng_key_next"] = np.asarray(stats["parameters/rng_key_next"],
                                                dtype=np.uint64)
@@ -116,7 +134,16 @@ def load_raw(filename):
     lw = np.concatenate([data[f"run_{n}/log_weights"] for n in runs])
     E = np.concatenate([data[f"run_{n}/energy"] for n in runs])
     p = [int(v) for v in data["parameters/p_array"]]
-    return dict(log_weights=lw, energy=E, p_array=p, m_array=[q // 2 for q in p])
+    names = _diag_names(data)
+    diag = ({name: np.concatenate([data[f"run_{n}/diag/{name}"] for n in runs])
+             for name in names} if names else None)
+    return dict(log_weights=lw, energy=E, p_array=p, m_array=[q // 2 for q in p], diag=diag,
+                diag_operators=names or None)
+
+
+def _diag_names(data):
+    s = str(data.get("parameters/diag_operators", ""))
+    return [x for x in s.split(",") if x]
 
 
 def run_from_checkpoint(filename, hamiltonian, guide=None, n_steps=None, **run_kwargs):
@@ -145,9 +172,29 @@ def run_from_checkpoint(filename, hamiltonian, guide=None, n_steps=None, **run_k
     method = str(data["parameters/method"])
     lam = float(data["parameters/lam"])
     key = tuple(int(v) for v in data[f"run_{last}/rng_key_next"])
-    return run_gfmc(w, n_steps, hamiltonian, float(data["parameters/step_size"]), guide,
-                    [int(v) for v in data["parameters/p_array"]], method=method,
-                    lam=None if np.isnan(lam) else lam, rng_key=key,
-                    avg_log_weight=float(data[f"run_{last}/avg_log_weight"]), warmup_steps=0,
-                    lanes=int(data["parameters/lanes_per_walker"]),
-                    precision=str(data["parameters/precision"]), **run_kwargs)
+    names = _diag_names(data)
+    given = list(run_kwargs.get("diag_operators") or {})
+    if given != names:
+        raise ValueError(f"{filename}: the stored run measured diagonal operators {names}, "
+                         f"this call {given}; a continuation must measure the same ones")
+    out = run_gfmc(w, n_steps, hamiltonian, float(data["parameters/step_size"]), guide,
+                   [int(v) for v in data["parameters/p_array"]], method=method,
+                   lam=None if np.isnan(lam) else lam, rng_key=key,
+                   avg_log_weight=float(data[f"run_{last}/avg_log_weight"]), warmup_steps=0,
+                   lanes=int(data["parameters/lanes_per_walker"]),
+                   precision=str(data["parameters/precision"]), **run_kwargs)
+    if names and len(out) > 4 and out[4] is not None and out[4].get("diag") is not None:
+        raw = out[4]
+        spliced = {}
+        for name in names:
+            arr = np.array(raw["diag"][name], copy=True)
+            lead = data.get(f"run_{last}/additional_diag/{name}")
+            if lead is not None:
+                if lead.shape[1:] != arr.shape[1:]:
+                    raise ValueError(f"{filename}: the stored lead-in of {name!r} has rows "
+                                     f"{lead.shape[1:]}, this run {arr.shape[1:]}")
+                k = min(lead.shape[0], arr.shape[0])
+                arr[:k] = lead[:k]
+            spliced[name] = arr
+        raw["diag"] = spliced
+    return out

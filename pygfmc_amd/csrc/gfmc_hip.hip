@@ -1903,6 +1903,15 @@ static bool lanes_lds_fits(const gfmc_problem *p, int lanes) {
     return b <= 64 * 1024;
 }
 
+// diagonal forward walking (gfmc_diag.inl)
+struct gfmc_diag;
+static int diag_check(const char *fn, const gfmc_diag *d, const gfmc_problem *p, void *ring,
+                      double *dwork);
+static int64_t diag_row_doubles(const gfmc_diag *d);
+static int diag_record_impl(const gfmc_diag *d, int64_t nw, const uint32_t *spins,
+                            const int32_t *parents, void *ring, double *row, double *work,
+                            hipStream_t st);
+
 static void free_problem(gfmc_problem *p) {
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;
@@ -2366,17 +2375,27 @@ int gfmc_gather(int32_t n_sites, int64_t nw, const uint32_t *spins_in, const dou
     return 0;
 }
 
-int gfmc_run_steps(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
-                   double *e_loc, uint32_t *spins_b, double *e_loc_b, int32_t *parents,
-                   double *work, int32_t n_steps, double beta, double alw, uint64_t seed,
-                   uint32_t step0, uint32_t tag_prop, uint32_t tag_reconf, double *series,
-                   int32_t *evals, void *stream) {
-    int rc = check_walkers("gfmc_run_steps", p, nw);
+// the step loop, with the diagonal forward-walking record after each
+// resampling when dg is non-null (gfmc_diag.inl)
+static int run_steps_impl(const char *fn, const gfmc_problem *p, int64_t nw, uint32_t *spins,
+                          double *log_w, double *e_loc, uint32_t *spins_b, double *e_loc_b,
+                          int32_t *parents, double *work, int32_t n_steps, double beta,
+                          double alw, uint64_t seed, uint32_t step0, uint32_t tag_prop,
+                          uint32_t tag_reconf, double *series, int32_t *evals,
+                          const gfmc_diag *dg, void *ring, double *dseries, double *dwork,
+                          void *stream) {
+    int rc = check_walkers(fn, p, nw);
     if (rc) return rc;
-    if (nw == 0 || n_steps < 0) return fail(GFMC_EINVAL, "gfmc_run_steps: bad size");
+    if (nw == 0 || n_steps < 0) return fail(GFMC_EINVAL, std::string(fn) + ": bad size");
     if (!spins || !log_w || !e_loc || !spins_b || !e_loc_b || !parents || !work ||
         (n_steps > 0 && !series))
-        return fail(GFMC_EINVAL, "gfmc_run_steps: NULL buffer");
+        return fail(GFMC_EINVAL, std::string(fn) + ": NULL buffer");
+    if (dg) {
+        rc = diag_check(fn, dg, p, ring, dwork);
+        if (rc) return rc;
+        if (n_steps > 0 && !dseries)
+            return fail(GFMC_EINVAL, std::string(fn) + ": NULL diagonal series");
+    }
     const int words = p->dp.words;
     uint32_t *sa = spins, *sb = spins_b;
     double *ea = e_loc, *eb = e_loc_b;
@@ -2389,6 +2408,12 @@ int gfmc_run_steps(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
         if (rc) return rc;
         rc = gfmc_gather(p->dp.N, nw, sa, ea, parents, sb, eb, log_w, stream);
         if (rc) return rc;
+        if (dg) {
+            rc = diag_record_impl(dg, nw, sb, parents, ring,
+                                  dseries + (int64_t)t * diag_row_doubles(dg), dwork,
+                                  (hipStream_t)stream);
+            if (rc) return rc;
+        }
         uint32_t *ts = sa;
         sa = sb;
         sb = ts;
@@ -2403,6 +2428,16 @@ int gfmc_run_steps(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
                                (hipStream_t)stream));
     }
     return 0;
+}
+
+int gfmc_run_steps(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                   double *e_loc, uint32_t *spins_b, double *e_loc_b, int32_t *parents,
+                   double *work, int32_t n_steps, double beta, double alw, uint64_t seed,
+                   uint32_t step0, uint32_t tag_prop, uint32_t tag_reconf, double *series,
+                   int32_t *evals, void *stream) {
+    return run_steps_impl("gfmc_run_steps", p, nw, spins, log_w, e_loc, spins_b, e_loc_b,
+                          parents, work, n_steps, beta, alw, seed, step0, tag_prop, tag_reconf,
+                          series, evals, nullptr, nullptr, nullptr, nullptr, stream);
 }
 
 int gfmc_sample_index(int32_t precision, int32_t n, const double *weights, int32_t n_draws,
@@ -2453,3 +2488,4 @@ int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32
 #include "gfmc_batch.inl"
 #include "gfmc_graph.inl"
 #include "gfmc_estim.inl"
+#include "gfmc_diag.inl"

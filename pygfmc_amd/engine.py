@@ -4,12 +4,14 @@ walkers/reconfiguration.py), driving the HIP kernels.
 
 Methods: continuous time (``"cont_time"``, evolution.py:559-721) and the
 discrete ``"single_step"`` / ``"multi_step"`` propagators with the shift
-``lam`` (evolution.py:322-556).  The diagonal / off-diagonal forward-walking
-observables are not implemented and raise.
+``lam`` (evolution.py:322-556).  Diagonal operators are measured by forward
+walking on the device (driver.py:240-288); off-diagonal operators (the
+shadow-chain estimator) are not implemented and raise.
 """
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import math
 import time
 
@@ -17,7 +19,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .estimators import (check_population, estimate_energy, validated_int,
+from .estimators import (check_population, estimate_diagonal, estimate_energy, validated_int,
                          validated_p_array)
 from .problem import get_problem
 from .walkers import PhiloxKey, Walker, as_key
@@ -109,19 +111,30 @@ class Chain:
         self.work = torch.empty(_lib.work_doubles(self.nw), dtype=torch.float64, device=dev)
         self.evals = torch.zeros(self.nw, dtype=torch.int32, device=dev)
 
-    def steps(self, n, beta, alw, seed, step0, warmup=False, series=None, evals=False):
-        """Enqueue ``n`` branching steps; returns the (n, 2) device series."""
+    def steps(self, n, beta, alw, seed, step0, warmup=False, series=None, evals=False,
+              diag=None, dseries=None):
+        """Enqueue ``n`` branching steps; returns the (n, 2) device series.
+        With ``diag`` (a :class:`~pygfmc_amd.diagfw.DiagRecorder`) every step
+        also appends its forward-walking row to ``dseries`` (n, n_cols,
+        n_p)."""
         if series is None:
             series = torch.empty((n, 2), dtype=torch.float64, device=self.spins.device)
         tp = _lib.TAG_WARMUP_PROPAGATE if warmup else _lib.TAG_PROPAGATE
         tr = _lib.TAG_WARMUP_RECONFIG if warmup else _lib.TAG_RECONFIG
-        _lib.check(_lib.load().gfmc_run_steps(
-            self.p.handle, self.nw, _lib.ptr(self.spins), _lib.ptr(self.log_w),
-            _lib.ptr(self.e_loc), _lib.ptr(self.spins_b), _lib.ptr(self.e_loc_b),
-            _lib.ptr(self.parents), _lib.ptr(self.work), int(n), float(beta), float(alw),
-            int(seed) & 0xFFFFFFFFFFFFFFFF, int(step0) & 0xFFFFFFFF, tp, tr,
-            _lib.ptr(series), _lib.ptr(self.evals) if evals else None, _lib.stream()),
-            "gfmc_run_steps")
+        common = (self.p.handle, self.nw, _lib.ptr(self.spins), _lib.ptr(self.log_w),
+                  _lib.ptr(self.e_loc), _lib.ptr(self.spins_b), _lib.ptr(self.e_loc_b),
+                  _lib.ptr(self.parents), _lib.ptr(self.work), int(n), float(beta), float(alw),
+                  int(seed) & 0xFFFFFFFFFFFFFFFF, int(step0) & 0xFFFFFFFF, tp, tr,
+                  _lib.ptr(series), _lib.ptr(self.evals) if evals else None)
+        if diag is None:
+            _lib.check(_lib.load().gfmc_run_steps(*common, _lib.stream()), "gfmc_run_steps")
+        else:
+            if dseries is None:
+                dseries = diag.series(n, self.spins.device)
+            _lib.check(_lib.load().gfmc_run_steps_diag(*common, diag.handle,
+                                                       _lib.ptr(diag.ring), _lib.ptr(dseries),
+                                                       _lib.ptr(diag.work), _lib.stream()),
+                       "gfmc_run_steps_diag")
         return series
 
     def step(self, beta, alw, seed, step, out2, events=None, evals=True):
@@ -158,10 +171,11 @@ class StepGraph:
     a HIP graph (gfmc_graph_capture_steps) and replayed by :meth:`launch`;
     each launch runs positions ``step0 .. step0 + n - 1`` into ``series``."""
 
-    def __init__(self, chain, n, beta, alw, seed, evals=False, warmup=False):
+    def __init__(self, chain, n, beta, alw, seed, evals=False, warmup=False, diag=None):
         lib = _lib.load()
         dev = chain.spins.device
         self.n = int(n)
+        self.diag = diag
         self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
         batch = hasattr(chain, "b")
         shape = (self.n, chain.b.n_lat, 2) if batch else (self.n, 2)
@@ -175,10 +189,19 @@ class StepGraph:
                   _lib.ptr(chain.spins_b), _lib.ptr(chain.e_loc_b), _lib.ptr(chain.parents),
                   _lib.ptr(chain.work), self.n, float(beta))
         if batch:
+            if diag is not None:
+                raise NotImplementedError("batches measure the energy only")
             self._alw = alw
             rc = lib.gfmc_graph_capture_batch_steps(chain.b.handle, *common, _lib.ptr(alw), seed,
                                                     tp, tr, _lib.ptr(self.series), ev,
                                                     _lib.ptr(self.counter), ctypes.byref(h))
+        elif diag is not None:
+            self.dseries = diag.series(self.n, dev)
+            rc = lib.gfmc_graph_capture_steps_diag(chain.p.handle, chain.nw, *common, float(alw),
+                                                   seed, tp, tr, _lib.ptr(self.series), ev,
+                                                   diag.handle, _lib.ptr(diag.ring),
+                                                   _lib.ptr(self.dseries), _lib.ptr(diag.work),
+                                                   _lib.ptr(self.counter), ctypes.byref(h))
         else:
             rc = lib.gfmc_graph_capture_steps(chain.p.handle, chain.nw, *common, float(alw),
                                               seed, tp, tr, _lib.ptr(self.series), ev,
@@ -187,15 +210,18 @@ class StepGraph:
         self.handle = h
         self.chain = chain
 
-    def launch(self, step0, out=None):
+    def launch(self, step0, out=None, dout=None):
         """Run the ``n`` steps from position ``step0``; ``out`` (optional)
-        receives a copy of the series rows."""
+        receives a copy of the series rows, ``dout`` of the forward-walking
+        rows of a graph captured with ``diag``."""
         v = int(step0) & 0xFFFFFFFF
         self.counter.fill_(v - 2 ** 32 if v >= 2 ** 31 else v)
         _lib.check(_lib.load().gfmc_graph_launch(self.handle, _lib.stream()),
                    "gfmc_graph_launch")
         if out is not None:
             out.copy_(self.series)
+        if dout is not None:
+            dout.copy_(self.dseries)
         return self.series
 
     def __del__(self):
@@ -336,10 +362,14 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     """Single-GPU GFMC run (run_gfmc, driver.py:1148-1547), energy only.
 
     Returns ``(walkers, energy, obs, stats, raw)`` (or the first four with
-    ``return_raw=False``): ``energy`` {p: E_p}, ``obs`` {} (no operators),
+    ``return_raw=False``): ``energy`` {p: E_p}, ``obs`` {name: {p: value}}
+    of the diagonal operators (``diag_operators``: {name: SpinHalf or
+    SzSzStructureFactor}; a scalar per p, or one value per wavevector),
     ``stats`` with ``avg_log_weight``, ``it_per_sec``, ``runtime``,
     ``total_wall_time``, ``number_of_devices`` and ``parameters/*``, ``raw``
-    with the per-step ``log_weights`` and ``energy`` series.  ``rng_key``:
+    with the per-step ``log_weights`` and ``energy`` series, ``diag`` {name:
+    (n_steps, [n_vals,] n_p)} and the lead-in ``additional_diag`` (the
+    m_max steps after the run, traced with its ancestry).  ``rng_key``:
     an int seed or (seed, position); ``stats['parameters/rng_key_next']``
     continues the chain.  ``precision``: 'fp64' / 'fp32' (default from the
     Hamiltonian's dtype).  ``use_graph``: production chunks replay one
@@ -348,9 +378,10 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     the autotuner (pygfmc_amd.autotune); pin it for bit-reproducible runs.
     """
     wall_start = time.time()
-    if diag_operators or off_diag_operators:
-        raise NotImplementedError("pygfmc_amd measures the energy only; diagonal and "
-                                  "off-diagonal operators are not implemented.")
+    if off_diag_operators:
+        raise NotImplementedError("off-diagonal operators (the shadow-chain forward-walking "
+                                  "estimator) are not implemented; diagonal operators are.")
+    diag_ops = dict(diag_operators) if diag_operators else {}
     if rng_key is None:
         import secrets
         rng_key = secrets.randbits(32)
@@ -389,6 +420,14 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     print_every = max(1, n_steps // max(1, number_of_outputs))
     dev = chain.spins.device
     series = torch.empty((n_steps, 2), dtype=torch.float64, device=dev)
+    m_array = [q // 2 for q in p_sorted]
+    diag = None
+    dseries = None
+    if diag_ops:
+        from .diagfw import DiagRecorder
+        diag = DiagRecorder(diag_ops, walkers.n_sites, m_array, walkers.n_walkers, dev)
+        dseries = diag.series(n_steps, dev)
+    dchunks = []
     evals_sum, evals_n = 0.0, 0
     torch.cuda.synchronize()
     t_start = time.time()
@@ -402,13 +441,15 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         if step0 < n_steps:
             n = min(print_every, n_steps - step0)
             chain.evals.zero_()
+            drows = dseries[step0:step0 + n] if diag is not None else None
             if use_graph and n == print_every and n_steps >= 2 * print_every:
                 if graph is None:
-                    graph = StepGraph(chain, n, step_f, alw, key.seed, evals=True)
-                graph.launch(key.position + step0, out=series[step0:step0 + n])
+                    graph = StepGraph(chain, n, step_f, alw, key.seed, evals=True, diag=diag)
+                graph.launch(key.position + step0, out=series[step0:step0 + n], dout=drows)
             else:
                 chain.steps(n, step_f, alw, key.seed, key.position + step0,
-                            series=series[step0:step0 + n], evals=True)
+                            series=series[step0:step0 + n], evals=True, diag=diag,
+                            dseries=drows)
             fw.update(series[step0:step0 + n])
             ev = chain.evals.to(torch.float64).mean() / n
             nxt = (step0, n, ev)
@@ -420,6 +461,8 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
             chunk = series[s0:s0 + n].cpu().numpy()
             check_population(chunk[:, 0], s0, chunk[:, 1])
             host_chunks.append(chunk)
+            if diag is not None:
+                dchunks.append(dseries[s0:s0 + n].cpu().numpy())
             evals_sum += float(ev.item())
             evals_n += 1
             if verbosity >= 3 and len(host_chunks) % max(1, number_of_outputs // 10) == 0:
@@ -431,6 +474,21 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     log_weights, E = arr[:, 0].copy(), arr[:, 1].copy()
     energy = estimate_energy(p_sorted, log_weights, E)
     energy_device = fw.result()
+    out_walkers = chain.walkers()
+    diag_np = additional_diag = None
+    obs = {}
+    if diag is not None:
+        diag_np = diag.split(np.concatenate(dchunks, axis=0))
+        obs = estimate_diagonal(p_sorted, log_weights, diag_np)
+        if diag.m_max > 0:
+            # the lead-in a continuation substitutes for its first rows
+            # (driver.py:1100-1127): m_max further steps traced with this
+            # run's ancestry; the returned walkers stay the production-final
+            # ones
+            extra = diag.series(diag.m_max, dev)
+            chain.steps(diag.m_max, step_f, alw, key.seed, key.position + n_steps,
+                        diag=diag, dseries=extra)
+            additional_diag = diag.split(extra.cpu().numpy())
     key_next = key.advance(n_steps)
     stats = {
         "avg_log_weight": float(alw if alw_given is None else alw_given),
@@ -459,18 +517,22 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         "parameters/rng_key": np.asarray([key.seed, key.position], dtype=np.uint64),
         "parameters/rng_key_next": np.asarray([key_next.seed, key_next.position],
                                               dtype=np.uint64),
+        "parameters/n_diag_operators": len(diag_ops),
+        "parameters/diag_operators": np.asarray(list(diag_ops), dtype="S64"),
+        "parameters/diag_operator_hashes": np.asarray(
+            [hashlib.sha1(op.fingerprint()).hexdigest()[:32] for op in diag_ops.values()],
+            dtype="S32"),
     }
-    out_walkers = chain.walkers()
     if verbosity >= 1:
         print(f"[gfmc] - [run_gfmc] - {n_steps} steps @ {stats['it_per_sec']:.1f} it/s; "
               f"E = {energy}", flush=True)
     if return_raw:
-        raw = dict(log_weights=log_weights, energy=E, diag=None, off_diag=None,
-                   off_diag_norm=None, off_diag_steps=None, additional_diag=None,
-                   p_array=list(p_sorted), m_array=[q // 2 for q in p_sorted],
-                   diag_operators=None, off_diag_operators=None)
-        return out_walkers, energy, {}, stats, raw
-    return out_walkers, energy, {}, stats
+        raw = dict(log_weights=log_weights, energy=E, diag=diag_np, off_diag=None,
+                   off_diag_norm=None, off_diag_steps=None, additional_diag=additional_diag,
+                   p_array=list(p_sorted), m_array=m_array,
+                   diag_operators=list(diag_ops) or None, off_diag_operators=None)
+        return out_walkers, energy, obs, stats, raw
+    return out_walkers, energy, obs, stats
 
 
 def run(walkers, n_steps, hamiltonian, step_size, guide, p_array, *, method="cont_time",

@@ -10,8 +10,8 @@ import warnings
 
 import numpy as np
 
-__all__ = ["validated_p_array", "estimate_energy", "evaluate_observables", "DeviceEnergyEstimator",
-           "validated_int", "check_population"]
+__all__ = ["validated_p_array", "estimate_energy", "estimate_diagonal", "evaluate_observables",
+           "DeviceEnergyEstimator", "validated_int", "check_population"]
 
 
 def validated_int(name, value, minimum):
@@ -75,6 +75,57 @@ def estimate_energy(p_array, log_weights, E):
     return out
 
 
+def estimate_diagonal(p_array, log_weights, diag, m_max_common=0):
+    """Forward-walking estimates of diagonal operators,
+    obs[name][p] = sum_i G_p(i) D(i+p) / sum_i G_p(i) over the windows
+    i >= m_max_common, with D the operator's series at p's column and G_p
+    the energy estimator's window weights (estimators.py:268-305, 540-622).
+    ``diag``: {name: (n, n_p) | (n, n_vals, n_p)}; a scalar per p for the
+    former, an (n_vals,) array for the latter.  ``m_max_common`` is the
+    reference's common window trim, max(m) when off-diagonal operators share
+    the run, else 0."""
+    w = np.asarray(log_weights, dtype=np.float64)
+    sorted_p = validated_p_array(p_array)
+    n = len(w)
+    too_large = [p for p in sorted_p if p >= n]
+    if too_large:
+        raise ValueError(f"Each p value must be smaller than the number of available "
+                         f"samples ({n}); invalid values: {too_large}.")
+    if not np.all(np.isfinite(w)):
+        raise ValueError("Branch weights contain nonfinite values.")
+    m_max_common = int(m_max_common)
+    short = [p for p in sorted_p if n - p - m_max_common <= 0]
+    if short:
+        raise ValueError(f"Insufficient samples for the window trim at p={short}; each "
+                         f"requires n_samples > p + {m_max_common}.")
+    arrs = {}
+    for name, arr in diag.items():
+        a = np.asarray(arr, dtype=np.float64)
+        if a.ndim not in (2, 3):
+            raise ValueError(f"diag['{name}'] must be (n_samples, n_p) for a scalar operator "
+                             f"or (n_samples, n_vals, n_p) for a vector-valued one; got "
+                             f"{a.shape}.")
+        if a.shape[0] != n or a.shape[-1] != len(sorted_p):
+            raise ValueError(f"diag['{name}'] has shape {a.shape}; its first axis must be the "
+                             f"{n} samples and its last the {len(sorted_p)} p values.")
+        if not np.all(np.isfinite(a)):
+            raise ValueError(f"diag['{name}'] contains nonfinite values.")
+        arrs[name] = a
+    C = np.cumsum(w)
+    out = {name: {} for name in arrs}
+    for j, p in enumerate(sorted_p):
+        log_G = (C[p:] - C[:-p])[m_max_common:]
+        G = np.exp(log_G - np.max(log_G))
+        G_sum = float(np.sum(G))
+        for name, a in arrs.items():
+            if a.ndim == 2:
+                out[name][p] = float(np.sum(G * a[:, j][p:][m_max_common:])) / G_sum
+            else:
+                out[name][p] = np.sum(G[:, None] * a[:, :, j][p:][m_max_common:],
+                                      axis=0) / G_sum
+    return out
+
+
 def evaluate_observables(raw, p_array=None, n_blocks=None):
     """Re-estimate the energy from ``raw`` (``log_weights``, ``energy``,
     ``p_array``), optionally with the blocked jackknife error: delete each of
@@ -82,7 +133,8 @@ def evaluate_observables(raw, p_array=None, n_blocks=None):
     err = sqrt((nb-1)/nb * sum (x_b - mean)^2) (io.py:921-1157).  Trailing
     samples that do not fill a block are dropped from both the estimate and
     the error.  Returns ``(energy, obs)`` or ``(energy, obs, energy_err,
-    obs_err)``; ``obs`` is empty (no operator series in this engine)."""
+    obs_err)``; ``obs`` holds the diagonal operators of ``raw['diag']``
+    (empty without them)."""
     if raw.get("energy") is None:
         raise ValueError("raw holds no 'energy' series, so nothing can be re-estimated.")
     if raw.get("log_weights") is None:
@@ -94,8 +146,17 @@ def evaluate_observables(raw, p_array=None, n_blocks=None):
     if grid is None:
         raise ValueError("no p grid: pass p_array or a raw dict that carries one")
     grid = validated_p_array(grid)
+    diag = raw.get("diag") or {}
+    if diag and p_array is not None and list(grid) != list(validated_p_array(raw["p_array"])):
+        raise ValueError("the diagonal series are stored per p of the run; re-estimate them "
+                         "on the run's own p_array")
+
+    def estimate(lw_, E_, diag_):
+        e = estimate_energy(grid, lw_, E_)
+        return e, (estimate_diagonal(grid, lw_, diag_) if diag_ else {})
+
     if n_blocks is None:
-        return estimate_energy(grid, lw, E), {}
+        return estimate(lw, E, diag)
     n_blocks = validated_int("n_blocks", n_blocks, 1)
     n = E.shape[0]
     blen = n // n_blocks
@@ -107,21 +168,31 @@ def evaluate_observables(raw, p_array=None, n_blocks=None):
         warnings.warn(f"Discarding {n - usable} trailing samples for equal jackknife blocks; "
                       "point estimates and errors both use the retained samples.", stacklevel=2)
         E, lw = E[:usable], lw[:usable]
+        diag = {k: np.asarray(v)[:usable] for k, v in diag.items()}
         n = usable
     if n_blocks < 20:
         warnings.warn(f"n_blocks={n_blocks} is few: the error on the error is then "
                       "comparable to the error itself. ~20 or more is advisable.", stacklevel=2)
-    est = estimate_energy(grid, lw, E)
-    reps = []
+    est, obs = estimate(lw, E, diag)
+    reps, obs_reps = [], {name: {p: [] for p in grid} for name in obs}
     for b in range(n_blocks):
         m = np.ones(n, dtype=bool)
         m[b * blen:(b + 1) * blen] = False
-        e = estimate_energy(grid, lw[m], E[m])
+        e, o = estimate(lw[m], E[m], {k: np.asarray(v)[m] for k, v in diag.items()})
         reps.append([e[p] for p in grid])
-    arr = np.asarray(reps, dtype=float)
-    centred = arr - arr.mean(axis=0, keepdims=True)
-    err = np.sqrt((n_blocks - 1) / n_blocks * np.sum(centred ** 2, axis=0))
-    return est, {}, dict(zip(grid, (float(x) for x in err))), {}
+        for name in o:
+            for p in grid:
+                obs_reps[name][p].append(np.asarray(o[name][p], dtype=float))
+
+    def jk(x):
+        x = np.asarray(x, dtype=float)
+        return np.sqrt((n_blocks - 1) / n_blocks * np.sum((x - x.mean(axis=0)) ** 2, axis=0))
+
+    err = jk(reps)
+    obs_err = {name: {p: (float(v) if np.ndim(v) == 0 else v)
+                      for p, v in ((p, jk(obs_reps[name][p])) for p in grid)}
+               for name in obs_reps}
+    return est, obs, dict(zip(grid, (float(x) for x in err))), obs_err
 
 
 def check_population(log_G_chunk, step0, E_chunk=None):

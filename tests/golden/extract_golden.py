@@ -11,6 +11,13 @@ Kept (data only):
 * ``<case>/raw/log_weights``, ``<case>/raw/energy`` (300,) and
   ``<case>/energy/<p>`` of every case that has them: the windowed energy
   estimator's inputs and outputs.
+* ``spinhalf_*/raw/diag/{zz,sf}`` (300, [3,] 2), ``spinhalf_*/obs/{zz,sf}/<p>``
+  and ``spinhalf_*/raw/additional_diag/{zz,sf}``: the forward-walking series
+  of the diagonal operators zz = SpinHalf.create(edges, J_z=1) and sf =
+  SzSzStructureFactor(positions 0..7, k = 2 pi m / 8, m = 0, 1, 2) and
+  their estimates at p = 2, 4 (test_golden_outputs.py:191-211; the runs also
+  measured off-diagonal operators, so the estimates use the common window
+  trim i >= max(m) = 2).
 
 Run in the build container only (the reference is not on the GPU box):
     python tests/golden/extract_golden.py
@@ -38,6 +45,12 @@ def main():
         for k in g.files:
             if k.startswith(f"{case}/energy/"):
                 out[k] = g[k]
+        if case.startswith("spinhalf_"):
+            for op in ("zz", "sf"):
+                for k in g.files:
+                    if (k in (f"{case}/raw/diag/{op}", f"{case}/raw/additional_diag/{op}")
+                            or k.startswith(f"{case}/obs/{op}/")):
+                        out[k] = g[k]
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays from {len(cases)} cases")
 
