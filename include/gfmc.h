@@ -45,6 +45,7 @@ extern "C" {
 #define GFMC_GUIDE_NONE 0
 #define GFMC_GUIDE_JASTROW 1
 #define GFMC_GUIDE_RBM 2
+#define GFMC_GUIDE_MLP 3
 
 /* propagators (evolution.py): continuous time (evolve_walker_cont_time,
  * :559-721), single_step (:322-424) and multi_step (:427-556) */
@@ -106,6 +107,22 @@ typedef struct gfmc_problem_desc {
     const int32_t *dep_slots;
     const int32_t *edge_off;       /* [n_off+1] (required when has_jz)          */
     const int32_t *dep_edges;
+    /* two-layer MLP guide (no reference counterpart; the reference's network
+     * guide is ClusterMLP, wavefunction/cluster_mlp.py:133-439):
+     *   log psi = b3 + sum_c w3_c s(b2_c + sum_k W2_ck h_k),
+     *   h_k = bf16(s(b1_k + sum_n W1_kn x_n)),
+     * s the tabulated tanh (2048 linear pieces on [0, 8), odd, saturated).
+     * W1, b1 and W2 are rounded to the BF16 grid (magnitudes below 2^-24
+     * flushed to 0), so the layer-1 sums are exact in FP64; layer 2 runs on
+     * MFMA (BF16 inputs, FP32 accumulation).  h1 a multiple of 32, h2 one
+     * of 32, 64, 128, 256. */
+    int32_t mlp_h1, mlp_h2;
+    const float *mlp_W1;           /* [h1*N] row-major                          */
+    const float *mlp_b1;           /* [h1]                                      */
+    const float *mlp_W2;           /* [h2*h1] row-major                         */
+    const float *mlp_b2;           /* [h2]                                      */
+    const float *mlp_w3;           /* [h2]                                      */
+    const float *mlp_b3;           /* [1]                                       */
 } gfmc_problem_desc;
 
 typedef struct gfmc_problem gfmc_problem;   /* opaque, device tables */

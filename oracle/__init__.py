@@ -38,6 +38,9 @@ class Desc(ctypes.Structure):
         ("rbm_hidden", ctypes.c_int32),
         ("rbm_W", _f32p), ("rbm_a", _f32p), ("rbm_b", _f32p),
         ("dep_off", _i32p), ("dep_slots", _i32p), ("edge_off", _i32p), ("dep_edges", _i32p),
+        ("mlp_h1", ctypes.c_int32), ("mlp_h2", ctypes.c_int32),
+        ("mlp_W1", _f32p), ("mlp_b1", _f32p), ("mlp_W2", _f32p), ("mlp_b2", _f32p),
+        ("mlp_w3", _f32p), ("mlp_b3", _f32p),
     ]
 
 
@@ -93,6 +96,12 @@ def lib():
                                         _f64p]
         L.orc_rbm_theta_batch.argtypes = [D, ctypes.c_int64, _u32p, _f64p]
         L.orc_diag_values.argtypes = [ctypes.POINTER(DiagDesc), ctypes.c_int64, _u32p, _f64p]
+        L.orc_mlp_table.argtypes = [_f32p, _f32p]
+        L.orc_mlp_tanh.restype = ctypes.c_float
+        L.orc_mlp_tanh.argtypes = [ctypes.c_float]
+        L.orc_bf16_flush.restype = ctypes.c_float
+        L.orc_bf16_flush.argtypes = [ctypes.c_float]
+        L.orc_mlp_hidden.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f32p]
         del vp
         _LIB = L
     return _LIB
@@ -134,6 +143,17 @@ class Problem:
                               _p(r, ctypes.c_double), _p(e, ctypes.c_double),
                               _p(ld, ctypes.c_double))
         return r, e, ld
+
+    def mlp_hidden(self, config):
+        """Layer-1 activations (BF16 values) of +-1 configurations."""
+        c = np.atleast_2d(np.asarray(config)).astype(np.int8)
+        H1 = int(self.lowered.scalars["mlp_h1"])
+        out = np.empty((c.shape[0], H1), dtype=np.float32)
+        for i in range(c.shape[0]):
+            row = np.ascontiguousarray(c[i])
+            lib().orc_mlp_hidden(ctypes.byref(self.desc), row.ctypes.data_as(
+                ctypes.POINTER(ctypes.c_int8)), _p(out[i], ctypes.c_float))
+        return out
 
     def log_psi(self, spins):
         spins = np.ascontiguousarray(spins, dtype=np.uint32)
@@ -292,6 +312,24 @@ def run_chain(prob, spins, n_steps, beta, alw, seed, step0, warmup=False, n_thre
         spins, e = gather(prob.N, spins, e, par)
         lw = np.zeros(nw)
     return spins, e, series
+
+
+def mlp_table():
+    """(T, D) of the tabulated tanh the MLP guide uses."""
+    T = np.empty(2048, dtype=np.float32)
+    D = np.empty(2048, dtype=np.float32)
+    lib().orc_mlp_table(_p(T, ctypes.c_float), _p(D, ctypes.c_float))
+    return T, D
+
+
+def mlp_tanh(z):
+    return np.array([lib().orc_mlp_tanh(float(v)) for v in np.ravel(z)],
+                    dtype=np.float32).reshape(np.shape(z))
+
+
+def bf16_flush(v):
+    return np.array([lib().orc_bf16_flush(float(a)) for a in np.ravel(v)],
+                    dtype=np.float32).reshape(np.shape(v))
 
 
 class Diag:

@@ -224,10 +224,120 @@ static double rbm_log_ratio(const gfmc_problem_desc *d, const int8_t *x, const d
     return lr + acc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* MLP guide (no reference counterpart; nearest ClusterMLP,                  */
+/* wavefunction/cluster_mlp.py:133-439):                                      */
+/*   log psi = b3 + sum_c w3_c s(b2_c + sum_k W2_ck h_k),                     */
+/*   h_k = bf16(s(b1_k + sum_n W1_kn x_n)),                                   */
+/* in the library's arithmetic: W1, b1, W2 on the BF16 grid (nearest even,   */
+/* below 2^-24 flushed), layer-1 sums in FP64 (exact on that grid), s the    */
+/* tabulated tanh in FP32 with an explicit fma, h rounded to BF16; the       */
+/* layer-2 sums in FP64 rounded once to FP32 (the device accumulates them in */
+/* FP32 on MFMA: the one step not mirrored bit for bit), then the output     */
+/* sum in the device's order: per column c mod 32 over c / 32, then a        */
+/* butterfly over the 32 columns.                                             */
+/* ------------------------------------------------------------------------ */
+#define MLP_NT 2048
+static float mlp_T[MLP_NT], mlp_D[MLP_NT];
+static int mlp_tab_ready = 0;
+
+static void mlp_table(void) {
+    if (mlp_tab_ready) return;
+    for (int j = 0; j < MLP_NT; ++j) {
+        float t0 = (float)tanh((double)j / 256.0), t1 = (float)tanh((double)(j + 1) / 256.0);
+        mlp_T[j] = t0;
+        mlp_D[j] = t1 - t0;
+    }
+    mlp_tab_ready = 1;
+}
+
+void orc_mlp_table(float *T, float *D) {
+    mlp_table();
+    memcpy(T, mlp_T, sizeof mlp_T);
+    memcpy(D, mlp_D, sizeof mlp_D);
+}
+
+float orc_mlp_tanh(float z) {
+    mlp_table();
+    float a = fabsf(z);
+    float t = fminf(a * 256.0f, (float)MLP_NT);
+    int j = (int)t;
+    if (j > MLP_NT - 1) j = MLP_NT - 1;
+    float f = t - (float)j;
+    return copysignf(fmaf(f, mlp_D[j], mlp_T[j]), z);
+}
+
+static float bf16_rne(float v) {
+    uint32_t b;
+    memcpy(&b, &v, 4);
+    b = ((b + 0x7fffu + ((b >> 16) & 1u)) >> 16) << 16;
+    float r;
+    memcpy(&r, &b, 4);
+    return r;
+}
+
+float orc_bf16_flush(float v) {
+    float r = bf16_rne(v);
+    return fabsf(r) < 5.9604644775390625e-08f ? 0.0f : r;
+}
+
+/* layer-1 activations h (BF16 values) of x */
+void orc_mlp_hidden(const gfmc_problem_desc *d, const int8_t *x, float *h) {
+    int N = d->n_sites, H1 = d->mlp_h1;
+    for (int k = 0; k < H1; ++k) {
+        double z = (double)orc_bf16_flush(d->mlp_b1[k]);
+        for (int n = 0; n < N; ++n)
+            z += (double)x[n] * (double)orc_bf16_flush(d->mlp_W1[(size_t)k * N + n]);
+        h[k] = bf16_rne(orc_mlp_tanh((float)z));
+    }
+}
+
+float orc_mlp_log_psi(const gfmc_problem_desc *d, const int8_t *x) {
+    int H1 = d->mlp_h1, H2 = d->mlp_h2, NB = H2 / 32;
+    float *h = (float *)malloc(sizeof(float) * (size_t)H1);
+    orc_mlp_hidden(d, x, h);
+    float s[32];
+    for (int l = 0; l < 32; ++l) s[l] = 0.0f;
+    for (int t = 0; t < NB; ++t)
+        for (int l = 0; l < 32; ++l) {
+            int c = t * 32 + l;
+            double z2 = 0.0;
+            for (int k = 0; k < H1; ++k)
+                z2 += (double)orc_bf16_flush(d->mlp_W2[(size_t)c * H1 + k]) * (double)h[k];
+            float v = d->mlp_w3[c] * orc_mlp_tanh((float)z2 + d->mlp_b2[c]);
+            s[l] = s[l] + v;
+        }
+    for (int dd = 16; dd >= 1; dd >>= 1) {
+        float o[32];
+        for (int l = 0; l < 32; ++l) o[l] = s[l] + s[l ^ dd];
+        memcpy(s, o, sizeof o);
+    }
+    free(h);
+    return s[0] + d->mlp_b3[0];
+}
+
+static double mlp_log_ratio(const gfmc_problem_desc *d, const int8_t *x, const double *lp0,
+                            int i, int j) {
+    int N = d->n_sites;
+    int8_t *y = (int8_t *)malloc((size_t)N);
+    memcpy(y, x, (size_t)N);
+    y[i] = (int8_t)-y[i];
+    if (j >= 0) y[j] = (int8_t)-y[j];
+    float out = orc_mlp_log_psi(d, y);
+    free(y);
+    return (double)(out - (float)lp0[0]);
+}
+
 /* Guide log-ratio of off-diagonal slot s (log_diff_from_operator,
- * jastrow_spin_half.py:369-398); theta: the RBM's theta of x (RBM only). */
+ * jastrow_spin_half.py:369-398); theta: the RBM's theta of x (RBM), or
+ * log psi(x) as computed for the MLP guide. */
 static double slot_log_diff(const gfmc_problem_desc *d, const int8_t *x, int s,
                             const double *theta) {
+    if (d->guide == GFMC_GUIDE_MLP) {
+        if (s < d->n_hx) return mlp_log_ratio(d, x, theta, s, -1);
+        int e = s - d->n_hx;
+        return mlp_log_ratio(d, x, theta, d->edge_i[e], d->edge_j[e]);
+    }
     if (d->guide == GFMC_GUIDE_RBM) {
         if (s < d->n_hx) return rbm_log_ratio(d, x, theta, s, -1);
         int e = s - d->n_hx;
@@ -268,6 +378,11 @@ static double slot_rate(const gfmc_problem_desc *d, const int8_t *x, int s, doub
 }
 
 static double *theta_of(const gfmc_problem_desc *d, const int8_t *x) {
+    if (d->guide == GFMC_GUIDE_MLP) {
+        double *t = (double *)malloc(sizeof(double));
+        t[0] = (double)orc_mlp_log_psi(d, x);
+        return t;
+    }
     if (d->guide != GFMC_GUIDE_RBM) return NULL;
     double *t = (double *)malloc(sizeof(double) * (size_t)d->rbm_hidden);
     orc_rbm_theta(d, x, t);
@@ -329,6 +444,7 @@ double orc_rates(const gfmc_problem_desc *d, const int8_t *x, double *rates, dou
 /* log psi_T (JastrowSpinHalf.log_psi, jastrow_spin_half.py:299-324):
  * 0.5 * sum_k sum_q J_kq x_k x_other + sum_k h_k x_k, sequential order. */
 double orc_log_psi(const gfmc_problem_desc *d, const int8_t *x) {
+    if (d->guide == GFMC_GUIDE_MLP) return (double)orc_mlp_log_psi(d, x);
     if (d->guide == GFMC_GUIDE_RBM) {
         double *t = theta_of(d, x);
         double field = 0.0, hid = 0.0;

@@ -17,6 +17,7 @@ GFMC_FP32 = 1
 GUIDE_NONE = 0
 GUIDE_JASTROW = 1
 GUIDE_RBM = 2
+GUIDE_MLP = 3
 TAG_PROPAGATE = 0
 TAG_RECONFIG = 1
 TAG_INIT = 2
@@ -45,6 +46,9 @@ class ProblemDesc(ctypes.Structure):
         ("rbm_hidden", ctypes.c_int32),
         ("rbm_W", _f32p), ("rbm_a", _f32p), ("rbm_b", _f32p),
         ("dep_off", _i32p), ("dep_slots", _i32p), ("edge_off", _i32p), ("dep_edges", _i32p),
+        ("mlp_h1", ctypes.c_int32), ("mlp_h2", ctypes.c_int32),
+        ("mlp_W1", _f32p), ("mlp_b1", _f32p), ("mlp_W2", _f32p), ("mlp_b2", _f32p),
+        ("mlp_w3", _f32p), ("mlp_b3", _f32p),
     ]
 
 

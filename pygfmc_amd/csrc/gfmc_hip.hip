@@ -103,10 +103,24 @@ struct DevProblem {
     int H, w_words, rbm_wave_bytes;
     const uint32_t *rbm_W;   // [H][N] BF16, packed two per word (N even-padded)
     const float *rbm_a, *rbm_b, *rbm_cs;   // [N], [H], column sums sum_h W_hk [N]
+    // MLP guide (layer-1 sums exact in FP64, layer 2 on MFMA); rbm_wave_bytes
+    // is also the per-wavefront LDS of its walker kernels
+    int mlp_h1, mlp_h2;
+    const uint16_t *mlp_W1T;  // [N][h1] BF16 bits (site-major)
+    const double *mlp_b1;     // [h1] BF16-grid values
+    const uint16_t *mlp_W2;   // [h2][h1] BF16 bits
+    const float *mlp_b2, *mlp_w3;
+    float mlp_b3;
+    const float2 *mlp_tab;    // [2048] {tanh(j / 256), tanh((j + 1) / 256) - tanh(j / 256)}
 };
+
+struct MlpWork;
+static void free_mlp_work(MlpWork *m);
+static MlpWork *new_mlp_work(int blocks_per_walker);
 
 struct gfmc_problem {
     DevProblem dp;
+    MlpWork *mlp = nullptr;   // the MLP guide's propagation workspace (grown on demand)
     Tables<double> t64;
     Tables<float> t32;
     int precision;
@@ -1903,6 +1917,27 @@ static bool lanes_lds_fits(const gfmc_problem *p, int lanes) {
     return b <= 64 * 1024;
 }
 
+// MLP guide (gfmc_mlp.inl)
+static int mlp_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double *rates,
+                     double *e_loc, double *log_diff, double *log_psi, hipStream_t st);
+static int mlp_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                         double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
+                         uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st);
+#define MLP_NT 2048
+#define MLP_SW 4
+
+// BF16 bits of v (round to nearest even), magnitudes below 2^-24 flushed to 0
+static uint16_t mlp_bf16_bits(float v) {
+    uint32_t bits;
+    memcpy(&bits, &v, 4);
+    uint32_t r = (bits + 0x7fffu + ((bits >> 16) & 1u)) >> 16;
+    const uint32_t back = r << 16;
+    float f;
+    memcpy(&f, &back, 4);
+    if (std::fabs(f) < 5.9604644775390625e-08f) r = 0;
+    return (uint16_t)r;
+}
+
 // diagonal forward walking (gfmc_diag.inl)
 struct gfmc_diag;
 static int diag_check(const char *fn, const gfmc_diag *d, const gfmc_problem *p, void *ring,
@@ -1913,6 +1948,7 @@ static int diag_record_impl(const gfmc_diag *d, int64_t nw, const uint32_t *spin
                             hipStream_t st);
 
 static void free_problem(gfmc_problem *p) {
+    free_mlp_work(p->mlp);
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;
 }
@@ -1942,8 +1978,20 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         if (d->rbm_hidden <= 0 || !d->rbm_W || !d->rbm_a || !d->rbm_b)
             return fail(GFMC_EINVAL, "gfmc_problem_create: incomplete RBM parameters");
     }
+    if (d->guide == GFMC_GUIDE_MLP) {
+        if (precision != GFMC_FP32)
+            return fail(GFMC_EINVAL, "gfmc_problem_create: the MLP guide runs in FP32 "
+                                     "(BF16 layers); pass GFMC_FP32");
+        if (d->mlp_h1 <= 0 || d->mlp_h1 % 32 != 0 || d->mlp_h1 > 4096 ||
+            !(d->mlp_h2 == 32 || d->mlp_h2 == 64 || d->mlp_h2 == 128 || d->mlp_h2 == 256))
+            return fail(GFMC_EINVAL, "gfmc_problem_create: MLP hidden sizes must be a multiple "
+                                     "of 32 (layer 1, at most 4096) and 32, 64, 128 or 256 "
+                                     "(layer 2)");
+        if (!d->mlp_W1 || !d->mlp_b1 || !d->mlp_W2 || !d->mlp_b2 || !d->mlp_w3 || !d->mlp_b3)
+            return fail(GFMC_EINVAL, "gfmc_problem_create: incomplete MLP parameters");
+    }
     if (d->guide != GFMC_GUIDE_NONE && d->guide != GFMC_GUIDE_JASTROW &&
-        d->guide != GFMC_GUIDE_RBM)
+        d->guide != GFMC_GUIDE_RBM && d->guide != GFMC_GUIDE_MLP)
         return fail(GFMC_EINVAL, "gfmc_problem_create: unknown guide kind");
     const int N = d->n_sites, E = d->n_edges, n_off = d->n_hx + d->n_jpm;
     if (d->guide == GFMC_GUIDE_JASTROW &&
@@ -2133,6 +2181,37 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         UP(cs.data(), cs.size(), &P.rbm_cs);
         P.rbm_wave_bytes = round_up(P.K * WAVE * 4 + 3 * H * 4 + n_off * 4 + P.n_pad + P.e_pad, 16);
     }
+    if (d->guide == GFMC_GUIDE_MLP) {
+        const int h1 = d->mlp_h1, h2 = d->mlp_h2;
+        P.mlp_h1 = h1;
+        P.mlp_h2 = h2;
+        std::vector<uint16_t> w1t((size_t)N * h1), w2((size_t)h2 * h1);
+        std::vector<double> b1(h1);
+        for (int k = 0; k < h1; ++k)
+            for (int n = 0; n < N; ++n) w1t[(size_t)n * h1 + k] = mlp_bf16_bits(d->mlp_W1[(size_t)k * N + n]);
+        for (size_t i = 0; i < w2.size(); ++i) w2[i] = mlp_bf16_bits(d->mlp_W2[i]);
+        for (int k = 0; k < h1; ++k) {
+            const uint32_t bits = (uint32_t)mlp_bf16_bits(d->mlp_b1[k]) << 16;
+            float v;
+            memcpy(&v, &bits, 4);
+            b1[k] = (double)v;
+        }
+        std::vector<float2> tab(MLP_NT);
+        for (int j = 0; j < MLP_NT; ++j) {
+            const float t0 = (float)std::tanh((double)j / 256.0);
+            const float t1 = (float)std::tanh((double)(j + 1) / 256.0);
+            tab[j] = make_float2(t0, t1 - t0);
+        }
+        UP(w1t.data(), w1t.size(), &P.mlp_W1T);
+        UP(w2.data(), w2.size(), &P.mlp_W2);
+        UP(b1.data(), b1.size(), &P.mlp_b1);
+        UP(d->mlp_b2, (size_t)h2, &P.mlp_b2);
+        UP(d->mlp_w3, (size_t)h2, &P.mlp_w3);
+        UP(tab.data(), tab.size(), &P.mlp_tab);
+        P.mlp_b3 = d->mlp_b3[0];
+        P.rbm_wave_bytes = round_up(P.K * WAVE * 4 + P.n_pad + P.e_pad, 16);
+        p->mlp = new_mlp_work((n_off + 1 + 31) / 32);
+    }
 #undef UP
 #undef UP32
     const size_t wbytes = (size_t)P.K * WAVE * (precision == GFMC_FP64 ? 8 : 4);
@@ -2150,6 +2229,13 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         HIP_TRY(hipFuncSetAttribute((const void *)k_rates_rbm,
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)p->lds_bytes));
+    } else if (d->guide == GFMC_GUIDE_MLP) {
+        p->lds_bytes = (int64_t)MLP_SW * P.rbm_wave_bytes;
+        if (p->lds_bytes > 64 * 1024) {
+            free_problem(p);
+            return fail(GFMC_EUNSUP, "gfmc_problem_create: MLP walker state exceeds 64 KiB of "
+                                     "LDS per workgroup");
+        }
     } else if (p->lds_bytes > 64 * 1024) {
         free_problem(p);
         return fail(GFMC_EUNSUP,
@@ -2168,7 +2254,8 @@ int gfmc_problem_set_lanes(gfmc_problem *p, int32_t lanes) {
     if (!p) return fail(GFMC_EINVAL, "gfmc_problem_set_lanes: NULL problem");
     if (lanes != 64 && lanes != 32 && lanes != 16)
         return fail(GFMC_EINVAL, "gfmc_problem_set_lanes: lanes must be 64, 32 or 16");
-    if (lanes != 64 && (p->dp.guide == GFMC_GUIDE_RBM || !p->dp.fast ||
+    if (lanes != 64 && (p->dp.guide == GFMC_GUIDE_RBM || p->dp.guide == GFMC_GUIDE_MLP ||
+                        !p->dp.fast ||
                         !p->dp.g_ok[gidx_of(lanes)] || !lanes_lds_fits(p, lanes)))
         return fail(GFMC_EUNSUP, "gfmc_problem_set_lanes: this problem runs one walker per "
                                  "wavefront only (record path, moves within the group, LDS)");
@@ -2184,9 +2271,9 @@ int gfmc_problem_set_method(gfmc_problem *p, int32_t method, double lam) {
     if (method != GFMC_METHOD_CONT) {
         if (!std::isfinite(lam))
             return fail(GFMC_EINVAL, "gfmc_problem_set_method: lam must be finite");
-        if (p->dp.guide == GFMC_GUIDE_RBM)
-            return fail(GFMC_EUNSUP, "gfmc_problem_set_method: the RBM guide propagates in "
-                                     "continuous time only");
+        if (p->dp.guide == GFMC_GUIDE_RBM || p->dp.guide == GFMC_GUIDE_MLP)
+            return fail(GFMC_EUNSUP, "gfmc_problem_set_method: the network guides propagate "
+                                     "in continuous time only");
     }
     p->dp.method = method;
     p->dp.lam = method == GFMC_METHOD_CONT ? 0.0 : lam;
@@ -2207,7 +2294,8 @@ int gfmc_problem_get_info(const gfmc_problem *p, gfmc_problem_info *out) {
     out->lanes = p->lanes;
     out->lanes_ok = 0;
     for (int gi = 0; gi < 3; ++gi)
-        if (gi == 0 || (p->dp.fast && p->dp.g_ok[gi] && lanes_lds_fits(p, WAVE >> gi)))
+        if (gi == 0 || (p->dp.guide != GFMC_GUIDE_MLP && p->dp.fast && p->dp.g_ok[gi] &&
+                        lanes_lds_fits(p, WAVE >> gi)))
             out->lanes_ok |= WAVE >> gi;
     return 0;
 }
@@ -2278,6 +2366,8 @@ int gfmc_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double 
     if (rc) return rc;
     if (nw == 0) return 0;
     if (!spins) return fail(GFMC_EINVAL, "gfmc_rates: NULL spins");
+    if (p->dp.guide == GFMC_GUIDE_MLP)
+        return mlp_rates(p, nw, spins, rates, e_loc, log_diff, nullptr, (hipStream_t)stream);
     if (p->dp.guide == GFMC_GUIDE_RBM)
         hipLaunchKernelGGL(k_rates_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
                            (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
@@ -2296,6 +2386,8 @@ int gfmc_log_psi(const gfmc_problem *p, int64_t nw, const uint32_t *spins, doubl
     if (rc) return rc;
     if (nw == 0) return 0;
     if (!spins || !out) return fail(GFMC_EINVAL, "gfmc_log_psi: NULL buffer");
+    if (p->dp.guide == GFMC_GUIDE_MLP)
+        return mlp_rates(p, nw, spins, nullptr, nullptr, nullptr, out, (hipStream_t)stream);
     if (p->dp.guide == GFMC_GUIDE_RBM)
         hipLaunchKernelGGL(k_rates_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
                            (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
@@ -2320,6 +2412,9 @@ int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
         return fail(GFMC_EINVAL, "gfmc_propagate: the discrete methods take an integer "
                                  "number of sub-steps");
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    if (p->dp.guide == GFMC_GUIDE_MLP)
+        return mlp_propagate(p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
+                             (hipStream_t)stream);
     if (p->dp.guide == GFMC_GUIDE_RBM)
         hipLaunchKernelGGL(k_propagate_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
                            (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
@@ -2489,3 +2584,4 @@ int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32
 #include "gfmc_graph.inl"
 #include "gfmc_estim.inl"
 #include "gfmc_diag.inl"
+#include "gfmc_mlp.inl"

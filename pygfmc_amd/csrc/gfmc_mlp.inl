@@ -1,0 +1,613 @@
+// gfmc_mlp.inl -- the two-layer MLP guide: log psi, rates and continuous-time
+// propagation (included at the end of gfmc_hip.hip, one translation unit).
+//
+//   log psi(x) = b3 + sum_c w3_c s(b2_c + sum_k W2_ck h_k(x)),
+//   h_k(x)     = bf16(s(z_k(x))),  z(x) = b1 + W1 x   (x = +-1),
+//
+// s the tabulated tanh.  A guide ratio changes every hidden unit, so each
+// hop re-evaluates the network on every candidate move.  The propagation
+// runs in bulk-synchronous iterations, one hop of every active walker each:
+//   k_mlp_scan*   rows of each active walker (its configuration, then one row
+//                 per slot with a non-zero bare rate), padded to 32-row
+//                 blocks; an exclusive scan gives each walker its blocks;
+//   k_mlp_gemm    one wavefront per 32-row block (one walker): the layer-1
+//                 activations of its rows are built in registers straight in
+//                 the MFMA A layout from z (FP64, exact: W1 and b1 lie on the
+//                 BF16 grid) and the two W1 columns of the row's move, W2 is
+//                 staged in LDS in 32-wide K chunks, v_mfma_f32_32x32x16_bf16
+//                 accumulates the 32 x h2 block, the epilogue applies s, w3
+//                 and a fixed-order row reduction;
+//   k_mlp_select  one wavefront per walker: rates from the row values,
+//                 e_L, the hold and the move exactly as the other
+//                 continuous-time kernels (blocked order, Philox counter
+//                 (hop, walker, step, tag)), then z and the candidate list of
+//                 the new configuration.
+// The host reads the number of row blocks after each scan (one 4-byte copy)
+// and stops when no walker is active.  Every value a configuration gets is a
+// function of that configuration alone (exact z, fixed MFMA and reduction
+// order for every row), so log psi(x') as a move candidate equals log psi(x')
+// as the current configuration later.
+
+// MLP_NT (2048): linear pieces of the tabulated tanh on [0, 8)
+#define MLP_TSCALE 256.0f   // pieces per unit of |z|
+#define MLP_GW 4            // wavefronts (32-row blocks) per GEMM workgroup
+#define MLP_KC 32           // K chunk of W2 staged in LDS
+#define MLP_BSTR (MLP_KC + 8)
+// MLP_SW (4): walker wavefronts per start / select workgroup
+
+typedef __attribute__((ext_vector_type(8))) __bf16 mlp_bf16x8;
+typedef __attribute__((ext_vector_type(16))) float mlp_f32x16;
+
+struct MlpWs {
+    double *z1;          // [cap][h1]   layer-1 pre-activations of the current config
+    int32_t *cand;       // [cap][n_off] candidate slots, ascending
+    int32_t *cnt;        // [cap]       rows: 1 + candidates
+    int32_t *active;     // [cap]
+    double *t, *lw, *eL; // [cap]
+    uint32_t *it;        // [cap]
+    int32_t *blk_off;    // [cap]       first 32-row block of the walker
+    int32_t *blk_walker; // [cap * blocks per walker]
+    int32_t *part;       // [cap / 1024 + 1] scan partials
+    int32_t *total;      // [1]         row blocks of the iteration
+    float *logpsi;       // [cap * blocks per walker * 32]
+};
+
+struct MlpWork {
+    int64_t cap = 0;
+    int bpw = 0;                 // blocks per walker at most
+    MlpWs ws{};
+    std::vector<void *> allocs;
+    int32_t *host_total = nullptr;   // pinned
+};
+
+__device__ __forceinline__ float mlp_tanh(const float2 *tab, float z) {
+    const float a = fabsf(z);
+    const float t = fminf(a * MLP_TSCALE, (float)MLP_NT);
+    int j = (int)t;
+    if (j > MLP_NT - 1) j = MLP_NT - 1;
+    const float f = t - (float)j;
+    const float2 e = tab[j];
+    return copysignf(__builtin_fmaf(f, e.y, e.x), z);
+}
+
+__device__ __forceinline__ float bf16_val(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ float mlp_bare(const DevProblem &P, const Tables<float> &tb,
+                                          const signed char *x, int s) {
+    if (s < P.n_hx) return tb.hx_half[s];
+    const int e = s - P.n_hx;
+    const int pk = P.edge_pack[e];
+    return x[pk & 0xffff] != x[(pk >> 16) & 0xffff] ? tb.jpm[e] : tb.jpmpm[e];
+}
+
+// slots with a non-zero bare rate, ascending; returns their number
+__device__ int mlp_candidates(const DevProblem &P, const Tables<float> &tb, const signed char *x,
+                              int32_t *cand, int lane) {
+    int base = 0;
+    for (int c = 0; c < P.n_off; c += WAVE) {
+        const int s = c + lane;
+        const bool act = s < P.n_off && mlp_bare(P, tb, x, s) != 0.0f;
+        const unsigned long long m = __ballot(act);
+        if (act) cand[base + __popcll(m & ((1ull << lane) - 1ull))] = s;
+        base += __popcll(m);
+    }
+    return base;
+}
+
+// z = b1 + W1 x from scratch, eight consecutive hidden units per lane
+__device__ void mlp_z_init(const DevProblem &P, const signed char *x, double *z1, int lane) {
+    for (int e0 = lane * 8; e0 < P.mlp_h1; e0 += WAVE * 8) {
+        double z[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) z[q] = P.mlp_b1[e0 + q];
+        for (int n = 0; n < P.N; ++n) {
+            const uint4 v = *(const uint4 *)(P.mlp_W1T + (size_t)n * P.mlp_h1 + e0);
+            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+            const double xn = (double)x[n];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                z[q] += xn * (double)bf16_val((uint16_t)(wv[q >> 1] >> ((q & 1) * 16)));
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) z1[e0 + q] = z[q];
+    }
+}
+
+// z += the change of flipping site i (j >= 0 too) at the pre-flip x
+__device__ void mlp_z_move(const DevProblem &P, const signed char *x, double *z1, int i, int j,
+                           int lane) {
+    const double ci = -2.0 * (double)x[i], cj = j >= 0 ? -2.0 * (double)x[j] : 0.0;
+    for (int e = lane; e < P.mlp_h1; e += WAVE) {
+        double z = z1[e] + ci * (double)bf16_val(P.mlp_W1T[(size_t)i * P.mlp_h1 + e]);
+        if (j >= 0) z += cj * (double)bf16_val(P.mlp_W1T[(size_t)j * P.mlp_h1 + e]);
+        z1[e] = z;
+    }
+}
+
+// start of a propagation / evaluation: spins to LDS, z from scratch, the
+// candidate rows (stay_only: the configuration row alone), walker state
+__global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_start(DevProblem P, Tables<float> tb,
+                                                             int64_t nw, const uint32_t *spins,
+                                                             const double *log_w, int stay_only,
+                                                             MlpWs W) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    signed char *x = (signed char *)lds + (size_t)wave * P.n_pad;
+    for (int64_t w = (int64_t)blockIdx.x * MLP_SW + wave; w < nw;
+         w += (int64_t)gridDim.x * MLP_SW) {
+        load_spins(P, spins + w * P.words, x, lane);
+        wave_sync();
+        mlp_z_init(P, x, W.z1 + (size_t)w * P.mlp_h1, lane);
+        const int c = stay_only ? 0 : mlp_candidates(P, tb, x, W.cand + (size_t)w * P.n_off, lane);
+        if (lane == 0) {
+            W.cnt[w] = c + 1;
+            W.active[w] = 1;
+            W.t[w] = 0.0;
+            W.lw[w] = log_w ? log_w[w] : 0.0;
+            W.eL[w] = 0.0;
+            W.it[w] = 0u;
+        }
+        wave_sync();
+    }
+}
+
+__device__ __forceinline__ int mlp_block_excl_scan(int v, int *sh, int *total) {
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    int incl = v;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int o = __shfl_up(incl, d, WAVE);
+        if (lane >= d) incl += o;
+    }
+    if (lane == WAVE - 1) sh[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int i = 0; i < (int)blockDim.x / WAVE; ++i) {
+            const int t = sh[i];
+            sh[i] = run;
+            run += t;
+        }
+        sh[WAVE] = run;
+    }
+    __syncthreads();
+    *total = sh[WAVE];
+    return sh[wave] + incl - v;
+}
+
+__global__ __launch_bounds__(1024) void k_mlp_scan1(int64_t nw, MlpWs W) {
+    __shared__ int sh[WAVE + 1];
+    const int64_t w = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    const int nb = (w < nw && W.active[w]) ? (W.cnt[w] + 31) / 32 : 0;
+    int tot;
+    const int ex = mlp_block_excl_scan(nb, sh, &tot);
+    if (w < nw) W.blk_off[w] = ex;
+    if (threadIdx.x == 0) W.part[blockIdx.x] = tot;
+}
+
+// exclusive scan of the n workgroup totals in place; the sum to total[0]
+__global__ __launch_bounds__(1024) void k_mlp_scan2(int n, MlpWs W) {
+    __shared__ int sh[WAVE + 1];
+    int carry = 0;
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + (int)threadIdx.x;
+        const int v = i < n ? W.part[i] : 0;
+        int tot;
+        const int ex = mlp_block_excl_scan(v, sh, &tot);
+        if (i < n) W.part[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) W.total[0] = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_mlp_scan3(int64_t nw, MlpWs W) {
+    const int64_t w = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    if (w >= nw || !W.active[w]) return;
+    const int off = W.blk_off[w] + W.part[blockIdx.x];
+    W.blk_off[w] = off;
+    const int nb = (W.cnt[w] + 31) / 32;
+    for (int q = 0; q < nb; ++q) W.blk_walker[off + q] = (int32_t)w;
+}
+
+// the network on every row block; NB = h2 / 32 column blocks per wavefront
+template <int NB>
+__global__ __launch_bounds__(MLP_GW * WAVE) void k_mlp_gemm(DevProblem P, const uint32_t *spins,
+                                                            MlpWs W) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float2 *tab = (float2 *)lds;
+    uint16_t *Bs = (uint16_t *)(lds + MLP_NT * sizeof(float2));
+    for (int i = threadIdx.x; i < MLP_NT; i += blockDim.x) tab[i] = P.mlp_tab[i];
+    const int total = W.total[0];
+    const int tiles = (total + MLP_GW - 1) / MLP_GW;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int H1 = P.mlp_h1, H2 = NB * 32;
+    const int half = lane >> 5, col = lane & 31;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int b = tile * MLP_GW + wave;
+        const bool bvalid = b < total;
+        int64_t w = 0;
+        int r0 = 0, cnt = 0;
+        if (bvalid) {
+            w = W.blk_walker[b];
+            r0 = 32 * (b - W.blk_off[w]);
+            cnt = W.cnt[w];
+        }
+        const int row = r0 + col;
+        const bool rvalid = bvalid && row < cnt;
+        int si = -1, sj = -1;
+        double ci = 0.0, cj = 0.0;
+        if (rvalid && row > 0) {
+            slot_sites(P, W.cand[(size_t)w * P.n_off + row - 1], &si, &sj);
+            const uint32_t *sp = spins + w * P.words;
+            ci = ((sp[si >> 5] >> (si & 31)) & 1u) ? -2.0 : 2.0;
+            if (sj >= 0) cj = ((sp[sj >> 5] >> (sj & 31)) & 1u) ? -2.0 : 2.0;
+        }
+        const double *z1 = W.z1 + (size_t)w * H1;
+        const uint16_t *w1i = P.mlp_W1T + (size_t)(si < 0 ? 0 : si) * H1;
+        const uint16_t *w1j = P.mlp_W1T + (size_t)(sj < 0 ? 0 : sj) * H1;
+        mlp_f32x16 acc[NB];
+#pragma unroll
+        for (int t = 0; t < NB; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+        for (int kc = 0; kc < H1; kc += MLP_KC) {
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < H2 * (MLP_KC / 8); idx += blockDim.x) {
+                const int j = idx / (MLP_KC / 8), q = idx % (MLP_KC / 8);
+                *(uint4 *)(Bs + j * MLP_BSTR + q * 8) =
+                    *(const uint4 *)(P.mlp_W2 + (size_t)j * H1 + kc + q * 8);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < MLP_KC; ks += 16) {
+                const int k0 = kc + ks + 8 * half;
+                mlp_bf16x8 a;
+                if (rvalid) {
+                    const uint4 vi = si >= 0 ? *(const uint4 *)(w1i + k0) : make_uint4(0, 0, 0, 0);
+                    const uint4 vj = sj >= 0 ? *(const uint4 *)(w1j + k0) : make_uint4(0, 0, 0, 0);
+                    const uint32_t wi[4] = {vi.x, vi.y, vi.z, vi.w};
+                    const uint32_t wj[4] = {vj.x, vj.y, vj.z, vj.w};
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        double z = z1[k0 + q];
+                        z += ci * (double)bf16_val((uint16_t)(wi[q >> 1] >> ((q & 1) * 16)));
+                        z += cj * (double)bf16_val((uint16_t)(wj[q >> 1] >> ((q & 1) * 16)));
+                        a[q] = (__bf16)mlp_tanh(tab, (float)z);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) a[q] = (__bf16)0.0f;
+                }
+#pragma unroll
+                for (int t = 0; t < NB; ++t) {
+                    const mlp_bf16x8 bv =
+                        *(const mlp_bf16x8 *)(Bs + (t * 32 + col) * MLP_BSTR + ks + 8 * half);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        // epilogue: row i of register r is 8 (r / 4) + 4 half + r % 4, column
+        // t * 32 + col; per lane a sum over t, then a butterfly over col
+        float part[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) part[r] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            const int j = t * 32 + col;
+            const float bb = P.mlp_b2[j], ww = P.mlp_w3[j];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) part[r] = part[r] + ww * mlp_tanh(tab, acc[t][r] + bb);
+        }
+#pragma unroll
+        for (int d = 16; d >= 1; d >>= 1)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) part[r] = part[r] + __shfl_xor(part[r], d, WAVE);
+        if (bvalid && col == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 8 * (r / 4) + 4 * half + r % 4;
+                if (r0 + i < cnt) W.logpsi[(size_t)b * 32 + i] = part[r] + P.mlp_b3;
+            }
+        }
+    }
+}
+
+// one hop of every active walker (the loop body of the continuous-time
+// kernels) from the row values of this iteration
+__global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_select(
+    DevProblem P, Tables<float> tb, int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
+    double beta, double alw, uint32_t key0, uint32_t key1, uint32_t step, uint32_t tag,
+    int32_t *evals, MlpWs W) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    WalkerLds<float> L = carve<float>(P, lds + (size_t)wave * P.rbm_wave_bytes);
+    for (int64_t w = (int64_t)blockIdx.x * MLP_SW + wave; w < nw;
+         w += (int64_t)gridDim.x * MLP_SW) {
+        if (!W.active[w]) continue;
+        uint32_t *sp = spins + w * P.words;
+        load_spins(P, sp, L.x, lane);
+        for (int s = lane; s < P.K * WAVE; s += WAVE) L.w[s] = 0.0f;
+        wave_sync();
+        const size_t base = (size_t)W.blk_off[w] * 32;
+        const int cnt = W.cnt[w];
+        const float lp0 = W.logpsi[base];
+        int32_t *cand = W.cand + (size_t)w * P.n_off;
+        for (int c = lane; c < cnt - 1; c += WAVE) {
+            const int s = cand[c];
+            const float lr = W.logpsi[base + 1 + c] - lp0;
+            L.w[wpos(s, P.K)] = -(mlp_bare(P, tb, L.x, s) * __expf(lr));
+        }
+        int cnt_anti = 0;
+        if (P.has_jz) {
+            for (int e = lane; e < P.n_edges; e += WAVE) {
+                const int pk = P.edge_pack[e];
+                const unsigned char a = L.x[pk & 0xffff] != L.x[(pk >> 16) & 0xffff];
+                if (P.flags) L.anti[e] = a;
+                cnt_anti += a;
+            }
+        }
+        wave_sync();
+        const int n_anti = wave_sum_int(cnt_anti);
+        const double diag = diag_value(P, L.x, L.anti, n_anti, lane);
+        double t = W.t[w], lw = W.lw[w];
+        uint32_t it = W.it[w];
+        float lane_sum;
+        const float incl = lane_scan(P, L, lane, &lane_sum);
+        const float R = lane_bcast(incl, WAVE - 1);
+        const double eL = diag - (double)R;
+        const gfmc_u4 r = gfmc_philox(it, (uint32_t)w, step, tag, key0, key1);
+        const double psi = gfmc_u53(r.z, r.w);
+        int idx = -1;
+        double tau_raw = INFINITY;
+        if (R > 0.0f) {
+            float u = gfmc_u24(r.x) * R;
+            const float cap = below(R);
+            if (u > cap) u = cap;
+            idx = sample_slot(P, L, incl, lane_sum, u, lane);
+            tau_raw = -log1p(-psi) / (double)R;
+        }
+        const double rem = beta - t;
+        const double tau = tau_raw < rem ? tau_raw : rem;
+        lw = lw - eL * tau;
+        t = t + tau;
+        ++it;
+        if (tau_raw < rem && idx >= 0) {
+            int i, j;
+            slot_sites(P, idx, &i, &j);
+            mlp_z_move(P, L.x, W.z1 + (size_t)w * P.mlp_h1, i, j, lane);
+            wave_sync();
+            if (lane == 0) {
+                L.x[i] = (signed char)-L.x[i];
+                if (j >= 0) L.x[j] = (signed char)-L.x[j];
+            }
+            wave_sync();
+            store_spins(P, sp, L.x, lane);
+            const int c = mlp_candidates(P, tb, L.x, cand, lane);
+            if (lane == 0) W.cnt[w] = c + 1;
+        }
+        const bool more = t < beta;
+        if (lane == 0) {
+            W.t[w] = t;
+            W.lw[w] = lw;
+            W.eL[w] = eL;
+            W.it[w] = it;
+            W.active[w] = more ? 1 : 0;
+            if (!more) {
+                log_w[w] = lw - alw * beta;
+                e_loc[w] = eL;
+                if (evals) evals[w] += (int32_t)it;
+            }
+        }
+        wave_sync();
+    }
+}
+
+// rates / log-differences / e_L / log psi of every walker from one
+// evaluation's rows (gfmc_rates, gfmc_log_psi)
+__global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_rates(DevProblem P, Tables<float> tb,
+                                                             int64_t nw, const uint32_t *spins,
+                                                             double *rates, double *e_loc,
+                                                             double *log_diff, double *log_psi,
+                                                             MlpWs W) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    WalkerLds<float> L = carve<float>(P, lds + (size_t)wave * P.rbm_wave_bytes);
+    const int64_t M = P.n_off + 1;
+    for (int64_t w = (int64_t)blockIdx.x * MLP_SW + wave; w < nw;
+         w += (int64_t)gridDim.x * MLP_SW) {
+        load_spins(P, spins + w * P.words, L.x, lane);
+        for (int s = lane; s < P.K * WAVE; s += WAVE) L.w[s] = 0.0f;
+        wave_sync();
+        const size_t base = (size_t)W.blk_off[w] * 32;
+        const int cnt = W.cnt[w];
+        const float lp0 = W.logpsi[base];
+        if (log_psi && lane == 0) log_psi[w] = (double)lp0;
+        if (!(rates || e_loc || log_diff)) continue;
+        const int32_t *cand = W.cand + (size_t)w * P.n_off;
+        if (log_diff)
+            for (int s = lane; s < P.n_off; s += WAVE) log_diff[w * M + 1 + s] = 0.0;
+        wave_sync();
+        for (int c = lane; c < cnt - 1; c += WAVE) {
+            const int s = cand[c];
+            const float lr = W.logpsi[base + 1 + c] - lp0;
+            L.w[wpos(s, P.K)] = -(mlp_bare(P, tb, L.x, s) * __expf(lr));
+            if (log_diff) log_diff[w * M + 1 + s] = (double)lr;
+        }
+        int cnt_anti = 0;
+        if (P.has_jz) {
+            for (int e = lane; e < P.n_edges; e += WAVE) {
+                const int pk = P.edge_pack[e];
+                const unsigned char a = L.x[pk & 0xffff] != L.x[(pk >> 16) & 0xffff];
+                if (P.flags) L.anti[e] = a;
+                cnt_anti += a;
+            }
+        }
+        wave_sync();
+        const int n_anti = wave_sum_int(cnt_anti);
+        const double diag = diag_value(P, L.x, L.anti, n_anti, lane);
+        float lane_sum;
+        const float incl = lane_scan(P, L, lane, &lane_sum);
+        const float R = lane_bcast(incl, WAVE - 1);
+        if (rates) {
+            if (lane == 0) rates[w * M] = diag;
+            for (int s = lane; s < P.n_off; s += WAVE)
+                rates[w * M + 1 + s] = -(double)L.w[wpos(s, P.K)];
+        }
+        if (log_diff && lane == 0) log_diff[w * M] = 0.0;
+        if (e_loc && lane == 0) e_loc[w] = diag - (double)R;
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static MlpWork *new_mlp_work(int blocks_per_walker) {
+    MlpWork *m = new MlpWork;
+    m->bpw = blocks_per_walker;
+    return m;
+}
+
+static void free_mlp_work(MlpWork *m) {
+    if (!m) return;
+    for (void *a : m->allocs) (void)hipFree(a);
+    if (m->host_total) (void)hipHostFree(m->host_total);
+    delete m;
+}
+
+static int mlp_reserve(const gfmc_problem *p, int64_t nw) {
+    MlpWork *m = p->mlp;
+    if (m->cap >= nw) return 0;
+    if ((int64_t)nw * m->bpw > 0x7fffffffLL / 32)
+        return fail(GFMC_EUNSUP, "MLP guide: too many walkers x candidate rows for one launch");
+    for (void *a : m->allocs) (void)hipFree(a);
+    m->allocs.clear();
+    m->cap = 0;
+    const DevProblem &P = p->dp;
+    auto get = [&](size_t bytes, void **dst) -> int {
+        void *a = nullptr;
+        if (hipMalloc(&a, bytes ? bytes : 16) != hipSuccess)
+            return fail(GFMC_ENOMEM, "MLP guide: workspace allocation failed");
+        m->allocs.push_back(a);
+        *dst = a;
+        return 0;
+    };
+    MlpWs &W = m->ws;
+    const size_t n = (size_t)nw, nb = n * (size_t)m->bpw;
+    int rc = 0;
+    if (!rc) rc = get(n * P.mlp_h1 * 8, (void **)&W.z1);
+    if (!rc) rc = get(n * (size_t)(P.n_off > 0 ? P.n_off : 1) * 4, (void **)&W.cand);
+    if (!rc) rc = get(n * 4, (void **)&W.cnt);
+    if (!rc) rc = get(n * 4, (void **)&W.active);
+    if (!rc) rc = get(n * 8, (void **)&W.t);
+    if (!rc) rc = get(n * 8, (void **)&W.lw);
+    if (!rc) rc = get(n * 8, (void **)&W.eL);
+    if (!rc) rc = get(n * 4, (void **)&W.it);
+    if (!rc) rc = get(n * 4, (void **)&W.blk_off);
+    if (!rc) rc = get(nb * 4, (void **)&W.blk_walker);
+    if (!rc) rc = get((n / 1024 + 2) * 4, (void **)&W.part);
+    if (!rc) rc = get(16, (void **)&W.total);
+    if (!rc) rc = get(nb * 32 * 4, (void **)&W.logpsi);
+    if (rc) {
+        for (void *a : m->allocs) (void)hipFree(a);
+        m->allocs.clear();
+        return rc;
+    }
+    if (!m->host_total) HIP_TRY(hipHostMalloc((void **)&m->host_total, 16, hipHostMallocDefault));
+    m->cap = nw;
+    return 0;
+}
+
+static size_t mlp_walker_lds(const gfmc_problem *p) { return (size_t)MLP_SW * p->dp.rbm_wave_bytes; }
+
+static size_t mlp_gemm_lds(const gfmc_problem *p) {
+    return (size_t)MLP_NT * sizeof(float2) + (size_t)p->dp.mlp_h2 * MLP_BSTR * 2;
+}
+
+static unsigned mlp_walker_grid(const gfmc_problem *p, int64_t nw) {
+    const int64_t need = (nw + MLP_SW - 1) / MLP_SW;
+    const int64_t cap = (int64_t)p->n_cu * 8;
+    return (unsigned)(need < cap ? need : cap);
+}
+
+static void launch_mlp_gemm(const gfmc_problem *p, const uint32_t *spins, int total,
+                            hipStream_t st) {
+    const int tiles = (total + MLP_GW - 1) / MLP_GW;
+    const int cap = p->n_cu * 2;
+    const unsigned grid = (unsigned)(tiles < cap ? tiles : cap);
+    const size_t lds = mlp_gemm_lds(p);
+    const MlpWs &W = p->mlp->ws;
+    switch (p->dp.mlp_h2) {
+    case 32: hipLaunchKernelGGL(k_mlp_gemm<1>, dim3(grid), dim3(MLP_GW * WAVE), lds, st, p->dp, spins, W); break;
+    case 64: hipLaunchKernelGGL(k_mlp_gemm<2>, dim3(grid), dim3(MLP_GW * WAVE), lds, st, p->dp, spins, W); break;
+    case 128: hipLaunchKernelGGL(k_mlp_gemm<4>, dim3(grid), dim3(MLP_GW * WAVE), lds, st, p->dp, spins, W); break;
+    default: hipLaunchKernelGGL(k_mlp_gemm<8>, dim3(grid), dim3(MLP_GW * WAVE), lds, st, p->dp, spins, W); break;
+    }
+}
+
+// the row blocks of the active walkers; returns their number (host sync)
+static int mlp_scan(const gfmc_problem *p, int64_t nw, hipStream_t st, int *total) {
+    const MlpWs &W = p->mlp->ws;
+    const unsigned nb = (unsigned)((nw + 1023) / 1024);
+    hipLaunchKernelGGL(k_mlp_scan1, dim3(nb), dim3(1024), 0, st, nw, W);
+    hipLaunchKernelGGL(k_mlp_scan2, dim3(1), dim3(1024), 0, st, (int)nb, W);
+    hipLaunchKernelGGL(k_mlp_scan3, dim3(nb), dim3(1024), 0, st, nw, W);
+    HIP_TRY(hipMemcpyAsync(p->mlp->host_total, W.total, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *total = p->mlp->host_total[0];
+    return 0;
+}
+
+static int mlp_start(const gfmc_problem *p, int64_t nw, const uint32_t *spins,
+                     const double *log_w, int stay_only, hipStream_t st) {
+    int rc = mlp_reserve(p, nw);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mlp_start, dim3(mlp_walker_grid(p, nw)), dim3(MLP_SW * WAVE),
+                       (size_t)MLP_SW * p->dp.n_pad, st, p->dp, p->t32, nw, spins, log_w,
+                       stay_only, p->mlp->ws);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int mlp_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                         double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
+                         uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st) {
+    if (g_step_base)
+        return fail(GFMC_EUNSUP, "the MLP guide's propagation synchronises with the host once "
+                                 "per hop and cannot be captured into a graph");
+    int rc = mlp_start(p, nw, spins, log_w, 0, st);
+    if (rc) return rc;
+    for (int64_t iter = 0;; ++iter) {
+        int total = 0;
+        rc = mlp_scan(p, nw, st, &total);
+        if (rc) return rc;
+        if (total == 0) break;
+        if (iter >= 10000000)
+            return fail(GFMC_EINVAL, "MLP guide: a walker did not finish its branching step "
+                                     "in 10^7 hops (non-finite rates?)");
+        launch_mlp_gemm(p, spins, total, st);
+        hipLaunchKernelGGL(k_mlp_select, dim3(mlp_walker_grid(p, nw)), dim3(MLP_SW * WAVE),
+                           mlp_walker_lds(p), st, p->dp, p->t32, nw, spins, log_w, e_loc, beta,
+                           alw, k0, k1, step, tag, evals, p->mlp->ws);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+static int mlp_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double *rates,
+                     double *e_loc, double *log_diff, double *log_psi, hipStream_t st) {
+    if (g_step_base)
+        return fail(GFMC_EUNSUP, "the MLP guide's evaluation cannot be captured into a graph");
+    const int stay_only = !(rates || e_loc || log_diff);
+    int rc = mlp_start(p, nw, spins, nullptr, stay_only, st);
+    if (rc) return rc;
+    int total = 0;
+    rc = mlp_scan(p, nw, st, &total);
+    if (rc) return rc;
+    launch_mlp_gemm(p, spins, total, st);
+    hipLaunchKernelGGL(k_mlp_rates, dim3(mlp_walker_grid(p, nw)), dim3(MLP_SW * WAVE),
+                       mlp_walker_lds(p), st, p->dp, p->t32, nw, spins, rates, e_loc, log_diff,
+                       log_psi, p->mlp->ws);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}

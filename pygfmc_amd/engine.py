@@ -421,6 +421,8 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     dev = chain.spins.device
     series = torch.empty((n_steps, 2), dtype=torch.float64, device=dev)
     m_array = [q // 2 for q in p_sorted]
+    # the MLP guide's propagation synchronises with the host once per hop
+    graphable = p.guide_kind != _lib.GUIDE_MLP
     diag = None
     dseries = None
     if diag_ops:
@@ -442,7 +444,8 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
             n = min(print_every, n_steps - step0)
             chain.evals.zero_()
             drows = dseries[step0:step0 + n] if diag is not None else None
-            if use_graph and n == print_every and n_steps >= 2 * print_every:
+            if (use_graph and graphable and n == print_every
+                    and n_steps >= 2 * print_every):
                 if graph is None:
                     graph = StepGraph(chain, n, step_f, alw, key.seed, evals=True, diag=diag)
                 graph.launch(key.position + step0, out=series[step0:step0 + n], dout=drows)

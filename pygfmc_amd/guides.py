@@ -201,3 +201,83 @@ class RBMSpinHalf:
     def log_diff_from_configs(self, config, new_configs, cache=None):
         new = np.asarray(new_configs, dtype=float)
         return self.log_psi(new) - self.log_psi(np.asarray(config, dtype=float)[None])[0]
+
+
+def bf16_flush(a):
+    """BF16 rounding (nearest even) with magnitudes below 2^-24 flushed to 0:
+    the grid the MLP guide's W1, b1 and W2 live on."""
+    r = bf16_round(a)
+    return np.where(np.abs(r) < np.float32(2.0 ** -24), np.float32(0.0), r).astype(np.float32)
+
+
+def mlp_tanh_table():
+    """(T, D) of the tabulated tanh: T_j = tanh(j / 256) and D_j = T_{j+1} - T_j
+    in FP32, j = 0..2047."""
+    j = np.arange(2049, dtype=np.float64)
+    T = np.tanh(j / 256.0).astype(np.float32)
+    return T[:-1].copy(), (T[1:] - T[:-1]).astype(np.float32)
+
+
+class MLPSpinHalf:
+    """Two-layer perceptron guide (no counterpart in the reference, whose
+    network guide is ClusterMLP, wavefunction/cluster_mlp.py:133-439):
+
+        log psi(x) = b3 + sum_c w3_c s(b2_c + sum_k W2_ck h_k),
+        h_k = bf16(s(b1_k + sum_n W1_kn x_n)),
+
+    ``s`` the tabulated tanh (2048 linear pieces on [0, 8), odd, saturating
+    at tanh(8)), ``bf16`` rounding to nearest even.  W1, b1 and W2 are held on
+    the BF16 grid (magnitudes below 2^-24 flushed to 0); b2, w3 and b3 are
+    FP32.  ``hidden = (h1, h2)``: h1 a multiple of 32, h2 one of 32, 64, 128,
+    256.
+    """
+
+    def __init__(self, N, hidden=(512, 256), W1=None, b1=None, W2=None, b2=None, w3=None,
+                 b3=0.0, seed=0, scale=1.0, out_scale=0.1, dtype=np.float32):
+        self.N = int(N)
+        h1, h2 = (int(hidden[0]), int(hidden[1]))
+        if h1 <= 0 or h1 % 32 or h2 not in (32, 64, 128, 256):
+            raise ValueError(f"hidden must be (multiple of 32, one of 32/64/128/256); got "
+                             f"{hidden!r}")
+        self.hidden = (h1, h2)
+        rng = np.random.default_rng(seed)
+        if W1 is None:
+            W1 = rng.normal(scale=scale / np.sqrt(self.N), size=(h1, self.N))
+        if W2 is None:
+            W2 = rng.normal(scale=scale / np.sqrt(h1), size=(h2, h1))
+        if w3 is None:
+            w3 = rng.normal(scale=out_scale / np.sqrt(h2), size=h2)
+        self.W1 = bf16_flush(np.asarray(W1, dtype=np.float32).reshape(h1, self.N))
+        self.W2 = bf16_flush(np.asarray(W2, dtype=np.float32).reshape(h2, h1))
+        self.b1 = bf16_flush(np.zeros(h1) if b1 is None else np.asarray(b1, dtype=np.float32))
+        self.b2 = np.zeros(h2, np.float32) if b2 is None else np.asarray(b2, np.float32).copy()
+        self.w3 = np.asarray(w3, dtype=np.float32).reshape(h2).copy()
+        self.b3 = np.float32(b3)
+        self.dtype = np.dtype(dtype)
+        if self.b1.shape != (h1,) or self.b2.shape != (h2,):
+            raise ValueError("b1 must have shape (h1,) and b2 shape (h2,)")
+
+    def parameters(self):
+        return dict(W1=self.W1.copy(), b1=self.b1.copy(), W2=self.W2.copy(), b2=self.b2.copy(),
+                    w3=self.w3.copy(), b3=np.float32(self.b3))
+
+    def with_parameters(self, W1, b1, W2, b2, w3, b3):
+        return MLPSpinHalf(self.N, self.hidden, W1=W1, b1=b1, W2=W2, b2=b2, w3=w3, b3=b3,
+                           dtype=self.dtype)
+
+    def fingerprint(self):
+        return (b"mlp|" + np.asarray([self.N, *self.hidden], np.int64).tobytes()
+                + self.W1.tobytes() + self.b1.tobytes() + self.W2.tobytes()
+                + self.b2.tobytes() + self.w3.tobytes() + np.float32(self.b3).tobytes())
+
+    def log_psi(self, config, cache=None):
+        from .problem import device_log_psi
+        return device_log_psi(self, config)
+
+    def log_diff_from_operator(self, config, op, cache=None):
+        from .problem import device_log_diff
+        return device_log_diff(op, self, config)
+
+    def log_diff_from_configs(self, config, new_configs, cache=None):
+        new = np.asarray(new_configs, dtype=float)
+        return self.log_psi(new) - self.log_psi(np.asarray(config, dtype=float)[None])[0]

@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .guides import JastrowSpinHalf, RBMSpinHalf
+from .guides import JastrowSpinHalf, MLPSpinHalf, RBMSpinHalf
 from .hamiltonian import SpinHalf
 
 __all__ = ["lower", "Problem", "get_problem"]
@@ -93,9 +93,10 @@ class Lowered:
         if not isinstance(H, SpinHalf):
             raise TypeError(f"pygfmc_amd propagates SpinHalf Hamiltonians; got "
                             f"{type(H).__name__}.")
-        if guide is not None and not isinstance(guide, (JastrowSpinHalf, RBMSpinHalf)):
-            raise TypeError(f"guide must be None, JastrowSpinHalf or RBMSpinHalf, got "
-                            f"{type(guide).__name__}")
+        if guide is not None and not isinstance(guide, (JastrowSpinHalf, RBMSpinHalf,
+                                                        MLPSpinHalf)):
+            raise TypeError(f"guide must be None, JastrowSpinHalf, RBMSpinHalf or MLPSpinHalf, "
+                            f"got {type(guide).__name__}")
         if guide is not None and guide.N != H.N:
             raise ValueError(f"guide has N={guide.N} sites, the Hamiltonian {H.N}")
         self.H, self.guide = H, guide
@@ -110,7 +111,15 @@ class Lowered:
         a["hx_half"] = _f64(H.per_site(H.h_x_half)) if H.n_hx else None
         cache = None
         rbm = isinstance(guide, RBMSpinHalf)
-        if rbm:
+        mlp = isinstance(guide, MLPSpinHalf)
+        if mlp:
+            a["mlp_W1"] = np.ascontiguousarray(guide.W1.reshape(-1), dtype=np.float32)
+            a["mlp_b1"] = np.ascontiguousarray(guide.b1, dtype=np.float32)
+            a["mlp_W2"] = np.ascontiguousarray(guide.W2.reshape(-1), dtype=np.float32)
+            a["mlp_b2"] = np.ascontiguousarray(guide.b2, dtype=np.float32)
+            a["mlp_w3"] = np.ascontiguousarray(guide.w3, dtype=np.float32)
+            a["mlp_b3"] = np.asarray([guide.b3], dtype=np.float32)
+        elif rbm:
             a["rbm_W"] = np.ascontiguousarray(guide.W.reshape(-1), dtype=np.float32)
             a["rbm_a"] = np.ascontiguousarray(guide.a, dtype=np.float32)
             a["rbm_b"] = np.ascontiguousarray(guide.b, dtype=np.float32)
@@ -122,8 +131,8 @@ class Lowered:
             a["jas_edge_other"] = _i32(cache.edge_other.reshape(-1))
             a["jas_edge_J"] = _f64(cache.edge_J.reshape(-1))
         n_off = H.n_off
-        if rbm:
-            full = True         # theta changes everywhere: every slot after every move
+        if rbm or mlp:
+            full = True         # the network changes everywhere: every slot after every move
         else:
             dep = dependency_lists(H, guide, cache)
             if full is None:
@@ -134,10 +143,14 @@ class Lowered:
         self.scalars = dict(n_sites=H.N, n_edges=E, n_hx=H.n_hx, n_jpm=H.n_jpm,
                             has_jz=int(H.has_jz), uniform_jz=int(H.uniform_jz),
                             has_hz=int(H.has_hz),
-                            guide=(_lib.GUIDE_RBM if rbm else _lib.GUIDE_JASTROW
-                                   if guide is not None else _lib.GUIDE_NONE),
-                            jas_max_deg=guide.max_deg if (guide is not None and not rbm) else 0,
-                            rbm_hidden=guide.hidden if rbm else 0)
+                            guide=(_lib.GUIDE_RBM if rbm else _lib.GUIDE_MLP if mlp
+                                   else _lib.GUIDE_JASTROW if guide is not None
+                                   else _lib.GUIDE_NONE),
+                            jas_max_deg=(guide.max_deg if isinstance(guide, JastrowSpinHalf)
+                                         else 0),
+                            rbm_hidden=guide.hidden if rbm else 0,
+                            mlp_h1=guide.hidden[0] if mlp else 0,
+                            mlp_h2=guide.hidden[1] if mlp else 0)
         self.full = bool(full)
 
     def desc(self):
@@ -178,6 +191,7 @@ class Problem:
         self.n_sites = info.n_sites
         self.n_words = info.n_words
         self.m_total = info.m_total
+        self.guide_kind = int(info.guide)
 
     def __del__(self):
         try:
@@ -253,7 +267,7 @@ def get_problem(H, guide=None, precision=None, full=None):
     """Cached :class:`Problem` for (H, guide, precision).  The RBM guide runs
     in FP32 on BF16 weights whatever the Hamiltonian's dtype."""
     prec = precision_of(H, precision)
-    if precision is None and isinstance(guide, RBMSpinHalf):
+    if precision is None and isinstance(guide, (RBMSpinHalf, MLPSpinHalf)):
         prec = _lib.GFMC_FP32
     h = hashlib.sha1(H.fingerprint())
     h.update(b"none" if guide is None else guide.fingerprint())
@@ -299,7 +313,8 @@ def device_log_diff(H, guide, config):
 def device_log_psi(guide, config):
     # a field-free Hamiltonian on no edges carries the guide's tables
     H = SpinHalf.create([], N=guide.N, dtype=np.float64)
-    p = get_problem(H, guide, precision=(_lib.GFMC_FP32 if isinstance(guide, RBMSpinHalf)
+    p = get_problem(H, guide, precision=(_lib.GFMC_FP32
+                                         if isinstance(guide, (RBMSpinHalf, MLPSpinHalf))
                                          else _lib.GFMC_FP64))
     spins, single = _configs_to_spins(config, guide.N)
     out = p.log_psi(spins).cpu().numpy()
