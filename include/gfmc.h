@@ -114,7 +114,7 @@ typedef struct gfmc_problem_desc {
      * s the tabulated tanh (2048 linear pieces on [0, 8), odd, saturated).
      * W1, b1 and W2 are rounded to the BF16 grid (magnitudes below 2^-24
      * flushed to 0), so the layer-1 sums are exact in FP64; layer 2 runs on
-     * MFMA (BF16 inputs, FP32 accumulation).  h1 a multiple of 32, h2 one
+     * MFMA (BF16 inputs, FP32 accumulation).  h1 a multiple of 64, h2 one
      * of 32, 64, 128, 256. */
     int32_t mlp_h1, mlp_h2;
     const float *mlp_W1;           /* [h1*N] row-major                          */
@@ -392,6 +392,31 @@ int gfmc_rbm_vmin_grad(const gfmc_problem *p, int64_t n, const uint32_t *spins,
 int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *v,
                    int64_t step, float lr, float beta1, float beta2, float eps,
                    void *stream);
+
+/* ---- MLP guide training ----------------------------------------------------
+ * No counterpart in the reference (see the RBM section above).  Parameters
+ * travel as one flat FP32 vector [W1 (h1 x n_sites, row-major), b1 (h1),
+ * W2 (h2 x h1, row-major), b2 (h2), w3 (h2), b3] of gfmc_mlp_n_params
+ * elements; the problem holds W1, b1, W2 on the BF16 grid.  The gradient
+ * call synchronises the caller's stream with the host once (it sizes the
+ * candidate rows) and grows a workspace the problem owns. */
+
+/* h1*N + h1 + h2*h1 + 2*h2 + 1 of an MLP problem, -1 otherwise. */
+int64_t gfmc_mlp_n_params(const gfmc_problem *p);
+
+/* The problem's parameters from a flat device vector (W1, b1, W2 rounded to
+ * BF16, nearest even, magnitudes below 2^-24 flushed to 0). */
+int gfmc_mlp_set_params(gfmc_problem *p, const float *flat, void *stream);
+
+/* Gradient of L = (1/n) sum_i (E_i - mean E)^2 over n packed samples, as
+ * gfmc_rbm_vmin_grad: dL/dp = sum_i g_i sum_k r_ik (O_p(x_i^k) - O_p(x_i)),
+ * O = d log psi / dp with straight-through BF16 rounding of the layer-1
+ * activations and the slopes of the tabulated tanh; g_i = 2 (E_i - mean)/n,
+ * clamped to +-clip*sqrt(L) when clip > 0.  Layer-2 products on BF16 MFMA
+ * (FP32 operands split hi + lo).  grad: flat FP32 [n_params]; out2 (device)
+ * <- (mean E, L).  Deterministic (fixed-order reductions). */
+int gfmc_mlp_vmin_grad(const gfmc_problem *p, int64_t n, const uint32_t *spins, float clip,
+                       float *grad, double *out2, void *stream);
 
 /* ---- Diagonal operators measured by forward walking ---------------------
  * The diagonal half of the production step (driver.py:240-288,

@@ -102,6 +102,8 @@ def lib():
         L.orc_bf16_flush.restype = ctypes.c_float
         L.orc_bf16_flush.argtypes = [ctypes.c_float]
         L.orc_mlp_hidden.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f32p]
+        L.orc_mlp_O.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f64p]
+        L.orc_mlp_vmin_grad.argtypes = [D, ctypes.c_int64, _u32p, ctypes.c_double, _f64p, _f64p]
         del vp
         _LIB = L
     return _LIB
@@ -143,6 +145,28 @@ class Problem:
                               _p(r, ctypes.c_double), _p(e, ctypes.c_double),
                               _p(ld, ctypes.c_double))
         return r, e, ld
+
+    def mlp_n_params(self):
+        sc = self.lowered.scalars
+        h1, h2 = int(sc["mlp_h1"]), int(sc["mlp_h2"])
+        return h1 * self.N + h1 + h2 * h1 + 2 * h2 + 1
+
+    def mlp_O(self, config):
+        """d log psi / dp of one +-1 configuration (flat parameter order)."""
+        c = np.ascontiguousarray(np.asarray(config), dtype=np.int8)
+        out = np.empty(self.mlp_n_params())
+        lib().orc_mlp_O(ctypes.byref(self.desc), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                        _p(out, ctypes.c_double))
+        return out
+
+    def mlp_vmin_grad(self, spins, clip=0.0):
+        """(flat gradient, mean E, variance) of the local-energy variance."""
+        s = np.ascontiguousarray(spins, dtype=np.uint32)
+        g = np.empty(self.mlp_n_params())
+        out2 = np.empty(2)
+        lib().orc_mlp_vmin_grad(ctypes.byref(self.desc), s.shape[0], _p(s, ctypes.c_uint32),
+                                float(clip), _p(g, ctypes.c_double), _p(out2, ctypes.c_double))
+        return g, float(out2[0]), float(out2[1])
 
     def mlp_hidden(self, config):
         """Layer-1 activations (BF16 values) of +-1 configurations."""

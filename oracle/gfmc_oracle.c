@@ -234,8 +234,8 @@ static double rbm_log_ratio(const gfmc_problem_desc *d, const int8_t *x, const d
 /* tabulated tanh in FP32 with an explicit fma, h rounded to BF16; the       */
 /* layer-2 sums in FP64 rounded once to FP32 (the device accumulates them in */
 /* FP32 on MFMA: the one step not mirrored bit for bit), then the output     */
-/* sum in the device's order: per column c mod 32 over c / 32, then a        */
-/* butterfly over the 32 columns.                                             */
+/* sum in the device's order: per column c mod 16 over c / 16, then a        */
+/* butterfly over the 16 columns.                                             */
 /* ------------------------------------------------------------------------ */
 #define MLP_NT 2048
 static float mlp_T[MLP_NT], mlp_D[MLP_NT];
@@ -257,13 +257,13 @@ void orc_mlp_table(float *T, float *D) {
     memcpy(D, mlp_D, sizeof mlp_D);
 }
 
+/* s(z) = sign(z) (T_j + f D_j), t = min(256 |z|, 2048 - 2^-13), j = floor(t),
+ * f = t - j */
 float orc_mlp_tanh(float z) {
     mlp_table();
-    float a = fabsf(z);
-    float t = fminf(a * 256.0f, (float)MLP_NT);
+    float t = fminf(fabsf(z) * 256.0f, 2047.9998779296875f);
     int j = (int)t;
-    if (j > MLP_NT - 1) j = MLP_NT - 1;
-    float f = t - (float)j;
+    float f = t - floorf(t);
     return copysignf(fmaf(f, mlp_D[j], mlp_T[j]), z);
 }
 
@@ -293,23 +293,23 @@ void orc_mlp_hidden(const gfmc_problem_desc *d, const int8_t *x, float *h) {
 }
 
 float orc_mlp_log_psi(const gfmc_problem_desc *d, const int8_t *x) {
-    int H1 = d->mlp_h1, H2 = d->mlp_h2, NB = H2 / 32;
+    int H1 = d->mlp_h1, H2 = d->mlp_h2, NC = H2 / 16;
     float *h = (float *)malloc(sizeof(float) * (size_t)H1);
     orc_mlp_hidden(d, x, h);
-    float s[32];
-    for (int l = 0; l < 32; ++l) s[l] = 0.0f;
-    for (int t = 0; t < NB; ++t)
-        for (int l = 0; l < 32; ++l) {
-            int c = t * 32 + l;
+    float s[16];
+    for (int l = 0; l < 16; ++l) s[l] = 0.0f;
+    for (int t = 0; t < NC; ++t)
+        for (int l = 0; l < 16; ++l) {
+            int c = t * 16 + l;
             double z2 = 0.0;
             for (int k = 0; k < H1; ++k)
                 z2 += (double)orc_bf16_flush(d->mlp_W2[(size_t)c * H1 + k]) * (double)h[k];
             float v = d->mlp_w3[c] * orc_mlp_tanh((float)z2 + d->mlp_b2[c]);
             s[l] = s[l] + v;
         }
-    for (int dd = 16; dd >= 1; dd >>= 1) {
-        float o[32];
-        for (int l = 0; l < 32; ++l) o[l] = s[l] + s[l ^ dd];
+    for (int dd = 8; dd >= 1; dd >>= 1) {
+        float o[16];
+        for (int l = 0; l < 16; ++l) o[l] = s[l] + s[l ^ dd];
         memcpy(s, o, sizeof o);
     }
     free(h);
@@ -968,4 +968,110 @@ void orc_diag_values(const gfmc_diag_desc *d, int64_t nw, const uint32_t *spins,
         for (int v = 0; v < V; ++v) out[i * V + v] = diag_column(d, x, v);
     }
     free(x);
+}
+
+/* ------------------------------------------------------------------------ */
+/* MLP guide training: O(x) = d log psi(x) / dp for the flat parameter      */
+/* vector [W1, b1, W2, b2, w3, b3] by backpropagation in FP64, with the     */
+/* straight-through BF16 rounding of h and the slopes of the tabulated tanh */
+/* (256 D_j below saturation, 0 at it), and the variance gradient by its    */
+/* definition, every O(x') from scratch (no reference counterpart).         */
+/* ------------------------------------------------------------------------ */
+static double mlp_slope_of(float z) {
+    float t = fminf(fabsf(z) * 256.0f, 2047.9998779296875f);
+    if (t == 2047.9998779296875f) return 0.0;
+    mlp_table();
+    return 256.0 * (double)mlp_D[(int)t];
+}
+
+static void mlp_O(const gfmc_problem_desc *d, const int8_t *x, double *O) {
+    int N = d->n_sites, H1 = d->mlp_h1, H2 = d->mlp_h2;
+    size_t oW1 = 0, ob1 = (size_t)H1 * N, oW2 = ob1 + H1, ob2 = oW2 + (size_t)H2 * H1;
+    size_t ow3 = ob2 + H2, ob3 = ow3 + H2;
+    float *h = (float *)malloc(sizeof(float) * (size_t)H1);
+    double *s1 = (double *)malloc(sizeof(double) * (size_t)H1);
+    double *d2 = (double *)malloc(sizeof(double) * (size_t)H2);
+    for (int k = 0; k < H1; ++k) {
+        double z = (double)orc_bf16_flush(d->mlp_b1[k]);
+        for (int n = 0; n < N; ++n)
+            z += (double)x[n] * (double)orc_bf16_flush(d->mlp_W1[(size_t)k * N + n]);
+        h[k] = bf16_rne(orc_mlp_tanh((float)z));
+        s1[k] = mlp_slope_of((float)z);
+    }
+    for (int c = 0; c < H2; ++c) {
+        double z2 = 0.0;
+        for (int k = 0; k < H1; ++k)
+            z2 += (double)orc_bf16_flush(d->mlp_W2[(size_t)c * H1 + k]) * (double)h[k];
+        float u = (float)z2 + d->mlp_b2[c];
+        O[ow3 + c] = (double)orc_mlp_tanh(u);
+        d2[c] = (double)d->mlp_w3[c] * mlp_slope_of(u);
+        O[ob2 + c] = d2[c];
+        for (int k = 0; k < H1; ++k) O[oW2 + (size_t)c * H1 + k] = d2[c] * (double)h[k];
+    }
+    O[ob3] = 1.0;
+    for (int k = 0; k < H1; ++k) {
+        double e = 0.0;
+        for (int c = 0; c < H2; ++c)
+            e += (double)orc_bf16_flush(d->mlp_W2[(size_t)c * H1 + k]) * d2[c];
+        double d1 = e * s1[k];
+        O[ob1 + k] = d1;
+        for (int n = 0; n < N; ++n) O[oW1 + (size_t)k * N + n] = d1 * (double)x[n];
+    }
+    free(h);
+    free(s1);
+    free(d2);
+}
+
+void orc_mlp_O(const gfmc_problem_desc *d, const int8_t *x, double *O) { mlp_O(d, x, O); }
+
+void orc_mlp_vmin_grad(const gfmc_problem_desc *d, int64_t n, const uint32_t *spins, double clip,
+                       double *grad, double *out2) {
+    int N = d->n_sites, W = n_words_of(N), n_off = n_off_of(d);
+    int M = 1 + n_off, H1 = d->mlp_h1, H2 = d->mlp_h2;
+    size_t P = (size_t)H1 * N + H1 + (size_t)H2 * H1 + 2 * (size_t)H2 + 1;
+    double *rates = (double *)malloc(sizeof(double) * (size_t)n * M);
+    double *E = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    int8_t *x = (int8_t *)malloc((size_t)N), *xs = (int8_t *)malloc((size_t)N);
+    double *O = (double *)malloc(sizeof(double) * P), *Os = (double *)malloc(sizeof(double) * P);
+    for (size_t q = 0; q < P; ++q) grad[q] = 0.0;
+    double mean = 0.0, var = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        unpack1(spins + i * W, N, x);
+        E[i] = orc_rates(d, x, rates + i * M, NULL);
+        mean += E[i];
+    }
+    mean /= (double)n;
+    for (int64_t i = 0; i < n; ++i) var += (E[i] - mean) * (E[i] - mean);
+    var /= (double)n;
+    double lim = clip > 0.0 ? clip * sqrt(var) : INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        double dev = E[i] - mean;
+        if (dev > lim) dev = lim;
+        if (dev < -lim) dev = -lim;
+        double g = 2.0 * dev / (double)n;
+        unpack1(spins + i * W, N, x);
+        mlp_O(d, x, O);
+        for (int s = 0; s < n_off; ++s) {
+            double r = rates[i * M + 1 + s];
+            if (r == 0.0) continue;
+            memcpy(xs, x, (size_t)N);
+            if (s < d->n_hx) {
+                xs[s] = (int8_t)-xs[s];
+            } else {
+                int e = s - d->n_hx;
+                xs[d->edge_i[e]] = (int8_t)-xs[d->edge_i[e]];
+                xs[d->edge_j[e]] = (int8_t)-xs[d->edge_j[e]];
+            }
+            mlp_O(d, xs, Os);
+            for (size_t q = 0; q < P; ++q) grad[q] += g * r * (Os[q] - O[q]);
+        }
+    }
+    out2[0] = mean;
+    out2[1] = var;
+    free(rates);
+    free(E);
+    free(x);
+    free(xs);
+    free(O);
+    free(Os);
 }

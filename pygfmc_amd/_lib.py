@@ -154,6 +154,10 @@ SIGNATURES = [
     ("gfmc_adam_step", ctypes.c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int64,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, _vp]),
+    ("gfmc_mlp_n_params", ctypes.c_int64, [_vp]),
+    ("gfmc_mlp_set_params", ctypes.c_int, [_vp, _vp, _vp]),
+    ("gfmc_mlp_vmin_grad", ctypes.c_int, [_vp, ctypes.c_int64, _vp, ctypes.c_float, _vp, _vp,
+                                          _vp]),
     ("gfmc_diag_create", ctypes.c_int, [ctypes.POINTER(DiagDesc), ctypes.POINTER(_vp)]),
     ("gfmc_diag_destroy", ctypes.c_int, [_vp]),
     ("gfmc_diag_ring_bytes", ctypes.c_int64, [_vp, ctypes.c_int64]),

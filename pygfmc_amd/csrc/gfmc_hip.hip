@@ -110,17 +110,20 @@ struct DevProblem {
     const double *mlp_b1;     // [h1] BF16-grid values
     const uint16_t *mlp_W2;   // [h2][h1] BF16 bits
     const float *mlp_b2, *mlp_w3;
-    float mlp_b3;
+    const float *mlp_b3p;     // [1]
     const float2 *mlp_tab;    // [2048] {tanh(j / 256), tanh((j + 1) / 256) - tanh(j / 256)}
 };
 
 struct MlpWork;
+struct MlpTrainWork;
 static void free_mlp_work(MlpWork *m);
+static void free_mlp_train(MlpTrainWork *t);
 static MlpWork *new_mlp_work(int blocks_per_walker);
 
 struct gfmc_problem {
     DevProblem dp;
     MlpWork *mlp = nullptr;   // the MLP guide's propagation workspace (grown on demand)
+    mutable MlpTrainWork *mlpt = nullptr;   // its training workspace (grown on demand)
     Tables<double> t64;
     Tables<float> t32;
     int precision;
@@ -1949,6 +1952,7 @@ static int diag_record_impl(const gfmc_diag *d, int64_t nw, const uint32_t *spin
 
 static void free_problem(gfmc_problem *p) {
     free_mlp_work(p->mlp);
+    free_mlp_train(p->mlpt);
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;
 }
@@ -1982,10 +1986,10 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         if (precision != GFMC_FP32)
             return fail(GFMC_EINVAL, "gfmc_problem_create: the MLP guide runs in FP32 "
                                      "(BF16 layers); pass GFMC_FP32");
-        if (d->mlp_h1 <= 0 || d->mlp_h1 % 32 != 0 || d->mlp_h1 > 4096 ||
+        if (d->mlp_h1 <= 0 || d->mlp_h1 % 64 != 0 || d->mlp_h1 > 4096 ||
             !(d->mlp_h2 == 32 || d->mlp_h2 == 64 || d->mlp_h2 == 128 || d->mlp_h2 == 256))
             return fail(GFMC_EINVAL, "gfmc_problem_create: MLP hidden sizes must be a multiple "
-                                     "of 32 (layer 1, at most 4096) and 32, 64, 128 or 256 "
+                                     "of 64 (layer 1, at most 4096) and 32, 64, 128 or 256 "
                                      "(layer 2)");
         if (!d->mlp_W1 || !d->mlp_b1 || !d->mlp_W2 || !d->mlp_b2 || !d->mlp_w3 || !d->mlp_b3)
             return fail(GFMC_EINVAL, "gfmc_problem_create: incomplete MLP parameters");
@@ -2203,14 +2207,15 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
             tab[j] = make_float2(t0, t1 - t0);
         }
         UP(w1t.data(), w1t.size(), &P.mlp_W1T);
+
         UP(w2.data(), w2.size(), &P.mlp_W2);
         UP(b1.data(), b1.size(), &P.mlp_b1);
         UP(d->mlp_b2, (size_t)h2, &P.mlp_b2);
         UP(d->mlp_w3, (size_t)h2, &P.mlp_w3);
         UP(tab.data(), tab.size(), &P.mlp_tab);
-        P.mlp_b3 = d->mlp_b3[0];
+        UP(d->mlp_b3, (size_t)1, &P.mlp_b3p);
         P.rbm_wave_bytes = round_up(P.K * WAVE * 4 + P.n_pad + P.e_pad, 16);
-        p->mlp = new_mlp_work((n_off + 1 + 31) / 32);
+        p->mlp = new_mlp_work((n_off + 1 + 15) / 16);
     }
 #undef UP
 #undef UP32
@@ -2585,3 +2590,4 @@ int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32
 #include "gfmc_estim.inl"
 #include "gfmc_diag.inl"
 #include "gfmc_mlp.inl"
+#include "gfmc_mlp_train.inl"

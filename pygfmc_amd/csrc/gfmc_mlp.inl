@@ -8,14 +8,14 @@
 // hop re-evaluates the network on every candidate move.  The propagation
 // runs in bulk-synchronous iterations, one hop of every active walker each:
 //   k_mlp_scan*   rows of each active walker (its configuration, then one row
-//                 per slot with a non-zero bare rate), padded to 32-row
+//                 per slot with a non-zero bare rate), padded to 16-row
 //                 blocks; an exclusive scan gives each walker its blocks;
-//   k_mlp_gemm    one wavefront per 32-row block (one walker): the layer-1
+//   k_mlp_gemm    one wavefront per 16-row block (one walker): the layer-1
 //                 activations of its rows are built in registers straight in
 //                 the MFMA A layout from z (FP64, exact: W1 and b1 lie on the
 //                 BF16 grid) and the two W1 columns of the row's move, W2 is
-//                 staged in LDS in 32-wide K chunks, v_mfma_f32_32x32x16_bf16
-//                 accumulates the 32 x h2 block, the epilogue applies s, w3
+//                 staged in LDS in 32-wide K chunks, v_mfma_f32_16x16x32_bf16
+//                 accumulates the 16 x h2 block, the epilogue applies s, w3
 //                 and a fixed-order row reduction;
 //   k_mlp_select  one wavefront per walker: rates from the row values,
 //                 e_L, the hold and the move exactly as the other
@@ -30,13 +30,14 @@
 
 // MLP_NT (2048): linear pieces of the tabulated tanh on [0, 8)
 #define MLP_TSCALE 256.0f   // pieces per unit of |z|
-#define MLP_GW 4            // wavefronts (32-row blocks) per GEMM workgroup
+#define MLP_GW 8            // wavefronts (32-row blocks) per GEMM workgroup
 #define MLP_KC 32           // K chunk of W2 staged in LDS
+#define MLP_RB 16           // rows per block (one MFMA row tile)
 #define MLP_BSTR (MLP_KC + 8)
 // MLP_SW (4): walker wavefronts per start / select workgroup
 
 typedef __attribute__((ext_vector_type(8))) __bf16 mlp_bf16x8;
-typedef __attribute__((ext_vector_type(16))) float mlp_f32x16;
+typedef __attribute__((ext_vector_type(4))) float mlp_f32x4;
 
 struct MlpWs {
     double *z1;          // [cap][h1]   layer-1 pre-activations of the current config
@@ -45,11 +46,24 @@ struct MlpWs {
     int32_t *active;     // [cap]
     double *t, *lw, *eL; // [cap]
     uint32_t *it;        // [cap]
-    int32_t *blk_off;    // [cap]       first 32-row block of the walker
+    int32_t *blk_off;    // [cap]       first 16-row block of the walker
     int32_t *blk_walker; // [cap * blocks per walker]
     int32_t *part;       // [cap / 1024 + 1] scan partials
     int32_t *total;      // [1]         row blocks of the iteration
-    float *logpsi;       // [cap * blocks per walker * 32]
+    float *logpsi;       // [cap * blocks per walker * 16]
+};
+
+// rows of one training evaluation (gfmc_mlp_vmin_grad): the upstream
+// coefficient of every row, its BF16 activations, its table indices (bit 15:
+// saturated), its row record, the layer-2 upstream gradients, and per-wave
+// partial sums of the w3 / b3 gradients
+struct MlpTws {
+    const float *coef;   // [rows]
+    uint16_t *Hb;        // [rows][h1]
+    uint16_t *Jx;        // [rows][h1]
+    int2 *rinfo;         // [rows] (walker, si | sj << 16), 0xffff = none
+    float *G2;           // [rows][h2]
+    float *wpart;        // [waves][h2 + 1]
 };
 
 struct MlpWork {
@@ -60,14 +74,14 @@ struct MlpWork {
     int32_t *host_total = nullptr;   // pinned
 };
 
+// s(z) = sign(z) (T_j + f D_j): t = min(256 |z|, 2048 - 2^-13), j = floor(t),
+// f = t - j (saturating at T_2047 + 0.99988 D_2047 ~ tanh(8))
+#define MLP_TMAX 2047.9998779296875f
+
 __device__ __forceinline__ float mlp_tanh(const float2 *tab, float z) {
-    const float a = fabsf(z);
-    const float t = fminf(a * MLP_TSCALE, (float)MLP_NT);
-    int j = (int)t;
-    if (j > MLP_NT - 1) j = MLP_NT - 1;
-    const float f = t - (float)j;
-    const float2 e = tab[j];
-    return copysignf(__builtin_fmaf(f, e.y, e.x), z);
+    const float t = fminf(fabsf(z) * MLP_TSCALE, MLP_TMAX);
+    const float2 e = tab[(int)t];
+    return copysignf(__builtin_fmaf(__builtin_amdgcn_fractf(t), e.y, e.x), z);
 }
 
 __device__ __forceinline__ float bf16_val(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
@@ -177,7 +191,7 @@ __device__ __forceinline__ int mlp_block_excl_scan(int v, int *sh, int *total) {
 __global__ __launch_bounds__(1024) void k_mlp_scan1(int64_t nw, MlpWs W) {
     __shared__ int sh[WAVE + 1];
     const int64_t w = (int64_t)blockIdx.x * 1024 + threadIdx.x;
-    const int nb = (w < nw && W.active[w]) ? (W.cnt[w] + 31) / 32 : 0;
+    const int nb = (w < nw && W.active[w]) ? (W.cnt[w] + MLP_RB - 1) / MLP_RB : 0;
     int tot;
     const int ex = mlp_block_excl_scan(nb, sh, &tot);
     if (w < nw) W.blk_off[w] = ex;
@@ -205,23 +219,108 @@ __global__ __launch_bounds__(1024) void k_mlp_scan3(int64_t nw, MlpWs W) {
     if (w >= nw || !W.active[w]) return;
     const int off = W.blk_off[w] + W.part[blockIdx.x];
     W.blk_off[w] = off;
-    const int nb = (W.cnt[w] + 31) / 32;
+    const int nb = (W.cnt[w] + MLP_RB - 1) / MLP_RB;
     for (int q = 0; q < nb; ++q) W.blk_walker[off + q] = (int32_t)w;
 }
 
-// the network on every row block; NB = h2 / 32 column blocks per wavefront
-template <int NB>
-__global__ __launch_bounds__(MLP_GW * WAVE) void k_mlp_gemm(DevProblem P, const uint32_t *spins,
-                                                            MlpWs W) {
+// eight layer-1 activations of one row at hidden units k0 .. k0 + 7 from the
+// prefetched z (FP64) and W1 columns (BF16) of the row's move (ci, cj: -2 x
+// of the flipped sites, 0 when absent)
+struct MlpAIn {
+    double z[8];
+    uint4 wi, wj;
+};
+
+__device__ __forceinline__ void load8d(double *o, const double *p) {
+    const double4 a = ((const double4 *)p)[0], b = ((const double4 *)p)[1];
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+}
+
+// rows past the walker's count load zeros: z = 0 gives the activation 0
+__device__ __forceinline__ void mlp_a_load(MlpAIn &in, const double *z1, const uint16_t *w1i,
+                                           const uint16_t *w1j, int k0, bool rvalid) {
+    if (rvalid) {
+        load8d(in.z, z1 + k0);
+        in.wi = *(const uint4 *)(w1i + k0);
+        in.wj = *(const uint4 *)(w1j + k0);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) in.z[q] = 0.0;
+        in.wi = make_uint4(0, 0, 0, 0);
+        in.wj = make_uint4(0, 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ double bf16_pair(const uint4 &v, int q) {
+    const uint32_t w = q < 2 ? v.x : q < 4 ? v.y : q < 6 ? v.z : v.w;
+    return (double)__uint_as_float((q & 1) ? (w & 0xffff0000u) : (w << 16));
+}
+
+// the eight activations; the layer-1 sums stay exact (FP64 fma of exact
+// terms), the eight table reads are issued before any is used
+__device__ __forceinline__ mlp_bf16x8 mlp_a_build(const MlpAIn &in, double ci, double cj,
+                                                  const float2 *tab, uint4 *jout = nullptr) {
+    float t[8], zf[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double z = __builtin_fma(cj, bf16_pair(in.wj, q),
+                                       __builtin_fma(ci, bf16_pair(in.wi, q), in.z[q]));
+        zf[q] = (float)z;
+        t[q] = fminf(fabsf(zf[q]) * MLP_TSCALE, MLP_TMAX);
+    }
+    float2 e[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) e[q] = tab[(int)t[q]];
+    mlp_bf16x8 a;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        a[q] = (__bf16)copysignf(__builtin_fmaf(__builtin_amdgcn_fractf(t[q]), e[q].y, e[q].x),
+                                 zf[q]);
+    if (jout) {
+        uint32_t jj[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) jj[q] = (uint32_t)(int)t[q] | (t[q] == MLP_TMAX ? 0x8000u : 0u);
+        *jout = make_uint4(jj[0] | (jj[1] << 16), jj[2] | (jj[3] << 16), jj[4] | (jj[5] << 16),
+                           jj[6] | (jj[7] << 16));
+    }
+    return a;
+}
+
+// the slope of the tabulated tanh at a table index (bit 15: saturated, 0)
+__device__ __forceinline__ float mlp_slope(const float2 *tab, uint32_t j) {
+    return (j & 0x8000u) ? 0.0f : MLP_TSCALE * tab[j & 0x7ff].y;
+}
+
+// the network on every row block; NC = h2 / 16 column blocks per wavefront
+// (v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][8 (l / 16) + q], B[8 (l /
+// 16) + q][l % 16], D register r = row 4 (l / 16) + r, column l % 16).
+// MLP_GW wavefronts per workgroup share a double-buffered W2 chunk: chunk
+// c + 1 is fetched into registers while chunk c feeds the MFMAs, then
+// stored to the other buffer (one barrier per chunk); the A operand of K
+// step s + 1 is loaded while step s is multiplied.
+#ifndef MLP_WPE
+#define MLP_WPE 2
+#endif
+template <int NC, bool TRAIN>
+__global__ __launch_bounds__(MLP_GW * WAVE) __attribute__((amdgpu_waves_per_eu(MLP_WPE)))
+void k_mlp_gemm(DevProblem P, const uint32_t *spins, MlpWs W, MlpTws T) {
     extern __shared__ __align__(16) unsigned char lds[];
     float2 *tab = (float2 *)lds;
-    uint16_t *Bs = (uint16_t *)(lds + MLP_NT * sizeof(float2));
+    constexpr int H2 = NC * 16;
+    constexpr int CH = H2 * (MLP_KC / 8);                 // uint4 per chunk
+    constexpr int PER = (CH + MLP_GW * WAVE - 1) / (MLP_GW * WAVE);
+    uint16_t *Bs0 = (uint16_t *)(lds + MLP_NT * sizeof(float2));
+    uint16_t *Bs1 = Bs0 + H2 * MLP_BSTR;
     for (int i = threadIdx.x; i < MLP_NT; i += blockDim.x) tab[i] = P.mlp_tab[i];
     const int total = W.total[0];
     const int tiles = (total + MLP_GW - 1) / MLP_GW;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int H1 = P.mlp_h1, H2 = NB * 32;
-    const int half = lane >> 5, col = lane & 31;
+    const int H1 = P.mlp_h1, nchunks = H1 / MLP_KC;
+    const int quad = lane >> 4, col = lane & 15;
+    float acc3[NC], accb = 0.0f;   // TRAIN: this wave's w3 / b3 gradient partials
+#pragma unroll
+    for (int t = 0; t < NC; ++t) acc3[t] = 0.0f;
     for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const int b = tile * MLP_GW + wave;
         const bool bvalid = b < total;
@@ -229,7 +328,7 @@ __global__ __launch_bounds__(MLP_GW * WAVE) void k_mlp_gemm(DevProblem P, const 
         int r0 = 0, cnt = 0;
         if (bvalid) {
             w = W.blk_walker[b];
-            r0 = 32 * (b - W.blk_off[w]);
+            r0 = MLP_RB * (b - W.blk_off[w]);
             cnt = W.cnt[w];
         }
         const int row = r0 + col;
@@ -245,70 +344,144 @@ __global__ __launch_bounds__(MLP_GW * WAVE) void k_mlp_gemm(DevProblem P, const 
         const double *z1 = W.z1 + (size_t)w * H1;
         const uint16_t *w1i = P.mlp_W1T + (size_t)(si < 0 ? 0 : si) * H1;
         const uint16_t *w1j = P.mlp_W1T + (size_t)(sj < 0 ? 0 : sj) * H1;
-        mlp_f32x16 acc[NB];
+        const size_t grow = (size_t)b * MLP_RB + col;   // this lane's row (TRAIN stores)
+        if (TRAIN && bvalid && quad == 0)
+            T.rinfo[grow] = make_int2((int)w, (si & 0xffff) | ((sj & 0xffff) << 16));
+        mlp_f32x4 acc[NC];
 #pragma unroll
-        for (int t = 0; t < NB; ++t)
+        for (int t = 0; t < NC; ++t)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
-        for (int kc = 0; kc < H1; kc += MLP_KC) {
-            __syncthreads();
-            for (int idx = threadIdx.x; idx < H2 * (MLP_KC / 8); idx += blockDim.x) {
+            for (int r = 0; r < 4; ++r) acc[t][r] = 0.0f;
+        // chunk 0 of W2 into buffer 0
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int idx = threadIdx.x + u * MLP_GW * WAVE;
+            if (idx < CH) {
                 const int j = idx / (MLP_KC / 8), q = idx % (MLP_KC / 8);
-                *(uint4 *)(Bs + j * MLP_BSTR + q * 8) =
-                    *(const uint4 *)(P.mlp_W2 + (size_t)j * H1 + kc + q * 8);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int ks = 0; ks < MLP_KC; ks += 16) {
-                const int k0 = kc + ks + 8 * half;
-                mlp_bf16x8 a;
-                if (rvalid) {
-                    const uint4 vi = si >= 0 ? *(const uint4 *)(w1i + k0) : make_uint4(0, 0, 0, 0);
-                    const uint4 vj = sj >= 0 ? *(const uint4 *)(w1j + k0) : make_uint4(0, 0, 0, 0);
-                    const uint32_t wi[4] = {vi.x, vi.y, vi.z, vi.w};
-                    const uint32_t wj[4] = {vj.x, vj.y, vj.z, vj.w};
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        double z = z1[k0 + q];
-                        z += ci * (double)bf16_val((uint16_t)(wi[q >> 1] >> ((q & 1) * 16)));
-                        z += cj * (double)bf16_val((uint16_t)(wj[q >> 1] >> ((q & 1) * 16)));
-                        a[q] = (__bf16)mlp_tanh(tab, (float)z);
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) a[q] = (__bf16)0.0f;
-                }
-#pragma unroll
-                for (int t = 0; t < NB; ++t) {
-                    const mlp_bf16x8 bv =
-                        *(const mlp_bf16x8 *)(Bs + (t * 32 + col) * MLP_BSTR + ks + 8 * half);
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[t], 0, 0, 0);
-                }
+                *(uint4 *)(Bs0 + j * MLP_BSTR + q * 8) =
+                    *(const uint4 *)(P.mlp_W2 + (size_t)j * H1 + q * 8);
             }
         }
-        // epilogue: row i of register r is 8 (r / 4) + 4 half + r % 4, column
-        // t * 32 + col; per lane a sum over t, then a butterfly over col
-        float part[16];
+        MlpAIn ain;
+        mlp_a_load(ain, z1, w1i, w1j, 8 * quad, rvalid);
+        __syncthreads();
+        static_assert(PER <= 4, "W2 chunk prefetch holds four uint4 per thread");
+        // the thread's chunk elements idx_u = threadIdx.x + u * blockDim.x
+        const int TPB = MLP_GW * WAVE;
+        const int jq = threadIdx.x / (MLP_KC / 8), qq = threadIdx.x % (MLP_KC / 8);
+        const int jstep = TPB / (MLP_KC / 8);
+        uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
+        for (int c = 0; c < nchunks; ++c) {
+            const uint16_t *Bc = (c & 1) ? Bs1 : Bs0;
+            uint16_t *Bn = (c & 1) ? Bs0 : Bs1;
+            const bool more = c + 1 < nchunks;
+            if (more) {
+                const uint16_t *src = P.mlp_W2 + (size_t)jq * H1 + (c + 1) * MLP_KC + qq * 8;
+                if ((int)threadIdx.x < CH) pre0 = *(const uint4 *)src;
+                if (PER > 1 && (int)threadIdx.x + TPB < CH)
+                    pre1 = *(const uint4 *)(src + (size_t)jstep * H1);
+                if (PER > 2 && (int)threadIdx.x + 2 * TPB < CH)
+                    pre2 = *(const uint4 *)(src + (size_t)2 * jstep * H1);
+                if (PER > 3 && (int)threadIdx.x + 3 * TPB < CH)
+                    pre3 = *(const uint4 *)(src + (size_t)3 * jstep * H1);
+            }
 #pragma unroll
-        for (int r = 0; r < 16; ++r) part[r] = 0.0f;
+            for (int ks = 0; ks < MLP_KC; ks += 32) {
+                // build this K step's operand, then start the next step's loads
+                uint4 jv;
+                const mlp_bf16x8 a = mlp_a_build(ain, ci, cj, tab, TRAIN ? &jv : nullptr);
+                if (TRAIN && bvalid) {
+                    const size_t off = grow * H1 + c * MLP_KC + ks + 8 * quad;
+                    *(mlp_bf16x8 *)(T.Hb + off) = a;
+                    *(uint4 *)(T.Jx + off) = jv;
+                }
+                const int knext = c * MLP_KC + ks + 32 + 8 * quad;
+                if (knext < H1) mlp_a_load(ain, z1, w1i, w1j, knext, rvalid);
 #pragma unroll
-        for (int t = 0; t < NB; ++t) {
-            const int j = t * 32 + col;
+                for (int t = 0; t < NC; ++t) {
+                    const mlp_bf16x8 bv =
+                        *(const mlp_bf16x8 *)(Bc + (t * 16 + col) * MLP_BSTR + ks + 8 * quad);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
+                }
+            }
+            if (more) {
+                uint16_t *dst = Bn + jq * MLP_BSTR + qq * 8;
+                if ((int)threadIdx.x < CH) *(uint4 *)dst = pre0;
+                if (PER > 1 && (int)threadIdx.x + TPB < CH) *(uint4 *)(dst + jstep * MLP_BSTR) = pre1;
+                if (PER > 2 && (int)threadIdx.x + 2 * TPB < CH)
+                    *(uint4 *)(dst + 2 * jstep * MLP_BSTR) = pre2;
+                if (PER > 3 && (int)threadIdx.x + 3 * TPB < CH)
+                    *(uint4 *)(dst + 3 * jstep * MLP_BSTR) = pre3;
+            }
+            __syncthreads();
+        }
+        if constexpr (TRAIN) {
+            // G2 = coef w3 s'(u) per (row, column); the w3 and b3 gradient
+            // partials sum coef s(u) and coef over this wave's rows
+            float cf[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                cf[r] = bvalid ? T.coef[(size_t)b * MLP_RB + 4 * quad + r] : 0.0f;
+#pragma unroll
+            for (int t = 0; t < NC; ++t) {
+                const int j = t * 16 + col;
+                const float bb = P.mlp_b2[j], ww = P.mlp_w3[j];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float u = acc[t][r] + bb;
+                    const float tt = fminf(fabsf(u) * MLP_TSCALE, MLP_TMAX);
+                    const uint32_t ji = (uint32_t)(int)tt | (tt == MLP_TMAX ? 0x8000u : 0u);
+                    const float su = mlp_tanh(tab, u);
+                    acc3[t] = acc3[t] + cf[r] * su;
+                    if (bvalid)
+                        T.G2[((size_t)b * MLP_RB + 4 * quad + r) * H2 + j] =
+                            cf[r] * ww * mlp_slope(tab, ji);
+                }
+            }
+            if (col == 0)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) accb = accb + cf[r];
+            continue;
+        }
+        // epilogue: per lane a sum over the column blocks t (column t * 16 +
+        // col), then a butterfly over col
+        float part[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[r] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < NC; ++t) {
+            const int j = t * 16 + col;
             const float bb = P.mlp_b2[j], ww = P.mlp_w3[j];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) part[r] = part[r] + ww * mlp_tanh(tab, acc[t][r] + bb);
+            for (int r = 0; r < 4; ++r) part[r] = part[r] + ww * mlp_tanh(tab, acc[t][r] + bb);
         }
 #pragma unroll
-        for (int d = 16; d >= 1; d >>= 1)
+        for (int d = 8; d >= 1; d >>= 1)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) part[r] = part[r] + __shfl_xor(part[r], d, WAVE);
+            for (int r = 0; r < 4; ++r) part[r] = part[r] + __shfl_xor(part[r], d, WAVE);
         if (bvalid && col == 0) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = 8 * (r / 4) + 4 * half + r % 4;
-                if (r0 + i < cnt) W.logpsi[(size_t)b * 32 + i] = part[r] + P.mlp_b3;
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * quad + r;
+                if (r0 + i < cnt) W.logpsi[(size_t)b * MLP_RB + i] = part[r] + P.mlp_b3p[0];
             }
         }
+    }
+    if constexpr (TRAIN) {
+        // quads hold the same columns for different rows: combine in order
+#pragma unroll
+        for (int t = 0; t < NC; ++t) {
+            acc3[t] = acc3[t] + __shfl_xor(acc3[t], 16, WAVE);
+            acc3[t] = acc3[t] + __shfl_xor(acc3[t], 32, WAVE);
+        }
+        accb = accb + __shfl_xor(accb, 16, WAVE);
+        accb = accb + __shfl_xor(accb, 32, WAVE);
+        float *wp = T.wpart + ((size_t)blockIdx.x * MLP_GW + wave) * (H2 + 1);
+        if (quad == 0) {
+#pragma unroll
+            for (int t = 0; t < NC; ++t) wp[t * 16 + col] = acc3[t];
+        }
+        if (lane == 0) wp[H2] = accb;
     }
 }
 
@@ -328,7 +501,7 @@ __global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_select(
         load_spins(P, sp, L.x, lane);
         for (int s = lane; s < P.K * WAVE; s += WAVE) L.w[s] = 0.0f;
         wave_sync();
-        const size_t base = (size_t)W.blk_off[w] * 32;
+        const size_t base = (size_t)W.blk_off[w] * MLP_RB;
         const int cnt = W.cnt[w];
         const float lp0 = W.logpsi[base];
         int32_t *cand = W.cand + (size_t)w * P.n_off;
@@ -418,7 +591,7 @@ __global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_rates(DevProblem P, Table
         load_spins(P, spins + w * P.words, L.x, lane);
         for (int s = lane; s < P.K * WAVE; s += WAVE) L.w[s] = 0.0f;
         wave_sync();
-        const size_t base = (size_t)W.blk_off[w] * 32;
+        const size_t base = (size_t)W.blk_off[w] * MLP_RB;
         const int cnt = W.cnt[w];
         const float lp0 = W.logpsi[base];
         if (log_psi && lane == 0) log_psi[w] = (double)lp0;
@@ -478,7 +651,7 @@ static void free_mlp_work(MlpWork *m) {
 static int mlp_reserve(const gfmc_problem *p, int64_t nw) {
     MlpWork *m = p->mlp;
     if (m->cap >= nw) return 0;
-    if ((int64_t)nw * m->bpw > 0x7fffffffLL / 32)
+    if ((int64_t)nw * m->bpw > 0x7fffffffLL / MLP_RB)
         return fail(GFMC_EUNSUP, "MLP guide: too many walkers x candidate rows for one launch");
     for (void *a : m->allocs) (void)hipFree(a);
     m->allocs.clear();
@@ -507,7 +680,7 @@ static int mlp_reserve(const gfmc_problem *p, int64_t nw) {
     if (!rc) rc = get(nb * 4, (void **)&W.blk_walker);
     if (!rc) rc = get((n / 1024 + 2) * 4, (void **)&W.part);
     if (!rc) rc = get(16, (void **)&W.total);
-    if (!rc) rc = get(nb * 32 * 4, (void **)&W.logpsi);
+    if (!rc) rc = get(nb * MLP_RB * 4, (void **)&W.logpsi);
     if (rc) {
         for (void *a : m->allocs) (void)hipFree(a);
         m->allocs.clear();
@@ -521,7 +694,7 @@ static int mlp_reserve(const gfmc_problem *p, int64_t nw) {
 static size_t mlp_walker_lds(const gfmc_problem *p) { return (size_t)MLP_SW * p->dp.rbm_wave_bytes; }
 
 static size_t mlp_gemm_lds(const gfmc_problem *p) {
-    return (size_t)MLP_NT * sizeof(float2) + (size_t)p->dp.mlp_h2 * MLP_BSTR * 2;
+    return (size_t)MLP_NT * sizeof(float2) + 2 * (size_t)p->dp.mlp_h2 * MLP_BSTR * 2;
 }
 
 static unsigned mlp_walker_grid(const gfmc_problem *p, int64_t nw) {
@@ -530,19 +703,22 @@ static unsigned mlp_walker_grid(const gfmc_problem *p, int64_t nw) {
     return (unsigned)(need < cap ? need : cap);
 }
 
-static void launch_mlp_gemm(const gfmc_problem *p, const uint32_t *spins, int total,
-                            hipStream_t st) {
+template <bool TRAIN>
+static unsigned launch_mlp_gemm(const gfmc_problem *p, const uint32_t *spins, int total,
+                                hipStream_t st, const MlpTws &T = MlpTws{}) {
     const int tiles = (total + MLP_GW - 1) / MLP_GW;
-    const int cap = p->n_cu * 2;
-    const unsigned grid = (unsigned)(tiles < cap ? tiles : cap);
+    const int cap = p->n_cu * 4;
+    const unsigned grid = (unsigned)(tiles < cap ? (tiles > 0 ? tiles : 1) : cap);
     const size_t lds = mlp_gemm_lds(p);
     const MlpWs &W = p->mlp->ws;
+    const dim3 g(grid), b(MLP_GW * WAVE);
     switch (p->dp.mlp_h2) {
-    case 32: hipLaunchKernelGGL(k_mlp_gemm<1>, dim3(grid), dim3(MLP_GW * WAVE), lds, st, p->dp, spins, W); break;
-    case 64: hipLaunchKernelGGL(k_mlp_gemm<2>, dim3(grid), dim3(MLP_GW * WAVE), lds, st, p->dp, spins, W); break;
-    case 128: hipLaunchKernelGGL(k_mlp_gemm<4>, dim3(grid), dim3(MLP_GW * WAVE), lds, st, p->dp, spins, W); break;
-    default: hipLaunchKernelGGL(k_mlp_gemm<8>, dim3(grid), dim3(MLP_GW * WAVE), lds, st, p->dp, spins, W); break;
+    case 32: hipLaunchKernelGGL((k_mlp_gemm<2, TRAIN>), g, b, lds, st, p->dp, spins, W, T); break;
+    case 64: hipLaunchKernelGGL((k_mlp_gemm<4, TRAIN>), g, b, lds, st, p->dp, spins, W, T); break;
+    case 128: hipLaunchKernelGGL((k_mlp_gemm<8, TRAIN>), g, b, lds, st, p->dp, spins, W, T); break;
+    default: hipLaunchKernelGGL((k_mlp_gemm<16, TRAIN>), g, b, lds, st, p->dp, spins, W, T); break;
     }
+    return grid;
 }
 
 // the row blocks of the active walkers; returns their number (host sync)
@@ -585,7 +761,7 @@ static int mlp_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, dou
         if (iter >= 10000000)
             return fail(GFMC_EINVAL, "MLP guide: a walker did not finish its branching step "
                                      "in 10^7 hops (non-finite rates?)");
-        launch_mlp_gemm(p, spins, total, st);
+        launch_mlp_gemm<false>(p, spins, total, st);
         hipLaunchKernelGGL(k_mlp_select, dim3(mlp_walker_grid(p, nw)), dim3(MLP_SW * WAVE),
                            mlp_walker_lds(p), st, p->dp, p->t32, nw, spins, log_w, e_loc, beta,
                            alw, k0, k1, step, tag, evals, p->mlp->ws);
@@ -604,7 +780,7 @@ static int mlp_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, d
     int total = 0;
     rc = mlp_scan(p, nw, st, &total);
     if (rc) return rc;
-    launch_mlp_gemm(p, spins, total, st);
+    launch_mlp_gemm<false>(p, spins, total, st);
     hipLaunchKernelGGL(k_mlp_rates, dim3(mlp_walker_grid(p, nw)), dim3(MLP_SW * WAVE),
                        mlp_walker_lds(p), st, p->dp, p->t32, nw, spins, rates, e_loc, log_diff,
                        log_psi, p->mlp->ws);

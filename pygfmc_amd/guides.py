@@ -228,7 +228,7 @@ class MLPSpinHalf:
     ``s`` the tabulated tanh (2048 linear pieces on [0, 8), odd, saturating
     at tanh(8)), ``bf16`` rounding to nearest even.  W1, b1 and W2 are held on
     the BF16 grid (magnitudes below 2^-24 flushed to 0); b2, w3 and b3 are
-    FP32.  ``hidden = (h1, h2)``: h1 a multiple of 32, h2 one of 32, 64, 128,
+    FP32.  ``hidden = (h1, h2)``: h1 a multiple of 64, h2 one of 32, 64, 128,
     256.
     """
 
@@ -236,8 +236,8 @@ class MLPSpinHalf:
                  b3=0.0, seed=0, scale=1.0, out_scale=0.1, dtype=np.float32):
         self.N = int(N)
         h1, h2 = (int(hidden[0]), int(hidden[1]))
-        if h1 <= 0 or h1 % 32 or h2 not in (32, 64, 128, 256):
-            raise ValueError(f"hidden must be (multiple of 32, one of 32/64/128/256); got "
+        if h1 <= 0 or h1 % 64 or h2 not in (32, 64, 128, 256):
+            raise ValueError(f"hidden must be (multiple of 64, one of 32/64/128/256); got "
                              f"{hidden!r}")
         self.hidden = (h1, h2)
         rng = np.random.default_rng(seed)

@@ -1,4 +1,5 @@
-"""Training of the RBM guide on configurations GFMC generates.
+"""Training of the network guides (RBM, two-layer MLP) on configurations
+GFMC generates.
 
 The loss is the variance of the local energy over the current walker
 population, L = (1/n) sum_i (E_i - mean E)^2; it vanishes for an exact
@@ -7,8 +8,10 @@ guides are trained by NetKet VMC energy minimisation (train_with_netket,
 generic_netket_training.py:331-473).  Every step runs on the GPU through
 the C-ABI: GFMC branching steps with the current guide (gfmc_run_steps), the
 gradient (gfmc_rbm_vmin_grad: rates, theta by BF16 MFMA, the per-move
-backward pass, the B^T X product by MFMA), Adam (gfmc_adam_step) and the
-re-packing of the weights into the problem (gfmc_rbm_set_params).
+backward pass, the B^T X product by MFMA; gfmc_mlp_vmin_grad: the rows of
+every candidate move through the network and back on BF16 MFMA), Adam
+(gfmc_adam_step) and the re-packing of the weights into the problem
+(gfmc_rbm_set_params / gfmc_mlp_set_params).
 """
 from __future__ import annotations
 
@@ -18,11 +21,11 @@ import numpy as np
 import torch
 
 from . import _lib
-from .guides import RBMSpinHalf
+from .guides import MLPSpinHalf, RBMSpinHalf
 from .problem import Problem
 from .walkers import as_key
 
-__all__ = ["RBMTrainer", "vmin_gradient", "train_variance"]
+__all__ = ["RBMTrainer", "MLPTrainer", "vmin_gradient", "train_variance"]
 
 
 def flat_params(rbm):
@@ -99,11 +102,69 @@ class RBMTrainer:
         return rbm_from_flat(self.rbm, self.params.cpu().numpy())
 
 
+def mlp_flat_params(g):
+    """[W1 (h1 x N), b1, W2 (h2 x h1), b2, w3, b3] as one float32 vector."""
+    return np.concatenate([g.W1.reshape(-1), g.b1, g.W2.reshape(-1), g.b2, g.w3,
+                           [g.b3]]).astype(np.float32)
+
+
+def mlp_from_flat(g, flat):
+    flat = np.asarray(flat, dtype=np.float32)
+    N, (h1, h2) = g.N, g.hidden
+    o = np.cumsum([0, h1 * N, h1, h2 * h1, h2, h2, 1])
+    return g.with_parameters(flat[o[0]:o[1]].reshape(h1, N), flat[o[1]:o[2]],
+                             flat[o[2]:o[3]].reshape(h2, h1), flat[o[3]:o[4]],
+                             flat[o[4]:o[5]], float(flat[o[5]]))
+
+
+class MLPTrainer(RBMTrainer):
+    """Device state of variance minimisation for one MLP problem (the same
+    interface as :class:`RBMTrainer`; the gradient workspace is the
+    problem's own)."""
+
+    def __init__(self, problem, mlp, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clip=0.0):
+        if not isinstance(mlp, MLPSpinHalf):
+            raise TypeError(f"MLPTrainer trains an MLPSpinHalf, got {type(mlp).__name__}")
+        if problem.info.guide != _lib.GUIDE_MLP:
+            raise ValueError("the problem's guide is not an MLP")
+        self.p = problem
+        self.rbm = mlp
+        self.n_params = int(_lib.load().gfmc_mlp_n_params(problem.handle))
+        dev = _lib.device()
+        self.params = torch.as_tensor(mlp_flat_params(mlp), device=dev)
+        if self.params.numel() != self.n_params:
+            raise ValueError("parameter count differs from the problem's")
+        self.grad = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.t = 0
+        self.lr, self.betas, self.eps, self.clip = float(lr), betas, float(eps), float(clip)
+
+    def gradient(self, spins, out2):
+        _lib.check(_lib.load().gfmc_mlp_vmin_grad(
+            self.p.handle, spins.shape[0], _lib.ptr(spins), self.clip, _lib.ptr(self.grad),
+            _lib.ptr(out2), _lib.stream()), "gfmc_mlp_vmin_grad")
+
+    def push(self):
+        _lib.check(_lib.load().gfmc_mlp_set_params(self.p.handle, _lib.ptr(self.params),
+                                                   _lib.stream()), "gfmc_mlp_set_params")
+
+    def guide(self):
+        return mlp_from_flat(self.rbm, self.params.cpu().numpy())
+
+
+def _trainer(problem, guide, **kw):
+    if isinstance(guide, MLPSpinHalf):
+        return MLPTrainer(problem, guide, **kw)
+    return RBMTrainer(problem, guide, **kw)
+
+
 def vmin_gradient(hamiltonian, rbm, spins, clip=0.0):
-    """(flat gradient, mean E, variance) of the local-energy variance of
-    ``rbm`` over the packed device configurations ``spins``."""
+    """(flat gradient, mean E, variance) of the local-energy variance of the
+    network guide ``rbm`` (RBM or MLP) over the packed device configurations
+    ``spins``."""
     p = Problem(hamiltonian, rbm, _lib.GFMC_FP32)
-    tr = RBMTrainer(p, rbm, clip=clip)
+    tr = _trainer(p, rbm, clip=clip)
     out2 = torch.empty(2, dtype=torch.float64, device=spins.device)
     tr.gradient(spins, out2)
     o = out2.cpu().numpy()
@@ -113,7 +174,8 @@ def vmin_gradient(hamiltonian, rbm, spins, clip=0.0):
 def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, step_size=0.2,
                    lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0, rng_key=0,
                    warmup_steps=40, sampler="self", verbosity=0, trainer=None):
-    """Variance minimisation of ``rbm`` on GFMC samples.
+    """Variance minimisation of the network guide ``rbm`` (RBMSpinHalf or
+    MLPSpinHalf) on GFMC samples.
 
     The walkers evolve under GFMC guided by ``sampler``: ``"self"`` (the RBM
     being trained), ``None`` (no guide) or any guide ``run`` accepts, e.g. a
@@ -123,7 +185,7 @@ def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, s
     ``warmup_steps`` branching steps, then per iteration ``steps_per_iter``
     steps followed by one gradient + Adam update on the resampled
     population.  ``trainer`` continues an earlier run (its parameters and
-    Adam moments).  Returns ``(trained RBMSpinHalf, history)``: arrays
+    Adam moments).  Returns ``(trained guide, history)``: arrays
     ``mean_e`` and ``variance`` (of the RBM's local energies, before each
     update), ``seconds`` (GPU wall time), ``walker_steps`` and ``trainer``.
     """
@@ -132,7 +194,7 @@ def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, s
     from .problem import get_problem
     if trainer is None:
         p = Problem(hamiltonian, rbm, _lib.GFMC_FP32)
-        trainer = RBMTrainer(p, rbm, lr=lr, betas=betas, eps=eps, clip=clip)
+        trainer = _trainer(p, rbm, lr=lr, betas=betas, eps=eps, clip=clip)
     else:
         p = trainer.p
         trainer.lr, trainer.betas, trainer.eps, trainer.clip = float(lr), betas, float(eps), \

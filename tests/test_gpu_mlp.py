@@ -32,8 +32,8 @@ def to_dev(spins):
 def systems():
     out = []
     L = kagome(2, 2)
-    out.append(("kagome2x2_32x32", SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32),
-                MLPSpinHalf(12, (32, 32), seed=1, scale=1.5, out_scale=1.0)))
+    out.append(("kagome2x2_64x32", SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32),
+                MLPSpinHalf(12, (64, 32), seed=1, scale=1.5, out_scale=1.0)))
     L = chain(8)
     out.append(("chain8_hx_64x64", SpinHalf.create(L.edges, -1.0, -0.4, 0.7, h_x=0.6,
                                                    dtype=np.float32),
@@ -167,4 +167,4 @@ def test_mlp_problem_rejects_what_it_does_not_run():
     with pytest.raises(_lib.GfmcError):
         set_method(p, "single_step", 5.0)
     with pytest.raises(_lib.GfmcError):
-        Problem(H, MLPSpinHalf(12, (32, 32), seed=1), _lib.GFMC_FP64)
+        Problem(H, MLPSpinHalf(12, (64, 32), seed=1), _lib.GFMC_FP64)

@@ -18,7 +18,8 @@ def test_tabulated_tanh(orc):
     T, D = orc.mlp_table()
     nodes = (np.arange(2048) / 256.0).astype(np.float32)
     np.testing.assert_array_equal(orc.mlp_tanh(nodes), T)
-    assert orc.mlp_tanh(np.float32(50.0)) == np.float32(np.tanh(8.0))
+    sat = orc.mlp_tanh(np.float32(50.0))
+    assert sat == orc.mlp_tanh(np.float32(8.5)) and abs(sat - np.tanh(8.0)) < 1e-6
     T2, D2 = mlp_tanh_table()
     np.testing.assert_array_equal(T, T2)
     np.testing.assert_array_equal(D, D2)
@@ -34,9 +35,9 @@ def _bf16(a):
 def _tanh_tab(z, T, D):
     z = np.asarray(z, dtype=np.float32)
     a = np.abs(z)
-    t = np.minimum(a * np.float32(256.0), np.float32(2048.0))
-    j = np.minimum(t.astype(np.int64), 2047)
-    f = (t - j.astype(np.float32)).astype(np.float32)
+    t = np.minimum(a * np.float32(256.0), np.float32(2047.9998779296875))
+    j = t.astype(np.int64)
+    f = (t - np.floor(t)).astype(np.float32)
     h = (f.astype(np.float64) * D[j].astype(np.float64) + T[j].astype(np.float64)).astype(
         np.float32)
     return np.copysign(h, z).astype(np.float32)
@@ -45,7 +46,7 @@ def _tanh_tab(z, T, D):
 def numpy_log_psi(g, configs):
     """The MLP guide's definition in matrix form: exact FP64 layer 1, BF16
     activations, layer 2 in FP64 rounded to FP32, the output in the order
-    (c mod 32 over c / 32, then a butterfly over the 32 columns)."""
+    (c mod 16 over c / 16, then a butterfly over the 16 columns)."""
     T, D = mlp_tanh_table()
     x = np.atleast_2d(np.asarray(configs, dtype=np.float64))
     W1 = bf16_flush(g.W1).astype(np.float64)
@@ -55,16 +56,16 @@ def numpy_log_psi(g, configs):
     h = _bf16(_tanh_tab(z1.astype(np.float32), T, D))
     z2 = (h.astype(np.float64) @ W2.T).astype(np.float32)
     v = (g.w3 * _tanh_tab(z2 + g.b2, T, D)).astype(np.float32)
-    NB = g.hidden[1] // 32
-    s = np.zeros((x.shape[0], 32), dtype=np.float32)
-    for t in range(NB):
-        s = (s + v[:, t * 32:(t + 1) * 32]).astype(np.float32)
-    for d in (16, 8, 4, 2, 1):
-        s = (s + s[:, np.arange(32) ^ d]).astype(np.float32)
+    NC = g.hidden[1] // 16
+    s = np.zeros((x.shape[0], 16), dtype=np.float32)
+    for t in range(NC):
+        s = (s + v[:, t * 16:(t + 1) * 16]).astype(np.float32)
+    for d in (8, 4, 2, 1):
+        s = (s + s[:, np.arange(16) ^ d]).astype(np.float32)
     return (s[:, 0] + np.float32(g.b3)).astype(np.float32)
 
 
-NETS = [((32, 32), 12, 0), ((64, 64), 12, 1), ((128, 256), 24, 2), ((96, 128), 8, 3)]
+NETS = [((64, 32), 12, 0), ((64, 64), 12, 1), ((128, 256), 24, 2), ((192, 128), 8, 3)]
 
 
 @pytest.mark.parametrize("hidden,N,seed", NETS, ids=[f"{h[0]}x{h[1]}" for h, _, _ in NETS])
@@ -103,3 +104,88 @@ def test_local_energies_equal_the_matrix_column(orc, name):
         lr = (lp[pos[rows]] - lp[c]).astype(np.float32).astype(np.float64)
         want = float(np.sum(vals * np.exp(lr)))
         assert abs(e_loc[c] - want) <= 1e-9 * (1 + abs(want))
+
+
+def _torch_tab(z, T, D):
+    """sign(z) (T_j + f D_j) written as sign(z) T_j + D_j (256 z - j sign(z))
+    below saturation, so that autograd gives the slope 256 D_j at z = 0 as
+    well (the derivative of |z| there would be taken as 0)."""
+    import torch
+    tmax = 2047.9998779296875
+    t = torch.clamp(torch.abs(z) * 256.0, max=tmax).detach()
+    j = torch.floor(t).long()
+    sg = torch.sign(z).detach()
+    free = (torch.abs(z) * 256.0 < tmax).detach()
+    lin = sg * T[j] + D[j] * (256.0 * z - j.double() * sg)
+    sat = sg * (T[j] + (tmax - j.double()) * D[j])
+    return torch.where(free, lin, sat)
+
+
+def _torch_bf16(v):
+    import torch
+    b = torch.as_tensor(_bf16(v.detach().numpy().astype(np.float32)), dtype=torch.float64)
+    return v + (b - v).detach()
+
+
+@pytest.mark.parametrize("name", ["kagome2x2", "chain8_hx"])
+def test_variance_gradient_equals_torch_autograd(orc, name):
+    """The oracle's gradient of the local-energy variance against FP64
+    autograd of an independent torch evaluation: straight-through BF16
+    rounding, the table interpolation differentiated piecewise."""
+    import torch
+    if name == "kagome2x2":
+        L = kagome(2, 2)
+        H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    else:
+        H = SpinHalf.create(chain(8).edges, -1.0, -0.4, 0.7, h_x=0.6, dtype=np.float32)
+    g = MLPSpinHalf(H.N, (64, 32), seed=7, scale=1.5, out_scale=1.0)
+    N, (h1, h2) = H.N, g.hidden
+    rng = np.random.default_rng(1)
+    cfg = rng.choice([-1.0, 1.0], size=(30, N))
+    P = orc.Problem(lower(H, g))
+    grad, mean, var = P.mlp_vmin_grad(orc.pack(cfg))
+    T, D = (torch.as_tensor(a, dtype=torch.float64) for a in mlp_tanh_table())
+    flat = np.concatenate([g.W1.ravel(), g.b1, g.W2.ravel(), g.b2, g.w3, [g.b3]])
+    p = torch.tensor(flat, dtype=torch.float64, requires_grad=True)
+    o = np.cumsum([0, h1 * N, h1, h2 * h1, h2, h2, 1])
+    W1, b1 = p[o[0]:o[1]].reshape(h1, N), p[o[1]:o[2]]
+    W2, b2 = p[o[2]:o[3]].reshape(h2, h1), p[o[3]:o[4]]
+    w3, b3 = p[o[4]:o[5]], p[o[5]]
+
+    def log_psi(x):
+        z1 = x @ W1.T + b1
+        h = _torch_bf16(_torch_tab(z1, T, D))
+        u = h @ W2.T + b2
+        return _torch_tab(u, T, D) @ w3 + b3
+
+    J_pm = float(np.asarray(H.J_pm).ravel()[0])
+    J_pmpm = float(np.asarray(H.J_pmpm).ravel()[0])
+    hx = float(np.asarray(H.h_x_half).ravel()[0]) if H.n_hx else 0.0
+    Es = []
+    for x in cfg:
+        xs, bare = [], []
+        for s in range(H.n_off):
+            y = x.copy()
+            if s < H.n_hx:
+                y[s] = -y[s]
+                b = hx
+            else:
+                i, j = H.edge_i[s - H.n_hx], H.edge_j[s - H.n_hx]
+                b = J_pm if x[i] != x[j] else J_pmpm
+                y[i], y[j] = -y[i], -y[j]
+            if b != 0.0:
+                xs.append(y)
+                bare.append(b)
+        jz = float(np.asarray(H.J_z).ravel()[0])
+        diag = sum(jz / 4 * (1 if x[i] == x[j] else -1) for i, j in zip(H.edge_i, H.edge_j))
+        lp0 = log_psi(torch.as_tensor(x[None], dtype=torch.float64))[0]
+        lps = log_psi(torch.as_tensor(np.array(xs), dtype=torch.float64))
+        Es.append(diag + torch.sum(torch.as_tensor(bare, dtype=torch.float64) *
+                                   torch.exp(lps - lp0)))
+    E = torch.stack(Es)
+    Lvar = torch.mean((E - E.mean()) ** 2)
+    Lvar.backward()
+    want = p.grad.numpy()
+    assert abs(var - float(Lvar.detach())) < 1e-5 * (1 + float(Lvar.detach()))
+    assert np.max(np.abs(grad - want)) < 2e-5 * np.max(np.abs(want))
+    assert np.max(np.abs(want)) > 1e-3
