@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--no-race", action="store_true",
                     help="skip the second metric (GPU-seconds to a fixed energy error bar, "
                          "learned RBM guide vs Jastrow, 10x10 Heisenberg)")
+    ap.add_argument("--no-mlp", action="store_true",
+                    help="skip the large learned-guide section (kagome 12x12, two-layer MLP, "
+                         "262 144 walkers)")
+    ap.add_argument("--mlp-walkers", type=int, default=262144)
+    ap.add_argument("--mlp-steps", type=int, default=3)
     ap.add_argument("--race-train-iters", type=int, default=600)
     ap.add_argument("--race-rbm-steps", type=int, default=300)
     ap.add_argument("--summarize-profile", default=None,
@@ -243,6 +248,8 @@ def main():
         out["heisenberg_chain100"] = chain100_section()
         out["batched_tfim_chains"] = tfim_batch_section()
         out["published_throughput_config"] = fig3_section()
+    if not args.no_mlp:
+        out["kagome12_mlp"] = kagome_mlp_section(args)
     if not args.no_race:
         out["sigma_race"] = sigma_race(args)
     print(json.dumps(out), flush=True)
@@ -339,6 +346,69 @@ def tfim_batch_section():
             "z_rms_all": float(np.sqrt(np.mean(zs ** 2))),
             "z_rms_L_le_64": float(np.sqrt(np.mean(zs[small] ** 2))),
             "frac_abs_z_lt_3": float(np.mean(np.abs(zs) < 3))}
+
+
+def kagome_mlp_section(args):
+    """Large learned guide: kagome XXZ 12 x 12 unit cells (432 sites, 864
+    edges, J_x = J_y = -1, J_z = 1, Sz = 0), two-layer MLP guide 432 -> 512 ->
+    256 -> 1 (BF16 MFMA, random initialisation: trained-guide quality does
+    not change the work per row), step 0.2, FP32 rates, FP64 weights.  One
+    untimed step, then ``--mlp-steps`` timed branching steps; the layer-2
+    MFMA work is counted from the row blocks the propagation processed.
+    Then one training gradient (variance minimisation) over 4096 of the
+    walkers."""
+    import torch
+
+    from pygfmc_amd import MLPSpinHalf, SpinHalf, kagome, prepare_walkers_sz
+    from pygfmc_amd import _lib
+    from pygfmc_amd.engine import Chain
+    from pygfmc_amd.problem import get_problem
+    from pygfmc_amd.training import vmin_gradient
+    import ctypes
+    lat = kagome(12, 12)
+    H = SpinHalf.create(lat.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    G = MLPSpinHalf(lat.N, (512, 256), seed=0, scale=1.0, out_scale=0.1)
+    p = get_problem(H, G)
+    nw, beta = args.mlp_walkers, 0.2
+    w = prepare_walkers_sz(nw, lat.N, 3)
+    chain = Chain(p, w)
+    _, e, _ = p.rates(chain.spins, want_rates=False)
+    alw = float(torch.mean(-e).item())
+    chain.steps(1, beta, alw, 11, 0)
+    cnt0 = (ctypes.c_int64 * 2)()
+    _lib.check(_lib.load().gfmc_mlp_counters(p.handle, cnt0), "gfmc_mlp_counters")
+    chain.evals.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ser = chain.steps(args.mlp_steps, beta, alw, 11, 1, evals=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    cnt1 = (ctypes.c_int64 * 2)()
+    _lib.check(_lib.load().gfmc_mlp_counters(p.handle, cnt1), "gfmc_mlp_counters")
+    blocks = cnt1[0] - cnt0[0]
+    evals = float(chain.evals.to(torch.float64).sum().item())
+    flop = blocks * 16 * 2.0 * 512 * 256
+    if not np.all(np.isfinite(ser[:, 0].cpu().numpy())):
+        raise SystemExit("population extinct in the MLP section")
+    n_s = 4096
+    sub = chain.spins[:n_s].contiguous()
+    vmin_gradient(H, G, sub)   # untimed (workspace)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    vmin_gradient(H, G, sub)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t1
+    return {"workload": f"kagome12x12_xxz_mlp512x256_bf16_w{nw}_step0.2",
+            "walker_steps_per_s": nw * args.mlp_steps / dt,
+            "s_per_step": dt / args.mlp_steps, "steps": args.mlp_steps,
+            "evals_per_walker_step": evals / (nw * args.mlp_steps),
+            "rows_per_eval": blocks * 16 / evals,
+            "hop_iterations_per_step": (cnt1[1] - cnt0[1]) / args.mlp_steps,
+            "mfma_layer2_tflops_per_s": flop / dt / 1e12,
+            "mfma_layer2_frac_of_bf16_peak": flop / dt / 2.5e15,
+            "train_gradient_seconds_4096_samples": t_grad,
+            "note": "layer-2 FLOP counted over 16-row blocks (padding included); the "
+                    "per-row layer-1 activations (512 tanh + BF16) run on VALU"}
 
 
 def timed_run(walkers, n_steps, H, G, p_array, seed, warmup):

@@ -401,6 +401,11 @@ int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *
  * call synchronises the caller's stream with the host once (it sizes the
  * candidate rows) and grows a workspace the problem owns. */
 
+/* Totals since the problem's creation of its propagation: out[0] 16-row
+ * blocks through the network (each 16 x h1 x h2 multiply-adds on MFMA),
+ * out[1] bulk-synchronous hop iterations. */
+int gfmc_mlp_counters(const gfmc_problem *p, int64_t *out);
+
 /* h1*N + h1 + h2*h1 + 2*h2 + 1 of an MLP problem, -1 otherwise. */
 int64_t gfmc_mlp_n_params(const gfmc_problem *p);
 

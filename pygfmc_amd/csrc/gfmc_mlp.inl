@@ -69,6 +69,7 @@ struct MlpTws {
 struct MlpWork {
     int64_t cap = 0;
     int bpw = 0;                 // blocks per walker at most
+    int64_t blocks_done = 0, iters_done = 0;   // row blocks / hop iterations propagated
     MlpWs ws{};
     std::vector<void *> allocs;
     int32_t *host_total = nullptr;   // pinned
@@ -758,6 +759,8 @@ static int mlp_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, dou
         rc = mlp_scan(p, nw, st, &total);
         if (rc) return rc;
         if (total == 0) break;
+        p->mlp->blocks_done += total;
+        p->mlp->iters_done += 1;
         if (iter >= 10000000)
             return fail(GFMC_EINVAL, "MLP guide: a walker did not finish its branching step "
                                      "in 10^7 hops (non-finite rates?)");
@@ -785,5 +788,14 @@ static int mlp_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, d
                        mlp_walker_lds(p), st, p->dp, p->t32, nw, spins, rates, e_loc, log_diff,
                        log_psi, p->mlp->ws);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int gfmc_mlp_counters(const gfmc_problem *p, int64_t *out) {
+    if (!p || !out) return fail(GFMC_EINVAL, "gfmc_mlp_counters: NULL argument");
+    if (p->dp.guide != GFMC_GUIDE_MLP || !p->mlp)
+        return fail(GFMC_EINVAL, "gfmc_mlp_counters: the problem's guide is not an MLP");
+    out[0] = p->mlp->blocks_done;
+    out[1] = p->mlp->iters_done;
     return 0;
 }
