@@ -59,6 +59,9 @@ extern "C" {
 #define GFMC_TAG_INIT 2u
 #define GFMC_TAG_WARMUP_PROPAGATE 3u
 #define GFMC_TAG_WARMUP_RECONFIG 4u
+/* operator application of the i-th off-diagonal measurement operator:
+ * GFMC_TAG_OPERATOR + i */
+#define GFMC_TAG_OPERATOR 16u
 
 /*
  * Problem descriptor: a SpinHalf Hamiltonian (codes/pygfmc/hamiltonian/
@@ -511,6 +514,53 @@ int gfmc_graph_capture_steps_diag(const gfmc_problem *p, int64_t n_walkers, uint
                                   int32_t *evals, const gfmc_diag *d, void *ring,
                                   double *diag_series, double *diag_work, uint32_t *counter,
                                   gfmc_graph **out);
+
+/*
+ * Off-diagonal forward walking: the pure estimator of an operator O with
+ * off-diagonal elements (forward_walk_operator, evolution.py:892-1101).  O
+ * is a SpinHalf lowered like a Hamiltonian (gfmc_problem_create) with the
+ * run's guide, so gfmc_rates gives its guide-transformed rates r_k(x) =
+ * O_{x'x} psi(x') / psi(x) (slot 0 the diagonal element), all >= 0 for a
+ * measurement operator.
+ */
+
+/* Doubles of the rates workspace of gfmc_apply_operator. */
+int64_t gfmc_apply_operator_work_doubles(const gfmc_problem *op, int64_t n_walkers);
+
+/* Apply O to every walker (apply_operator_one, evolution.py:848-889):
+ * vals[w] = sum_k r_k(x_w) and spins_out[w] = the successor of slot k drawn
+ * with probability r_k / vals[w] (blocked inverse CDF of the propagator over
+ * the 1 + n_off slots, u from Philox (0, w, step, tag)); vals <= 0 keeps the
+ * configuration with value 0; an operator without off-diagonal slots keeps
+ * it with its diagonal element.  spins_out must not alias spins. */
+int gfmc_apply_operator(const gfmc_problem *op, int64_t n_walkers, const uint32_t *spins,
+                        uint64_t seed, uint32_t step, uint32_t tag, uint32_t *spins_out,
+                        double *vals, double *rates_work, void *stream);
+
+/* Bytes of the (256-byte aligned) workspace of gfmc_forward_walk_operator
+ * for lags up to m_max (op may be NULL: a diagonal measurement). */
+int64_t gfmc_fw_operator_work_bytes(const gfmc_problem *h, const gfmc_problem *op,
+                                    int64_t n_walkers, int32_t m_max);
+
+/* One forward-walking measurement of O at main-chain position `step` on the
+ * post-resampling walkers `spins` of Hamiltonian h: apply O (tag_op), or,
+ * with op == NULL, start from `values` (a diagonal operator's per-walker
+ * values, configurations unchanged); run the shadow chain for m[n_p - 1]
+ * branching steps of h at positions step + 1, step + 2, ... with the tags
+ * tag_prop / tag_reconf (the production tags: the main chain's own future
+ * draws), the values following their walkers through each resampling.
+ * Outputs (device, [n_p] each, m a HOST array, non-decreasing, n_p <= 64):
+ *   fw[j]   = mean of the values after m[j] shadow steps,
+ *   norm[j] = sum of the first m[j] shadow steps' log G (0 at m = 0),
+ * the pair (val, log_norm) of forward_walk_operator.  Stream-ordered;
+ * deterministic (fixed-order sums).  h's method, lanes and precision apply
+ * to the shadow steps. */
+int gfmc_forward_walk_operator(const gfmc_problem *h, const gfmc_problem *op, int64_t n_walkers,
+                               const uint32_t *spins, const double *values, int32_t n_p,
+                               const int32_t *m, double beta, double alw, uint64_t seed,
+                               uint32_t step, uint32_t tag_op, uint32_t tag_prop,
+                               uint32_t tag_reconf, void *work, double *fw, double *norm,
+                               void *stream);
 
 #ifdef __cplusplus
 }

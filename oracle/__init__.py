@@ -104,6 +104,8 @@ def lib():
         L.orc_mlp_hidden.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f32p]
         L.orc_mlp_O.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f64p]
         L.orc_mlp_vmin_grad.argtypes = [D, ctypes.c_int64, _u32p, ctypes.c_double, _f64p, _f64p]
+        L.orc_apply_operator.argtypes = [D, ctypes.c_int64, _u32p, ctypes.c_uint64,
+                                         ctypes.c_uint32, ctypes.c_uint32, _u32p, _f64p]
         del vp
         _LIB = L
     return _LIB
@@ -205,6 +207,17 @@ class Problem:
         lib().orc_rbm_theta_batch(ctypes.byref(self.desc), spins.shape[0],
                                   _p(spins, ctypes.c_uint32), _p(out, ctypes.c_double))
         return out
+
+    def apply_operator(self, spins, seed, step, tag):
+        """This problem as a measurement operator (orc_apply_operator):
+        returns (successor spins, values)."""
+        spins = np.ascontiguousarray(spins, dtype=np.uint32)
+        out = np.empty_like(spins)
+        vals = np.empty(spins.shape[0])
+        lib().orc_apply_operator(ctypes.byref(self.desc), spins.shape[0],
+                                 _p(spins, ctypes.c_uint32), int(seed), int(step) & 0xFFFFFFFF,
+                                 int(tag), _p(out, ctypes.c_uint32), _p(vals, ctypes.c_double))
+        return out, vals
 
     def propagate(self, spins, log_w, e_loc, beta, alw, seed, step, tag, n_threads=0,
                   lanes=64, method=0, lam=0.0):
@@ -433,6 +446,47 @@ def diag_chain(prob, diag, m_list, spins, n_steps, beta, alw, seed, step0, lanes
     if history:
         return spins, e, series, rows, pars, vals
     return spins, e, series, rows
+
+
+def forward_walk_operator(h, op, spins, m_list, beta, alw, seed, step, tag_op, values=None,
+                          lanes=64, method=0, lam=0.0, history=False):
+    """forward_walk_operator (evolution.py:892-1101) over the oracle's
+    pieces: apply ``op`` (an oracle Problem) at position ``step`` (or start
+    from ``values`` with the configurations unchanged when ``op`` is None),
+    then max(m_list) branching steps of ``h`` at positions step + 1, ...
+    with the production tags, the values following their walkers through
+    every resampling.  Returns (fw, norm) at the lags m_list: the walker
+    mean of the values after m steps and 0.0 + log G_1 + ... + log G_m; with
+    ``history`` also the per-step log G and means."""
+    m_list = [int(m) for m in m_list]
+    if op is None:
+        s = np.array(spins, dtype=np.uint32, copy=True)
+        v = np.array(values, dtype=np.float64, copy=True)
+    else:
+        s, v = op.apply_operator(spins, seed, step, tag_op)
+    nw = s.shape[0]
+    means, logG = [float(np.mean(v))], [0.0]
+    e = np.zeros(nw)
+    for k in range(1, max(m_list) + 1):
+        pos = (int(step) + k) & 0xFFFFFFFF
+        s, lw, e, _ = h.propagate(s, np.zeros(nw), e, beta, alw, seed, pos, 0, lanes=lanes,
+                                  method=method, lam=lam)
+        par, lg, _, _ = reconfigure(lw, e, reconfig_psi(seed, pos, 1))
+        s, e = gather(h.N, s, e, par)
+        v = v[par]
+        means.append(float(np.mean(v)))
+        logG.append(float(lg))
+    fw, norm = [], []
+    acc, k = 0.0, 0
+    for m in m_list:
+        while k < m:
+            k += 1
+            acc = acc + logG[k]
+        fw.append(means[m])
+        norm.append(acc)
+    if history:
+        return np.array(fw), np.array(norm), np.array(logG), np.array(means)
+    return np.array(fw), np.array(norm)
 
 
 class lanes:

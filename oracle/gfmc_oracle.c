@@ -782,6 +782,58 @@ void orc_gather(int N, int64_t n, const uint32_t *spins_in, const double *e_in,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Operator application (apply_operator_one, evolution.py:848-889).  The    */
+/* operator's guide-transformed rates at x (slot 0 its diagonal element,    */
+/* then the off-diagonal slots, as orc_rates lays them out); v = their      */
+/* total in the blocked order over 64 lanes.  With v > 0 slot k is drawn by */
+/* the blocked inverse CDF at u = min(U v, nextafter(v, 0)), U from Philox  */
+/* (0, walker, step, tag), and the configuration moves by slot k - 1 when   */
+/* k > 0; v <= 0 keeps the configuration with value 0 (the reference's      */
+/* zero-weight guard); an operator without off-diagonal slots keeps it with */
+/* its signed diagonal element.                                             */
+/* ------------------------------------------------------------------------ */
+void orc_apply_operator(const gfmc_problem_desc *d, int64_t nw, const uint32_t *spins,
+                        uint64_t seed, uint32_t step, uint32_t tag, uint32_t *out,
+                        double *vals) {
+    int N = d->n_sites, W = n_words_of(N), M = 1 + n_off_of(d);
+    int saved = g_lanes;
+    g_lanes = WAVE;
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int8_t *x = (int8_t *)malloc((size_t)N);
+    double *rates = (double *)malloc(sizeof(double) * (size_t)M);
+    for (int64_t w = 0; w < nw; ++w) {
+        unpack1(spins + w * W, N, x);
+        orc_rates(d, x, rates, NULL);
+        double v;
+        int idx = 0;
+        if (M == 1) {
+            v = rates[0];
+        } else {
+            double incl[WAVE];
+            double R = orc_blocked_scan(rates, M, incl);
+            if (R <= 0.0) {
+                v = 0.0;
+            } else {
+                uint32_t ctr[4] = {0u, (uint32_t)w, step, tag};
+                uint32_t r[4];
+                orc_philox(ctr, key, r);
+                double u = orc_u53(r[0], r[1]) * R;
+                double cap = nextafter(R, 0.0);
+                if (u > cap) u = cap;
+                idx = orc_sample_blocked(rates, M, incl, u);
+                v = R;
+            }
+        }
+        if (idx > 0) flip_slot(d, x, idx - 1);
+        pack1(x, N, out + w * W);
+        vals[w] = v;
+    }
+    free(rates);
+    free(x);
+    g_lanes = saved;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Walker initialisers (walker.py:143-306), Philox-driven.                   */
 /* ------------------------------------------------------------------------ */
 /* Fisher-Yates from the top of an ascending base multiset (n_sites - n_up

@@ -23,6 +23,7 @@ TAG_RECONFIG = 1
 TAG_INIT = 2
 TAG_WARMUP_PROPAGATE = 3
 TAG_WARMUP_RECONFIG = 4
+TAG_OPERATOR = 16
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -177,6 +178,19 @@ SIGNATURES = [
                                                      ctypes.c_uint64, ctypes.c_uint32,
                                                      ctypes.c_uint32, _vp, _vp, _vp, _vp,
                                                      _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    ("gfmc_apply_operator_work_doubles", ctypes.c_int64, [_vp, ctypes.c_int64]),
+    ("gfmc_apply_operator", ctypes.c_int, [_vp, ctypes.c_int64, _vp, ctypes.c_uint64,
+                                           ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
+                                           _vp]),
+    ("gfmc_fw_operator_work_bytes", ctypes.c_int64, [_vp, _vp, ctypes.c_int64,
+                                                     ctypes.c_int32]),
+    ("gfmc_forward_walk_operator", ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp, _vp,
+                                                  ctypes.c_int32,
+                                                  ctypes.POINTER(ctypes.c_int32),
+                                                  ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_uint64, ctypes.c_uint32,
+                                                  ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.c_uint32, _vp, _vp, _vp, _vp]),
 ]
 
 _LIB = None

@@ -2591,3 +2591,4 @@ int gfmc_philox_host(int32_t n, const uint32_t *ctr, const uint32_t *key, uint32
 #include "gfmc_diag.inl"
 #include "gfmc_mlp.inl"
 #include "gfmc_mlp_train.inl"
+#include "gfmc_offdiag.inl"

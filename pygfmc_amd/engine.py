@@ -19,7 +19,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .estimators import (check_population, estimate_diagonal, estimate_energy, validated_int,
+from .estimators import (check_population, estimate_diagonal, estimate_energy,
+                         estimate_off_diagonal, validated_int,
                          validated_p_array)
 from .problem import get_problem
 from .walkers import PhiloxKey, Walker, as_key
@@ -357,19 +358,30 @@ def _warmup(chain, key, step_f, alw, warmup_steps, verbosity, method="cont_time"
 def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
              method="cont_time", rng_key=None, lam=None, diag_operators=None,
              off_diag_operators=None, return_raw=True, avg_log_weight=None,
-             warmup_steps=None, verbosity=7, number_of_outputs=100, precision=None,
-             use_graph=True, lanes="auto", **_unused):
-    """Single-GPU GFMC run (run_gfmc, driver.py:1148-1547), energy only.
+             warmup_steps=None, verbosity=7, number_of_outputs=100, off_diag_stride=1,
+             off_diag_batch_size=None, precision=None, use_graph=True, lanes="auto",
+             **_unused):
+    """Single-GPU GFMC run (run_gfmc, driver.py:1148-1547).
 
     Returns ``(walkers, energy, obs, stats, raw)`` (or the first four with
     ``return_raw=False``): ``energy`` {p: E_p}, ``obs`` {name: {p: value}}
-    of the diagonal operators (``diag_operators``: {name: SpinHalf or
-    SzSzStructureFactor}; a scalar per p, or one value per wavevector),
+    of the diagonal operators (``diag_operators``: {name: SpinHalf,
+    SzSzStructureFactor or Identity}; a scalar per p, or one value per
+    wavevector) and of the off-diagonal operators (``off_diag_operators``:
+    {name: SpinHalf with rates >= 0, or a diagonal SpinHalf / Identity},
+    forward walking with one shadow chain per operator at every measured
+    step: every ``off_diag_stride``-th step, with ``return_raw=False`` only
+    those every window can use),
     ``stats`` with ``avg_log_weight``, ``it_per_sec``, ``runtime``,
     ``total_wall_time``, ``number_of_devices`` and ``parameters/*``, ``raw``
     with the per-step ``log_weights`` and ``energy`` series, ``diag`` {name:
-    (n_steps, [n_vals,] n_p)} and the lead-in ``additional_diag`` (the
-    m_max steps after the run, traced with its ancestry).  ``rng_key``:
+    (n_steps, [n_vals,] n_p)}, the lead-in ``additional_diag`` (the m_max
+    steps after the run, traced with its ancestry), ``off_diag`` and
+    ``off_diag_norm`` {name: (n_meas, n_p)} (the walker mean of the operator
+    values after m shadow steps and the shadow's log weight product) and
+    ``off_diag_steps`` (n_meas,) (the 0-based step of each row).
+    ``off_diag_batch_size`` is accepted for compatibility (it must divide
+    the walker count); the GPU applies an operator to all walkers at once.  ``rng_key``:
     an int seed or (seed, position); ``stats['parameters/rng_key_next']``
     continues the chain.  ``precision``: 'fp64' / 'fp32' (default from the
     Hamiltonian's dtype).  ``use_graph``: production chunks replay one
@@ -378,10 +390,8 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     the autotuner (pygfmc_amd.autotune); pin it for bit-reproducible runs.
     """
     wall_start = time.time()
-    if off_diag_operators:
-        raise NotImplementedError("off-diagonal operators (the shadow-chain forward-walking "
-                                  "estimator) are not implemented; diagonal operators are.")
     diag_ops = dict(diag_operators) if diag_operators else {}
+    off_ops = dict(off_diag_operators) if off_diag_operators else {}
     if rng_key is None:
         import secrets
         rng_key = secrets.randbits(32)
@@ -397,10 +407,23 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         warmup_steps = DEFAULT_WARMUP_STEPS if avg_log_weight is None else 0
     warmup_steps = validated_int("warmup_steps", warmup_steps, 0)
     number_of_outputs = validated_int("number_of_outputs", number_of_outputs, 1)
+    off_diag_stride = validated_int("off_diag_stride", off_diag_stride, 1)
+    if off_diag_batch_size is not None:
+        off_diag_batch_size = validated_int("off_diag_batch_size", off_diag_batch_size, 1)
+        if walkers.n_walkers % off_diag_batch_size:
+            raise ValueError(f"off_diag_batch_size={off_diag_batch_size} must divide "
+                             f"N_walkers={walkers.n_walkers}")
     p_sorted = validated_p_array(p_array)
-    if p_sorted[-1] >= n_steps:
-        raise ValueError(f"every p value must be smaller than n_steps: got n_steps={n_steps} "
-                         f"and max(p)={p_sorted[-1]}")
+    collisions = set(diag_ops) & set(off_ops)
+    if collisions:
+        raise ValueError(f"diag_operators and off_diag_operators share name(s) "
+                         f"{sorted(collisions)}; the returned obs dict is keyed by name, so "
+                         f"colliding entries would silently overwrite each other.")
+    from .offdiagfw import check_even_p, measured_steps, measurement_plan
+    if off_ops:
+        check_even_p(off_ops, p_sorted)
+    plan = measurement_plan(n_steps, p_sorted, len(off_ops), off_diag_stride, return_raw)
+    measured = measured_steps(n_steps, plan) if off_ops else None
     if avg_log_weight is None and warmup_steps == 0:
         raise ValueError("warmup_steps=0 needs an avg_log_weight: without a warm-up nothing "
                          "measures it.")
@@ -429,6 +452,15 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         from .diagfw import DiagRecorder
         diag = DiagRecorder(diag_ops, walkers.n_sites, m_array, walkers.n_walkers, dev)
         dseries = diag.series(n_steps, dev)
+    od = od_fw = od_norm = None
+    od_row = None
+    if off_ops:
+        from .offdiagfw import OffDiagMeasurer
+        od = OffDiagMeasurer(off_ops, p, guide, m_array, walkers.n_walkers, dev)
+        od.check_positivity(chain.spins, verbose=verbosity >= 1)
+        n_meas = int(measured.sum())
+        od_fw, od_norm = od.rows(n_meas, dev)
+        od_row = np.cumsum(measured) - 1
     dchunks = []
     evals_sum, evals_n = 0.0, 0
     torch.cuda.synchronize()
@@ -444,7 +476,19 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
             n = min(print_every, n_steps - step0)
             chain.evals.zero_()
             drows = dseries[step0:step0 + n] if diag is not None else None
-            if (use_graph and graphable and n == print_every
+            if od is not None:
+                # one step at a time: a measured step launches the shadow
+                # chains right after its resampling (driver.py:290-331)
+                for k in range(n):
+                    t = step0 + k
+                    chain.steps(1, step_f, alw, key.seed, key.position + t,
+                                series=series[t:t + 1], evals=True, diag=diag,
+                                dseries=drows[k:k + 1] if diag is not None else None)
+                    if measured[t]:
+                        r = int(od_row[t])
+                        od.measure(chain.spins, step_f, alw, key.seed, key.position + t,
+                                   od_fw[r], od_norm[r])
+            elif (use_graph and graphable and n == print_every
                     and n_steps >= 2 * print_every):
                 if graph is None:
                     graph = StepGraph(chain, n, step_f, alw, key.seed, evals=True, diag=diag)
@@ -479,10 +523,21 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     energy_device = fw.result()
     out_walkers = chain.walkers()
     diag_np = additional_diag = None
+    off_np = off_norm_np = off_steps = None
     obs = {}
+    m_common = max(m_array) if (diag is not None and od is not None) else 0
     if diag is not None:
         diag_np = diag.split(np.concatenate(dchunks, axis=0))
-        obs = estimate_diagonal(p_sorted, log_weights, diag_np)
+        obs = estimate_diagonal(p_sorted, log_weights, diag_np, m_max_common=m_common)
+    if od is not None:
+        fw_h, nrm_h = od_fw.cpu().numpy(), od_norm.cpu().numpy()
+        off_np = {name: np.ascontiguousarray(fw_h[:, i]) for i, name in enumerate(od.names)}
+        off_norm_np = {name: np.ascontiguousarray(nrm_h[:, i])
+                       for i, name in enumerate(od.names)}
+        off_steps = np.flatnonzero(measured).astype(np.int64)
+        obs.update(estimate_off_diagonal(p_sorted, log_weights, off_np, off_norm_np, off_steps,
+                                         m_array, m_common, od.names))
+    if diag is not None:
         if diag.m_max > 0:
             # the lead-in a continuation substitutes for its first rows
             # (driver.py:1100-1127): m_max further steps traced with this
@@ -525,15 +580,24 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         "parameters/diag_operator_hashes": np.asarray(
             [hashlib.sha1(op.fingerprint()).hexdigest()[:32] for op in diag_ops.values()],
             dtype="S32"),
+        "parameters/off_diag_stride": int(plan.off_diag_stride),
+        "parameters/off_diag_offset": int(plan.off_diag_offset),
+        "parameters/off_diag_equal_windows": int(bool(plan.equal_windows)),
+        "parameters/n_off_diag_operators": len(off_ops),
+        "parameters/off_diag_operators": np.asarray(list(off_ops), dtype="S64"),
+        "parameters/off_diag_operator_hashes": np.asarray(
+            [hashlib.sha1(op.fingerprint()).hexdigest()[:32] for op in off_ops.values()],
+            dtype="S32"),
     }
     if verbosity >= 1:
         print(f"[gfmc] - [run_gfmc] - {n_steps} steps @ {stats['it_per_sec']:.1f} it/s; "
               f"E = {energy}", flush=True)
     if return_raw:
-        raw = dict(log_weights=log_weights, energy=E, diag=diag_np, off_diag=None,
-                   off_diag_norm=None, off_diag_steps=None, additional_diag=additional_diag,
-                   p_array=list(p_sorted), m_array=m_array,
-                   diag_operators=list(diag_ops) or None, off_diag_operators=None)
+        raw = dict(log_weights=log_weights, energy=E, diag=diag_np, off_diag=off_np,
+                   off_diag_norm=off_norm_np, off_diag_steps=off_steps,
+                   additional_diag=additional_diag, p_array=list(p_sorted), m_array=m_array,
+                   diag_operators=list(diag_ops) or None,
+                   off_diag_operators=list(off_ops) or None)
         return out_walkers, energy, obs, stats, raw
     return out_walkers, energy, obs, stats
 

@@ -8,7 +8,8 @@ measured on the GPU:
   ``sum_e J_z/4 s_i s_j - sum_k h_z s_k / 2`` (SpinHalf.py:291-314), one
   value per walker;
 * :class:`SzSzStructureFactor` -- ``(|sum_k e^{i q.r_k} s_k|^2 - N) / 4``
-  per wavevector q (observables.py:68-165), one value per wavevector.
+  per wavevector q (observables.py:68-165), one value per wavevector;
+* :class:`Identity` -- a constant, one value per walker.
 
 ``lower_diag_operators`` turns a ``{name: operator}`` mapping into the
 column tables of ``gfmc_diag_desc``: one ``GFMC_DIAG_PAIR`` column per
@@ -22,7 +23,8 @@ import numpy as np
 
 from .hamiltonian import SpinHalf
 
-__all__ = ["SzSzStructureFactor", "is_valid_k", "lower_diag_operators", "DiagLowering"]
+__all__ = ["SzSzStructureFactor", "Identity", "is_valid_k", "lower_diag_operators",
+           "DiagLowering"]
 
 
 def is_valid_k(k_vectors, lattice_vectors, atol=1e-5):
@@ -90,6 +92,30 @@ class SzSzStructureFactor:
         return b"|".join([b"sf", np.ascontiguousarray(self.cos).tobytes(),
                           np.ascontiguousarray(self.sin).tobytes(),
                           np.float64(self.offset).tobytes()])
+
+
+class Identity:
+    """Measurement-only constant operator on N sites: one diagonal element
+    ``value`` (1 by default) and no off-diagonal slot (M_total = 1).  As an
+    off-diagonal measurement its forward-walking value is the shadow chain's
+    bare weight product, which replays the main chain exactly when the
+    shadow consumes the main chain's own future draws, so its estimate is 1
+    for every window (the identity check of test_identity_forward_walk.py)."""
+
+    M_total = 1
+    n_off = 0
+
+    def __init__(self, N, value=1.0):
+        self.N = int(N)
+        self.value = float(value)
+
+    def diagonal_weight(self, config):
+        c = np.asarray(config, dtype=np.float64)
+        return np.full(c.shape[:-1], self.value)
+
+    def fingerprint(self):
+        return b"identity|" + np.asarray([self.N], dtype=np.int64).tobytes() + \
+            np.float64(self.value).tobytes()
 
 
 class DiagLowering:
@@ -184,9 +210,21 @@ def lower_diag_operators(ops, N):
                 a_rows.append(op.cos[q])
                 b_rows.append(op.sin[q])
                 c.append(op.offset)
+        elif isinstance(op, Identity):
+            if op.N != N:
+                raise ValueError(f"diagonal operator {name!r} has {op.N} sites, the "
+                                 f"Hamiltonian {N}")
+            names.append(name)
+            widths.append(1)
+            kind.append(0)
+            edge_off.append(len(ei))
+            a_rows.append(np.zeros(N))
+            b_rows.append(np.zeros(N))
+            c.append(op.value)
         else:
             raise TypeError(f"diagonal operator {name!r} of type {type(op).__name__} is not "
-                            "supported on the GPU (SpinHalf and SzSzStructureFactor are)")
+                            "supported on the GPU (SpinHalf, SzSzStructureFactor and "
+                            "Identity are)")
     if len(kind) > 256:
         raise ValueError(f"{len(kind)} value columns; at most 256 are supported")
     return DiagLowering(names, widths, kind, edge_off, ei, ej, ec, np.stack(a_rows),

@@ -18,6 +18,13 @@ Kept (data only):
   their estimates at p = 2, 4 (test_golden_outputs.py:191-211; the runs also
   measured off-diagonal operators, so the estimates use the common window
   trim i >= max(m) = 2).
+* ``spinhalf_*/raw/off_diag/{spsm,sx,sfx}``, ``.../raw/off_diag_norm/...``
+  (300, 2), ``.../raw/off_diag_steps`` (300,) and ``spinhalf_*/obs/{spsm,
+  sx,sfx}/<p>``: the off-diagonal forward-walking rows (walker mean of the
+  operator values after m shadow steps, the shadow's log weight product) of
+  spsm = SpinHalf.create(edges, J_x=2, J_y=2), sx = SpinHalf.create(edges,
+  h_x=-1) and sfx = 4 S^x_i S^x_j (1 + cos(pi/2)) on the bonds
+  (test_golden_outputs.py:181-205), and their pure estimates at p = 2, 4.
 
 Run in the build container only (the reference is not on the GPU box):
     python tests/golden/extract_golden.py
@@ -51,6 +58,13 @@ def main():
                     if (k in (f"{case}/raw/diag/{op}", f"{case}/raw/additional_diag/{op}")
                             or k.startswith(f"{case}/obs/{op}/")):
                         out[k] = g[k]
+            for op in ("spsm", "sx", "sfx"):
+                for k in g.files:
+                    if (k in (f"{case}/raw/off_diag/{op}", f"{case}/raw/off_diag_norm/{op}")
+                            or k.startswith(f"{case}/obs/{op}/")):
+                        out[k] = g[k]
+            if f"{case}/raw/off_diag_steps" in g.files:
+                out[f"{case}/raw/off_diag_steps"] = g[f"{case}/raw/off_diag_steps"]
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays from {len(cases)} cases")
 
