@@ -14,7 +14,10 @@ archive (no pickled objects; ``numpy.load(allow_pickle=False)`` reads it):
 * ``run_<n>/{log_weights,energy,rng_key,rng_key_next,avg_log_weight}`` per
   appended run, and with diagonal operators ``run_<n>/diag/<name>`` and the
   lead-in ``run_<n>/additional_diag/<name>`` (the diagonal rows of the
-  max(m) steps after the run, traced with its ancestry).
+  max(m) steps after the run, traced with its ancestry); with off-diagonal
+  operators ``run_<n>/off_diag/<name>``, ``run_<n>/off_diag_norm/<name>``
+  and ``run_<n>/off_diag_steps`` (the run's own 0-based steps; ``load_raw``
+  offsets them by the steps of the runs before, io.py:505-515).
 
 ``run_from_checkpoint`` continues the stored chain without a warm-up, with
 the stored shift, walkers and Philox position: the series of a run split
@@ -38,7 +41,14 @@ from .walkers import Walker
 __all__ = ["write_checkpoint", "read_checkpoint", "load_raw", "run_from_checkpoint"]
 
 MUST_MATCH = ("step_size", "p_array", "method", "lam", "precision", "n_sites", "n_walkers",
-              "hamiltonian_sha1", "guide_sha1", "diag_operators", "diag_operator_hashes")
+              "hamiltonian_sha1", "guide_sha1", "diag_operators", "diag_operator_hashes",
+              "off_diag_operators", "off_diag_operator_hashes", "off_diag_stride",
+              "off_diag_equal_windows")
+
+
+def _joined(stats, key):
+    return np.asarray(",".join(x.decode() if isinstance(x, bytes) else str(x)
+                               for x in stats.get(key, [])))
 
 
 def _sha1(obj):
@@ -73,17 +83,17 @@ def write_checkpoint(filename, stats, walkers, hamiltonian, guide=None, raw=None
         "n_walkers": np.int64(walkers.n_walkers),
         "hamiltonian_sha1": np.asarray(_sha1(hamiltonian)),
         "guide_sha1": np.asarray(_sha1(guide)),
-        "diag_operators": np.asarray(",".join(
-            x.decode() if isinstance(x, bytes) else str(x)
-            for x in stats.get("parameters/diag_operators", []))),
-        "diag_operator_hashes": np.asarray(",".join(
-            x.decode() if isinstance(x, bytes) else str(x)
-            for x in stats.get("parameters/diag_operator_hashes", []))),
+        "diag_operators": _joined(stats, "parameters/diag_operators"),
+        "diag_operator_hashes": _joined(stats, "parameters/diag_operator_hashes"),
+        "off_diag_operators": _joined(stats, "parameters/off_diag_operators"),
+        "off_diag_operator_hashes": _joined(stats, "parameters/off_diag_operator_hashes"),
+        "off_diag_stride": np.int64(stats.get("parameters/off_diag_stride", 1)),
+        "off_diag_equal_windows": np.int64(stats.get("parameters/off_diag_equal_windows", 0)),
     }
     data = _read(filename)
     if data:
         for k in MUST_MATCH:
-            old, new = data[f"parameters/{k}"], params[k]
+            old, new = data.get(f"parameters/{k}", params[k]), params[k]
             same = (np.array_equal(old, new) or
                     (np.asarray(old).dtype.kind == "f" and np.isnan(old) and np.isnan(new)))
             if not same:
@@ -96,11 +106,13 @@ def write_checkpoint(filename, stats, walkers, hamiltonian, guide=None, raw=None
     if raw is not None:
         data[f"run_{n}/log_weights"] = np.asarray(raw["log_weights"], dtype=np.float64)
         data[f"run_{n}/energy"] = np.asarray(raw["energy"], dtype=np.float64)
-        for group in ("diag", "additional_diag"):
+        for group in ("diag", "additional_diag", "off_diag", "off_diag_norm"):
             for name, arr in (raw.get(group) or {}).items():
-                if "/" in name:
-                    raise ValueError(f"operator name {name!r} contains '/'")
+                if "/" in name or "," in name:
+                    raise ValueError(f"operator name {name!r} contains '/' or ','")
                 data[f"run_{n}/{group}/{name}"] = np.asarray(arr, dtype=np.float64)
+        if raw.get("off_diag_steps") is not None:
+            data[f"run_{n}/off_diag_steps"] = np.asarray(raw["off_diag_steps"], dtype=np.int64)
     data[f"run_{n}/rng_key"] = np.asarray(stats["parameters/rng_key"], dtype=np.uint64)
     data[f"run_{n}/rng_key_next"] = np.asarray(stats["parameters/rng_key_next"],
                                                dtype=np.uint64)
@@ -137,12 +149,23 @@ def load_raw(filename):
     names = _diag_names(data)
     diag = ({name: np.concatenate([data[f"run_{n}/diag/{name}"] for n in runs])
              for name in names} if names else None)
+    onames = _diag_names(data, "off_diag_operators")
+    off = off_norm = off_steps = None
+    if onames:
+        off = {name: np.concatenate([data[f"run_{n}/off_diag/{name}"] for n in runs])
+               for name in onames}
+        off_norm = {name: np.concatenate([data[f"run_{n}/off_diag_norm/{name}"] for n in runs])
+                    for name in onames}
+        starts = np.cumsum([0] + [len(data[f"run_{n}/energy"]) for n in runs])
+        off_steps = np.concatenate([data[f"run_{n}/off_diag_steps"] + starts[i]
+                                    for i, n in enumerate(runs)]).astype(np.int64)
     return dict(log_weights=lw, energy=E, p_array=p, m_array=[q // 2 for q in p], diag=diag,
-                diag_operators=names or None)
+                diag_operators=names or None, off_diag=off, off_diag_norm=off_norm,
+                off_diag_steps=off_steps, off_diag_operators=onames or None)
 
 
-def _diag_names(data):
-    s = str(data.get("parameters/diag_operators", ""))
+def _diag_names(data, key="diag_operators"):
+    s = str(data.get(f"parameters/{key}", ""))
     return [x for x in s.split(",") if x]
 
 
@@ -177,6 +200,19 @@ def run_from_checkpoint(filename, hamiltonian, guide=None, n_steps=None, **run_k
     if given != names:
         raise ValueError(f"{filename}: the stored run measured diagonal operators {names}, "
                          f"this call {given}; a continuation must measure the same ones")
+    onames = _diag_names(data, "off_diag_operators")
+    ogiven = list(run_kwargs.get("off_diag_operators") or {})
+    if ogiven != onames:
+        raise ValueError(f"{filename}: the stored run measured off-diagonal operators "
+                         f"{onames}, this call {ogiven}; a continuation must measure the same "
+                         f"ones")
+    for k in ("off_diag_stride", "return_raw"):
+        if k in run_kwargs:
+            raise ValueError(f"{k!r} comes from the checkpoint and cannot be overridden")
+    if "parameters/off_diag_stride" in data:
+        run_kwargs["off_diag_stride"] = int(data["parameters/off_diag_stride"])
+    if int(data.get("parameters/off_diag_equal_windows", 0)):
+        run_kwargs["return_raw"] = False
     out = run_gfmc(w, n_steps, hamiltonian, float(data["parameters/step_size"]), guide,
                    [int(v) for v in data["parameters/p_array"]], method=method,
                    lam=None if np.isnan(lam) else lam, rng_key=key,

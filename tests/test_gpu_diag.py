@@ -174,7 +174,8 @@ def test_run_reports_estimates_of_its_series():
     _, _, obs_e, _, raw_e = run(w, 400, H, 0.2, G, [2, 4, 8], rng_key=3, warmup_steps=40,
                                 verbosity=0, diag_operators=ops, lanes=64, use_graph=False)
     np.testing.assert_array_equal(raw_e["diag"]["sf"], raw["diag"]["sf"])
-    with pytest.raises(NotImplementedError):
+    # the Hamiltonian itself is no measurement operator: its rates are <= 0
+    with pytest.raises(ValueError, match="negative rate"):
         run(w, 50, H, 0.2, G, [2], rng_key=3, warmup_steps=8, verbosity=0,
             off_diag_operators={"x": H})
 

@@ -239,3 +239,40 @@ def test_forward_walking_reaches_the_pure_off_diagonal_values():
             f"{oe[k][24]:.5f}; pure {pure[k]:.5f}")
         assert abs(o[k][24] - pure[k]) < 5 * oe[k][24] + 0.01
     assert abs(E[24] - vals[0]) < 0.01
+
+
+def test_continuation_equals_uninterrupted_run(tmp_path):
+    """A run split at a multiple of the stride and continued from its
+    checkpoint measures the same off-diagonal rows as the uninterrupted run
+    (the shadow chains use absolute Philox positions); load_raw offsets the
+    continuation's steps."""
+    say("offdiag resume")
+    from pygfmc_amd import load_raw, run_from_checkpoint, write_checkpoint
+    H, G = golden_system()
+    ops = {"spsm": ops8()["spsm"], "one": Identity(8)}
+    kw = dict(rng_key=4, verbosity=0, lanes=64, number_of_outputs=4, off_diag_operators=ops,
+              off_diag_stride=3)
+    w = prepare_walkers_random(192, 8, 6)
+    _, _, _, _, raw_all = run(w, 96, H, 0.2, G, [2, 6], warmup_steps=30, **kw)
+    w1, _, _, st1, raw1 = run(w, 60, H, 0.2, G, [2, 6], warmup_steps=30, **kw)
+    f = str(tmp_path / "o.npz")
+    write_checkpoint(f, st1, w1, H, G, raw1)
+    w2, _, _, st2, raw2 = run_from_checkpoint(f, H, G, n_steps=36, verbosity=0,
+                                              number_of_outputs=4, off_diag_operators=ops)
+    for name in ops:
+        for grp in ("off_diag", "off_diag_norm"):
+            np.testing.assert_array_equal(np.concatenate([raw1[grp][name], raw2[grp][name]]),
+                                          raw_all[grp][name])
+    np.testing.assert_array_equal(np.concatenate([raw1["off_diag_steps"],
+                                                  raw2["off_diag_steps"] + 60]),
+                                  raw_all["off_diag_steps"])
+    write_checkpoint(f, st2, w2, H, G, raw2)
+    both = load_raw(f)
+    np.testing.assert_array_equal(both["off_diag_steps"], raw_all["off_diag_steps"])
+    E_a, o_a = evaluate_observables(both)
+    E_b, o_b = evaluate_observables(raw_all)
+    for name in ops:
+        for p in (2, 6):
+            assert o_a[name][p] == o_b[name][p]
+    with pytest.raises(ValueError, match="off-diagonal operators"):
+        run_from_checkpoint(f, H, G, n_steps=12, verbosity=0)
