@@ -104,6 +104,8 @@ def lib():
         L.orc_mlp_hidden.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f32p]
         L.orc_mlp_O.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f64p]
         L.orc_mlp_vmin_grad.argtypes = [D, ctypes.c_int64, _u32p, ctypes.c_double, _f64p, _f64p]
+        L.orc_neg_log_unit.restype = ctypes.c_double
+        L.orc_neg_log_unit.argtypes = [ctypes.c_double]
         L.orc_apply_operator.argtypes = [D, ctypes.c_int64, _u32p, ctypes.c_uint64,
                                          ctypes.c_uint32, ctypes.c_uint32, _u32p, _f64p]
         del vp
@@ -249,6 +251,11 @@ def philox(ctr, key):
     lib().orc_philox(_p(ctr, ctypes.c_uint32), _p(key, ctypes.c_uint32),
                      _p(out, ctypes.c_uint32))
     return out
+
+
+def neg_log_unit(q):
+    """-log(q) as the continuous-time hold computes it (orc_neg_log_unit)."""
+    return np.array([lib().orc_neg_log_unit(float(x)) for x in np.atleast_1d(q)])
 
 
 def wave_tree(v):

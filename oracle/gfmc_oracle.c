@@ -125,25 +125,78 @@ double orc_blocked_scan(const double *v, int n, double *lane_incl) {
     return s[G - 1];
 }
 
-/* Inverse-CDF draw of a slot of w[0:n] (all >= 0, total > 0) at u in
- * [0, total): the first lane whose inclusive sum exceeds u, then the first
- * slot of that lane whose running CDF (started at the previous lane's
- * inclusive sum) exceeds u.  Where rounding leaves u past that lane's last
- * slot, the last positive slot at or before it is taken -- the rule of
- * sample_index (evolution.py:30-59): a zero-weight slot is never returned. */
-int orc_sample_blocked(const double *w, int n, const double *lane_incl, double u) {
-    int K = block_k(n);
-    int j = 0;
-    while (j < g_lanes - 1 && !(lane_incl[j] > u)) ++j;
-    double c = (j == 0) ? 0.0 : lane_incl[j - 1];
-    int lo = j * K, hi = lo + K < n ? lo + K : n;
-    for (int k = lo; k < hi; ++k) {
-        c += w[k];
-        if (c > u) return k;
-    }
-    for (int k = (hi < n ? hi : n) - 1; k >= 0; --k)
+/* The last positive slot of w[0:n] at or before slot lim (a zero-weight
+ * slot is never returned, sample_index's rule, evolution.py:30-59); 0 when
+ * there is none. */
+static int last_positive(const double *w, int n, int lim) {
+    for (int k = (lim < n ? lim : n - 1); k >= 0; --k)
         if (w[k] > 0) return k;
     return 0;
+}
+
+/* Inverse-CDF draw of a slot of w[0:n] (all >= 0, total > 0) at u in
+ * [0, total): j = the first lane whose inclusive sum exceeds u; inside lane
+ * j's block of K slots the CDF is base + the group's tree scan (wave_tree_g
+ * over G lanes) of G consecutive slots at a time, base starting at the
+ * previous lane's inclusive sum and advancing by each chunk's total; the
+ * first slot whose CDF exceeds u is drawn.  A zero-weight slot that rounding
+ * selects gives way to the last positive slot before it; where rounding
+ * leaves u past lane j's last slot, the last positive slot of lane j's block
+ * or before it is taken (sample_index, evolution.py:30-59). */
+int orc_sample_blocked(const double *w, int n, const double *lane_incl, double u) {
+    int K = block_k(n), G = g_lanes;
+    int j = 0;
+    while (j < G - 1 && !(lane_incl[j] > u)) ++j;
+    double base = (j == 0) ? 0.0 : lane_incl[j - 1];
+    double s[WAVE];
+    for (int c0 = 0; c0 < K; c0 += G) {
+        int ok[WAVE];
+        for (int l = 0; l < G; ++l) {
+            int k = c0 + l, slot = j * K + k;
+            ok[l] = k < K && slot < n;
+            s[l] = ok[l] ? w[slot] : 0.0;
+        }
+        wave_tree_g(s, G);
+        for (int l = 0; l < G; ++l) {
+            if (ok[l] && base + s[l] > u) {
+                int slot = j * K + c0 + l;
+                return w[slot] > 0 ? slot : last_positive(w, n, slot - 1);
+            }
+        }
+        base = base + s[G - 1];
+    }
+    int end = j * K + K - 1;
+    return last_positive(w, n, end);
+}
+
+/* -log(q) for q in [2^-1022, 1] (the continuous-time hold, q = 1 - psi
+ * exactly): fdlibm's e_log.c reduction and polynomial (Sun Microsystems,
+ * 1993), q = 2^k (1 + f), 1 + f in [sqrt(2)/2, sqrt(2)), s = f / (2 + f),
+ * log q = k ln2_hi - ((f^2/2 - (s (f^2/2 + R(s^2)) + k ln2_lo)) - f), the
+ * operations of the kernels in the same order. */
+double orc_neg_log_unit(double q) {
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double L1 = 6.666666666666735130e-01, L2 = 3.999999999940941908e-01;
+    const double L3 = 2.857142874366239149e-01, L4 = 2.222219843214978396e-01;
+    const double L5 = 1.818357216161805012e-01, L6 = 1.531383769920937332e-01;
+    const double L7 = 1.479819860511658591e-01;
+    uint64_t b;
+    memcpy(&b, &q, sizeof b);
+    uint32_t hi = (uint32_t)(b >> 32) & 0x000fffffu;
+    uint32_t up = (hi + 0x95f64u) & 0x100000u;
+    int k = (int)((b >> 52) & 0x7ffu) - 1023 + (int)(up >> 20);
+    uint64_t mb = ((uint64_t)(hi | (up ^ 0x3ff00000u)) << 32) | (b & 0xffffffffull);
+    double m;
+    memcpy(&m, &mb, sizeof m);
+    double f = m - 1.0;
+    double sr = f / (2.0 + f);
+    double z = sr * sr, ww = z * z;
+    double t1 = ww * (L2 + ww * (L4 + ww * L6));
+    double t2 = z * (L1 + ww * (L3 + ww * (L5 + ww * L7)));
+    double R = t2 + t1;
+    double hfsq = 0.5 * f * f;
+    double dk = (double)k;
+    return -(dk * ln2_hi - ((hfsq - (sr * (hfsq + R) + dk * ln2_lo)) - f));
 }
 
 /* Host entry used by the sampler tests: draws for several uniforms in [0,1)
@@ -575,7 +628,7 @@ static void propagate_one(const gfmc_problem_desc *d, int64_t widx, uint32_t *sp
                 double u = U1 * R, cap = nextafter(R, 0.0);
                 if (u > cap) u = cap;
                 idx = orc_sample_blocked(w, n_off, incl, u);
-                tau_raw = -log1p(-psi) / R;
+                tau_raw = orc_neg_log_unit(1.0 - psi) / R;
             }
             double rem = beta - t;
             double tau = tau_raw < rem ? tau_raw : rem;

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/gfmc.h"
+#include "gfmc_math.h"
 #include "philox.h"
 
 #define WAVE 64
@@ -514,53 +515,57 @@ __device__ __forceinline__ T lane_scan(const DevProblem &P, const WalkerLds<T> &
     return grp_incl_scan<G>(acc);
 }
 
+// The last slot of weight > 0 at or before slot lim, G slots per round
+// (slot t lives at LDS position (t % K) G + t / K); 0 when there is none.
+template <typename T, int G = WAVE>
+__device__ int last_positive_slot(const DevProblem &P, const WalkerLds<T> &L, int K, int lim,
+                                  int lane) {
+    const int gl = lane & (G - 1);
+    if (lim > P.n_off - 1) lim = P.n_off - 1;
+    for (int top = lim; top >= 0; top -= G) {
+        const int t = top - gl;
+        const bool pos = t >= 0 && L.w[wpos_g<G>(t, K)] > (T)0;
+        const unsigned long long b = grp_ballot<G>(pos, lane);
+        if (b) return top - (__ffsll((unsigned long long)b) - 1);
+    }
+    return 0;
+}
+
 // Inverse-CDF draw (sample_index, evolution.py:30-59) in blocked order over
-// the group's G lanes; u in [0, R).  Mirrors orc_sample_blocked.
+// the group's G lanes; u in [0, R).  Lane j, the first whose inclusive sum
+// exceeds u, holds the draw; inside its block of K slots the CDF is base +
+// the group's tree scan of G consecutive slots at a time (one LDS read and
+// one DPP scan per lane and chunk), base starting at lane j - 1's inclusive
+// sum; the first slot whose CDF exceeds u is drawn.  A zero-weight slot that
+// rounding selects gives way to the last positive slot before it, and where
+// rounding leaves u past lane j's block the last positive slot of the block
+// or before it is taken.  Mirrors orc_sample_blocked.
 template <typename T, int G = WAVE>
 __device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T lane_sum, T u,
                            int lane) {
+    (void)lane_sum;
     const int gl = lane & (G - 1);
     const int K = G == WAVE ? P.K : P.Kg[gidx<G>()];
-    unsigned long long hit = grp_ballot<G>(incl > u, lane);
-    int j = hit ? __ffsll((unsigned long long)hit) - 1 : G - 1;
-    T prev = grp_bcast<G>(incl, j > 0 ? j - 1 : 0);
-    if (j == 0) prev = (T)0;
-    int idx = -1;
-    if (gl == j) {
-        // sequential running sum of lane j's block, eight LDS reads in flight
-        T c = prev;
-        for (int k0 = 0; k0 < K && idx < 0; k0 += 8) {
-            T v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = (k0 + q < K) ? L.w[(k0 + q) * G + j] : (T)0;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int s = j * K + k0 + q;
-                if (idx < 0 && k0 + q < K && s < P.n_off) {
-                    c += v[q];
-                    if (c > u) idx = s;
-                }
-            }
+    const unsigned long long hit = grp_ballot<G>(incl > u, lane);
+    const int j = hit ? __ffsll((unsigned long long)hit) - 1 : G - 1;
+    T base = grp_bcast<G>(incl, j > 0 ? j - 1 : 0);
+    if (j == 0) base = (T)0;
+    for (int c0 = 0; c0 < K; c0 += G) {
+        const int k = c0 + gl;
+        const bool ok = k < K && j * K + k < P.n_off;
+        const T v = ok ? L.w[k * G + j] : (T)0;
+        const T cdf = base + grp_incl_scan<G>(v);
+        const unsigned long long h = grp_ballot<G>(ok && cdf > u, lane);
+        if (h) {
+            const int g0 = __ffsll((unsigned long long)h) - 1;
+            const int slot = j * K + c0 + g0;
+            const unsigned long long pos = grp_ballot<G>(v > (T)0, lane);
+            if ((pos >> g0) & 1ull) return slot;
+            return last_positive_slot<T, G>(P, L, K, slot - 1, lane);
         }
+        base = grp_bcast<G>(cdf, G - 1);
     }
-    idx = grp_bcast<G>(idx, j);
-    if (idx < 0) {
-        // rounding left u past lane j: the last positive slot at or before it
-        unsigned long long pos = grp_ballot<G>(lane_sum > (T)0 && gl <= j, lane);
-        int jj = pos ? 63 - __clzll((long long)pos) : 0;
-        int found = 0;
-        if (gl == jj) {
-            for (int k = K - 1; k >= 0; --k) {
-                int s = jj * K + k;
-                if (s < P.n_off && L.w[k * G + jj] > (T)0) {
-                    found = s;
-                    break;
-                }
-            }
-        }
-        idx = grp_bcast<G>(found, jj);
-    }
-    return idx;
+    return last_positive_slot<T, G>(P, L, K, j * K + K - 1, lane);
 }
 
 // Data of move idx that the record path prefetches before the hold
@@ -680,6 +685,9 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                                                  uint32_t key0, uint32_t key1, uint32_t step,
                                                  uint32_t tag) {
     const int gl = lane & (G - 1);
+    // one walker per wavefront: its index is wave-uniform, so the Philox
+    // rounds of every hop run on the scalar unit
+    if constexpr (G == WAVE) wk = __builtin_amdgcn_readfirstlane(wk);
     WalkerLds<T> L = FAST ? carve_g<T, G>(P, lds) : carve<T>(P, lds);
     load_spins<G>(P, sp, L.x, gl);
     wave_sync();
@@ -719,7 +727,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                 if (u > cap) u = cap;
                 idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
                 if constexpr (FAST) m = prefetch_move(P, idx, gl);  // in flight during the hold math
-                tau_raw = -log1p(-psi) / (double)R;
+                tau_raw = gfmc_neg_log_unit(1.0 - psi) / (double)R;
             }
             double rem = beta - t;
             double tau = tau_raw < rem ? tau_raw : rem;
@@ -1142,7 +1150,7 @@ __global__ __launch_bounds__(RBM_WAVES * WAVE) void k_propagate_rbm(
                 const float cap = below(R);
                 if (u > cap) u = cap;
                 idx = sample_slot(P, WL, incl, lane_sum, u, lane);
-                tau_raw = -log1p(-psi) / (double)R;
+                tau_raw = gfmc_neg_log_unit(1.0 - psi) / (double)R;
             }
             const double rem = beta - t;
             const double tau = tau_raw < rem ? tau_raw : rem;

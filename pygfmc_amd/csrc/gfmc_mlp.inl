@@ -538,7 +538,7 @@ __global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_select(
             const float cap = below(R);
             if (u > cap) u = cap;
             idx = sample_slot(P, L, incl, lane_sum, u, lane);
-            tau_raw = -log1p(-psi) / (double)R;
+            tau_raw = gfmc_neg_log_unit(1.0 - psi) / (double)R;
         }
         const double rem = beta - t;
         const double tau = tau_raw < rem ? tau_raw : rem;

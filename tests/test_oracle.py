@@ -222,6 +222,57 @@ def test_sampler_matches_exact_searchsorted_on_exact_sums(orc):
     np.testing.assert_array_equal(idx, ref)
 
 
+def test_in_block_tree_order_differs_from_a_sequential_cdf(orc):
+    """Inside the drawn lane's block the CDF is the group's tree scan of the
+    block, not a sequential running sum: at a point u between the two
+    roundings of one slot's CDF value they draw different slots (so the
+    kernels' bit-exact move tests tell the orders apart)."""
+    import ctypes
+    rng = np.random.default_rng(5)
+    K = 40
+    n = 64 * K
+    w = rng.uniform(0.5, 1.5, n) * 10.0 ** rng.integers(-6, 3, n)
+    tot, incl = orc.blocked_scan(w)
+    L = orc.lib()
+    L.orc_sample_blocked.restype = ctypes.c_int
+    L.orc_sample_blocked.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.c_double]
+    wp = w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ip = incl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    found = 0
+    for j in range(1, 64):
+        blk = w[j * K:(j + 1) * K]
+        c, seq_exact = incl[j - 1], []
+        for v in blk:
+            c = c + v
+            seq_exact.append(c)
+        tree = incl[j - 1] + orc.wave_tree(np.concatenate([blk, np.zeros(64 - K)]))[:K]
+        for k in range(K - 1):
+            if tree[k] > seq_exact[k]:
+                x = seq_exact[k]                     # tree: slot k, sequential: later
+                assert L.orc_sample_blocked(wp, n, ip, x) == j * K + k
+                found += 1
+            elif tree[k] < seq_exact[k]:
+                x = tree[k]                          # tree: later, sequential: slot k
+                assert L.orc_sample_blocked(wp, n, ip, x) > j * K + k
+                found += 1
+    assert found > 20
+
+
+def test_neg_log_unit_is_minus_log_within_one_ulp(orc):
+    """The hold's -log(1 - psi) (fdlibm's reduction and polynomial) against
+    libm's log on 53-bit uniforms, the ends q = 1 and q = 2^-53 included."""
+    rng = np.random.default_rng(3)
+    psi = (rng.integers(0, 2 ** 53, 200000, dtype=np.int64)).astype(np.float64) / 2.0 ** 53
+    q = 1.0 - psi
+    got = orc.neg_log_unit(q)
+    want = -np.log(q)
+    nz = want != 0
+    assert np.max(np.abs(got[nz] - want[nz]) / np.spacing(want[nz])) <= 1.0
+    assert orc.neg_log_unit([1.0])[0] == 0.0
+    assert orc.neg_log_unit([2.0 ** -53])[0] == -np.log(2.0 ** -53)
+
+
 # -- systematic resampling KATs (test_walker_io.py:60-77) --------------------------
 
 def _parents_from_w(orc, w, psi):
