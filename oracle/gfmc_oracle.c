@@ -10,10 +10,13 @@
  * uniforms -- is the reduction order of the off-diagonal weight sum and its
  * CDF: "blocked wave order" (blocked_scan below), where slot s belongs to
  * lane s / K (K = ceil(n / 64)), each lane sums its block sequentially and
- * the 64 lane sums are combined by the wave-tree inclusive scan.  The
- * reference's own order is XLA's (cumsum, evolution.py:633); both are the
- * same sum to within a few ulps.  Reconfiguration uses the same idea with
- * 1024-walker blocks (rc_block_incl below).
+ * the 64 lane sums are combined by the wave-tree inclusive scan; inside the
+ * drawn lane's block the CDF is a tree scan of the block's slots
+ * (orc_sample_blocked).  The reference's own order is XLA's (cumsum,
+ * evolution.py:633); both are the same sum to within a few ulps.
+ * Reconfiguration uses the same idea with 1024-walker blocks (rc_block_incl
+ * below).  The continuous-time hold's logarithm is a fixed operation
+ * sequence (orc_neg_log_unit) shared with the kernels.
  *
  * Pinning: the rates against the reference's golden local energies and a
  * Kronecker-product matrix, the sampler and resampling against the
