@@ -341,29 +341,35 @@ template <typename T>
 __device__ __forceinline__ T eval_rec(const DevProblem &P, const Tables<T> &tb, const Rec &r,
                                       const signed char *x) {
     const int i = rec_f(r, 1), j = rec_f(r, 2), mask = rec_f(r, 3) & 0xffff;
-    const int D = P.max_deg;
     const bool jas = P.guide == GFMC_GUIDE_JASTROW;
+    // all REC_D partner fields are read: the fields past the guide's degree
+    // hold site i with a clear mask bit, so their terms are +-0 and leave
+    // the sums bit-identical to sums over the degree; no branches, so the
+    // partner spins are read from LDS in one batch
     if (j < 0) {
         const T bare = tb.hx0;
         if (bare == (T)0 || !jas) return -bare;
+        signed char xs[REC_D];
+#pragma unroll
+        for (int q = 0; q < REC_D; ++q) xs[q] = x[rec_f(r, 4 + q)];
         T mf = (T)0;
 #pragma unroll
-        for (int q = 0; q < REC_D; ++q)
-            if (q < D) mf += (((mask >> q) & 1) ? tb.J0 : (T)0) * (T)x[rec_f(r, 4 + q)];
+        for (int q = 0; q < REC_D; ++q) mf += (((mask >> q) & 1) ? tb.J0 : (T)0) * (T)xs[q];
         const T xk = (T)x[i];
         return -(bare * dev_exp(((T)-2 * xk) * mf - ((T)2 * tb.h0) * xk));
     }
     const T xi = (T)x[i], xj = (T)x[j];
     const T bare = (x[i] != x[j]) ? tb.jpm0 : tb.jpmpm0;
     if (bare == (T)0 || !jas) return -bare;
+    signed char xs[2 * REC_D];
+#pragma unroll
+    for (int q = 0; q < 2 * REC_D; ++q) xs[q] = x[rec_f(r, 4 + q)];
     T mfi = (T)0, mfj = (T)0;
 #pragma unroll
-    for (int q = 0; q < REC_D; ++q)
-        if (q < D) mfi += (((mask >> q) & 1) ? tb.J0 : (T)0) * (T)x[rec_f(r, 4 + q)];
+    for (int q = 0; q < REC_D; ++q) mfi += (((mask >> q) & 1) ? tb.J0 : (T)0) * (T)xs[q];
 #pragma unroll
     for (int q = 0; q < REC_D; ++q)
-        if (q < D)
-            mfj += (((mask >> (REC_D + q)) & 1) ? tb.J0 : (T)0) * (T)x[rec_f(r, 4 + REC_D + q)];
+        mfj += (((mask >> (REC_D + q)) & 1) ? tb.J0 : (T)0) * (T)xs[REC_D + q];
     const T pair = xi * mfi + xj * mfj;
     const T hterm = tb.h0 * xi + tb.h0 * xj;
     return -(bare * dev_exp((T)-2 * pair - (T)2 * hterm));
