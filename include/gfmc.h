@@ -516,6 +516,50 @@ int gfmc_graph_capture_steps_diag(const gfmc_problem *p, int64_t n_walkers, uint
                                   gfmc_graph **out);
 
 /*
+ * Carried slot weights (the reference's carried rates: initial_carried /
+ * gather_carried, engine/evolution.py, driver.py:181-342).  With a carry
+ * buffer the step loop ends every propagation by storing each walker's slot
+ * weights, and the next propagation starts each walker from its parent's
+ * stored weights instead of re-evaluating every slot.  The stored weights
+ * are exactly what a full evaluation of the same configuration gives, so
+ * the results are bit-identical to runs without a carry.  Used by the
+ * record path (uniform couplings, Jastrow or no guide); 0 bytes otherwise.
+ * The buffer is caller-owned, 256-byte aligned, and starts (or restarts,
+ * e.g. after the walkers were changed outside these loops) with its first
+ * 256 bytes zero.
+ */
+int64_t gfmc_carry_bytes(const gfmc_problem *p, int64_t n_walkers);
+
+/* gfmc_propagate with a carry buffer: starts walker w from the stored rows
+ * of parents[w] (the last resampling's parents) when the carry holds rows,
+ * and stores this propagation's rows. */
+int gfmc_propagate_carry(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
+                         double *log_w, double *e_loc, double beta, double alw, uint64_t seed,
+                         uint32_t step, uint32_t tag, int32_t *evals, const int32_t *parents,
+                         void *carry, void *stream);
+
+/* gfmc_run_steps_diag (d may be NULL: no diagonal record) with a carry
+ * buffer of gfmc_carry_bytes bytes (NULL: none). */
+int gfmc_run_steps_carry(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
+                         double *log_w, double *e_loc, uint32_t *spins_b, double *e_loc_b,
+                         int32_t *parents, double *work, int32_t n_steps, double beta,
+                         double alw, uint64_t seed, uint32_t step0, uint32_t tag_prop,
+                         uint32_t tag_reconf, double *series, int32_t *evals,
+                         const gfmc_diag *d, void *ring, double *diag_series,
+                         double *diag_work, void *carry, void *stream);
+
+/* gfmc_graph_capture_steps over gfmc_run_steps_carry (the carry's state is
+ * device memory, so replays continue it). */
+int gfmc_graph_capture_steps_carry(const gfmc_problem *p, int64_t n_walkers, uint32_t *spins,
+                                   double *log_w, double *e_loc, uint32_t *spins_b,
+                                   double *e_loc_b, int32_t *parents, double *work,
+                                   int32_t n_steps, double beta, double alw, uint64_t seed,
+                                   uint32_t tag_prop, uint32_t tag_reconf, double *series,
+                                   int32_t *evals, const gfmc_diag *d, void *ring,
+                                   double *diag_series, double *diag_work, void *carry,
+                                   uint32_t *counter, gfmc_graph **out);
+
+/*
  * Off-diagonal forward walking: the pure estimator of an operator O with
  * off-diagonal elements (forward_walk_operator, evolution.py:892-1101).  O
  * is a SpinHalf lowered like a Hamiltonian (gfmc_problem_create) with the

@@ -178,6 +178,22 @@ SIGNATURES = [
                                                      ctypes.c_uint64, ctypes.c_uint32,
                                                      ctypes.c_uint32, _vp, _vp, _vp, _vp,
                                                      _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    ("gfmc_carry_bytes", ctypes.c_int64, [_vp, ctypes.c_int64]),
+    ("gfmc_propagate_carry", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
+                                            ctypes.c_uint32, _vp, _vp, _vp, _vp]),
+    ("gfmc_run_steps_carry", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            _vp, ctypes.c_int32, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
+                                            _vp, _vp, _vp, _vp, _vp]),
+    ("gfmc_graph_capture_steps_carry", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp,
+                                                      _vp, _vp, _vp, ctypes.c_int32,
+                                                      ctypes.c_double, ctypes.c_double,
+                                                      ctypes.c_uint64, ctypes.c_uint32,
+                                                      ctypes.c_uint32, _vp, _vp, _vp, _vp,
+                                                      _vp, _vp, _vp, _vp,
+                                                      ctypes.POINTER(_vp)]),
     ("gfmc_apply_operator_work_doubles", ctypes.c_int64, [_vp, ctypes.c_int64]),
     ("gfmc_apply_operator", ctypes.c_int, [_vp, ctypes.c_int64, _vp, ctypes.c_uint64,
                                            ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,

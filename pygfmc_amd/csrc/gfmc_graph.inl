@@ -79,6 +79,27 @@ int gfmc_graph_capture_steps(const gfmc_problem *p, int64_t nw, uint32_t *spins,
     }, out);
 }
 
+int gfmc_graph_capture_steps_carry(const gfmc_problem *p, int64_t nw, uint32_t *spins,
+                                   double *log_w, double *e_loc, uint32_t *spins_b,
+                                   double *e_loc_b, int32_t *parents, double *work,
+                                   int32_t n_steps, double beta, double alw, uint64_t seed,
+                                   uint32_t tag_prop, uint32_t tag_reconf, double *series,
+                                   int32_t *evals, const gfmc_diag *d, void *ring,
+                                   double *diag_series, double *diag_work, void *carry,
+                                   uint32_t *counter, gfmc_graph **out) {
+    if (n_steps < 1) return fail(GFMC_EINVAL, "gfmc_graph_capture_steps_carry: n_steps < 1");
+    return capture_graph("gfmc_graph_capture_steps_carry", counter, [&](void *st) {
+        int rc = gfmc_run_steps_carry(p, nw, spins, log_w, e_loc, spins_b, e_loc_b, parents,
+                                      work, n_steps, beta, alw, seed, 0u, tag_prop, tag_reconf,
+                                      series, evals, d, ring, diag_series, diag_work, carry, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(WAVE), 0, (hipStream_t)st, counter,
+                           (uint32_t)n_steps);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }, out);
+}
+
 int gfmc_graph_capture_batch_steps(const gfmc_batch *b, uint32_t *spins, double *log_w,
                                    double *e_loc, uint32_t *spins_b, double *e_loc_b,
                                    int32_t *parents, double *work, int32_t n_steps, double beta,

@@ -40,6 +40,25 @@ static thread_local std::string g_err;
 // while a step loop is captured into a graph (null otherwise)
 static thread_local const uint32_t *g_step_base = nullptr;
 
+// Carried slot weights of the record path (driver.py's carried rates,
+// initial_carried / gather_carried): a propagation ends by storing every
+// walker's slot weights and anti-parallel count; the next one, after
+// resampling, starts walker w from the rows of its parent instead of
+// re-evaluating every slot.  The stored weights are the exact values a full
+// evaluation of the same configuration gives (each weight is always
+// eval_rec of the current configuration), so results are bit-identical.
+// state[0] = rows valid, state[1] = buffer holding the latest rows,
+// state[2] = lanes per walker they were laid out for.
+struct CarryDev {
+    void *w[2];
+    int32_t *anti[2];
+    int32_t *state;
+    const int32_t *parents;
+    int64_t row;   // elements per walker row
+    int on;
+};
+static thread_local const CarryDev *g_carry = nullptr;
+
 static int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
@@ -689,15 +708,35 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                                                  uint32_t *sp, double *lw_p, double *el_p,
                                                  int32_t *ev_p, double step_size, double alw,
                                                  uint32_t key0, uint32_t key1, uint32_t step,
-                                                 uint32_t tag) {
+                                                 uint32_t tag, const CarryDev &cy) {
     const int gl = lane & (G - 1);
     // one walker per wavefront: its index is wave-uniform, so the Philox
     // rounds of every hop run on the scalar unit
     if constexpr (G == WAVE) wk = __builtin_amdgcn_readfirstlane(wk);
     WalkerLds<T> L = FAST ? carve_g<T, G>(P, lds) : carve<T>(P, lds);
     load_spins<G>(P, sp, L.x, gl);
-    wave_sync();
-    int n_anti = full_eval<T, FAST, G>(P, tb, L, gl);
+    int n_anti = 0;
+    int cur = 0;
+    bool carried = false;
+    if constexpr (FAST) {
+        if (cy.on) {
+            cur = cy.state[1];
+            carried = cy.state[0] != 0 && cy.state[2] == G;
+        }
+    }
+    if (carried) {
+        // the parent's weights and anti-parallel count from the last step
+        const int n = P.Kg[gidx<G>()] * G;
+        const int64_t par = cy.parents[wk];
+        const T *src = (const T *)cy.w[cur] + par * cy.row;
+#pragma unroll 8
+        for (int q = gl; q < n; q += G) L.w[q] = src[q];
+        n_anti = cy.anti[cur][par];
+        wave_sync();
+    } else {
+        wave_sync();
+        n_anti = full_eval<T, FAST, G>(P, tb, L, gl);
+    }
     wave_sync();
     double diag = diag_value<G>(P, L.x, L.anti, n_anti, gl);
     double lw = *lw_p;
@@ -815,10 +854,29 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
         lw = lw - alw * (double)n_bra;
     }
     store_spins<G>(P, sp, L.x, gl);
+    if constexpr (FAST) {
+        if (cy.on) {
+            wave_sync();
+            const int n = P.Kg[gidx<G>()] * G;
+            T *dst = (T *)cy.w[cur ^ 1] + (int64_t)wk * cy.row;
+#pragma unroll 8
+            for (int q = gl; q < n; q += G) dst[q] = L.w[q];
+            if (gl == 0) cy.anti[cur ^ 1][wk] = n_anti;
+        }
+    }
     if (gl == 0) {
         *lw_p = lw;
         *el_p = eL;
         if (ev_p) *ev_p += (int32_t)it;
+    }
+}
+
+// after a propagation that stored its rows: they are the latest, for G lanes
+__global__ void k_carry_commit(int32_t *state, int G) {
+    if (threadIdx.x == 0) {
+        state[0] = 1;
+        state[1] ^= 1;
+        state[2] = G;
     }
 }
 
@@ -829,7 +887,8 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
                                                     double *e_loc, double beta, double alw,
                                                     uint32_t key0, uint32_t key1,
                                                     uint32_t step, uint32_t tag,
-                                                    int32_t *evals, const uint32_t *step_base) {
+                                                    int32_t *evals, const uint32_t *step_base,
+                                                    CarryDev cy) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int grp = threadIdx.x / G;
     const int64_t wk = (int64_t)blockIdx.x * (WAVE / G) + grp;
@@ -839,7 +898,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
     propagate_walker<T, FAST, G, METHOD>(P, tb, base, threadIdx.x, (uint32_t)wk,
                                          spins + wk * P.words,
                                  log_w + wk, e_loc + wk, evals ? evals + wk : nullptr, beta,
-                                 alw, key0, key1, st, tag);
+                                 alw, key0, key1, st, tag, cy);
 }
 
 // A batch of independent lattices in one launch: walker g belongs to lattice
@@ -868,10 +927,11 @@ __global__ __launch_bounds__(WAVE) void k_propagate_batch(
     const uint64_t key = seed + (uint64_t)lo;
     const DevProblem &P = Ps[lo];
     unsigned char *base = G == WAVE ? lds : lds + (size_t)grp * walker_lds_bytes<T, G>(P);
+    const CarryDev no_carry{};
     propagate_walker<T, FAST, G, METHOD>(P, Ts[lo], base, threadIdx.x, (uint32_t)(g - woff[lo]),
                                  spins + g * stride, log_w + g, e_loc + g,
                                  evals ? evals + g : nullptr, beta, alw[lo], (uint32_t)key,
-                                 (uint32_t)(key >> 32), step, tag);
+                                 (uint32_t)(key >> 32), step, tag, no_carry);
 }
 
 // ---------------------------------------------------------------------------
@@ -1863,18 +1923,21 @@ static void launch_prop_g(const gfmc_problem *p, const Tables<T> &tb, unsigned g
                           int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
                           double beta, double alw, uint32_t k0, uint32_t k1, uint32_t step,
                           uint32_t tag, int32_t *evals, hipStream_t st) {
+    CarryDev cy{};
+    if (FAST && g_carry && g_carry->on && !p->dp.flags) cy = *g_carry;
     if (p->dp.method == GFMC_METHOD_SINGLE)
         hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_SINGLE>), dim3(grid), dim3(WAVE),
                            lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
-                           tag, evals, g_step_base);
+                           tag, evals, g_step_base, cy);
     else if (p->dp.method == GFMC_METHOD_MULTI)
         hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_MULTI>), dim3(grid), dim3(WAVE),
                            lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
-                           tag, evals, g_step_base);
+                           tag, evals, g_step_base, cy);
     else
         hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_CONT>), dim3(grid), dim3(WAVE),
                            lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
-                           tag, evals, g_step_base);
+                           tag, evals, g_step_base, cy);
+    if (cy.on) hipLaunchKernelGGL(k_carry_commit, dim3(1), dim3(WAVE), 0, st, cy.state, G);
 }
 
 template <typename T>
@@ -2489,6 +2552,45 @@ int gfmc_gather(int32_t n_sites, int64_t nw, const uint32_t *spins_in, const dou
     return 0;
 }
 
+// elements of one carried row: the largest weight array over the usable
+// lane counts (0 when the problem's path carries nothing)
+static int64_t carry_row(const gfmc_problem *p) {
+    if (!p->dp.fast || p->dp.flags) return 0;
+    int64_t row = 0;
+    for (int gi = 0; gi < 3; ++gi) row = std::max(row, (int64_t)p->dp.Kg[gi] * (WAVE >> gi));
+    return row;
+}
+
+static size_t carry_align(size_t b) { return (b + 255) / 256 * 256; }
+
+static int64_t carry_bytes_of(const gfmc_problem *p, int64_t nw) {
+    const int64_t row = carry_row(p);
+    if (row == 0 || nw <= 0) return 0;
+    const size_t el = p->precision == GFMC_FP64 ? 8 : 4;
+    return (int64_t)(256 + 2 * carry_align((size_t)nw * row * el) +
+                     2 * carry_align((size_t)nw * 4));
+}
+
+static CarryDev carry_dev(const gfmc_problem *p, int64_t nw, void *carry,
+                          const int32_t *parents) {
+    CarryDev c{};
+    const int64_t row = carry_row(p);
+    if (row == 0) return c;
+    const size_t el = p->precision == GFMC_FP64 ? 8 : 4;
+    unsigned char *b = (unsigned char *)carry;
+    c.state = (int32_t *)b;
+    size_t o = 256;
+    const size_t wb = carry_align((size_t)nw * row * el), ab = carry_align((size_t)nw * 4);
+    c.w[0] = b + o;
+    c.w[1] = b + o + wb;
+    c.anti[0] = (int32_t *)(b + o + 2 * wb);
+    c.anti[1] = (int32_t *)(b + o + 2 * wb + ab);
+    c.parents = parents;
+    c.row = row;
+    c.on = 1;
+    return c;
+}
+
 // the step loop, with the diagonal forward-walking record after each
 // resampling when dg is non-null (gfmc_diag.inl)
 static int run_steps_impl(const char *fn, const gfmc_problem *p, int64_t nw, uint32_t *spins,
@@ -2497,7 +2599,7 @@ static int run_steps_impl(const char *fn, const gfmc_problem *p, int64_t nw, uin
                           double alw, uint64_t seed, uint32_t step0, uint32_t tag_prop,
                           uint32_t tag_reconf, double *series, int32_t *evals,
                           const gfmc_diag *dg, void *ring, double *dseries, double *dwork,
-                          void *stream) {
+                          void *stream, void *carry = nullptr) {
     int rc = check_walkers(fn, p, nw);
     if (rc) return rc;
     if (nw == 0 || n_steps < 0) return fail(GFMC_EINVAL, std::string(fn) + ": bad size");
@@ -2513,9 +2615,13 @@ static int run_steps_impl(const char *fn, const gfmc_problem *p, int64_t nw, uin
     const int words = p->dp.words;
     uint32_t *sa = spins, *sb = spins_b;
     double *ea = e_loc, *eb = e_loc_b;
+    CarryDev cy{};
+    if (carry) cy = carry_dev(p, nw, carry, parents);
     for (int32_t t = 0; t < n_steps; ++t) {
+        g_carry = cy.on ? &cy : nullptr;
         rc = gfmc_propagate(p, nw, sa, log_w, ea, beta, alw, seed, step0 + (uint32_t)t,
                             tag_prop, evals, stream);
+        g_carry = nullptr;
         if (rc) return rc;
         rc = gfmc_reconfigure(nw, log_w, ea, seed, step0 + (uint32_t)t, tag_reconf, nullptr,
                               parents, series + 2 * (int64_t)t, work, stream);
@@ -2552,6 +2658,41 @@ int gfmc_run_steps(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
     return run_steps_impl("gfmc_run_steps", p, nw, spins, log_w, e_loc, spins_b, e_loc_b,
                           parents, work, n_steps, beta, alw, seed, step0, tag_prop, tag_reconf,
                           series, evals, nullptr, nullptr, nullptr, nullptr, stream);
+}
+
+int64_t gfmc_carry_bytes(const gfmc_problem *p, int64_t n_walkers) {
+    if (!p) return 0;
+    return carry_bytes_of(p, n_walkers);
+}
+
+int gfmc_propagate_carry(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                         double *e_loc, double beta, double alw, uint64_t seed, uint32_t step,
+                         uint32_t tag, int32_t *evals, const int32_t *parents, void *carry,
+                         void *stream) {
+    if (carry && ((uintptr_t)carry & 255))
+        return fail(GFMC_EINVAL, "gfmc_propagate_carry: carry must be 256-byte aligned");
+    if (carry && !parents) return fail(GFMC_EINVAL, "gfmc_propagate_carry: NULL parents");
+    int rc = check_walkers("gfmc_propagate_carry", p, nw);
+    if (rc) return rc;
+    CarryDev cy{};
+    if (carry && nw > 0) cy = carry_dev(p, nw, carry, parents);
+    g_carry = cy.on ? &cy : nullptr;
+    rc = gfmc_propagate(p, nw, spins, log_w, e_loc, beta, alw, seed, step, tag, evals, stream);
+    g_carry = nullptr;
+    return rc;
+}
+
+int gfmc_run_steps_carry(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                         double *e_loc, uint32_t *spins_b, double *e_loc_b, int32_t *parents,
+                         double *work, int32_t n_steps, double beta, double alw, uint64_t seed,
+                         uint32_t step0, uint32_t tag_prop, uint32_t tag_reconf, double *series,
+                         int32_t *evals, const gfmc_diag *d, void *ring, double *diag_series,
+                         double *diag_work, void *carry, void *stream) {
+    if (carry && ((uintptr_t)carry & 255))
+        return fail(GFMC_EINVAL, "gfmc_run_steps_carry: carry must be 256-byte aligned");
+    return run_steps_impl("gfmc_run_steps_carry", p, nw, spins, log_w, e_loc, spins_b, e_loc_b,
+                          parents, work, n_steps, beta, alw, seed, step0, tag_prop, tag_reconf,
+                          series, evals, d, ring, diag_series, diag_work, stream, carry);
 }
 
 int gfmc_sample_index(int32_t precision, int32_t n, const double *weights, int32_t n_draws,

@@ -111,6 +111,20 @@ class Chain:
         self.parents = torch.empty(self.nw, dtype=torch.int32, device=dev)
         self.work = torch.empty(_lib.work_doubles(self.nw), dtype=torch.float64, device=dev)
         self.evals = torch.zeros(self.nw, dtype=torch.int32, device=dev)
+        # carried slot weights of the record path (gfmc_carry_bytes; none for
+        # the other paths): zero = nothing carried yet
+        nb = int(_lib.load().gfmc_carry_bytes(problem.handle, self.nw))
+        self._carry = torch.zeros(nb + 256, dtype=torch.uint8, device=dev) if nb else None
+        self.carry = None
+        if nb:
+            off = (-self._carry.data_ptr()) % 256
+            self.carry = self._carry[off:off + nb]
+
+    def drop_carry(self):
+        """Forget the carried weights (the walkers changed outside the step
+        loops); the next step re-evaluates every slot."""
+        if self.carry is not None:
+            self.carry[:256].zero_()
 
     def steps(self, n, beta, alw, seed, step0, warmup=False, series=None, evals=False,
               diag=None, dseries=None):
@@ -127,27 +141,28 @@ class Chain:
                   _lib.ptr(self.parents), _lib.ptr(self.work), int(n), float(beta), float(alw),
                   int(seed) & 0xFFFFFFFFFFFFFFFF, int(step0) & 0xFFFFFFFF, tp, tr,
                   _lib.ptr(series), _lib.ptr(self.evals) if evals else None)
-        if diag is None:
-            _lib.check(_lib.load().gfmc_run_steps(*common, _lib.stream()), "gfmc_run_steps")
-        else:
-            if dseries is None:
-                dseries = diag.series(n, self.spins.device)
-            _lib.check(_lib.load().gfmc_run_steps_diag(*common, diag.handle,
-                                                       _lib.ptr(diag.ring), _lib.ptr(dseries),
-                                                       _lib.ptr(diag.work), _lib.stream()),
-                       "gfmc_run_steps_diag")
+        if diag is not None and dseries is None:
+            dseries = diag.series(n, self.spins.device)
+        dargs = ((diag.handle, _lib.ptr(diag.ring), _lib.ptr(dseries), _lib.ptr(diag.work))
+                 if diag is not None else (None, None, None, None))
+        _lib.check(_lib.load().gfmc_run_steps_carry(*common, *dargs, _lib.ptr(self.carry),
+                                                    _lib.stream()), "gfmc_run_steps_carry")
         return series
 
     def step(self, beta, alw, seed, step, out2, events=None, evals=True):
-        """One production step as three separate launches (propagate,
-        reconfigure, gather) on torch's current stream; ``events`` (start,
-        end) bracket the propagation kernel.  ``out2``: 2-double device row."""
+        """One production step as three separate launches (propagate with
+        the chain's carried weights, reconfigure, gather) on torch's current
+        stream; ``events`` (start, end) bracket the propagation kernel.
+        ``out2``: 2-double device row."""
         lib = _lib.load()
         st = _lib.stream()
         if events is not None:
             events[0].record()
-        self.p.propagate(self.spins, self.log_w, self.e_loc, beta, alw, seed, step,
-                         _lib.TAG_PROPAGATE, self.evals if evals else None)
+        _lib.check(lib.gfmc_propagate_carry(
+            self.p.handle, self.nw, _lib.ptr(self.spins), _lib.ptr(self.log_w),
+            _lib.ptr(self.e_loc), float(beta), float(alw), int(seed) & 0xFFFFFFFFFFFFFFFF,
+            int(step) & 0xFFFFFFFF, _lib.TAG_PROPAGATE, _lib.ptr(self.evals) if evals else None,
+            _lib.ptr(self.parents), _lib.ptr(self.carry), st), "gfmc_propagate_carry")
         if events is not None:
             events[1].record()
         _lib.check(lib.gfmc_reconfigure(self.nw, _lib.ptr(self.log_w), _lib.ptr(self.e_loc),
@@ -196,17 +211,17 @@ class StepGraph:
             rc = lib.gfmc_graph_capture_batch_steps(chain.b.handle, *common, _lib.ptr(alw), seed,
                                                     tp, tr, _lib.ptr(self.series), ev,
                                                     _lib.ptr(self.counter), ctypes.byref(h))
-        elif diag is not None:
-            self.dseries = diag.series(self.n, dev)
-            rc = lib.gfmc_graph_capture_steps_diag(chain.p.handle, chain.nw, *common, float(alw),
-                                                   seed, tp, tr, _lib.ptr(self.series), ev,
-                                                   diag.handle, _lib.ptr(diag.ring),
-                                                   _lib.ptr(self.dseries), _lib.ptr(diag.work),
-                                                   _lib.ptr(self.counter), ctypes.byref(h))
         else:
-            rc = lib.gfmc_graph_capture_steps(chain.p.handle, chain.nw, *common, float(alw),
-                                              seed, tp, tr, _lib.ptr(self.series), ev,
-                                              _lib.ptr(self.counter), ctypes.byref(h))
+            dargs = (None, None, None, None)
+            if diag is not None:
+                self.dseries = diag.series(self.n, dev)
+                dargs = (diag.handle, _lib.ptr(diag.ring), _lib.ptr(self.dseries),
+                         _lib.ptr(diag.work))
+            rc = lib.gfmc_graph_capture_steps_carry(chain.p.handle, chain.nw, *common,
+                                                    float(alw), seed, tp, tr,
+                                                    _lib.ptr(self.series), ev, *dargs,
+                                                    _lib.ptr(chain.carry),
+                                                    _lib.ptr(self.counter), ctypes.byref(h))
         _lib.check(rc, "gfmc_graph_capture")
         self.handle = h
         self.chain = chain
