@@ -28,7 +28,11 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps after the workload's own GFMC warm-up")
+    ap.add_argument("--gfmc-warmup", type=int, default=1000,
+                    help="the workload's GFMC warm-up ramp (driver.py DEFAULT_WARMUP_STEPS), "
+                         "always run before any timing")
     ap.add_argument("--delta", type=float, default=1.0)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--walkers", type=int, default=65536)
@@ -100,6 +104,11 @@ def summarize_profile(db, prefix="k_propagate<float, true", tail=None):
     return "\n".join(lines)
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.summarize_profile:
@@ -131,8 +140,6 @@ def main():
     p = get_problem(H, G)
     w = prepare_walkers_sz(nw, N, 12345)
     chain = Chain(p, w)
-    _, e, _ = p.rates(chain.spins, want_rates=False)
-    alw = float(torch.mean(-e).item())
     seed = 777
     series = torch.empty((max(args.steps, args.warmup, 1), 2), dtype=torch.float64,
                          device=chain.spins.device)
@@ -140,15 +147,20 @@ def main():
     def one_step(t, row, ev_pair=None):
         chain.step(beta, alw, seed, t, series[row], events=ev_pair)
 
-    # warm-up: equilibrate and settle the shift (untimed).  From random
-    # Sz = 0 configurations the rate evaluations per walker-step fall from
-    # ~46 to a steady ~17.5 within ~200 steps; the default 300 steps put the
-    # timed steps in that steady state.
+    # the workload's own GFMC warm-up (untimed, whatever --warmup says): the
+    # four-stage step ramp of run_gfmc, which also measures the shift.  From
+    # random Sz = 0 configurations the rate evaluations per walker-step fall
+    # from ~46 to a steady ~17.5 within ~200 steps; the ramp's 1000 steps
+    # put every timed step in that steady state.
+    from pygfmc_amd.engine import _warmup
+    from pygfmc_amd.walkers import PhiloxKey
+    p.set_lanes(64)
+    alw = _warmup(chain, PhiloxKey(seed, 0), beta, None, args.gfmc_warmup, 0)
+    pos0 = 1 << 20             # production positions, past the warm-up's
+    # then the harness's untimed steps (shift held)
     for t in range(args.warmup):
-        one_step(t, t)
-        if t % 5 == 4:
-            lg = series[t - 4: t + 1, 0].cpu().numpy()
-            alw += float(np.mean(lg)) / beta
+        one_step(pos0 + t, t)
+    pos0 += args.warmup
     torch.cuda.synchronize()
     # lanes per walker of the propagation kernel: the autotuner, or pinned
     if args.lanes == "auto":
@@ -173,7 +185,7 @@ def main():
               for _ in range(n_k)]
     chain.evals.zero_()
     for k in range(n_k):
-        one_step(args.warmup + k, k, events[k])
+        one_step(pos0 + k, k, events[k])
     torch.cuda.synchronize()
     prop_ms = [a.elapsed_time(b) for a, b in events]
     prop_avg_s = float(np.mean(prop_ms)) / 1e3
@@ -186,7 +198,7 @@ def main():
     n_rep = max(1, args.steps // chunk)
     steps_timed = n_rep * chunk
     graph = StepGraph(chain, chunk, beta, alw, seed, evals=True)
-    pos = args.warmup + n_k
+    pos = pos0 + n_k
     graph.launch(pos)          # untimed: first replay
     pos += chunk
     chain.evals.zero_()
@@ -255,14 +267,20 @@ def main():
                      "flop_per_eval": flop_per_eval},
         "cpu_baseline": cpu,
     }
+    progress(f"headline done: {value:.4g} walker-steps/s")
     if not args.no_configs:
         out["heisenberg_chain100"] = chain100_section()
+        progress("chain100 done")
         out["batched_tfim_chains"] = tfim_batch_section()
+        progress("tfim batch done")
         out["published_throughput_config"] = fig3_section()
+        progress("published configuration done")
     if not args.no_mlp:
         out["kagome12_mlp"] = kagome_mlp_section(args)
+        progress("kagome mlp done")
     if not args.no_race:
         out["sigma_race"] = sigma_race(args)
+        progress("sigma race done")
     print(json.dumps(out), flush=True)
 
 

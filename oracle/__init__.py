@@ -81,6 +81,11 @@ def lib():
                                     ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
                                     ctypes.c_uint32, _i32p, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_double]
+        L.orc_propagate_ids.restype = ctypes.c_int
+        L.orc_propagate_ids.argtypes = [D, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _u32p,
+                                        _f64p, _f64p, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _i32p,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.orc_reconfigure.restype = ctypes.c_int
         L.orc_reconfigure.argtypes = [ctypes.c_int64, _f64p, _f64p, ctypes.c_double, _i32p,
                                       _f64p, _f64p]
@@ -239,6 +244,29 @@ class Problem:
                                 float(beta), float(alw), int(seed), int(step) & 0xFFFFFFFF,
                                 int(tag), _p(ev, ctypes.c_int32), int(n_threads), int(method),
                                 float(lam))
+        finally:
+            lib().orc_set_lanes(64)
+        return spins, log_w, e_loc, ev
+
+    def propagate_ids(self, ids, spins, log_w, e_loc, beta, alw, seed, step, tag, n_threads=0,
+                      lanes=64, method=0, lam=0.0):
+        """:meth:`propagate` of a sample of a larger population: row i is
+        walker ``ids[i]`` (its Philox counter index)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        spins = np.array(spins, dtype=np.uint32, copy=True)
+        log_w = np.array(log_w, dtype=np.float64, copy=True)
+        e_loc = np.array(e_loc, dtype=np.float64, copy=True)
+        nw = spins.shape[0]
+        assert ids.shape == (nw,)
+        ev = np.zeros(nw, dtype=np.int32)
+        lib().orc_set_lanes(int(lanes))
+        try:
+            lib().orc_propagate_ids(ctypes.byref(self.desc), nw, _p(ids, ctypes.c_int64),
+                                    _p(spins, ctypes.c_uint32), _p(log_w, ctypes.c_double),
+                                    _p(e_loc, ctypes.c_double), float(beta), float(alw),
+                                    int(seed), int(step) & 0xFFFFFFFF, int(tag),
+                                    _p(ev, ctypes.c_int32), int(n_threads), int(method),
+                                    float(lam))
         finally:
             lib().orc_set_lanes(64)
         return spins, log_w, e_loc, ev

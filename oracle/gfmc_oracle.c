@@ -337,31 +337,82 @@ float orc_bf16_flush(float v) {
     return fabsf(r) < 5.9604644775390625e-08f ? 0.0f : r;
 }
 
+/* The BF16-grid weights of the MLP (W1, b1, W2 after orc_bf16_flush), made
+ * once per configuration evaluation so the rows of its candidates do not
+ * re-round them; the arithmetic is the same as rounding on the fly. */
+typedef struct {
+    float *W1, *b1, *W2;
+} mlp_grid_t;
+
+static size_t mlp_grid_floats(const gfmc_problem_desc *d) {
+    return (size_t)d->mlp_h1 * (size_t)d->n_sites + (size_t)d->mlp_h1 +
+           (size_t)d->mlp_h2 * (size_t)d->mlp_h1;
+}
+
+static mlp_grid_t mlp_grid_fill(const gfmc_problem_desc *d, float *buf) {
+    mlp_grid_t g;
+    size_t n1 = (size_t)d->mlp_h1 * (size_t)d->n_sites, n2 = (size_t)d->mlp_h2 * d->mlp_h1;
+    g.W1 = buf;
+    g.b1 = buf + n1;
+    g.W2 = buf + n1 + d->mlp_h1;
+    for (size_t q = 0; q < n1; ++q) g.W1[q] = orc_bf16_flush(d->mlp_W1[q]);
+    for (int k = 0; k < d->mlp_h1; ++k) g.b1[k] = orc_bf16_flush(d->mlp_b1[k]);
+    for (size_t q = 0; q < n2; ++q) g.W2[q] = orc_bf16_flush(d->mlp_W2[q]);
+    return g;
+}
+
 /* layer-1 activations h (BF16 values) of x */
-void orc_mlp_hidden(const gfmc_problem_desc *d, const int8_t *x, float *h) {
-    int N = d->n_sites, H1 = d->mlp_h1;
-    for (int k = 0; k < H1; ++k) {
-        double z = (double)orc_bf16_flush(d->mlp_b1[k]);
-        for (int n = 0; n < N; ++n)
-            z += (double)x[n] * (double)orc_bf16_flush(d->mlp_W1[(size_t)k * N + n]);
+static void mlp_hidden_g(const gfmc_problem_desc *d, const mlp_grid_t *g, const int8_t *x,
+                         float *h) {
+    int N = d->n_sites, H1 = d->mlp_h1, k = 0;
+    /* four hidden units at a time: four independent sums, each in site order */
+    for (; k + 4 <= H1; k += 4) {
+        const float *w0 = g->W1 + (size_t)k * N, *w1 = w0 + N, *w2 = w1 + N, *w3 = w2 + N;
+        double z0 = (double)g->b1[k], z1 = (double)g->b1[k + 1];
+        double z2 = (double)g->b1[k + 2], z3 = (double)g->b1[k + 3];
+        for (int n = 0; n < N; ++n) {
+            const double xn = (double)x[n];
+            z0 += xn * (double)w0[n];
+            z1 += xn * (double)w1[n];
+            z2 += xn * (double)w2[n];
+            z3 += xn * (double)w3[n];
+        }
+        h[k] = bf16_rne(orc_mlp_tanh((float)z0));
+        h[k + 1] = bf16_rne(orc_mlp_tanh((float)z1));
+        h[k + 2] = bf16_rne(orc_mlp_tanh((float)z2));
+        h[k + 3] = bf16_rne(orc_mlp_tanh((float)z3));
+    }
+    for (; k < H1; ++k) {
+        double z = (double)g->b1[k];
+        for (int n = 0; n < N; ++n) z += (double)x[n] * (double)g->W1[(size_t)k * N + n];
         h[k] = bf16_rne(orc_mlp_tanh((float)z));
     }
 }
 
-float orc_mlp_log_psi(const gfmc_problem_desc *d, const int8_t *x) {
+static float mlp_log_psi_g(const gfmc_problem_desc *d, const mlp_grid_t *g, const int8_t *x) {
     int H1 = d->mlp_h1, H2 = d->mlp_h2, NC = H2 / 16;
     float *h = (float *)malloc(sizeof(float) * (size_t)H1);
-    orc_mlp_hidden(d, x, h);
+    mlp_hidden_g(d, g, x, h);
     float s[16];
     for (int l = 0; l < 16; ++l) s[l] = 0.0f;
     for (int t = 0; t < NC; ++t)
-        for (int l = 0; l < 16; ++l) {
-            int c = t * 16 + l;
-            double z2 = 0.0;
-            for (int k = 0; k < H1; ++k)
-                z2 += (double)orc_bf16_flush(d->mlp_W2[(size_t)c * H1 + k]) * (double)h[k];
-            float v = d->mlp_w3[c] * orc_mlp_tanh((float)z2 + d->mlp_b2[c]);
-            s[l] = s[l] + v;
+        for (int l = 0; l < 16; l += 4) {
+            /* four columns at a time: four independent sums, each in k order */
+            const int c = t * 16 + l;
+            const float *w0 = g->W2 + (size_t)c * H1, *w1 = w0 + H1, *w2 = w1 + H1,
+                        *w3 = w2 + H1;
+            double z[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < H1; ++k) {
+                const double hk = (double)h[k];
+                z[0] += (double)w0[k] * hk;
+                z[1] += (double)w1[k] * hk;
+                z[2] += (double)w2[k] * hk;
+                z[3] += (double)w3[k] * hk;
+            }
+            for (int q = 0; q < 4; ++q) {
+                float v = d->mlp_w3[c + q] * orc_mlp_tanh((float)z[q] + d->mlp_b2[c + q]);
+                s[l + q] = s[l + q] + v;
+            }
         }
     for (int dd = 8; dd >= 1; dd >>= 1) {
         float o[16];
@@ -372,6 +423,33 @@ float orc_mlp_log_psi(const gfmc_problem_desc *d, const int8_t *x) {
     return s[0] + d->mlp_b3[0];
 }
 
+void orc_mlp_hidden(const gfmc_problem_desc *d, const int8_t *x, float *h) {
+    float *buf = (float *)malloc(sizeof(float) * mlp_grid_floats(d));
+    mlp_grid_t g = mlp_grid_fill(d, buf);
+    mlp_hidden_g(d, &g, x, h);
+    free(buf);
+}
+
+float orc_mlp_log_psi(const gfmc_problem_desc *d, const int8_t *x) {
+    float *buf = (float *)malloc(sizeof(float) * mlp_grid_floats(d));
+    mlp_grid_t g = mlp_grid_fill(d, buf);
+    float r = mlp_log_psi_g(d, &g, x);
+    free(buf);
+    return r;
+}
+
+/* theta of the MLP guide: theta[0] = log psi(x), followed by the grid
+ * weights (floats) its candidate rows use */
+static mlp_grid_t mlp_grid_of_theta(const gfmc_problem_desc *d, const double *theta) {
+    float *buf = (float *)(void *)(theta + 1);
+    mlp_grid_t g;
+    size_t n1 = (size_t)d->mlp_h1 * (size_t)d->n_sites;
+    g.W1 = buf;
+    g.b1 = buf + n1;
+    g.W2 = buf + n1 + d->mlp_h1;
+    return g;
+}
+
 static double mlp_log_ratio(const gfmc_problem_desc *d, const int8_t *x, const double *lp0,
                             int i, int j) {
     int N = d->n_sites;
@@ -379,7 +457,8 @@ static double mlp_log_ratio(const gfmc_problem_desc *d, const int8_t *x, const d
     memcpy(y, x, (size_t)N);
     y[i] = (int8_t)-y[i];
     if (j >= 0) y[j] = (int8_t)-y[j];
-    float out = orc_mlp_log_psi(d, y);
+    const mlp_grid_t g = mlp_grid_of_theta(d, lp0);
+    float out = mlp_log_psi_g(d, &g, y);
     free(y);
     return (double)(out - (float)lp0[0]);
 }
@@ -435,8 +514,9 @@ static double slot_rate(const gfmc_problem_desc *d, const int8_t *x, int s, doub
 
 static double *theta_of(const gfmc_problem_desc *d, const int8_t *x) {
     if (d->guide == GFMC_GUIDE_MLP) {
-        double *t = (double *)malloc(sizeof(double));
-        t[0] = (double)orc_mlp_log_psi(d, x);
+        double *t = (double *)malloc(sizeof(double) + sizeof(float) * mlp_grid_floats(d));
+        mlp_grid_t g = mlp_grid_fill(d, (float *)(void *)(t + 1));
+        t[0] = (double)mlp_log_psi_g(d, &g, x);
         return t;
     }
     if (d->guide != GFMC_GUIDE_RBM) return NULL;
@@ -604,6 +684,9 @@ static double weights_of(const gfmc_problem_desc *d, const int8_t *x, double *w)
  *    stay with weight w0 = lam - H_xx, hop to slot s with weight w_s; the
  *    draw u = min(U (w0 + R), nextafter(w0 + R, 0)) stays when u < w0, else
  *    u - w0 is drawn from the blocked CDF of the slots; log_w += log(w0 + R);
+ *    a total w0 + R <= 0 stays (the reference's sample_index returns slot
+ *    M - 1 then, which is the stay when there are no off-diagonal slots;
+ *    the walker's weight is zero either way);
  * 2: the same with geometric holds (evolve_walker_multi_step, :427-556):
  *    k = min(floor(log1p(-psi) / log1p(-R / b)) + 1, remaining + 1), the hop
  *    applied when k <= remaining, log_w += log(b) k, b = lam - H_xx + R.
@@ -653,7 +736,9 @@ static void propagate_one(const gfmc_problem_desc *d, int64_t widx, uint32_t *sp
                 double u = U1 * W, cap = nextafter(W, 0.0);
                 if (u > cap) u = cap;
                 int idx = -1;
-                if (!(u < w0)) idx = orc_sample_blocked(w, n_off, incl, u - w0);
+                /* a zero (or negative) total keeps the walker (it dies through
+                 * log(W)): no slot has weight to draw */
+                if (W > 0.0 && !(u < w0)) idx = orc_sample_blocked(w, n_off, incl, u - w0);
                 lw = lw + log(w0 + R);
                 ++it;
                 if (idx >= 0) flip_slot(d, x, idx);
@@ -716,6 +801,34 @@ int orc_propagate(const gfmc_problem_desc *d, int64_t nw, uint32_t *spins, doubl
         for (int64_t i = 0; i < nw; ++i)
             propagate_one(d, i, spins + i * W, log_w + i, e_loc + i, beta, alw, seed, step, tag,
                           evals ? evals + i : NULL, x, w, method, lam);
+        free(x);
+        free(w);
+    }
+    return 0;
+}
+
+/* The same for a sample of a larger population: walker i of the buffers is
+ * walker ids[i] of the population (its Philox counter index), so a few
+ * walkers of a full-size GPU launch can be checked without propagating the
+ * rest. */
+int orc_propagate_ids(const gfmc_problem_desc *d, int64_t nw, const int64_t *ids,
+                      uint32_t *spins, double *log_w, double *e_loc, double beta, double alw,
+                      uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals, int n_threads,
+                      int method, double lam) {
+    int N = d->n_sites, W = n_words_of(N), n_off = n_off_of(d);
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+#pragma omp parallel
+    {
+        int8_t *x = (int8_t *)malloc((size_t)N);
+        double *w = (double *)malloc(sizeof(double) * (size_t)(n_off > 0 ? n_off : 1));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t i = 0; i < nw; ++i)
+            propagate_one(d, ids[i], spins + i * W, log_w + i, e_loc + i, beta, alw, seed, step,
+                          tag, evals ? evals + i : NULL, x, w, method, lam);
         free(x);
         free(w);
     }

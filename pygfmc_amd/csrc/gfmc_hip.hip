@@ -802,7 +802,9 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                 if (u > cap) u = cap;
                 int idx = -1;
                 MovePrefetch m;
-                if (!(u < w0)) {
+                // a total W <= 0 stays (no slot has weight; the walker dies
+                // through log W), also when there are no off-diagonal slots
+                if (W > (T)0 && !(u < w0)) {
                     idx = sample_slot<T, G>(P, L, incl, lane_sum, u - w0, lane);
                     if constexpr (FAST) m = prefetch_move(P, idx, gl);
                 }

@@ -375,7 +375,7 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
              off_diag_operators=None, return_raw=True, avg_log_weight=None,
              warmup_steps=None, verbosity=7, number_of_outputs=100, off_diag_stride=1,
              off_diag_batch_size=None, precision=None, use_graph=True, lanes="auto",
-             **_unused):
+             _step_offset=0):
     """Single-GPU GFMC run (run_gfmc, driver.py:1148-1547).
 
     Returns ``(walkers, energy, obs, stats, raw)`` (or the first four with
@@ -403,8 +403,15 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     captured HIP graph (identical results to eager launches).  ``lanes``:
     lanes per walker of the propagation kernel (64, 32, 16) or "auto" for
     the autotuner (pygfmc_amd.autotune); pin it for bit-reproducible runs.
+    ``_step_offset`` (the reference's continuation offset) must be 0: a
+    continuation resumes from the position in ``rng_key`` instead.  As in
+    the reference there is no catch-all: an unknown keyword is a
+    ``TypeError``.
     """
     wall_start = time.time()
+    if _step_offset != 0:
+        raise ValueError("_step_offset is not used: continue a chain by passing "
+                         "stats['parameters/rng_key_next'] as rng_key")
     diag_ops = dict(diag_operators) if diag_operators else {}
     off_ops = dict(off_diag_operators) if off_diag_operators else {}
     if rng_key is None:
@@ -618,8 +625,15 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
 
 
 def run(walkers, n_steps, hamiltonian, step_size, guide, p_array, *, method="cont_time",
-        rng_key=None, lam=None, **kw):
-    """Entry point (multi_gpu.run, multi_gpu.py:822-868): one GPU, so it is
-    :func:`run_gfmc`."""
+        rng_key=None, lam=None, n_devices=None, **kw):
+    """Entry point (multi_gpu.run, multi_gpu.py:822-868): the keywords of
+    :func:`run_gfmc` plus ``n_devices`` (validated as the reference does,
+    an integer >= 1).  This engine runs one GPU: ``n_devices`` above 1
+    raises ``ValueError`` instead of sharding the walkers."""
+    if n_devices is not None:
+        n_devices = validated_int("n_devices", n_devices, 1)
+        if n_devices > 1:
+            raise ValueError(f"n_devices={n_devices}: this engine runs on one GPU; sharding "
+                             "walkers over several GPUs is not implemented")
     return run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, method=method,
                     rng_key=rng_key, lam=lam, **kw)

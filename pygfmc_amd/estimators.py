@@ -325,16 +325,17 @@ def check_population(log_G_chunk, step0, E_chunk=None):
         raise ValueError(
             f"[gfmc] population extinct at step {step0 + int(bad[0])}: the mean branch "
             "weight of the step is not finite, i.e. every walker's weight is zero (or one "
-            "overflowed).  The ensemble cannot be renormalised; check the step size and "
-            "the guide.")
+            "overflowed).  The ensemble cannot be renormalised; check lam against the "
+            "diagonal elements (discrete methods), the step size and the guide.")
     if E_chunk is not None:
         bad = np.flatnonzero(~np.isfinite(E_chunk))
         if bad.size:
             raise ValueError(
                 f"[gfmc] non-finite energy at step {step0 + int(bad[0])}: the weighted mean "
                 "of the local energies is not finite while the branch weights are, i.e. a "
-                "walker's local energy overflowed; check the guide and the Hamiltonian on "
-                "the configurations the walk reaches.")
+                "walker's local energy overflowed (a guide ratio or a rate of its "
+                "configuration is inf or NaN); check the guide and the Hamiltonian on the "
+                "configurations the walk reaches.")
 
 
 class DeviceEnergyEstimator:

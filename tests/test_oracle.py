@@ -577,3 +577,23 @@ def test_oracle_discrete_gfmc_energy_matches_ed(orc, method):
         warnings.simplefilter("ignore")
         E, _, err, _ = evaluate_observables(raw, n_blocks=20)
     assert abs(E[16] - e0) < 4 * err[16] + 1e-3
+
+
+def test_oracle_propagation_of_a_sample_equals_the_full_population(orc):
+    """propagate_ids on selected rows gives exactly those rows of the full
+    population's propagation (the walker index is the Philox counter)."""
+    from helpers import random_model
+    H, G = random_model(N=8, seed=3)
+    prob = orc.Problem(lower(H, G))
+    nw = 200
+    spins = orc.init_random(8, nw, 9)
+    lw0 = np.linspace(-0.2, 0.2, nw)
+    full = prob.propagate(spins, lw0, np.zeros(nw), 0.3, 0.1, 77, 5, 0)
+    ids = np.array([0, 3, 17, 150, 199])
+    part = prob.propagate_ids(ids, spins[ids], lw0[ids], np.zeros(len(ids)), 0.3, 0.1, 77, 5, 0)
+    for a, b in zip(part, full):
+        np.testing.assert_array_equal(a, b[ids])
+    # a different index is a different stream
+    other = prob.propagate_ids(ids + 1, spins[ids], lw0[ids], np.zeros(len(ids)), 0.3, 0.1, 77,
+                               5, 0)
+    assert not np.array_equal(other[1], part[1])
