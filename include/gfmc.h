@@ -228,6 +228,23 @@ int gfmc_reconfigure(int64_t n_walkers, const double *log_w, const double *e_loc
                      double *work, void *stream);
 int64_t gfmc_reconfigure_work_doubles(int64_t n_walkers);
 
+/* gfmc_reconfigure without a workspace argument, and without any device
+ * allocation: no counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, kernels and results as gfmc_reconfigure
+ * (bit-identical parents and out2); its scratch is a static device array of
+ * gfmc_reconfigure_work_doubles(GFMC_RECONFIGURE_STATIC_MAX_WALKERS) doubles
+ * (16.9 MB), one per GPU, held until the process ends.  n_walkers above
+ * GFMC_RECONFIGURE_STATIC_MAX_WALKERS is rejected with GFMC_EINVAL.  The
+ * stream must belong to the current device.  Every call on one GPU shares
+ * that GPU's array: calls that overlap in time on one GPU (different streams
+ * or host threads without ordering) race. */
+#define GFMC_RECONFIGURE_STATIC_MAX_WALKERS 2097152
+int gfmc_reconfigure_static(int64_t n_walkers, const double *log_w, const double *e_loc,
+                            uint64_t seed, uint32_t step, uint32_t tag,
+                            const double *psi_opt, int32_t *parents, double *out2,
+                            void *stream);
+
 /* The gather that follows reconfiguration: spins_out[i] = spins_in[parents[i]],
  * e_loc_out[i] = e_loc_in[parents[i]], log_w_out[i] = 0 (NULL skips). */
 int gfmc_gather(int32_t n_sites, int64_t n_walkers, const uint32_t *spins_in,

@@ -24,6 +24,8 @@ TAG_INIT = 2
 TAG_WARMUP_PROPAGATE = 3
 TAG_WARMUP_RECONFIG = 4
 TAG_OPERATOR = 16
+# largest population gfmc_reconfigure_static accepts (its static scratch)
+RECONFIGURE_STATIC_MAX_WALKERS = 2097152
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -103,6 +105,9 @@ SIGNATURES = [
                                         ctypes.c_uint32, ctypes.c_uint32, _f64p, _vp, _vp,
                                         _vp, _vp]),
     ("gfmc_reconfigure_work_doubles", ctypes.c_int64, [ctypes.c_int64]),
+    ("gfmc_reconfigure_static", ctypes.c_int, [ctypes.c_int64, _vp, _vp, ctypes.c_uint64,
+                                               ctypes.c_uint32, ctypes.c_uint32, _f64p, _vp,
+                                               _vp, _vp]),
     ("gfmc_gather", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp,
                                    _vp, _vp]),
     ("gfmc_run_steps", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2537,6 +2538,71 @@ int gfmc_reconfigure(int64_t nw, const double *log_w, const double *e_loc, uint6
 int64_t gfmc_reconfigure_work_doubles(int64_t n_walkers) {
     if (n_walkers <= 0) return 0;
     return n_walkers + 7 * ((n_walkers + RB - 1) / RB) + 1;
+}
+
+// ---------------------------------------------------------------------------
+// reconfiguration that allocates nothing: its scratch (the prefix sum and the
+// per-block partials of gfmc_reconfigure's workspace) is a file-scope device
+// array, one copy per GPU, resident until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in
+// symbol): the device address comes from hipGetSymbolAddress, once per device
+// with that device current, and is kept per device.
+// ---------------------------------------------------------------------------
+#define RC_STATIC_DOUBLES                                                      \
+    (GFMC_RECONFIGURE_STATIC_MAX_WALKERS +                                     \
+     7 * ((GFMC_RECONFIGURE_STATIC_MAX_WALKERS + RB - 1) / RB) + 1)
+static __device__ __align__(256) double scratch[RC_STATIC_DOUBLES];
+
+#define RC_STATIC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[RC_STATIC_MAX_DEVICES];
+
+static int rc_static_scratch(double **out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= RC_STATIC_MAX_DEVICES)
+        return fail(GFMC_EUNSUP, "gfmc_reconfigure_static: device ordinal beyond " +
+                                     std::to_string(RC_STATIC_MAX_DEVICES - 1));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(GFMC_EHIP, std::string("gfmc_reconfigure_static: hipGetSymbolAddress: ") +
+                                       hipGetErrorString(e));
+        g_scratch_dev[dev] = (double *)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+int gfmc_reconfigure_static(int64_t nw, const double *log_w, const double *e_loc,
+                            uint64_t seed, uint32_t step, uint32_t tag, const double *psi_opt,
+                            int32_t *parents, double *out2, void *stream) {
+    if (nw <= 0 || nw > GFMC_RECONFIGURE_STATIC_MAX_WALKERS)
+        return fail(GFMC_EINVAL, "gfmc_reconfigure_static: n_walkers must be in [1, " +
+                                     std::to_string(GFMC_RECONFIGURE_STATIC_MAX_WALKERS) +
+                                     "] (the size of its static scratch); use "
+                                     "gfmc_reconfigure with a workspace for more");
+    if (!log_w || !e_loc || !parents || !out2)
+        return fail(GFMC_EINVAL, "gfmc_reconfigure_static: NULL buffer");
+    double psi = psi_opt ? *psi_opt : 0.0;
+    if (psi_opt && !(psi >= 0.0 && psi < 1.0))
+        return fail(GFMC_EINVAL, "gfmc_reconfigure_static: psi must lie in [0, 1)");
+    double *work = nullptr;
+    int rc = rc_static_scratch(&work);
+    if (rc) return rc;
+    // every scratch entry the kernels read is written earlier in the same
+    // call (block partials, then the prefix sum), so nothing needs clearing
+    RcSegs S;
+    S.nseg = 0;
+    S.n = nw;
+    S.w = nullptr;
+    S.b = nullptr;
+    launch_reconfigure(S, nw, (unsigned)((nw + RB - 1) / RB), log_w, e_loc, seed, step, tag,
+                       psi_opt ? 1 : 0, psi, parents, out2, work, (hipStream_t)stream);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 int gfmc_gather(int32_t n_sites, int64_t nw, const uint32_t *spins_in, const double *e_loc_in,
