@@ -124,6 +124,12 @@ struct DevProblem {
     int H, w_words, rbm_wave_bytes;
     const uint32_t *rbm_W;   // [H][N] BF16, packed two per word (N even-padded)
     const float *rbm_a, *rbm_b, *rbm_cs;   // [N], [H], column sums sum_h W_hk [N]
+    // lanes-over-hidden-units kernels (gfmc_rbm.inl): rbh = their M = Hp / 64
+    // (0: the hidden layer is too wide for them), WT [N][Hp] BF16 bits,
+    // the per-launch walker queue
+    int rbh, Hp, wt_bytes, rbh_wave_bytes;
+    const uint16_t *rbm_WT;
+    int *rbm_queue;
     // MLP guide (layer-1 sums exact in FP64, layer 2 on MFMA); rbm_wave_bytes
     // is also the per-wavefront LDS of its walker kernels
     int mlp_h1, mlp_h2;
@@ -150,6 +156,7 @@ struct gfmc_problem {
     int precision;
     std::vector<void *> allocs;
     int64_t lds_bytes;
+    int64_t rbh_lds_bytes = 0;   // LDS per workgroup of the gfmc_rbm.inl kernels
     int n_cu;
     int lanes = WAVE;   // lanes per walker of the propagation kernel (64, 32, 16)
 };
@@ -1964,6 +1971,72 @@ static void launch_propagate(const gfmc_problem *p, const Tables<T> &tb, int64_t
                                    e_loc, beta, alw, k0, k1, step, tag, evals, st);
 }
 
+#include "gfmc_rbm.inl"
+
+// the gfmc_rbm.inl kernels for M = Hp / 64 hidden-unit chunks per lane
+template <int M>
+static int rbh_set_lds_m(int bytes) {
+    HIP_TRY(hipFuncSetAttribute((const void *)k_propagate_rbh<M>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_rates_rbh<M>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    return 0;
+}
+
+static int rbh_set_lds(int M, int bytes) {
+    switch (M) {
+    case 1: return rbh_set_lds_m<1>(bytes);
+    case 2: return rbh_set_lds_m<2>(bytes);
+    case 3: return rbh_set_lds_m<3>(bytes);
+    case 4: return rbh_set_lds_m<4>(bytes);
+    case 5: return rbh_set_lds_m<5>(bytes);
+    case 6: return rbh_set_lds_m<6>(bytes);
+    case 7: return rbh_set_lds_m<7>(bytes);
+    case 8: return rbh_set_lds_m<8>(bytes);
+    }
+    return fail(GFMC_EUNSUP, "RBM: hidden layer too wide for the lanes-over-units kernels");
+}
+
+// persistent grid of the gfmc_rbm.inl kernels: as many workgroups as fit
+static unsigned rbh_grid(const gfmc_problem *p, int64_t nw) {
+    const int64_t need = (nw + RBH_WAVES - 1) / RBH_WAVES;
+    const int64_t per_cu = (160 * 1024) / (p->rbh_lds_bytes > 0 ? p->rbh_lds_bytes : 1);
+    const int64_t cap = (int64_t)p->n_cu * (per_cu > 0 ? per_cu : 1);
+    return (unsigned)(need < cap ? need : cap);
+}
+
+template <int M>
+static void rbh_launch_m(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                         double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
+                         uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st) {
+    hipLaunchKernelGGL(k_rbh_queue_reset, dim3(1), dim3(WAVE), 0, st, p->dp.rbm_queue);
+    hipLaunchKernelGGL(k_propagate_rbh<M>, dim3(rbh_grid(p, nw)), dim3(RBH_WAVES * WAVE),
+                       (size_t)p->rbh_lds_bytes, st, p->dp, p->t32, nw, spins, log_w, e_loc,
+                       beta, alw, k0, k1, step, tag, evals, g_step_base);
+}
+
+template <int M>
+static void rbh_rates_m(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double *rates,
+                        double *e_loc, double *log_diff, double *log_psi, hipStream_t st) {
+    hipLaunchKernelGGL(k_rates_rbh<M>, dim3(rbh_grid(p, nw)), dim3(RBH_WAVES * WAVE),
+                       (size_t)p->rbh_lds_bytes, st, p->dp, p->t32, nw, spins, rates, e_loc,
+                       log_diff, log_psi);
+}
+
+#define RBH_SWITCH(M, FN, ...)                      \
+    do {                                            \
+        switch (M) {                                \
+        case 1: FN<1>(__VA_ARGS__); break;          \
+        case 2: FN<2>(__VA_ARGS__); break;          \
+        case 3: FN<3>(__VA_ARGS__); break;          \
+        case 4: FN<4>(__VA_ARGS__); break;          \
+        case 5: FN<5>(__VA_ARGS__); break;          \
+        case 6: FN<6>(__VA_ARGS__); break;          \
+        case 7: FN<7>(__VA_ARGS__); break;          \
+        default: FN<8>(__VA_ARGS__); break;         \
+        }                                           \
+    } while (0)
+
 // persistent RBM grid: as many workgroups as fit (one per CU at full LDS)
 static unsigned rbm_grid(const gfmc_problem *p, int64_t nw) {
     const int64_t need = (nw + RBM_WAVES - 1) / RBM_WAVES;
@@ -2264,6 +2337,26 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         UP(d->rbm_b, (size_t)H, &P.rbm_b);
         UP(cs.data(), cs.size(), &P.rbm_cs);
         P.rbm_wave_bytes = round_up(P.K * WAVE * 4 + 3 * H * 4 + n_off * 4 + P.n_pad + P.e_pad, 16);
+        // the lanes-over-hidden-units kernels: W transposed, [N][Hp] BF16
+        P.Hp = round_up(H, WAVE);
+        P.wt_bytes = round_up(N * P.Hp * 2, 16);
+        P.rbh_wave_bytes = round_up(P.K * WAVE * 4 + (n_off + 2) * 12 + P.n_pad + P.e_pad, 16);
+        const int64_t rbh_lds = (int64_t)P.wt_bytes + (int64_t)RBH_WAVES * P.rbh_wave_bytes;
+        P.rbh = (P.Hp / WAVE <= RBH_MAX_M && rbh_lds <= 160 * 1024) ? P.Hp / WAVE : 0;
+        if (P.rbh) {
+            std::vector<uint16_t> wt((size_t)P.wt_bytes / 2, (uint16_t)0);
+            for (int h = 0; h < H; ++h)
+                for (int k = 0; k < N; ++k) {
+                    const size_t idx = (size_t)h * Np + k;
+                    wt[(size_t)k * P.Hp + h] = (uint16_t)(wp[idx >> 1] >> ((idx & 1) ? 16 : 0));
+                }
+            UP(wt.data(), wt.size(), &P.rbm_WT);
+            const int zero = 0;
+            const int *q = nullptr;
+            UP(&zero, (size_t)1, &q);
+            P.rbm_queue = const_cast<int *>(q);
+            p->rbh_lds_bytes = rbh_lds;
+        }
     }
     if (d->guide == GFMC_GUIDE_MLP) {
         const int h1 = d->mlp_h1, h2 = d->mlp_h2;
@@ -2314,6 +2407,13 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         HIP_TRY(hipFuncSetAttribute((const void *)k_rates_rbm,
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)p->lds_bytes));
+        if (P.rbh) {
+            int rc2 = rbh_set_lds(P.rbh, (int)p->rbh_lds_bytes);
+            if (rc2) {
+                free_problem(p);
+                return rc2;
+            }
+        }
     } else if (d->guide == GFMC_GUIDE_MLP) {
         p->lds_bytes = (int64_t)MLP_SW * P.rbm_wave_bytes;
         if (p->lds_bytes > 64 * 1024) {
@@ -2453,7 +2553,10 @@ int gfmc_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, double 
     if (!spins) return fail(GFMC_EINVAL, "gfmc_rates: NULL spins");
     if (p->dp.guide == GFMC_GUIDE_MLP)
         return mlp_rates(p, nw, spins, rates, e_loc, log_diff, nullptr, (hipStream_t)stream);
-    if (p->dp.guide == GFMC_GUIDE_RBM)
+    if (p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbh)
+        RBH_SWITCH(p->dp.rbh, rbh_rates_m, p, nw, spins, rates, e_loc, log_diff,
+                   (double *)nullptr, (hipStream_t)stream);
+    else if (p->dp.guide == GFMC_GUIDE_RBM)
         hipLaunchKernelGGL(k_rates_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
                            (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
                            rates, e_loc, log_diff, (double *)nullptr);
@@ -2473,7 +2576,10 @@ int gfmc_log_psi(const gfmc_problem *p, int64_t nw, const uint32_t *spins, doubl
     if (!spins || !out) return fail(GFMC_EINVAL, "gfmc_log_psi: NULL buffer");
     if (p->dp.guide == GFMC_GUIDE_MLP)
         return mlp_rates(p, nw, spins, nullptr, nullptr, nullptr, out, (hipStream_t)stream);
-    if (p->dp.guide == GFMC_GUIDE_RBM)
+    if (p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbh)
+        RBH_SWITCH(p->dp.rbh, rbh_rates_m, p, nw, spins, (double *)nullptr, (double *)nullptr,
+                   (double *)nullptr, out, (hipStream_t)stream);
+    else if (p->dp.guide == GFMC_GUIDE_RBM)
         hipLaunchKernelGGL(k_rates_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
                            (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
                            (double *)nullptr, (double *)nullptr, (double *)nullptr, out);
@@ -2500,7 +2606,10 @@ int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
     if (p->dp.guide == GFMC_GUIDE_MLP)
         return mlp_propagate(p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
                              (hipStream_t)stream);
-    if (p->dp.guide == GFMC_GUIDE_RBM)
+    if (p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbh)
+        RBH_SWITCH(p->dp.rbh, rbh_launch_m, p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
+                   tag, evals, (hipStream_t)stream);
+    else if (p->dp.guide == GFMC_GUIDE_RBM)
         hipLaunchKernelGGL(k_propagate_rbm, dim3(rbm_grid(p, nw)), dim3(RBM_WAVES * WAVE),
                            (size_t)p->lds_bytes, (hipStream_t)stream, p->dp, p->t32, nw, spins,
                            log_w, e_loc, beta, alw, k0, k1, step, tag, evals, g_step_base);

@@ -413,6 +413,9 @@ int gfmc_rbm_set_params(gfmc_problem *p, const float *flat, void *stream) {
                        (float *)p->dp.rbm_b);
     hipLaunchKernelGGL(k_rbm_colsum, dim3(grid1d(N, 256)), dim3(256), 0, (hipStream_t)stream, N,
                        H, Np, p->dp.rbm_W, (float *)p->dp.rbm_cs);
+    if (p->dp.rbh)
+        hipLaunchKernelGGL(k_rbm_pack_t, dim3(grid1d((int64_t)N * p->dp.Hp, 256)), dim3(256), 0,
+                           (hipStream_t)stream, N, H, p->dp.Hp, flat, (uint16_t *)p->dp.rbm_WT);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -39,6 +39,11 @@ def rbm_systems():
     L = square(10, 10)
     H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
     out.append(("square10_alpha4", H, random_rbm(100, alpha=4, seed=3, scale=0.05)))
+    # wider than the lanes-over-hidden-units kernels take (H > 512): the
+    # lanes-over-slots kernels
+    L = chain(12)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    out.append(("chain12_wide", H, random_rbm(12, alpha=48, seed=4, scale=0.1)))
     return out
 
 
@@ -118,3 +123,32 @@ def test_rbm_guided_gfmc_energy_matches_ed():
         E, _, err, _ = evaluate_observables(raw, n_blocks=20)
     say("  E", E[32], "+-", err[32], "exact", e0, "it/s", stats["it_per_sec"])
     assert abs(E[32] - e0) < 4 * err[32] + 2e-3
+
+
+@pytest.mark.parametrize("which", [2, 3], ids=["square10_alpha4", "chain12_wide"])
+def test_rbm_propagation_matches_fp64_oracle_at_size(orc, which):
+    """The 10 x 10 alpha = 4 configuration's shape (and the wide fallback):
+    one propagation from the same walkers with the same uniforms, most
+    walkers on the oracle's path with its evaluation counts and local
+    energies."""
+    name, H, G = SYSTEMS[which]
+    say("rbm propagate", name)
+    prob = Problem(H, G, _lib.GFMC_FP32)
+    nw = 192
+    spins = orc.init_sz(H.N, nw, H.N // 2, 13)
+    s_ref, lw_ref, e_ref, ev_ref = orc.Problem(lower(H, G)).propagate(
+        spins, np.zeros(nw), np.zeros(nw), 0.2, -50.0, 91, 5, 0)
+    s = to_dev(spins)
+    lw = torch.zeros(nw, dtype=torch.float64, device="cuda")
+    e = torch.zeros(nw, dtype=torch.float64, device="cuda")
+    ev = torch.zeros(nw, dtype=torch.int32, device="cuda")
+    prob.propagate(s, lw, e, 0.2, -50.0, 91, 5, 0, ev)
+    torch.cuda.synchronize()
+    got = s.cpu().numpy().view(np.uint32)
+    same = np.all(got == s_ref, axis=1) & (ev.cpu().numpy() == ev_ref)
+    say("  same path:", same.mean(), "evals", ev_ref.mean())
+    assert same.mean() > 0.9 and ev_ref.mean() > 2
+    np.testing.assert_allclose(e.cpu().numpy()[same], e_ref[same], rtol=RTOL,
+                               atol=RTOL * np.abs(e_ref).max())
+    scale = np.abs(e_ref).max() * 0.2
+    np.testing.assert_allclose(lw.cpu().numpy()[same], lw_ref[same], rtol=RTOL, atol=RTOL * scale)
