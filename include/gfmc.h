@@ -409,6 +409,41 @@ int gfmc_rbm_vmin_grad(const gfmc_problem *p, int64_t n, const uint32_t *spins,
 
 /* One Adam step (Kingma & Ba 2015, in torch.optim.Adam's form) on n FP32
  * device parameters with moment buffers m, v; step counts from 1. */
+/* Tied parameters: replace every entry of v[0:n] by the mean of its orbit
+ * (orbit o = the indices members[offsets[o] .. offsets[o+1]), device int32
+ * arrays, summed in that order in FP64; indices outside [0, n) are skipped).
+ * Applied to a gradient it trains a translation-invariant guide with the
+ * other kernels unchanged.  No counterpart in the reference, whose NetKet
+ * trainer differentiates tied parameters directly (the Jastrow guides' tied
+ * beta/h, wavefunction/_tied_params.py). */
+int gfmc_orbit_mean(int64_t n, float *v, int32_t n_orbits, const int32_t *offsets,
+                    const int32_t *members, void *stream);
+
+/* Tied parameters (continued): out[o] = sum of v over orbit o's members;
+ * v[p] += alpha * x[orbit of p].  Same orbit tables as gfmc_orbit_mean. */
+int gfmc_orbit_sum(int64_t n, const float *v, int32_t n_orbits, const int32_t *offsets,
+                   const int32_t *members, float *out, void *stream);
+int gfmc_orbit_axpy(int64_t n, float *v, int32_t n_orbits, const int32_t *offsets,
+                    const int32_t *members, const float *x, float alpha, void *stream);
+
+/* Natural-gradient (stochastic-reconfiguration) updates of a tied RBM, the
+ * SR option of the reference's trainer (generic_netket_training.py:459-461),
+ * here for local-energy variance minimisation on GFMC samples.
+ * gfmc_rbm_orbit_O: O [n x n_orbits] float, the orbit-summed log-derivatives
+ * of each sample (theta: [n x H] float scratch; n <= 65535).
+ * gfmc_sr_solve: S = covariance of the rows of O (FP64), then conjugate
+ * gradients on (S + eps_rel diag S + eps_abs I) delta = g (at most max_iter
+ * iterations, until |r| <= tol |g|), scaled so that sqrt(delta^T S delta),
+ * the rms change of log psi over the samples, is at most max_norm (0: no
+ * limit); work: gfmc_sr_work_doubles(m) doubles. */
+int gfmc_rbm_orbit_O(const gfmc_problem *p, int64_t n, const uint32_t *spins, int32_t n_orbits,
+                     const int32_t *offsets, const int32_t *members, float *O, float *theta,
+                     void *stream);
+int64_t gfmc_sr_work_doubles(int32_t m);
+int gfmc_sr_solve(int64_t n, int32_t m, const float *O, const float *g, double eps_rel,
+                  double eps_abs, int32_t max_iter, double tol, double max_norm, float *delta,
+                  double *work, void *stream);
+
 int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *v,
                    int64_t step, float lr, float beta1, float beta2, float eps,
                    void *stream);

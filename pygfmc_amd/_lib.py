@@ -157,6 +157,16 @@ SIGNATURES = [
     ("gfmc_rbm_vmin_work_bytes", ctypes.c_int64, [_vp, ctypes.c_int64]),
     ("gfmc_rbm_vmin_grad", ctypes.c_int, [_vp, ctypes.c_int64, _vp, ctypes.c_float, _vp, _vp,
                                           _vp, _vp]),
+    ("gfmc_orbit_mean", ctypes.c_int, [ctypes.c_int64, _vp, ctypes.c_int32, _vp, _vp, _vp]),
+    ("gfmc_orbit_sum", ctypes.c_int, [ctypes.c_int64, _vp, ctypes.c_int32, _vp, _vp, _vp, _vp]),
+    ("gfmc_orbit_axpy", ctypes.c_int, [ctypes.c_int64, _vp, ctypes.c_int32, _vp, _vp, _vp,
+                                       ctypes.c_float, _vp]),
+    ("gfmc_rbm_orbit_O", ctypes.c_int, [_vp, ctypes.c_int64, _vp, ctypes.c_int32, _vp, _vp, _vp,
+                                        _vp, _vp]),
+    ("gfmc_sr_work_doubles", ctypes.c_int64, [ctypes.c_int32]),
+    ("gfmc_sr_solve", ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _vp, _vp, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+                                     ctypes.c_double, _vp, _vp, _vp]),
     ("gfmc_adam_step", ctypes.c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int64,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, _vp]),

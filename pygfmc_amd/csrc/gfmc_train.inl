@@ -344,6 +344,194 @@ __global__ __launch_bounds__(256) void k_rbm_colsum(int N, int H, int Np, const 
     cs[k] = (float)s;
 }
 
+// mean over every orbit of tied parameters, in place: orbit o holds the flat
+// indices members[off[o] .. off[o + 1]), added in that order in FP64; indices
+// outside [0, n) are skipped
+__global__ __launch_bounds__(256) void k_orbit_mean(int64_t n, int32_t n_orb, const int32_t *off,
+                                                    const int32_t *members, float *v) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_orb) return;
+    const int a = off[o], b = off[o + 1];
+    if (b <= a) return;
+    double s = 0.0;
+    int cnt = 0;
+    for (int t = a; t < b; ++t) {
+        const int q = members[t];
+        if (q < 0 || q >= n) continue;
+        s += (double)v[q];
+        ++cnt;
+    }
+    if (cnt == 0) return;
+    const float m = (float)(s / (double)cnt);
+    for (int t = a; t < b; ++t) {
+        const int q = members[t];
+        if (q >= 0 && q < n) v[q] = m;
+    }
+}
+
+// v[q] = sum of v_full over orbit q's members (in members order, FP64)
+__global__ __launch_bounds__(256) void k_orbit_sum(int64_t n, int32_t n_orb, const int32_t *off,
+                                                   const int32_t *members, const float *v,
+                                                   float *out) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_orb) return;
+    double s = 0.0;
+    for (int t = off[o]; t < off[o + 1]; ++t) {
+        const int q = members[t];
+        if (q >= 0 && q < n) s += (double)v[q];
+    }
+    out[o] = (float)s;
+}
+
+// v_full[p] += alpha * x[orbit of p]
+__global__ __launch_bounds__(256) void k_orbit_axpy(int64_t n, int32_t n_orb, const int32_t *off,
+                                                    const int32_t *members, const float *x,
+                                                    float alpha, float *v) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_orb) return;
+    const float d = alpha * x[o];
+    for (int t = off[o]; t < off[o + 1]; ++t) {
+        const int q = members[t];
+        if (q >= 0 && q < n) v[q] += d;
+    }
+}
+
+// Orbit-summed log-derivatives of the RBM, O[i][o] = sum over the members p
+// of orbit o of d log psi(x_i) / d p: tanh(theta_ih) x_ik for W_hk, x_ik for
+// a_k, tanh(theta_ih) for b_h; thread per (orbit, sample)
+__global__ __launch_bounds__(256) void k_rbm_orbit_O(DevProblem P, int64_t n, const uint32_t *spins,
+                                                     const float *theta, int32_t n_orb,
+                                                     const int32_t *off, const int32_t *members,
+                                                     float *O) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (o >= n_orb || i >= n) return;
+    const int N = P.N, H = P.H;
+    const int64_t HN = (int64_t)H * N;
+    const uint32_t *sp = spins + i * P.words;
+    const float *th = theta + i * H;
+    double s = 0.0;
+    for (int t = off[o]; t < off[o + 1]; ++t) {
+        const int64_t q = members[t];
+        if (q < HN) {
+            const int h = (int)(q / N), k = (int)(q % N);
+            const float xk = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1.0f : -1.0f;
+            s += (double)(tanhf(th[h]) * xk);
+        } else if (q < HN + N) {
+            const int k = (int)(q - HN);
+            s += ((sp[k >> 5] >> (k & 31)) & 1u) ? 1.0 : -1.0;
+        } else if (q < HN + N + H) {
+            s += (double)tanhf(th[q - HN - N]);
+        }
+    }
+    O[i * n_orb + o] = (float)s;
+}
+
+// natural-gradient solve (stochastic reconfiguration): column means of O
+// (n x m), then the covariance S = (1/n) sum_i (O_i - mean)(O_i - mean)^T in
+// 16 x 16 tiles (FP64 accumulation, samples in order), then conjugate
+// gradients on (S + eps_rel diag S + eps_abs) d = g in one workgroup
+#define SR_T 16
+__global__ __launch_bounds__(256) void k_sr_mean(int64_t n, int m, const float *O, double *mean) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += (double)O[i * m + q];
+    mean[q] = s / (double)n;
+}
+
+__global__ __launch_bounds__(SR_T * SR_T) void k_sr_cov(int64_t n, int m, const float *O,
+                                                       const double *mean, double *S) {
+    __shared__ double a[SR_T][SR_T], b[SR_T][SR_T];
+    const int tx = threadIdx.x % SR_T, ty = threadIdx.x / SR_T;
+    const int q1 = blockIdx.y * SR_T + ty, q2 = blockIdx.x * SR_T + tx;
+    double acc = 0.0;
+    for (int64_t i0 = 0; i0 < n; i0 += SR_T) {
+        // a[r][c]: sample i0 + r, column blockIdx.y*T + c; b likewise for x
+        const int64_t ia = i0 + ty;
+        const int ca = blockIdx.y * SR_T + tx, cb = blockIdx.x * SR_T + tx;
+        a[ty][tx] = (ia < n && ca < m) ? (double)O[ia * m + ca] - mean[ca] : 0.0;
+        b[ty][tx] = (ia < n && cb < m) ? (double)O[ia * m + cb] - mean[cb] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < SR_T; ++r) acc += a[r][ty] * b[r][tx];
+        __syncthreads();
+    }
+    if (q1 < m && q2 < m) S[(int64_t)q1 * m + q2] = acc / (double)n;
+}
+
+__device__ double sr_block_sum(double v, double *sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x / WAVE); ++q) t += sh[q];
+    __syncthreads();
+    return t;
+}
+
+// y = (S + eps_rel diag S + eps_abs) x, rows over the block's threads
+__device__ void sr_apply(int m, const double *S, double eps_rel, double eps_abs, const double *x,
+                         double *y) {
+    for (int r = threadIdx.x; r < m; r += blockDim.x) {
+        const double *row = S + (int64_t)r * m;
+        double acc = 0.0;
+        for (int c = 0; c < m; ++c) acc += row[c] * x[c];
+        y[r] = acc + (eps_rel * row[r] + eps_abs) * x[r];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_sr_cg(int m, const double *S, double eps_rel,
+                                               double eps_abs, const float *g, int max_iter,
+                                               double tol, double max_norm, double *x,
+                                               double *r, double *pv, double *Ap,
+                                               float *delta) {
+    __shared__ double sh[16];
+    for (int q = threadIdx.x; q < m; q += blockDim.x) {
+        x[q] = 0.0;
+        r[q] = (double)g[q];
+        pv[q] = r[q];
+    }
+    __syncthreads();
+    double rr = 0.0;
+    for (int q = threadIdx.x; q < m; q += blockDim.x) rr += r[q] * r[q];
+    rr = sr_block_sum(rr, sh);
+    const double stop = tol * tol * rr;
+    for (int it = 0; it < max_iter && rr > stop && rr > 0.0; ++it) {
+        sr_apply(m, S, eps_rel, eps_abs, pv, Ap);
+        double pAp = 0.0;
+        for (int q = threadIdx.x; q < m; q += blockDim.x) pAp += pv[q] * Ap[q];
+        pAp = sr_block_sum(pAp, sh);
+        if (!(pAp > 0.0)) break;
+        const double alpha = rr / pAp;
+        double rr2 = 0.0;
+        for (int q = threadIdx.x; q < m; q += blockDim.x) {
+            x[q] += alpha * pv[q];
+            r[q] -= alpha * Ap[q];
+            rr2 += r[q] * r[q];
+        }
+        rr2 = sr_block_sum(rr2, sh);
+        const double beta = rr2 / rr;
+        rr = rr2;
+        for (int q = threadIdx.x; q < m; q += blockDim.x) pv[q] = r[q] + beta * pv[q];
+        __syncthreads();
+    }
+    // trust region: the step's rms change of log psi over the samples,
+    // sqrt(x^T S x), at most max_norm (0: no limit)
+    double scale = 1.0;
+    if (max_norm > 0.0) {
+        sr_apply(m, S, 0.0, 0.0, x, Ap);
+        double xSx = 0.0;
+        for (int q = threadIdx.x; q < m; q += blockDim.x) xSx += x[q] * Ap[q];
+        xSx = sr_block_sum(xSx, sh);
+        if (xSx > max_norm * max_norm) scale = max_norm / sqrt(xSx);
+    }
+    for (int q = threadIdx.x; q < m; q += blockDim.x) delta[q] = (float)(x[q] * scale);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -416,6 +604,90 @@ int gfmc_rbm_set_params(gfmc_problem *p, const float *flat, void *stream) {
     if (p->dp.rbh)
         hipLaunchKernelGGL(k_rbm_pack_t, dim3(grid1d((int64_t)N * p->dp.Hp, 256)), dim3(256), 0,
                            (hipStream_t)stream, N, H, p->dp.Hp, flat, (uint16_t *)p->dp.rbm_WT);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_orbit_mean(int64_t n, float *v, int32_t n_orbits, const int32_t *offsets,
+                    const int32_t *members, void *stream) {
+    if (n <= 0 || n > 0x7fffffffLL || n_orbits <= 0)
+        return fail(GFMC_EINVAL, "gfmc_orbit_mean: bad size");
+    if (!v || !offsets || !members) return fail(GFMC_EINVAL, "gfmc_orbit_mean: NULL buffer");
+    hipLaunchKernelGGL(k_orbit_mean, dim3(grid1d(n_orbits, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, n_orbits, offsets, members, v);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int orbit_args(const char *fn, int64_t n, int32_t n_orbits, const void *a, const void *b,
+                      const void *c) {
+    if (n <= 0 || n > 0x7fffffffLL || n_orbits <= 0)
+        return fail(GFMC_EINVAL, std::string(fn) + ": bad size");
+    if (!a || !b || !c) return fail(GFMC_EINVAL, std::string(fn) + ": NULL buffer");
+    return 0;
+}
+
+int gfmc_orbit_sum(int64_t n, const float *v, int32_t n_orbits, const int32_t *offsets,
+                   const int32_t *members, float *out, void *stream) {
+    int rc = orbit_args("gfmc_orbit_sum", n, n_orbits, v, offsets, members);
+    if (rc) return rc;
+    if (!out) return fail(GFMC_EINVAL, "gfmc_orbit_sum: NULL output");
+    hipLaunchKernelGGL(k_orbit_sum, dim3(grid1d(n_orbits, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, n_orbits, offsets, members, v, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_orbit_axpy(int64_t n, float *v, int32_t n_orbits, const int32_t *offsets,
+                    const int32_t *members, const float *x, float alpha, void *stream) {
+    int rc = orbit_args("gfmc_orbit_axpy", n, n_orbits, v, offsets, members);
+    if (rc) return rc;
+    if (!x) return fail(GFMC_EINVAL, "gfmc_orbit_axpy: NULL input");
+    hipLaunchKernelGGL(k_orbit_axpy, dim3(grid1d(n_orbits, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, n_orbits, offsets, members, x, alpha, v);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gfmc_rbm_orbit_O(const gfmc_problem *p, int64_t n, const uint32_t *spins, int32_t n_orbits,
+                     const int32_t *offsets, const int32_t *members, float *O, float *theta,
+                     void *stream) {
+    int rc = check_rbm("gfmc_rbm_orbit_O", p);
+    if (rc) return rc;
+    if (n < 1 || n > 65535 || n_orbits < 1)
+        return fail(GFMC_EINVAL, "gfmc_rbm_orbit_O: n must be in [1, 65535], n_orbits >= 1");
+    if (!spins || !offsets || !members || !O || !theta)
+        return fail(GFMC_EINVAL, "gfmc_rbm_orbit_O: NULL buffer");
+    rc = gfmc_rbm_theta(p, n, spins, theta, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rbm_orbit_O, dim3(grid1d(n_orbits, 256), (unsigned)n), dim3(256), 0,
+                       (hipStream_t)stream, p->dp, n, spins, theta, n_orbits, offsets, members,
+                       O);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int64_t gfmc_sr_work_doubles(int32_t m) {
+    if (m < 1 || m > 8192) return -1;
+    return (int64_t)m * m + 5 * (int64_t)m;
+}
+
+int gfmc_sr_solve(int64_t n, int32_t m, const float *O, const float *g, double eps_rel,
+                  double eps_abs, int32_t max_iter, double tol, double max_norm, float *delta,
+                  double *work, void *stream) {
+    if (n < 1 || m < 1 || m > 8192 || max_iter < 0)
+        return fail(GFMC_EINVAL, "gfmc_sr_solve: bad size");
+    if (!O || !g || !delta || !work) return fail(GFMC_EINVAL, "gfmc_sr_solve: NULL buffer");
+    if (!(eps_rel >= 0.0) || !(eps_abs >= 0.0) || !(tol >= 0.0) || !(max_norm >= 0.0))
+        return fail(GFMC_EINVAL, "gfmc_sr_solve: eps, tol and max_norm must be >= 0");
+    const hipStream_t st = (hipStream_t)stream;
+    double *S = work, *mean = work + (int64_t)m * m, *x = mean + m, *r = x + m, *pv = r + m,
+           *Ap = pv + m;
+    hipLaunchKernelGGL(k_sr_mean, dim3(grid1d(m, 256)), dim3(256), 0, st, n, m, O, mean);
+    const unsigned tiles = (unsigned)((m + SR_T - 1) / SR_T);
+    hipLaunchKernelGGL(k_sr_cov, dim3(tiles, tiles), dim3(SR_T * SR_T), 0, st, n, m, O, mean, S);
+    hipLaunchKernelGGL(k_sr_cg, dim3(1), dim3(1024), 0, st, m, (const double *)S, eps_rel,
+                       eps_abs, g, max_iter, tol, max_norm, x, r, pv, Ap, delta);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -163,7 +163,7 @@ class RBMSpinHalf:
     """
 
     def __init__(self, N, alpha=4, hidden=None, W=None, a=None, b=None, seed=0, scale=0.01,
-                 dtype=np.float32):
+                 dtype=np.float32, orbits=None):
         self.N = int(N)
         H = int(hidden) if hidden is not None else int(alpha) * self.N
         rng = np.random.default_rng(seed)
@@ -179,12 +179,79 @@ class RBMSpinHalf:
         self.b = np.zeros(H, np.float32) if b is None else np.asarray(b, np.float32).copy()
         if self.a.shape != (self.N,) or self.b.shape != (H,):
             raise ValueError("a must have shape (N,) and b shape (hidden,)")
+        # tied parameters: orbits[q] = the orbit of flat parameter q (W row-major,
+        # then a, then b); parameters of one orbit are equal and train together
+        self.orbits = None
+        if orbits is not None:
+            o = np.asarray(orbits, dtype=np.int64).reshape(-1)
+            if o.shape != (H * self.N + self.N + H,) or o.min() < 0:
+                raise ValueError("orbits must give every flat parameter an orbit index >= 0")
+            self.orbits = o
+
+    @classmethod
+    def symmetric(cls, translations, n_filters, filters=None, b=None, a=0.0, seed=0,
+                  scale=0.01):
+        """A translation-invariant RBM: hidden unit (f, r) = f * n_T + r is
+        filter f moved by translation r, W[(f, r), T_r(k)] = filters[f, k],
+        b[(f, r)] = b[f], a_k = a (one value).  ``translations``: (n_T, N)
+        site permutations T_r (row r maps site k to T_r(k)); n_filters * n_T
+        hidden units; the tied parameters are recorded in ``orbits``."""
+        T = np.asarray(translations, dtype=np.int64)
+        n_T, N = T.shape
+        F = int(n_filters)
+        rng = np.random.default_rng(seed)
+        filt = (rng.normal(scale=scale, size=(F, N)) if filters is None
+                else np.asarray(filters, dtype=np.float64).reshape(F, N))
+        bf = np.zeros(F) if b is None else np.asarray(b, dtype=np.float64).reshape(F)
+        H = F * n_T
+        W = np.zeros((H, N))
+        oW = np.zeros((H, N), dtype=np.int64)
+        for f in range(F):
+            for r in range(n_T):
+                W[f * n_T + r, T[r]] = filt[f]
+                oW[f * n_T + r, T[r]] = f * N + np.arange(N)
+        orbits = np.concatenate([oW.reshape(-1), np.full(N, F * N + F),
+                                 F * N + np.repeat(np.arange(F), n_T)])
+        return cls(N, hidden=H, W=W, a=np.full(N, a), b=np.repeat(bf, n_T), orbits=orbits)
+
+    @classmethod
+    def from_jastrow(cls, edges, translations, beta, n_filters, seed=0, scale=0.01):
+        """A translation-invariant RBM that starts as the nearest-neighbour
+        Jastrow guide log psi = -beta sum_edges x_i x_j (+ const): one filter
+        per class of translation-equivalent edges, w at site 0 and -w at the
+        edge's other end, lncosh(w (x_i - x_j)) = lncosh(2 w) on anti-parallel
+        bonds and 0 on parallel ones, so lncosh(2 w) / 2 = beta,
+        w = acosh(e^{2 beta}) / 2 (before BF16 rounding); the remaining
+        filters start small and random."""
+        T = np.asarray(translations, dtype=np.int64)
+        n_T, N = T.shape
+        inv = np.empty_like(T)
+        for r in range(n_T):
+            inv[r, T[r]] = np.arange(N)
+        at = {int(T[r, 0]): r for r in range(n_T)}
+        classes = set()
+        for i, j in edges:
+            d1, d2 = int(inv[at[int(i)], int(j)]), int(inv[at[int(j)], int(i)])
+            classes.add(min(d1, d2))
+        classes = sorted(classes)
+        if len(classes) > n_filters:
+            raise ValueError(f"{len(classes)} edge classes need at least as many filters, "
+                             f"got n_filters={n_filters}")
+        w = float(np.arccosh(np.exp(2.0 * float(beta)))) / 2.0
+        rng = np.random.default_rng(seed)
+        filt = rng.normal(scale=scale, size=(int(n_filters), N))
+        for f, d in enumerate(classes):
+            filt[f] = 0.0
+            filt[f, 0] = w
+            filt[f, d] = -w
+        return cls.symmetric(T, n_filters, filters=filt)
 
     def parameters(self):
         return dict(W=self.W.copy(), a=self.a.copy(), b=self.b.copy())
 
     def with_parameters(self, W, a, b):
-        return RBMSpinHalf(self.N, hidden=self.hidden, W=W, a=a, b=b, dtype=self.dtype)
+        return RBMSpinHalf(self.N, hidden=self.hidden, W=W, a=a, b=b, dtype=self.dtype,
+                           orbits=self.orbits)
 
     def fingerprint(self):
         return (b"rbm|" + np.int64(self.N).tobytes() + np.int64(self.hidden).tobytes()

@@ -55,8 +55,20 @@ def square(Lx: int, Ly: int, periodic: bool = True) -> Lattice:
                 if i != j:
                     out.add((min(i, j), max(i, j)))
     pos = np.array([[x, y] for y in range(Ly) for x in range(Lx)], dtype=float)
-    return Lattice(name="square", N=Lx * Ly, edges=sorted(out),
-                   extras=dict(positions=pos, supercell_dims=np.array([Lx, Ly])))
+    extras = dict(positions=pos, supercell_dims=np.array([Lx, Ly]))
+    if periodic:
+        extras["translations"] = square_translations(Lx, Ly)
+    return Lattice(name="square", N=Lx * Ly, edges=sorted(out), extras=extras)
+
+
+def square_translations(Lx: int, Ly: int) -> np.ndarray:
+    """The Lx*Ly translations of the periodic square lattice as site
+    permutations: row r = dx + Lx*dy maps site x + Lx*y to
+    (x + dx) % Lx + Lx*((y + dy) % Ly)."""
+    x = np.arange(Lx * Ly) % Lx
+    y = np.arange(Lx * Ly) // Lx
+    return np.array([((x + dx) % Lx) + Lx * ((y + dy) % Ly)
+                     for dy in range(Ly) for dx in range(Lx)], dtype=np.int64)
 
 
 def kagome(Lx: int, Ly: int, periodic: bool = True) -> Lattice:

@@ -28,6 +28,18 @@ from .walkers import as_key
 __all__ = ["RBMTrainer", "MLPTrainer", "vmin_gradient", "train_variance"]
 
 
+def orbit_tables(orbits, dev):
+    """Device (offsets, members, n_orbits) of an orbit index per flat
+    parameter: the members of orbit o in ascending flat order."""
+    o = np.asarray(orbits, dtype=np.int64)
+    uniq, inv = np.unique(o, return_inverse=True)
+    members = np.argsort(inv, kind="stable").astype(np.int32)
+    counts = np.bincount(inv, minlength=len(uniq))
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    return (torch.as_tensor(offsets, device=dev), torch.as_tensor(members, device=dev),
+            len(uniq))
+
+
 def flat_params(rbm):
     """[W (hidden x N row-major), a, b] as one float32 vector."""
     return np.concatenate([rbm.W.reshape(-1), rbm.a, rbm.b]).astype(np.float32)
@@ -44,7 +56,8 @@ class RBMTrainer:
     """Device state of variance minimisation for one RBM problem: FP32 master
     parameters, Adam moments, the gradient and its workspace."""
 
-    def __init__(self, problem, rbm, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0):
+    def __init__(self, problem, rbm, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0,
+                 optimizer="adam", sr_shift=1e-2, sr_abs=1e-4, sr_iter=500, sr_max_norm=0.02):
         if not isinstance(rbm, RBMSpinHalf):
             raise TypeError(f"RBMTrainer trains an RBMSpinHalf, got {type(rbm).__name__}")
         if problem.info.guide != _lib.GUIDE_RBM:
@@ -64,6 +77,34 @@ class RBMTrainer:
         self.lr, self.betas, self.eps, self.clip = float(lr), betas, float(eps), float(clip)
         self._work = None
         self._work_n = -1
+        self._orbits = None
+        if getattr(rbm, "orbits", None) is not None:
+            self._orbits = orbit_tables(rbm.orbits, dev)
+        if optimizer not in ("adam", "sr"):
+            raise ValueError(f"optimizer must be 'adam' or 'sr', got {optimizer!r}")
+        self.optimizer = optimizer
+        self.sr_shift, self.sr_abs, self.sr_iter = float(sr_shift), float(sr_abs), int(sr_iter)
+        self.sr_max_norm = float(sr_max_norm)
+        if optimizer == "sr":
+            if self._orbits is None:
+                raise ValueError("the natural-gradient (SR) update needs a tied guide "
+                                 "(RBMSpinHalf.symmetric / from_jastrow)")
+            m = self._orbits[2]
+            self._g_t = torch.zeros(m, dtype=torch.float32, device=dev)
+            self._delta = torch.zeros(m, dtype=torch.float32, device=dev)
+            self._sr_work = torch.empty(int(lib.gfmc_sr_work_doubles(m)), dtype=torch.float64,
+                                        device=dev)
+            self._O = None
+            self._theta = None
+        self._last_spins = None
+
+    def tie(self, v):
+        """Orbit means of ``v`` in place (no-op for an untied guide)."""
+        if self._orbits is not None:
+            off, mem, n_orb = self._orbits
+            _lib.check(_lib.load().gfmc_orbit_mean(v.numel(), _lib.ptr(v), n_orb, _lib.ptr(off),
+                                                   _lib.ptr(mem), _lib.stream()),
+                       "gfmc_orbit_mean")
 
     def _workspace(self, n):
         if n != self._work_n:
@@ -81,12 +122,49 @@ class RBMTrainer:
         _lib.check(_lib.load().gfmc_rbm_vmin_grad(
             self.p.handle, n, _lib.ptr(spins), self.clip, _lib.ptr(self.grad), _lib.ptr(out2),
             _lib.ptr(self._workspace(n)), _lib.stream()), "gfmc_rbm_vmin_grad")
+        if self.optimizer == "sr":
+            self._last_spins = spins
+        else:
+            self.tie(self.grad)
+
+    def _sr_direction(self):
+        """delta = (S + shift diag S + abs) ^ -1 g over the orbits of the
+        last gradient's samples."""
+        lib = _lib.load()
+        off, mem, m = self._orbits
+        spins = self._last_spins
+        n = spins.shape[0]
+        if self._O is None or self._O.shape[0] != n:
+            self._O = torch.empty((n, m), dtype=torch.float32, device=spins.device)
+            self._theta = torch.empty((n, self.rbm.hidden), dtype=torch.float32,
+                                      device=spins.device)
+        _lib.check(lib.gfmc_orbit_sum(self.n_params, _lib.ptr(self.grad), m, _lib.ptr(off),
+                                      _lib.ptr(mem), _lib.ptr(self._g_t), _lib.stream()),
+                   "gfmc_orbit_sum")
+        _lib.check(lib.gfmc_rbm_orbit_O(self.p.handle, n, _lib.ptr(spins), m, _lib.ptr(off),
+                                        _lib.ptr(mem), _lib.ptr(self._O), _lib.ptr(self._theta),
+                                        _lib.stream()), "gfmc_rbm_orbit_O")
+        _lib.check(lib.gfmc_sr_solve(n, m, _lib.ptr(self._O), _lib.ptr(self._g_t),
+                                     self.sr_shift, self.sr_abs, self.sr_iter, 1e-6,
+                                     self.sr_max_norm, _lib.ptr(self._delta),
+                                     _lib.ptr(self._sr_work),
+                                     _lib.stream()), "gfmc_sr_solve")
+        return self._delta
 
     def update(self):
-        """One Adam step on the master parameters, then the problem's BF16
-        weights from them."""
+        """One Adam step (or, with ``optimizer="sr"``, one natural-gradient
+        step params -= lr delta on the tied parameters) on the master
+        parameters, then the problem's BF16 weights from them."""
         self.t += 1
         lib = _lib.load()
+        if self.optimizer == "sr":
+            off, mem, m = self._orbits
+            delta = self._sr_direction()
+            _lib.check(lib.gfmc_orbit_axpy(self.n_params, _lib.ptr(self.params), m,
+                                           _lib.ptr(off), _lib.ptr(mem), _lib.ptr(delta),
+                                           -self.lr, _lib.stream()), "gfmc_orbit_axpy")
+            self.push()
+            return
         _lib.check(lib.gfmc_adam_step(self.n_params, _lib.ptr(self.params), _lib.ptr(self.grad),
                                       _lib.ptr(self.m), _lib.ptr(self.v), self.t, self.lr,
                                       float(self.betas[0]), float(self.betas[1]), self.eps,
@@ -139,6 +217,7 @@ class MLPTrainer(RBMTrainer):
         self.v = torch.zeros_like(self.params)
         self.t = 0
         self.lr, self.betas, self.eps, self.clip = float(lr), betas, float(eps), float(clip)
+        self._orbits = None
 
     def gradient(self, spins, out2):
         _lib.check(_lib.load().gfmc_mlp_vmin_grad(
@@ -155,6 +234,10 @@ class MLPTrainer(RBMTrainer):
 
 def _trainer(problem, guide, **kw):
     if isinstance(guide, MLPSpinHalf):
+        if kw.pop("optimizer", "adam") != "adam":
+            raise ValueError("the MLP guide trains with Adam")
+        for k in ("sr_shift", "sr_abs", "sr_iter", "sr_max_norm"):
+            kw.pop(k, None)
         return MLPTrainer(problem, guide, **kw)
     return RBMTrainer(problem, guide, **kw)
 
@@ -173,7 +256,9 @@ def vmin_gradient(hamiltonian, rbm, spins, clip=0.0):
 
 def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, step_size=0.2,
                    lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0, rng_key=0,
-                   warmup_steps=40, sampler="self", verbosity=0, trainer=None):
+                   warmup_steps=40, sampler="self", verbosity=0, trainer=None,
+                   optimizer="adam", sr_shift=1e-2, sr_abs=1e-4, sr_iter=500,
+                   sr_max_norm=0.02):
     """Variance minimisation of the network guide ``rbm`` (RBMSpinHalf or
     MLPSpinHalf) on GFMC samples.
 
@@ -184,8 +269,12 @@ def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, s
     psi_RBM phi_0 but can freeze when the RBM suppresses every move.
     ``warmup_steps`` branching steps, then per iteration ``steps_per_iter``
     steps followed by one gradient + Adam update on the resampled
-    population.  ``trainer`` continues an earlier run (its parameters and
-    Adam moments).  Returns ``(trained guide, history)``: arrays
+    population.  ``optimizer="sr"`` replaces Adam by natural-gradient steps
+    (stochastic reconfiguration: the covariance S of the samples' tied
+    log-derivatives, (S + sr_shift diag S + sr_abs) delta = g by conjugate
+    gradients, scaled so that the rms change of log psi over the samples is at
+    most ``sr_max_norm``; params -= lr delta) for a tied guide.  ``trainer`` continues
+    an earlier run (its parameters and optimizer state).  Returns ``(trained guide, history)``: arrays
     ``mean_e`` and ``variance`` (of the RBM's local energies, before each
     update), ``seconds`` (GPU wall time), ``walker_steps`` and ``trainer``.
     """
@@ -194,7 +283,9 @@ def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, s
     from .problem import get_problem
     if trainer is None:
         p = Problem(hamiltonian, rbm, _lib.GFMC_FP32)
-        trainer = _trainer(p, rbm, lr=lr, betas=betas, eps=eps, clip=clip)
+        trainer = _trainer(p, rbm, lr=lr, betas=betas, eps=eps, clip=clip, optimizer=optimizer,
+                           sr_shift=sr_shift, sr_abs=sr_abs, sr_iter=sr_iter,
+                           sr_max_norm=sr_max_norm)
     else:
         p = trainer.p
         trainer.lr, trainer.betas, trainer.eps, trainer.clip = float(lr), betas, float(eps), \

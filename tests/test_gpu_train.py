@@ -191,3 +191,74 @@ def test_training_shrinks_variance_and_beats_the_plain_error_bar():
         assert abs(E - e0) < 4 * err + 2e-3, name
     assert res["trained"][1] < 0.5 * res["jastrow"][1]
     assert res["trained"][1] < 0.5 * res["rbm0"][1]
+
+
+def test_tied_rbm_natural_gradient_pieces_match_numpy(orc):
+    """The orbit-summed log-derivatives of a translation-invariant RBM
+    against the oracle's FP64 theta, the orbit sum and update, and the
+    natural-gradient solve against a dense FP64 solve of the same system."""
+    from pygfmc_amd import RBMSpinHalf
+    from pygfmc_amd.training import RBMTrainer, flat_params
+    from pygfmc_amd.problem import Problem, lower
+    say("tied rbm sr pieces")
+    L = square(4, 4)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    G = RBMSpinHalf.symmetric(L.extras["translations"], 3, seed=4, scale=0.3)
+    p = Problem(H, G, _lib.GFMC_FP32)
+    tr = RBMTrainer(p, G, optimizer="sr", sr_shift=1e-2, sr_abs=1e-4)
+    off, mem, m = tr._orbits
+    assert m == 3 * 16 + 3 + 1
+    n = 700
+    spins = orc.init_sz(16, n, 8, 3)
+    sp = torch.as_tensor(spins.view(np.int32), device="cuda")
+    # O against the oracle's theta
+    theta = orc.Problem(lower(H, G)).rbm_theta(spins)
+    x = orc.unpack(spins, 16)
+    t = np.tanh(theta)
+    HN = G.hidden * 16
+    d_full = np.concatenate([(t[:, :, None] * x[:, None, :]).reshape(n, HN), x, t], axis=1)
+    O_ref = np.zeros((n, m))
+    uniq, inv = np.unique(G.orbits, return_inverse=True)
+    for q in range(len(uniq)):
+        O_ref[:, q] = d_full[:, inv == q].sum(axis=1)
+    O = torch.empty((n, m), dtype=torch.float32, device="cuda")
+    th = torch.empty((n, G.hidden), dtype=torch.float32, device="cuda")
+    L_ = _lib.load()
+    _lib.check(L_.gfmc_rbm_orbit_O(p.handle, n, _lib.ptr(sp), m, _lib.ptr(off), _lib.ptr(mem),
+                                   _lib.ptr(O), _lib.ptr(th), _lib.stream()), "orbit_O")
+    np.testing.assert_allclose(O.cpu().numpy(), O_ref, atol=2e-4)
+    # orbit sum of a vector and the update
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=tr.n_params).astype(np.float32)
+    vd = torch.as_tensor(v, device="cuda")
+    out = torch.empty(m, dtype=torch.float32, device="cuda")
+    _lib.check(L_.gfmc_orbit_sum(tr.n_params, _lib.ptr(vd), m, _lib.ptr(off), _lib.ptr(mem),
+                                 _lib.ptr(out), _lib.stream()), "orbit_sum")
+    want = np.array([v[inv == q].astype(np.float64).sum() for q in range(m)])
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-5, atol=1e-5)
+    params = torch.as_tensor(flat_params(G), device="cuda")
+    step = torch.as_tensor(rng.normal(size=m).astype(np.float32), device="cuda")
+    _lib.check(L_.gfmc_orbit_axpy(tr.n_params, _lib.ptr(params), m, _lib.ptr(off), _lib.ptr(mem),
+                                  _lib.ptr(step), -0.5, _lib.stream()), "orbit_axpy")
+    np.testing.assert_allclose(params.cpu().numpy(),
+                               flat_params(G) - 0.5 * step.cpu().numpy()[inv], rtol=1e-6,
+                               atol=1e-6)
+    # the SR solve
+    g = torch.as_tensor(rng.normal(size=m).astype(np.float32), device="cuda")
+    delta = torch.empty(m, dtype=torch.float32, device="cuda")
+    work = torch.empty(int(L_.gfmc_sr_work_doubles(m)), dtype=torch.float64, device="cuda")
+    _lib.check(L_.gfmc_sr_solve(n, m, _lib.ptr(O), _lib.ptr(g), 1e-2, 1e-3, 500, 1e-10, 0.0,
+                                _lib.ptr(delta), _lib.ptr(work), _lib.stream()), "sr_solve")
+    Oh = O.cpu().numpy().astype(np.float64)
+    Oc = Oh - Oh.mean(axis=0)
+    S = Oc.T @ Oc / n
+    A = S + np.diag(1e-2 * np.diag(S) + 1e-3)
+    d_ref = np.linalg.solve(A, g.cpu().numpy().astype(np.float64))
+    np.testing.assert_allclose(delta.cpu().numpy(), d_ref, rtol=1e-4,
+                               atol=1e-4 * np.abs(d_ref).max())
+    # the trust region scales the same direction to sqrt(d^T S d) = max_norm
+    _lib.check(L_.gfmc_sr_solve(n, m, _lib.ptr(O), _lib.ptr(g), 1e-2, 1e-3, 500, 1e-10, 0.01,
+                                _lib.ptr(delta), _lib.ptr(work), _lib.stream()), "sr_solve")
+    d2 = delta.cpu().numpy().astype(np.float64)
+    assert np.sqrt(d2 @ S @ d2) == pytest.approx(min(0.01, np.sqrt(d_ref @ S @ d_ref)), rel=1e-3)
+    np.testing.assert_allclose(d2 / np.linalg.norm(d2), d_ref / np.linalg.norm(d_ref), atol=1e-3)

@@ -115,3 +115,47 @@ def test_more_than_one_device_is_refused():
 def test_step_offset_must_be_zero():
     with pytest.raises(ValueError, match="rng_key_next"):
         run_gfmc(None, 10, None, 0.2, None, [2], _step_offset=5)
+
+
+def test_square_translations_are_lattice_automorphisms():
+    L = square(5, 4)
+    T = L.extras["translations"]
+    assert T.shape == (20, 20)
+    edges = set(map(tuple, L.edges))
+    for r in range(20):
+        assert sorted(T[r]) == list(range(20))
+        for i, j in L.edges:
+            a, b = int(T[r, i]), int(T[r, j])
+            assert (min(a, b), max(a, b)) in edges
+
+
+def test_translation_invariant_rbm_ties_its_parameters():
+    from pygfmc_amd import RBMSpinHalf
+    from pygfmc_amd.training import flat_params
+    L = square(4, 4)
+    g = RBMSpinHalf.symmetric(L.extras["translations"], 3, seed=2, scale=0.3)
+    assert g.hidden == 48
+    flat = flat_params(g)
+    for o in np.unique(g.orbits):
+        vals = flat[g.orbits == o]
+        assert np.all(vals == vals[0])
+    assert len(np.unique(g.orbits)) == 3 * 16 + 3 + 1
+
+
+def test_rbm_from_jastrow_is_the_jastrow_guide():
+    """log psi_RBM - log psi_Jastrow is the same constant on every
+    configuration (the BF16 rounding of the filter weight moves beta to
+    lncosh(2 bf16(w)) / 2)."""
+    from helpers import basis, rbm_log_psi
+    from pygfmc_amd import JastrowSpinHalf, RBMSpinHalf
+    L = square(4, 4)
+    g = RBMSpinHalf.from_jastrow(L.edges, L.extras["translations"], 0.15, 4, scale=0.0)
+    w = float(g.W[0, 0])
+    beta_eff = np.log(np.cosh(2 * w)) / 2
+    assert abs(beta_eff - 0.15) < 2e-3
+    J = JastrowSpinHalf(16, L.edges, beta=beta_eff)
+    x = basis(16)[::97]
+    lj = -beta_eff * np.sum([x[:, i] * x[:, j] for i, j in L.edges], axis=0)
+    d = rbm_log_psi(g, x) - lj
+    np.testing.assert_allclose(d, d[0], atol=1e-9)
+    assert J.N == 16
