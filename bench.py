@@ -52,8 +52,8 @@ def parse():
                          "262 144 walkers)")
     ap.add_argument("--mlp-walkers", type=int, default=262144)
     ap.add_argument("--mlp-steps", type=int, default=3)
-    ap.add_argument("--race-train-iters", type=int, default=600)
-    ap.add_argument("--race-rbm-steps", type=int, default=300)
+    ap.add_argument("--race-train-iters", type=int, default=1500)
+    ap.add_argument("--race-rbm-steps", type=int, default=1000)
     ap.add_argument("--summarize-profile", default=None,
                     help="print a text summary of a rocprofv3 database and exit")
     ap.add_argument("--summarize-kernel", default="k_propagate<float, true")
@@ -466,16 +466,21 @@ def sigma_race(args):
     -1, J_z = 1, Sz = 0), p = [4, 16, 64, 128], 16 384 walkers, step 0.2:
     GPU-seconds until the jackknife error bar (20 blocks) of E at p = 128
     reaches that of the Jastrow-FP64 run (1000 production steps after 400
-    warm-up steps), for the learned BF16 RBM guide (alpha = 4, trained on the
-    GPU by local-energy variance minimisation on GFMC samples of the beta =
-    0.15 Jastrow guide) with and without its training time.  The RBM's time
-    to the target error bar is extrapolated from its own run by sigma^2 ~
-    1 / production steps.  Two Jastrow baselines: beta = 0.3 (the reference
-    configurations' guide) and beta = 0.15 (the best of a scan)."""
+    warm-up steps), for the learned BF16 RBM guide (alpha = 4: four
+    translation-invariant filters, 400 hidden units; started as the
+    nearest-neighbour Jastrow guide and trained on the GPU by local-energy
+    variance minimisation with natural-gradient steps on the walkers its own
+    GFMC generates) with and without its training time.  The RBM's time to
+    the target error bar is extrapolated from its own run by sigma^2 ~ 1 /
+    production steps.  Two Jastrow baselines: beta = 0.15 (the best of a
+    scan) and beta = 0.3 (the reference configurations' guide, which is
+    biased on this lattice at this walker count: population-control bias
+    from its large local-energy variance)."""
     import torch
 
     from pygfmc_amd import (JastrowSpinHalf, RBMSpinHalf, SpinHalf, prepare_walkers_sz, square,
                             train_variance)
+    from pygfmc_amd.problem import get_problem
     lat = square(10, 10)
     P = [4, 16, 64, 128]
     nw = 16384
@@ -485,35 +490,43 @@ def sigma_race(args):
            "exact_E": -67.1549, "exact_source": "Sandvik, PRB 56, 11678 (1997): E/N = -0.671549 "
                                                  "at L = 10"}
     jas = {}
-    for beta in (0.3, 0.15):
+    for beta in (0.15, 0.3):
         G = JastrowSpinHalf(100, lat.edges, beta=beta, dtype=np.float64)
         w = prepare_walkers_sz(nw, 100, 0)
         secs, E, err, st = timed_run(w, 1000, H64, G, P, 1, 400)
         jas[beta] = dict(beta=beta, gpu_s=secs, E=E, sigma=err, steps=1000, warmup=400,
-                         walkers=nw, evals_per_step=st["evals_per_step_mean"])
-    out["jastrow_fp64_beta0.3"] = jas[0.3]
+                         walkers=nw, evals_per_step=st["evals_per_step_mean"],
+                         ms_per_step=st["runtime"] / 1000 * 1e3)
+        progress(f"jastrow beta={beta}: E {E:.4f} +- {err:.4f}")
     out["jastrow_fp64_beta0.15"] = jas[0.15]
-    # training
-    sampler = JastrowSpinHalf(100, lat.edges, beta=0.15, dtype=np.float32)
-    rbm0 = RBMSpinHalf(100, alpha=4, seed=0, scale=0.01)
+    out["jastrow_fp64_beta0.3"] = jas[0.3]
+    # training: the RBM starts as the beta = 0.15 Jastrow guide
+    rbm0 = RBMSpinHalf.from_jastrow(lat.edges, lat.extras["translations"], 0.15, 4, seed=1,
+                                    scale=0.01)
     w = prepare_walkers_sz(4096, 100, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     G, hist = train_variance(H32, rbm0, w, n_iter=args.race_train_iters, steps_per_iter=2,
-                             lr=1e-3, clip=3.0, rng_key=3, sampler=sampler)
+                             lr=1.0, clip=3.0, rng_key=3, sampler="self", warmup_steps=100,
+                             optimizer="sr", sr_shift=1e-2, sr_abs=1e-4, sr_max_norm=0.02)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     v = hist["variance"]
+    progress(f"rbm trained: variance {v[:5].mean():.3f} -> {v[-20:].mean():.3f} in {train_s:.1f}s")
     # production with the trained guide
     w = prepare_walkers_sz(nw, 100, 0)
     warm = 100
     secs, E, err, st = timed_run(w, args.race_rbm_steps, H32, G, P, 1, warm)
     frac_prod = args.race_rbm_steps / (args.race_rbm_steps + warm)
-    out["rbm_bf16"] = dict(alpha=4, hidden=400, train_gpu_s=train_s,
-                           train_iters=args.race_train_iters, train_walkers=4096,
-                           variance_start=float(v[:5].mean()), variance_end=float(v[-20:].mean()),
-                           gpu_s=secs, E=E, sigma=err, steps=args.race_rbm_steps, warmup=warm,
-                           walkers=nw, evals_per_step=st["evals_per_step_mean"])
+    rbm = dict(alpha=4, hidden=400, filters=4, train_gpu_s=train_s,
+               train_iters=args.race_train_iters, train_walkers=4096, optimizer="natural gradient",
+               variance_start=float(v[:5].mean()), variance_end=float(v[-20:].mean()),
+               gpu_s=secs, E=E, sigma=err, steps=args.race_rbm_steps, warmup=warm, walkers=nw,
+               evals_per_step=st["evals_per_step_mean"],
+               ms_per_step=st["runtime"] / args.race_rbm_steps * 1e3)
+    # the propagation kernel's roofline on this workload and the CPU oracle
+    rbm["roofline"], rbm["cpu_baseline"] = rbm_kernel_measure(H32, G, args.cpu_seconds)
+    out["rbm_bf16"] = rbm
     for beta, J in jas.items():
         t_warm = secs * (1 - frac_prod)
         t_need = t_warm + secs * frac_prod * (err / J["sigma"]) ** 2
@@ -522,28 +535,92 @@ def sigma_race(args):
             target_sigma=J["sigma"], jastrow_gpu_s=J["gpu_s"], rbm_gpu_s=t_need,
             rbm_gpu_s_with_training=t_need + train_s,
             speedup=J["gpu_s"] / t_need, speedup_with_training=J["gpu_s"] / (t_need + train_s),
-            energy_z=float(z))
+            energy_z=float(z),
+            per_step_variance_ratio=(J["sigma"] ** 2 * J["steps"]) / (err ** 2 * rbm["steps"]),
+            per_step_cost_ratio=rbm["ms_per_step"] / J["ms_per_step"])
     return out
 
 
-def cpu_baseline(H, G, chain, N, beta, alw, seed, seconds):
+def rbm_kernel_measure(H, G, cpu_seconds):
+    """k_propagate_rbh on 16 384 walkers of the trained guide, equilibrated:
+    average launch time by HIP events on the launch stream against the
+    algorithmic FP32 work of the reference algorithm (every slot's log-ratio
+    over all hidden units: n_edges H 8 flop per rate evaluation), and the FP64
+    oracle propagating a bounded sample of the same walkers on the host."""
+    import torch
+
+    from pygfmc_amd import prepare_walkers_sz
+    from pygfmc_amd.engine import Chain
+    from pygfmc_amd.problem import get_problem
+    p = get_problem(H, G)
+    nw = 16384
+    ch = Chain(p, prepare_walkers_sz(nw, H.N, 5))
+    _, e, _ = p.rates(ch.spins, want_rates=False)
+    alw = float(torch.mean(-e).item())
+    ch.steps(200, 0.2, alw, 9, 0)
+    n_k = 10
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(n_k)]
+    out2 = torch.empty(2, dtype=torch.float64, device=ch.spins.device)
+    ch.evals.zero_()
+    for k in range(n_k):
+        ch.step(0.2, alw, 9, 1000 + k, out2, events=ev[k])
+    torch.cuda.synchronize()
+    t = float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
+    evals = float(ch.evals.to(torch.float64).sum().item()) / n_k
+    flop = evals * len(H.edge_i) * G.hidden * 8.0
+    roof = {"bound": "mfma", "compute_unit": "FP32 vector ALU (no GEMM in the hop; the MI355X "
+                                              "FP32 vector and matrix peaks are both 157.3 "
+                                              "TFLOP/s)",
+            "achieved": flop / t / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+            "frac": flop / t / 1e12 / 157.3, "traffic": None,
+            "kernel": "k_propagate_rbh<7>", "kernel_avg_ms": t * 1e3,
+            "evals_per_walker_step": evals / nw, "flop_per_eval": len(H.edge_i) * G.hidden * 8.0,
+            "transcendentals_per_eval": len(H.edge_i) * G.hidden * 2.0}
+    cpu = cpu_baseline(H, G, ch, H.N, 0.2, alw, 9, cpu_seconds)
+    return roof, cpu
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo, as lscpu reports it)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(H, G, chain, N, beta, alw, seed, seconds, beta_sample=None, evals_per_ws=None,
+                 max_walkers=None):
     """The FP64 oracle on the host cores: propagation of a bounded sample of
-    the equilibrated walkers (same Hamiltonian, guide and step)."""
+    the equilibrated walkers (same Hamiltonian, guide and step).  With
+    ``beta_sample`` < ``beta`` (a guide too slow on the host for whole
+    steps) the sample propagates for beta_sample only and walker-steps/s is
+    the oracle's rate evaluations per second divided by the GPU run's
+    evaluations per walker-step ``evals_per_ws``."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import oracle
     from pygfmc_amd.problem import lower
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     prob = oracle.Problem(lower(H, G))
     spins = chain.spins.cpu().numpy().view(np.uint32)
-    n = 16
+    if max_walkers:
+        spins = spins[:max_walkers]
+    bs = beta if beta_sample is None else beta_sample
+    n = min(16, len(spins))
     done = 0
+    evals = 0
     t0 = time.perf_counter()
     step = 0
     while True:
         take = spins[:n]
-        prob.propagate(take, np.zeros(n), np.zeros(n), beta, alw, seed, 10_000_000 + step, 0,
-                       n_threads=threads)
+        _, _, _, ev = prob.propagate(take, np.zeros(n), np.zeros(n), bs, alw, seed,
+                                     10_000_000 + step, 0, n_threads=threads)
         done += n
+        evals += int(ev.sum())
         step += 1
         el = time.perf_counter() - t0
         if el > seconds:
@@ -551,9 +628,18 @@ def cpu_baseline(H, G, chain, N, beta, alw, seed, seconds):
         if el < seconds / 8 and n < len(spins):
             n = min(len(spins), n * 2)
     el = time.perf_counter() - t0
-    return {"value": done / el, "unit": "walker-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{done} walker-propagations of the equilibrated bench walkers "
-                      f"({el:.1f} s, FP64 C oracle, OpenMP over walkers)"}
+    if beta_sample is None:
+        value = done / el
+        sample = (f"{done} walker-propagations of the equilibrated bench walkers ({el:.1f} s, "
+                  f"FP64 C oracle, OpenMP over walkers)")
+    else:
+        value = evals / el / evals_per_ws
+        sample = (f"{done} walker-propagations over imaginary time {bs:g} of the equilibrated "
+                  f"bench walkers ({evals} rate evaluations in {el:.1f} s, FP64 C oracle, OpenMP "
+                  f"over walkers), scaled by the GPU run's {evals_per_ws:.2f} evaluations per "
+                  f"walker-step")
+    return {"value": value, "unit": "walker-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "sample": sample}
 
 
 if __name__ == "__main__":

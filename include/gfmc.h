@@ -460,6 +460,18 @@ int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *
  * blocks through the network (each 16 x h1 x h2 multiply-adds on MFMA),
  * out[1] bulk-synchronous hop iterations. */
 int gfmc_mlp_counters(const gfmc_problem *p, int64_t *out);
+/* The MLP guide's propagation workspace is the caller's (no counterpart in
+ * the reference, whose guide evaluation is XLA's): per walker the exact FP64
+ * layer-1 pre-activations, the candidate slots and the hold state, per hop
+ * the 16-row block map and one FP32 log psi per row.  gfmc_mlp_work_bytes:
+ * the bytes for n_walkers (-1 if not an MLP problem or too large);
+ * gfmc_mlp_set_work: attach a 256-byte aligned device buffer of at least
+ * that size, good for calls with up to n_walkers walkers on this problem
+ * (rates, log psi, propagation, the training gradient) until the next
+ * gfmc_mlp_set_work; it zeroes the propagation counters (stream-ordered). */
+int64_t gfmc_mlp_work_bytes(const gfmc_problem *p, int64_t n_walkers);
+int gfmc_mlp_set_work(gfmc_problem *p, void *work, int64_t bytes, int64_t n_walkers,
+                      void *stream);
 
 /* h1*N + h1 + h2*h1 + 2*h2 + 1 of an MLP problem, -1 otherwise. */
 int64_t gfmc_mlp_n_params(const gfmc_problem *p);

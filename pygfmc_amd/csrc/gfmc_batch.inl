@@ -108,8 +108,9 @@ int gfmc_batch_create(const gfmc_problem *const *ps, int32_t n_lat, const int64_
     for (int l = 0; l < n_lat; ++l) {
         const gfmc_problem *p = ps[l];
         if (!p) return fail(GFMC_EINVAL, "gfmc_batch_create: NULL problem");
-        if (p->dp.guide == GFMC_GUIDE_RBM)
-            return fail(GFMC_EUNSUP, "gfmc_batch_create: RBM-guided problems do not batch");
+        if (p->dp.guide == GFMC_GUIDE_RBM || p->dp.guide == GFMC_GUIDE_MLP)
+            return fail(GFMC_EUNSUP, "gfmc_batch_create: network-guided (RBM, MLP) problems do "
+                                     "not batch");
         if (p->precision != prec)
             return fail(GFMC_EINVAL, "gfmc_batch_create: every lattice needs one precision");
         if (p->dp.fast != fast)

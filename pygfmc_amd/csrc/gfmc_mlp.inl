@@ -51,6 +51,7 @@ struct MlpWs {
     int32_t *part;       // [cap / 1024 + 1] scan partials
     int32_t *total;      // [1]         row blocks of the iteration
     float *logpsi;       // [cap * blocks per walker * 16]
+    long long *counters; // [2]         row blocks / hop iterations propagated (device sums)
 };
 
 // rows of one training evaluation (gfmc_mlp_vmin_grad): the upstream
@@ -66,13 +67,13 @@ struct MlpTws {
     float *wpart;        // [waves][h2 + 1]
 };
 
+// the problem's view of the caller's propagation workspace (gfmc_mlp_set_work)
 struct MlpWork {
-    int64_t cap = 0;
+    int64_t cap = 0;             // walkers the attached workspace holds (0: none attached)
     int bpw = 0;                 // blocks per walker at most
-    int64_t blocks_done = 0, iters_done = 0;   // row blocks / hop iterations propagated
+    int counting = 0;            // the propagation adds to ws.counters
     MlpWs ws{};
-    std::vector<void *> allocs;
-    int32_t *host_total = nullptr;   // pinned
+    int32_t *host_total = nullptr;   // pinned, allocated with the problem
 };
 
 // s(z) = sign(z) (T_j + f D_j): t = min(256 |z|, 2048 - 2^-13), j = floor(t),
@@ -213,6 +214,14 @@ __global__ __launch_bounds__(1024) void k_mlp_scan2(int n, MlpWs W) {
         __syncthreads();
     }
     if (threadIdx.x == 0) W.total[0] = carry;
+}
+
+// the propagation's counters: row blocks and (non-empty) hop iterations
+__global__ void k_mlp_count(MlpWs W) {
+    if (threadIdx.x == 0 && W.total[0] > 0) {
+        W.counters[0] += W.total[0];
+        W.counters[1] += 1;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_mlp_scan3(int64_t nw, MlpWs W) {
@@ -639,57 +648,38 @@ __global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_rates(DevProblem P, Table
 static MlpWork *new_mlp_work(int blocks_per_walker) {
     MlpWork *m = new MlpWork;
     m->bpw = blocks_per_walker;
+    if (hipHostMalloc((void **)&m->host_total, 16, hipHostMallocDefault) != hipSuccess)
+        m->host_total = nullptr;
     return m;
 }
 
 static void free_mlp_work(MlpWork *m) {
     if (!m) return;
-    for (void *a : m->allocs) (void)hipFree(a);
     if (m->host_total) (void)hipHostFree(m->host_total);
     delete m;
 }
 
-static int mlp_reserve(const gfmc_problem *p, int64_t nw) {
-    MlpWork *m = p->mlp;
-    if (m->cap >= nw) return 0;
-    if ((int64_t)nw * m->bpw > 0x7fffffffLL / MLP_RB)
-        return fail(GFMC_EUNSUP, "MLP guide: too many walkers x candidate rows for one launch");
-    for (void *a : m->allocs) (void)hipFree(a);
-    m->allocs.clear();
-    m->cap = 0;
-    const DevProblem &P = p->dp;
-    auto get = [&](size_t bytes, void **dst) -> int {
-        void *a = nullptr;
-        if (hipMalloc(&a, bytes ? bytes : 16) != hipSuccess)
-            return fail(GFMC_ENOMEM, "MLP guide: workspace allocation failed");
-        m->allocs.push_back(a);
-        *dst = a;
-        return 0;
-    };
-    MlpWs &W = m->ws;
-    const size_t n = (size_t)nw, nb = n * (size_t)m->bpw;
-    int rc = 0;
-    if (!rc) rc = get(n * P.mlp_h1 * 8, (void **)&W.z1);
-    if (!rc) rc = get(n * (size_t)(P.n_off > 0 ? P.n_off : 1) * 4, (void **)&W.cand);
-    if (!rc) rc = get(n * 4, (void **)&W.cnt);
-    if (!rc) rc = get(n * 4, (void **)&W.active);
-    if (!rc) rc = get(n * 8, (void **)&W.t);
-    if (!rc) rc = get(n * 8, (void **)&W.lw);
-    if (!rc) rc = get(n * 8, (void **)&W.eL);
-    if (!rc) rc = get(n * 4, (void **)&W.it);
-    if (!rc) rc = get(n * 4, (void **)&W.blk_off);
-    if (!rc) rc = get(nb * 4, (void **)&W.blk_walker);
-    if (!rc) rc = get((n / 1024 + 2) * 4, (void **)&W.part);
-    if (!rc) rc = get(16, (void **)&W.total);
-    if (!rc) rc = get(nb * MLP_RB * 4, (void **)&W.logpsi);
-    if (rc) {
-        for (void *a : m->allocs) (void)hipFree(a);
-        m->allocs.clear();
-        return rc;
+// layout of the propagation workspace for nw walkers: 256-byte aligned
+// pieces in the order of MlpWs; returns the total bytes (offsets into off)
+static size_t mlp_work_layout(const DevProblem &P, int bpw, int64_t nw, size_t *off) {
+    const size_t n = (size_t)nw, nb = n * (size_t)bpw;
+    const size_t sizes[14] = {n * P.mlp_h1 * 8, n * (size_t)(P.n_off > 0 ? P.n_off : 1) * 4,
+                              n * 4, n * 4, n * 8, n * 8, n * 8, n * 4, n * 4, nb * 4,
+                              (n / 1024 + 2) * 4, 16, nb * MLP_RB * 4, 16};
+    size_t o = 0;
+    for (int i = 0; i < 14; ++i) {
+        if (off) off[i] = o;
+        o += (sizes[i] + 255) / 256 * 256;
     }
-    if (!m->host_total) HIP_TRY(hipHostMalloc((void **)&m->host_total, 16, hipHostMallocDefault));
-    m->cap = nw;
-    return 0;
+    return o;
+}
+
+static int mlp_reserve(const gfmc_problem *p, int64_t nw) {
+    const MlpWork *m = p->mlp;
+    if (m->cap >= nw) return 0;
+    return fail(GFMC_EINVAL, "MLP guide: the attached workspace holds " + std::to_string(m->cap) +
+                                 " walkers; attach gfmc_mlp_work_bytes(problem, " +
+                                 std::to_string(nw) + ") bytes with gfmc_mlp_set_work");
 }
 
 static size_t mlp_walker_lds(const gfmc_problem *p) { return (size_t)MLP_SW * p->dp.rbm_wave_bytes; }
@@ -704,12 +694,12 @@ static unsigned mlp_walker_grid(const gfmc_problem *p, int64_t nw) {
     return (unsigned)(need < cap ? need : cap);
 }
 
+// the layer-2 GEMM over the row blocks the last scan counted (W.total, read
+// on the device): a fixed persistent grid, so the host need not know the count
 template <bool TRAIN>
-static unsigned launch_mlp_gemm(const gfmc_problem *p, const uint32_t *spins, int total,
-                                hipStream_t st, const MlpTws &T = MlpTws{}) {
-    const int tiles = (total + MLP_GW - 1) / MLP_GW;
-    const int cap = p->n_cu * 4;
-    const unsigned grid = (unsigned)(tiles < cap ? (tiles > 0 ? tiles : 1) : cap);
+static unsigned launch_mlp_gemm(const gfmc_problem *p, const uint32_t *spins, hipStream_t st,
+                                const MlpTws &T = MlpTws{}) {
+    const unsigned grid = (unsigned)(p->n_cu * 4);
     const size_t lds = mlp_gemm_lds(p);
     const MlpWs &W = p->mlp->ws;
     const dim3 g(grid), b(MLP_GW * WAVE);
@@ -722,18 +712,26 @@ static unsigned launch_mlp_gemm(const gfmc_problem *p, const uint32_t *spins, in
     return grid;
 }
 
-// the row blocks of the active walkers; returns their number (host sync)
-static int mlp_scan(const gfmc_problem *p, int64_t nw, hipStream_t st, int *total) {
+// the row blocks of the active walkers, their count to W.total (device)
+static void mlp_scan(const gfmc_problem *p, int64_t nw, hipStream_t st) {
     const MlpWs &W = p->mlp->ws;
     const unsigned nb = (unsigned)((nw + 1023) / 1024);
     hipLaunchKernelGGL(k_mlp_scan1, dim3(nb), dim3(1024), 0, st, nw, W);
     hipLaunchKernelGGL(k_mlp_scan2, dim3(1), dim3(1024), 0, st, (int)nb, W);
     hipLaunchKernelGGL(k_mlp_scan3, dim3(nb), dim3(1024), 0, st, nw, W);
-    HIP_TRY(hipMemcpyAsync(p->mlp->host_total, W.total, 4, hipMemcpyDeviceToHost, st));
+}
+
+// the count of the last scan on the host (one synchronisation)
+static int mlp_read_total(const gfmc_problem *p, hipStream_t st, int *total) {
+    if (!p->mlp->host_total) return fail(GFMC_ENOMEM, "MLP guide: no pinned host word");
+    HIP_TRY(hipMemcpyAsync(p->mlp->host_total, p->mlp->ws.total, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     *total = p->mlp->host_total[0];
     return 0;
 }
+
+// hop iterations enqueued between two looks at the row count
+#define MLP_CHECK_EVERY 16
 
 static int mlp_start(const gfmc_problem *p, int64_t nw, const uint32_t *spins,
                      const double *log_w, int stay_only, hipStream_t st) {
@@ -754,21 +752,27 @@ static int mlp_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, dou
                                  "per hop and cannot be captured into a graph");
     int rc = mlp_start(p, nw, spins, log_w, 0, st);
     if (rc) return rc;
+    // hop iterations of every active walker (scan -> GEMM -> select), the
+    // GEMM and the select reading the row count on the device; after every
+    // MLP_CHECK_EVERY iterations the host looks at the count of the last
+    // scan (once all walkers are done the iterations are empty launches)
     for (int64_t iter = 0;; ++iter) {
-        int total = 0;
-        rc = mlp_scan(p, nw, st, &total);
-        if (rc) return rc;
-        if (total == 0) break;
-        p->mlp->blocks_done += total;
-        p->mlp->iters_done += 1;
-        if (iter >= 10000000)
-            return fail(GFMC_EINVAL, "MLP guide: a walker did not finish its branching step "
-                                     "in 10^7 hops (non-finite rates?)");
-        launch_mlp_gemm<false>(p, spins, total, st);
+        mlp_scan(p, nw, st);
+        hipLaunchKernelGGL(k_mlp_count, dim3(1), dim3(WAVE), 0, st, p->mlp->ws);
+        launch_mlp_gemm<false>(p, spins, st);
         hipLaunchKernelGGL(k_mlp_select, dim3(mlp_walker_grid(p, nw)), dim3(MLP_SW * WAVE),
                            mlp_walker_lds(p), st, p->dp, p->t32, nw, spins, log_w, e_loc, beta,
                            alw, k0, k1, step, tag, evals, p->mlp->ws);
         HIP_TRY(hipGetLastError());
+        if ((iter + 1) % MLP_CHECK_EVERY == 0) {
+            int total = 0;
+            rc = mlp_read_total(p, st, &total);
+            if (rc) return rc;
+            if (total == 0) break;
+            if (iter >= 10000000)
+                return fail(GFMC_EINVAL, "MLP guide: a walker did not finish its branching "
+                                         "step in 10^7 hops (non-finite rates?)");
+        }
     }
     return 0;
 }
@@ -780,10 +784,8 @@ static int mlp_rates(const gfmc_problem *p, int64_t nw, const uint32_t *spins, d
     const int stay_only = !(rates || e_loc || log_diff);
     int rc = mlp_start(p, nw, spins, nullptr, stay_only, st);
     if (rc) return rc;
-    int total = 0;
-    rc = mlp_scan(p, nw, st, &total);
-    if (rc) return rc;
-    launch_mlp_gemm<false>(p, spins, total, st);
+    mlp_scan(p, nw, st);
+    launch_mlp_gemm<false>(p, spins, st);
     hipLaunchKernelGGL(k_mlp_rates, dim3(mlp_walker_grid(p, nw)), dim3(MLP_SW * WAVE),
                        mlp_walker_lds(p), st, p->dp, p->t32, nw, spins, rates, e_loc, log_diff,
                        log_psi, p->mlp->ws);
@@ -795,7 +797,55 @@ extern "C" int gfmc_mlp_counters(const gfmc_problem *p, int64_t *out) {
     if (!p || !out) return fail(GFMC_EINVAL, "gfmc_mlp_counters: NULL argument");
     if (p->dp.guide != GFMC_GUIDE_MLP || !p->mlp)
         return fail(GFMC_EINVAL, "gfmc_mlp_counters: the problem's guide is not an MLP");
-    out[0] = p->mlp->blocks_done;
-    out[1] = p->mlp->iters_done;
+    if (p->mlp->cap == 0) {
+        out[0] = out[1] = 0;
+        return 0;
+    }
+    long long c[2];
+    HIP_TRY(hipMemcpy(c, p->mlp->ws.counters, sizeof c, hipMemcpyDeviceToHost));
+    out[0] = c[0];
+    out[1] = c[1];
+    return 0;
+}
+
+extern "C" int64_t gfmc_mlp_work_bytes(const gfmc_problem *p, int64_t n_walkers) {
+    if (!p || p->dp.guide != GFMC_GUIDE_MLP || !p->mlp || n_walkers < 1) return -1;
+    if (n_walkers * p->mlp->bpw > 0x7fffffffLL / MLP_RB) return -1;
+    return (int64_t)mlp_work_layout(p->dp, p->mlp->bpw, n_walkers, nullptr);
+}
+
+extern "C" int gfmc_mlp_set_work(gfmc_problem *p, void *work, int64_t bytes, int64_t n_walkers,
+                                 void *stream) {
+    if (!p || p->dp.guide != GFMC_GUIDE_MLP || !p->mlp)
+        return fail(GFMC_EINVAL, "gfmc_mlp_set_work: the problem's guide is not an MLP");
+    if (n_walkers < 1 || !work) return fail(GFMC_EINVAL, "gfmc_mlp_set_work: bad workspace");
+    const int64_t need = gfmc_mlp_work_bytes(p, n_walkers);
+    if (need < 0) return fail(GFMC_EUNSUP, "gfmc_mlp_set_work: too many walkers x candidate "
+                                           "rows for one launch");
+    if (bytes < need)
+        return fail(GFMC_EINVAL, "gfmc_mlp_set_work: " + std::to_string(n_walkers) +
+                                     " walkers need " + std::to_string(need) + " bytes");
+    if (((uintptr_t)work & 255) != 0)
+        return fail(GFMC_EINVAL, "gfmc_mlp_set_work: the workspace must be 256-byte aligned");
+    size_t off[14];
+    mlp_work_layout(p->dp, p->mlp->bpw, n_walkers, off);
+    unsigned char *b = (unsigned char *)work;
+    MlpWs &W = p->mlp->ws;
+    W.z1 = (double *)(b + off[0]);
+    W.cand = (int32_t *)(b + off[1]);
+    W.cnt = (int32_t *)(b + off[2]);
+    W.active = (int32_t *)(b + off[3]);
+    W.t = (double *)(b + off[4]);
+    W.lw = (double *)(b + off[5]);
+    W.eL = (double *)(b + off[6]);
+    W.it = (uint32_t *)(b + off[7]);
+    W.blk_off = (int32_t *)(b + off[8]);
+    W.blk_walker = (int32_t *)(b + off[9]);
+    W.part = (int32_t *)(b + off[10]);
+    W.total = (int32_t *)(b + off[11]);
+    W.logpsi = (float *)(b + off[12]);
+    W.counters = (long long *)(b + off[13]);
+    HIP_TRY(hipMemsetAsync(W.counters, 0, 16, (hipStream_t)stream));
+    p->mlp->cap = n_walkers;
     return 0;
 }

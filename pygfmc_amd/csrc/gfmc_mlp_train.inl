@@ -394,7 +394,10 @@ int gfmc_mlp_vmin_grad(const gfmc_problem *p, int64_t n, const uint32_t *spins, 
     MlpTrainWork *t = p->mlpt;
     rc = mlp_rates(p, n, spins, t->rates, t->e, nullptr, nullptr, st);
     if (rc) return rc;
-    const int64_t rows = (int64_t)p->mlp->host_total[0] * MLP_RB;
+    int blocks = 0;
+    rc = mlp_read_total(p, st, &blocks);
+    if (rc) return rc;
+    const int64_t rows = (int64_t)blocks * MLP_RB;
     const int64_t splits = (rows + MLP_KS - 1) / MLP_KS;
     const int ld1 = (N + 1 + 31) / 32 * 32;
     const int64_t p2n = splits * h2 * (int64_t)(h1 + 32), p1n = splits * h1 * (int64_t)ld1;
@@ -405,7 +408,7 @@ int gfmc_mlp_vmin_grad(const gfmc_problem *p, int64_t n, const uint32_t *spins, 
                        out2);
     hipLaunchKernelGGL(k_mlp_coef, dim3(grid1d(n, 256)), dim3(256), 0, st, P, n, t->rates, t->g,
                        p->mlp->ws, t->coef);
-    const unsigned fgrid = launch_mlp_gemm<true>(p, spins, (int)(rows / MLP_RB), st, t->T);
+    const unsigned fgrid = launch_mlp_gemm<true>(p, spins, st, t->T);
     const int n_waves = (int)fgrid * MLP_GW;
     const int tiles2 = (int)(splits * (h2 / 32) * (h1 / 32 + 1));
     hipLaunchKernelGGL(k_mlp_gw2, dim3((unsigned)((tiles2 + 3) / 4)), dim3(256), 0, st, h1, h2,

@@ -432,11 +432,24 @@ __global__ __launch_bounds__(256) void k_rbm_orbit_O(DevProblem P, int64_t n, co
 // 16 x 16 tiles (FP64 accumulation, samples in order), then conjugate
 // gradients on (S + eps_rel diag S + eps_abs) d = g in one workgroup
 #define SR_T 16
-__global__ __launch_bounds__(256) void k_sr_mean(int64_t n, int m, const float *O, double *mean) {
+#define SR_MC 32   // sample chunks of the column means
+__global__ __launch_bounds__(256) void k_sr_mean_part(int64_t n, int m, const float *O,
+                                                      double *part) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (q >= m) return;
+    const int64_t per = (n + SR_MC - 1) / SR_MC, i0 = c * per, i1 = i0 + per < n ? i0 + per : n;
+    double s = 0.0;
+    for (int64_t i = i0; i < i1; ++i) s += (double)O[i * m + q];
+    part[(int64_t)c * m + q] = s;
+}
+
+__global__ __launch_bounds__(256) void k_sr_mean(int64_t n, int m, const double *part,
+                                                 double *mean) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m) return;
     double s = 0.0;
-    for (int64_t i = 0; i < n; ++i) s += (double)O[i * m + q];
+    for (int c = 0; c < SR_MC; ++c) s += part[(int64_t)c * m + q];
     mean[q] = s / (double)n;
 }
 
@@ -472,14 +485,18 @@ __device__ double sr_block_sum(double v, double *sh) {
     return t;
 }
 
-// y = (S + eps_rel diag S + eps_abs) x, rows over the block's threads
+// y = (S + eps_rel diag S + eps_abs) x: a wavefront per row, lanes over the
+// columns (coalesced), a butterfly per row (a fixed order)
 __device__ void sr_apply(int m, const double *S, double eps_rel, double eps_abs, const double *x,
                          double *y) {
-    for (int r = threadIdx.x; r < m; r += blockDim.x) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    for (int r = wv; r < m; r += nwv) {
         const double *row = S + (int64_t)r * m;
         double acc = 0.0;
-        for (int c = 0; c < m; ++c) acc += row[c] * x[c];
-        y[r] = acc + (eps_rel * row[r] + eps_abs) * x[r];
+        for (int c = lane; c < m; c += 64) acc += row[c] * x[c];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, WAVE);
+        if (lane == 0) y[r] = acc + (eps_rel * row[r] + eps_abs) * x[r];
     }
     __syncthreads();
 }
@@ -669,7 +686,7 @@ int gfmc_rbm_orbit_O(const gfmc_problem *p, int64_t n, const uint32_t *spins, in
 
 int64_t gfmc_sr_work_doubles(int32_t m) {
     if (m < 1 || m > 8192) return -1;
-    return (int64_t)m * m + 5 * (int64_t)m;
+    return (int64_t)m * m + (5 + SR_MC) * (int64_t)m;
 }
 
 int gfmc_sr_solve(int64_t n, int32_t m, const float *O, const float *g, double eps_rel,
@@ -682,8 +699,11 @@ int gfmc_sr_solve(int64_t n, int32_t m, const float *O, const float *g, double e
         return fail(GFMC_EINVAL, "gfmc_sr_solve: eps, tol and max_norm must be >= 0");
     const hipStream_t st = (hipStream_t)stream;
     double *S = work, *mean = work + (int64_t)m * m, *x = mean + m, *r = x + m, *pv = r + m,
-           *Ap = pv + m;
-    hipLaunchKernelGGL(k_sr_mean, dim3(grid1d(m, 256)), dim3(256), 0, st, n, m, O, mean);
+           *Ap = pv + m, *part = Ap + m;
+    hipLaunchKernelGGL(k_sr_mean_part, dim3(grid1d(m, 256), SR_MC), dim3(256), 0, st, n, m, O,
+                       part);
+    hipLaunchKernelGGL(k_sr_mean, dim3(grid1d(m, 256)), dim3(256), 0, st, n, m,
+                       (const double *)part, mean);
     const unsigned tiles = (unsigned)((m + SR_T - 1) / SR_T);
     hipLaunchKernelGGL(k_sr_cov, dim3(tiles, tiles), dim3(SR_T * SR_T), 0, st, n, m, O, mean, S);
     hipLaunchKernelGGL(k_sr_cg, dim3(1), dim3(1024), 0, st, m, (const double *)S, eps_rel,

@@ -102,6 +102,7 @@ class Chain:
                              f"{problem.n_sites}")
         self.p = problem
         self.nw = walkers.n_walkers
+        problem.reserve(self.nw)
         dev = walkers.spins.device
         self.spins = walkers.spins.clone()
         self.log_w = walkers.log_weight_dev.clone()

@@ -113,6 +113,7 @@ class OffDiagMeasurer:
                     raise ValueError(f"off-diagonal operator {name!r} has {op.N} sites, the "
                                      f"Hamiltonian {h.n_sites}")
                 prob = get_problem(op, guide, h.precision)
+                prob.reserve(self.n_walkers)
                 self.entries.append(SimpleNamespace(name=name, op=op, prob=prob, rec=None))
                 handle = prob.handle
             elif isinstance(op, (SpinHalf, Identity)):

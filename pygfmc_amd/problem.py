@@ -192,6 +192,27 @@ class Problem:
         self.n_words = info.n_words
         self.m_total = info.m_total
         self.guide_kind = int(info.guide)
+        self._mlp_buf = None
+        self._mlp_cap = 0
+
+    def reserve(self, n_walkers):
+        """Attach (or grow) the MLP guide's propagation workspace, a device
+        buffer this object owns (gfmc_mlp_work_bytes / gfmc_mlp_set_work),
+        for calls on up to ``n_walkers`` walkers; a no-op for the other
+        guides, whose kernels need no workspace."""
+        n = int(n_walkers)
+        if self.guide_kind != _lib.GUIDE_MLP or n <= self._mlp_cap:
+            return
+        lib = _lib.load()
+        nb = int(lib.gfmc_mlp_work_bytes(self.handle, n))
+        if nb < 0:
+            raise _lib.GfmcError(f"MLP guide: no workspace size for {n} walkers")
+        buf = torch.empty(nb + 256, dtype=torch.uint8, device=_lib.device())
+        off = (-buf.data_ptr()) % 256
+        _lib.check(lib.gfmc_mlp_set_work(self.handle, buf.data_ptr() + off, nb, n,
+                                         _lib.stream()), "gfmc_mlp_set_work")
+        self._mlp_buf = buf
+        self._mlp_cap = n
 
     def __del__(self):
         try:
@@ -226,6 +247,7 @@ class Problem:
     # -- calls ---------------------------------------------------------------
     def rates(self, spins, want_rates=True, want_eloc=True, want_logdiff=False):
         nw = spins.shape[0]
+        self.reserve(nw)
         dev = spins.device
         r = torch.empty((nw, self.m_total), dtype=torch.float64, device=dev) if want_rates else None
         e = torch.empty(nw, dtype=torch.float64, device=dev) if want_eloc else None
@@ -237,12 +259,14 @@ class Problem:
         return r, e, ld
 
     def log_psi(self, spins):
+        self.reserve(spins.shape[0])
         out = torch.empty(spins.shape[0], dtype=torch.float64, device=spins.device)
         _lib.check(_lib.load().gfmc_log_psi(self.handle, spins.shape[0], _lib.ptr(spins),
                                             _lib.ptr(out), _lib.stream()), "gfmc_log_psi")
         return out
 
     def propagate(self, spins, log_w, e_loc, beta, alw, seed, step, tag, evals=None):
+        self.reserve(spins.shape[0])
         _lib.check(_lib.load().gfmc_propagate(
             self.handle, spins.shape[0], _lib.ptr(spins), _lib.ptr(log_w), _lib.ptr(e_loc),
             float(beta), float(alw), int(seed) & 0xFFFFFFFFFFFFFFFF, int(step) & 0xFFFFFFFF,

@@ -57,7 +57,7 @@ class RBMTrainer:
     parameters, Adam moments, the gradient and its workspace."""
 
     def __init__(self, problem, rbm, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0,
-                 optimizer="adam", sr_shift=1e-2, sr_abs=1e-4, sr_iter=500, sr_max_norm=0.02):
+                 optimizer="adam", sr_shift=1e-2, sr_abs=1e-4, sr_iter=100, sr_max_norm=0.02):
         if not isinstance(rbm, RBMSpinHalf):
             raise TypeError(f"RBMTrainer trains an RBMSpinHalf, got {type(rbm).__name__}")
         if problem.info.guide != _lib.GUIDE_RBM:
@@ -145,7 +145,7 @@ class RBMTrainer:
                                         _lib.ptr(mem), _lib.ptr(self._O), _lib.ptr(self._theta),
                                         _lib.stream()), "gfmc_rbm_orbit_O")
         _lib.check(lib.gfmc_sr_solve(n, m, _lib.ptr(self._O), _lib.ptr(self._g_t),
-                                     self.sr_shift, self.sr_abs, self.sr_iter, 1e-6,
+                                     self.sr_shift, self.sr_abs, self.sr_iter, 1e-5,
                                      self.sr_max_norm, _lib.ptr(self._delta),
                                      _lib.ptr(self._sr_work),
                                      _lib.stream()), "gfmc_sr_solve")
@@ -218,8 +218,10 @@ class MLPTrainer(RBMTrainer):
         self.t = 0
         self.lr, self.betas, self.eps, self.clip = float(lr), betas, float(eps), float(clip)
         self._orbits = None
+        self.optimizer = "adam"
 
     def gradient(self, spins, out2):
+        self.p.reserve(spins.shape[0])
         _lib.check(_lib.load().gfmc_mlp_vmin_grad(
             self.p.handle, spins.shape[0], _lib.ptr(spins), self.clip, _lib.ptr(self.grad),
             _lib.ptr(out2), _lib.stream()), "gfmc_mlp_vmin_grad")
@@ -257,7 +259,7 @@ def vmin_gradient(hamiltonian, rbm, spins, clip=0.0):
 def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, step_size=0.2,
                    lr=1e-2, betas=(0.9, 0.999), eps=1e-8, clip=0.0, rng_key=0,
                    warmup_steps=40, sampler="self", verbosity=0, trainer=None,
-                   optimizer="adam", sr_shift=1e-2, sr_abs=1e-4, sr_iter=500,
+                   optimizer="adam", sr_shift=1e-2, sr_abs=1e-4, sr_iter=100,
                    sr_max_norm=0.02):
     """Variance minimisation of the network guide ``rbm`` (RBMSpinHalf or
     MLPSpinHalf) on GFMC samples.

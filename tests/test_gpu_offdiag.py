@@ -43,6 +43,7 @@ def ops8(dtype=np.float64):
 
 def apply_dev(prob, spins, seed, step, tag):
     nw = spins.shape[0]
+    prob.reserve(nw)      # the MLP guide's caller-owned workspace (no-op otherwise)
     s = to_dev(spins)
     out = torch.empty_like(s)
     vals = torch.empty(nw, dtype=torch.float64, device="cuda")
