@@ -30,6 +30,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=20,
                     help="untimed steps after the workload's own GFMC warm-up")
+    ap.add_argument("--settle", type=int, default=500,
+                    help="untimed production steps after the GFMC warm-up (the population "
+                         "keeps relaxing for a few hundred steps after the ramp)")
     ap.add_argument("--gfmc-warmup", type=int, default=1000,
                     help="the workload's GFMC warm-up ramp (driver.py DEFAULT_WARMUP_STEPS), "
                          "always run before any timing")
@@ -56,7 +59,7 @@ def parse():
     ap.add_argument("--race-rbm-steps", type=int, default=1000)
     ap.add_argument("--summarize-profile", default=None,
                     help="print a text summary of a rocprofv3 database and exit")
-    ap.add_argument("--summarize-kernel", default="k_propagate<float, true")
+    ap.add_argument("--summarize-kernel", default="k_propagate<float, 2, 64")
     ap.add_argument("--summarize-tail", default=None,
                     help="kernel-pass,graph launch counts of the propagation kernel")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -65,43 +68,76 @@ def parse():
     return ap.parse_args()
 
 
-def summarize_profile(db, prefix="k_propagate<float, true", tail=None):
-    """Text summary of a rocprofv3 database (rocpd SQLite) of this script:
-    per-kernel totals, and for the propagation kernel the average duration
-    of its last launches (``tail`` = the bench's kernel pass + graph region),
-    split into the kernel pass (events-timed) and the graph replays; for a
-    --pmc database the per-kernel averages of each counter."""
-    import sqlite3
-    con = sqlite3.connect(db)
-    cur = con.cursor()
-    lines = [f"database: {os.path.basename(db)}"]
-    rows = cur.execute("select name, total_calls, average, percentage from top_kernels "
-                       "order by total_duration desc limit 14").fetchall()
-    lines.append("| kernel | calls | avg us | % |")
-    lines.append("|---|---|---|---|")
-    for n, c, a, pc in rows:
-        short = n.split("(")[0]
-        lines.append(f"| {short} | {c} | {a:.1f} | {pc:.2f} |")
-    pmc = cur.execute("select kernel_name, counter_name, avg(value), count(*) from "
-                      "counters_collection group by kernel_name, counter_name").fetchall()
+def summarize_profile(path, prefix="k_propagate<float, 2, 64", tail=None):
+    """Text summary of a rocprofv3 CSV output directory (--output-format
+    csv) of this script: the kernel statistics table; for the propagation
+    kernel (name prefix ``prefix``) the average duration of its last
+    launches, ``tail`` = (kernel pass, graph replays) launch counts; for a
+    --pmc run each counter's average over the same last launches of that
+    kernel and over every dispatch of each other kernel."""
+    import csv
+    import glob
+
+    def rows(suffix):
+        out = []
+        for f in sorted(glob.glob(os.path.join(path, "**", "*" + suffix), recursive=True)):
+            with open(f, newline="") as fh:
+                out += list(csv.DictReader(fh))
+        return out
+
+    def short(name):
+        name = name.split("(")[0]
+        return name[5:] if name.startswith("void ") else name
+
+    lines = [f"profile: {os.path.basename(os.path.normpath(path))}"]
+    stats = rows("kernel_stats.csv")
+    if stats:
+        stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
+        lines.append("| kernel | calls | avg us | % |")
+        lines.append("|---|---|---|---|")
+        for r in stats[:14]:
+            lines.append(f"| {short(r['Name'])} | {r['Calls']} | "
+                         f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+    n_k, n_graph = tail if tail else (0, 0)
+    trace = [r for r in rows("kernel_trace.csv") if short(r["Kernel_Name"]).startswith(prefix)]
+    if trace and tail:
+        trace.sort(key=lambda r: int(r["Start_Timestamp"]))
+        d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in trace]
+        kp, gr = d[len(d) - n_graph - n_k:len(d) - n_graph], d[len(d) - n_graph:]
+        lines.append("")
+        lines.append(f"{prefix}...>: {len(d)} launches; kernel pass (last {n_k} before the "
+                     f"graph): avg {np.mean(kp):.4f} ms; graph replays ({n_graph}): avg "
+                     f"{np.mean(gr):.4f} ms")
+    pmc = rows("counter_collection.csv")
     if pmc:
+        by = {}
+        for r in pmc:
+            by.setdefault((short(r["Kernel_Name"]), r["Counter_Name"]), []).append(
+                (int(r["Dispatch_Id"]), float(r["Counter_Value"])))
         lines.append("")
         lines.append("| kernel | counter | avg per dispatch | dispatches |")
         lines.append("|---|---|---|---|")
-        for kn, cn, v, c in sorted(pmc, key=lambda r: -r[2])[:12]:
-            lines.append(f"| {kn.split('(')[0][:60]} | {cn} | {v:.1f} | {c} |")
-    if tail:
-        n_k, n_graph = tail
-        d = [r[0] for r in cur.execute("select duration from kernels where name like ? "
-                                       "order by start", ("%" + prefix + "%",)).fetchall()]
-        if len(d) >= n_k + n_graph:
-            kp = d[len(d) - n_graph - n_k:len(d) - n_graph]
-            gr = d[len(d) - n_graph:]
-            lines.append("")
-            lines.append(f"{prefix}...>: {len(d)} launches; kernel pass (last {n_k} before "
-                         f"the graph): avg {np.mean(kp) / 1e6:.4f} ms; graph replays "
-                         f"({n_graph}): avg {np.mean(gr) / 1e6:.4f} ms")
+        for (kn, cn), v in sorted(by.items(), key=lambda kv: -np.mean([x[1] for x in kv[1]])):
+            v.sort()
+            if kn.startswith(prefix) and tail:
+                v = v[-(n_k + n_graph):]
+                kn += f" (last {len(v)})"
+            lines.append(f"| {kn[:70]} | {cn} | {np.mean([x[1] for x in v]):.1f} | {len(v)} |")
     return "\n".join(lines)
+
+
+def profiled_traffic(stem):
+    """The HBM bytes per launch recorded in profiles/<stem>.md (a line
+    ``traffic_bytes_per_launch: <int>``), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", stem + ".md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return {"traffic_bytes_per_launch": float(line.split(":")[1]),
+                        "source": "profiles/" + stem + ".md"}
+    return None
 
 
 def progress(msg):
@@ -148,19 +184,20 @@ def main():
         chain.step(beta, alw, seed, t, series[row], events=ev_pair)
 
     # the workload's own GFMC warm-up (untimed, whatever --warmup says): the
-    # four-stage step ramp of run_gfmc, which also measures the shift.  From
-    # random Sz = 0 configurations the rate evaluations per walker-step fall
-    # from ~46 to a steady ~17.5 within ~200 steps; the ramp's 1000 steps
-    # put every timed step in that steady state.
+    # four-stage step ramp of run_gfmc (250 steps each at step 0.025, 0.05,
+    # 0.1, 0.2), which also measures the shift.
     from pygfmc_amd.engine import _warmup
     from pygfmc_amd.walkers import PhiloxKey
     p.set_lanes(64)
     alw = _warmup(chain, PhiloxKey(seed, 0), beta, None, args.gfmc_warmup, 0)
     pos0 = 1 << 20             # production positions, past the warm-up's
-    # then the harness's untimed steps (shift held)
-    for t in range(args.warmup):
-        one_step(pos0 + t, t)
-    pos0 += args.warmup
+    # untimed production steps (shift held): after the ramp the population
+    # keeps relaxing (the evaluations per walker-step can sit near 28 for a
+    # hundred steps or more before settling near 17), then the harness's
+    # untimed steps
+    for t in range(args.settle + args.warmup):
+        one_step(pos0 + t, t % series.shape[0])
+    pos0 += args.settle + args.warmup
     torch.cuda.synchronize()
     # lanes per walker of the propagation kernel: the autotuner, or pinned
     if args.lanes == "auto":
@@ -229,12 +266,13 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(H, G, chain, N, beta, alw, seed, args.cpu_seconds)
 
-    # HBM bytes per k_propagate launch of the default workload, measured by
-    # two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000,
-    # factors from a 256 MiB device copy); not measurable inside this run
-    default_workload = (L == 32 and nw == 65536 and args.precision == "fp32" and lanes == 64
-                        and args.delta == 1.0)
-    traffic = 42.9e6 if default_workload else None
+    # HBM bytes per launch: two rocprofv3 --pmc passes (FETCH_SIZE,
+    # WRITE_SIZE, each scaled by what it reads for a device copy of known
+    # size) of this same command, kept in the kernel's profile summary; not
+    # measurable inside this run, so null for another workload
+    workload = f"xxz_square{L}x{L}_delta{args.delta:g}_jastrow0.3_w{nw}_step0.2_{args.precision}"
+    prof = profiled_traffic(f"k_propagate_{workload}_lanes{lanes}")
+    traffic = prof["traffic_bytes_per_launch"] if prof else None
 
     out = {
         "metric": "walker-steps/s; GPU-sec to fixed energy σ, learned BF16 guide vs plain",
@@ -249,19 +287,28 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32" if args.precision == "fp32" else "fp64",
         "data": "synthetic",
-        "config": {"workload": f"xxz_square{L}x{L}_delta{args.delta:g}_jastrow0.3_"
-                               f"w{nw}_step0.2_{args.precision}",
+        "config": {"workload": workload,
                    "lattice": f"square {L}x{L} periodic", "n_sites": N, "n_edges": E,
                    "n_walkers": nw, "delta": args.delta, "step_size": beta,
                    "guide": "Jastrow beta=0.3", "global_batch": nw, "seq_len": N,
                    "parallelism": "none (1 GPU)", "hip_graph_steps": chunk,
                    "lanes_per_walker": lanes, "autotune_seconds": tune_times,
                    "evals_per_walker_step": evals_per_ws},
-        "roofline": {"bound": "mfma", "compute_unit": "VALU FP32 (no GEMM on this path)",
+        "roofline": {"bound": "valu",
+                     "compute_unit": "FP32 vector ALU (no GEMM on this path; MI355X's FP32 "
+                                     "vector and matrix peaks are both 157.3 TFLOP/s)",
                      "achieved": achieved_tf, "peak": peak_tf, "unit": "TFLOP/s",
-                     "frac": achieved_tf / peak_tf, "traffic": traffic,
-                     "traffic_unit": "bytes per launch (rocprofv3 --pmc, default workload)",
-                     "kernel": f"k_propagate<float, record path, {lanes} lanes>",
+                     "frac": achieved_tf / peak_tf,
+                     "work": "reference-equivalent FLOPs: every slot of every rate "
+                             "evaluation, n_edges (4 (d - 1) + 6) per evaluation; the kernel "
+                             "itself re-evaluates only the slots a move changes",
+                     "traffic": traffic,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE + "
+                                     "WRITE_SIZE, copy-calibrated)",
+                     "traffic_source": prof["source"] if prof else None,
+                     "hbm_GBps": traffic / prop_avg_s / 1e9 if traffic else None,
+                     "hbm_frac": traffic / prop_avg_s / 8e12 if traffic else None,
+                     "kernel": f"k_propagate<float, local-field records, {lanes} lanes>",
                      "kernel_avg_ms": prop_avg_s * 1e3, "kernel_steps": n_k,
                      "evals_per_walker_step": evals_kernel / (nw * n_k),
                      "flop_per_eval": flop_per_eval},

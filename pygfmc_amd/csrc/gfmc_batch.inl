@@ -11,7 +11,7 @@
 // seed seed + l.
 
 struct gfmc_batch {
-    int n_lat = 0, precision = 0, fast = 0, stride = 0, lanes = WAVE, method = 0;
+    int n_lat = 0, precision = 0, fast = 0, fld = 0, stride = 0, lanes = WAVE, method = 0;
     int64_t n_total = 0, lds_bytes = 0, work_doubles = 0;
     int64_t lds_g[3] = {0, 0, 0};   // per wavefront with 64 / 32 / 16 lanes per walker
     int g_ok[3] = {1, 0, 0};
@@ -49,7 +49,7 @@ static RcSegs batch_segs(const gfmc_batch *b) {
     return S;
 }
 
-template <typename T, bool FAST, int G>
+template <typename T, int FAST, int G>
 static void launch_batch_g(const gfmc_batch *b, const Tables<T> *Ts, unsigned grid, size_t lds,
                            uint32_t *spins, double *log_w, double *e_loc, double beta,
                            const double *alw, uint64_t seed, uint32_t step, uint32_t tag,
@@ -71,24 +71,44 @@ static void launch_batch_g(const gfmc_batch *b, const Tables<T> *Ts, unsigned gr
                            g_step_base);
 }
 
+// the record path (FAST 1: records, 2: their local-field form, every
+// lattice having it)
+template <typename T, int FAST>
+static void launch_batch_rec(const gfmc_batch *b, const Tables<T> *Ts, uint32_t *spins,
+                             double *log_w, double *e_loc, double beta, const double *alw,
+                             uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals,
+                             hipStream_t st) {
+    const unsigned n = (unsigned)b->n_total;
+    if (b->lanes == 32)
+        launch_batch_g<T, FAST, 32>(b, Ts, (n + 1) / 2, (size_t)b->lds_g[1], spins, log_w,
+                                    e_loc, beta, alw, seed, step, tag, evals, st);
+    else if (b->lanes == 16)
+        launch_batch_g<T, FAST, 16>(b, Ts, (n + 3) / 4, (size_t)b->lds_g[2], spins, log_w,
+                                    e_loc, beta, alw, seed, step, tag, evals, st);
+    else
+        launch_batch_g<T, FAST, 64>(b, Ts, n, (size_t)b->lds_bytes, spins, log_w, e_loc, beta,
+                                    alw, seed, step, tag, evals, st);
+}
+
 template <typename T>
 static void launch_propagate_batch(const gfmc_batch *b, const Tables<T> *Ts, uint32_t *spins,
                                    double *log_w, double *e_loc, double beta, const double *alw,
                                    uint64_t seed, uint32_t step, uint32_t tag, int32_t *evals,
                                    hipStream_t st) {
     const unsigned n = (unsigned)b->n_total;
-    if (!b->fast)
-        launch_batch_g<T, false, 64>(b, Ts, n, (size_t)b->lds_bytes, spins, log_w, e_loc, beta,
-                                     alw, seed, step, tag, evals, st);
-    else if (b->lanes == 32)
-        launch_batch_g<T, true, 32>(b, Ts, (n + 1) / 2, (size_t)b->lds_g[1], spins, log_w,
-                                    e_loc, beta, alw, seed, step, tag, evals, st);
-    else if (b->lanes == 16)
-        launch_batch_g<T, true, 16>(b, Ts, (n + 3) / 4, (size_t)b->lds_g[2], spins, log_w,
-                                    e_loc, beta, alw, seed, step, tag, evals, st);
-    else
-        launch_batch_g<T, true, 64>(b, Ts, n, (size_t)b->lds_bytes, spins, log_w, e_loc, beta,
-                                    alw, seed, step, tag, evals, st);
+    if (!b->fast) {
+        launch_batch_g<T, 0, 64>(b, Ts, n, (size_t)b->lds_bytes, spins, log_w, e_loc, beta,
+                                 alw, seed, step, tag, evals, st);
+        return;
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        if (b->fld) {
+            launch_batch_rec<T, 2>(b, Ts, spins, log_w, e_loc, beta, alw, seed, step, tag, evals,
+                                   st);
+            return;
+        }
+    }
+    launch_batch_rec<T, 1>(b, Ts, spins, log_w, e_loc, beta, alw, seed, step, tag, evals, st);
 }
 
 extern "C" {
@@ -151,6 +171,8 @@ int gfmc_batch_create(const gfmc_problem *const *ps, int32_t n_lat, const int64_
     b->n_lat = n_lat;
     b->precision = prec;
     b->fast = fast;
+    b->fld = fast;
+    for (int l = 0; l < n_lat; ++l) b->fld = b->fld && ps[l]->dp.fld;
     b->method = ps[0]->dp.method;
     b->stride = stride;
     b->n_total = total;

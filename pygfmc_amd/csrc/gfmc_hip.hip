@@ -24,6 +24,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/gfmc.h"
@@ -80,6 +81,9 @@ struct Tables {
     const T *jpm, *jpmpm, *hx_half, *jas_h, *jas_site_J, *jas_edge_J;
     // uniform values used by the record path
     T jpm0, jpmpm0, hx0, J0, h0;
+    // its local-field form: log2 of a Jastrow ratio = lJ * (integer pair sum)
+    // + lh * (integer spin sum), lJ = -2 J0 log2(e), lh = -2 h0 log2(e)
+    T lJ, lh;
 };
 
 // Record path ("fast"): usable when the couplings are uniform, the Jastrow
@@ -120,6 +124,17 @@ struct DevProblem {
     const int *mv_sites;     // [n_off]  i | (j << 16), j = 0xffff for a site flip
     const int *mv_edges;     // [n_off * RE] edges whose state a move may change (-1 pad)
     const int *edge_pack;    // [n_edges] i | (j << 16)
+    // local-field form of the records (FP32 kernels, when fld): records of two
+    // words {slot | i << 16, j | flags << 16} in the LDS order of each group
+    // size and per move; the field updates of each move, mv_fnb[s * FD + t] =
+    // partner | (flipped site << 16), t < D for site i and D <= t < 2D for
+    // site j; every site's partners, 8 int16 per site, site N (a zero spin)
+    // as padding; f_pad bytes of fields per walker
+    int fld, FD, f_pad;
+    const uint2 *slot_rf_g[3];
+    const uint2 *mv_rf;      // [n_off * R]
+    const int *mv_fnb;       // [n_off * FD]
+    const uint4 *fsite;      // [N]
     // RBM guide (FP32 arithmetic on BF16-valued parameters)
     int H, w_words, rbm_wave_bytes;
     const uint32_t *rbm_W;   // [H][N] BF16, packed two per word (N even-padded)
@@ -402,6 +417,50 @@ __device__ __forceinline__ T eval_rec(const DevProblem &P, const Tables<T> &tb, 
     return -(bare * dev_exp((T)-2 * pair - (T)2 * hterm));
 }
 
+// Local-field form of the FP32 record path.  F_k, the sum of the spins of
+// site k's Jastrow partners, is kept in LDS (one byte per site) and updated
+// by every move, so a slot weight reads two spins and two fields instead of
+// 2 REC_D partner spins.  The host enables it when every record's partners
+// are the Jastrow partners of its site(s), less the other flipped site where
+// that one is a partner (flag bits 0 and 1); then the partner sums of
+// eval_rec are s_i = F_i - c_i x_j and s_j = F_j - c_j x_i, exact integers,
+// and the weight is -bare 2^(lJ (x_i s_i + x_j s_j) + lh (x_i + x_j)) (a
+// site flip: -hx0 2^(lJ x_k F_k + lh x_k)); the same value as eval_rec up
+// to FP32 rounding of the exponent.
+__device__ __forceinline__ float eval_recf(const Tables<float> &tb, uint2 r,
+                                           const signed char *x, const signed char *F) {
+    const int i = (int)(r.x >> 16);
+    const int j = (int)(int16_t)(r.y & 0xffffu);
+    const int fl = (int)(r.y >> 16);
+    if (j < 0) {
+        const int xk = x[i], fk = F[i];
+        const float e = tb.lJ * (float)(xk * fk) + tb.lh * (float)xk;
+        return tb.hx0 == 0.0f ? -tb.hx0 : -(tb.hx0 * __builtin_amdgcn_exp2f(e));
+    }
+    const int xi = x[i], xj = x[j], fi = F[i], fj = F[j];
+    const float bare = xi != xj ? tb.jpm0 : tb.jpmpm0;
+    const int si = fi - (fl & 1) * xj, sj = fj - ((fl >> 1) & 1) * xi;
+    const float e = tb.lJ * (float)(xi * si + xj * sj) + tb.lh * (float)(xi + xj);
+    return bare == 0.0f ? -bare : -(bare * __builtin_amdgcn_exp2f(e));
+}
+
+// the local fields of the walker's spins (spins already in LDS); site N is
+// the zero spin the padding entries of the partner tables name
+template <int G>
+__device__ __forceinline__ void fld_init(const DevProblem &P, signed char *x, signed char *F,
+                                         int gl) {
+    if (gl == 0) x[P.N] = 0;
+    wave_sync();
+    const bool wide = P.max_deg > 4;
+    for (int k = gl; k < P.N; k += G) {
+        const uint4 q = P.fsite[k];
+        int s = x[q.x & 0xffffu] + x[q.x >> 16] + x[q.y & 0xffffu] + x[q.y >> 16];
+        if (wide) s += x[q.z & 0xffffu] + x[q.z >> 16] + x[q.w & 0xffffu] + x[q.w >> 16];
+        F[k] = (signed char)s;
+    }
+    wave_sync();
+}
+
 // Diagonal element (SpinHalf.diagonal_weight, SpinHalf.py:291-314); gl is
 // the lane in the walker's group of G lanes (blocked order over G lanes).
 template <int G = WAVE>
@@ -443,6 +502,7 @@ struct WalkerLds {
     T *w;                 // [K*64], slot s at (s % K) * 64 + s / K
     signed char *x;       // [n_pad]
     unsigned char *anti;  // [e_pad]
+    signed char *f;       // [f_pad] local fields (the records' local-field form)
 };
 
 template <typename T>
@@ -451,6 +511,7 @@ __device__ __forceinline__ WalkerLds<T> carve(const DevProblem &P, unsigned char
     L.w = (T *)lds;
     L.x = (signed char *)(lds + (size_t)P.K * WAVE * sizeof(T));
     L.anti = (unsigned char *)(L.x + P.n_pad);
+    L.f = (signed char *)(L.anti + P.e_pad);
     return L;
 }
 
@@ -458,7 +519,7 @@ __device__ __forceinline__ WalkerLds<T> carve(const DevProblem &P, unsigned char
 template <typename T, int G>
 __host__ __device__ inline int walker_lds_bytes(const DevProblem &P) {
     const int K = (P.n_off > 0 ? P.n_off + G - 1 : G) / G;
-    return (int)(((size_t)K * G * sizeof(T) + P.n_pad + P.e_pad + 15) / 16 * 16);
+    return (int)(((size_t)K * G * sizeof(T) + P.n_pad + P.e_pad + P.f_pad + 15) / 16 * 16);
 }
 
 template <typename T, int G>
@@ -467,6 +528,7 @@ __device__ __forceinline__ WalkerLds<T> carve_g(const DevProblem &P, unsigned ch
     L.w = (T *)lds;
     L.x = (signed char *)(lds + (size_t)P.Kg[gidx<G>()] * G * sizeof(T));
     L.anti = (unsigned char *)(L.x + P.n_pad);
+    L.f = (signed char *)(L.anti + P.e_pad);
     return L;
 }
 
@@ -496,9 +558,28 @@ __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
 
 // every slot weight, every edge state; returns the anti-parallel count.
 // gl: lane in the walker's group of G lanes (the generic path: G = 64).
-template <typename T, bool FAST, int G = WAVE>
+template <typename T, int FAST, int G = WAVE>
 __device__ int full_eval(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L, int gl) {
-    if constexpr (FAST) {
+    if constexpr (FAST == 2) {
+        // local-field records (fields already initialised), four in flight
+        const int n = P.Kg[gidx<G>()] * G;
+        const uint2 *recs = P.slot_rf_g[gidx<G>()];
+        for (int p0 = 0; p0 < n; p0 += 4 * G) {
+            uint2 r[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int p = p0 + q * G + gl;
+                r[q] = p < n ? recs[p] : make_uint2(0xffffffffu, 0xffffffffu);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int p = p0 + q * G + gl;
+                if (p < n)
+                    L.w[p] = (int16_t)(r[q].x & 0xffffu) >= 0 ? eval_recf(tb, r[q], L.x, L.f)
+                                                             : (T)0;
+            }
+        }
+    } else if constexpr (FAST) {
         // slot records are stored in LDS order (record p describes the slot
         // kept at LDS position p, slot -1 = padding): coalesced reads,
         // conflict-free writes, four records in flight per lane
@@ -601,17 +682,54 @@ __device__ int sample_slot(const DevProblem &P, const WalkerLds<T> &L, T incl, T
     return last_positive_slot<T, G>(P, L, K, j * K + K - 1, lane);
 }
 
+// The Philox blocks of a walker's hops, G at a time: lane gl of the group
+// holds the block of hop it0 + gl under the counters of gfmc_philox(it, wk,
+// step, tag, key), so the ten rounds of G hops run once, on the vector unit,
+// and hop it reads its block from group lane it % G (the same numbers as one
+// gfmc_philox call per hop).
+template <int G>
+struct HopRng {
+    gfmc_u4 v;
+    uint32_t it0 = 0xffffffffu;
+
+    __device__ __forceinline__ gfmc_u4 at(uint32_t it, uint32_t wk, uint32_t step,
+                                          uint32_t tag, uint32_t k0, uint32_t k1, int gl) {
+        const uint32_t base = it & ~(uint32_t)(G - 1);
+        if (base != it0) {
+            it0 = base;
+            v = gfmc_philox(base + (uint32_t)gl, wk, step, tag, k0, k1);
+        }
+        const int src = (int)(it & (uint32_t)(G - 1));
+        gfmc_u4 r;
+        r.x = (uint32_t)grp_bcast<G>((int)v.x, src);
+        r.y = (uint32_t)grp_bcast<G>((int)v.y, src);
+        r.z = (uint32_t)grp_bcast<G>((int)v.z, src);
+        r.w = (uint32_t)grp_bcast<G>((int)v.w, src);
+        return r;
+    }
+};
+
 // Data of move idx that the record path prefetches before the hold
 // arithmetic: this lane's record, this lane's neighbour entry, the sites.
+// The local-field form prefetches the two-word record and this lane's
+// field update instead.
 struct MovePrefetch {
     Rec rec;
+    uint2 rf;
+    int fnb;
     int nbr;
     int sites;
 };
 
+template <int FAST>
 __device__ __forceinline__ MovePrefetch prefetch_move(const DevProblem &P, int idx, int gl) {
     MovePrefetch m;
-    m.rec = gl < P.R ? load_rec(P.mv_rec, (size_t)idx * P.R + gl) : empty_rec();
+    if constexpr (FAST == 2) {
+        m.rf = gl < P.R ? P.mv_rf[(size_t)idx * P.R + gl] : make_uint2(0xffffffffu, 0xffffffffu);
+        m.fnb = gl < P.FD ? P.mv_fnb[(size_t)idx * P.FD + gl] : P.N | (P.N << 16);
+    } else {
+        m.rec = gl < P.R ? load_rec(P.mv_rec, (size_t)idx * P.R + gl) : empty_rec();
+    }
     m.nbr = (P.has_jz && P.uniform_jz && gl < P.RN) ? P.mv_nbr[(size_t)idx * P.RN + gl] : -1;
     m.sites = P.mv_sites[idx];
     return m;
@@ -619,7 +737,7 @@ __device__ __forceinline__ MovePrefetch prefetch_move(const DevProblem &P, int i
 
 // record path of apply_move: count the edges that turn (anti-)parallel,
 // flip, re-evaluate the prefetched records
-template <typename T, int G = WAVE>
+template <typename T, int FAST, int G = WAVE>
 __device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLds<T> &L,
                               const MovePrefetch &m, int idx, int n_anti, int lane) {
     const int gl = lane & (G - 1);
@@ -639,8 +757,22 @@ __device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLd
         if (b != 0xffff) L.x[b] = (signed char)-L.x[b];
     }
     wave_sync();
-    const int s = rec_f(m.rec, 0);
-    if (s >= 0) L.w[wpos_g<G>(s, G == WAVE ? P.K : P.Kg[gidx<G>()])] = eval_rec(P, tb, m.rec, L.x);
+    if constexpr (FAST == 2) {
+        // the fields: the partners of site i, then those of site j (a partner
+        // of both takes both updates, one pass after the other)
+        const int fp = m.fnb & 0xffff, ff = (m.fnb >> 16) & 0xffff;
+        if (gl < P.max_deg) L.f[fp] = (signed char)(L.f[fp] + 2 * L.x[ff]);
+        wave_sync();
+        if (gl >= P.max_deg && gl < P.FD) L.f[fp] = (signed char)(L.f[fp] + 2 * L.x[ff]);
+        wave_sync();
+        const int s = (int16_t)(m.rf.x & 0xffffu);
+        if (s >= 0)
+            L.w[wpos_g<G>(s, G == WAVE ? P.K : P.Kg[gidx<G>()])] = eval_recf(tb, m.rf, L.x, L.f);
+    } else {
+        const int s = rec_f(m.rec, 0);
+        if (s >= 0)
+            L.w[wpos_g<G>(s, G == WAVE ? P.K : P.Kg[gidx<G>()])] = eval_rec(P, tb, m.rec, L.x);
+    }
     if (P.has_jz && !P.uniform_jz) {
         for (int t = gl; t < P.RE; t += G) {
             const int e = P.mv_edges[(size_t)idx * P.RE + t];
@@ -710,7 +842,7 @@ __device__ int apply_move(const DevProblem &P, const Tables<T> &tb, WalkerLds<T>
 // (continuous time) or the sub-step count n_bra (discrete methods, shift
 // P.lam).  Groups of one wavefront run independently (no workgroup
 // barriers on this path).
-template <typename T, bool FAST, int G, int METHOD>
+template <typename T, int FAST, int G, int METHOD>
 __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tables<T> &tb,
                                                  unsigned char *lds, int lane, uint32_t wk,
                                                  uint32_t *sp, double *lw_p, double *el_p,
@@ -723,6 +855,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
     if constexpr (G == WAVE) wk = __builtin_amdgcn_readfirstlane(wk);
     WalkerLds<T> L = FAST ? carve_g<T, G>(P, lds) : carve<T>(P, lds);
     load_spins<G>(P, sp, L.x, gl);
+    if constexpr (FAST == 2) fld_init<G>(P, L.x, L.f, gl);
     int n_anti = 0;
     int cur = 0;
     bool carried = false;
@@ -750,10 +883,11 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
     double lw = *lw_p;
     double eL = 0.0;
     uint32_t it = 0;
+    HopRng<G> rng;
     // flip the sites of slot idx, re-evaluate what depends on them, new diag
     auto move = [&](int idx, const MovePrefetch &m) {
         if constexpr (FAST)
-            n_anti = apply_move_rec<T, G>(P, tb, L, m, idx, n_anti, lane);
+            n_anti = apply_move_rec<T, FAST, G>(P, tb, L, m, idx, n_anti, lane);
         else
             n_anti = apply_move(P, tb, L, idx, n_anti, lane);
         if (!(P.has_jz && P.uniform_jz) || P.has_hz)
@@ -769,7 +903,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
             T incl = lane_scan<T, G>(P, L, gl, &lane_sum);
             T R = grp_bcast<G>(incl, G - 1);
             eL = diag - (double)R;
-            gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
+            gfmc_u4 r = rng.at(it, wk, step, tag, key0, key1, gl);
             double psi = gfmc_u53(r.z, r.w);
             int idx = -1;
             double tau_raw = INFINITY;
@@ -779,8 +913,16 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                 T cap = below(R);
                 if (u > cap) u = cap;
                 idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
-                if constexpr (FAST) m = prefetch_move(P, idx, gl);  // in flight during the hold math
-                tau_raw = gfmc_neg_log_unit(1.0 - psi) / (double)R;
+                if constexpr (FAST) m = prefetch_move<FAST>(P, idx, gl);  // in flight during the hold math
+                if constexpr (FAST == 2) {
+                    // FP32 kernels: -log(1 - psi) / R in FP32 (v_log_f32), the
+                    // precision the rates carry
+                    const float nl = -0.69314718055994531f *
+                                     __builtin_amdgcn_logf((float)(1.0 - psi));
+                    tau_raw = (double)(nl * __builtin_amdgcn_rcpf((float)R));
+                } else {
+                    tau_raw = gfmc_neg_log_unit(1.0 - psi) / (double)R;
+                }
             }
             double rem = beta - t;
             double tau = tau_raw < rem ? tau_raw : rem;
@@ -804,7 +946,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                 const double w0d = P.lam - diag;
                 const T w0 = (T)w0d;
                 const T W = w0 + R;
-                gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
+                gfmc_u4 r = rng.at(it, wk, step, tag, key0, key1, gl);
                 T u = uniform_for<T>(r) * W;
                 const T cap = below(W);
                 if (u > cap) u = cap;
@@ -814,7 +956,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                 // through log W), also when there are no off-diagonal slots
                 if (W > (T)0 && !(u < w0)) {
                     idx = sample_slot<T, G>(P, L, incl, lane_sum, u - w0, lane);
-                    if constexpr (FAST) m = prefetch_move(P, idx, gl);
+                    if constexpr (FAST) m = prefetch_move<FAST>(P, idx, gl);
                 }
                 lw = lw + log(w0d + (double)R);
                 ++it;
@@ -829,7 +971,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                 T incl = lane_scan<T, G>(P, L, gl, &lane_sum);
                 T R = grp_bcast<G>(incl, G - 1);
                 const double bx = (P.lam - diag) + (double)R;
-                gfmc_u4 r = gfmc_philox(it, wk, step, tag, key0, key1);
+                gfmc_u4 r = rng.at(it, wk, step, tag, key0, key1, gl);
                 const double psi = gfmc_u53(r.z, r.w);
                 int idx = -1;
                 MovePrefetch m;
@@ -838,7 +980,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                     T cap = below(R);
                     if (u > cap) u = cap;
                     idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
-                    if constexpr (FAST) m = prefetch_move(P, idx, gl);
+                    if constexpr (FAST) m = prefetch_move<FAST>(P, idx, gl);
                 }
                 const int remaining = n_bra - used;
                 const double p_hop = (double)R / bx;
@@ -891,7 +1033,7 @@ __global__ void k_carry_commit(int32_t *state, int G) {
 }
 
 // one wavefront per workgroup, 64 / G walkers per wavefront
-template <typename T, bool FAST, int G, int METHOD>
+template <typename T, int FAST, int G, int METHOD>
 __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
                                                     uint32_t *spins, double *log_w,
                                                     double *e_loc, double beta, double alw,
@@ -917,7 +1059,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, 
 // spins of every lattice use the common stride `stride` words.  With G < 64
 // every lattice's walker count is a multiple of 64 / G, so the lattice is
 // uniform over a wavefront.
-template <typename T, bool FAST, int G, int METHOD>
+template <typename T, int FAST, int G, int METHOD>
 __global__ __launch_bounds__(WAVE) void k_propagate_batch(
     const DevProblem *Ps, const Tables<T> *Ts, int n_lat, const int64_t *woff, int64_t n_total,
     int stride, uint32_t *spins, double *log_w, double *e_loc, double beta, const double *alw,
@@ -947,7 +1089,7 @@ __global__ __launch_bounds__(WAVE) void k_propagate_batch(
 // ---------------------------------------------------------------------------
 // rates / log psi kernels
 // ---------------------------------------------------------------------------
-template <typename T, bool FAST>
+template <typename T, int FAST>
 __global__ __launch_bounds__(WAVE) void k_rates(DevProblem P, Tables<T> tb, int64_t nw,
                                                 const uint32_t *spins, double *rates,
                                                 double *e_loc, double *log_diff) {
@@ -958,6 +1100,7 @@ __global__ __launch_bounds__(WAVE) void k_rates(DevProblem P, Tables<T> tb, int6
     WalkerLds<T> L = carve<T>(P, lds);
     load_spins(P, spins + wk * P.words, L.x, lane);
     __syncthreads();
+    if constexpr (FAST == 2) fld_init<WAVE>(P, L.x, L.f, lane);
     int n_anti = full_eval<T, FAST>(P, tb, L, lane);
     __syncthreads();
     double diag = diag_value(P, L.x, L.anti, n_anti, lane);
@@ -1777,7 +1920,97 @@ struct HostRecords {
     std::vector<int> mv_nbr, mv_sites, mv_edges, edge_pack;
     int R = 0, RN = 0, RE = 0;
     double J0 = 0.0, h0 = 0.0;
+    // local-field form (DevProblem::fld)
+    int fld = 0, FD = 0;
+    std::vector<uint32_t> slot_rf_g[3], mv_rf, fsite;   // 2 words per record, 4 per site
+    std::vector<int> mv_fnb;
 };
+
+static int get16(const uint32_t *rec, int k) {
+    const int v = (int)((rec[k >> 1] >> ((k & 1) * 16)) & 0xffffu);
+    return (v ^ 0x8000) - 0x8000;
+}
+
+// The local-field form of the records: every record's partners (mask bits
+// set) must be the Jastrow partners of its site(s), less the other flipped
+// site where that one is a partner, and no site may list a partner twice.
+// Builds the two-word records, the per-move field updates and the per-site
+// partner lists (site N, a zero spin, as padding).
+static void build_field_records(const gfmc_problem_desc *d, HostRecords &hr) {
+    const int N = d->n_sites, n_off = d->n_hx + d->n_jpm, D = d->jas_max_deg;
+    hr.fld = 0;
+    if (d->guide != GFMC_GUIDE_JASTROW || D < 1 || D > 8 || N + 1 >= 32768) return;
+    std::vector<std::vector<int>> nj(N);
+    for (int k = 0; k < N; ++k) {
+        for (int q = 0; q < D; ++q)
+            if (d->jas_site_J[(size_t)k * D + q] != 0.0)
+                nj[k].push_back(d->jas_site_other[(size_t)k * D + q]);
+        std::vector<int> u = nj[k];
+        std::sort(u.begin(), u.end());
+        if (std::adjacent_find(u.begin(), u.end()) != u.end()) return;
+    }
+    std::vector<uint32_t> rf((size_t)n_off * 2);
+    for (int s = 0; s < n_off; ++s) {
+        const uint32_t *r = &hr.slot_rec[(size_t)s * 8];
+        const int i = get16(r, 1), j = get16(r, 2), mask = get16(r, 3) & 0xffff;
+        int flags = 0;
+        for (int side = 0; side < (j >= 0 ? 2 : 1); ++side) {
+            const int site = side ? j : i, other = side ? i : j;
+            std::vector<int> got, want = nj[site];
+            for (int q = 0; q < REC_D; ++q)
+                if ((mask >> (side * REC_D + q)) & 1) got.push_back(get16(r, 4 + side * REC_D + q));
+            if (other >= 0) {
+                auto it = std::find(want.begin(), want.end(), other);
+                if (it != want.end()) {
+                    want.erase(it);
+                    flags |= 1 << side;
+                }
+            }
+            std::sort(got.begin(), got.end());
+            std::sort(want.begin(), want.end());
+            if (got != want) return;
+        }
+        rf[(size_t)s * 2] = (uint32_t)(s & 0xffff) | ((uint32_t)i << 16);
+        rf[(size_t)s * 2 + 1] = (uint32_t)(j & 0xffff) | ((uint32_t)flags << 16);
+    }
+    for (int gi = 0; gi < 3; ++gi) {
+        const int G = WAVE >> gi;
+        const int K = (n_off + G - 1) / G;
+        hr.slot_rf_g[gi].assign((size_t)K * G * 2, 0xffffffffu);
+        for (int s = 0; s < n_off; ++s) {
+            const size_t p = (size_t)((s % K) * G + s / K);
+            hr.slot_rf_g[gi][p * 2] = rf[(size_t)s * 2];
+            hr.slot_rf_g[gi][p * 2 + 1] = rf[(size_t)s * 2 + 1];
+        }
+    }
+    hr.mv_rf.assign((size_t)n_off * hr.R * 2, 0xffffffffu);
+    for (int s = 0; s < n_off; ++s)
+        for (int t = d->dep_off[s]; t < d->dep_off[s + 1]; ++t) {
+            const size_t o = ((size_t)s * hr.R + (t - d->dep_off[s])) * 2;
+            hr.mv_rf[o] = rf[(size_t)d->dep_slots[t] * 2];
+            hr.mv_rf[o + 1] = rf[(size_t)d->dep_slots[t] * 2 + 1];
+        }
+    hr.FD = 2 * D;
+    const int Z = N | (N << 16);
+    hr.mv_fnb.assign((size_t)n_off * hr.FD, Z);
+    for (int s = 0; s < n_off; ++s) {
+        const int a = hr.mv_sites[s] & 0xffff, b = (hr.mv_sites[s] >> 16) & 0xffff;
+        for (int side = 0; side < 2; ++side) {
+            const int f = side ? b : a;
+            if (f == 0xffff) continue;
+            for (size_t q = 0; q < nj[f].size(); ++q)
+                hr.mv_fnb[(size_t)s * hr.FD + side * D + q] = nj[f][q] | (f << 16);
+        }
+    }
+    hr.fsite.assign((size_t)N * 4, (uint32_t)Z);
+    for (int k = 0; k < N; ++k)
+        for (size_t q = 0; q < nj[k].size(); ++q) {
+            uint32_t &w = hr.fsite[(size_t)k * 4 + q / 2];
+            const int sh = (int)(q & 1) * 16;
+            w = (w & ~(0xffffu << sh)) | ((uint32_t)nj[k][q] << sh);
+        }
+    hr.fld = 1;
+}
 
 static void put16(uint32_t *rec, int k, int v) {
     uint32_t &w = rec[k >> 1];
@@ -1913,6 +2146,7 @@ static bool build_records(const gfmc_problem_desc *d, HostRecords &hr) {
     } else {
         hr.mv_edges.assign(1, -1);
     }
+    build_field_records(d, hr);
     return true;
 }
 
@@ -1920,6 +2154,14 @@ template <typename T>
 static void launch_rates(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
                          const uint32_t *spins, double *rates, double *e_loc, double *log_diff,
                          hipStream_t st) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (p->dp.fld) {
+            hipLaunchKernelGGL((k_rates<T, 2>), dim3((unsigned)nw), dim3(WAVE),
+                               (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, rates, e_loc,
+                               log_diff);
+            return;
+        }
+    }
     if (p->dp.fast)
         hipLaunchKernelGGL((k_rates<T, true>), dim3((unsigned)nw), dim3(WAVE),
                            (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, rates, e_loc, log_diff);
@@ -1928,7 +2170,7 @@ static void launch_rates(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
                            (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, rates, e_loc, log_diff);
 }
 
-template <typename T, bool FAST, int G>
+template <typename T, int FAST, int G>
 static void launch_prop_g(const gfmc_problem *p, const Tables<T> &tb, unsigned grid, size_t lds,
                           int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
                           double beta, double alw, uint32_t k0, uint32_t k1, uint32_t step,
@@ -1950,25 +2192,45 @@ static void launch_prop_g(const gfmc_problem *p, const Tables<T> &tb, unsigned g
     if (cy.on) hipLaunchKernelGGL(k_carry_commit, dim3(1), dim3(WAVE), 0, st, cy.state, G);
 }
 
+// the record path at the problem's lanes per walker (FAST 1: records, 2:
+// their local-field form)
+template <typename T, int FAST>
+static void launch_prop_rec(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
+                            uint32_t *spins, double *log_w, double *e_loc, double beta,
+                            double alw, uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
+                            int32_t *evals, hipStream_t st) {
+    if (p->lanes == 32)
+        launch_prop_g<T, FAST, 32>(p, tb, (unsigned)((nw + 1) / 2),
+                                   (size_t)(2 * walker_lds_bytes<T, 32>(p->dp)), nw, spins,
+                                   log_w, e_loc, beta, alw, k0, k1, step, tag, evals, st);
+    else if (p->lanes == 16)
+        launch_prop_g<T, FAST, 16>(p, tb, (unsigned)((nw + 3) / 4),
+                                   (size_t)(4 * walker_lds_bytes<T, 16>(p->dp)), nw, spins,
+                                   log_w, e_loc, beta, alw, k0, k1, step, tag, evals, st);
+    else
+        launch_prop_g<T, FAST, 64>(p, tb, (unsigned)nw, (size_t)p->lds_bytes, nw, spins, log_w,
+                                   e_loc, beta, alw, k0, k1, step, tag, evals, st);
+}
+
 template <typename T>
 static void launch_propagate(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
                              uint32_t *spins, double *log_w, double *e_loc, double beta,
                              double alw, uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
                              int32_t *evals, hipStream_t st) {
-    if (!p->dp.fast)
-        launch_prop_g<T, false, 64>(p, tb, (unsigned)nw, (size_t)p->lds_bytes, nw, spins, log_w,
-                                    e_loc, beta, alw, k0, k1, step, tag, evals, st);
-    else if (p->lanes == 32)
-        launch_prop_g<T, true, 32>(p, tb, (unsigned)((nw + 1) / 2),
-                                   (size_t)(2 * walker_lds_bytes<T, 32>(p->dp)), nw, spins,
-                                   log_w, e_loc, beta, alw, k0, k1, step, tag, evals, st);
-    else if (p->lanes == 16)
-        launch_prop_g<T, true, 16>(p, tb, (unsigned)((nw + 3) / 4),
-                                   (size_t)(4 * walker_lds_bytes<T, 16>(p->dp)), nw, spins,
-                                   log_w, e_loc, beta, alw, k0, k1, step, tag, evals, st);
-    else
-        launch_prop_g<T, true, 64>(p, tb, (unsigned)nw, (size_t)p->lds_bytes, nw, spins, log_w,
-                                   e_loc, beta, alw, k0, k1, step, tag, evals, st);
+    if (!p->dp.fast) {
+        launch_prop_g<T, 0, 64>(p, tb, (unsigned)nw, (size_t)p->lds_bytes, nw, spins, log_w,
+                                e_loc, beta, alw, k0, k1, step, tag, evals, st);
+        return;
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        if (p->dp.fld) {
+            launch_prop_rec<T, 2>(p, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
+                                  evals, st);
+            return;
+        }
+    }
+    launch_prop_rec<T, 1>(p, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
+                          st);
 }
 
 #include "gfmc_rbm.inl"
@@ -2222,8 +2484,12 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     P.RN = hr.RN;
     P.RE = hr.RE;
     P.flags = d->has_jz && (!d->uniform_jz || !P.fast);
-    P.n_pad = round_up(N, 16);
+    // the local-field form runs in the FP32 kernels; its zero spin is site N
+    P.fld = P.fast && hr.fld && precision == GFMC_FP32 && d->guide == GFMC_GUIDE_JASTROW;
+    P.FD = P.fld ? hr.FD : 0;
+    P.n_pad = round_up(N + (P.fld ? 1 : 0), 16);
     P.e_pad = P.flags ? round_up(E > 0 ? E : 1, 16) : 0;
+    P.f_pad = P.fld ? P.n_pad : 0;
     {
         const double jpm0 = E ? d->jpm[0] : 0.0, jpmpm0 = E ? d->jpmpm[0] : 0.0;
         const double hx0 = d->n_hx ? d->hx_half[0] : 0.0;
@@ -2237,6 +2503,10 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         p->t32.hx0 = (float)hx0;
         p->t32.J0 = (float)hr.J0;
         p->t32.h0 = (float)hr.h0;
+        p->t32.lJ = (float)(-2.0 * hr.J0 * 1.4426950408889634);
+        p->t32.lh = (float)(-2.0 * hr.h0 * 1.4426950408889634);
+        p->t64.lJ = -2.0 * hr.J0 * 1.4426950408889634;
+        p->t64.lh = -2.0 * hr.h0 * 1.4426950408889634;
     }
     int rc = 0;
 #define UP(src, n, dst)                              \
@@ -2277,6 +2547,19 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         UP(hr.mv_nbr.data(), hr.mv_nbr.size(), &P.mv_nbr);
         UP(hr.mv_sites.data(), hr.mv_sites.size(), &P.mv_sites);
         UP(hr.mv_edges.data(), hr.mv_edges.size(), &P.mv_edges);
+        if (P.fld) {
+            for (int gi = 0; gi < 3; ++gi) {
+                const uint32_t *sg = nullptr;
+                UP(hr.slot_rf_g[gi].data(), hr.slot_rf_g[gi].size(), &sg);
+                P.slot_rf_g[gi] = (const uint2 *)sg;
+            }
+            const uint32_t *mr2 = nullptr, *fs = nullptr;
+            UP(hr.mv_rf.data(), hr.mv_rf.size(), &mr2);
+            UP(hr.fsite.data(), hr.fsite.size(), &fs);
+            P.mv_rf = (const uint2 *)mr2;
+            P.fsite = (const uint4 *)fs;
+            UP(hr.mv_fnb.data(), hr.mv_fnb.size(), &P.mv_fnb);
+        }
     }
     if (d->has_jz) UP(d->jz, (size_t)E, &P.jz);
     if (d->has_hz) UP(d->hz, (size_t)N, &P.hz);
@@ -2393,7 +2676,7 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
 #undef UP
 #undef UP32
     const size_t wbytes = (size_t)P.K * WAVE * (precision == GFMC_FP64 ? 8 : 4);
-    p->lds_bytes = (int64_t)(wbytes + P.n_pad + P.e_pad);
+    p->lds_bytes = (int64_t)(wbytes + P.n_pad + P.e_pad + P.f_pad);
     if (d->guide == GFMC_GUIDE_RBM) {
         p->lds_bytes = (int64_t)P.w_words * 4 + (int64_t)RBM_WAVES * P.rbm_wave_bytes;
         if (p->lds_bytes > 160 * 1024) {
