@@ -56,7 +56,8 @@ struct CarryDev {
     int32_t *anti[2];
     int32_t *state;
     const int32_t *parents;
-    int64_t row;   // elements per walker row
+    int64_t row;     // elements per walker row
+    int64_t f_off;   // element offset of the local fields in a row (0: none)
     int on;
 };
 static thread_local const CarryDev *g_carry = nullptr;
@@ -537,22 +538,45 @@ __device__ __forceinline__ int wpos(int s, int K) { return (s % K) * WAVE + s / 
 template <int G>
 __device__ __forceinline__ int wpos_g(int s, int K) { return (s % K) * G + s / K; }
 
+// Spins between the packed words (bit k of word k / 32: site k up) and LDS
+// bytes (+1 / -1), four sites per lane operation: bits b0..b3 of a nibble
+// spread to bytes by n * 0x00204081 & 0x01010101 (bit q lands at 8q, no
+// carries); a byte 0 or 1 becomes -1 or +1 as 0x01 | (~(b * 0xff) & 0xfe).
+// The bytes of a partial last quad lie inside the n_pad bytes of x.
 template <int G = WAVE>
 __device__ __forceinline__ void load_spins(const DevProblem &P, const uint32_t *sp,
                                            signed char *x, int gl) {
-    for (int k = gl; k < P.N; k += G) x[k] = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1 : -1;
+    const int nq = (P.N + 3) >> 2;
+    for (int t = gl; t < nq; t += G) {
+        const uint32_t nib = (sp[t >> 3] >> ((t & 7) * 4)) & 0xfu;
+        const uint32_t b = (nib * 0x00204081u) & 0x01010101u;
+        *(uint32_t *)(x + 4 * t) = 0x01010101u | (~(b * 0xffu) & 0xfefefefeu);
+    }
 }
 
+// The inverse: bit 1 of a byte's complement is 1 for +1 (0x01) and 0 for
+// -1 (0xff); (v * 0x00204081) >> 21 gathers the four byte bits to a nibble;
+// the eight nibbles of a word are OR-combined across eight consecutive
+// lanes.  Bits past site N - 1 stay clear.
 template <int G = WAVE>
 __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
                                             const signed char *x, int gl) {
-    for (int q = gl; q < P.words; q += G) {
-        uint32_t v = 0;
-        for (int b = 0; b < 32; ++b) {
-            int k = q * 32 + b;
-            if (k < P.N && x[k] > 0) v |= 1u << b;
+    const int nq = (P.N + 3) >> 2;
+    for (int t0 = 0; t0 < P.words * 8; t0 += G) {
+        const int t = t0 + gl;
+        uint32_t nib = 0u;
+        if (t < nq) {
+            const uint32_t d = *(const uint32_t *)(x + 4 * t);
+            const uint32_t v = (~d >> 1) & 0x01010101u;
+            nib = ((v * 0x00204081u) >> 21) & 0xfu;
+            const int left = P.N - 4 * t;
+            if (left < 4) nib &= (1u << left) - 1u;
         }
-        sp[q] = v;
+        uint32_t w = nib << ((t & 7) * 4);
+        w |= (uint32_t)__shfl_xor((int)w, 1, G);
+        w |= (uint32_t)__shfl_xor((int)w, 2, G);
+        w |= (uint32_t)__shfl_xor((int)w, 4, G);
+        if ((t & 7) == 0 && t < P.words * 8) sp[t >> 3] = w;
     }
 }
 
@@ -855,7 +879,6 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
     if constexpr (G == WAVE) wk = __builtin_amdgcn_readfirstlane(wk);
     WalkerLds<T> L = FAST ? carve_g<T, G>(P, lds) : carve<T>(P, lds);
     load_spins<G>(P, sp, L.x, gl);
-    if constexpr (FAST == 2) fld_init<G>(P, L.x, L.f, gl);
     int n_anti = 0;
     int cur = 0;
     bool carried = false;
@@ -867,14 +890,25 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
     }
     if (carried) {
         // the parent's weights and anti-parallel count from the last step
-        const int n = P.Kg[gidx<G>()] * G;
+        // 16-byte pieces: rows and the LDS block are 16-byte aligned, n is a
+        // multiple of G >= 16 values
+        const int n16 = P.Kg[gidx<G>()] * G * (int)sizeof(T) / 16;
         const int64_t par = cy.parents[wk];
-        const T *src = (const T *)cy.w[cur] + par * cy.row;
-#pragma unroll 8
-        for (int q = gl; q < n; q += G) L.w[q] = src[q];
+        const uint4 *src = (const uint4 *)((const T *)cy.w[cur] + par * cy.row);
+        uint4 *dst = (uint4 *)L.w;
+#pragma unroll 4
+        for (int q = gl; q < n16; q += G) dst[q] = src[q];
+        if constexpr (FAST == 2) {
+            // the parent's local fields (those of the spins just loaded) and
+            // the zero spin
+            const uint4 *fs = (const uint4 *)((const T *)cy.w[cur] + par * cy.row + cy.f_off);
+            for (int q = gl; q < P.f_pad / 16; q += G) ((uint4 *)L.f)[q] = fs[q];
+            if (gl == 0) L.x[P.N] = 0;
+        }
         n_anti = cy.anti[cur][par];
         wave_sync();
     } else {
+        if constexpr (FAST == 2) fld_init<G>(P, L.x, L.f, gl);
         wave_sync();
         n_anti = full_eval<T, FAST, G>(P, tb, L, gl);
     }
@@ -1009,10 +1043,15 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
     if constexpr (FAST) {
         if (cy.on) {
             wave_sync();
-            const int n = P.Kg[gidx<G>()] * G;
-            T *dst = (T *)cy.w[cur ^ 1] + (int64_t)wk * cy.row;
-#pragma unroll 8
-            for (int q = gl; q < n; q += G) dst[q] = L.w[q];
+            const int n16 = P.Kg[gidx<G>()] * G * (int)sizeof(T) / 16;
+            uint4 *dst = (uint4 *)((T *)cy.w[cur ^ 1] + (int64_t)wk * cy.row);
+            const uint4 *src = (const uint4 *)L.w;
+#pragma unroll 4
+            for (int q = gl; q < n16; q += G) dst[q] = src[q];
+            if constexpr (FAST == 2) {
+                uint4 *fd = (uint4 *)((T *)cy.w[cur ^ 1] + (int64_t)wk * cy.row + cy.f_off);
+                for (int q = gl; q < P.f_pad / 16; q += G) fd[q] = ((const uint4 *)L.f)[q];
+            }
             if (gl == 0) cy.anti[cur ^ 1][wk] = n_anti;
         }
     }
@@ -3014,11 +3053,20 @@ int gfmc_gather(int32_t n_sites, int64_t nw, const uint32_t *spins_in, const dou
 
 // elements of one carried row: the largest weight array over the usable
 // lane counts (0 when the problem's path carries nothing)
-static int64_t carry_row(const gfmc_problem *p) {
+static int64_t carry_weights(const gfmc_problem *p) {
     if (!p->dp.fast || p->dp.flags) return 0;
     int64_t row = 0;
     for (int gi = 0; gi < 3; ++gi) row = std::max(row, (int64_t)p->dp.Kg[gi] * (WAVE >> gi));
     return row;
+}
+
+// ... followed, for the local-field form, by the walker's fields (f_pad
+// bytes, a multiple of 16)
+static int64_t carry_row(const gfmc_problem *p) {
+    const int64_t w = carry_weights(p);
+    if (w == 0) return 0;
+    const int64_t el = p->precision == GFMC_FP64 ? 8 : 4;
+    return w + (p->dp.fld ? p->dp.f_pad / el : 0);
 }
 
 static size_t carry_align(size_t b) { return (b + 255) / 256 * 256; }
@@ -3046,6 +3094,7 @@ static CarryDev carry_dev(const gfmc_problem *p, int64_t nw, void *carry,
     c.anti[0] = (int32_t *)(b + o + 2 * wb);
     c.anti[1] = (int32_t *)(b + o + 2 * wb + ab);
     c.parents = parents;
+    c.f_off = p->dp.fld ? carry_weights(p) : 0;
     c.row = row;
     c.on = 1;
     return c;
