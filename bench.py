@@ -53,28 +53,29 @@ def parse():
     ap.add_argument("--no-mlp", action="store_true",
                     help="skip the large learned-guide section (kagome 12x12, two-layer MLP, "
                          "262 144 walkers)")
+    ap.add_argument("--only-mlp", action="store_true",
+                    help="run the kagome MLP section alone (for profiling its kernels)")
     ap.add_argument("--mlp-walkers", type=int, default=262144)
     ap.add_argument("--mlp-steps", type=int, default=3)
     ap.add_argument("--race-train-iters", type=int, default=1500)
     ap.add_argument("--race-rbm-steps", type=int, default=1000)
     ap.add_argument("--summarize-profile", default=None,
                     help="print a text summary of a rocprofv3 database and exit")
-    ap.add_argument("--summarize-kernel", default="k_propagate<float, 2, 64")
-    ap.add_argument("--summarize-tail", default=None,
-                    help="kernel-pass,graph launch counts of the propagation kernel")
+    ap.add_argument("--summarize-kernel", action="append", default=None,
+                    help="PREFIX@N[,SKIP]: summarise the N dispatches (0: all) before the "
+                         "last SKIP of the kernels whose name starts with PREFIX (repeatable)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before the timed steps, copy a tensor of this many MiB once "
                          "(a plain copy kernel of known size, to scale HBM counters)")
     return ap.parse_args()
 
 
-def summarize_profile(path, prefix="k_propagate<float, 2, 64", tail=None):
+def summarize_profile(path, kernels):
     """Text summary of a rocprofv3 CSV output directory (--output-format
-    csv) of this script: the kernel statistics table; for the propagation
-    kernel (name prefix ``prefix``) the average duration of its last
-    launches, ``tail`` = (kernel pass, graph replays) launch counts; for a
-    --pmc run each counter's average over the same last launches of that
-    kernel and over every dispatch of each other kernel."""
+    csv) of this script: the kernel statistics table, then per kernel of
+    ``kernels`` [(name prefix, n_last, skip)] the average duration (kernel
+    trace) or counter value (--pmc) of the n_last dispatches before its last
+    ``skip`` ones (every dispatch when n_last = 0), and their sum."""
     import csv
     import glob
 
@@ -95,46 +96,62 @@ def summarize_profile(path, prefix="k_propagate<float, 2, 64", tail=None):
         stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
         lines.append("| kernel | calls | avg us | % |")
         lines.append("|---|---|---|---|")
-        for r in stats[:14]:
-            lines.append(f"| {short(r['Name'])} | {r['Calls']} | "
+        for r in stats[:16]:
+            lines.append(f"| {short(r['Name'])[:90]} | {r['Calls']} | "
                          f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
-    n_k, n_graph = tail if tail else (0, 0)
-    trace = [r for r in rows("kernel_trace.csv") if short(r["Kernel_Name"]).startswith(prefix)]
-    if trace and tail:
-        trace.sort(key=lambda r: int(r["Start_Timestamp"]))
-        d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in trace]
-        kp, gr = d[len(d) - n_graph - n_k:len(d) - n_graph], d[len(d) - n_graph:]
-        lines.append("")
-        lines.append(f"{prefix}...>: {len(d)} launches; kernel pass (last {n_k} before the "
-                     f"graph): avg {np.mean(kp):.4f} ms; graph replays ({n_graph}): avg "
-                     f"{np.mean(gr):.4f} ms")
+    trace = rows("kernel_trace.csv")
     pmc = rows("counter_collection.csv")
-    if pmc:
+    for prefix, n_last, skip in kernels:
+        tr = sorted((r for r in trace if short(r["Kernel_Name"]).startswith(prefix)),
+                    key=lambda r: int(r["Start_Timestamp"]))
+        if tr:
+            d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in tr]
+            d = d[:len(d) - skip]
+            last = d[-n_last:] if n_last else d
+            lines.append("")
+            lines.append(f"{prefix}...: {len(d) + skip} launches; {len(last)} before the last "
+                         f"{skip}: avg "
+                         f"{np.mean(last):.4f} ms, total {np.sum(last):.3f} ms")
         by = {}
         for r in pmc:
-            by.setdefault((short(r["Kernel_Name"]), r["Counter_Name"]), []).append(
-                (int(r["Dispatch_Id"]), float(r["Counter_Value"])))
-        lines.append("")
-        lines.append("| kernel | counter | avg per dispatch | dispatches |")
-        lines.append("|---|---|---|---|")
-        for (kn, cn), v in sorted(by.items(), key=lambda kv: -np.mean([x[1] for x in kv[1]])):
+            if short(r["Kernel_Name"]).startswith(prefix):
+                by.setdefault(r["Counter_Name"], []).append(
+                    (int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+        for cn, v in sorted(by.items()):
             v.sort()
-            if kn.startswith(prefix) and tail:
-                v = v[-(n_k + n_graph):]
-                kn += f" (last {len(v)})"
-            lines.append(f"| {kn[:70]} | {cn} | {np.mean([x[1] for x in v]):.1f} | {len(v)} |")
+            v = v[:len(v) - skip]
+            last = [x[1] for x in (v[-n_last:] if n_last else v)]
+            lines.append(f"{prefix}... {cn}: {len(v) + skip} dispatches; {len(last)} before the "
+                         f"last {skip}: avg "
+                         f"{np.mean(last):.1f}, sum {np.sum(last):.1f}")
+    # counters of the copy kernel (calibration of the HBM byte counters)
+    cp = [r for r in pmc if "copyBuffer" in r["Kernel_Name"]]
+    if cp:
+        top = max(cp, key=lambda r: float(r["Counter_Value"]))
+        lines.append(f"largest device copy (grid {top['Grid_Size']}): {top['Counter_Name']} = "
+                     f"{float(top['Counter_Value']):.1f}")
     return "\n".join(lines)
 
 
-def profiled_traffic(stem):
-    """The HBM bytes per launch recorded in profiles/<stem>.md (a line
-    ``traffic_bytes_per_launch: <int>``), or None."""
+def calib_copy(mib):
+    """One device copy of ``mib`` MiB (a plain copy kernel of known size, to
+    scale the HBM byte counters of a --pmc run)."""
+    import torch
+    src = torch.ones(mib * (1 << 18), dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def profiled_traffic(stem, key="traffic_bytes_per_launch"):
+    """The HBM bytes recorded in profiles/<stem>.md (a line ``<key>: <int>``),
+    or None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", stem + ".md")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return {"traffic_bytes_per_launch": float(line.split(":")[1]),
                         "source": "profiles/" + stem + ".md"}
     return None
@@ -148,11 +165,12 @@ def progress(msg):
 def main():
     args = parse()
     if args.summarize_profile:
-        tail = None
-        if args.summarize_tail:
-            a, b = args.summarize_tail.split(",")
-            tail = (int(a), int(b))
-        print(summarize_profile(args.summarize_profile, args.summarize_kernel, tail))
+        ks = []
+        for k in args.summarize_kernel or ["k_propagate<float, 2, 64@60"]:
+            pre, _, n = k.rpartition("@") if "@" in k else (k, "", "0")
+            n_last, _, skip = n.partition(",")
+            ks.append((pre, int(n_last), int(skip or 0)))
+        print(summarize_profile(args.summarize_profile, ks))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; runs on more than one GPU "
@@ -162,6 +180,11 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.only_mlp:
+        if args.calib_copy_mib > 0:
+            calib_copy(args.calib_copy_mib)
+        print(json.dumps({"kagome12_mlp": kagome_mlp_section(args)}), flush=True)
+        return
     from pygfmc_amd import JastrowSpinHalf, SpinHalf, prepare_walkers_sz, square
     from pygfmc_amd import _lib
     from pygfmc_amd.engine import Chain
@@ -208,12 +231,7 @@ def main():
         lanes, tune_times = int(args.lanes), None
 
     if args.calib_copy_mib > 0:
-        src = torch.ones(args.calib_copy_mib * (1 << 18), dtype=torch.float32,
-                         device=chain.spins.device)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calib_copy(args.calib_copy_mib)
 
     # kernel pass: per-step launches, HIP events around the propagation
     # kernel on its launch stream (the roofline's kernel time and work)
@@ -454,11 +472,16 @@ def kagome_mlp_section(args):
     cnt0 = (ctypes.c_int64 * 2)()
     _lib.check(_lib.load().gfmc_mlp_counters(p.handle, cnt0), "gfmc_mlp_counters")
     chain.evals.zero_()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.mlp_steps)]
+    ser = torch.empty((args.mlp_steps, 2), dtype=torch.float64, device=chain.spins.device)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ser = chain.steps(args.mlp_steps, beta, alw, 11, 1, evals=True)
+    for k in range(args.mlp_steps):
+        chain.step(beta, alw, 11, 1 + k, ser[k], events=ev[k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    prop_s = float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
     cnt1 = (ctypes.c_int64 * 2)()
     _lib.check(_lib.load().gfmc_mlp_counters(p.handle, cnt1), "gfmc_mlp_counters")
     blocks = cnt1[0] - cnt0[0]
@@ -466,6 +489,24 @@ def kagome_mlp_section(args):
     flop = blocks * 16 * 2.0 * 512 * 256
     if not np.all(np.isfinite(ser[:, 0].cpu().numpy())):
         raise SystemExit("population extinct in the MLP section")
+    evals_ws = evals / (nw * args.mlp_steps)
+    prof = profiled_traffic(f"k_mlp_gemm_kagome12x12_mlp512x256_w{nw}", "traffic_bytes_per_step")
+    roof = {"bound": "mfma", "compute_unit": "BF16 MFMA 16x16x32, FP32 accumulation (layer 2)",
+            "achieved": flop / args.mlp_steps / prop_s / 1e12, "peak": 2500.0,
+            "unit": "TFLOP/s", "frac": flop / args.mlp_steps / prop_s / 2.5e15,
+            "work": "layer-2 FLOPs 2 * 512 * 256 per network row over the 16-row blocks the "
+                    "propagation processed (padding rows included)",
+            "kernel": "k_mlp_gemm<16, false> (the propagation of one branching step, timed "
+                      "with HIP events around it, includes its row selection and scans)",
+            "kernel_avg_ms": prop_s * 1e3,
+            "traffic": prof["traffic_bytes_per_launch"] if prof else None,
+            "traffic_unit": "HBM bytes per branching step (rocprofv3 --pmc FETCH_SIZE + "
+                            "WRITE_SIZE over the step's k_mlp_gemm launches, copy-calibrated)",
+            "traffic_source": prof["source"] if prof else None}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(H, G, chain, lat.N, beta, alw, 11, args.cpu_seconds,
+                           beta_sample=0.002, evals_per_ws=evals_ws, max_walkers=256)
     n_s = 4096
     sub = chain.spins[:n_s].contiguous()
     vmin_gradient(H, G, sub)   # untimed (workspace)
@@ -477,14 +518,15 @@ def kagome_mlp_section(args):
     return {"workload": f"kagome12x12_xxz_mlp512x256_bf16_w{nw}_step0.2",
             "walker_steps_per_s": nw * args.mlp_steps / dt,
             "s_per_step": dt / args.mlp_steps, "steps": args.mlp_steps,
-            "evals_per_walker_step": evals / (nw * args.mlp_steps),
+            "evals_per_walker_step": evals_ws,
             "rows_per_eval": blocks * 16 / evals,
             "hop_iterations_per_step": (cnt1[1] - cnt0[1]) / args.mlp_steps,
             "mfma_layer2_tflops_per_s": flop / dt / 1e12,
             "mfma_layer2_frac_of_bf16_peak": flop / dt / 2.5e15,
             "train_gradient_seconds_4096_samples": t_grad,
             "note": "layer-2 FLOP counted over 16-row blocks (padding included); the "
-                    "per-row layer-1 activations (512 tanh + BF16) run on VALU"}
+                    "per-row layer-1 activations (512 tanh + BF16) run on VALU",
+            "roofline": roof, "cpu_baseline": cpu}
 
 
 def timed_run(walkers, n_steps, H, G, p_array, seed, warmup):
@@ -616,11 +658,18 @@ def rbm_kernel_measure(H, G, cpu_seconds):
     t = float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
     evals = float(ch.evals.to(torch.float64).sum().item()) / n_k
     flop = evals * len(H.edge_i) * G.hidden * 8.0
-    roof = {"bound": "mfma", "compute_unit": "FP32 vector ALU (no GEMM in the hop; the MI355X "
+    prof = profiled_traffic("k_propagate_rbh_heisenberg_square10x10_rbm400_w16384")
+    roof = {"bound": "valu", "compute_unit": "FP32 vector ALU (no GEMM in the hop; the MI355X "
                                               "FP32 vector and matrix peaks are both 157.3 "
                                               "TFLOP/s)",
             "achieved": flop / t / 1e12, "peak": 157.3, "unit": "TFLOP/s",
-            "frac": flop / t / 1e12 / 157.3, "traffic": None,
+            "frac": flop / t / 1e12 / 157.3,
+            "work": "reference-equivalent FLOPs: every slot's log-ratio over all hidden units, "
+                    "n_edges H 8 per rate evaluation",
+            "traffic": prof["traffic_bytes_per_launch"] if prof else None,
+            "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, "
+                            "copy-calibrated)",
+            "traffic_source": prof["source"] if prof else None,
             "kernel": "k_propagate_rbh<7>", "kernel_avg_ms": t * 1e3,
             "evals_per_walker_step": evals / nw, "flop_per_eval": len(H.edge_i) * G.hidden * 8.0,
             "transcendentals_per_eval": len(H.edge_i) * G.hidden * 2.0}
