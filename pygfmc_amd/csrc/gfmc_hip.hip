@@ -58,6 +58,7 @@ struct CarryDev {
     const int32_t *parents;
     int64_t row;     // elements per walker row
     int64_t f_off;   // element offset of the local fields in a row (0: none)
+    int64_t s_off;   // element offset of the walker's final spins in a row
     int on;
 };
 static thread_local const CarryDev *g_carry = nullptr;
@@ -560,7 +561,8 @@ __device__ __forceinline__ void load_spins(const DevProblem &P, const uint32_t *
 // lanes.  Bits past site N - 1 stay clear.
 template <int G = WAVE>
 __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
-                                            const signed char *x, int gl) {
+                                            const signed char *x, int gl,
+                                            uint32_t *copy = nullptr) {
     const int nq = (P.N + 3) >> 2;
     for (int t0 = 0; t0 < P.words * 8; t0 += G) {
         const int t = t0 + gl;
@@ -576,7 +578,10 @@ __device__ __forceinline__ void store_spins(const DevProblem &P, uint32_t *sp,
         w |= (uint32_t)__shfl_xor((int)w, 1, G);
         w |= (uint32_t)__shfl_xor((int)w, 2, G);
         w |= (uint32_t)__shfl_xor((int)w, 4, G);
-        if ((t & 7) == 0 && t < P.words * 8) sp[t >> 3] = w;
+        if ((t & 7) == 0 && t < P.words * 8) {
+            sp[t >> 3] = w;
+            if (copy) copy[t >> 3] = w;
+        }
     }
 }
 
@@ -886,6 +891,17 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
         if (cy.on) {
             cur = cy.state[1];
             carried = cy.state[0] != 0 && cy.state[2] == G;
+            if (carried) {
+                // the parent's row holds the spins it ended its step with;
+                // a walker whose spins differ (changed outside the step
+                // loop) re-evaluates everything
+                const uint32_t *cs = (const uint32_t *)((const T *)cy.w[cur] +
+                                                        (int64_t)cy.parents[wk] * cy.row +
+                                                        cy.s_off);
+                bool same = true;
+                for (int q = gl; q < P.words; q += G) same = same && cs[q] == sp[q];
+                carried = grp_ballot<G>(!same, lane) == 0ull;
+            }
         }
     }
     if (carried) {
@@ -1039,7 +1055,10 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
         ++it;
         lw = lw - alw * (double)n_bra;
     }
-    store_spins<G>(P, sp, L.x, gl);
+    store_spins<G>(P, sp, L.x, gl,
+                   FAST && cy.on ? (uint32_t *)((T *)cy.w[cur ^ 1] + (int64_t)wk * cy.row +
+                                                cy.s_off)
+                                 : nullptr);
     if constexpr (FAST) {
         if (cy.on) {
             wave_sync();
@@ -3061,12 +3080,19 @@ static int64_t carry_weights(const gfmc_problem *p) {
 }
 
 // ... followed, for the local-field form, by the walker's fields (f_pad
-// bytes, a multiple of 16)
+// bytes, a multiple of 16), then by the spins the walker ended its step with
+// (the words rounded up to 16 bytes), against which the next step checks the
+// spins it loads
+static int64_t carry_spins_off(const gfmc_problem *p) {
+    const int64_t el = p->precision == GFMC_FP64 ? 8 : 4;
+    return carry_weights(p) + (p->dp.fld ? p->dp.f_pad / el : 0);
+}
+
 static int64_t carry_row(const gfmc_problem *p) {
     const int64_t w = carry_weights(p);
     if (w == 0) return 0;
     const int64_t el = p->precision == GFMC_FP64 ? 8 : 4;
-    return w + (p->dp.fld ? p->dp.f_pad / el : 0);
+    return carry_spins_off(p) + ((int64_t)p->dp.words * 4 + 15) / 16 * 16 / el;
 }
 
 static size_t carry_align(size_t b) { return (b + 255) / 256 * 256; }
@@ -3095,6 +3121,7 @@ static CarryDev carry_dev(const gfmc_problem *p, int64_t nw, void *carry,
     c.anti[1] = (int32_t *)(b + o + 2 * wb + ab);
     c.parents = parents;
     c.f_off = p->dp.fld ? carry_weights(p) : 0;
+    c.s_off = carry_spins_off(p);
     c.row = row;
     c.on = 1;
     return c;

@@ -344,16 +344,20 @@ def estimate_average_log_branch_weight(walkers, hamiltonian, guide, method="cont
 
 # -- the driver ----------------------------------------------------------------
 
-def _warmup(chain, key, step_f, alw, warmup_steps, verbosity, method="cont_time", lam=None):
+def _warmup(chain, key, step_f, alw, warmup_steps, verbosity, method="cont_time", lam=None,
+            trace=None):
     """Four-stage ramp; the shift is updated per stage unless pinned
-    (driver.py:713-788): alw <- alw + (LSE(log G_stage) - log n) / step."""
+    (driver.py:713-788): alw <- alw + (LSE(log G_stage) - log n) / step.
+    ``trace`` (a list) receives (step size, steps, measured shift) per
+    stage."""
     pinned = alw is not None
     if not pinned:
         _, e, _ = chain.p.rates(chain.spins, want_rates=False)
         alw = float(torch.mean(-e).item() if lam is None else
                     torch.mean(torch.log(lam - e)).item())
     counter = key.position
-    for i, (step_, n_w) in enumerate(warmup_schedule(step_f, warmup_steps, method)):
+    stages = warmup_schedule(step_f, warmup_steps, method)
+    for i, (step_, n_w) in enumerate(stages):
         t0 = time.time()
         series = chain.steps(n_w, step_, alw, key.seed, counter, warmup=True)
         counter += n_w
@@ -362,12 +366,25 @@ def _warmup(chain, key, step_f, alw, warmup_steps, verbosity, method="cont_time"
         m = np.max(lg)
         lse = m + np.log(np.sum(np.exp(lg - m)))
         measured = alw + (lse - np.log(n_w)) / step_
+        if trace is not None:
+            trace.append((step_, n_w, float(measured)))
         if verbosity >= 8:
-            print(f"[gfmc] - [run_warmup {i + 1}/4] - step={step_}, {n_w} steps @ "
-                  f"{n_w / max(time.time() - t0, 1e-9):.1f} it/s, avg_log_weight = "
-                  f"{measured:.6f}", flush=True)
+            label = f"{i + 1}/{len(stages)}"
+            rate = n_w / max(time.time() - t0, 1e-9)
+            if method == "cont_time":
+                print(f"[gfmc] - [run_warmup {label}] - step={step_}, {n_w} steps @ "
+                      f"{rate:.1f} it/s, avg_log_weight = {measured:.6f}", flush=True)
+            else:
+                print(f"[gfmc] - [run_warmup {label}] - step={step_}, {n_w} steps @ "
+                      f"{rate:.1f} it/s, average_b_x = {np.exp(measured):.6f}", flush=True)
+            if pinned:
+                print(f"[gfmc] - [run_warmup {label}] - alw pinned at {alw:.6f}; this stage "
+                      f"would have measured {measured:.6f}", flush=True)
         if not pinned:
             alw = float(measured)
+    if not stages and verbosity >= 8:
+        print(f"[gfmc] - [run_gfmc] - no warmup (warmup_steps=0), avg_log_weight={alw:.6f}",
+              flush=True)
     return alw
 
 
