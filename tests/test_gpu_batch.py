@@ -140,3 +140,37 @@ def test_batch_tfim_energies_match_free_fermions():
         e0 = tfim_chain_ground_energy(L, 1.0, h)
         say(f"  L={L} h={h:.3f}: E {E[64]:.5f} +- {err[64]:.5f} exact {e0:.5f}")
         assert abs(E[64] - e0) < 4 * err[64] + 1e-5 * abs(e0)
+
+
+def test_batch_tfim_longest_chains_match_free_fermions():
+    """The twelve longest chains of the 64-chain draw (L ~ U{16..256}, h ~
+    U[0.5, 1.5] from seed 0; 210 <= L <= 256) away from the critical region
+    (h < 0.8 or h > 1.1) in one batch: E(p = 64) within 4 sigma of the
+    free-fermion energy (Pfeuty 1970).  The guides are the Jastrow couplings
+    of least energy variance for such fields (beta -0.45 below the
+    transition, -0.2 above it), and the ordered chains start fully
+    polarised: from random configurations their populations keep a pair of
+    domain walls (energy 4 (J - h)) for longer than the run."""
+    say("batch tfim longest chains")
+    from pygfmc_amd.walkers import Walker
+    rng = np.random.default_rng(0)
+    Ls_all = rng.integers(16, 257, 64)
+    hs_all = rng.uniform(0.5, 1.5, 64)
+    keep = [i for i in np.argsort(-Ls_all) if hs_all[i] < 0.8 or hs_all[i] > 1.1][:12]
+    Ls = [int(Ls_all[i]) for i in keep]
+    hs = [float(hs_all[i]) for i in keep]
+    assert min(Ls) >= 210
+    Hs = [transverse_field_ising(chain(L).edges, 1.0, h, dtype=np.float32)
+          for L, h in zip(Ls, hs)]
+    Gs = [JastrowSpinHalf(L, chain(L).edges, beta=-0.45 if h < 1 else -0.2, dtype=np.float32)
+          for L, h in zip(Ls, hs)]
+    walkers = [Walker(np.ones((2048, L))) if h < 1 else prepare_walkers_random(2048, L, l)
+               for l, (L, h) in enumerate(zip(Ls, hs))]
+    _, raws, _ = run_batch(walkers, 1500, Hs, 0.2, Gs, [4, 16, 64], rng_key=7, warmup_steps=400)
+    for l, (L, h) in enumerate(zip(Ls, hs)):
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            E, _, err, _ = evaluate_observables(raws[l], n_blocks=20)
+        e0 = tfim_chain_ground_energy(L, 1.0, h)
+        say(f"  L={L} h={h:.3f}: E {E[64]:.5f} +- {err[64]:.5f} exact {e0:.5f}")
+        assert abs(E[64] - e0) < 4 * err[64] + 1e-5 * abs(e0)
