@@ -586,11 +586,14 @@ int gfmc_graph_capture_steps_diag(const gfmc_problem *p, int64_t n_walkers, uint
  * weights, and the next propagation starts each walker from its parent's
  * stored weights instead of re-evaluating every slot.  The stored weights
  * are exactly what a full evaluation of the same configuration gives, so
- * the results are bit-identical to runs without a carry.  Used by the
- * record path (uniform couplings, Jastrow or no guide); 0 bytes otherwise.
- * The buffer is caller-owned, 256-byte aligned, and starts (or restarts,
- * e.g. after the walkers were changed outside these loops) with its first
- * 256 bytes zero.
+ * the results are bit-identical to runs without a carry.  A row holds the
+ * walker's slot weights, its local fields (the FP32 records' local-field
+ * form) and the spins it ended the step with; a walker whose spins differ
+ * from its parent row's (changed outside these loops) re-evaluates every
+ * slot.  Used by the record path (uniform couplings, Jastrow or no guide);
+ * 0 bytes otherwise.  The buffer is caller-owned, 256-byte aligned, and
+ * starts with its first 256 bytes zero (zeroing them again forgets every
+ * row).
  */
 int64_t gfmc_carry_bytes(const gfmc_problem *p, int64_t n_walkers);
 

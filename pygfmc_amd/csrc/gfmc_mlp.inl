@@ -748,8 +748,9 @@ static int mlp_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, dou
                          double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
                          uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st) {
     if (g_step_base)
-        return fail(GFMC_EUNSUP, "the MLP guide's propagation synchronises with the host once "
-                                 "per hop and cannot be captured into a graph");
+        return fail(GFMC_EUNSUP, "the MLP guide's propagation looks at its row count on the "
+                                 "host every 16 hop iterations and cannot be captured into a "
+                                 "graph");
     int rc = mlp_start(p, nw, spins, log_w, 0, st);
     if (rc) return rc;
     // hop iterations of every active walker (scan -> GEMM -> select), the

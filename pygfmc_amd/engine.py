@@ -5,8 +5,8 @@ walkers/reconfiguration.py), driving the HIP kernels.
 Methods: continuous time (``"cont_time"``, evolution.py:559-721) and the
 discrete ``"single_step"`` / ``"multi_step"`` propagators with the shift
 ``lam`` (evolution.py:322-556).  Diagonal operators are measured by forward
-walking on the device (driver.py:240-288); off-diagonal operators (the
-shadow-chain estimator) are not implemented and raise.
+walking on the device (driver.py:240-288), off-diagonal operators by
+shadow chains (driver.py:290-331, evolution.py:848-1101; ``offdiagfw``).
 """
 from __future__ import annotations
 
@@ -484,7 +484,8 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     dev = chain.spins.device
     series = torch.empty((n_steps, 2), dtype=torch.float64, device=dev)
     m_array = [q // 2 for q in p_sorted]
-    # the MLP guide's propagation synchronises with the host once per hop
+    # the MLP guide's propagation looks at its row count on the host every
+    # 16 hop iterations (a data-dependent loop length), so it is not captured
     graphable = p.guide_kind != _lib.GUIDE_MLP
     diag = None
     dseries = None
