@@ -334,6 +334,8 @@ def main():
     }
     progress(f"headline done: {value:.4g} walker-steps/s")
     if not args.no_configs:
+        out["xxz_square32_deltas"] = xxz_deltas_section(args)
+        progress("xxz deltas done")
         out["heisenberg_chain100"] = chain100_section()
         progress("chain100 done")
         out["batched_tfim_chains"] = tfim_batch_section()
@@ -347,6 +349,47 @@ def main():
         out["sigma_race"] = sigma_race(args)
         progress("sigma race done")
     print(json.dumps(out), flush=True)
+
+
+def xxz_deltas_section(args):
+    """The 32 x 32 XXZ configuration at its three anisotropies (Delta = 0.5,
+    1, 2): the same protocol as the headline (the 1000-step warm-up ramp,
+    500 untimed steps, then 20 steps as replays of one captured 10-step
+    graph) with the autotuned lanes per walker; walker-steps/s and the rate
+    evaluations per walker-step of the timed steps."""
+    import torch
+
+    from pygfmc_amd import JastrowSpinHalf, SpinHalf, prepare_walkers_sz, square
+    from pygfmc_amd.autotune import tune_problem
+    from pygfmc_amd.engine import Chain, StepGraph, _warmup
+    from pygfmc_amd.problem import get_problem
+    from pygfmc_amd.walkers import PhiloxKey
+    lat = square(32, 32)
+    res = {}
+    for delta in (0.5, 1.0, 2.0):
+        H = SpinHalf.create(lat.edges, J_x=-1.0, J_y=-1.0, J_z=delta, dtype=np.float32)
+        G = JastrowSpinHalf(1024, lat.edges, beta=0.3)
+        p = get_problem(H, G)
+        p.set_lanes(64)
+        chain = Chain(p, prepare_walkers_sz(args.walkers, 1024, 12345))
+        alw = _warmup(chain, PhiloxKey(777, 0), 0.2, None, args.gfmc_warmup, 0)
+        chain.steps(500, 0.2, alw, 777, 1 << 20)
+        lanes, _ = tune_problem(p, chain.spins, 0.2, alw)
+        graph = StepGraph(chain, 10, 0.2, alw, 777, evals=True)
+        pos = (1 << 20) + 500
+        graph.launch(pos)
+        chain.evals.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(2):
+            graph.launch(pos + 10 * (r + 1))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[f"delta{delta:g}"] = {
+            "walker_steps_per_s": args.walkers * 20 / dt, "ms_per_step": dt / 20 * 1e3,
+            "evals_per_walker_step": float(chain.evals.to(torch.float64).sum().item())
+            / (args.walkers * 20), "lanes_per_walker": lanes}
+    return {"workload": f"xxz_square32x32_jastrow0.3_w{args.walkers}_step0.2_fp32", **res}
 
 
 def chain100_section():

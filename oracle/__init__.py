@@ -15,7 +15,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# PYGFMC_ORACLE_LIB selects another build of the same source (the
+# sanitizer build of build.build_sanitized, run by the tests)
+LIB_PATH = os.environ.get("PYGFMC_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)

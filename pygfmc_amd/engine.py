@@ -23,6 +23,7 @@ from .estimators import (check_population, estimate_diagonal, estimate_energy,
                          estimate_off_diagonal, validated_int,
                          validated_p_array)
 from .problem import get_problem
+from .trace import span
 from .walkers import PhiloxKey, Walker, as_key
 
 __all__ = ["run", "run_gfmc", "evolve_walkers", "reconfiguration", "systematic_parents",
@@ -359,7 +360,8 @@ def _warmup(chain, key, step_f, alw, warmup_steps, verbosity, method="cont_time"
     stages = warmup_schedule(step_f, warmup_steps, method)
     for i, (step_, n_w) in enumerate(stages):
         t0 = time.time()
-        series = chain.steps(n_w, step_, alw, key.seed, counter, warmup=True)
+        with span(f"gfmc warmup {i + 1}/{len(stages)} step={step_:g} n={n_w}"):
+            series = chain.steps(n_w, step_, alw, key.seed, counter, warmup=True)
         counter += n_w
         lg = series[:, 0].cpu().numpy()
         check_population(lg, 0)
@@ -515,31 +517,32 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     while step0 < n_steps or pending is not None:
         if step0 < n_steps:
             n = min(print_every, n_steps - step0)
-            chain.evals.zero_()
-            drows = dseries[step0:step0 + n] if diag is not None else None
-            if od is not None:
-                # one step at a time: a measured step launches the shadow
-                # chains right after its resampling (driver.py:290-331)
-                for k in range(n):
-                    t = step0 + k
-                    chain.steps(1, step_f, alw, key.seed, key.position + t,
-                                series=series[t:t + 1], evals=True, diag=diag,
-                                dseries=drows[k:k + 1] if diag is not None else None)
-                    if measured[t]:
-                        r = int(od_row[t])
-                        od.measure(chain.spins, step_f, alw, key.seed, key.position + t,
-                                   od_fw[r], od_norm[r])
-            elif (use_graph and graphable and n == print_every
-                    and n_steps >= 2 * print_every):
-                if graph is None:
-                    graph = StepGraph(chain, n, step_f, alw, key.seed, evals=True, diag=diag)
-                graph.launch(key.position + step0, out=series[step0:step0 + n], dout=drows)
-            else:
-                chain.steps(n, step_f, alw, key.seed, key.position + step0,
-                            series=series[step0:step0 + n], evals=True, diag=diag,
-                            dseries=drows)
-            fw.update(series[step0:step0 + n])
-            ev = chain.evals.to(torch.float64).mean() / n
+            with span(f"gfmc production steps {step0}-{step0 + n - 1}"):
+                chain.evals.zero_()
+                drows = dseries[step0:step0 + n] if diag is not None else None
+                if od is not None:
+                    # one step at a time: a measured step launches the shadow
+                    # chains right after its resampling (driver.py:290-331)
+                    for k in range(n):
+                        t = step0 + k
+                        chain.steps(1, step_f, alw, key.seed, key.position + t,
+                                    series=series[t:t + 1], evals=True, diag=diag,
+                                    dseries=drows[k:k + 1] if diag is not None else None)
+                        if measured[t]:
+                            r = int(od_row[t])
+                            od.measure(chain.spins, step_f, alw, key.seed, key.position + t,
+                                       od_fw[r], od_norm[r])
+                elif (use_graph and graphable and n == print_every
+                        and n_steps >= 2 * print_every):
+                    if graph is None:
+                        graph = StepGraph(chain, n, step_f, alw, key.seed, evals=True, diag=diag)
+                    graph.launch(key.position + step0, out=series[step0:step0 + n], dout=drows)
+                else:
+                    chain.steps(n, step_f, alw, key.seed, key.position + step0,
+                                series=series[step0:step0 + n], evals=True, diag=diag,
+                                dseries=drows)
+                fw.update(series[step0:step0 + n])
+                ev = chain.evals.to(torch.float64).mean() / n
             nxt = (step0, n, ev)
             step0 += n
         else:
