@@ -802,7 +802,7 @@ __device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLd
         if (s >= 0)
             L.w[wpos_g<G>(s, G == WAVE ? P.K : P.Kg[gidx<G>()])] = eval_rec(P, tb, m.rec, L.x);
     }
-    if (P.has_jz && !P.uniform_jz) {
+    if (FAST != 2 && P.has_jz && !P.uniform_jz) {
         for (int t = gl; t < P.RE; t += G) {
             const int e = P.mv_edges[(size_t)idx * P.RE + t];
             if (e >= 0) {
@@ -940,10 +940,15 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
             n_anti = apply_move_rec<T, FAST, G>(P, tb, L, m, idx, n_anti, lane);
         else
             n_anti = apply_move(P, tb, L, idx, n_anti, lane);
-        if (!(P.has_jz && P.uniform_jz) || P.has_hz)
-            diag = diag_value<G>(P, L.x, L.anti, n_anti, gl);
-        else
-            diag = (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti);
+        if constexpr (FAST == 2) {
+            // the local-field form implies uniform J_z (or none) and no h_z
+            diag = P.has_jz ? (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti) : 0.0;
+        } else {
+            if (!(P.has_jz && P.uniform_jz) || P.has_hz)
+                diag = diag_value<G>(P, L.x, L.anti, n_anti, gl);
+            else
+                diag = (P.jz0 / 4) * (double)((P.n_edges - n_anti) - n_anti);
+        }
     };
     if constexpr (METHOD == GFMC_METHOD_CONT) {
         const double beta = step_size;
@@ -2543,7 +2548,10 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     P.RE = hr.RE;
     P.flags = d->has_jz && (!d->uniform_jz || !P.fast);
     // the local-field form runs in the FP32 kernels; its zero spin is site N
-    P.fld = P.fast && hr.fld && precision == GFMC_FP32 && d->guide == GFMC_GUIDE_JASTROW;
+    // (its kernels take the diagonal from the anti-parallel count: uniform
+    // J_z or none, no h_z)
+    P.fld = P.fast && hr.fld && precision == GFMC_FP32 && d->guide == GFMC_GUIDE_JASTROW &&
+            !d->has_hz && (!d->has_jz || d->uniform_jz);
     P.FD = P.fld ? hr.FD : 0;
     P.n_pad = round_up(N + (P.fld ? 1 : 0), 16);
     P.e_pad = P.flags ? round_up(E > 0 ? E : 1, 16) : 0;
