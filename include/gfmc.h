@@ -460,6 +460,17 @@ int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *
  * blocks through the network (each 16 x h1 x h2 multiply-adds on MFMA),
  * out[1] bulk-synchronous hop iterations. */
 int gfmc_mlp_counters(const gfmc_problem *p, int64_t *out);
+/* A propagation captured into a graph (gfmc_graph_capture_steps*) cannot
+ * look at its row count on the host, so each captured step runs hop_cap
+ * bulk-synchronous hop iterations (default 512; iterations after the last
+ * walker finished are empty launches) and then counts, on the device, a
+ * step that still had a walker unfinished.  gfmc_mlp_unfinished: out[0]
+ * that count, out[1] the largest number of hop iterations one step has
+ * needed, both since the workspace was attached (a caller sizes the cap from
+ * out[1] of eager steps and checks out[0] after the replays).  No
+ * counterpart in the reference (its step is one XLA scan). */
+int gfmc_mlp_set_hop_cap(gfmc_problem *p, int32_t hop_cap);
+int gfmc_mlp_unfinished(const gfmc_problem *p, int64_t *out);
 /* The MLP guide's propagation workspace is the caller's (no counterpart in
  * the reference, whose guide evaluation is XLA's): per walker the exact FP64
  * layer-1 pre-activations, the candidate slots and the hold state, per hop

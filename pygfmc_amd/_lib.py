@@ -172,6 +172,8 @@ SIGNATURES = [
                                       ctypes.c_float, _vp]),
     ("gfmc_mlp_n_params", ctypes.c_int64, [_vp]),
     ("gfmc_mlp_counters", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    ("gfmc_mlp_set_hop_cap", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("gfmc_mlp_unfinished", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     ("gfmc_mlp_work_bytes", ctypes.c_int64, [_vp, ctypes.c_int64]),
     ("gfmc_mlp_set_work", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp]),
     ("gfmc_mlp_set_params", ctypes.c_int, [_vp, _vp, _vp]),

@@ -128,13 +128,14 @@ struct DevProblem {
     const int *edge_pack;    // [n_edges] i | (j << 16)
     // local-field form of the records (FP32 kernels, when fld): records of two
     // words {slot | i << 16, j | flags << 16} in the LDS order of each group
-    // size and per move; the field updates of each move, mv_fnb[s * FD + t] =
+    // size, and per move and group size {LDS position | i << 16, ...}; the
+    // field updates of each move, mv_fnb[s * FD + t] =
     // partner | (flipped site << 16), t < D for site i and D <= t < 2D for
     // site j; every site's partners, 8 int16 per site, site N (a zero spin)
     // as padding; f_pad bytes of fields per walker
     int fld, FD, f_pad;
     const uint2 *slot_rf_g[3];
-    const uint2 *mv_rf;      // [n_off * R]
+    const uint2 *mv_rf_g[3]; // [n_off * R] per group layout, first field the LDS position
     const int *mv_fnb;       // [n_off * FD]
     const uint4 *fsite;      // [N]
     // RBM guide (FP32 arithmetic on BF16-valued parameters)
@@ -750,11 +751,12 @@ struct MovePrefetch {
     int sites;
 };
 
-template <int FAST>
+template <int FAST, int G>
 __device__ __forceinline__ MovePrefetch prefetch_move(const DevProblem &P, int idx, int gl) {
     MovePrefetch m;
     if constexpr (FAST == 2) {
-        m.rf = gl < P.R ? P.mv_rf[(size_t)idx * P.R + gl] : make_uint2(0xffffffffu, 0xffffffffu);
+        m.rf = gl < P.R ? P.mv_rf_g[gidx<G>()][(size_t)idx * P.R + gl]
+                        : make_uint2(0xffffffffu, 0xffffffffu);
         m.fnb = gl < P.FD ? P.mv_fnb[(size_t)idx * P.FD + gl] : P.N | (P.N << 16);
     } else {
         m.rec = gl < P.R ? load_rec(P.mv_rec, (size_t)idx * P.R + gl) : empty_rec();
@@ -794,9 +796,7 @@ __device__ int apply_move_rec(const DevProblem &P, const Tables<T> &tb, WalkerLd
         wave_sync();
         if (gl >= P.max_deg && gl < P.FD) L.f[fp] = (signed char)(L.f[fp] + 2 * L.x[ff]);
         wave_sync();
-        const int s = (int16_t)(m.rf.x & 0xffffu);
-        if (s >= 0)
-            L.w[wpos_g<G>(s, G == WAVE ? P.K : P.Kg[gidx<G>()])] = eval_recf(tb, m.rf, L.x, L.f);
+        if (m.rf.x != 0xffffffffu) L.w[m.rf.x & 0xffffu] = eval_recf(tb, m.rf, L.x, L.f);
     } else {
         const int s = rec_f(m.rec, 0);
         if (s >= 0)
@@ -968,7 +968,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                 T cap = below(R);
                 if (u > cap) u = cap;
                 idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
-                if constexpr (FAST) m = prefetch_move<FAST>(P, idx, gl);  // in flight during the hold math
+                if constexpr (FAST) m = prefetch_move<FAST, G>(P, idx, gl);  // in flight during the hold math
                 if constexpr (FAST == 2) {
                     // FP32 kernels: -log(1 - psi) / R in FP32 (v_log_f32), the
                     // precision the rates carry
@@ -1011,7 +1011,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                 // through log W), also when there are no off-diagonal slots
                 if (W > (T)0 && !(u < w0)) {
                     idx = sample_slot<T, G>(P, L, incl, lane_sum, u - w0, lane);
-                    if constexpr (FAST) m = prefetch_move<FAST>(P, idx, gl);
+                    if constexpr (FAST) m = prefetch_move<FAST, G>(P, idx, gl);
                 }
                 lw = lw + log(w0d + (double)R);
                 ++it;
@@ -1035,7 +1035,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
                     T cap = below(R);
                     if (u > cap) u = cap;
                     idx = sample_slot<T, G>(P, L, incl, lane_sum, u, lane);
-                    if constexpr (FAST) m = prefetch_move<FAST>(P, idx, gl);
+                    if constexpr (FAST) m = prefetch_move<FAST, G>(P, idx, gl);
                 }
                 const int remaining = n_bra - used;
                 const double p_hop = (double)R / bx;
@@ -1985,7 +1985,7 @@ struct HostRecords {
     double J0 = 0.0, h0 = 0.0;
     // local-field form (DevProblem::fld)
     int fld = 0, FD = 0;
-    std::vector<uint32_t> slot_rf_g[3], mv_rf, fsite;   // 2 words per record, 4 per site
+    std::vector<uint32_t> slot_rf_g[3], mv_rf_g[3], fsite;   // 2 words per record, 4 per site
     std::vector<int> mv_fnb;
 };
 
@@ -2046,13 +2046,21 @@ static void build_field_records(const gfmc_problem_desc *d, HostRecords &hr) {
             hr.slot_rf_g[gi][p * 2 + 1] = rf[(size_t)s * 2 + 1];
         }
     }
-    hr.mv_rf.assign((size_t)n_off * hr.R * 2, 0xffffffffu);
-    for (int s = 0; s < n_off; ++s)
-        for (int t = d->dep_off[s]; t < d->dep_off[s + 1]; ++t) {
-            const size_t o = ((size_t)s * hr.R + (t - d->dep_off[s])) * 2;
-            hr.mv_rf[o] = rf[(size_t)d->dep_slots[t] * 2];
-            hr.mv_rf[o + 1] = rf[(size_t)d->dep_slots[t] * 2 + 1];
-        }
+    // per move, in the first field the LDS position of the slot in each
+    // group layout instead of the slot index
+    for (int gi = 0; gi < 3; ++gi) {
+        const int G = WAVE >> gi;
+        const int K = (n_off + G - 1) / G;
+        hr.mv_rf_g[gi].assign((size_t)n_off * hr.R * 2, 0xffffffffu);
+        for (int s = 0; s < n_off; ++s)
+            for (int t = d->dep_off[s]; t < d->dep_off[s + 1]; ++t) {
+                const size_t o = ((size_t)s * hr.R + (t - d->dep_off[s])) * 2;
+                const int slot = d->dep_slots[t];
+                const uint32_t pos = (uint32_t)((slot % K) * G + slot / K);
+                hr.mv_rf_g[gi][o] = (rf[(size_t)slot * 2] & 0xffff0000u) | pos;
+                hr.mv_rf_g[gi][o + 1] = rf[(size_t)slot * 2 + 1];
+            }
+    }
     hr.FD = 2 * D;
     const int Z = N | (N << 16);
     hr.mv_fnb.assign((size_t)n_off * hr.FD, Z);
@@ -2619,10 +2627,13 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
                 UP(hr.slot_rf_g[gi].data(), hr.slot_rf_g[gi].size(), &sg);
                 P.slot_rf_g[gi] = (const uint2 *)sg;
             }
-            const uint32_t *mr2 = nullptr, *fs = nullptr;
-            UP(hr.mv_rf.data(), hr.mv_rf.size(), &mr2);
+            for (int gi = 0; gi < 3; ++gi) {
+                const uint32_t *mg = nullptr;
+                UP(hr.mv_rf_g[gi].data(), hr.mv_rf_g[gi].size(), &mg);
+                P.mv_rf_g[gi] = (const uint2 *)mg;
+            }
+            const uint32_t *fs = nullptr;
             UP(hr.fsite.data(), hr.fsite.size(), &fs);
-            P.mv_rf = (const uint2 *)mr2;
             P.fsite = (const uint4 *)fs;
             UP(hr.mv_fnb.data(), hr.mv_fnb.size(), &P.mv_fnb);
         }

@@ -22,8 +22,11 @@
 //                 continuous-time kernels (blocked order, Philox counter
 //                 (hop, walker, step, tag)), then z and the candidate list of
 //                 the new configuration.
-// The host reads the number of row blocks after each scan (one 4-byte copy)
-// and stops when no walker is active.  Every value a configuration gets is a
+// The host looks at the number of row blocks of the last scan once per 16
+// hop iterations (one 4-byte copy) and stops when no walker is active; a
+// step captured into a graph runs a fixed number of hop iterations instead
+// (gfmc_mlp_set_hop_cap) and counts a step that left a walker unfinished
+// (gfmc_mlp_unfinished).  Every value a configuration gets is a
 // function of that configuration alone (exact z, fixed MFMA and reduction
 // order for every row), so log psi(x') as a move candidate equals log psi(x')
 // as the current configuration later.
@@ -51,7 +54,9 @@ struct MlpWs {
     int32_t *part;       // [cap / 1024 + 1] scan partials
     int32_t *total;      // [1]         row blocks of the iteration
     float *logpsi;       // [cap * blocks per walker * 16]
-    long long *counters; // [2]         row blocks / hop iterations propagated (device sums)
+    long long *counters; // [5]         row blocks / hop iterations propagated (device sums),
+                         //             captured steps left unfinished by the hop cap, the
+                         //             largest hop-iteration count of a step, this step's
 };
 
 // rows of one training evaluation (gfmc_mlp_vmin_grad): the upstream
@@ -72,6 +77,7 @@ struct MlpWork {
     int64_t cap = 0;             // walkers the attached workspace holds (0: none attached)
     int bpw = 0;                 // blocks per walker at most
     int counting = 0;            // the propagation adds to ws.counters
+    int hop_cap = 512;           // hop iterations of a step captured into a graph
     MlpWs ws{};
     int32_t *host_total = nullptr;   // pinned, allocated with the problem
 };
@@ -149,6 +155,7 @@ __global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_start(DevProblem P, Table
     extern __shared__ __align__(16) unsigned char lds[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     signed char *x = (signed char *)lds + (size_t)wave * P.n_pad;
+    if (blockIdx.x == 0 && threadIdx.x == 0) W.counters[4] = 0;   // this step's hop iterations
     for (int64_t w = (int64_t)blockIdx.x * MLP_SW + wave; w < nw;
          w += (int64_t)gridDim.x * MLP_SW) {
         load_spins(P, spins + w * P.words, x, lane);
@@ -221,7 +228,15 @@ __global__ void k_mlp_count(MlpWs W) {
     if (threadIdx.x == 0 && W.total[0] > 0) {
         W.counters[0] += W.total[0];
         W.counters[1] += 1;
+        W.counters[4] += 1;
+        if (W.counters[4] > W.counters[3]) W.counters[3] = W.counters[4];
     }
+}
+
+// after the capped hop iterations of a captured step: rows still to do
+// mean walkers that did not finish their step
+__global__ void k_mlp_capcheck(MlpWs W) {
+    if (threadIdx.x == 0 && W.total[0] > 0) W.counters[2] += 1;
 }
 
 __global__ __launch_bounds__(1024) void k_mlp_scan3(int64_t nw, MlpWs W) {
@@ -500,10 +515,11 @@ void k_mlp_gemm(DevProblem P, const uint32_t *spins, MlpWs W, MlpTws T) {
 __global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_select(
     DevProblem P, Tables<float> tb, int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
     double beta, double alw, uint32_t key0, uint32_t key1, uint32_t step, uint32_t tag,
-    int32_t *evals, MlpWs W) {
+    int32_t *evals, MlpWs W, const uint32_t *step_base) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     WalkerLds<float> L = carve<float>(P, lds + (size_t)wave * P.rbm_wave_bytes);
+    step += step_base ? *step_base : 0u;
     for (int64_t w = (int64_t)blockIdx.x * MLP_SW + wave; w < nw;
          w += (int64_t)gridDim.x * MLP_SW) {
         if (!W.active[w]) continue;
@@ -665,7 +681,7 @@ static size_t mlp_work_layout(const DevProblem &P, int bpw, int64_t nw, size_t *
     const size_t n = (size_t)nw, nb = n * (size_t)bpw;
     const size_t sizes[14] = {n * P.mlp_h1 * 8, n * (size_t)(P.n_off > 0 ? P.n_off : 1) * 4,
                               n * 4, n * 4, n * 8, n * 8, n * 8, n * 4, n * 4, nb * 4,
-                              (n / 1024 + 2) * 4, 16, nb * MLP_RB * 4, 16};
+                              (n / 1024 + 2) * 4, 16, nb * MLP_RB * 4, 40};
     size_t o = 0;
     for (int i = 0; i < 14; ++i) {
         if (off) off[i] = o;
@@ -747,12 +763,26 @@ static int mlp_start(const gfmc_problem *p, int64_t nw, const uint32_t *spins,
 static int mlp_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
                          double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
                          uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st) {
-    if (g_step_base)
-        return fail(GFMC_EUNSUP, "the MLP guide's propagation looks at its row count on the "
-                                 "host every 16 hop iterations and cannot be captured into a "
-                                 "graph");
     int rc = mlp_start(p, nw, spins, log_w, 0, st);
     if (rc) return rc;
+    if (g_step_base) {
+        // captured into a graph: a fixed number of hop iterations (walkers
+        // that are done make them empty launches), then one more scan, whose
+        // row count says whether a walker was left unfinished
+        for (int iter = 0; iter < p->mlp->hop_cap; ++iter) {
+            mlp_scan(p, nw, st);
+            hipLaunchKernelGGL(k_mlp_count, dim3(1), dim3(WAVE), 0, st, p->mlp->ws);
+            launch_mlp_gemm<false>(p, spins, st);
+            hipLaunchKernelGGL(k_mlp_select, dim3(mlp_walker_grid(p, nw)),
+                               dim3(MLP_SW * WAVE), mlp_walker_lds(p), st, p->dp, p->t32, nw,
+                               spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
+                               p->mlp->ws, g_step_base);
+        }
+        mlp_scan(p, nw, st);
+        hipLaunchKernelGGL(k_mlp_capcheck, dim3(1), dim3(WAVE), 0, st, p->mlp->ws);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     // hop iterations of every active walker (scan -> GEMM -> select), the
     // GEMM and the select reading the row count on the device; after every
     // MLP_CHECK_EVERY iterations the host looks at the count of the last
@@ -763,7 +793,7 @@ static int mlp_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, dou
         launch_mlp_gemm<false>(p, spins, st);
         hipLaunchKernelGGL(k_mlp_select, dim3(mlp_walker_grid(p, nw)), dim3(MLP_SW * WAVE),
                            mlp_walker_lds(p), st, p->dp, p->t32, nw, spins, log_w, e_loc, beta,
-                           alw, k0, k1, step, tag, evals, p->mlp->ws);
+                           alw, k0, k1, step, tag, evals, p->mlp->ws, nullptr);
         HIP_TRY(hipGetLastError());
         if ((iter + 1) % MLP_CHECK_EVERY == 0) {
             int total = 0;
@@ -809,6 +839,29 @@ extern "C" int gfmc_mlp_counters(const gfmc_problem *p, int64_t *out) {
     return 0;
 }
 
+extern "C" int gfmc_mlp_set_hop_cap(gfmc_problem *p, int32_t hop_cap) {
+    if (!p || p->dp.guide != GFMC_GUIDE_MLP || !p->mlp)
+        return fail(GFMC_EINVAL, "gfmc_mlp_set_hop_cap: the problem's guide is not an MLP");
+    if (hop_cap < 1) return fail(GFMC_EINVAL, "gfmc_mlp_set_hop_cap: hop_cap must be >= 1");
+    p->mlp->hop_cap = hop_cap;
+    return 0;
+}
+
+extern "C" int gfmc_mlp_unfinished(const gfmc_problem *p, int64_t *out) {
+    if (!p || !out) return fail(GFMC_EINVAL, "gfmc_mlp_unfinished: NULL argument");
+    if (p->dp.guide != GFMC_GUIDE_MLP || !p->mlp)
+        return fail(GFMC_EINVAL, "gfmc_mlp_unfinished: the problem's guide is not an MLP");
+    if (p->mlp->cap == 0) {
+        out[0] = out[1] = 0;
+        return 0;
+    }
+    long long c[2] = {0, 0};
+    HIP_TRY(hipMemcpy(c, p->mlp->ws.counters + 2, sizeof c, hipMemcpyDeviceToHost));
+    out[0] = c[0];
+    out[1] = c[1];
+    return 0;
+}
+
 extern "C" int64_t gfmc_mlp_work_bytes(const gfmc_problem *p, int64_t n_walkers) {
     if (!p || p->dp.guide != GFMC_GUIDE_MLP || !p->mlp || n_walkers < 1) return -1;
     if (n_walkers * p->mlp->bpw > 0x7fffffffLL / MLP_RB) return -1;
@@ -846,7 +899,7 @@ extern "C" int gfmc_mlp_set_work(gfmc_problem *p, void *work, int64_t bytes, int
     W.total = (int32_t *)(b + off[11]);
     W.logpsi = (float *)(b + off[12]);
     W.counters = (long long *)(b + off[13]);
-    HIP_TRY(hipMemsetAsync(W.counters, 0, 16, (hipStream_t)stream));
+    HIP_TRY(hipMemsetAsync(W.counters, 0, 40, (hipStream_t)stream));
     p->mlp->cap = n_walkers;
     return 0;
 }

@@ -486,9 +486,13 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
     dev = chain.spins.device
     series = torch.empty((n_steps, 2), dtype=torch.float64, device=dev)
     m_array = [q // 2 for q in p_sorted]
-    # the MLP guide's propagation looks at its row count on the host every
-    # 16 hop iterations (a data-dependent loop length), so it is not captured
-    graphable = p.guide_kind != _lib.GUIDE_MLP
+    # a captured MLP-guided step runs a fixed number of hop iterations: twice
+    # the most any eager step needed, plus 16; the count of steps it left
+    # unfinished is checked after every chunk
+    graphable = True
+    unfinished0, max_hops = p.mlp_step_stats()
+    if p.guide_kind == _lib.GUIDE_MLP and max_hops > 0:
+        p.set_mlp_hop_cap(2 * max_hops + 16)
     diag = None
     dseries = None
     if diag_ops:
@@ -550,6 +554,11 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         if pending is not None:
             s0, n, ev = pending
             chunk = series[s0:s0 + n].cpu().numpy()
+            if p.guide_kind == _lib.GUIDE_MLP and p.mlp_step_stats()[0] > unfinished0:
+                raise RuntimeError(
+                    f"MLP guide: a branching step captured into a graph needed more hop "
+                    f"iterations than its cap (steps {s0}..{s0 + n - 1}); raise it with "
+                    f"Problem.set_mlp_hop_cap or run with use_graph=False")
             check_population(chunk[:, 0], s0, chunk[:, 1])
             host_chunks.append(chunk)
             if diag is not None:

@@ -214,6 +214,22 @@ class Problem:
         self._mlp_buf = buf
         self._mlp_cap = n
 
+    def mlp_step_stats(self):
+        """(steps of the MLP guide captured into a graph that ended with a
+        walker unfinished, the largest hop-iteration count of one step)
+        since the workspace was attached (gfmc_mlp_unfinished; zeros for
+        the other guides)."""
+        if self.guide_kind != _lib.GUIDE_MLP:
+            return 0, 0
+        out = (ctypes.c_int64 * 2)()
+        _lib.check(_lib.load().gfmc_mlp_unfinished(self.handle, out), "gfmc_mlp_unfinished")
+        return int(out[0]), int(out[1])
+
+    def set_mlp_hop_cap(self, hop_cap):
+        """Hop iterations of an MLP-guided step captured into a graph."""
+        _lib.check(_lib.load().gfmc_mlp_set_hop_cap(self.handle, int(hop_cap)),
+                   "gfmc_mlp_set_hop_cap")
+
     def __del__(self):
         try:
             if getattr(self, "handle", None):

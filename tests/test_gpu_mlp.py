@@ -155,7 +155,51 @@ def test_gfmc_energy_with_mlp_guide_matches_ed(case):
     _, _, err, _ = evaluate_observables(raw, n_blocks=20)
     say("  E", E[32], "+-", err[32], "ED", E0, "hip_graph", st["hip_graph"])
     assert abs(E[32] - E0) < 4 * err[32] + 2e-3
-    assert not st["hip_graph"]
+    assert st["hip_graph"]
+
+
+def test_captured_mlp_steps_equal_eager_steps():
+    """The MLP-guided step loop captured into HIP graphs (a fixed number
+    of hop iterations per step) gives bitwise the run of eager steps."""
+    say("mlp graph vs eager")
+    H, G = SYSTEMS[0][1], SYSTEMS[0][2]
+    out = []
+    for use_graph in (True, False):
+        w = prepare_walkers_sz(512, 12, 3)
+        wf, E, _, st, raw = run(w, 120, H, 0.2, G, [8], rng_key=11, warmup_steps=40,
+                                verbosity=0, use_graph=use_graph, number_of_outputs=6)
+        assert st["hip_graph"] == use_graph
+        out.append((np.asarray(raw["log_weights"]), np.asarray(raw["energy"]),
+                    wf.spins.cpu().numpy()))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_captured_mlp_step_over_its_hop_cap_is_an_error():
+    """A captured step that needs more hop iterations than the cap raises
+    after the replay (a graph captured with the cap set to 1 from the host;
+    every kernel still ends normally)."""
+    H, G = SYSTEMS[0][1], SYSTEMS[0][2]
+    from pygfmc_amd.engine import Chain, StepGraph
+    from pygfmc_amd.problem import get_problem
+    p = get_problem(H, G)
+    ch = Chain(p, prepare_walkers_sz(512, 12, 3))
+    ch.steps(5, 0.2, 0.0, 11, 0)
+    u0, mx = p.mlp_step_stats()
+    assert mx >= 2
+    p.set_mlp_hop_cap(1)
+    try:
+        g = StepGraph(ch, 3, 0.2, 0.0, 11)
+        g.launch(5)
+        torch.cuda.synchronize()
+        assert p.mlp_step_stats()[0] == u0 + 3
+        p.set_mlp_hop_cap(2 * mx + 16)
+        g2 = StepGraph(ch, 3, 0.2, 0.0, 11)
+        g2.launch(8)
+        torch.cuda.synchronize()
+        assert p.mlp_step_stats()[0] == u0 + 3
+    finally:
+        p.set_mlp_hop_cap(512)
 
 
 def test_mlp_problem_rejects_what_it_does_not_run():
