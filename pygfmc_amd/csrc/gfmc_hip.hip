@@ -131,9 +131,10 @@ struct DevProblem {
     // size, and per move and group size {LDS position | i << 16, ...}; the
     // field updates of each move, mv_fnb[s * FD + t] =
     // partner | (flipped site << 16), t < D for site i and D <= t < 2D for
-    // site j; every site's partners, 8 int16 per site, site N (a zero spin)
-    // as padding; f_pad bytes of fields per walker
-    int fld, FD, f_pad;
+    // site j; every site's partners, 8 int16 per site; site N, a zero spin,
+    // as padding where an entry pads (zsite: irregular partner counts, or
+    // site flips with pair moves); f_pad bytes of fields per walker
+    int fld, FD, f_pad, zsite;
     const uint2 *slot_rf_g[3];
     const uint2 *mv_rf_g[3]; // [n_off * R] per group layout, first field the LDS position
     const int *mv_fnb;       // [n_off * FD]
@@ -448,17 +449,22 @@ __device__ __forceinline__ float eval_recf(const Tables<float> &tb, uint2 r,
 }
 
 // the local fields of the walker's spins (spins already in LDS); site N is
-// the zero spin the padding entries of the partner tables name
+// the zero spin that padding entries of the partner tables name (present
+// when zsite: irregular partner counts, or site flips with pair moves)
 template <int G>
 __device__ __forceinline__ void fld_init(const DevProblem &P, signed char *x, signed char *F,
                                          int gl) {
-    if (gl == 0) x[P.N] = 0;
+    if (P.zsite && gl == 0) x[P.N] = 0;
     wave_sync();
-    const bool wide = P.max_deg > 4;
+    const int D = P.max_deg;
     for (int k = gl; k < P.N; k += G) {
         const uint4 q = P.fsite[k];
-        int s = x[q.x & 0xffffu] + x[q.x >> 16] + x[q.y & 0xffffu] + x[q.y >> 16];
-        if (wide) s += x[q.z & 0xffffu] + x[q.z >> 16] + x[q.w & 0xffffu] + x[q.w >> 16];
+        const uint32_t f[4] = {q.x, q.y, q.z, q.w};
+        int s = 0;
+        for (int t = 0; t < D; ++t) {
+            const uint32_t w = f[t >> 1];
+            s += x[(t & 1) ? (w >> 16) : (w & 0xffffu)];
+        }
         F[k] = (signed char)s;
     }
     wave_sync();
@@ -919,7 +925,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
             // the zero spin
             const uint4 *fs = (const uint4 *)((const T *)cy.w[cur] + par * cy.row + cy.f_off);
             for (int q = gl; q < P.f_pad / 16; q += G) ((uint4 *)L.f)[q] = fs[q];
-            if (gl == 0) L.x[P.N] = 0;
+            if (P.zsite && gl == 0) L.x[P.N] = 0;
         }
         n_anti = cy.anti[cur][par];
         wave_sync();
@@ -1984,7 +1990,7 @@ struct HostRecords {
     int R = 0, RN = 0, RE = 0;
     double J0 = 0.0, h0 = 0.0;
     // local-field form (DevProblem::fld)
-    int fld = 0, FD = 0;
+    int fld = 0, FD = 0, zsite = 1;
     std::vector<uint32_t> slot_rf_g[3], mv_rf_g[3], fsite;   // 2 words per record, 4 per site
     std::vector<int> mv_fnb;
 };
@@ -2061,7 +2067,12 @@ static void build_field_records(const gfmc_problem_desc *d, HostRecords &hr) {
                 hr.mv_rf_g[gi][o + 1] = rf[(size_t)slot * 2 + 1];
             }
     }
-    hr.FD = 2 * D;
+    // a zero spin (site N) is needed only where a table entry pads: sites
+    // with fewer than D partners, or site flips and pair moves in one problem
+    bool regular = true;
+    for (int k = 0; k < N; ++k) regular = regular && (int)nj[k].size() == D;
+    hr.zsite = (!regular || (d->n_hx > 0 && d->n_jpm > 0)) ? 1 : 0;
+    hr.FD = d->n_jpm > 0 ? 2 * D : D;
     const int Z = N | (N << 16);
     hr.mv_fnb.assign((size_t)n_off * hr.FD, Z);
     for (int s = 0; s < n_off; ++s) {
@@ -2561,7 +2572,8 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     P.fld = P.fast && hr.fld && precision == GFMC_FP32 && d->guide == GFMC_GUIDE_JASTROW &&
             !d->has_hz && (!d->has_jz || d->uniform_jz);
     P.FD = P.fld ? hr.FD : 0;
-    P.n_pad = round_up(N + (P.fld ? 1 : 0), 16);
+    P.zsite = P.fld ? hr.zsite : 0;
+    P.n_pad = round_up(N + P.zsite, 16);
     P.e_pad = P.flags ? round_up(E > 0 ? E : 1, 16) : 0;
     P.f_pad = P.fld ? P.n_pad : 0;
     {
