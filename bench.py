@@ -55,6 +55,9 @@ def parse():
                          "262 144 walkers)")
     ap.add_argument("--only-mlp", action="store_true",
                     help="run the kagome MLP section alone (for profiling its kernels)")
+    ap.add_argument("--only-race", action="store_true",
+                    help="run the learned-guide section alone (10x10 Heisenberg, RBM vs "
+                         "Jastrow; for profiling its kernels)")
     ap.add_argument("--mlp-walkers", type=int, default=262144)
     ap.add_argument("--mlp-steps", type=int, default=3)
     ap.add_argument("--race-train-iters", type=int, default=1500)
@@ -184,6 +187,9 @@ def main():
         if args.calib_copy_mib > 0:
             calib_copy(args.calib_copy_mib)
         print(json.dumps({"kagome12_mlp": kagome_mlp_section(args)}), flush=True)
+        return
+    if args.only_race:
+        print(json.dumps({"sigma_race": sigma_race(args)}), flush=True)
         return
     from pygfmc_amd import JastrowSpinHalf, SpinHalf, prepare_walkers_sz, square
     from pygfmc_amd import _lib
@@ -718,7 +724,9 @@ def sigma_race(args):
 
 
 def rbm_kernel_measure(H, G, cpu_seconds):
-    """k_propagate_rbh on 16 384 walkers of the trained guide, equilibrated:
+    """The RBM propagation kernel (k_propagate_rbt, table factors, for the
+    translation-tied guide; k_propagate_rbh otherwise) on 16 384 walkers of
+    the trained guide, equilibrated:
     average launch time by HIP events on the launch stream against the
     algorithmic FP32 work of the reference algorithm (every slot's log-ratio
     over all hidden units: n_edges H 8 flop per rate evaluation), and the FP64
@@ -745,7 +753,8 @@ def rbm_kernel_measure(H, G, cpu_seconds):
     t = float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
     evals = float(ch.evals.to(torch.float64).sum().item()) / n_k
     flop = evals * len(H.edge_i) * G.hidden * 8.0
-    prof = profiled_traffic("k_propagate_rbh_heisenberg_square10x10_rbm400_w16384")
+    kname = "k_propagate_rbt" if p.tied is not None else "k_propagate_rbh"
+    prof = profiled_traffic(f"{kname}_heisenberg_square10x10_rbm400_w16384")
     roof = {"bound": "valu", "compute_unit": "FP32 vector ALU (no GEMM in the hop; the MI355X "
                                               "FP32 vector and matrix peaks are both 157.3 "
                                               "TFLOP/s)",
@@ -757,7 +766,7 @@ def rbm_kernel_measure(H, G, cpu_seconds):
             "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, "
                             "copy-calibrated)",
             "traffic_source": prof["source"] if prof else None,
-            "kernel": "k_propagate_rbh<7>", "kernel_avg_ms": t * 1e3,
+            "kernel": f"{kname}<7>", "kernel_avg_ms": t * 1e3,
             "evals_per_walker_step": evals / nw, "flop_per_eval": len(H.edge_i) * G.hidden * 8.0,
             "transcendentals_per_eval": len(H.edge_i) * G.hidden * 2.0}
     cpu = cpu_baseline(H, G, ch, H.N, 0.2, alw, 9, cpu_seconds)

@@ -389,6 +389,29 @@ int64_t gfmc_rbm_n_params(const gfmc_problem *p);
  * (nearest even), a and b copied, W's column sums recomputed. */
 int gfmc_rbm_set_params(gfmc_problem *p, const float *flat, void *stream);
 
+/* Translation-tied RBM on a torus (no counterpart in the reference; the
+ * propagation of gfmc_propagate for an RBM guide whose hidden unit
+ * h = f n_T + r sees W[h][k] = w_f(k - r)).  The factor of unit h in a pair
+ * slot's ratio then comes from a table of (slot class, x_i, f, site i - r)
+ * instead of an exponential per factor; moves, weights and energies are
+ * those of the untied kernels up to the order of one FP32 sum over lanes.
+ * Tables (int32, built by pygfmc_amd.torus.tied_tables): unit_off[Hp]
+ * (Hp = hidden rounded up to 64) the offset f Q + O - code(r) of every
+ * hidden unit (padded units F Q + O), slot_code[n_off] code(i) | class <<
+ * 20, site_code[n_sites], q_site[Q] the site of entry q, q_pair[n_cls Q]
+ * the partner site of entry q in class c (-1: none).  Every table read the
+ * kernel can make is checked against the table's bounds here.  The problem
+ * must have pair moves only with J_pmpm = 0 and at most 512 hidden units;
+ * otherwise GFMC_EUNSUP and nothing changes.  gfmc_rbm_set_params rebuilds
+ * the tables from the rows r = 0: parameters set later must stay tied (the
+ * orbit-tied trainers keep them so), or call gfmc_rbm_untie first, which
+ * returns the problem to the untied kernels. */
+int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q,
+                      int32_t n_cls, const int32_t *unit_off, const int32_t *slot_code,
+                      const int32_t *site_code, const int32_t *q_site, const int32_t *q_pair,
+                      void *stream);
+int gfmc_rbm_untie(gfmc_problem *p);
+
 /* theta[i][h] = b_h + sum_k W_hk x_ik for n packed configurations (BF16
  * MFMA, FP32 accumulation); theta: [n][hidden] FP32 device buffer. */
 int gfmc_rbm_theta(const gfmc_problem *p, int64_t n, const uint32_t *spins,

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libgfmc_hip.so")
 
 GFMC_FP64 = 0
 GFMC_FP32 = 1
+GFMC_EUNSUP = -4
 GUIDE_NONE = 0
 GUIDE_JASTROW = 1
 GUIDE_RBM = 2
@@ -153,6 +154,9 @@ SIGNATURES = [
                                       _vp, _vp]),
     ("gfmc_rbm_n_params", ctypes.c_int64, [_vp]),
     ("gfmc_rbm_set_params", ctypes.c_int, [_vp, _vp, _vp]),
+    ("gfmc_rbm_set_tied", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("gfmc_rbm_untie", ctypes.c_int, [_vp]),
     ("gfmc_rbm_theta", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
     ("gfmc_rbm_vmin_work_bytes", ctypes.c_int64, [_vp, ctypes.c_int64]),
     ("gfmc_rbm_vmin_grad", ctypes.c_int, [_vp, ctypes.c_int64, _vp, ctypes.c_float, _vp, _vp,

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -149,6 +150,16 @@ struct DevProblem {
     int rbh, Hp, wt_bytes, rbh_wave_bytes;
     const uint16_t *rbm_WT;
     int *rbm_queue;
+    // translation-tied RBM on a torus (gfmc_rbm_set_tied; gfmc_rbm.inl): rbt =
+    // M when on; factor tables of rbt_words floats ((2 rbt_ncls + 1) rbt_F
+    // blocks of rbt_Q, then rbt_Q zeros, rounded up to a multiple of 4)
+    int rbt, rbt_F, rbt_nT, rbt_Q, rbt_ncls, rbt_words;
+    const float *rbt_tab;
+    const int *rbt_unit;    // [Hp]     f Q + O - code(r), padded units F Q + O
+    const int *rbt_scode;   // [n_off]  code(i) | class << 20
+    const int *rbt_kcode;   // [N]      code(k)
+    const int *rbt_qsite;   // [Q]      the site of entry q
+    const int *rbt_qpair;   // [ncls Q] the partner site of entry q in class c, -1: none
     // MLP guide (layer-1 sums exact in FP64, layer 2 on MFMA); rbm_wave_bytes
     // is also the per-wavefront LDS of its walker kernels
     int mlp_h1, mlp_h2;
@@ -176,6 +187,9 @@ struct gfmc_problem {
     std::vector<void *> allocs;
     int64_t lds_bytes;
     int64_t rbh_lds_bytes = 0;   // LDS per workgroup of the gfmc_rbm.inl kernels
+    int64_t rbt_lds_bytes = 0;   // LDS per workgroup of the tied-RBM kernel
+    int rbt_waves = 0;           // its wavefronts per workgroup
+    int jpmpm_zero = 0;          // every edge's J_pmpm is zero
     int n_cu;
     int lanes = WAVE;   // lanes per walker of the propagation kernel (64, 32, 16)
 };
@@ -2367,6 +2381,53 @@ static void rbh_rates_m(const gfmc_problem *p, int64_t nw, const uint32_t *spins
                        log_diff, log_psi);
 }
 
+// the tied-RBM kernel (gfmc_rbm_set_tied) for M hidden-unit chunks per lane
+static void rbt_build_tables(const gfmc_problem *p, hipStream_t st) {
+    const int64_t total = (int64_t)(2 * p->dp.rbt_ncls + 1) * p->dp.rbt_F * p->dp.rbt_Q +
+                          p->dp.rbt_Q;
+    hipLaunchKernelGGL(k_rbt_tables, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       p->dp, (float *)p->dp.rbt_tab);
+}
+
+template <int M>
+static int rbt_set_lds_m(int bytes) {
+    HIP_TRY(hipFuncSetAttribute((const void *)k_propagate_rbt<M>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    return 0;
+}
+
+static int rbt_set_lds(int M, int bytes) {
+    switch (M) {
+    case 1: return rbt_set_lds_m<1>(bytes);
+    case 2: return rbt_set_lds_m<2>(bytes);
+    case 3: return rbt_set_lds_m<3>(bytes);
+    case 4: return rbt_set_lds_m<4>(bytes);
+    case 5: return rbt_set_lds_m<5>(bytes);
+    case 6: return rbt_set_lds_m<6>(bytes);
+    case 7: return rbt_set_lds_m<7>(bytes);
+    case 8: return rbt_set_lds_m<8>(bytes);
+    }
+    return fail(GFMC_EUNSUP, "RBM: hidden layer too wide for the tied kernel");
+}
+
+// persistent grid: as many workgroups of rbt_waves walkers as fit
+static unsigned rbt_grid(const gfmc_problem *p, int64_t nw) {
+    const int64_t need = (nw + p->rbt_waves - 1) / p->rbt_waves;
+    const int64_t per_cu = (160 * 1024) / (p->rbt_lds_bytes > 0 ? p->rbt_lds_bytes : 1);
+    const int64_t cap = (int64_t)p->n_cu * (per_cu > 0 ? per_cu : 1);
+    return (unsigned)(need < cap ? need : cap);
+}
+
+template <int M>
+static void rbt_launch_m(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                         double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
+                         uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st) {
+    hipLaunchKernelGGL(k_rbh_queue_reset, dim3(1), dim3(WAVE), 0, st, p->dp.rbm_queue);
+    hipLaunchKernelGGL(k_propagate_rbt<M>, dim3(rbt_grid(p, nw)), dim3(p->rbt_waves * WAVE),
+                       (size_t)p->rbt_lds_bytes, st, p->dp, p->t32, nw, spins, log_w, e_loc,
+                       beta, alw, k0, k1, step, tag, evals, g_step_base);
+}
+
 #define RBH_SWITCH(M, FN, ...)                      \
     do {                                            \
         switch (M) {                                \
@@ -2669,6 +2730,9 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     UP(d->jpmpm, (size_t)E, &p->t64.jpmpm);
     UP32(d->jpm, (size_t)E, &p->t32.jpm);
     UP32(d->jpmpm, (size_t)E, &p->t32.jpmpm);
+    p->jpmpm_zero = 1;
+    for (int e = 0; e < E; ++e)
+        if ((float)d->jpmpm[e] != 0.0f) p->jpmpm_zero = 0;
     if (d->n_hx) {
         UP(d->hx_half, (size_t)N, &p->t64.hx_half);
         UP32(d->hx_half, (size_t)N, &p->t32.hx_half);
@@ -2712,7 +2776,7 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         // the lanes-over-hidden-units kernels: W transposed, [N][Hp] BF16
         P.Hp = round_up(H, WAVE);
         P.wt_bytes = round_up(N * P.Hp * 2, 16);
-        P.rbh_wave_bytes = round_up(P.K * WAVE * 4 + (n_off + 2) * 12 + P.n_pad + P.e_pad, 16);
+        P.rbh_wave_bytes = round_up(P.K * WAVE * 4 + (n_off + 4) * 12 + P.n_pad + P.e_pad, 16);
         const int64_t rbh_lds = (int64_t)P.wt_bytes + (int64_t)RBH_WAVES * P.rbh_wave_bytes;
         P.rbh = (P.Hp / WAVE <= RBH_MAX_M && rbh_lds <= 160 * 1024) ? P.Hp / WAVE : 0;
         if (P.rbh) {
@@ -2804,6 +2868,103 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
 
 int gfmc_problem_destroy(gfmc_problem *p) {
     if (p) free_problem(p);
+    return 0;
+}
+
+int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q,
+                      int32_t n_cls, const int32_t *unit_off, const int32_t *slot_code,
+                      const int32_t *site_code, const int32_t *q_site, const int32_t *q_pair,
+                      void *stream) {
+    const char *fn = "gfmc_rbm_set_tied";
+    if (!p) return fail(GFMC_EINVAL, std::string(fn) + ": NULL problem");
+    DevProblem &P = p->dp;
+    if (P.guide != GFMC_GUIDE_RBM)
+        return fail(GFMC_EINVAL, std::string(fn) + ": the problem's guide is not an RBM");
+    if (P.rbt) return fail(GFMC_EINVAL, std::string(fn) + ": the tables are already set");
+    if (!P.rbh)
+        return fail(GFMC_EUNSUP, std::string(fn) + ": hidden layer wider than the "
+                                                   "lanes-over-units kernels take");
+    if (P.n_hx != 0 || !p->jpmpm_zero || P.n_off != P.n_edges)
+        return fail(GFMC_EUNSUP, std::string(fn) + ": the tied kernel takes pair moves of "
+                                                   "anti-parallel spins only (no h_x, J_pmpm = 0)");
+    if (n_filters <= 0 || n_T <= 0 || (int64_t)n_filters * n_T != P.H || Q <= 0 ||
+        Q >= (1 << 20) || n_cls <= 0 || n_cls > 2047)
+        return fail(GFMC_EINVAL, std::string(fn) + ": bad sizes");
+    if (!unit_off || !slot_code || !site_code || !q_site || !q_pair)
+        return fail(GFMC_EINVAL, std::string(fn) + ": NULL table");
+    const int N = P.N, Hp = P.Hp, n_off = P.n_off;
+    const int64_t FQ = (int64_t)n_filters * Q;
+    const int64_t raw = (2 * (int64_t)n_cls + 1) * FQ + Q;
+    const int64_t words = (raw + 3) / 4 * 4;
+    // every table read of the kernels stays inside the table: theta reads
+    // 2 n_cls FQ + code(k) + off(h), factor reads (2 cls + sg) FQ + code(i) + off(h)
+    int64_t kmin = INT64_MAX, kmax = INT64_MIN, smin = INT64_MAX, smax = INT64_MIN;
+    for (int k = 0; k < N; ++k) {
+        if (site_code[k] < 0 || site_code[k] >= Q)
+            return fail(GFMC_EINVAL, std::string(fn) + ": site code out of range");
+        kmin = std::min<int64_t>(kmin, site_code[k]);
+        kmax = std::max<int64_t>(kmax, site_code[k]);
+    }
+    for (int s = 0; s < n_off; ++s) {
+        const int code = slot_code[s] & 0xfffff, cls = slot_code[s] >> 20;
+        if (cls < 0 || cls >= n_cls)
+            return fail(GFMC_EINVAL, std::string(fn) + ": slot class out of range");
+        smin = std::min<int64_t>(smin, 2 * (int64_t)cls * FQ + code);
+        smax = std::max<int64_t>(smax, (2 * (int64_t)cls + 1) * FQ + code);
+    }
+    for (int h = 0; h < Hp; ++h) {
+        const int64_t v = unit_off[h];
+        if (2 * n_cls * FQ + kmin + v < 0 || 2 * n_cls * FQ + kmax + v >= raw || smin + v < 0 ||
+            smax + v >= raw)
+            return fail(GFMC_EINVAL, std::string(fn) + ": unit offset reads outside the tables");
+    }
+    for (int q = 0; q < Q; ++q)
+        if (q_site[q] < 0 || q_site[q] >= N)
+            return fail(GFMC_EINVAL, std::string(fn) + ": entry site out of range");
+    for (int64_t t = 0; t < (int64_t)n_cls * Q; ++t)
+        if (q_pair[t] < -1 || q_pair[t] >= N)
+            return fail(GFMC_EINVAL, std::string(fn) + ": entry partner out of range");
+    int waves = 16;
+    while (waves >= 4 && words * 4 + (int64_t)waves * P.rbh_wave_bytes > 160 * 1024) waves /= 2;
+    if (waves < 4)
+        return fail(GFMC_EUNSUP, std::string(fn) + ": tables plus walker state exceed the LDS");
+    const int64_t lds = words * 4 + (int64_t)waves * P.rbh_wave_bytes;
+    int rc = 0;
+    const int *d_unit = nullptr, *d_scode = nullptr, *d_kcode = nullptr, *d_qsite = nullptr,
+              *d_qpair = nullptr;
+    if ((rc = upload(p, unit_off, (size_t)Hp, &d_unit)) ||
+        (rc = upload(p, slot_code, (size_t)n_off, &d_scode)) ||
+        (rc = upload(p, site_code, (size_t)N, &d_kcode)) ||
+        (rc = upload(p, q_site, (size_t)Q, &d_qsite)) ||
+        (rc = upload(p, q_pair, (size_t)n_cls * Q, &d_qpair)))
+        return rc;
+    void *tab = nullptr;
+    HIP_TRY(hipMalloc(&tab, (size_t)words * 4));
+    p->allocs.push_back(tab);
+    HIP_TRY(hipMemset(tab, 0, (size_t)words * 4));
+    if ((rc = rbt_set_lds(P.rbh, (int)lds))) return rc;
+    P.rbt_F = n_filters;
+    P.rbt_nT = n_T;
+    P.rbt_Q = Q;
+    P.rbt_ncls = n_cls;
+    P.rbt_words = (int)words;
+    P.rbt_tab = (const float *)tab;
+    P.rbt_unit = d_unit;
+    P.rbt_scode = d_scode;
+    P.rbt_kcode = d_kcode;
+    P.rbt_qsite = d_qsite;
+    P.rbt_qpair = d_qpair;
+    p->rbt_lds_bytes = lds;
+    p->rbt_waves = waves;
+    rbt_build_tables(p, (hipStream_t)stream);
+    HIP_TRY(hipGetLastError());
+    P.rbt = P.rbh;
+    return 0;
+}
+
+int gfmc_rbm_untie(gfmc_problem *p) {
+    if (!p) return fail(GFMC_EINVAL, "gfmc_rbm_untie: NULL problem");
+    p->dp.rbt = 0;
     return 0;
 }
 
@@ -2978,7 +3139,10 @@ int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
     if (p->dp.guide == GFMC_GUIDE_MLP)
         return mlp_propagate(p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
                              (hipStream_t)stream);
-    if (p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbh)
+    if (p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbt)
+        RBH_SWITCH(p->dp.rbt, rbt_launch_m, p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
+                   tag, evals, (hipStream_t)stream);
+    else if (p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbh)
         RBH_SWITCH(p->dp.rbh, rbh_launch_m, p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
                    tag, evals, (hipStream_t)stream);
     else if (p->dp.guide == GFMC_GUIDE_RBM)

@@ -31,15 +31,23 @@ struct RbhLds {
     unsigned char *anti; // [e_pad]
 };
 
-__device__ __forceinline__ RbhLds rbh_carve(const DevProblem &P, unsigned char *lds, int wave) {
-    unsigned char *base = lds + (size_t)P.wt_bytes + (size_t)wave * P.rbh_wave_bytes;
+// the walker state of wavefront `wave`; `skip` bytes of shared tables come
+// first (the BF16 weights of the rbh kernels, the factor tables of the tied
+// kernel).  The active-slot list has room for four padding entries.
+__device__ __forceinline__ RbhLds rbh_carve_at(const DevProblem &P, unsigned char *lds, int wave,
+                                               size_t skip) {
+    unsigned char *base = lds + skip + (size_t)wave * P.rbh_wave_bytes;
     RbhLds L;
     L.w = (float *)base;
     L.ent = (int2 *)(L.w + P.K * WAVE);
-    L.g = (float *)(L.ent + (P.n_off + 2));
-    L.x = (signed char *)(L.g + (P.n_off + 2));
+    L.g = (float *)(L.ent + (P.n_off + 4));
+    L.x = (signed char *)(L.g + (P.n_off + 4));
     L.anti = (unsigned char *)(L.x + P.n_pad);
     return L;
+}
+
+__device__ __forceinline__ RbhLds rbh_carve(const DevProblem &P, unsigned char *lds, int wave) {
+    return rbh_carve_at(P, lds, wave, (size_t)P.wt_bytes);
 }
 
 // flags of an active-slot entry (bits above the 24-bit LDS position)
@@ -424,4 +432,292 @@ __global__ __launch_bounds__(256) void k_rbm_pack_t(int N, int H, int Hp, const 
     }
     const uint32_t b = __float_as_uint(flat[(int64_t)h * N + k]);
     WT[q] = (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);   // nearest even
+}
+
+// ---------------------------------------------------------------------------
+// translation-tied RBM on a torus (gfmc_rbm_set_tied)
+// ---------------------------------------------------------------------------
+// Hidden unit h = f n_T + r sees W[h][k] = w_f(k - r).  The factor of unit h
+// in the ratio of pair slot (i, j) with x_j = -x_i,
+//   p_h 2^{c_i W_hi + c_j W_hj} + q_h,  c_i = -4 log2(e) x_i,  c_j = -c_i,
+// depends on the walker only through p_h, q_h: the power is a function of
+// (slot class, x_i, f, the site i - r) and sits in a table,
+//   tab[((cls 2 + (x_i < 0)) F + f) Q + q],  q = code(i) + off(h) - f Q,
+// computed with the expression the rbh kernels evaluate per factor (the
+// same bits).  Block 2 n_cls holds w_f itself (theta updates), then Q
+// zeros that the padded hidden units read.  Per lane and chunk m the kernel
+// keeps off(h) in a register; a slot adds its block base and code(i) (one
+// integer add, one LDS read, one fma and one multiply per factor).  Slots are
+// evaluated four at a time; the four lane sums of log2 of the products are
+// reduced together: v_permlane32_swap pairs lanes l and l + 32 of two slots
+// at once, v_permlane16_swap rows 0 / 1 and 2 / 3 of the two pairs, then a
+// row scan (row_shr 1, 2, 4, 8) leaves slot t + {0, 2, 1, 3} in lane 15 of
+// rows 0..3.  Each slot's sum is (x_l + x_{l+32}) + (x_{l+16} + x_{l+48})
+// over l < 16, then the row scan: the same tree for every slot, so a slot's
+// weight does not depend on its place in the list.
+
+__global__ __launch_bounds__(256) void k_rbt_tables(DevProblem P, float *tab) {
+    const int F = P.rbt_F, Q = P.rbt_Q, nc = P.rbt_ncls;
+    const int64_t fq = (int64_t)F * Q;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t total = (int64_t)(2 * nc + 1) * fq + Q;
+    if (t >= total) return;
+    const int64_t blk = t / fq;
+    if (blk > 2 * nc) {   // the zero block
+        tab[t] = 0.0f;
+        return;
+    }
+    const int q = (int)(t % Q), f = (int)((t / Q) % F);
+    const int hrow = f * P.rbt_nT;   // r = 0: the identity translation
+    const float wi = bf16_u16(P.rbm_WT[(size_t)P.rbt_qsite[q] * P.Hp + hrow]);
+    if (blk == 2 * nc) {
+        tab[t] = wi;
+        return;
+    }
+    const int c = (int)(blk >> 1), sg = (int)(blk & 1);
+    const int d2 = P.rbt_qpair[(size_t)c * Q + q];
+    if (d2 < 0) {   // an entry of another basis site: never read
+        tab[t] = 1.0f;
+        return;
+    }
+    const float wj = bf16_u16(P.rbm_WT[(size_t)d2 * P.Hp + hrow]);
+    const float C = 4.0f * 1.4426950408889634f;
+    const float ci = sg ? C : -C, cj = -ci;
+    tab[t] = __builtin_amdgcn_exp2f(fmaf(cj, wj, ci * wi));
+}
+
+// p, q of the lane's units; padded units (h >= H, last chunk only) get p = 0,
+// q = 1: their factor is exactly 1
+template <int M>
+__device__ __forceinline__ void rbt_pq(RbhState<M> &S, bool pad_last) {
+    rbh_pq(S);
+    if (pad_last) {
+        S.p[M - 1] = 0.0f;
+        S.q[M - 1] = 1.0f;
+    }
+}
+
+template <int M>
+__device__ void rbt_start(const DevProblem &P, const float *Wt, RbhLds &L, RbhState<M> &S,
+                          const int *vb, bool pad_last, const uint32_t *sp, int lane) {
+    for (int k = lane; k < P.N; k += WAVE) L.x[k] = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1 : -1;
+    for (int s = P.n_off + lane; s < P.K * WAVE; s += WAVE) L.w[wpos(s, P.K)] = 0.0f;
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int h = lane + 64 * m;
+        S.th[m] = h < P.H ? P.rbm_b[h] : 0.0f;
+    }
+    for (int k = 0; k < P.N; ++k) {
+        const float xk = (float)L.x[k];
+        const int ck = P.rbt_kcode[k];
+#pragma unroll
+        for (int m = 0; m < M; ++m) S.th[m] += Wt[ck + vb[m]] * xk;
+    }
+    rbt_pq(S, pad_last);
+}
+
+template <int M>
+__device__ __forceinline__ void rbt_move_theta(const DevProblem &P, const float *Wt,
+                                               const RbhLds &L, RbhState<M> &S, const int *vb,
+                                               bool pad_last, int i, int j) {
+    const float xi = (float)L.x[i], xj = (float)L.x[j];
+    const int ci = P.rbt_kcode[i], cj = P.rbt_kcode[j];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        float d = -2.0f * xi * Wt[ci + vb[m]];
+        d += -2.0f * xj * Wt[cj + vb[m]];
+        S.th[m] = S.th[m] + d;
+    }
+    rbt_pq(S, pad_last);
+}
+
+__device__ __forceinline__ float as_f(uint32_t v) { return __uint_as_float(v); }
+__device__ __forceinline__ uint32_t as_u(float v) { return __float_as_uint(v); }
+
+// every slot weight and the edge states; returns the anti count (the rbh
+// kernels' rbh_eval_all with table factors, four slots at a time)
+template <int M>
+__device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const float *T,
+                            RbhLds &L, const RbhState<M> &S, const int *vb, int lane) {
+    const int FQ = P.rbt_F * P.rbt_Q;
+    int n_act = 0;
+    for (int c = 0; c < P.n_off; c += WAVE) {
+        const int s = c + lane;
+        float bare = 0.0f;
+        int i = 0, j = 0;
+        if (s < P.n_off) {
+            slot_sites(P, s, &i, &j);
+            bare = (L.x[i] != L.x[j]) ? tb.jpm[s] : tb.jpmpm[s];
+        }
+        const bool act = s < P.n_off && bare != 0.0f;
+        if (s < P.n_off && !act) L.w[wpos(s, P.K)] = -bare;
+        const unsigned long long msk = __ballot(act);
+        const int pos = n_act + __popcll(msk & ((1ull << lane) - 1ull));
+        if (act) {
+            const float xi = (float)L.x[i], xj = (float)L.x[j];
+            int a = wpos(s, P.K);
+            if (bare > 0.0f) a |= RBH_POS_BARE;
+            const int sc = P.rbt_scode[s];
+            L.ent[pos] = make_int2(a, ((sc >> 20) * 2 + (xi < 0.0f ? 1 : 0)) * FQ + (sc & 0xfffff));
+            L.g[pos] = rbh_field(P, i, j, xi, xj) + __logf(fabsf(bare));
+        }
+        n_act += __popcll(msk);
+    }
+    wave_sync();
+    const int padn = (4 - (n_act & 3)) & 3;
+    if (n_act > 0 && lane < padn) {   // a multiple of four: repeat the last entry
+        L.ent[n_act + lane] = L.ent[n_act - 1];
+        L.g[n_act + lane] = L.g[n_act - 1];
+    }
+    wave_sync();
+    const int row = lane >> 4;
+    const int mine = row == 0 ? 0 : (row == 1 ? 2 : (row == 2 ? 1 : 3));
+    for (int t = 0; t < n_act; t += 4) {
+        const int4 e01 = *(const int4 *)(L.ent + t);
+        const int4 e23 = *(const int4 *)(L.ent + t + 2);
+        const int b0 = __builtin_amdgcn_readfirstlane(e01.y);
+        const int b1 = __builtin_amdgcn_readfirstlane(e01.w);
+        const int b2 = __builtin_amdgcn_readfirstlane(e23.y);
+        const int b3 = __builtin_amdgcn_readfirstlane(e23.w);
+        float p0 = 1.0f, p1 = 1.0f, p2 = 1.0f, p3 = 1.0f;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const float g0 = T[b0 + vb[m]], g1 = T[b1 + vb[m]];
+            const float g2 = T[b2 + vb[m]], g3 = T[b3 + vb[m]];
+            p0 *= fmaf(S.p[m], g0, S.q[m]);
+            p1 *= fmaf(S.p[m], g1, S.q[m]);
+            p2 *= fmaf(S.p[m], g2, S.q[m]);
+            p3 *= fmaf(S.p[m], g3, S.q[m]);
+        }
+        const bool ok = p0 > 1e-30f && p0 < 1e30f && p1 > 1e-30f && p1 < 1e30f &&
+                        p2 > 1e-30f && p2 < 1e30f && p3 > 1e-30f && p3 < 1e30f;
+        float l0, l1, l2, l3;
+        if (__ballot(!ok) == 0ull) {
+            l0 = __builtin_amdgcn_logf(p0);
+            l1 = __builtin_amdgcn_logf(p1);
+            l2 = __builtin_amdgcn_logf(p2);
+            l3 = __builtin_amdgcn_logf(p3);
+        } else {   // rare: factors far from 1, one logarithm each
+            l0 = l1 = l2 = l3 = 0.0f;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                l0 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b0 + vb[m]], S.q[m]));
+                l1 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b1 + vb[m]], S.q[m]));
+                l2 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b2 + vb[m]], S.q[m]));
+                l3 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b3 + vb[m]], S.q[m]));
+            }
+        }
+        // lanes 0-31: slot t (l + 32 added), 32-63: slot t + 1; likewise t + 2, t + 3
+        const auto r01 = __builtin_amdgcn_permlane32_swap(as_u(l0), as_u(l1), false, false);
+        const auto r23 = __builtin_amdgcn_permlane32_swap(as_u(l2), as_u(l3), false, false);
+        const float s01 = as_f(r01[0]) + as_f(r01[1]);
+        const float s23 = as_f(r23[0]) + as_f(r23[1]);
+        // rows 0..3: slots t, t + 2, t + 1, t + 3 (l + 16 added)
+        const auto r = __builtin_amdgcn_permlane16_swap(as_u(s01), as_u(s23), false, false);
+        float v = as_f(r[0]) + as_f(r[1]);
+        v += dpp_get<0x111, 0xf>(v);
+        v += dpp_get<0x112, 0xf>(v);
+        v += dpp_get<0x114, 0xf>(v);
+        v += dpp_get<0x118, 0xf>(v);
+        if ((lane & 15) == 15) {
+            const int a = L.ent[t + mine].x;
+            const float w = __expf(L.g[t + mine] + 0.69314718055994531f * v);
+            L.w[a & 0xffffff] = (a & RBH_POS_BARE) ? -w : w;
+        }
+    }
+    int cnt = 0;
+    if (P.has_jz) {
+        for (int e = lane; e < P.n_edges; e += WAVE) {
+            const int pk = P.edge_pack[e];
+            const unsigned char a = L.x[pk & 0xffff] != L.x[(pk >> 16) & 0xffff];
+            if (P.flags) L.anti[e] = a;
+            cnt += a;
+        }
+    }
+    wave_sync();
+    return wave_sum_int(cnt);
+}
+
+// the continuous-time propagation of k_propagate_rbh with table factors;
+// the factor tables are staged in LDS once per workgroup, walkers come from
+// the same per-launch queue
+template <int M>
+__global__ __launch_bounds__(1024) void k_propagate_rbt(
+    DevProblem P, Tables<float> tb, int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
+    double beta, double alw, uint32_t key0, uint32_t key1, uint32_t step, uint32_t tag,
+    int32_t *evals, const uint32_t *step_base) {
+    step += step_base ? *step_base : 0u;
+    extern __shared__ __align__(16) unsigned char lds[];
+    float *T = (float *)lds;
+    {
+        const uint4 *src = (const uint4 *)P.rbt_tab;
+        uint4 *dst = (uint4 *)T;
+        for (int t = threadIdx.x; t < P.rbt_words / 4; t += blockDim.x) dst[t] = src[t];
+        __syncthreads();
+    }
+    const float *Wt = T + (size_t)2 * P.rbt_ncls * P.rbt_F * P.rbt_Q;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    RbhLds L = rbh_carve_at(P, lds, wave, (size_t)P.rbt_words * 4);
+    WalkerLds<float> WL;
+    WL.w = L.w;
+    WL.x = L.x;
+    WL.anti = L.anti;
+    int vb[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) vb[m] = P.rbt_unit[lane + 64 * m];
+    const bool pad_last = lane + 64 * (M - 1) >= P.H;
+    RbhState<M> S;
+    for (int64_t wk = rbh_next(P.rbm_queue, lane); wk < nw; wk = rbh_next(P.rbm_queue, lane)) {
+        uint32_t *sp = spins + wk * P.words;
+        rbt_start<M>(P, Wt, L, S, vb, pad_last, sp, lane);
+        int n_anti = rbt_eval_all<M>(P, tb, T, L, S, vb, lane);
+        double diag = diag_value(P, L.x, L.anti, n_anti, lane);
+        double lw = log_w[wk];
+        double t = 0.0, eL = 0.0;
+        uint32_t it = 0;
+        while (t < beta) {
+            float lane_sum;
+            const float incl = lane_scan(P, WL, lane, &lane_sum);
+            const float R = lane_bcast(incl, WAVE - 1);
+            eL = diag - (double)R;
+            const gfmc_u4 r = gfmc_philox(it, (uint32_t)wk, step, tag, key0, key1);
+            const double psi = gfmc_u53(r.z, r.w);
+            int idx = -1;
+            double tau_raw = INFINITY;
+            if (R > 0.0f) {
+                float u = gfmc_u24(r.x) * R;
+                const float cap = below(R);
+                if (u > cap) u = cap;
+                idx = sample_slot(P, WL, incl, lane_sum, u, lane);
+                tau_raw = gfmc_neg_log_unit(1.0 - psi) / (double)R;
+            }
+            const double rem = beta - t;
+            const double tau = tau_raw < rem ? tau_raw : rem;
+            lw = lw - eL * tau;
+            t = t + tau;
+            ++it;
+            if (tau_raw < rem && idx >= 0) {
+                int i, j;
+                slot_sites(P, idx, &i, &j);
+                rbt_move_theta<M>(P, Wt, L, S, vb, pad_last, i, j);
+                wave_sync();
+                if (lane == 0) {
+                    L.x[i] = (signed char)-L.x[i];
+                    L.x[j] = (signed char)-L.x[j];
+                }
+                wave_sync();
+                n_anti = rbt_eval_all<M>(P, tb, T, L, S, vb, lane);
+                diag = diag_value(P, L.x, L.anti, n_anti, lane);
+            }
+        }
+        lw = lw - alw * beta;
+        store_spins(P, sp, L.x, lane);
+        if (lane == 0) {
+            log_w[wk] = lw;
+            e_loc[wk] = eL;
+            if (evals) evals[wk] += (int32_t)it;
+        }
+        wave_sync();
+    }
 }

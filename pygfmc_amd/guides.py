@@ -163,7 +163,7 @@ class RBMSpinHalf:
     """
 
     def __init__(self, N, alpha=4, hidden=None, W=None, a=None, b=None, seed=0, scale=0.01,
-                 dtype=np.float32, orbits=None):
+                 dtype=np.float32, orbits=None, translations=None):
         self.N = int(N)
         H = int(hidden) if hidden is not None else int(alpha) * self.N
         rng = np.random.default_rng(seed)
@@ -187,6 +187,10 @@ class RBMSpinHalf:
             if o.shape != (H * self.N + self.N + H,) or o.min() < 0:
                 raise ValueError("orbits must give every flat parameter an orbit index >= 0")
             self.orbits = o
+        # the site permutations of a translation-invariant RBM (hidden unit
+        # f * n_T + r is filter f moved by translation r), when it is one
+        self.translations = (None if translations is None
+                             else np.asarray(translations, dtype=np.int64))
 
     @classmethod
     def symmetric(cls, translations, n_filters, filters=None, b=None, a=0.0, seed=0,
@@ -212,7 +216,8 @@ class RBMSpinHalf:
                 oW[f * n_T + r, T[r]] = f * N + np.arange(N)
         orbits = np.concatenate([oW.reshape(-1), np.full(N, F * N + F),
                                  F * N + np.repeat(np.arange(F), n_T)])
-        return cls(N, hidden=H, W=W, a=np.full(N, a), b=np.repeat(bf, n_T), orbits=orbits)
+        return cls(N, hidden=H, W=W, a=np.full(N, a), b=np.repeat(bf, n_T), orbits=orbits,
+                   translations=T)
 
     @classmethod
     def from_jastrow(cls, edges, translations, beta, n_filters, seed=0, scale=0.01):
@@ -251,7 +256,7 @@ class RBMSpinHalf:
 
     def with_parameters(self, W, a, b):
         return RBMSpinHalf(self.N, hidden=self.hidden, W=W, a=a, b=b, dtype=self.dtype,
-                           orbits=self.orbits)
+                           orbits=self.orbits, translations=self.translations)
 
     def fingerprint(self):
         return (b"rbm|" + np.int64(self.N).tobytes() + np.int64(self.hidden).tobytes()

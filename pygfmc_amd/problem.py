@@ -174,7 +174,7 @@ def lower(H, guide=None, full=None):
 class Problem:
     """Device tables of one (SpinHalf, guide, precision) and the calls on them."""
 
-    def __init__(self, H, guide, precision, full=None):
+    def __init__(self, H, guide, precision, full=None, tied=True):
         lib = _lib.load()
         _lib.device()
         self.H, self.guide = H, guide
@@ -185,6 +185,22 @@ class Problem:
         _lib.check(lib.gfmc_problem_create(ctypes.byref(d), precision, ctypes.byref(h)),
                    "gfmc_problem_create")
         self.handle = h
+        # a translation-tied RBM on a torus propagates with table factors
+        # (gfmc_rbm_set_tied) where the problem allows it (pair moves with
+        # J_pmpm = 0, at most 512 hidden units, tables within the LDS)
+        self.tied = None
+        if tied and isinstance(guide, RBMSpinHalf) and H.n_hx == 0:
+            from .torus import tied_tables
+            t = tied_tables(H.edge_i, H.edge_j, guide)
+            if t is not None:
+                rc = lib.gfmc_rbm_set_tied(
+                    h, t["F"], t["n_T"], t["Q"], t["n_cls"], t["unit_off"].ctypes.data,
+                    t["slot_code"].ctypes.data, t["site_code"].ctypes.data,
+                    t["q_site"].ctypes.data, t["q_pair"].ctypes.data, _lib.stream())
+                if rc == 0:
+                    self.tied = t
+                elif rc != _lib.GFMC_EUNSUP:
+                    _lib.check(rc, "gfmc_rbm_set_tied")
         info = _lib.ProblemInfo()
         _lib.check(lib.gfmc_problem_get_info(h, ctypes.byref(info)), "gfmc_problem_get_info")
         self.info = info

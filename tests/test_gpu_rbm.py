@@ -152,3 +152,137 @@ def test_rbm_propagation_matches_fp64_oracle_at_size(orc, which):
                                atol=RTOL * np.abs(e_ref).max())
     scale = np.abs(e_ref).max() * 0.2
     np.testing.assert_allclose(lw.cpu().numpy()[same], lw_ref[same], rtol=RTOL, atol=RTOL * scale)
+
+
+# ---- translation-tied RBM on a torus: the table-factor kernel ---------------
+
+def tied_systems():
+    from pygfmc_amd import RBMSpinHalf
+    rng = np.random.default_rng(11)
+    out = []
+    L = square(10, 10)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    out.append(("square10_tied", H,
+                RBMSpinHalf.symmetric(L.extras["translations"], 4, b=rng.normal(scale=0.2, size=4),
+                                      a=0.05, seed=3, scale=0.05)))
+    L = square(6, 4)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 0.7, dtype=np.float64)
+    out.append(("square6x4_xxz_tied", H,
+                RBMSpinHalf.symmetric(L.extras["translations"], 3, b=rng.normal(scale=0.2, size=3),
+                                      a=-0.1, seed=5, scale=0.2)))
+    L = square(10, 10)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    out.append(("square10_from_jastrow", H,
+                RBMSpinHalf.from_jastrow(L.edges, L.extras["translations"], 0.15, 4, seed=1,
+                                         scale=0.01)))
+    return out
+
+
+TIED = tied_systems()
+TIED_IDS = [s[0] for s in TIED]
+
+
+def propagate_once(prob, spins, beta, alw, seed, step):
+    nw = spins.shape[0]
+    s = to_dev(spins)
+    lw = torch.zeros(nw, dtype=torch.float64, device="cuda")
+    e = torch.zeros(nw, dtype=torch.float64, device="cuda")
+    ev = torch.zeros(nw, dtype=torch.int32, device="cuda")
+    prob.propagate(s, lw, e, beta, alw, seed, step, 0, ev)
+    torch.cuda.synchronize()
+    return (s.cpu().numpy().view(np.uint32), lw.cpu().numpy(), e.cpu().numpy(),
+            ev.cpu().numpy())
+
+
+@pytest.mark.parametrize("name,H,G", TIED, ids=TIED_IDS)
+def test_tied_rbm_propagation_matches_fp64_oracle(orc, name, H, G):
+    """The table-factor kernel against the FP64 oracle: one propagation from
+    the same walkers with the same uniforms, most walkers on the oracle's
+    path with its evaluation counts, local energies and log weights."""
+    say("tied rbm propagate", name)
+    prob = Problem(H, G, _lib.GFMC_FP32)
+    assert prob.tied is not None
+    nw = 192
+    spins = orc.init_sz(H.N, nw, H.N // 2, 13)
+    s_ref, lw_ref, e_ref, ev_ref = orc.Problem(lower(H, G)).propagate(
+        spins, np.zeros(nw), np.zeros(nw), 0.2, -50.0, 91, 5, 0)
+    s, lw, e, ev = propagate_once(prob, spins, 0.2, -50.0, 91, 5)
+    same = np.all(s == s_ref, axis=1) & (ev == ev_ref)
+    say("  same path:", same.mean(), "evals", ev_ref.mean())
+    assert same.mean() > 0.9 and ev_ref.mean() > 2
+    np.testing.assert_allclose(e[same], e_ref[same], rtol=RTOL, atol=RTOL * np.abs(e_ref).max())
+    scale = np.abs(e_ref).max() * 0.2
+    np.testing.assert_allclose(lw[same], lw_ref[same], rtol=RTOL, atol=RTOL * scale)
+
+
+@pytest.mark.parametrize("name,H,G", TIED, ids=TIED_IDS)
+def test_tied_rbm_matches_untied_kernel(name, H, G):
+    """Tied and untied kernels evaluate the same factors (bit-identical lane
+    products) and differ only in the order of one FP32 sum over lanes: the
+    same moves for nearly every walker, energies to FP32 rounding."""
+    say("tied vs untied", name)
+    pt = Problem(H, G, _lib.GFMC_FP32)
+    pu = Problem(H, G, _lib.GFMC_FP32, tied=False)
+    assert pt.tied is not None and pu.tied is None
+    from oracle import init_sz
+    spins = init_sz(H.N, 1024, H.N // 2, 21)
+    a = propagate_once(pt, spins, 0.2, -50.0, 7, 3)
+    b = propagate_once(pu, spins, 0.2, -50.0, 7, 3)
+    same = np.all(a[0] == b[0], axis=1) & (a[3] == b[3])
+    say("  same path:", same.mean())
+    assert same.mean() > 0.99
+    np.testing.assert_allclose(a[2][same], b[2][same], rtol=1e-5,
+                               atol=1e-5 * np.abs(b[2]).max())
+    np.testing.assert_allclose(a[1][same], b[1][same], rtol=1e-5,
+                               atol=1e-5 * np.abs(b[2]).max())
+
+
+def test_tied_tables_follow_set_params():
+    """gfmc_rbm_set_params on a tied problem rebuilds the factor tables: the
+    propagation then equals an untied problem built with the new
+    parameters."""
+    from pygfmc_amd import RBMSpinHalf
+    from pygfmc_amd.training import flat_params
+    say("tied set_params")
+    L = square(6, 6)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    T = L.extras["translations"]
+    G0 = RBMSpinHalf.symmetric(T, 2, seed=1, scale=0.1)
+    G1 = RBMSpinHalf.symmetric(T, 2, b=[0.3, -0.2], a=0.1, seed=2, scale=0.25)
+    pt = Problem(H, G0, _lib.GFMC_FP32)
+    assert pt.tied is not None
+    flat = torch.as_tensor(flat_params(G1), device="cuda")
+    _lib.check(_lib.load().gfmc_rbm_set_params(pt.handle, _lib.ptr(flat), _lib.stream()),
+               "gfmc_rbm_set_params")
+    pu = Problem(H, G1, _lib.GFMC_FP32, tied=False)
+    from oracle import init_sz
+    spins = init_sz(36, 512, 18, 4)
+    a = propagate_once(pt, spins, 0.2, -20.0, 5, 1)
+    b = propagate_once(pu, spins, 0.2, -20.0, 5, 1)
+    same = np.all(a[0] == b[0], axis=1) & (a[3] == b[3])
+    say("  same path:", same.mean())
+    assert same.mean() > 0.99
+    np.testing.assert_allclose(a[2][same], b[2][same], rtol=1e-5,
+                               atol=1e-5 * np.abs(b[2]).max())
+
+
+def test_tied_rbm_gfmc_energy_matches_ed():
+    """Any positive guide gives the exact ground energy: 4 x 4 Heisenberg
+    with a tied RBM guide on the table-factor kernel, within 4 sigma + 2e-3
+    of ED."""
+    from pygfmc_amd import RBMSpinHalf
+    from pygfmc_amd.problem import get_problem
+    say("tied rbm ED square4")
+    L = square(4, 4)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    e0 = ground_energy(H)
+    G = RBMSpinHalf.from_jastrow(L.edges, L.extras["translations"], 0.2, 3, seed=2, scale=0.05)
+    assert get_problem(H, G).tied is not None
+    w = prepare_walkers_sz(1024, 16, 1)
+    _, E, _, stats, raw = run(w, 2000, H, 0.2, G, [4, 32], rng_key=4, warmup_steps=400,
+                              verbosity=0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        E, _, err, _ = evaluate_observables(raw, n_blocks=20)
+    say("  E", E[32], "+-", err[32], "exact", e0, "it/s", stats["it_per_sec"])
+    assert abs(E[32] - e0) < 4 * err[32] + 2e-3
