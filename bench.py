@@ -61,7 +61,11 @@ def parse():
     ap.add_argument("--mlp-walkers", type=int, default=262144)
     ap.add_argument("--mlp-steps", type=int, default=3)
     ap.add_argument("--race-train-iters", type=int, default=1500)
-    ap.add_argument("--race-rbm-steps", type=int, default=1000)
+    ap.add_argument("--race-steps", type=int, default=10000,
+                    help="production steps of each run of the learned-guide section; the "
+                         "jackknife takes blocks of 200 steps (longer than the p = 128 "
+                         "estimator's window): 50 blocks at 10000 steps, a ~20 %% relative "
+                         "error of each run's sigma^2")
     ap.add_argument("--summarize-profile", default=None,
                     help="print a text summary of a rocprofv3 database and exit")
     ap.add_argument("--summarize-kernel", action="append", default=None,
@@ -231,10 +235,11 @@ def main():
     # lanes per walker of the propagation kernel: the autotuner, or pinned
     if args.lanes == "auto":
         from pygfmc_amd.autotune import tune_problem
-        lanes, tune_times = tune_problem(p, chain.spins, beta, alw)
+        shape, tune_times = tune_problem(p, chain.spins, beta, alw)
     else:
         p.set_lanes(int(args.lanes))
-        lanes, tune_times = int(args.lanes), None
+        shape, tune_times = (int(args.lanes), 1, 0), None
+    lanes = shape[0]
 
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib)
@@ -316,7 +321,8 @@ def main():
                    "n_walkers": nw, "delta": args.delta, "step_size": beta,
                    "guide": "Jastrow beta=0.3", "global_batch": nw, "seq_len": N,
                    "parallelism": "none (1 GPU)", "hip_graph_steps": chunk,
-                   "lanes_per_walker": lanes, "autotune_seconds": tune_times,
+                   "lanes_per_walker": lanes, "waves_per_workgroup": shape[1],
+                   "xcd_contiguous_walkers": bool(shape[2]), "autotune_seconds": tune_times,
                    "evals_per_walker_step": evals_per_ws},
         "roofline": {"bound": "valu",
                      "compute_unit": "FP32 vector ALU (no GEMM on this path; MI355X's FP32 "
@@ -380,7 +386,7 @@ def xxz_deltas_section(args):
         chain = Chain(p, prepare_walkers_sz(args.walkers, 1024, 12345))
         alw = _warmup(chain, PhiloxKey(777, 0), 0.2, None, args.gfmc_warmup, 0)
         chain.steps(500, 0.2, alw, 777, 1 << 20)
-        lanes, _ = tune_problem(p, chain.spins, 0.2, alw)
+        (lanes, wpb, xcd), _ = tune_problem(p, chain.spins, 0.2, alw)
         graph = StepGraph(chain, 10, 0.2, alw, 777, evals=True)
         pos = (1 << 20) + 500
         graph.launch(pos)
@@ -394,7 +400,8 @@ def xxz_deltas_section(args):
         res[f"delta{delta:g}"] = {
             "walker_steps_per_s": args.walkers * 20 / dt, "ms_per_step": dt / 20 * 1e3,
             "evals_per_walker_step": float(chain.evals.to(torch.float64).sum().item())
-            / (args.walkers * 20), "lanes_per_walker": lanes}
+            / (args.walkers * 20), "lanes_per_walker": lanes, "waves_per_workgroup": wpb,
+            "xcd_contiguous_walkers": bool(xcd)}
     return {"workload": f"xxz_square32x32_jastrow0.3_w{args.walkers}_step0.2_fp32", **res}
 
 
@@ -622,9 +629,9 @@ def kagome_mlp_section(args):
             "roofline": roof, "cpu_baseline": cpu}
 
 
-def timed_run(walkers, n_steps, H, G, p_array, seed, warmup):
+def timed_run(walkers, n_steps, H, G, p_array, seed, warmup, n_blocks=20):
     """run() bracketed by device synchronisation: (GPU seconds, E, sigma at
-    max p, stats)."""
+    max p, stats); the jackknife over ``n_blocks`` blocks."""
     import warnings
 
     import torch
@@ -638,7 +645,7 @@ def timed_run(walkers, n_steps, H, G, p_array, seed, warmup):
     secs = time.perf_counter() - t0
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
-        E, _, err, _ = evaluate_observables(raw, n_blocks=20)
+        E, _, err, _ = evaluate_observables(raw, n_blocks=n_blocks)
     pm = max(p_array)
     return secs, float(E[pm]), float(err[pm]), st
 
@@ -646,9 +653,9 @@ def timed_run(walkers, n_steps, H, G, p_array, seed, warmup):
 def sigma_race(args):
     """Second metric on the 10 x 10 periodic Heisenberg lattice (J_x = J_y =
     -1, J_z = 1, Sz = 0), p = [4, 16, 64, 128], 16 384 walkers, step 0.2:
-    GPU-seconds until the jackknife error bar (20 blocks) of E at p = 128
-    reaches that of the Jastrow-FP64 run (1000 production steps after 400
-    warm-up steps), for the learned BF16 RBM guide (alpha = 4: four
+    GPU-seconds until the jackknife error bar (blocks of 200 steps) of E at
+    p = 128 reaches that of the Jastrow-FP64 run (10 000 production steps
+    after 400 warm-up steps), for the learned BF16 RBM guide (alpha = 4: four
     translation-invariant filters, 400 hidden units; started as the
     nearest-neighbour Jastrow guide and trained on the GPU by local-energy
     variance minimisation with natural-gradient steps on the walkers its own
@@ -675,10 +682,11 @@ def sigma_race(args):
     for beta in (0.15, 0.3):
         G = JastrowSpinHalf(100, lat.edges, beta=beta, dtype=np.float64)
         w = prepare_walkers_sz(nw, 100, 0)
-        secs, E, err, st = timed_run(w, 1000, H64, G, P, 1, 400)
-        jas[beta] = dict(beta=beta, gpu_s=secs, E=E, sigma=err, steps=1000, warmup=400,
-                         walkers=nw, evals_per_step=st["evals_per_step_mean"],
-                         ms_per_step=st["runtime"] / 1000 * 1e3)
+        secs, E, err, st = timed_run(w, args.race_steps, H64, G, P, 1, 400,
+                                     n_blocks=args.race_steps // 200)
+        jas[beta] = dict(beta=beta, gpu_s=secs, E=E, sigma=err, steps=args.race_steps,
+                         warmup=400, walkers=nw, evals_per_step=st["evals_per_step_mean"],
+                         ms_per_step=st["runtime"] / args.race_steps * 1e3)
         progress(f"jastrow beta={beta}: E {E:.4f} +- {err:.4f}")
     out["jastrow_fp64_beta0.15"] = jas[0.15]
     out["jastrow_fp64_beta0.3"] = jas[0.3]
@@ -690,7 +698,7 @@ def sigma_race(args):
     t0 = time.perf_counter()
     G, hist = train_variance(H32, rbm0, w, n_iter=args.race_train_iters, steps_per_iter=2,
                              lr=1.0, clip=3.0, rng_key=3, sampler="self", warmup_steps=100,
-                             optimizer="sr", sr_shift=1e-2, sr_abs=1e-4, sr_max_norm=0.02)
+                             optimizer="sr", sr_shift=1e-2, sr_abs=1e-4, sr_max_norm=0.05)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     v = hist["variance"]
@@ -698,14 +706,15 @@ def sigma_race(args):
     # production with the trained guide
     w = prepare_walkers_sz(nw, 100, 0)
     warm = 100
-    secs, E, err, st = timed_run(w, args.race_rbm_steps, H32, G, P, 1, warm)
-    frac_prod = args.race_rbm_steps / (args.race_rbm_steps + warm)
+    secs, E, err, st = timed_run(w, args.race_steps, H32, G, P, 1, warm,
+                                 n_blocks=args.race_steps // 200)
+    frac_prod = args.race_steps / (args.race_steps + warm)
     rbm = dict(alpha=4, hidden=400, filters=4, train_gpu_s=train_s,
                train_iters=args.race_train_iters, train_walkers=4096, optimizer="natural gradient",
                variance_start=float(v[:5].mean()), variance_end=float(v[-20:].mean()),
-               gpu_s=secs, E=E, sigma=err, steps=args.race_rbm_steps, warmup=warm, walkers=nw,
+               gpu_s=secs, E=E, sigma=err, steps=args.race_steps, warmup=warm, walkers=nw,
                evals_per_step=st["evals_per_step_mean"],
-               ms_per_step=st["runtime"] / args.race_rbm_steps * 1e3)
+               ms_per_step=st["runtime"] / args.race_steps * 1e3)
     # the propagation kernel's roofline on this workload and the CPU oracle
     rbm["roofline"], rbm["cpu_baseline"] = rbm_kernel_measure(H32, G, args.cpu_seconds)
     out["rbm_bf16"] = rbm

@@ -158,6 +158,15 @@ int gfmc_problem_destroy(gfmc_problem *p);
  * last bits.  gfmc_problem_info.lanes_ok lists the usable values. */
 int gfmc_problem_set_lanes(gfmc_problem *p, int32_t lanes);
 
+/* Launch shape of the record-path propagation kernel (no reference
+ * counterpart; tuning knobs like the lanes): waves_per_block 1, 2 or 4
+ * wavefronts of walkers per workgroup (record paths only; fewer when the
+ * walkers' LDS would exceed 64 KiB per workgroup), xcd_remap 1 to map
+ * workgroups to walkers so that each of the eight XCDs holds a contiguous
+ * range of walkers.  Results do not depend on either (each walker's work is
+ * the same).  Default 1, 0. */
+int gfmc_problem_set_launch(gfmc_problem *p, int32_t waves_per_block, int32_t xcd_remap);
+
 /* Propagator of gfmc_propagate / gfmc_run_steps / the graphs and batches
  * built from this problem (require_lam, evolution.py:724-741): method
  * GFMC_METHOD_CONT (default; the step argument is beta) or
@@ -394,7 +403,8 @@ int gfmc_rbm_set_params(gfmc_problem *p, const float *flat, void *stream);
  * h = f n_T + r sees W[h][k] = w_f(k - r)).  The factor of unit h in a pair
  * slot's ratio then comes from a table of (slot class, x_i, f, site i - r)
  * instead of an exponential per factor; moves, weights and energies are
- * those of the untied kernels up to the order of one FP32 sum over lanes.
+ * those of the untied kernels up to FP32 rounding (the lane products are
+ * the same; they are combined across lanes in another order).
  * Tables (int32, built by pygfmc_amd.torus.tied_tables): unit_off[Hp]
  * (Hp = hidden rounded up to 64) the offset f Q + O - code(r) of every
  * hidden unit (padded units F Q + O), slot_code[n_off] code(i) | class <<

@@ -90,6 +90,7 @@ SIGNATURES = [
     ("gfmc_problem_destroy", ctypes.c_int, [_vp]),
     ("gfmc_problem_get_info", ctypes.c_int, [_vp, ctypes.POINTER(ProblemInfo)]),
     ("gfmc_problem_set_lanes", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("gfmc_problem_set_launch", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     ("gfmc_problem_set_method", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
     ("gfmc_init_walkers_sz", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                             ctypes.c_uint64, _vp, _vp]),

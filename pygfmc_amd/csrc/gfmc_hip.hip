@@ -115,6 +115,7 @@ struct DevProblem {
     // record path
     int fast, R, RN, RE, flags;
     int method;       // GFMC_METHOD_* of the propagation kernels
+    int wpb, xcd;     // k_propagate launch shape (gfmc_problem_set_launch)
     double lam;       // the shift lambda of the discrete methods
     const uint4 *slot_rec;   // [K * 64 * 2], in LDS order (record p -> LDS position p)
     // the same per walker-group size G = 64, 32, 16: K_G = ceil(n_off / G),
@@ -1115,22 +1116,34 @@ __global__ void k_carry_commit(int32_t *state, int G) {
     }
 }
 
-// one wavefront per workgroup, 64 / G walkers per wavefront
+// P.wpb wavefronts per workgroup (the record paths; 1 on the generic path),
+// 64 / G walkers per wavefront.  With P.xcd, workgroup b of a grid of nb
+// (nb8 = nb rounded down to a multiple of 8) runs block (b % 8) nb8 / 8 + b / 8
+// when b < nb8: the dispatcher hands workgroups to the eight XCDs in turn,
+// so each XCD then holds a contiguous range of walkers (and the siblings
+// that read one parent's carried row share that XCD's L2).
 template <typename T, int FAST, int G, int METHOD>
-__global__ __launch_bounds__(WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
-                                                    uint32_t *spins, double *log_w,
-                                                    double *e_loc, double beta, double alw,
-                                                    uint32_t key0, uint32_t key1,
-                                                    uint32_t step, uint32_t tag,
-                                                    int32_t *evals, const uint32_t *step_base,
-                                                    CarryDev cy) {
+__global__ __launch_bounds__(4 * WAVE) void k_propagate(DevProblem P, Tables<T> tb, int64_t nw,
+                                                        uint32_t *spins, double *log_w,
+                                                        double *e_loc, double beta, double alw,
+                                                        uint32_t key0, uint32_t key1,
+                                                        uint32_t step, uint32_t tag,
+                                                        int32_t *evals,
+                                                        const uint32_t *step_base, CarryDev cy) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int grp = threadIdx.x / G;
-    const int64_t wk = (int64_t)blockIdx.x * (WAVE / G) + grp;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int grp = lane / G;
+    int64_t b = blockIdx.x;
+    if (P.xcd) {
+        const int64_t nb8 = (int64_t)gridDim.x & ~(int64_t)7;
+        if (b < nb8) b = (b & 7) * (nb8 >> 3) + (b >> 3);
+    }
+    const int64_t wk = (b * P.wpb + wave) * (WAVE / G) + grp;
     if (wk >= nw) return;
     const uint32_t st = step + (step_base ? *step_base : 0u);
-    unsigned char *base = G == WAVE ? lds : lds + (size_t)grp * walker_lds_bytes<T, G>(P);
-    propagate_walker<T, FAST, G, METHOD>(P, tb, base, threadIdx.x, (uint32_t)wk,
+    unsigned char *base =
+        FAST ? lds + (size_t)(wave * (WAVE / G) + grp) * walker_lds_bytes<T, G>(P) : lds;
+    propagate_walker<T, FAST, G, METHOD>(P, tb, base, lane, (uint32_t)wk,
                                          spins + wk * P.words,
                                  log_w + wk, e_loc + wk, evals ? evals + wk : nullptr, beta,
                                  alw, key0, key1, st, tag, cy);
@@ -2266,24 +2279,35 @@ static void launch_rates(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
                            (size_t)p->lds_bytes, st, p->dp, tb, nw, spins, rates, e_loc, log_diff);
 }
 
+// grid: P.wpb wavefronts of 64 / G walkers per workgroup; lds: per workgroup
 template <typename T, int FAST, int G>
-static void launch_prop_g(const gfmc_problem *p, const Tables<T> &tb, unsigned grid, size_t lds,
-                          int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
-                          double beta, double alw, uint32_t k0, uint32_t k1, uint32_t step,
-                          uint32_t tag, int32_t *evals, hipStream_t st) {
+static void launch_prop_g(const gfmc_problem *p, const Tables<T> &tb, int64_t nw,
+                          uint32_t *spins, double *log_w, double *e_loc, double beta,
+                          double alw, uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
+                          int32_t *evals, hipStream_t st) {
     CarryDev cy{};
     if (FAST && g_carry && g_carry->on && !p->dp.flags) cy = *g_carry;
+    // at most 64 KiB of LDS per workgroup
+    const int64_t wbytes = walker_lds_bytes<T, G>(p->dp);
+    int wpb = FAST ? p->dp.wpb : 1;
+    while (wpb > 1 && (int64_t)wpb * (WAVE / G) * wbytes > 64 * 1024) wpb >>= 1;
+    const int64_t per_block = (int64_t)wpb * (WAVE / G);
+    const unsigned grid = (unsigned)((nw + per_block - 1) / per_block);
+    const size_t lds = FAST ? (size_t)(per_block * wbytes) : (size_t)p->lds_bytes;
+    const dim3 block(wpb * WAVE);
+    DevProblem dp = p->dp;   // the kernel indexes walkers with the launched shape
+    dp.wpb = wpb;
     if (p->dp.method == GFMC_METHOD_SINGLE)
-        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_SINGLE>), dim3(grid), dim3(WAVE),
-                           lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
+        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_SINGLE>), dim3(grid), block,
+                           lds, st, dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
                            tag, evals, g_step_base, cy);
     else if (p->dp.method == GFMC_METHOD_MULTI)
-        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_MULTI>), dim3(grid), dim3(WAVE),
-                           lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
+        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_MULTI>), dim3(grid), block,
+                           lds, st, dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
                            tag, evals, g_step_base, cy);
     else
-        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_CONT>), dim3(grid), dim3(WAVE),
-                           lds, st, p->dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
+        hipLaunchKernelGGL((k_propagate<T, FAST, G, GFMC_METHOD_CONT>), dim3(grid), block,
+                           lds, st, dp, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
                            tag, evals, g_step_base, cy);
     if (cy.on) hipLaunchKernelGGL(k_carry_commit, dim3(1), dim3(WAVE), 0, st, cy.state, G);
 }
@@ -2296,16 +2320,14 @@ static void launch_prop_rec(const gfmc_problem *p, const Tables<T> &tb, int64_t 
                             double alw, uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
                             int32_t *evals, hipStream_t st) {
     if (p->lanes == 32)
-        launch_prop_g<T, FAST, 32>(p, tb, (unsigned)((nw + 1) / 2),
-                                   (size_t)(2 * walker_lds_bytes<T, 32>(p->dp)), nw, spins,
-                                   log_w, e_loc, beta, alw, k0, k1, step, tag, evals, st);
+        launch_prop_g<T, FAST, 32>(p, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
+                                   evals, st);
     else if (p->lanes == 16)
-        launch_prop_g<T, FAST, 16>(p, tb, (unsigned)((nw + 3) / 4),
-                                   (size_t)(4 * walker_lds_bytes<T, 16>(p->dp)), nw, spins,
-                                   log_w, e_loc, beta, alw, k0, k1, step, tag, evals, st);
+        launch_prop_g<T, FAST, 16>(p, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
+                                   evals, st);
     else
-        launch_prop_g<T, FAST, 64>(p, tb, (unsigned)nw, (size_t)p->lds_bytes, nw, spins, log_w,
-                                   e_loc, beta, alw, k0, k1, step, tag, evals, st);
+        launch_prop_g<T, FAST, 64>(p, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
+                                   evals, st);
 }
 
 template <typename T>
@@ -2314,8 +2336,8 @@ static void launch_propagate(const gfmc_problem *p, const Tables<T> &tb, int64_t
                              double alw, uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
                              int32_t *evals, hipStream_t st) {
     if (!p->dp.fast) {
-        launch_prop_g<T, 0, 64>(p, tb, (unsigned)nw, (size_t)p->lds_bytes, nw, spins, log_w,
-                                e_loc, beta, alw, k0, k1, step, tag, evals, st);
+        launch_prop_g<T, 0, 64>(p, tb, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag,
+                                evals, st);
         return;
     }
     if constexpr (std::is_same<T, float>::value) {
@@ -2623,6 +2645,8 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
     const size_t D = (size_t)P.max_deg;
     HostRecords hr;
     P.fast = build_records(d, hr) ? 1 : 0;
+    P.wpb = 1;
+    P.xcd = 0;
     P.R = hr.R;
     P.RN = hr.RN;
     P.RE = hr.RE;
@@ -2965,6 +2989,20 @@ int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q
 int gfmc_rbm_untie(gfmc_problem *p) {
     if (!p) return fail(GFMC_EINVAL, "gfmc_rbm_untie: NULL problem");
     p->dp.rbt = 0;
+    return 0;
+}
+
+int gfmc_problem_set_launch(gfmc_problem *p, int32_t waves_per_block, int32_t xcd_remap) {
+    if (!p) return fail(GFMC_EINVAL, "gfmc_problem_set_launch: NULL problem");
+    if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
+        return fail(GFMC_EINVAL, "gfmc_problem_set_launch: waves_per_block must be 1, 2 or 4");
+    if (xcd_remap != 0 && xcd_remap != 1)
+        return fail(GFMC_EINVAL, "gfmc_problem_set_launch: xcd_remap must be 0 or 1");
+    if (waves_per_block > 1 && !p->dp.fast)
+        return fail(GFMC_EUNSUP, "gfmc_problem_set_launch: this problem's propagation runs one "
+                                 "wavefront per workgroup (generic path)");
+    p->dp.wpb = waves_per_block;
+    p->dp.xcd = xcd_remap;
     return 0;
 }
 

@@ -448,13 +448,15 @@ __global__ __launch_bounds__(256) void k_rbm_pack_t(int N, int H, int Hp, const 
 // zeros that the padded hidden units read.  Per lane and chunk m the kernel
 // keeps off(h) in a register; a slot adds its block base and code(i) (one
 // integer add, one LDS read, one fma and one multiply per factor).  Slots are
-// evaluated four at a time; the four lane sums of log2 of the products are
-// reduced together: v_permlane32_swap pairs lanes l and l + 32 of two slots
-// at once, v_permlane16_swap rows 0 / 1 and 2 / 3 of the two pairs, then a
-// row scan (row_shr 1, 2, 4, 8) leaves slot t + {0, 2, 1, 3} in lane 15 of
-// rows 0..3.  Each slot's sum is (x_l + x_{l+32}) + (x_{l+16} + x_{l+48})
-// over l < 16, then the row scan: the same tree for every slot, so a slot's
-// weight does not depend on its place in the list.
+// evaluated four at a time and their lane products combined together:
+// v_permlane32_swap pairs lanes l and l + 32 of two slots at once,
+// v_permlane16_swap rows 0 / 1 and 2 / 3 of the two pairs (products, so each
+// lane then holds 4 M factors of one slot: one log2 per lane for four slots),
+// then a row scan (row_shr 1, 2, 4, 8) of the logarithms leaves slot
+// t + {0, 2, 1, 3} in lane 15 of rows 0..3.  Each slot's value is
+// sum_{l<16} log2((x_l x_{l+32}) (x_{l+16} x_{l+48})) in the row scan's
+// order: the same tree for every slot, so a slot's weight does not depend on
+// its place in the list (unless the quad takes the per-factor fallback).
 
 __global__ __launch_bounds__(256) void k_rbt_tables(DevProblem P, float *tab) {
     const int F = P.rbt_F, Q = P.rbt_Q, nc = P.rbt_ncls;
@@ -590,16 +592,22 @@ __device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const 
             p2 *= fmaf(S.p[m], g2, S.q[m]);
             p3 *= fmaf(S.p[m], g3, S.q[m]);
         }
-        const bool ok = p0 > 1e-30f && p0 < 1e30f && p1 > 1e-30f && p1 < 1e30f &&
-                        p2 > 1e-30f && p2 < 1e30f && p3 > 1e-30f && p3 < 1e30f;
-        float l0, l1, l2, l3;
-        if (__ballot(!ok) == 0ull) {
-            l0 = __builtin_amdgcn_logf(p0);
-            l1 = __builtin_amdgcn_logf(p1);
-            l2 = __builtin_amdgcn_logf(p2);
-            l3 = __builtin_amdgcn_logf(p3);
-        } else {   // rare: factors far from 1, one logarithm each
-            l0 = l1 = l2 = l3 = 0.0f;
+        // the four lane products paired across lanes first (l with l + 32,
+        // then with l + 16: 28 factors per lane, one slot per row), one
+        // logarithm per lane, the row scan; a product outside [1e-30, 1e30]
+        // in any lane sends the wavefront to one logarithm per factor, summed
+        // in the same tree
+        const auto q01 = __builtin_amdgcn_permlane32_swap(as_u(p0), as_u(p1), false, false);
+        const auto q23 = __builtin_amdgcn_permlane32_swap(as_u(p2), as_u(p3), false, false);
+        const float m01 = as_f(q01[0]) * as_f(q01[1]);
+        const float m23 = as_f(q23[0]) * as_f(q23[1]);
+        const auto qq = __builtin_amdgcn_permlane16_swap(as_u(m01), as_u(m23), false, false);
+        const float pr = as_f(qq[0]) * as_f(qq[1]);
+        float v;
+        if (__ballot(!(pr > 1e-30f && pr < 1e30f)) == 0ull) {
+            v = __builtin_amdgcn_logf(pr);
+        } else {   // rare: factors far from 1
+            float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f, l3 = 0.0f;
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 l0 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b0 + vb[m]], S.q[m]));
@@ -607,15 +615,14 @@ __device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const 
                 l2 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b2 + vb[m]], S.q[m]));
                 l3 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b3 + vb[m]], S.q[m]));
             }
+            const auto r01 = __builtin_amdgcn_permlane32_swap(as_u(l0), as_u(l1), false, false);
+            const auto r23 = __builtin_amdgcn_permlane32_swap(as_u(l2), as_u(l3), false, false);
+            const float s01 = as_f(r01[0]) + as_f(r01[1]);
+            const float s23 = as_f(r23[0]) + as_f(r23[1]);
+            const auto r = __builtin_amdgcn_permlane16_swap(as_u(s01), as_u(s23), false, false);
+            v = as_f(r[0]) + as_f(r[1]);
         }
-        // lanes 0-31: slot t (l + 32 added), 32-63: slot t + 1; likewise t + 2, t + 3
-        const auto r01 = __builtin_amdgcn_permlane32_swap(as_u(l0), as_u(l1), false, false);
-        const auto r23 = __builtin_amdgcn_permlane32_swap(as_u(l2), as_u(l3), false, false);
-        const float s01 = as_f(r01[0]) + as_f(r01[1]);
-        const float s23 = as_f(r23[0]) + as_f(r23[1]);
-        // rows 0..3: slots t, t + 2, t + 1, t + 3 (l + 16 added)
-        const auto r = __builtin_amdgcn_permlane16_swap(as_u(s01), as_u(s23), false, false);
-        float v = as_f(r[0]) + as_f(r[1]);
+        // rows 0..3: slots t, t + 2, t + 1, t + 3
         v += dpp_get<0x111, 0xf>(v);
         v += dpp_get<0x112, 0xf>(v);
         v += dpp_get<0x114, 0xf>(v);

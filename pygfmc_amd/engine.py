@@ -625,6 +625,8 @@ def run_gfmc(walkers, n_steps, hamiltonian, step_size, guide, p_array, *,
         "parameters/has_guide": int(guide is not None),
         "parameters/precision": "fp64" if p.precision == _lib.GFMC_FP64 else "fp32",
         "parameters/lanes_per_walker": p.lanes,
+        "parameters/waves_per_workgroup": p.launch[0],
+        "parameters/xcd_contiguous_walkers": bool(p.launch[1]),
         "parameters/rng_key": np.asarray([key.seed, key.position], dtype=np.uint64),
         "parameters/rng_key_next": np.asarray([key_next.seed, key_next.position],
                                               dtype=np.uint64),

@@ -210,6 +210,8 @@ class Problem:
         self.guide_kind = int(info.guide)
         self._mlp_buf = None
         self._mlp_cap = 0
+        self.launch = (1, 0)
+        self._wpb_ok = None
 
     def reserve(self, n_walkers):
         """Attach (or grow) the MLP guide's propagation workspace, a device
@@ -275,6 +277,26 @@ class Problem:
                    "gfmc_problem_set_lanes")
         _lib.check(_lib.load().gfmc_problem_get_info(self.handle, ctypes.byref(self.info)),
                    "gfmc_problem_get_info")
+
+    @property
+    def wpb_ok(self):
+        """Usable wavefronts per workgroup of the propagation kernel: 1, 2,
+        4 on the record paths, 1 elsewhere (the library's answer, probed
+        once)."""
+        if self._wpb_ok is None:
+            lib = _lib.load()
+            ok = lib.gfmc_problem_set_launch(self.handle, 2, self.launch[1]) == 0
+            lib.gfmc_problem_set_launch(self.handle, self.launch[0], self.launch[1])
+            self._wpb_ok = (1, 2, 4) if ok else (1,)
+        return self._wpb_ok
+
+    def set_launch(self, waves_per_block, xcd_remap):
+        """Launch shape of the propagation kernel (gfmc_problem_set_launch):
+        1, 2 or 4 wavefronts per workgroup, XCD-contiguous walker ranges."""
+        _lib.check(_lib.load().gfmc_problem_set_launch(self.handle, int(waves_per_block),
+                                                       int(bool(xcd_remap))),
+                   "gfmc_problem_set_launch")
+        self.launch = (int(waves_per_block), int(bool(xcd_remap)))
 
     # -- calls ---------------------------------------------------------------
     def rates(self, spins, want_rates=True, want_eloc=True, want_logdiff=False):

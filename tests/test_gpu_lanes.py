@@ -159,6 +159,47 @@ def test_autotuner_picks_a_usable_lane_count_and_run_reports_it():
     w = prepare_walkers_random(4096, 16, 1)
     best, times = tune_problem(p, w.spins, 0.2, 0.0)
     say("  times", times, "->", best)
-    assert best in p.lanes_ok and p.lanes == best
+    assert best[0] in p.lanes_ok and p.lanes == best[0]
+    assert best[1] in p.wpb_ok and p.launch == tuple(best[1:])
+    assert len(times) == len(p.lanes_ok) * len(p.wpb_ok) * 2
     _, E, _, st, _ = run(w, 200, H, 0.2, G, [4, 16], rng_key=3, warmup_steps=40, verbosity=0)
     assert st["parameters/lanes_per_walker"] in (64, 32, 16)
+
+
+@pytest.mark.parametrize("lanes", [64, 32, 16])
+def test_launch_shape_does_not_change_results(lanes):
+    """Wavefronts per workgroup and the XCD-contiguous walker mapping change
+    only where walkers run: carried production steps on the 32 x 32 XXZ
+    workload (10 KiB of LDS per walker, so four 32- or 16-lane wavefronts
+    exceed 64 KiB per workgroup and the launch takes fewer) are bitwise
+    those of one wavefront per workgroup, and every walker is propagated."""
+    say("launch shapes, lanes", lanes)
+    L = square(32, 32)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 2.0, dtype=np.float32)
+    G = JastrowSpinHalf(1024, L.edges, beta=0.3, dtype=np.float32)
+    p = Problem(H, G, _lib.GFMC_FP32)
+    if lanes not in p.lanes_ok:
+        pytest.skip(f"{lanes} lanes per walker not usable here")
+    p.set_lanes(lanes)
+    assert p.wpb_ok == (1, 2, 4)
+    from oracle import init_sz
+    nw = 1000      # not a multiple of the walkers per workgroup
+    spins = init_sz(1024, nw, 512, 31)
+    res = {}
+    for wpb in (1, 2, 4):
+        for xcd in (0, 1):
+            p.set_launch(wpb, xcd)
+            z = torch.zeros(nw, dtype=torch.float64, device="cuda")
+            ch = Chain(p, Walker.from_device(to_dev(spins), z, z.clone(), 1024))
+            out = torch.empty((3, 2), dtype=torch.float64, device="cuda")
+            ch.steps(3, 0.2, -700.0, 41, 0, series=out, evals=True)
+            torch.cuda.synchronize()
+            ev = ch.evals.cpu().numpy()
+            assert ev.min() > 0, (wpb, xcd)
+            res[(wpb, xcd)] = (ch.spins.cpu().numpy(), ch.log_w.cpu().numpy(),
+                               ch.e_loc.cpu().numpy(), ev, out.cpu().numpy())
+    p.set_launch(1, 0)
+    ref = res[(1, 0)]
+    for k, v in res.items():
+        for a, b in zip(v, ref):
+            np.testing.assert_array_equal(a, b, err_msg=f"launch {k}")
