@@ -675,7 +675,19 @@ __device__ __forceinline__ T lane_scan(const DevProblem &P, const WalkerLds<T> &
                                        T *lane_sum) {
     const int K = G == WAVE ? P.K : P.Kg[gidx<G>()];
     T acc = (T)0;
-    for (int k = 0; k < K; ++k) acc += L.w[k * G + gl];
+    if (std::is_same<T, float>::value && K % 32 == 0) {
+        // FP32: 32 slots per chunk, every LDS read in flight before the
+        // first add (the same sequential sum)
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            T v[32];
+#pragma unroll
+            for (int k = 0; k < 32; ++k) v[k] = L.w[(k0 + k) * G + gl];
+#pragma unroll
+            for (int k = 0; k < 32; ++k) acc += v[k];
+        }
+    } else {
+        for (int k = 0; k < K; ++k) acc += L.w[k * G + gl];
+    }
     *lane_sum = acc;
     return grp_incl_scan<G>(acc);
 }
