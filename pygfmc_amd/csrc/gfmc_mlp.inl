@@ -510,6 +510,155 @@ void k_mlp_gemm(DevProblem P, const uint32_t *spins, MlpWs W, MlpTws T) {
     }
 }
 
+// The propagation's network rows in 32-row tiles: wavefront w of tile t
+// takes the 16-row blocks 2 (t GW2 + w) and 2 (t GW2 + w) + 1 on
+// v_mfma_f32_32x32x16_bf16 (lane l holds A[l % 32][8 (l / 32) + q] and
+// B[8 (l / 32) + q][l % 32]; D register j is row 8 (j / 4) + 4 (l / 32) +
+// j % 4, column l % 32), so every B fragment read from LDS feeds 32 rows
+// (half the LDS traffic per row of k_mlp_gemm); a K step's B fragments are
+// all in flight while its activations are built.  W2 is staged as in
+// k_mlp_gemm (double-buffered 32-wide K chunks, one barrier per chunk).
+// Epilogue: per lane the sum over the column tiles t (column t 32 + l % 32)
+// of w3 s(u + b2), then a butterfly over the 32 lanes of a row set: every
+// row's value is the same sequence of operations wherever the row sits, so
+// log psi(x') as a candidate row equals log psi(x') as a configuration row.
+#define MLP_GW2 8
+typedef __attribute__((ext_vector_type(16))) float mlp_f32x16;
+
+template <int NT>
+__global__ __launch_bounds__(MLP_GW2 * WAVE) __attribute__((amdgpu_waves_per_eu(2)))
+void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float2 *tab = (float2 *)lds;
+    constexpr int H2 = NT * 32;
+    constexpr int CH = H2 * (MLP_KC / 8);                 // uint4 per chunk
+    constexpr int TPB = MLP_GW2 * WAVE;
+    constexpr int PER = (CH + TPB - 1) / TPB;
+    static_assert(PER <= 4, "W2 chunk prefetch holds four uint4 per thread");
+    uint16_t *Bs0 = (uint16_t *)(lds + MLP_NT * sizeof(float2));
+    uint16_t *Bs1 = Bs0 + H2 * MLP_BSTR;
+    for (int i = threadIdx.x; i < MLP_NT; i += blockDim.x) tab[i] = P.mlp_tab[i];
+    const int total = W.total[0];
+    const int pairs = (total + 1) >> 1;
+    const int tiles = (pairs + MLP_GW2 - 1) / MLP_GW2;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int H1 = P.mlp_h1, nchunks = H1 / MLP_KC;
+    const int kh = lane >> 5, r32 = lane & 31;
+    const int jq = threadIdx.x / (MLP_KC / 8), qq = threadIdx.x % (MLP_KC / 8);
+    const int jstep = TPB / (MLP_KC / 8);
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int pb = tile * MLP_GW2 + wave;
+        const int b = 2 * pb + (r32 >> 4);          // the block of this lane's A row
+        const bool bvalid = b < total;
+        int64_t w = 0;
+        int r0 = 0, cnt = 0;
+        if (bvalid) {
+            w = W.blk_walker[b];
+            r0 = MLP_RB * (b - W.blk_off[w]);
+            cnt = W.cnt[w];
+        }
+        const int row = r0 + (r32 & 15);
+        const bool rvalid = bvalid && row < cnt;
+        int si = -1, sj = -1;
+        double ci = 0.0, cj = 0.0;
+        if (rvalid && row > 0) {
+            slot_sites(P, W.cand[(size_t)w * P.n_off + row - 1], &si, &sj);
+            const uint32_t *sp = spins + w * P.words;
+            ci = ((sp[si >> 5] >> (si & 31)) & 1u) ? -2.0 : 2.0;
+            if (sj >= 0) cj = ((sp[sj >> 5] >> (sj & 31)) & 1u) ? -2.0 : 2.0;
+        }
+        const double *z1 = W.z1 + (size_t)w * H1;
+        const uint16_t *w1i = P.mlp_W1T + (size_t)(si < 0 ? 0 : si) * H1;
+        const uint16_t *w1j = P.mlp_W1T + (size_t)(sj < 0 ? 0 : sj) * H1;
+        mlp_f32x16 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int idx = threadIdx.x + u * TPB;
+            if (idx < CH) {
+                const int j = idx / (MLP_KC / 8), q = idx % (MLP_KC / 8);
+                *(uint4 *)(Bs0 + j * MLP_BSTR + q * 8) =
+                    *(const uint4 *)(P.mlp_W2 + (size_t)j * H1 + q * 8);
+            }
+        }
+        MlpAIn ain;
+        mlp_a_load(ain, z1, w1i, w1j, 8 * kh, rvalid);
+        __syncthreads();
+        uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
+        for (int c = 0; c < nchunks; ++c) {
+            const uint16_t *Bc = (c & 1) ? Bs1 : Bs0;
+            uint16_t *Bn = (c & 1) ? Bs0 : Bs1;
+            const bool more = c + 1 < nchunks;
+            if (more) {
+                const uint16_t *src = P.mlp_W2 + (size_t)jq * H1 + (c + 1) * MLP_KC + qq * 8;
+                if ((int)threadIdx.x < CH) pre0 = *(const uint4 *)src;
+                if (PER > 1 && (int)threadIdx.x + TPB < CH)
+                    pre1 = *(const uint4 *)(src + (size_t)jstep * H1);
+                if (PER > 2 && (int)threadIdx.x + 2 * TPB < CH)
+                    pre2 = *(const uint4 *)(src + (size_t)2 * jstep * H1);
+                if (PER > 3 && (int)threadIdx.x + 3 * TPB < CH)
+                    pre3 = *(const uint4 *)(src + (size_t)3 * jstep * H1);
+            }
+#pragma unroll
+            for (int ks = 0; ks < MLP_KC; ks += 16) {
+                // every B fragment of the step in flight while the activations
+                // are built, then the MFMAs back to back
+                mlp_bf16x8 bv[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    bv[t] = *(const mlp_bf16x8 *)(Bc + (t * 32 + r32) * MLP_BSTR + ks + 8 * kh);
+                const mlp_bf16x8 a = mlp_a_build(ain, ci, cj, tab);
+                const int knext = c * MLP_KC + ks + 16 + 8 * kh;
+                if (knext < H1) mlp_a_load(ain, z1, w1i, w1j, knext, rvalid);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv[t], acc[t], 0, 0, 0);
+            }
+            if (more) {
+                uint16_t *dst = Bn + jq * MLP_BSTR + qq * 8;
+                if ((int)threadIdx.x < CH) *(uint4 *)dst = pre0;
+                if (PER > 1 && (int)threadIdx.x + TPB < CH) *(uint4 *)(dst + jstep * MLP_BSTR) = pre1;
+                if (PER > 2 && (int)threadIdx.x + 2 * TPB < CH)
+                    *(uint4 *)(dst + 2 * jstep * MLP_BSTR) = pre2;
+                if (PER > 3 && (int)threadIdx.x + 3 * TPB < CH)
+                    *(uint4 *)(dst + 3 * jstep * MLP_BSTR) = pre3;
+            }
+            __syncthreads();
+        }
+        // epilogue: rows 8 (j / 4) + 4 kh + j % 4 of the tile
+        float part[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) part[r] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int col = t * 32 + r32;
+            const float bb = P.mlp_b2[col], ww = P.mlp_w3[col];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) part[r] = part[r] + ww * mlp_tanh(tab, acc[t][r] + bb);
+        }
+#pragma unroll
+        for (int d = 16; d >= 1; d >>= 1)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) part[r] = part[r] + __shfl_xor(part[r], d, WAVE);
+        // the two blocks' row offsets and counts (held by lanes 0 and 16)
+        const int r0a = __shfl(r0, 0, WAVE), cnta = __shfl(cnt, 0, WAVE);
+        const int r0b = __shfl(r0, 16, WAVE), cntb = __shfl(cnt, 16, WAVE);
+        if (r32 == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int tr = 8 * (r >> 2) + 4 * kh + (r & 3);   // row of the tile
+                const int hb = tr >> 4, i = tr & 15;
+                const int bb = 2 * pb + hb;
+                if (bb < total && (hb ? r0b : r0a) + i < (hb ? cntb : cnta))
+                    W.logpsi[(size_t)bb * MLP_RB + i] = part[r] + P.mlp_b3p[0];
+            }
+        }
+    }
+}
+
 // one hop of every active walker (the loop body of the continuous-time
 // kernels) from the row values of this iteration
 __global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_select(
@@ -715,9 +864,21 @@ static unsigned mlp_walker_grid(const gfmc_problem *p, int64_t nw) {
 template <bool TRAIN>
 static unsigned launch_mlp_gemm(const gfmc_problem *p, const uint32_t *spins, hipStream_t st,
                                 const MlpTws &T = MlpTws{}) {
-    const unsigned grid = (unsigned)(p->n_cu * 4);
     const size_t lds = mlp_gemm_lds(p);
     const MlpWs &W = p->mlp->ws;
+    if (!TRAIN) {
+        // the propagation's and evaluations' rows: 32-row tiles
+        const unsigned grid = (unsigned)(p->n_cu * 2);
+        const dim3 g(grid), b(MLP_GW2 * WAVE);
+        switch (p->dp.mlp_h2) {
+        case 32: hipLaunchKernelGGL(k_mlp_gemm32<1>, g, b, lds, st, p->dp, spins, W); break;
+        case 64: hipLaunchKernelGGL(k_mlp_gemm32<2>, g, b, lds, st, p->dp, spins, W); break;
+        case 128: hipLaunchKernelGGL(k_mlp_gemm32<4>, g, b, lds, st, p->dp, spins, W); break;
+        default: hipLaunchKernelGGL(k_mlp_gemm32<8>, g, b, lds, st, p->dp, spins, W); break;
+        }
+        return grid;
+    }
+    const unsigned grid = (unsigned)(p->n_cu * 4);
     const dim3 g(grid), b(MLP_GW * WAVE);
     switch (p->dp.mlp_h2) {
     case 32: hipLaunchKernelGGL((k_mlp_gemm<2, TRAIN>), g, b, lds, st, p->dp, spins, W, T); break;
