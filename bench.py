@@ -193,6 +193,8 @@ def main():
         print(json.dumps({"kagome12_mlp": kagome_mlp_section(args)}), flush=True)
         return
     if args.only_race:
+        if args.calib_copy_mib > 0:
+            calib_copy(args.calib_copy_mib)
         print(json.dumps({"sigma_race": sigma_race(args)}), flush=True)
         return
     from pygfmc_amd import JastrowSpinHalf, SpinHalf, prepare_walkers_sz, square
@@ -590,18 +592,18 @@ def kagome_mlp_section(args):
     if not np.all(np.isfinite(ser[:, 0].cpu().numpy())):
         raise SystemExit("population extinct in the MLP section")
     evals_ws = evals / (nw * args.mlp_steps)
-    prof = profiled_traffic(f"k_mlp_gemm_kagome12x12_mlp512x256_w{nw}", "traffic_bytes_per_step")
-    roof = {"bound": "mfma", "compute_unit": "BF16 MFMA 16x16x32, FP32 accumulation (layer 2)",
+    prof = profiled_traffic(f"k_mlp_gemm32_kagome12x12_mlp512x256_w{nw}", "traffic_bytes_per_step")
+    roof = {"bound": "mfma", "compute_unit": "BF16 MFMA 32x32x16, FP32 accumulation (layer 2)",
             "achieved": flop / args.mlp_steps / prop_s / 1e12, "peak": 2500.0,
             "unit": "TFLOP/s", "frac": flop / args.mlp_steps / prop_s / 2.5e15,
             "work": "layer-2 FLOPs 2 * 512 * 256 per network row over the 16-row blocks the "
                     "propagation processed (padding rows included)",
-            "kernel": "k_mlp_gemm<16, false> (the propagation of one branching step, timed "
+            "kernel": "k_mlp_gemm32<8> (the propagation of one branching step, timed "
                       "with HIP events around it, includes its row selection and scans)",
             "kernel_avg_ms": prop_s * 1e3,
             "traffic": prof["traffic_bytes_per_launch"] if prof else None,
             "traffic_unit": "HBM bytes per branching step (rocprofv3 --pmc FETCH_SIZE + "
-                            "WRITE_SIZE over the step's k_mlp_gemm launches, copy-calibrated)",
+                            "WRITE_SIZE over the step's k_mlp_gemm32 launches, copy-calibrated)",
             "traffic_source": prof["source"] if prof else None}
     cpu = None
     if not args.no_cpu:
