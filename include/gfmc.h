@@ -468,14 +468,17 @@ int gfmc_orbit_axpy(int64_t n, float *v, int32_t n_orbits, const int32_t *offset
  * gradients on (S + eps_rel diag S + eps_abs I) delta = g (at most max_iter
  * iterations, until |r| <= tol |g|), scaled so that sqrt(delta^T S delta),
  * the rms change of log psi over the samples, is at most max_norm (0: no
- * limit); work: gfmc_sr_work_doubles(m) doubles. */
+ * limit); work: gfmc_sr_work_doubles(m) doubles.  warm_start 1 starts the
+ * iterations from the unscaled solution a previous call left in the same
+ * work buffer (successive natural-gradient steps change S and g little),
+ * 0 from delta = 0. */
 int gfmc_rbm_orbit_O(const gfmc_problem *p, int64_t n, const uint32_t *spins, int32_t n_orbits,
                      const int32_t *offsets, const int32_t *members, float *O, float *theta,
                      void *stream);
 int64_t gfmc_sr_work_doubles(int32_t m);
 int gfmc_sr_solve(int64_t n, int32_t m, const float *O, const float *g, double eps_rel,
-                  double eps_abs, int32_t max_iter, double tol, double max_norm, float *delta,
-                  double *work, void *stream);
+                  double eps_abs, int32_t max_iter, double tol, double max_norm,
+                  int32_t warm_start, float *delta, double *work, void *stream);
 
 int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *v,
                    int64_t step, float lr, float beta1, float beta2, float eps,

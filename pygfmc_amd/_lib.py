@@ -171,7 +171,7 @@ SIGNATURES = [
     ("gfmc_sr_work_doubles", ctypes.c_int64, [ctypes.c_int32]),
     ("gfmc_sr_solve", ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _vp, _vp, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_int32, ctypes.c_double,
-                                     ctypes.c_double, _vp, _vp, _vp]),
+                                     ctypes.c_double, ctypes.c_int32, _vp, _vp, _vp]),
     ("gfmc_adam_step", ctypes.c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int64,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, _vp]),

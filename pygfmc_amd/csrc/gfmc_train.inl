@@ -30,7 +30,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 #define BF16_ONE ((short)0x3F80)
 #define BF16_MINUS_ONE ((short)0xBF80)
-#define VM_SPW 64     // samples per wavefront in k_vmin_moves
+#define VM_SPW 16     // samples per wavefront in k_vmin_moves
 #define BX_KS 256     // samples per split in k_rbm_bx_mfma
 
 __device__ __forceinline__ uint32_t bf16_rne_bits(float v) {
@@ -501,16 +501,26 @@ __device__ void sr_apply(int m, const double *S, double eps_rel, double eps_abs,
     __syncthreads();
 }
 
+// warm: start from the x the workspace holds (the previous call's unscaled
+// solution) with r = g - A x, else from x = 0
 __global__ __launch_bounds__(1024) void k_sr_cg(int m, const double *S, double eps_rel,
                                                double eps_abs, const float *g, int max_iter,
-                                               double tol, double max_norm, double *x,
+                                               double tol, double max_norm, int warm, double *x,
                                                double *r, double *pv, double *Ap,
                                                float *delta) {
     __shared__ double sh[16];
-    for (int q = threadIdx.x; q < m; q += blockDim.x) {
-        x[q] = 0.0;
-        r[q] = (double)g[q];
-        pv[q] = r[q];
+    if (warm) {
+        sr_apply(m, S, eps_rel, eps_abs, x, Ap);
+        for (int q = threadIdx.x; q < m; q += blockDim.x) {
+            r[q] = (double)g[q] - Ap[q];
+            pv[q] = r[q];
+        }
+    } else {
+        for (int q = threadIdx.x; q < m; q += blockDim.x) {
+            x[q] = 0.0;
+            r[q] = (double)g[q];
+            pv[q] = r[q];
+        }
     }
     __syncthreads();
     double rr = 0.0;
@@ -691,8 +701,8 @@ int64_t gfmc_sr_work_doubles(int32_t m) {
 }
 
 int gfmc_sr_solve(int64_t n, int32_t m, const float *O, const float *g, double eps_rel,
-                  double eps_abs, int32_t max_iter, double tol, double max_norm, float *delta,
-                  double *work, void *stream) {
+                  double eps_abs, int32_t max_iter, double tol, double max_norm,
+                  int32_t warm_start, float *delta, double *work, void *stream) {
     if (n < 1 || m < 1 || m > 8192 || max_iter < 0)
         return fail(GFMC_EINVAL, "gfmc_sr_solve: bad size");
     if (!O || !g || !delta || !work) return fail(GFMC_EINVAL, "gfmc_sr_solve: NULL buffer");
@@ -708,7 +718,8 @@ int gfmc_sr_solve(int64_t n, int32_t m, const float *O, const float *g, double e
     const unsigned tiles = (unsigned)((m + SR_T - 1) / SR_T);
     hipLaunchKernelGGL(k_sr_cov, dim3(tiles, tiles), dim3(SR_T * SR_T), 0, st, n, m, O, mean, S);
     hipLaunchKernelGGL(k_sr_cg, dim3(1), dim3(1024), 0, st, m, (const double *)S, eps_rel,
-                       eps_abs, g, max_iter, tol, max_norm, x, r, pv, Ap, delta);
+                       eps_abs, g, max_iter, tol, max_norm, warm_start ? 1 : 0, x, r, pv, Ap,
+                       delta);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -96,6 +96,7 @@ class RBMTrainer:
                                         device=dev)
             self._O = None
             self._theta = None
+            self._sr_warm = False
         self._last_spins = None
 
     def tie(self, v):
@@ -144,11 +145,13 @@ class RBMTrainer:
         _lib.check(lib.gfmc_rbm_orbit_O(self.p.handle, n, _lib.ptr(spins), m, _lib.ptr(off),
                                         _lib.ptr(mem), _lib.ptr(self._O), _lib.ptr(self._theta),
                                         _lib.stream()), "gfmc_rbm_orbit_O")
+        # successive steps warm-start from the previous step's solution
         _lib.check(lib.gfmc_sr_solve(n, m, _lib.ptr(self._O), _lib.ptr(self._g_t),
                                      self.sr_shift, self.sr_abs, self.sr_iter, 1e-5,
-                                     self.sr_max_norm, _lib.ptr(self._delta),
-                                     _lib.ptr(self._sr_work),
+                                     self.sr_max_norm, int(self._sr_warm),
+                                     _lib.ptr(self._delta), _lib.ptr(self._sr_work),
                                      _lib.stream()), "gfmc_sr_solve")
+        self._sr_warm = True
         return self._delta
 
     def update(self):

@@ -247,7 +247,7 @@ def test_tied_rbm_natural_gradient_pieces_match_numpy(orc):
     g = torch.as_tensor(rng.normal(size=m).astype(np.float32), device="cuda")
     delta = torch.empty(m, dtype=torch.float32, device="cuda")
     work = torch.empty(int(L_.gfmc_sr_work_doubles(m)), dtype=torch.float64, device="cuda")
-    _lib.check(L_.gfmc_sr_solve(n, m, _lib.ptr(O), _lib.ptr(g), 1e-2, 1e-3, 500, 1e-10, 0.0,
+    _lib.check(L_.gfmc_sr_solve(n, m, _lib.ptr(O), _lib.ptr(g), 1e-2, 1e-3, 500, 1e-10, 0.0, 0,
                                 _lib.ptr(delta), _lib.ptr(work), _lib.stream()), "sr_solve")
     Oh = O.cpu().numpy().astype(np.float64)
     Oc = Oh - Oh.mean(axis=0)
@@ -257,8 +257,17 @@ def test_tied_rbm_natural_gradient_pieces_match_numpy(orc):
     np.testing.assert_allclose(delta.cpu().numpy(), d_ref, rtol=1e-4,
                                atol=1e-4 * np.abs(d_ref).max())
     # the trust region scales the same direction to sqrt(d^T S d) = max_norm
-    _lib.check(L_.gfmc_sr_solve(n, m, _lib.ptr(O), _lib.ptr(g), 1e-2, 1e-3, 500, 1e-10, 0.01,
+    # (warm-started from the first call's solution)
+    _lib.check(L_.gfmc_sr_solve(n, m, _lib.ptr(O), _lib.ptr(g), 1e-2, 1e-3, 500, 1e-10, 0.01, 1,
                                 _lib.ptr(delta), _lib.ptr(work), _lib.stream()), "sr_solve")
     d2 = delta.cpu().numpy().astype(np.float64)
     assert np.sqrt(d2 @ S @ d2) == pytest.approx(min(0.01, np.sqrt(d_ref @ S @ d_ref)), rel=1e-3)
     np.testing.assert_allclose(d2 / np.linalg.norm(d2), d_ref / np.linalg.norm(d_ref), atol=1e-3)
+    # a warm start from that solution for another right-hand side converges
+    # to the other solution
+    g2 = torch.as_tensor(rng.normal(size=m).astype(np.float32), device="cuda")
+    _lib.check(L_.gfmc_sr_solve(n, m, _lib.ptr(O), _lib.ptr(g2), 1e-2, 1e-3, 500, 1e-10, 0.0, 1,
+                                _lib.ptr(delta), _lib.ptr(work), _lib.stream()), "sr_solve")
+    d_ref2 = np.linalg.solve(A, g2.cpu().numpy().astype(np.float64))
+    np.testing.assert_allclose(delta.cpu().numpy(), d_ref2, rtol=1e-4,
+                               atol=1e-4 * np.abs(d_ref2).max())
