@@ -403,21 +403,29 @@ int gfmc_rbm_set_params(gfmc_problem *p, const float *flat, void *stream);
  * h = f n_T + r sees W[h][k] = w_f(k - r)).  The factor of unit h in a pair
  * slot's ratio then comes from a table of (slot class, x_i, f, site i - r)
  * instead of an exponential per factor; moves, weights and energies are
- * those of the untied kernels up to FP32 rounding (the lane products are
- * the same; they are combined across lanes in another order).
- * Tables (int32, built by pygfmc_amd.torus.tied_tables): unit_off[Hp]
- * (Hp = hidden rounded up to 64) the offset f Q + O - code(r) of every
- * hidden unit (padded units F Q + O), slot_code[n_off] code(i) | class <<
- * 20, site_code[n_sites], q_site[Q] the site of entry q, q_pair[n_cls Q]
- * the partner site of entry q in class c (-1: none).  Every table read the
- * kernel can make is checked against the table's bounds here.  The problem
- * must have pair moves only with J_pmpm = 0 and at most 512 hidden units;
- * otherwise GFMC_EUNSUP and nothing changes.  gfmc_rbm_set_params rebuilds
- * the tables from the rows r = 0: parameters set later must stay tied (the
- * orbit-tied trainers keep them so), or call gfmc_rbm_untie first, which
- * returns the problem to the untied kernels. */
+ * those of the untied kernels up to FP32 rounding (the factors are the
+ * same; they are multiplied and combined across lanes in another order).
+ * Hidden units come in pairs (f, r), (f, r + d) whose table entries lie
+ * pair_code apart, so one 8-byte read gives both factors of a pair.
+ * Tables (int32, built by pygfmc_amd.torus.tied_tables): lane_off[(n_pair_
+ * chunks + n_single_chunks) 64] the table offset f Q + O - code(r) of each
+ * lane chunk's (first) unit (padding F Q + O), lane_unit[(2 n_pair_chunks +
+ * n_single_chunks) 64] the hidden unit of each lane register (-1: padding;
+ * every unit exactly once), slot_code[n_off] code(first site) | class << 20
+ * | swapped << 30 (a slot read from its other end), site_code[n_sites],
+ * q_site[Q] the site of entry q, q_pair[n_cls Q] the partner site of entry q
+ * in class c (-1: none).  Every table read the kernel can make is checked
+ * against the table's bounds here.  The problem must have pair moves only
+ * with J_pmpm = 0 and at most 512 hidden units, and the chunk shape one the
+ * kernel is built for (at most 4 pair and 2 single chunks, 2 pair + single
+ * <= 8); otherwise GFMC_EUNSUP and nothing changes.  gfmc_rbm_set_params
+ * rebuilds the tables from the rows r = 0: parameters set later must stay
+ * tied (the orbit-tied trainers keep them so), or call gfmc_rbm_untie first,
+ * which returns the problem to the untied kernels. */
 int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q,
-                      int32_t n_cls, const int32_t *unit_off, const int32_t *slot_code,
+                      int32_t n_cls, int32_t pair_code, int32_t n_pair_chunks,
+                      int32_t n_single_chunks, const int32_t *lane_off,
+                      const int32_t *lane_unit, const int32_t *slot_code,
                       const int32_t *site_code, const int32_t *q_site, const int32_t *q_pair,
                       void *stream);
 int gfmc_rbm_untie(gfmc_problem *p);

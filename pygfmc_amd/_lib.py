@@ -156,7 +156,8 @@ SIGNATURES = [
     ("gfmc_rbm_n_params", ctypes.c_int64, [_vp]),
     ("gfmc_rbm_set_params", ctypes.c_int, [_vp, _vp, _vp]),
     ("gfmc_rbm_set_tied", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                          ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]),
+                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("gfmc_rbm_untie", ctypes.c_int, [_vp]),
     ("gfmc_rbm_theta", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
     ("gfmc_rbm_vmin_work_bytes", ctypes.c_int64, [_vp, ctypes.c_int64]),

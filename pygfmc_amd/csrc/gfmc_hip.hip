@@ -151,13 +151,18 @@ struct DevProblem {
     int rbh, Hp, wt_bytes, rbh_wave_bytes;
     const uint16_t *rbm_WT;
     int *rbm_queue;
-    // translation-tied RBM on a torus (gfmc_rbm_set_tied; gfmc_rbm.inl): rbt =
-    // M when on; factor tables of rbt_words floats ((2 rbt_ncls + 1) rbt_F
-    // blocks of rbt_Q, then rbt_Q zeros, rounded up to a multiple of 4)
-    int rbt, rbt_F, rbt_nT, rbt_Q, rbt_ncls, rbt_words;
+    // translation-tied RBM on a torus (gfmc_rbm_set_tied; gfmc_rbm.inl): rbt
+    // = 1 when on, with rbt_cp pair and rbt_cs single lane chunks (rbt_pk:
+    // one single chunk of at most 16 units, packed); raw
+    // tables rbt_raw ((2 rbt_ncls + 1) rbt_F blocks of rbt_Q, then rbt_Q
+    // zeros) and the kernel's table rbt_tab of rbt_words floats (entry idx =
+    // (raw[idx], raw[idx - rbt_D]), rounded up to a multiple of 4 floats)
+    int rbt, rbt_cp, rbt_cs, rbt_pk, rbt_D, rbt_F, rbt_nT, rbt_Q, rbt_ncls, rbt_words;
     const float *rbt_tab;
-    const int *rbt_unit;    // [Hp]     f Q + O - code(r), padded units F Q + O
-    const int *rbt_scode;   // [n_off]  code(i) | class << 20
+    float *rbt_raw;
+    const int *rbt_unit;    // [(cp + cs) 64] off(a) of each lane chunk, padding F Q + O
+    const int *rbt_hid;     // [(2 cp + cs) 64] hidden unit of each lane register, -1: pad
+    const int *rbt_scode;   // [n_off]  code(first) | class << 20 | swapped << 30
     const int *rbt_kcode;   // [N]      code(k)
     const int *rbt_qsite;   // [Q]      the site of entry q
     const int *rbt_qpair;   // [ncls Q] the partner site of entry q in class c, -1: none
@@ -2415,33 +2420,49 @@ static void rbh_rates_m(const gfmc_problem *p, int64_t nw, const uint32_t *spins
                        log_diff, log_psi);
 }
 
-// the tied-RBM kernel (gfmc_rbm_set_tied) for M hidden-unit chunks per lane
+// the tied-RBM kernel (gfmc_rbm_set_tied) for CP pair and CS single chunks
 static void rbt_build_tables(const gfmc_problem *p, hipStream_t st) {
-    const int64_t total = (int64_t)(2 * p->dp.rbt_ncls + 1) * p->dp.rbt_F * p->dp.rbt_Q +
-                          p->dp.rbt_Q;
-    hipLaunchKernelGGL(k_rbt_tables, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                       p->dp, (float *)p->dp.rbt_tab);
+    const DevProblem &P = p->dp;
+    const int64_t total = (int64_t)(2 * P.rbt_ncls + 1) * P.rbt_F * P.rbt_Q + P.rbt_Q;
+    const unsigned g = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(k_rbt_tables, dim3(g), dim3(256), 0, st, P, P.rbt_raw);
+    hipLaunchKernelGGL(k_rbt_pair, dim3(g), dim3(256), 0, st, total, P.rbt_D,
+                       (const float *)P.rbt_raw, (float2 *)P.rbt_tab);
 }
 
-template <int M>
+// the (CP, CS, PK) shapes the tied kernel is built for
+#define RBT_SWITCH(CP, CS, PK, FN, ...)                                \
+    [&]() -> int {                                                     \
+        switch ((CP) * 8 + (CS) * 2 + ((PK) ? 1 : 0)) {                \
+        case 2: return FN<0, 1, false>(__VA_ARGS__);                   \
+        case 3: return FN<0, 1, true>(__VA_ARGS__);                    \
+        case 4: return FN<0, 2, false>(__VA_ARGS__);                   \
+        case 8: return FN<1, 0, false>(__VA_ARGS__);                   \
+        case 10: return FN<1, 1, false>(__VA_ARGS__);                  \
+        case 11: return FN<1, 1, true>(__VA_ARGS__);                   \
+        case 12: return FN<1, 2, false>(__VA_ARGS__);                  \
+        case 16: return FN<2, 0, false>(__VA_ARGS__);                  \
+        case 18: return FN<2, 1, false>(__VA_ARGS__);                  \
+        case 19: return FN<2, 1, true>(__VA_ARGS__);                   \
+        case 20: return FN<2, 2, false>(__VA_ARGS__);                  \
+        case 24: return FN<3, 0, false>(__VA_ARGS__);                  \
+        case 26: return FN<3, 1, false>(__VA_ARGS__);                  \
+        case 27: return FN<3, 1, true>(__VA_ARGS__);                   \
+        case 28: return FN<3, 2, false>(__VA_ARGS__);                  \
+        case 32: return FN<4, 0, false>(__VA_ARGS__);                  \
+        }                                                              \
+        return fail(GFMC_EUNSUP, "RBM: no tied kernel for this chunk shape"); \
+    }()
+
+static bool rbt_shape_ok(int cp, int cs) {
+    return cp >= 0 && cs >= 0 && cp <= 4 && cs <= 2 && cp + cs >= 1 && 2 * cp + cs <= 8;
+}
+
+template <int CP, int CS, bool PK>
 static int rbt_set_lds_m(int bytes) {
-    HIP_TRY(hipFuncSetAttribute((const void *)k_propagate_rbt<M>,
+    HIP_TRY(hipFuncSetAttribute((const void *)k_propagate_rbt<CP, CS, PK>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     return 0;
-}
-
-static int rbt_set_lds(int M, int bytes) {
-    switch (M) {
-    case 1: return rbt_set_lds_m<1>(bytes);
-    case 2: return rbt_set_lds_m<2>(bytes);
-    case 3: return rbt_set_lds_m<3>(bytes);
-    case 4: return rbt_set_lds_m<4>(bytes);
-    case 5: return rbt_set_lds_m<5>(bytes);
-    case 6: return rbt_set_lds_m<6>(bytes);
-    case 7: return rbt_set_lds_m<7>(bytes);
-    case 8: return rbt_set_lds_m<8>(bytes);
-    }
-    return fail(GFMC_EUNSUP, "RBM: hidden layer too wide for the tied kernel");
 }
 
 // persistent grid: as many workgroups of rbt_waves walkers as fit
@@ -2452,14 +2473,16 @@ static unsigned rbt_grid(const gfmc_problem *p, int64_t nw) {
     return (unsigned)(need < cap ? need : cap);
 }
 
-template <int M>
-static void rbt_launch_m(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
-                         double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
-                         uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st) {
+template <int CP, int CS, bool PK>
+static int rbt_launch_m(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
+                        double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
+                        uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st) {
     hipLaunchKernelGGL(k_rbh_queue_reset, dim3(1), dim3(WAVE), 0, st, p->dp.rbm_queue);
-    hipLaunchKernelGGL(k_propagate_rbt<M>, dim3(rbt_grid(p, nw)), dim3(p->rbt_waves * WAVE),
-                       (size_t)p->rbt_lds_bytes, st, p->dp, p->t32, nw, spins, log_w, e_loc,
-                       beta, alw, k0, k1, step, tag, evals, g_step_base);
+    hipLaunchKernelGGL((k_propagate_rbt<CP, CS, PK>), dim3(rbt_grid(p, nw)),
+                       dim3(p->rbt_waves * WAVE), (size_t)p->rbt_lds_bytes, st, p->dp, p->t32,
+                       nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
+                       g_step_base);
+    return 0;
 }
 
 #define RBH_SWITCH(M, FN, ...)                      \
@@ -2908,7 +2931,9 @@ int gfmc_problem_destroy(gfmc_problem *p) {
 }
 
 int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q,
-                      int32_t n_cls, const int32_t *unit_off, const int32_t *slot_code,
+                      int32_t n_cls, int32_t pair_code, int32_t n_pair_chunks,
+                      int32_t n_single_chunks, const int32_t *lane_off,
+                      const int32_t *lane_unit, const int32_t *slot_code,
                       const int32_t *site_code, const int32_t *q_site, const int32_t *q_pair,
                       void *stream) {
     const char *fn = "gfmc_rbm_set_tied";
@@ -2924,16 +2949,36 @@ int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q
         return fail(GFMC_EUNSUP, std::string(fn) + ": the tied kernel takes pair moves of "
                                                    "anti-parallel spins only (no h_x, J_pmpm = 0)");
     if (n_filters <= 0 || n_T <= 0 || (int64_t)n_filters * n_T != P.H || Q <= 0 ||
-        Q >= (1 << 20) || n_cls <= 0 || n_cls > 2047)
+        Q >= (1 << 20) || n_cls <= 0 || n_cls >= (1 << 10))
         return fail(GFMC_EINVAL, std::string(fn) + ": bad sizes");
-    if (!unit_off || !slot_code || !site_code || !q_site || !q_pair)
+    if (!lane_off || !lane_unit || !slot_code || !site_code || !q_site || !q_pair)
         return fail(GFMC_EINVAL, std::string(fn) + ": NULL table");
-    const int N = P.N, Hp = P.Hp, n_off = P.n_off;
+    const int cp = n_pair_chunks, cs = n_single_chunks;
+    if (cp < 0 || cs < 0 || (int64_t)(2 * cp + cs) * WAVE < P.H)
+        return fail(GFMC_EINVAL, std::string(fn) + ": the lane chunks hold fewer units than H");
+    if (!rbt_shape_ok(cp, cs))
+        return fail(GFMC_EUNSUP, std::string(fn) + ": no tied kernel for this chunk shape");
+    const int N = P.N, n_off = P.n_off;
     const int64_t FQ = (int64_t)n_filters * Q;
     const int64_t raw = (2 * (int64_t)n_cls + 1) * FQ + Q;
-    const int64_t words = (raw + 3) / 4 * 4;
-    // every table read of the kernels stays inside the table: theta reads
-    // 2 n_cls FQ + code(k) + off(h), factor reads (2 cls + sg) FQ + code(i) + off(h)
+    const int64_t words = (2 * raw + 3) / 4 * 4;
+    if (pair_code < 0 || pair_code >= raw || 8 * raw >= INT32_MAX)
+        return fail(GFMC_EINVAL, std::string(fn) + ": bad pair code");
+    // every hidden unit sits in exactly one lane register
+    {
+        std::vector<int> seen(P.H, 0);
+        for (int64_t t = 0; t < (int64_t)(2 * cp + cs) * WAVE; ++t) {
+            const int h = lane_unit[t];
+            if (h < -1 || h >= P.H || (h >= 0 && seen[h]++))
+                return fail(GFMC_EINVAL, std::string(fn) + ": lane units are not a partition "
+                                                           "of the hidden units");
+        }
+        for (int h = 0; h < P.H; ++h)
+            if (!seen[h])
+                return fail(GFMC_EINVAL, std::string(fn) + ": a hidden unit has no lane");
+    }
+    // every table read of the kernel stays inside the table: theta reads
+    // 2 n_cls FQ + code(k) + off, factor reads (2 cls + sg) FQ + code(i) + off
     int64_t kmin = INT64_MAX, kmax = INT64_MIN, smin = INT64_MAX, smax = INT64_MIN;
     for (int k = 0; k < N; ++k) {
         if (site_code[k] < 0 || site_code[k] >= Q)
@@ -2942,14 +2987,14 @@ int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q
         kmax = std::max<int64_t>(kmax, site_code[k]);
     }
     for (int s = 0; s < n_off; ++s) {
-        const int code = slot_code[s] & 0xfffff, cls = slot_code[s] >> 20;
-        if (cls < 0 || cls >= n_cls)
+        const int code = slot_code[s] & 0xfffff, cls = (slot_code[s] >> 20) & 0x3ff;
+        if (slot_code[s] < 0 || (slot_code[s] >> 31) || cls >= n_cls)
             return fail(GFMC_EINVAL, std::string(fn) + ": slot class out of range");
         smin = std::min<int64_t>(smin, 2 * (int64_t)cls * FQ + code);
         smax = std::max<int64_t>(smax, (2 * (int64_t)cls + 1) * FQ + code);
     }
-    for (int h = 0; h < Hp; ++h) {
-        const int64_t v = unit_off[h];
+    for (int64_t t = 0; t < (int64_t)(cp + cs) * WAVE; ++t) {
+        const int64_t v = lane_off[t];
         if (2 * n_cls * FQ + kmin + v < 0 || 2 * n_cls * FQ + kmax + v >= raw || smin + v < 0 ||
             smax + v >= raw)
             return fail(GFMC_EINVAL, std::string(fn) + ": unit offset reads outside the tables");
@@ -2966,26 +3011,38 @@ int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q
         return fail(GFMC_EUNSUP, std::string(fn) + ": tables plus walker state exceed the LDS");
     const int64_t lds = words * 4 + (int64_t)waves * P.rbh_wave_bytes;
     int rc = 0;
-    const int *d_unit = nullptr, *d_scode = nullptr, *d_kcode = nullptr, *d_qsite = nullptr,
-              *d_qpair = nullptr;
-    if ((rc = upload(p, unit_off, (size_t)Hp, &d_unit)) ||
+    const int *d_unit = nullptr, *d_hid = nullptr, *d_scode = nullptr, *d_kcode = nullptr,
+              *d_qsite = nullptr, *d_qpair = nullptr;
+    if ((rc = upload(p, lane_off, (size_t)(cp + cs) * WAVE, &d_unit)) ||
+        (rc = upload(p, lane_unit, (size_t)(2 * cp + cs) * WAVE, &d_hid)) ||
         (rc = upload(p, slot_code, (size_t)n_off, &d_scode)) ||
         (rc = upload(p, site_code, (size_t)N, &d_kcode)) ||
         (rc = upload(p, q_site, (size_t)Q, &d_qsite)) ||
         (rc = upload(p, q_pair, (size_t)n_cls * Q, &d_qpair)))
         return rc;
-    void *tab = nullptr;
+    void *tab = nullptr, *rawp = nullptr;
     HIP_TRY(hipMalloc(&tab, (size_t)words * 4));
     p->allocs.push_back(tab);
     HIP_TRY(hipMemset(tab, 0, (size_t)words * 4));
-    if ((rc = rbt_set_lds(P.rbh, (int)lds))) return rc;
+    HIP_TRY(hipMalloc(&rawp, (size_t)raw * 4));
+    p->allocs.push_back(rawp);
+    // a single chunk of at most 16 units is packed four slots to a chunk
+    bool pk = cs == 1;
+    for (int l = 16; pk && l < WAVE; ++l) pk = lane_unit[(int64_t)2 * cp * WAVE + l] < 0;
+    if ((rc = RBT_SWITCH(cp, cs, pk, rbt_set_lds_m, (int)lds))) return rc;
+    P.rbt_pk = pk ? 1 : 0;
+    P.rbt_cp = cp;
+    P.rbt_cs = cs;
+    P.rbt_D = pair_code;
     P.rbt_F = n_filters;
     P.rbt_nT = n_T;
     P.rbt_Q = Q;
     P.rbt_ncls = n_cls;
     P.rbt_words = (int)words;
     P.rbt_tab = (const float *)tab;
+    P.rbt_raw = (float *)rawp;
     P.rbt_unit = d_unit;
+    P.rbt_hid = d_hid;
     P.rbt_scode = d_scode;
     P.rbt_kcode = d_kcode;
     P.rbt_qsite = d_qsite;
@@ -2994,7 +3051,7 @@ int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q
     p->rbt_waves = waves;
     rbt_build_tables(p, (hipStream_t)stream);
     HIP_TRY(hipGetLastError());
-    P.rbt = P.rbh;
+    P.rbt = 1;
     return 0;
 }
 
@@ -3190,8 +3247,8 @@ int gfmc_propagate(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *l
         return mlp_propagate(p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
                              (hipStream_t)stream);
     if (p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbt)
-        RBH_SWITCH(p->dp.rbt, rbt_launch_m, p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
-                   tag, evals, (hipStream_t)stream);
+        RBT_SWITCH(p->dp.rbt_cp, p->dp.rbt_cs, p->dp.rbt_pk, rbt_launch_m, p, nw, spins, log_w,
+                   e_loc, beta, alw, k0, k1, step, tag, evals, (hipStream_t)stream);
     else if (p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbh)
         RBH_SWITCH(p->dp.rbh, rbh_launch_m, p, nw, spins, log_w, e_loc, beta, alw, k0, k1, step,
                    tag, evals, (hipStream_t)stream);

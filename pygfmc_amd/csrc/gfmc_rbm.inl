@@ -441,19 +441,28 @@ __global__ __launch_bounds__(256) void k_rbm_pack_t(int N, int H, int Hp, const 
 // in the ratio of pair slot (i, j) with x_j = -x_i,
 //   p_h 2^{c_i W_hi + c_j W_hj} + q_h,  c_i = -4 log2(e) x_i,  c_j = -c_i,
 // depends on the walker only through p_h, q_h: the power is a function of
-// (slot class, x_i, f, the site i - r) and sits in a table,
-//   tab[((cls 2 + (x_i < 0)) F + f) Q + q],  q = code(i) + off(h) - f Q,
+// (slot class, x_i, f, the site i - r) and sits in a table T,
+//   T[((cls 2 + (x_i < 0)) F + f) Q + q],  q = code(i) + off(h) - f Q,
 // computed with the expression the rbh kernels evaluate per factor (the
-// same bits).  Block 2 n_cls holds w_f itself (theta updates), then Q
-// zeros that the padded hidden units read.  Per lane and chunk m the kernel
-// keeps off(h) in a register; a slot adds its block base and code(i) (one
-// integer add, one LDS read, one fma and one multiply per factor).  Slots are
-// evaluated four at a time and their lane products combined together:
-// v_permlane32_swap pairs lanes l and l + 32 of two slots at once,
-// v_permlane16_swap rows 0 / 1 and 2 / 3 of the two pairs (products, so each
-// lane then holds 4 M factors of one slot: one log2 per lane for four slots),
-// then a row scan (row_shr 1, 2, 4, 8) of the logarithms leaves slot
-// t + {0, 2, 1, 3} in lane 15 of rows 0..3.  Each slot's value is
+// same bits); a slot is read from its canonical end i (its class and the
+// reverse class share a block).  Block 2 n_cls holds w_f itself (theta
+// updates), then Q zeros.  Hidden units come in pairs (f, r), (f, r + d)
+// with off(b) = off(a) - D, D = code(d): entry idx of the LDS table is the
+// float2 (T[idx], T[idx - D]), so one 8-byte LDS read gives both factors of
+// a pair.  A lane owns CP pair chunks (registers 2c, 2c + 1) and CS single
+// chunks (register 2 CP + c, the .x half); per chunk it keeps the byte offset
+// 8 off(a) of its unit, a slot adds its byte base (one integer add, one LDS
+// read, then one fma and one multiply per factor).  Padded registers get
+// p = 0, q = 1: their factor is exactly 1.  A single chunk of at most 16
+// units is packed (PK): its units sit in lanes l % 16 of every row, and each
+// row evaluates it for the slot that row ends up holding, after the lane
+// products are combined.  Slots are evaluated four at a
+// time and their lane products combined together: v_permlane32_swap pairs
+// lanes l and l + 32 of two slots at once, v_permlane16_swap rows 0 / 1 and
+// 2 / 3 of the two pairs (products, so each lane then holds the factors of
+// four lanes of one slot: one log2 per lane for four slots), then a row scan
+// (row_shr 1, 2, 4, 8) of the logarithms leaves slot t + {0, 2, 1, 3} in
+// lane 15 of rows 0..3.  Each slot's value is
 // sum_{l<16} log2((x_l x_{l+32}) (x_{l+16} x_{l+48})) in the row scan's
 // order: the same tree for every slot, so a slot's weight does not depend on
 // its place in the list (unless the quad takes the per-factor fallback).
@@ -488,60 +497,159 @@ __global__ __launch_bounds__(256) void k_rbt_tables(DevProblem P, float *tab) {
     tab[t] = __builtin_amdgcn_exp2f(fmaf(cj, wj, ci * wi));
 }
 
-// p, q of the lane's units; padded units (h >= H, last chunk only) get p = 0,
-// q = 1: their factor is exactly 1
-template <int M>
-__device__ __forceinline__ void rbt_pq(RbhState<M> &S, bool pad_last) {
-    rbh_pq(S);
-    if (pad_last) {
-        S.p[M - 1] = 0.0f;
-        S.q[M - 1] = 1.0f;
+// the kernel's table: entry idx = (T[idx], T[idx - D]) (0 below D: never read)
+__global__ __launch_bounds__(256) void k_rbt_pair(int64_t n, int D, const float *T, float2 *T2) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    T2[t] = make_float2(T[t], t >= D ? T[t - D] : 0.0f);
+}
+
+template <int CP, int CS, bool PK>
+struct RbtState {
+    static constexpr int U = 2 * CP + CS;
+    float th[U], p[U], q[U];
+};
+
+// the lane's registers: byte offsets of its chunks, biases, the padded ones
+template <int CP, int CS, bool PK>
+struct RbtLane {
+    int off[CP + CS];
+    uint32_t pad;
+};
+
+__device__ __forceinline__ float2 rbt_ld2(const unsigned char *T2, int byte) {
+    return *(const float2 *)(T2 + byte);
+}
+__device__ __forceinline__ float rbt_ld1(const unsigned char *T2, int byte) {
+    return *(const float *)(T2 + byte);
+}
+
+// p, q of the lane's units; padded units get p = 0, q = 1
+template <int CP, int CS, bool PK>
+__device__ __forceinline__ void rbt_pq(RbtState<CP, CS, PK> &S, uint32_t pad) {
+#pragma unroll
+    for (int u = 0; u < 2 * CP + CS; ++u) {
+        const float e = __expf(-2.0f * fabsf(S.th[u]));
+        const float inv = __builtin_amdgcn_rcpf(1.0f + e);
+        const float big = inv, small = e * inv;
+        const bool pd = (pad >> u) & 1u;
+        S.p[u] = pd ? 0.0f : (S.th[u] >= 0.0f ? big : small);
+        S.q[u] = pd ? 1.0f : (S.th[u] >= 0.0f ? small : big);
     }
 }
 
-template <int M>
-__device__ void rbt_start(const DevProblem &P, const float *Wt, RbhLds &L, RbhState<M> &S,
-                          const int *vb, bool pad_last, const uint32_t *sp, int lane) {
+template <int CP, int CS, bool PK>
+__device__ void rbt_start(const DevProblem &P, const unsigned char *T2, RbhLds &L,
+                          RbtState<CP, CS, PK> &S, const RbtLane<CP, CS, PK> &ln, const uint32_t *sp,
+                          int lane) {
     for (int k = lane; k < P.N; k += WAVE) L.x[k] = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1 : -1;
     for (int s = P.n_off + lane; s < P.K * WAVE; s += WAVE) L.w[wpos(s, P.K)] = 0.0f;
     wave_sync();
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const int h = lane + 64 * m;
-        S.th[m] = h < P.H ? P.rbm_b[h] : 0.0f;
+    for (int u = 0; u < 2 * CP + CS; ++u) {
+        const int h = P.rbt_hid[(PK && u == 2 * CP ? (lane & 15) : lane) + 64 * u];
+        S.th[u] = h >= 0 ? P.rbm_b[h] : 0.0f;
     }
+    const int wb = 8 * 2 * P.rbt_ncls * P.rbt_F * P.rbt_Q;
     for (int k = 0; k < P.N; ++k) {
         const float xk = (float)L.x[k];
-        const int ck = P.rbt_kcode[k];
+        const int ck = wb + 8 * P.rbt_kcode[k];
 #pragma unroll
-        for (int m = 0; m < M; ++m) S.th[m] += Wt[ck + vb[m]] * xk;
+        for (int c = 0; c < CP; ++c) {
+            const float2 w = rbt_ld2(T2, ck + ln.off[c]);
+            S.th[2 * c] = fmaf(w.x, xk, S.th[2 * c]);   // w x_k is exact
+            S.th[2 * c + 1] = fmaf(w.y, xk, S.th[2 * c + 1]);
+        }
+#pragma unroll
+        for (int c = 0; c < CS; ++c)
+            S.th[2 * CP + c] = fmaf(rbt_ld1(T2, ck + ln.off[CP + c]), xk, S.th[2 * CP + c]);
     }
-    rbt_pq(S, pad_last);
+    rbt_pq(S, ln.pad);
 }
 
-template <int M>
-__device__ __forceinline__ void rbt_move_theta(const DevProblem &P, const float *Wt,
-                                               const RbhLds &L, RbhState<M> &S, const int *vb,
-                                               bool pad_last, int i, int j) {
+template <int CP, int CS, bool PK>
+__device__ __forceinline__ void rbt_move_theta(const DevProblem &P, const unsigned char *T2,
+                                               const RbhLds &L, RbtState<CP, CS, PK> &S,
+                                               const RbtLane<CP, CS, PK> &ln, int i, int j) {
     const float xi = (float)L.x[i], xj = (float)L.x[j];
-    const int ci = P.rbt_kcode[i], cj = P.rbt_kcode[j];
+    const int wb = 8 * 2 * P.rbt_ncls * P.rbt_F * P.rbt_Q;
+    const int ci = wb + 8 * P.rbt_kcode[i], cj = wb + 8 * P.rbt_kcode[j];
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-        float d = -2.0f * xi * Wt[ci + vb[m]];
-        d += -2.0f * xj * Wt[cj + vb[m]];
-        S.th[m] = S.th[m] + d;
+    for (int c = 0; c < CP; ++c) {
+        const float2 wi = rbt_ld2(T2, ci + ln.off[c]), wj = rbt_ld2(T2, cj + ln.off[c]);
+        S.th[2 * c] = S.th[2 * c] + fmaf(-2.0f * xj, wj.x, -2.0f * xi * wi.x);
+        S.th[2 * c + 1] = S.th[2 * c + 1] + fmaf(-2.0f * xj, wj.y, -2.0f * xi * wi.y);
     }
-    rbt_pq(S, pad_last);
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+        const float wi = rbt_ld1(T2, ci + ln.off[CP + c]), wj = rbt_ld1(T2, cj + ln.off[CP + c]);
+        S.th[2 * CP + c] = S.th[2 * CP + c] + fmaf(-2.0f * xj, wj, -2.0f * xi * wi);
+    }
+    rbt_pq(S, ln.pad);
 }
 
 __device__ __forceinline__ float as_f(uint32_t v) { return __uint_as_float(v); }
 __device__ __forceinline__ uint32_t as_u(float v) { return __float_as_uint(v); }
 
+// this lane's products of the factors of the four slots at byte bases b0..b3
+// (chunk by chunk, the four slots' reads issued together)
+template <int CP, int CS, bool PK>
+__device__ __forceinline__ void rbt_lane_prod4(const unsigned char *T2, const RbtState<CP, CS, PK> &S,
+                                               const RbtLane<CP, CS, PK> &ln, int b0, int b1, int b2,
+                                               int b3, float &p0, float &p1, float &p2,
+                                               float &p3) {
+    p0 = p1 = p2 = p3 = 1.0f;
+#pragma unroll
+    for (int c = 0; c < CP; ++c) {
+        const float2 g0 = rbt_ld2(T2, b0 + ln.off[c]), g1 = rbt_ld2(T2, b1 + ln.off[c]);
+        const float2 g2 = rbt_ld2(T2, b2 + ln.off[c]), g3 = rbt_ld2(T2, b3 + ln.off[c]);
+        const float pa = S.p[2 * c], qa = S.q[2 * c], pb = S.p[2 * c + 1], qb = S.q[2 * c + 1];
+        p0 *= fmaf(pa, g0.x, qa);
+        p1 *= fmaf(pa, g1.x, qa);
+        p2 *= fmaf(pa, g2.x, qa);
+        p3 *= fmaf(pa, g3.x, qa);
+        p0 *= fmaf(pb, g0.y, qb);
+        p1 *= fmaf(pb, g1.y, qb);
+        p2 *= fmaf(pb, g2.y, qb);
+        p3 *= fmaf(pb, g3.y, qb);
+    }
+#pragma unroll
+    for (int c = 0; c < (PK ? 0 : CS); ++c) {
+        const int o = ln.off[CP + c];
+        const float g0 = rbt_ld1(T2, b0 + o), g1 = rbt_ld1(T2, b1 + o);
+        const float g2 = rbt_ld1(T2, b2 + o), g3 = rbt_ld1(T2, b3 + o);
+        const float pa = S.p[2 * CP + c], qa = S.q[2 * CP + c];
+        p0 *= fmaf(pa, g0, qa);
+        p1 *= fmaf(pa, g1, qa);
+        p2 *= fmaf(pa, g2, qa);
+        p3 *= fmaf(pa, g3, qa);
+    }
+}
+
+// the same factors, one log2 each, summed
+template <int CP, int CS, bool PK>
+__device__ __forceinline__ float rbt_lane_logs(const unsigned char *T2, const RbtState<CP, CS, PK> &S,
+                                               const RbtLane<CP, CS, PK> &ln, int b) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < CP; ++c) {
+        const float2 g = rbt_ld2(T2, b + ln.off[c]);
+        acc += __builtin_amdgcn_logf(fmaf(S.p[2 * c], g.x, S.q[2 * c]));
+        acc += __builtin_amdgcn_logf(fmaf(S.p[2 * c + 1], g.y, S.q[2 * c + 1]));
+    }
+#pragma unroll
+    for (int c = 0; c < (PK ? 0 : CS); ++c)
+        acc += __builtin_amdgcn_logf(
+            fmaf(S.p[2 * CP + c], rbt_ld1(T2, b + ln.off[CP + c]), S.q[2 * CP + c]));
+    return acc;
+}
+
 // every slot weight and the edge states; returns the anti count (the rbh
 // kernels' rbh_eval_all with table factors, four slots at a time)
-template <int M>
-__device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const float *T,
-                            RbhLds &L, const RbhState<M> &S, const int *vb, int lane) {
+template <int CP, int CS, bool PK>
+__device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const unsigned char *T2,
+                            RbhLds &L, const RbtState<CP, CS, PK> &S, const RbtLane<CP, CS, PK> &ln,
+                            int lane) {
     const int FQ = P.rbt_F * P.rbt_Q;
     int n_act = 0;
     for (int c = 0; c < P.n_off; c += WAVE) {
@@ -561,7 +669,9 @@ __device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const 
             int a = wpos(s, P.K);
             if (bare > 0.0f) a |= RBH_POS_BARE;
             const int sc = P.rbt_scode[s];
-            L.ent[pos] = make_int2(a, ((sc >> 20) * 2 + (xi < 0.0f ? 1 : 0)) * FQ + (sc & 0xfffff));
+            const float xf = (sc >> 30) & 1 ? xj : xi;   // the canonical end's spin
+            L.ent[pos] = make_int2(a, 8 * ((((sc >> 20) & 0x3ff) * 2 + (xf < 0.0f ? 1 : 0)) * FQ +
+                                          (sc & 0xfffff)));
             L.g[pos] = rbh_field(P, i, j, xi, xj) + __logf(fabsf(bare));
         }
         n_act += __popcll(msk);
@@ -582,45 +692,36 @@ __device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const 
         const int b1 = __builtin_amdgcn_readfirstlane(e01.w);
         const int b2 = __builtin_amdgcn_readfirstlane(e23.y);
         const int b3 = __builtin_amdgcn_readfirstlane(e23.w);
-        float p0 = 1.0f, p1 = 1.0f, p2 = 1.0f, p3 = 1.0f;
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const float g0 = T[b0 + vb[m]], g1 = T[b1 + vb[m]];
-            const float g2 = T[b2 + vb[m]], g3 = T[b3 + vb[m]];
-            p0 *= fmaf(S.p[m], g0, S.q[m]);
-            p1 *= fmaf(S.p[m], g1, S.q[m]);
-            p2 *= fmaf(S.p[m], g2, S.q[m]);
-            p3 *= fmaf(S.p[m], g3, S.q[m]);
+        float fs = 1.0f;
+        if (PK) {   // the packed chunk: unit lane % 16 of this row's slot
+            const int lo = (row & 1) ? e23.y : e01.y, hi = (row & 1) ? e23.w : e01.w;
+            fs = fmaf(S.p[2 * CP], rbt_ld1(T2, ((row & 2) ? hi : lo) + ln.off[CP]), S.q[2 * CP]);
         }
+        float p0, p1, p2, p3;
+        rbt_lane_prod4(T2, S, ln, b0, b1, b2, b3, p0, p1, p2, p3);
         // the four lane products paired across lanes first (l with l + 32,
-        // then with l + 16: 28 factors per lane, one slot per row), one
-        // logarithm per lane, the row scan; a product outside [1e-30, 1e30]
-        // in any lane sends the wavefront to one logarithm per factor, summed
-        // in the same tree
+        // then with l + 16: one slot per row), one logarithm per lane, the
+        // row scan; a product outside [1e-30, 1e30] in any lane sends the
+        // wavefront to one logarithm per factor, summed in the same tree
         const auto q01 = __builtin_amdgcn_permlane32_swap(as_u(p0), as_u(p1), false, false);
         const auto q23 = __builtin_amdgcn_permlane32_swap(as_u(p2), as_u(p3), false, false);
         const float m01 = as_f(q01[0]) * as_f(q01[1]);
         const float m23 = as_f(q23[0]) * as_f(q23[1]);
         const auto qq = __builtin_amdgcn_permlane16_swap(as_u(m01), as_u(m23), false, false);
-        const float pr = as_f(qq[0]) * as_f(qq[1]);
+        const float pr = (as_f(qq[0]) * as_f(qq[1])) * fs;
         float v;
         if (__ballot(!(pr > 1e-30f && pr < 1e30f)) == 0ull) {
             v = __builtin_amdgcn_logf(pr);
         } else {   // rare: factors far from 1
-            float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f, l3 = 0.0f;
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                l0 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b0 + vb[m]], S.q[m]));
-                l1 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b1 + vb[m]], S.q[m]));
-                l2 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b2 + vb[m]], S.q[m]));
-                l3 += __builtin_amdgcn_logf(fmaf(S.p[m], T[b3 + vb[m]], S.q[m]));
-            }
+            const float l0 = rbt_lane_logs(T2, S, ln, b0), l1 = rbt_lane_logs(T2, S, ln, b1);
+            const float l2 = rbt_lane_logs(T2, S, ln, b2), l3 = rbt_lane_logs(T2, S, ln, b3);
             const auto r01 = __builtin_amdgcn_permlane32_swap(as_u(l0), as_u(l1), false, false);
             const auto r23 = __builtin_amdgcn_permlane32_swap(as_u(l2), as_u(l3), false, false);
             const float s01 = as_f(r01[0]) + as_f(r01[1]);
             const float s23 = as_f(r23[0]) + as_f(r23[1]);
             const auto r = __builtin_amdgcn_permlane16_swap(as_u(s01), as_u(s23), false, false);
             v = as_f(r[0]) + as_f(r[1]);
+            if (PK) v += __builtin_amdgcn_logf(fs);
         }
         // rows 0..3: slots t, t + 2, t + 1, t + 3
         v += dpp_get<0x111, 0xf>(v);
@@ -649,36 +750,40 @@ __device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const 
 // the continuous-time propagation of k_propagate_rbh with table factors;
 // the factor tables are staged in LDS once per workgroup, walkers come from
 // the same per-launch queue
-template <int M>
+template <int CP, int CS, bool PK>
 __global__ __launch_bounds__(1024) void k_propagate_rbt(
     DevProblem P, Tables<float> tb, int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
     double beta, double alw, uint32_t key0, uint32_t key1, uint32_t step, uint32_t tag,
     int32_t *evals, const uint32_t *step_base) {
     step += step_base ? *step_base : 0u;
     extern __shared__ __align__(16) unsigned char lds[];
-    float *T = (float *)lds;
+    const unsigned char *T2 = lds;
     {
         const uint4 *src = (const uint4 *)P.rbt_tab;
-        uint4 *dst = (uint4 *)T;
+        uint4 *dst = (uint4 *)lds;
         for (int t = threadIdx.x; t < P.rbt_words / 4; t += blockDim.x) dst[t] = src[t];
         __syncthreads();
     }
-    const float *Wt = T + (size_t)2 * P.rbt_ncls * P.rbt_F * P.rbt_Q;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     RbhLds L = rbh_carve_at(P, lds, wave, (size_t)P.rbt_words * 4);
     WalkerLds<float> WL;
     WL.w = L.w;
     WL.x = L.x;
     WL.anti = L.anti;
-    int vb[M];
+    RbtLane<CP, CS, PK> ln;
 #pragma unroll
-    for (int m = 0; m < M; ++m) vb[m] = P.rbt_unit[lane + 64 * m];
-    const bool pad_last = lane + 64 * (M - 1) >= P.H;
-    RbhState<M> S;
+    for (int c = 0; c < CP + CS; ++c)
+        ln.off[c] = 8 * P.rbt_unit[(PK && c == CP ? (lane & 15) : lane) + 64 * c];
+    ln.pad = 0u;
+#pragma unroll
+    for (int u = 0; u < 2 * CP + CS; ++u) {
+        if (P.rbt_hid[(PK && u == 2 * CP ? (lane & 15) : lane) + 64 * u] < 0) ln.pad |= 1u << u;
+    }
+    RbtState<CP, CS, PK> S;
     for (int64_t wk = rbh_next(P.rbm_queue, lane); wk < nw; wk = rbh_next(P.rbm_queue, lane)) {
         uint32_t *sp = spins + wk * P.words;
-        rbt_start<M>(P, Wt, L, S, vb, pad_last, sp, lane);
-        int n_anti = rbt_eval_all<M>(P, tb, T, L, S, vb, lane);
+        rbt_start(P, T2, L, S, ln, sp, lane);
+        int n_anti = rbt_eval_all(P, tb, T2, L, S, ln, lane);
         double diag = diag_value(P, L.x, L.anti, n_anti, lane);
         double lw = log_w[wk];
         double t = 0.0, eL = 0.0;
@@ -707,14 +812,14 @@ __global__ __launch_bounds__(1024) void k_propagate_rbt(
             if (tau_raw < rem && idx >= 0) {
                 int i, j;
                 slot_sites(P, idx, &i, &j);
-                rbt_move_theta<M>(P, Wt, L, S, vb, pad_last, i, j);
+                rbt_move_theta(P, T2, L, S, ln, i, j);
                 wave_sync();
                 if (lane == 0) {
                     L.x[i] = (signed char)-L.x[i];
                     L.x[j] = (signed char)-L.x[j];
                 }
                 wave_sync();
-                n_anti = rbt_eval_all<M>(P, tb, T, L, S, vb, lane);
+                n_anti = rbt_eval_all(P, tb, T2, L, S, ln, lane);
                 diag = diag_value(P, L.x, L.anti, n_anti, lane);
             }
         }

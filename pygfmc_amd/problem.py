@@ -194,7 +194,8 @@ class Problem:
             t = tied_tables(H.edge_i, H.edge_j, guide)
             if t is not None:
                 rc = lib.gfmc_rbm_set_tied(
-                    h, t["F"], t["n_T"], t["Q"], t["n_cls"], t["unit_off"].ctypes.data,
+                    h, t["F"], t["n_T"], t["Q"], t["n_cls"], t["pair_code"], t["n_pc"],
+                    t["n_sc"], t["lane_off"].ctypes.data, t["lane_unit"].ctypes.data,
                     t["slot_code"].ctypes.data, t["site_code"].ctypes.data,
                     t["q_site"].ctypes.data, t["q_pair"].ctypes.data, _lib.stream())
                 if rc == 0:

@@ -53,13 +53,39 @@ def find_torus(translations):
     return None
 
 
+def _pairing(Lx, Ly, B, S):
+    """The translation delta that pairs hidden units (f, r) and (f, r + delta)
+    without a wrap (r in the lower half along a doubled direction) and its
+    code (delta_y S + delta_x) B, or (None, 0) when both sides are odd."""
+    if Ly % 2 == 0:
+        return (0, Ly // 2), (Ly // 2) * S * B
+    if Lx % 2 == 0:
+        return (Lx // 2, 0), (Lx // 2) * B
+    return None, 0
+
+
 def tied_tables(edge_i, edge_j, guide, lanes=64):
     """The index tables of ``guide`` (RBMSpinHalf) for the pair slots
     (edge_i[s], edge_j[s]), or None when the guide is not a tied RBM on a
-    torus.  Returns dict(F, n_T, Q, n_cls, unit_off [Hp], slot_code
-    [n_slots] = code(i) | class << 20, site_code [N], q_site [Q], q_pair
-    [n_cls Q] (-1 where the entry's basis site is not class c's first
-    site)), all int32.  Checks that W is tied as the tables assume."""
+    torus.  All int32:
+
+    unit_off [Hp]      f Q + O - code(r) per hidden unit (padding: F Q + O)
+    slot_code [n_slots] code(first site) | class << 20 | swapped << 30: a
+                       slot is read from its canonical end (the one whose
+                       displacement key to the other is the smaller), so a
+                       class and its reverse share one table block
+    site_code [N], q_site [Q], q_pair [n_cls Q] (-1 where the entry's basis
+                       site is not class c's first site)
+    pair_code          code(delta) of the unit pairing (0: no pairing)
+    n_pc, n_sc         pair and single lane chunks
+    lane_off [(n_pc + n_sc) lanes]   the offset of the chunk's (first) unit
+    lane_unit [(2 n_pc + n_sc) lanes] hidden unit of register u = 2c, 2c + 1
+                       (pair chunk c) or 2 n_pc + c (single chunk c); -1 pads
+
+    Unit b = (f, r + delta) of a pair has off(b) = off(a) - pair_code, so the
+    kernel's entry (T[idx], T[idx - pair_code]) at idx = block + code(i) +
+    off(a) holds both factors of the pair.  Checks that W is tied as the
+    tables assume and that every read names the right weight."""
     T = getattr(guide, "translations", None)
     if T is None:
         return None
@@ -98,7 +124,8 @@ def tied_tables(edge_i, edge_j, guide, lanes=64):
     unit_off[:H] = f * Q + O - (ry * S + rx) * B
     # padded units read the zero block after the weight block (filters x Q
     # entries further on from filter 0's)
-    unit_off[H:] = F * Q + O
+    pad_off = F * Q + O
+    unit_off[H:] = pad_off
     # check: entry code(k) + off(f, r) - f Q of block f names the site k - r
     Tinv = np.empty_like(T)
     for rr in range(n_T):
@@ -106,10 +133,15 @@ def tied_tables(edge_i, edge_j, guide, lanes=64):
     qq = site_code[None, :] + (O - (ry[:n_T] * S + rx[:n_T]) * B)[:, None]
     if qq.min() < 0 or qq.max() >= Q or not np.array_equal(q_site[qq], Tinv):
         raise AssertionError("torus tables: site codes do not invert the translations")
-    # displacement classes of the slots
-    ei = np.asarray(edge_i, dtype=np.int64)
-    ej = np.asarray(edge_j, dtype=np.int64)
-    key = np.stack([kb[ei], kb[ej], (kcx[ej] - kcx[ei]) % Lx, (kcy[ej] - kcy[ei]) % Ly], axis=1)
+    # displacement classes of the slots, each read from its canonical end
+    ei0 = np.asarray(edge_i, dtype=np.int64)
+    ej0 = np.asarray(edge_j, dtype=np.int64)
+    fwd = np.stack([kb[ei0], kb[ej0], (kcx[ej0] - kcx[ei0]) % Lx, (kcy[ej0] - kcy[ei0]) % Ly], 1)
+    rev = np.stack([kb[ej0], kb[ei0], (kcx[ei0] - kcx[ej0]) % Lx, (kcy[ei0] - kcy[ej0]) % Ly], 1)
+    swap = np.array([tuple(b) < tuple(a) for a, b in zip(fwd, rev)], dtype=bool)
+    ei = np.where(swap, ej0, ei0)
+    ej = np.where(swap, ei0, ej0)
+    key = np.where(swap[:, None], rev, fwd)
     cls_keys, cls_of = np.unique(key, axis=0, return_inverse=True)
     cls_of = cls_of.reshape(-1)
     n_cls = len(cls_keys)
@@ -121,9 +153,42 @@ def tied_tables(edge_i, edge_j, guide, lanes=64):
     qs = site_code[ei][None, :] + (O - (ry[:n_T] * S + rx[:n_T]) * B)[:, None]
     if not np.array_equal(q_pair[cls_of[None, :], qs], Tinv[:, ej]):
         raise AssertionError("torus tables: pair entries do not name the partner sites")
-    if Q >= (1 << 20):
+    if Q >= (1 << 20) or n_cls >= (1 << 10):
         return None
+    # unit pairs (f, r), (f, r + delta) and their lane chunks: full pair
+    # chunks; a remainder of more than half a chunk of pairs takes one more
+    # pair chunk, a smaller one goes to single chunks as separate units
+    delta, pair_code = _pairing(Lx, Ly, B, S)
+    if delta is not None:
+        low = (ry < Ly // 2) if delta[1] else (rx < Lx // 2)
+        ua = h[low]
+        ub = ua + (delta[1] * Lx + delta[0])
+        pairs = np.stack([ua, ub], 1)
+        if not np.array_equal(np.sort(pairs.reshape(-1)), h) or \
+                not np.array_equal(unit_off[ub], unit_off[ua] - pair_code):
+            raise AssertionError("torus tables: unit pairs do not share a table entry")
+        n_pc, rem = divmod(len(pairs), lanes)
+        if rem > lanes // 2:
+            n_pc, rem = n_pc + 1, 0
+        singles = pairs[len(pairs) - rem:].reshape(-1) if rem else np.zeros(0, np.int64)
+        pairs = pairs[:len(pairs) - rem]
+    else:
+        pair_code, n_pc, pairs, singles = 0, 0, np.zeros((0, 2), np.int64), h
+    n_sc = (len(singles) + lanes - 1) // lanes
+    lane_off = np.full((n_pc + n_sc) * lanes, pad_off, dtype=np.int64)
+    lane_unit = np.full((2 * n_pc + n_sc) * lanes, -1, dtype=np.int64)
+    for t, (a, b) in enumerate(pairs):
+        c, l = divmod(t, lanes)
+        lane_off[c * lanes + l] = unit_off[a]
+        lane_unit[2 * c * lanes + l] = a
+        lane_unit[(2 * c + 1) * lanes + l] = b
+    for t, u in enumerate(singles):
+        c, l = divmod(t, lanes)
+        lane_off[(n_pc + c) * lanes + l] = unit_off[u]
+        lane_unit[(2 * n_pc + c) * lanes + l] = u
     i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
     return dict(F=F, n_T=n_T, Q=Q, n_cls=n_cls, unit_off=i32(unit_off),
-                slot_code=i32(site_code[ei] | (cls_of << 20)), site_code=i32(site_code),
-                q_site=i32(q_site), q_pair=i32(q_pair.reshape(-1)), torus=(B, Lx, Ly))
+                slot_code=i32(site_code[ei] | (cls_of << 20) | (swap.astype(np.int64) << 30)),
+                site_code=i32(site_code), q_site=i32(q_site), q_pair=i32(q_pair.reshape(-1)),
+                pair_code=int(pair_code), n_pc=int(n_pc), n_sc=int(n_sc),
+                lane_off=i32(lane_off), lane_unit=i32(lane_unit), torus=(B, Lx, Ly))

@@ -24,42 +24,80 @@ def test_find_torus_rejects_other_tables():
 
 
 def emulate(t, W, H, edge_i, edge_j):
-    """Every read of the kernel, from the tables, against W."""
-    F, Q, nT = t["F"], t["Q"], t["n_T"]
-    Hp = len(t["unit_off"])
-    wt = np.zeros((F + 1) * Q)               # the w block, then the zero block
-    for f in range(F):
-        wt[f * Q:(f + 1) * Q] = W[f * nT][t["q_site"]]
+    """Every read of the kernel, from the tables, against W: entry idx of the
+    kernel's table holds (T[idx], T[idx - pair_code]) for a pair chunk (the
+    lane's units a and b) and T[idx] for a single chunk."""
+    F, Q, nT, D = t["F"], t["Q"], t["n_T"], t["pair_code"]
+    n_pc, n_sc = t["n_pc"], t["n_sc"]
+    lanes = 64
     N = W.shape[1]
-    # theta reads
-    got = wt[t["site_code"][None, :] + t["unit_off"][:, None]]
-    ref = np.zeros((Hp, N))
-    ref[:H] = W
-    np.testing.assert_array_equal(got, ref)
-    # factor reads: entry q of block f names i - r (w_f there is W[h][i]) and,
-    # through the class's pair table, j - r (W[h][j])
+    ncls = t["n_cls"]
+    FQ = F * Q
+    # the raw table as (filter, first site, partner site) per entry; the
+    # weight block as (filter, site, -1); the zero block as (-1, -1, -1)
+    raw = np.full(((2 * ncls + 1) * FQ + Q, 3), -1, dtype=np.int64)
+    qp = t["q_pair"].reshape(ncls, Q)
+    for c in range(ncls):
+        for sg in range(2):
+            for f in range(F):
+                o = (2 * c + sg) * FQ + f * Q
+                raw[o:o + Q, 0] = f
+                raw[o:o + Q, 1] = t["q_site"]
+                raw[o:o + Q, 2] = qp[c]
+    for f in range(F):
+        o = 2 * ncls * FQ + f * Q
+        raw[o:o + Q, 0] = f
+        raw[o:o + Q, 1] = t["q_site"]
+
+    def w_of(e):          # the weight an entry names: w_f at its site
+        return np.where(e[..., 0] >= 0, W[np.maximum(e[..., 0], 0) * nT, np.maximum(e[..., 1], 0)],
+                        0.0)
+
+    def wj_of(e):
+        return W[e[..., 0] * nT, e[..., 2]]
+
+    # register u of a lane reads entry idx(lane chunk) (x) or idx - D (y)
+    regs = []
+    for c in range(n_pc):
+        regs.append((c, 0, t["lane_unit"][2 * c * lanes:(2 * c + 1) * lanes]))
+        regs.append((c, D, t["lane_unit"][(2 * c + 1) * lanes:(2 * c + 2) * lanes]))
+    for c in range(n_sc):
+        regs.append((n_pc + c, 0,
+                     t["lane_unit"][(2 * n_pc + c) * lanes:(2 * n_pc + c + 1) * lanes]))
+    seen = np.concatenate([u for _, _, u in regs])
+    assert np.array_equal(np.sort(seen[seen >= 0]), np.arange(H))
     code = t["slot_code"] & 0xfffff
-    cls = t["slot_code"] >> 20
-    h = np.arange(H)
-    f = h // nT
-    q = code[None, :] + t["unit_off"][:H, None] - (f * Q)[:, None]
-    assert q.min() >= 0 and q.max() < Q
-    wi = W[(f * nT)[:, None], t["q_site"][q]]
-    pj = t["q_pair"].reshape(t["n_cls"], Q)[cls[None, :], q]
-    assert pj.min() >= 0
-    wj = W[(f * nT)[:, None], pj]
-    np.testing.assert_array_equal(wi, W[:, edge_i])
-    np.testing.assert_array_equal(wj, W[:, edge_j])
+    cls = (t["slot_code"] >> 20) & 0x3ff
+    swap = (t["slot_code"] >> 30) & 1
+    first = np.where(swap == 1, edge_j, edge_i)
+    second = np.where(swap == 1, edge_i, edge_j)
+    assert np.array_equal(code, t["site_code"][first])
+    for c, shift, units in regs:
+        off = t["lane_off"][c * lanes:(c + 1) * lanes]
+        real = units >= 0
+        # theta reads: the weight block at code(k)
+        idx = 2 * ncls * FQ + t["site_code"][None, :] + off[:, None] - shift
+        assert idx.min() >= 0 and idx.max() < len(raw)
+        got = w_of(raw[idx])
+        np.testing.assert_array_equal(got[real], W[units[real]])
+        # factor reads of both signs: entry names w_f at first - r and
+        # second - r
+        for sg in range(2):
+            idx = (2 * cls + sg)[None, :] * FQ + code[None, :] + off[:, None] - shift
+            assert idx.min() >= 0 and idx.max() < len(raw)
+            e = raw[idx[real]]
+            np.testing.assert_array_equal(w_of(e), W[units[real]][:, first])
+            np.testing.assert_array_equal(wj_of(e), W[units[real]][:, second])
 
 
-@pytest.mark.parametrize("Lx,Ly,F", [(4, 4, 3), (10, 10, 4), (6, 4, 2)])
+@pytest.mark.parametrize("Lx,Ly,F", [(4, 4, 3), (10, 10, 4), (6, 4, 2), (6, 6, 2), (5, 5, 2)])
 def test_tables_reproduce_the_weights_square(Lx, Ly, F):
     L = square(Lx, Ly)
     g = RBMSpinHalf.symmetric(L.extras["translations"], F, seed=Lx + Ly, scale=0.3)
     H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32)
     t = tied_tables(H.edge_i, H.edge_j, g)
     assert t is not None and t["torus"] == (1, Lx, Ly)
-    assert t["n_cls"] == 4
+    assert t["n_cls"] == 2           # x and y bonds, each from its canonical end
     emulate(t, g.W, g.hidden, H.edge_i, H.edge_j)
 
 
