@@ -421,7 +421,13 @@ int gfmc_rbm_set_params(gfmc_problem *p, const float *flat, void *stream);
  * <= 8); otherwise GFMC_EUNSUP and nothing changes.  gfmc_rbm_set_params
  * rebuilds the tables from the rows r = 0: parameters set later must stay
  * tied (the orbit-tied trainers keep them so), or call gfmc_rbm_untie first,
- * which returns the problem to the untied kernels. */
+ * which returns the problem to the untied kernels.  theta is held in FP64,
+ * where it is exact (BF16 weights), and the kernel carries rows across steps
+ * (gfmc_carry_bytes / gfmc_propagate_carry / gfmc_run_steps_carry): slot
+ * weights, theta and the final spins of every walker, bit-identical to a
+ * re-evaluation; rows of earlier parameters (before gfmc_rbm_set_params) are
+ * not used.  A step graph captured before gfmc_rbm_set_params is captured
+ * again after it. */
 int gfmc_rbm_set_tied(gfmc_problem *p, int32_t n_filters, int32_t n_T, int32_t Q,
                       int32_t n_cls, int32_t pair_code, int32_t n_pair_chunks,
                       int32_t n_single_chunks, const int32_t *lane_off,

@@ -51,7 +51,10 @@ static thread_local const uint32_t *g_step_base = nullptr;
 // evaluation of the same configuration gives (each weight is always
 // eval_rec of the current configuration), so results are bit-identical.
 // state[0] = rows valid, state[1] = buffer holding the latest rows,
-// state[2] = lanes per walker they were laid out for.
+// state[2] = lanes per walker they were laid out for (the tied-RBM kernel:
+// its own tag), state[3] = the tied RBM's parameter generation.  The tied-RBM
+// rows hold the slot weights, theta (exact in FP64) and the final spins; f_off
+// is then theta's offset.
 struct CarryDev {
     void *w[2];
     int32_t *anti[2];
@@ -158,6 +161,7 @@ struct DevProblem {
     // zeros) and the kernel's table rbt_tab of rbt_words floats (entry idx =
     // (raw[idx], raw[idx - rbt_D]), rounded up to a multiple of 4 floats)
     int rbt, rbt_cp, rbt_cs, rbt_pk, rbt_D, rbt_F, rbt_nT, rbt_Q, rbt_ncls, rbt_words;
+    int rbt_gen;   // parameter generation: carried rows of older ones are not used
     const float *rbt_tab;
     float *rbt_raw;
     const int *rbt_unit;    // [(cp + cs) 64] off(a) of each lane chunk, padding F Q + O
@@ -2477,11 +2481,16 @@ template <int CP, int CS, bool PK>
 static int rbt_launch_m(const gfmc_problem *p, int64_t nw, uint32_t *spins, double *log_w,
                         double *e_loc, double beta, double alw, uint32_t k0, uint32_t k1,
                         uint32_t step, uint32_t tag, int32_t *evals, hipStream_t st) {
+    CarryDev cy{};
+    if (g_carry && g_carry->on) cy = *g_carry;   // on only where rbt_carries
     hipLaunchKernelGGL(k_rbh_queue_reset, dim3(1), dim3(WAVE), 0, st, p->dp.rbm_queue);
     hipLaunchKernelGGL((k_propagate_rbt<CP, CS, PK>), dim3(rbt_grid(p, nw)),
                        dim3(p->rbt_waves * WAVE), (size_t)p->rbt_lds_bytes, st, p->dp, p->t32,
                        nw, spins, log_w, e_loc, beta, alw, k0, k1, step, tag, evals,
-                       g_step_base);
+                       g_step_base, cy);
+    if (cy.on)
+        hipLaunchKernelGGL(k_rbt_carry_commit, dim3(1), dim3(WAVE), 0, st, cy.state,
+                           p->dp.rbt_gen);
     return 0;
 }
 
@@ -3372,6 +3381,16 @@ int gfmc_gather(int32_t n_sites, int64_t nw, const uint32_t *spins_in, const dou
     return 0;
 }
 
+// the tied-RBM kernel's rows (in 4-byte units): slot weights, theta, spins
+// (the diagonal then comes from the anti-parallel count alone: uniform J_z)
+static bool rbt_carries(const gfmc_problem *p) {
+    return p->dp.guide == GFMC_GUIDE_RBM && p->dp.rbt && (!p->dp.has_jz || p->dp.uniform_jz);
+}
+static int64_t rbt_theta_off(const gfmc_problem *p) { return (int64_t)p->dp.K * WAVE; }
+static int64_t rbt_spins_off(const gfmc_problem *p) {
+    return rbt_theta_off(p) + 2 * (int64_t)(2 * p->dp.rbt_cp + p->dp.rbt_cs) * WAVE;
+}
+
 // elements of one carried row: the largest weight array over the usable
 // lane counts (0 when the problem's path carries nothing)
 static int64_t carry_weights(const gfmc_problem *p) {
@@ -3391,6 +3410,7 @@ static int64_t carry_spins_off(const gfmc_problem *p) {
 }
 
 static int64_t carry_row(const gfmc_problem *p) {
+    if (rbt_carries(p)) return rbt_spins_off(p) + ((int64_t)p->dp.words * 4 + 15) / 16 * 4;
     const int64_t w = carry_weights(p);
     if (w == 0) return 0;
     const int64_t el = p->precision == GFMC_FP64 ? 8 : 4;
@@ -3399,10 +3419,14 @@ static int64_t carry_row(const gfmc_problem *p) {
 
 static size_t carry_align(size_t b) { return (b + 255) / 256 * 256; }
 
+static size_t carry_el(const gfmc_problem *p) {
+    return rbt_carries(p) ? 4 : (p->precision == GFMC_FP64 ? 8 : 4);
+}
+
 static int64_t carry_bytes_of(const gfmc_problem *p, int64_t nw) {
     const int64_t row = carry_row(p);
     if (row == 0 || nw <= 0) return 0;
-    const size_t el = p->precision == GFMC_FP64 ? 8 : 4;
+    const size_t el = carry_el(p);
     return (int64_t)(256 + 2 * carry_align((size_t)nw * row * el) +
                      2 * carry_align((size_t)nw * 4));
 }
@@ -3412,7 +3436,7 @@ static CarryDev carry_dev(const gfmc_problem *p, int64_t nw, void *carry,
     CarryDev c{};
     const int64_t row = carry_row(p);
     if (row == 0) return c;
-    const size_t el = p->precision == GFMC_FP64 ? 8 : 4;
+    const size_t el = carry_el(p);
     unsigned char *b = (unsigned char *)carry;
     c.state = (int32_t *)b;
     size_t o = 256;
@@ -3422,8 +3446,8 @@ static CarryDev carry_dev(const gfmc_problem *p, int64_t nw, void *carry,
     c.anti[0] = (int32_t *)(b + o + 2 * wb);
     c.anti[1] = (int32_t *)(b + o + 2 * wb + ab);
     c.parents = parents;
-    c.f_off = p->dp.fld ? carry_weights(p) : 0;
-    c.s_off = carry_spins_off(p);
+    c.f_off = rbt_carries(p) ? rbt_theta_off(p) : (p->dp.fld ? carry_weights(p) : 0);
+    c.s_off = rbt_carries(p) ? rbt_spins_off(p) : carry_spins_off(p);
     c.row = row;
     c.on = 1;
     return c;

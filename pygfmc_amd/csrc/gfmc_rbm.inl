@@ -504,10 +504,14 @@ __global__ __launch_bounds__(256) void k_rbt_pair(int64_t n, int D, const float 
     T2[t] = make_float2(T[t], t >= D ? T[t - D] : 0.0f);
 }
 
+// theta in FP64: b_h + sum_k W_hk x_k of BF16 weights (and an FP32 bias) is
+// exact there, so the value after any sequence of moves equals the value
+// computed from scratch (carried rows are bit-identical to re-evaluated ones)
 template <int CP, int CS, bool PK>
 struct RbtState {
     static constexpr int U = 2 * CP + CS;
-    float th[U], p[U], q[U];
+    double th[U];
+    float p[U], q[U];
 };
 
 // the lane's registers: byte offsets of its chunks, biases, the padded ones
@@ -516,6 +520,10 @@ struct RbtLane {
     int off[CP + CS];
     uint32_t pad;
 };
+
+// the layout tag of carried rows written by the tied kernel: [K 64 slot
+// weights (FP32)][U 64 theta (FP64)][final spins, 16-byte rounded]
+#define RBT_CARRY_TAG 4096
 
 __device__ __forceinline__ float2 rbt_ld2(const unsigned char *T2, int byte) {
     return *(const float2 *)(T2 + byte);
@@ -529,12 +537,13 @@ template <int CP, int CS, bool PK>
 __device__ __forceinline__ void rbt_pq(RbtState<CP, CS, PK> &S, uint32_t pad) {
 #pragma unroll
     for (int u = 0; u < 2 * CP + CS; ++u) {
-        const float e = __expf(-2.0f * fabsf(S.th[u]));
+        const float th = (float)S.th[u];
+        const float e = __expf(-2.0f * fabsf(th));
         const float inv = __builtin_amdgcn_rcpf(1.0f + e);
         const float big = inv, small = e * inv;
         const bool pd = (pad >> u) & 1u;
-        S.p[u] = pd ? 0.0f : (S.th[u] >= 0.0f ? big : small);
-        S.q[u] = pd ? 1.0f : (S.th[u] >= 0.0f ? small : big);
+        S.p[u] = pd ? 0.0f : (th >= 0.0f ? big : small);
+        S.q[u] = pd ? 1.0f : (th >= 0.0f ? small : big);
     }
 }
 
@@ -548,21 +557,22 @@ __device__ void rbt_start(const DevProblem &P, const unsigned char *T2, RbhLds &
 #pragma unroll
     for (int u = 0; u < 2 * CP + CS; ++u) {
         const int h = P.rbt_hid[(PK && u == 2 * CP ? (lane & 15) : lane) + 64 * u];
-        S.th[u] = h >= 0 ? P.rbm_b[h] : 0.0f;
+        S.th[u] = h >= 0 ? (double)P.rbm_b[h] : 0.0;
     }
     const int wb = 8 * 2 * P.rbt_ncls * P.rbt_F * P.rbt_Q;
     for (int k = 0; k < P.N; ++k) {
-        const float xk = (float)L.x[k];
+        const double xk = (double)L.x[k];
         const int ck = wb + 8 * P.rbt_kcode[k];
 #pragma unroll
         for (int c = 0; c < CP; ++c) {
             const float2 w = rbt_ld2(T2, ck + ln.off[c]);
-            S.th[2 * c] = fmaf(w.x, xk, S.th[2 * c]);   // w x_k is exact
-            S.th[2 * c + 1] = fmaf(w.y, xk, S.th[2 * c + 1]);
+            S.th[2 * c] = __builtin_fma((double)w.x, xk, S.th[2 * c]);
+            S.th[2 * c + 1] = __builtin_fma((double)w.y, xk, S.th[2 * c + 1]);
         }
 #pragma unroll
         for (int c = 0; c < CS; ++c)
-            S.th[2 * CP + c] = fmaf(rbt_ld1(T2, ck + ln.off[CP + c]), xk, S.th[2 * CP + c]);
+            S.th[2 * CP + c] =
+                __builtin_fma((double)rbt_ld1(T2, ck + ln.off[CP + c]), xk, S.th[2 * CP + c]);
     }
     rbt_pq(S, ln.pad);
 }
@@ -571,19 +581,21 @@ template <int CP, int CS, bool PK>
 __device__ __forceinline__ void rbt_move_theta(const DevProblem &P, const unsigned char *T2,
                                                const RbhLds &L, RbtState<CP, CS, PK> &S,
                                                const RbtLane<CP, CS, PK> &ln, int i, int j) {
-    const float xi = (float)L.x[i], xj = (float)L.x[j];
+    const double xi = -2.0 * (double)L.x[i], xj = -2.0 * (double)L.x[j];
     const int wb = 8 * 2 * P.rbt_ncls * P.rbt_F * P.rbt_Q;
     const int ci = wb + 8 * P.rbt_kcode[i], cj = wb + 8 * P.rbt_kcode[j];
 #pragma unroll
     for (int c = 0; c < CP; ++c) {
         const float2 wi = rbt_ld2(T2, ci + ln.off[c]), wj = rbt_ld2(T2, cj + ln.off[c]);
-        S.th[2 * c] = S.th[2 * c] + fmaf(-2.0f * xj, wj.x, -2.0f * xi * wi.x);
-        S.th[2 * c + 1] = S.th[2 * c + 1] + fmaf(-2.0f * xj, wj.y, -2.0f * xi * wi.y);
+        S.th[2 * c] = __builtin_fma(xj, (double)wj.x, __builtin_fma(xi, (double)wi.x, S.th[2 * c]));
+        S.th[2 * c + 1] =
+            __builtin_fma(xj, (double)wj.y, __builtin_fma(xi, (double)wi.y, S.th[2 * c + 1]));
     }
 #pragma unroll
     for (int c = 0; c < CS; ++c) {
         const float wi = rbt_ld1(T2, ci + ln.off[CP + c]), wj = rbt_ld1(T2, cj + ln.off[CP + c]);
-        S.th[2 * CP + c] = S.th[2 * CP + c] + fmaf(-2.0f * xj, wj, -2.0f * xi * wi);
+        S.th[2 * CP + c] = __builtin_fma(xj, (double)wj,
+                                         __builtin_fma(xi, (double)wi, S.th[2 * CP + c]));
     }
     rbt_pq(S, ln.pad);
 }
@@ -754,7 +766,7 @@ template <int CP, int CS, bool PK>
 __global__ __launch_bounds__(1024) void k_propagate_rbt(
     DevProblem P, Tables<float> tb, int64_t nw, uint32_t *spins, double *log_w, double *e_loc,
     double beta, double alw, uint32_t key0, uint32_t key1, uint32_t step, uint32_t tag,
-    int32_t *evals, const uint32_t *step_base) {
+    int32_t *evals, const uint32_t *step_base, CarryDev cy) {
     step += step_base ? *step_base : 0u;
     extern __shared__ __align__(16) unsigned char lds[];
     const unsigned char *T2 = lds;
@@ -782,8 +794,42 @@ __global__ __launch_bounds__(1024) void k_propagate_rbt(
     RbtState<CP, CS, PK> S;
     for (int64_t wk = rbh_next(P.rbm_queue, lane); wk < nw; wk = rbh_next(P.rbm_queue, lane)) {
         uint32_t *sp = spins + wk * P.words;
-        rbt_start(P, T2, L, S, ln, sp, lane);
-        int n_anti = rbt_eval_all(P, tb, T2, L, S, ln, lane);
+        // carried rows (gfmc_propagate_carry): the parent's slot weights,
+        // theta and anti-parallel count, valid when the rows are this
+        // kernel's, of the current parameters, and the parent ended its
+        // step with the spins just loaded
+        int cur = 0;
+        bool carried = false;
+        if (cy.on) {
+            cur = cy.state[1];
+            carried = cy.state[0] != 0 && cy.state[2] == RBT_CARRY_TAG && cy.state[3] == P.rbt_gen;
+            if (carried) {
+                const uint32_t *cs = (const uint32_t *)((const float *)cy.w[cur] +
+                                                        (int64_t)cy.parents[wk] * cy.row +
+                                                        cy.s_off);
+                bool same = true;
+                for (int q = lane; q < P.words; q += WAVE) same = same && cs[q] == sp[q];
+                carried = __ballot(!same) == 0ull;
+            }
+        }
+        int n_anti;
+        if (carried) {
+            const int64_t par = cy.parents[wk];
+            const float *row = (const float *)cy.w[cur] + par * cy.row;
+            for (int k = lane; k < P.N; k += WAVE) L.x[k] = ((sp[k >> 5] >> (k & 31)) & 1u) ? 1 : -1;
+            const uint4 *src = (const uint4 *)row;
+            uint4 *dst = (uint4 *)L.w;
+            for (int q = lane; q < P.K * 16; q += WAVE) dst[q] = src[q];
+            const double *th = (const double *)(row + cy.f_off);
+#pragma unroll
+            for (int u = 0; u < 2 * CP + CS; ++u) S.th[u] = th[u * WAVE + lane];
+            rbt_pq(S, ln.pad);
+            n_anti = cy.anti[cur][par];
+            wave_sync();
+        } else {
+            rbt_start(P, T2, L, S, ln, sp, lane);
+            n_anti = rbt_eval_all(P, tb, T2, L, S, ln, lane);
+        }
         double diag = diag_value(P, L.x, L.anti, n_anti, lane);
         double lw = log_w[wk];
         double t = 0.0, eL = 0.0;
@@ -824,12 +870,33 @@ __global__ __launch_bounds__(1024) void k_propagate_rbt(
             }
         }
         lw = lw - alw * beta;
-        store_spins(P, sp, L.x, lane);
+        float *crow = cy.on ? (float *)cy.w[cur ^ 1] + wk * cy.row : nullptr;
+        store_spins(P, sp, L.x, lane, cy.on ? (uint32_t *)(crow + cy.s_off) : nullptr);
+        if (cy.on) {
+            wave_sync();
+            const uint4 *src = (const uint4 *)L.w;
+            uint4 *dst = (uint4 *)crow;
+            for (int q = lane; q < P.K * 16; q += WAVE) dst[q] = src[q];
+            double *th = (double *)(crow + cy.f_off);
+#pragma unroll
+            for (int u = 0; u < 2 * CP + CS; ++u) th[u * WAVE + lane] = S.th[u];
+            if (lane == 0) cy.anti[cur ^ 1][wk] = n_anti;
+        }
         if (lane == 0) {
             log_w[wk] = lw;
             e_loc[wk] = eL;
             if (evals) evals[wk] += (int32_t)it;
         }
         wave_sync();
+    }
+}
+
+// after a tied-RBM propagation that stored its rows
+__global__ void k_rbt_carry_commit(int32_t *state, int32_t gen) {
+    if (threadIdx.x == 0) {
+        state[0] = 1;
+        state[1] ^= 1;
+        state[2] = RBT_CARRY_TAG;
+        state[3] = gen;
     }
 }

@@ -631,7 +631,10 @@ int gfmc_rbm_set_params(gfmc_problem *p, const float *flat, void *stream) {
     if (p->dp.rbh)
         hipLaunchKernelGGL(k_rbm_pack_t, dim3(grid1d((int64_t)N * p->dp.Hp, 256)), dim3(256), 0,
                            (hipStream_t)stream, N, H, p->dp.Hp, flat, (uint16_t *)p->dp.rbm_WT);
-    if (p->dp.rbt) rbt_build_tables(p, (hipStream_t)stream);
+    if (p->dp.rbt) {   // new tables; rows carried under the old parameters lapse
+        rbt_build_tables(p, (hipStream_t)stream);
+        ++p->dp.rbt_gen;
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -173,3 +173,31 @@ def test_walkers_changed_outside_the_loop_are_re_evaluated(prec, lanes):
     # the check would differ (the stored rows of the replaced walkers'
     # parents are stale) -- the carry buffer was live during these steps
     assert a.carry is not None and int(a.carry[:4].view(torch.int32)[0].item()) == 1
+
+
+def test_tied_rbm_carry_is_bit_identical():
+    """The tied-RBM kernel carries slot weights and theta (exact in FP64) from
+    step to step: a carrying chain equals a re-evaluating one bitwise, also
+    across a parameter change (rows of the old parameters lapse)."""
+    from pygfmc_amd import RBMSpinHalf
+    from pygfmc_amd.training import flat_params
+    say("carry tied rbm")
+    L = square(6, 6)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    T = L.extras["translations"]
+    G0 = RBMSpinHalf.symmetric(T, 4, b=[0.1, -0.2, 0.05, 0.3], seed=3, scale=0.2)
+    w = prepare_walkers_sz(700, 36, 2)
+    p, a, b = pair(H, G0, _lib.GFMC_FP32, w)
+    assert p.tied is not None and a.carry is not None
+    G1 = RBMSpinHalf.symmetric(T, 4, b=[-0.1, 0.2, 0.0, 0.1], seed=4, scale=0.25)
+    for n, s0, G in ((6, 0, None), (5, 6, None), (4, 11, G1), (3, 15, None)):
+        if G is not None:
+            flat = torch.as_tensor(flat_params(G), device="cuda")
+            _lib.check(_lib.load().gfmc_rbm_set_params(p.handle, _lib.ptr(flat), _lib.stream()),
+                       "gfmc_rbm_set_params")
+        sa = a.steps(n, 0.2, 0.1, 9, s0, evals=True)
+        sb = b.steps(n, 0.2, 0.1, 9, s0, evals=True)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(sa.cpu().numpy(), sb.cpu().numpy())
+        same(a, b)
+    np.testing.assert_array_equal(a.evals.cpu().numpy(), b.evals.cpu().numpy())
