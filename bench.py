@@ -61,10 +61,10 @@ def parse():
     ap.add_argument("--mlp-walkers", type=int, default=262144)
     ap.add_argument("--mlp-steps", type=int, default=3)
     ap.add_argument("--race-train-iters", type=int, default=1500)
-    ap.add_argument("--race-steps", type=int, default=10000,
+    ap.add_argument("--race-steps", type=int, default=20000,
                     help="production steps of each run of the learned-guide section; the "
                          "jackknife takes blocks of 200 steps (longer than the p = 128 "
-                         "estimator's window): 50 blocks at 10000 steps, a ~20 %% relative "
+                         "estimator's window): 100 blocks at 20000 steps, a ~14 %% relative "
                          "error of each run's sigma^2")
     ap.add_argument("--summarize-profile", default=None,
                     help="print a text summary of a rocprofv3 database and exit")
@@ -656,7 +656,7 @@ def sigma_race(args):
     """Second metric on the 10 x 10 periodic Heisenberg lattice (J_x = J_y =
     -1, J_z = 1, Sz = 0), p = [4, 16, 64, 128], 16 384 walkers, step 0.2:
     GPU-seconds until the jackknife error bar (blocks of 200 steps) of E at
-    p = 128 reaches that of the Jastrow-FP64 run (10 000 production steps
+    p = 128 reaches that of the Jastrow-FP64 run (20 000 production steps
     after 400 warm-up steps), for the learned BF16 RBM guide (alpha = 4: four
     translation-invariant filters, 400 hidden units; started as the
     nearest-neighbour Jastrow guide and trained on the GPU by local-energy
@@ -766,6 +766,11 @@ def rbm_kernel_measure(H, G, cpu_seconds):
     flop = evals * len(H.edge_i) * G.hidden * 8.0
     kname = "k_propagate_rbt" if p.tied is not None else "k_propagate_rbh"
     prof = profiled_traffic(f"{kname}_heisenberg_square10x10_rbm400_w16384")
+    if p.tied is not None:
+        td = p.tied
+        kfull = f"{kname}<{td['n_pc']}, {td['n_sc']}, {'true' if td['packed'] else 'false'}>"
+    else:
+        kfull = f"{kname}<{(G.hidden + 63) // 64}>"
     roof = {"bound": "valu", "compute_unit": "FP32 vector ALU (no GEMM in the hop; the MI355X "
                                               "FP32 vector and matrix peaks are both 157.3 "
                                               "TFLOP/s)",
@@ -777,7 +782,7 @@ def rbm_kernel_measure(H, G, cpu_seconds):
             "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, "
                             "copy-calibrated)",
             "traffic_source": prof["source"] if prof else None,
-            "kernel": f"{kname}<7>", "kernel_avg_ms": t * 1e3,
+            "kernel": kfull, "kernel_avg_ms": t * 1e3,
             "evals_per_walker_step": evals / nw, "flop_per_eval": len(H.edge_i) * G.hidden * 8.0,
             "transcendentals_per_eval": len(H.edge_i) * G.hidden * 2.0}
     cpu = cpu_baseline(H, G, ch, H.N, 0.2, alw, 9, cpu_seconds)

@@ -78,6 +78,8 @@ def tied_tables(edge_i, edge_j, guide, lanes=64):
                        site is not class c's first site)
     pair_code          code(delta) of the unit pairing (0: no pairing)
     n_pc, n_sc         pair and single lane chunks
+    packed             the single chunk holds at most 16 units (four slots
+                       share it)
     lane_off [(n_pc + n_sc) lanes]   the offset of the chunk's (first) unit
     lane_unit [(2 n_pc + n_sc) lanes] hidden unit of register u = 2c, 2c + 1
                        (pair chunk c) or 2 n_pc + c (single chunk c); -1 pads
@@ -186,8 +188,10 @@ def tied_tables(edge_i, edge_j, guide, lanes=64):
         c, l = divmod(t, lanes)
         lane_off[(n_pc + c) * lanes + l] = unit_off[u]
         lane_unit[(2 * n_pc + c) * lanes + l] = u
+    # a single chunk of at most 16 units is evaluated four slots at a time
+    packed = n_sc == 1 and bool(np.all(lane_unit[(2 * n_pc) * lanes + 16:] < 0))
     i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
-    return dict(F=F, n_T=n_T, Q=Q, n_cls=n_cls, unit_off=i32(unit_off),
+    return dict(F=F, n_T=n_T, Q=Q, n_cls=n_cls, unit_off=i32(unit_off), packed=packed,
                 slot_code=i32(site_code[ei] | (cls_of << 20) | (swap.astype(np.int64) << 30)),
                 site_code=i32(site_code), q_site=i32(q_site), q_pair=i32(q_pair.reshape(-1)),
                 pair_code=int(pair_code), n_pc=int(n_pc), n_sc=int(n_sc),
