@@ -180,12 +180,14 @@ class RBMSpinHalf:
         if self.a.shape != (self.N,) or self.b.shape != (H,):
             raise ValueError("a must have shape (N,) and b shape (hidden,)")
         # tied parameters: orbits[q] = the orbit of flat parameter q (W row-major,
-        # then a, then b); parameters of one orbit are equal and train together
+        # then a, then b); parameters of one orbit are equal and train together;
+        # orbit -1 holds the frozen parameters (kept at their value by training)
         self.orbits = None
         if orbits is not None:
             o = np.asarray(orbits, dtype=np.int64).reshape(-1)
-            if o.shape != (H * self.N + self.N + H,) or o.min() < 0:
-                raise ValueError("orbits must give every flat parameter an orbit index >= 0")
+            if o.shape != (H * self.N + self.N + H,) or o.min() < -1:
+                raise ValueError("orbits must give every flat parameter an orbit index >= 0, "
+                                 "or -1 for a frozen parameter")
             self.orbits = o
         # the site permutations of a translation-invariant RBM (hidden unit
         # f * n_T + r is filter f moved by translation r), when it is one
@@ -194,18 +196,24 @@ class RBMSpinHalf:
 
     @classmethod
     def symmetric(cls, translations, n_filters, filters=None, b=None, a=0.0, seed=0,
-                  scale=0.01):
+                  scale=0.01, support=None):
         """A translation-invariant RBM: hidden unit (f, r) = f * n_T + r is
         filter f moved by translation r, W[(f, r), T_r(k)] = filters[f, k],
         b[(f, r)] = b[f], a_k = a (one value).  ``translations``: (n_T, N)
         site permutations T_r (row r maps site k to T_r(k)); n_filters * n_T
-        hidden units; the tied parameters are recorded in ``orbits``."""
+        hidden units; the tied parameters are recorded in ``orbits``.
+        ``support`` (boolean, N or (n_filters, N)): the filter entries that
+        exist; the others are 0 and frozen (orbit -1), so every hidden unit
+        sees only the sites its filter's support covers."""
         T = np.asarray(translations, dtype=np.int64)
         n_T, N = T.shape
         F = int(n_filters)
         rng = np.random.default_rng(seed)
         filt = (rng.normal(scale=scale, size=(F, N)) if filters is None
                 else np.asarray(filters, dtype=np.float64).reshape(F, N))
+        sup = (np.ones((F, N), dtype=bool) if support is None
+               else np.broadcast_to(np.asarray(support, dtype=bool), (F, N)))
+        filt = np.where(sup, filt, 0.0)
         bf = np.zeros(F) if b is None else np.asarray(b, dtype=np.float64).reshape(F)
         H = F * n_T
         W = np.zeros((H, N))
@@ -213,21 +221,23 @@ class RBMSpinHalf:
         for f in range(F):
             for r in range(n_T):
                 W[f * n_T + r, T[r]] = filt[f]
-                oW[f * n_T + r, T[r]] = f * N + np.arange(N)
+                oW[f * n_T + r, T[r]] = np.where(sup[f], f * N + np.arange(N), -1)
         orbits = np.concatenate([oW.reshape(-1), np.full(N, F * N + F),
                                  F * N + np.repeat(np.arange(F), n_T)])
         return cls(N, hidden=H, W=W, a=np.full(N, a), b=np.repeat(bf, n_T), orbits=orbits,
                    translations=T)
 
     @classmethod
-    def from_jastrow(cls, edges, translations, beta, n_filters, seed=0, scale=0.01):
+    def from_jastrow(cls, edges, translations, beta, n_filters, seed=0, scale=0.01,
+                     support=None):
         """A translation-invariant RBM that starts as the nearest-neighbour
         Jastrow guide log psi = -beta sum_edges x_i x_j (+ const): one filter
         per class of translation-equivalent edges, w at site 0 and -w at the
         edge's other end, lncosh(w (x_i - x_j)) = lncosh(2 w) on anti-parallel
         bonds and 0 on parallel ones, so lncosh(2 w) / 2 = beta,
         w = acosh(e^{2 beta}) / 2 (before BF16 rounding); the remaining
-        filters start small and random."""
+        filters start small and random.  ``support``: as in
+        :meth:`symmetric` (must hold site 0 and each class's other end)."""
         T = np.asarray(translations, dtype=np.int64)
         n_T, N = T.shape
         inv = np.empty_like(T)
@@ -249,7 +259,10 @@ class RBMSpinHalf:
             filt[f] = 0.0
             filt[f, 0] = w
             filt[f, d] = -w
-        return cls.symmetric(T, n_filters, filters=filt)
+            if support is not None and not (np.broadcast_to(support, (len(filt), N))[f, [0, d]]
+                                            .all()):
+                raise ValueError(f"support must hold site 0 and site {d} of filter {f}")
+        return cls.symmetric(T, n_filters, filters=filt, support=support)
 
     def parameters(self):
         return dict(W=self.W.copy(), a=self.a.copy(), b=self.b.copy())

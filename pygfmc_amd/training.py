@@ -30,10 +30,12 @@ __all__ = ["RBMTrainer", "MLPTrainer", "vmin_gradient", "train_variance"]
 
 def orbit_tables(orbits, dev):
     """Device (offsets, members, n_orbits) of an orbit index per flat
-    parameter: the members of orbit o in ascending flat order."""
+    parameter: the members of orbit o in ascending flat order; parameters of
+    orbit -1 (frozen) belong to none."""
     o = np.asarray(orbits, dtype=np.int64)
-    uniq, inv = np.unique(o, return_inverse=True)
-    members = np.argsort(inv, kind="stable").astype(np.int32)
+    live = np.flatnonzero(o >= 0)
+    uniq, inv = np.unique(o[live], return_inverse=True)
+    members = live[np.argsort(inv, kind="stable")].astype(np.int32)
     counts = np.bincount(inv, minlength=len(uniq))
     offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
     return (torch.as_tensor(offsets, device=dev), torch.as_tensor(members, device=dev),
@@ -78,8 +80,12 @@ class RBMTrainer:
         self._work = None
         self._work_n = -1
         self._orbits = None
+        self._frozen = None
         if getattr(rbm, "orbits", None) is not None:
             self._orbits = orbit_tables(rbm.orbits, dev)
+            fz = np.flatnonzero(np.asarray(rbm.orbits) < 0)
+            if fz.size:
+                self._frozen = torch.as_tensor(fz, device=dev)
         if optimizer not in ("adam", "sr"):
             raise ValueError(f"optimizer must be 'adam' or 'sr', got {optimizer!r}")
         self.optimizer = optimizer
@@ -100,12 +106,15 @@ class RBMTrainer:
         self._last_spins = None
 
     def tie(self, v):
-        """Orbit means of ``v`` in place (no-op for an untied guide)."""
+        """Orbit means of ``v`` in place, frozen entries zeroed (no-op for an
+        untied guide)."""
         if self._orbits is not None:
             off, mem, n_orb = self._orbits
             _lib.check(_lib.load().gfmc_orbit_mean(v.numel(), _lib.ptr(v), n_orb, _lib.ptr(off),
                                                    _lib.ptr(mem), _lib.stream()),
                        "gfmc_orbit_mean")
+            if self._frozen is not None:
+                v[self._frozen] = 0.0
 
     def _workspace(self, n):
         if n != self._work_n:

@@ -142,6 +142,37 @@ def test_translation_invariant_rbm_ties_its_parameters():
     assert len(np.unique(g.orbits)) == 3 * 16 + 3 + 1
 
 
+def test_rbm_filter_support_freezes_the_entries_outside_it():
+    """A filter support keeps the entries outside it at 0 in every hidden
+    unit (orbit -1, in no orbit table), and the orbit tables hold exactly the
+    live parameters."""
+    import torch
+
+    from pygfmc_amd import RBMSpinHalf
+    from pygfmc_amd.training import flat_params, orbit_tables
+    L = square(6, 6)
+    x, y = np.arange(36) % 6, np.arange(36) // 6
+    dx, dy = np.minimum(x, 6 - x), np.minimum(y, 6 - y)
+    sup = np.maximum(dx, dy) <= 1
+    g = RBMSpinHalf.from_jastrow(L.edges, L.extras["translations"], 0.2, 3, seed=4, scale=0.3,
+                                 support=sup)
+    T = L.extras["translations"]
+    for f in range(3):
+        for r in range(36):
+            row = g.W[f * 36 + r]
+            assert np.all(row[T[r][~sup]] == 0.0)
+    flat = flat_params(g)
+    frozen = g.orbits < 0
+    assert frozen.sum() == 3 * 36 * (36 - sup.sum())
+    assert np.all(flat[frozen] == 0.0)
+    off, mem, m = orbit_tables(g.orbits, "cpu")
+    assert m == 3 * int(sup.sum()) + 1 + 3
+    assert sorted(mem.numpy().tolist()) == np.flatnonzero(~frozen).tolist()
+    assert int(off[-1]) == int((~frozen).sum())
+    with pytest.raises(ValueError, match="support must hold"):
+        RBMSpinHalf.from_jastrow(L.edges, T, 0.2, 3, support=(dx + dy) == 0)
+
+
 def test_rbm_from_jastrow_is_the_jastrow_guide():
     """log psi_RBM - log psi_Jastrow is the same constant on every
     configuration (the BF16 rounding of the filter weight moves beta to
