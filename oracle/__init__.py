@@ -108,6 +108,8 @@ def lib():
         L.orc_mlp_tanh.argtypes = [ctypes.c_float]
         L.orc_bf16_flush.restype = ctypes.c_float
         L.orc_bf16_flush.argtypes = [ctypes.c_float]
+        L.orc_mlp_l1_grid.restype = ctypes.c_float
+        L.orc_mlp_l1_grid.argtypes = [ctypes.c_float]
         L.orc_mlp_hidden.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f32p]
         L.orc_mlp_O.argtypes = [D, ctypes.POINTER(ctypes.c_int8), _f64p]
         L.orc_mlp_vmin_grad.argtypes = [D, ctypes.c_int64, _u32p, ctypes.c_double, _f64p, _f64p]
@@ -403,6 +405,11 @@ def mlp_tanh(z):
 
 def bf16_flush(v):
     return np.array([lib().orc_bf16_flush(float(a)) for a in np.ravel(v)],
+                    dtype=np.float32).reshape(np.shape(v))
+
+
+def mlp_l1_grid(v):
+    return np.array([lib().orc_mlp_l1_grid(float(a)) for a in np.ravel(v)],
                     dtype=np.float32).reshape(np.shape(v))
 
 

@@ -285,8 +285,9 @@ static double rbm_log_ratio(const gfmc_problem_desc *d, const int8_t *x, const d
 /* wavefunction/cluster_mlp.py:133-439):                                      */
 /*   log psi = b3 + sum_c w3_c s(b2_c + sum_k W2_ck h_k),                     */
 /*   h_k = bf16(s(b1_k + sum_n W1_kn x_n)),                                   */
-/* in the library's arithmetic: W1, b1, W2 on the BF16 grid (nearest even,   */
-/* below 2^-24 flushed), layer-1 sums in FP64 (exact on that grid), s the    */
+/* in the library's arithmetic: W1 and b1 on the layer-1 grid (BF16 nearest  */
+/* even, magnitudes below 2^-9 flushed, above 64 clamped), W2 on the BF16    */
+/* grid (below 2^-24 flushed), layer-1 sums in FP64 (exact on that grid), s the */
 /* tabulated tanh in FP32 with an explicit fma, h rounded to BF16; the       */
 /* layer-2 sums in FP64 rounded once to FP32 (the device accumulates them in */
 /* FP32 on MFMA: the one step not mirrored bit for bit), then the output     */
@@ -337,7 +338,15 @@ float orc_bf16_flush(float v) {
     return fabsf(r) < 5.9604644775390625e-08f ? 0.0f : r;
 }
 
-/* The BF16-grid weights of the MLP (W1, b1, W2 after orc_bf16_flush), made
+/* the layer-1 grid of W1 and b1: BF16, |v| < 2^-9 -> 0, |v| > 64 -> +-64 */
+float orc_mlp_l1_grid(float v) {
+    float r = bf16_rne(v);
+    if (fabsf(r) < 0.001953125f) return 0.0f;
+    return fabsf(r) > 64.0f ? copysignf(64.0f, r) : r;
+}
+
+/* The grid weights of the MLP (W1, b1 after orc_mlp_l1_grid, W2 after
+ * orc_bf16_flush), made
  * once per configuration evaluation so the rows of its candidates do not
  * re-round them; the arithmetic is the same as rounding on the fly. */
 typedef struct {
@@ -355,8 +364,8 @@ static mlp_grid_t mlp_grid_fill(const gfmc_problem_desc *d, float *buf) {
     g.W1 = buf;
     g.b1 = buf + n1;
     g.W2 = buf + n1 + d->mlp_h1;
-    for (size_t q = 0; q < n1; ++q) g.W1[q] = orc_bf16_flush(d->mlp_W1[q]);
-    for (int k = 0; k < d->mlp_h1; ++k) g.b1[k] = orc_bf16_flush(d->mlp_b1[k]);
+    for (size_t q = 0; q < n1; ++q) g.W1[q] = orc_mlp_l1_grid(d->mlp_W1[q]);
+    for (int k = 0; k < d->mlp_h1; ++k) g.b1[k] = orc_mlp_l1_grid(d->mlp_b1[k]);
     for (size_t q = 0; q < n2; ++q) g.W2[q] = orc_bf16_flush(d->mlp_W2[q]);
     return g;
 }
@@ -1213,9 +1222,9 @@ static void mlp_O(const gfmc_problem_desc *d, const int8_t *x, double *O) {
     double *s1 = (double *)malloc(sizeof(double) * (size_t)H1);
     double *d2 = (double *)malloc(sizeof(double) * (size_t)H2);
     for (int k = 0; k < H1; ++k) {
-        double z = (double)orc_bf16_flush(d->mlp_b1[k]);
+        double z = (double)orc_mlp_l1_grid(d->mlp_b1[k]);
         for (int n = 0; n < N; ++n)
-            z += (double)x[n] * (double)orc_bf16_flush(d->mlp_W1[(size_t)k * N + n]);
+            z += (double)x[n] * (double)orc_mlp_l1_grid(d->mlp_W1[(size_t)k * N + n]);
         h[k] = bf16_rne(orc_mlp_tanh((float)z));
         s1[k] = mlp_slope_of((float)z);
     }

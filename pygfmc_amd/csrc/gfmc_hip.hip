@@ -170,11 +170,11 @@ struct DevProblem {
     const int *rbt_kcode;   // [N]      code(k)
     const int *rbt_qsite;   // [Q]      the site of entry q
     const int *rbt_qpair;   // [ncls Q] the partner site of entry q in class c, -1: none
-    // MLP guide (layer-1 sums exact in FP64, layer 2 on MFMA); rbm_wave_bytes
-    // is also the per-wavefront LDS of its walker kernels
+    // MLP guide (layer-1 sums exact integers at scale 2^16, layer 2 on MFMA);
+    // rbm_wave_bytes is also the per-wavefront LDS of its walker kernels
     int mlp_h1, mlp_h2;
-    const uint16_t *mlp_W1T;  // [N][h1] BF16 bits (site-major)
-    const double *mlp_b1;     // [h1] BF16-grid values
+    const int32_t *mlp_W1I;   // [N][h1] layer-1 grid values x 2^16 (site-major)
+    const double *mlp_b1;     // [h1] layer-1 grid values
     const uint16_t *mlp_W2;   // [h2][h1] BF16 bits
     const float *mlp_b2, *mlp_w3;
     const float *mlp_b3p;     // [1]
@@ -2565,6 +2565,20 @@ static uint16_t mlp_bf16_bits(float v) {
     return (uint16_t)r;
 }
 
+// v on the MLP's layer-1 grid (W1, b1): BF16 (round to nearest even) with
+// magnitudes below 2^-9 flushed to 0 and above 64 clamped to +-64, so that
+// v x 2^16 is an integer of magnitude at most 2^22
+static float mlp_l1_value(float v) {
+    uint32_t bits;
+    memcpy(&bits, &v, 4);
+    const uint32_t back = ((bits + 0x7fffu + ((bits >> 16) & 1u)) >> 16) << 16;
+    float f;
+    memcpy(&f, &back, 4);
+    if (std::fabs(f) < 0.001953125f) return 0.0f;
+    if (std::fabs(f) > 64.0f) return std::copysign(64.0f, f);
+    return f;
+}
+
 // diagonal forward walking (gfmc_diag.inl)
 struct gfmc_diag;
 static int diag_check(const char *fn, const gfmc_diag *d, const gfmc_problem *p, void *ring,
@@ -2617,6 +2631,10 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
                                      "(layer 2)");
         if (!d->mlp_W1 || !d->mlp_b1 || !d->mlp_W2 || !d->mlp_b2 || !d->mlp_w3 || !d->mlp_b3)
             return fail(GFMC_EINVAL, "gfmc_problem_create: incomplete MLP parameters");
+        if (d->n_sites > 510)
+            return fail(GFMC_EINVAL, "gfmc_problem_create: the MLP guide takes at most 510 "
+                                     "sites (its layer-1 sums are 32-bit integers at scale "
+                                     "2^16 of weights up to 64)");
     }
     if (d->guide != GFMC_GUIDE_NONE && d->guide != GFMC_GUIDE_JASTROW &&
         d->guide != GFMC_GUIDE_RBM && d->guide != GFMC_GUIDE_MLP)
@@ -2866,24 +2884,22 @@ int gfmc_problem_create(const gfmc_problem_desc *d, int32_t precision, gfmc_prob
         const int h1 = d->mlp_h1, h2 = d->mlp_h2;
         P.mlp_h1 = h1;
         P.mlp_h2 = h2;
-        std::vector<uint16_t> w1t((size_t)N * h1), w2((size_t)h2 * h1);
+        std::vector<int32_t> w1i((size_t)N * h1);
+        std::vector<uint16_t> w2((size_t)h2 * h1);
         std::vector<double> b1(h1);
         for (int k = 0; k < h1; ++k)
-            for (int n = 0; n < N; ++n) w1t[(size_t)n * h1 + k] = mlp_bf16_bits(d->mlp_W1[(size_t)k * N + n]);
+            for (int n = 0; n < N; ++n)
+                w1i[(size_t)n * h1 + k] =
+                    (int32_t)((double)mlp_l1_value(d->mlp_W1[(size_t)k * N + n]) * 65536.0);
         for (size_t i = 0; i < w2.size(); ++i) w2[i] = mlp_bf16_bits(d->mlp_W2[i]);
-        for (int k = 0; k < h1; ++k) {
-            const uint32_t bits = (uint32_t)mlp_bf16_bits(d->mlp_b1[k]) << 16;
-            float v;
-            memcpy(&v, &bits, 4);
-            b1[k] = (double)v;
-        }
+        for (int k = 0; k < h1; ++k) b1[k] = (double)mlp_l1_value(d->mlp_b1[k]);
         std::vector<float2> tab(MLP_NT);
         for (int j = 0; j < MLP_NT; ++j) {
             const float t0 = (float)std::tanh((double)j / 256.0);
             const float t1 = (float)std::tanh((double)(j + 1) / 256.0);
             tab[j] = make_float2(t0, t1 - t0);
         }
-        UP(w1t.data(), w1t.size(), &P.mlp_W1T);
+        UP(w1i.data(), w1i.size(), &P.mlp_W1I);
 
         UP(w2.data(), w2.size(), &P.mlp_W2);
         UP(b1.data(), b1.size(), &P.mlp_b1);

@@ -12,8 +12,10 @@
 //                 blocks; an exclusive scan gives each walker its blocks;
 //   k_mlp_gemm    one wavefront per 16-row block (one walker): the layer-1
 //                 activations of its rows are built in registers straight in
-//                 the MFMA A layout from z (FP64, exact: W1 and b1 lie on the
-//                 BF16 grid) and the two W1 columns of the row's move, W2 is
+//                 the MFMA A layout from z (exact integers at scale 2^16: W1
+//                 and b1 lie on the layer-1 grid, BF16 values of magnitude
+//                 2^-9 .. 64, so every z is a multiple of 2^-16 below 2^15)
+//                 and the two W1 columns of the row's move, W2 is
 //                 staged in LDS in 32-wide K chunks, v_mfma_f32_16x16x32_bf16
 //                 accumulates the 16 x h2 block, the epilogue applies s, w3
 //                 and a fixed-order row reduction;
@@ -37,13 +39,18 @@
 #define MLP_KC 32           // K chunk of W2 staged in LDS
 #define MLP_RB 16           // rows per block (one MFMA row tile)
 #define MLP_BSTR (MLP_KC + 8)
+#define MLP_ZSCALE 65536.0  // z1 holds z x 2^16 (exact: layer-1 grid)
+#define MLP_SAT (1u << 19)  // |z| x 2^16 at |z| = 8, where s saturates
+// LDS of the GEMM kernels before W2: the FP32-route table, then the integer
+// route's 2049 entries (padded to 2050)
+#define MLP_TABS_BYTES ((2 * MLP_NT + 2) * 8)
 // MLP_SW (4): walker wavefronts per start / select workgroup
 
 typedef __attribute__((ext_vector_type(8))) __bf16 mlp_bf16x8;
 typedef __attribute__((ext_vector_type(4))) float mlp_f32x4;
 
 struct MlpWs {
-    double *z1;          // [cap][h1]   layer-1 pre-activations of the current config
+    int32_t *z1;         // [cap][h1]   layer-1 pre-activations x 2^16 of the current config
     int32_t *cand;       // [cap][n_off] candidate slots, ascending
     int32_t *cnt;        // [cap]       rows: 1 + candidates
     int32_t *active;     // [cap]
@@ -116,32 +123,37 @@ __device__ int mlp_candidates(const DevProblem &P, const Tables<float> &tb, cons
     return base;
 }
 
-// z = b1 + W1 x from scratch, eight consecutive hidden units per lane
-__device__ void mlp_z_init(const DevProblem &P, const signed char *x, double *z1, int lane) {
+// z = b1 + W1 x from scratch (x 2^16, integer sums: exact in any order),
+// eight consecutive hidden units per lane
+__device__ void mlp_z_init(const DevProblem &P, const signed char *x, int32_t *z1, int lane) {
     for (int e0 = lane * 8; e0 < P.mlp_h1; e0 += WAVE * 8) {
-        double z[8];
+        int32_t z[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) z[q] = P.mlp_b1[e0 + q];
+        for (int q = 0; q < 8; ++q) z[q] = (int32_t)(P.mlp_b1[e0 + q] * MLP_ZSCALE);
         for (int n = 0; n < P.N; ++n) {
-            const uint4 v = *(const uint4 *)(P.mlp_W1T + (size_t)n * P.mlp_h1 + e0);
-            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
-            const double xn = (double)x[n];
+            const int4 a = *(const int4 *)(P.mlp_W1I + (size_t)n * P.mlp_h1 + e0);
+            const int4 b = *(const int4 *)(P.mlp_W1I + (size_t)n * P.mlp_h1 + e0 + 4);
+            const int32_t wv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            if (x[n] > 0) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                z[q] += xn * (double)bf16_val((uint16_t)(wv[q >> 1] >> ((q & 1) * 16)));
+                for (int q = 0; q < 8; ++q) z[q] += wv[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) z[q] -= wv[q];
+            }
         }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) z1[e0 + q] = z[q];
+        *(int4 *)(z1 + e0) = make_int4(z[0], z[1], z[2], z[3]);
+        *(int4 *)(z1 + e0 + 4) = make_int4(z[4], z[5], z[6], z[7]);
     }
 }
 
 // z += the change of flipping site i (j >= 0 too) at the pre-flip x
-__device__ void mlp_z_move(const DevProblem &P, const signed char *x, double *z1, int i, int j,
+__device__ void mlp_z_move(const DevProblem &P, const signed char *x, int32_t *z1, int i, int j,
                            int lane) {
-    const double ci = -2.0 * (double)x[i], cj = j >= 0 ? -2.0 * (double)x[j] : 0.0;
+    const int32_t ci = -2 * (int32_t)x[i], cj = j >= 0 ? -2 * (int32_t)x[j] : 0;
     for (int e = lane; e < P.mlp_h1; e += WAVE) {
-        double z = z1[e] + ci * (double)bf16_val(P.mlp_W1T[(size_t)i * P.mlp_h1 + e]);
-        if (j >= 0) z += cj * (double)bf16_val(P.mlp_W1T[(size_t)j * P.mlp_h1 + e]);
+        int32_t z = z1[e] + ci * P.mlp_W1I[(size_t)i * P.mlp_h1 + e];
+        if (j >= 0) z += cj * P.mlp_W1I[(size_t)j * P.mlp_h1 + e];
         z1[e] = z;
     }
 }
@@ -251,65 +263,105 @@ __global__ __launch_bounds__(1024) void k_mlp_scan3(int64_t nw, MlpWs W) {
 // eight layer-1 activations of one row at hidden units k0 .. k0 + 7 from the
 // prefetched z (FP64) and W1 columns (BF16) of the row's move (ci, cj: -2 x
 // of the flipped sites, 0 when absent)
-struct MlpAIn {
-    double z[8];
-    uint4 wi, wj;
-};
-
-__device__ __forceinline__ void load8d(double *o, const double *p) {
-    const double4 a = ((const double4 *)p)[0], b = ((const double4 *)p)[1];
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+// c * w + z with 24-bit signed operands c and w, the sum wrapping (v_mad_i32_i24;
+// written out because the compiler turns a chain of two into two multiplies
+// and a three-operand add)
+__device__ __forceinline__ int32_t mad_i24(int32_t c, int32_t w, int32_t z) {
+    int32_t r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(c), "v"(w), "v"(z));
+    return r;
 }
 
+struct MlpAIn {
+    int4 z0, z1, i0, i1, j0, j1;   // z x 2^16 and the move's two W1 columns x 2^16
+};
+
 // rows past the walker's count load zeros: z = 0 gives the activation 0
-__device__ __forceinline__ void mlp_a_load(MlpAIn &in, const double *z1, const uint16_t *w1i,
-                                           const uint16_t *w1j, int k0, bool rvalid) {
+__device__ __forceinline__ void mlp_a_load(MlpAIn &in, const int32_t *z1, const int32_t *w1i,
+                                           const int32_t *w1j, int k0, bool rvalid) {
     if (rvalid) {
-        load8d(in.z, z1 + k0);
-        in.wi = *(const uint4 *)(w1i + k0);
-        in.wj = *(const uint4 *)(w1j + k0);
+        in.z0 = *(const int4 *)(z1 + k0);
+        in.z1 = *(const int4 *)(z1 + k0 + 4);
+        in.i0 = *(const int4 *)(w1i + k0);
+        in.i1 = *(const int4 *)(w1i + k0 + 4);
+        in.j0 = *(const int4 *)(w1j + k0);
+        in.j1 = *(const int4 *)(w1j + k0 + 4);
     } else {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) in.z[q] = 0.0;
-        in.wi = make_uint4(0, 0, 0, 0);
-        in.wj = make_uint4(0, 0, 0, 0);
+        in.z0 = in.z1 = in.i0 = in.i1 = in.j0 = in.j1 = make_int4(0, 0, 0, 0);
     }
 }
 
-__device__ __forceinline__ double bf16_pair(const uint4 &v, int q) {
-    const uint32_t w = q < 2 ? v.x : q < 4 ? v.y : q < 6 ? v.z : v.w;
-    return (double)__uint_as_float((q & 1) ? (w & 0xffff0000u) : (w << 16));
+// The table of the integer route (staged in LDS after the FP32 one): entry
+// j < 2048 is (T_j, D_j / 256), entry 2048 the saturated value
+// (T_2047 + (1 - 2^-13) D_2047, 0).  With a = min(|z| 2^16, 2^19) the
+// activation is T_j + f D_j / 256 at j = a >> 8, f = a & 255: t = 256 |z| =
+// a / 256 exactly (z is a multiple of 2^-16 below 2^15, so the FP32 z of the
+// FP32 route is exact wherever s does not saturate), floor(t) = j and
+// fract(t) = f / 256, and fma(f, D / 256, T) rounds the same real number as
+// fma(f / 256, D, T): bit for bit the value of mlp_tanh(z).
+__device__ __forceinline__ void mlp_stage_tabi(const float2 *tab, float2 *tabi) {
+    for (int i = threadIdx.x; i < MLP_NT; i += blockDim.x)
+        tabi[i] = make_float2(tab[i].x, tab[i].y * 0.00390625f);
+    if (threadIdx.x == 0) {
+        const float2 e = tab[MLP_NT - 1];
+        tabi[MLP_NT] = make_float2(__builtin_fmaf(MLP_TMAX - (float)(MLP_NT - 1), e.y, e.x), 0.0f);
+    }
 }
 
-// the eight activations; the layer-1 sums stay exact (FP64 fma of exact
-// terms), the eight table reads are issued before any is used
-__device__ __forceinline__ mlp_bf16x8 mlp_a_build(const MlpAIn &in, double ci, double cj,
-                                                  const float2 *tab, uint4 *jout = nullptr) {
-    float t[8], zf[8];
+// the eight activations from the exact integer sums; the eight table reads
+// are issued before any is used.  jout: the table index per activation (bit
+// 15: saturated), for the slopes of the training rows
+__device__ __forceinline__ mlp_bf16x8 mlp_a_build8(const int4 &za, const int4 &zb, const int4 &ia,
+                                                   const int4 &ib, const int4 &ja, const int4 &jb,
+                                                   int ci, int cj, const float2 *tabi,
+                                                   uint4 *jout = nullptr) {
+    const int32_t z0[8] = {za.x, za.y, za.z, za.w, zb.x, zb.y, zb.z, zb.w};
+    const int32_t wi[8] = {ia.x, ia.y, ia.z, ia.w, ib.x, ib.y, ib.z, ib.w};
+    const int32_t wj[8] = {ja.x, ja.y, ja.z, ja.w, jb.x, jb.y, jb.z, jb.w};
+    int32_t z[8];
+    uint32_t a[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const double z = __builtin_fma(cj, bf16_pair(in.wj, q),
-                                       __builtin_fma(ci, bf16_pair(in.wi, q), in.z[q]));
-        zf[q] = (float)z;
-        t[q] = fminf(fabsf(zf[q]) * MLP_TSCALE, MLP_TMAX);
+        // |ci|, |cj| <= 2 and |w| < 2^23: two 24-bit multiply-adds (the sum
+        // wraps for a padding row, which may hold any value; an exact z never does)
+        z[q] = mad_i24(cj, wj[q], mad_i24(ci, wi[q], z0[q]));
+        a[q] = min(z[q] < 0 ? 0u - (uint32_t)z[q] : (uint32_t)z[q], MLP_SAT);
     }
     float2 e[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) e[q] = tab[(int)t[q]];
-    mlp_bf16x8 a;
+    for (int q = 0; q < 8; ++q) e[q] = tabi[__builtin_amdgcn_ubfe(a[q], 8, 12)];
+    mlp_bf16x8 r;
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-        a[q] = (__bf16)copysignf(__builtin_fmaf(__builtin_amdgcn_fractf(t[q]), e[q].y, e[q].x),
-                                 zf[q]);
+    for (int q = 0; q < 8; ++q) {
+        const float v = __builtin_fmaf((float)(a[q] & 255u), e[q].y, e[q].x);
+        r[q] = (__bf16)copysignf(v, __int_as_float(z[q]));   // the sign bit of z
+    }
     if (jout) {
         uint32_t jj[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) jj[q] = (uint32_t)(int)t[q] | (t[q] == MLP_TMAX ? 0x8000u : 0u);
+        for (int q = 0; q < 8; ++q) jj[q] = (a[q] >> 8) | (a[q] >= MLP_SAT ? 0x8000u : 0u);
         *jout = make_uint4(jj[0] | (jj[1] << 16), jj[2] | (jj[3] << 16), jj[4] | (jj[5] << 16),
                            jj[6] | (jj[7] << 16));
     }
-    return a;
+    return r;
+}
+
+__device__ __forceinline__ mlp_bf16x8 mlp_a_build(const MlpAIn &in, int ci, int cj,
+                                                  const float2 *tabi, uint4 *jout = nullptr) {
+    return mlp_a_build8(in.z0, in.z1, in.i0, in.i1, in.j0, in.j1, ci, cj, tabi, jout);
+}
+
+// the move's two W1 columns (x 2^16) for one K step of the 32-row kernel
+struct MlpWIn {
+    int4 i0, i1, j0, j1;
+};
+
+__device__ __forceinline__ void mlp_w_load(MlpWIn &in, const int32_t *w1i, const int32_t *w1j,
+                                           int k0) {
+    in.i0 = *(const int4 *)(w1i + k0);
+    in.i1 = *(const int4 *)(w1i + k0 + 4);
+    in.j0 = *(const int4 *)(w1j + k0);
+    in.j1 = *(const int4 *)(w1j + k0 + 4);
 }
 
 // the slope of the tabulated tanh at a table index (bit 15: saturated, 0)
@@ -335,9 +387,11 @@ void k_mlp_gemm(DevProblem P, const uint32_t *spins, MlpWs W, MlpTws T) {
     constexpr int H2 = NC * 16;
     constexpr int CH = H2 * (MLP_KC / 8);                 // uint4 per chunk
     constexpr int PER = (CH + MLP_GW * WAVE - 1) / (MLP_GW * WAVE);
-    uint16_t *Bs0 = (uint16_t *)(lds + MLP_NT * sizeof(float2));
+    float2 *tabi = tab + MLP_NT;
+    uint16_t *Bs0 = (uint16_t *)(lds + MLP_TABS_BYTES);
     uint16_t *Bs1 = Bs0 + H2 * MLP_BSTR;
     for (int i = threadIdx.x; i < MLP_NT; i += blockDim.x) tab[i] = P.mlp_tab[i];
+    mlp_stage_tabi(P.mlp_tab, tabi);
     const int total = W.total[0];
     const int tiles = (total + MLP_GW - 1) / MLP_GW;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -359,16 +413,16 @@ void k_mlp_gemm(DevProblem P, const uint32_t *spins, MlpWs W, MlpTws T) {
         const int row = r0 + col;
         const bool rvalid = bvalid && row < cnt;
         int si = -1, sj = -1;
-        double ci = 0.0, cj = 0.0;
+        int ci = 0, cj = 0;
         if (rvalid && row > 0) {
             slot_sites(P, W.cand[(size_t)w * P.n_off + row - 1], &si, &sj);
             const uint32_t *sp = spins + w * P.words;
-            ci = ((sp[si >> 5] >> (si & 31)) & 1u) ? -2.0 : 2.0;
-            if (sj >= 0) cj = ((sp[sj >> 5] >> (sj & 31)) & 1u) ? -2.0 : 2.0;
+            ci = ((sp[si >> 5] >> (si & 31)) & 1u) ? -2 : 2;
+            if (sj >= 0) cj = ((sp[sj >> 5] >> (sj & 31)) & 1u) ? -2 : 2;
         }
-        const double *z1 = W.z1 + (size_t)w * H1;
-        const uint16_t *w1i = P.mlp_W1T + (size_t)(si < 0 ? 0 : si) * H1;
-        const uint16_t *w1j = P.mlp_W1T + (size_t)(sj < 0 ? 0 : sj) * H1;
+        const int32_t *z1 = W.z1 + (size_t)w * H1;
+        const int32_t *w1i = P.mlp_W1I + (size_t)(si < 0 ? 0 : si) * H1;
+        const int32_t *w1j = P.mlp_W1I + (size_t)(sj < 0 ? 0 : sj) * H1;
         const size_t grow = (size_t)b * MLP_RB + col;   // this lane's row (TRAIN stores)
         if (TRAIN && bvalid && quad == 0)
             T.rinfo[grow] = make_int2((int)w, (si & 0xffff) | ((sj & 0xffff) << 16));
@@ -414,7 +468,7 @@ void k_mlp_gemm(DevProblem P, const uint32_t *spins, MlpWs W, MlpTws T) {
             for (int ks = 0; ks < MLP_KC; ks += 32) {
                 // build this K step's operand, then start the next step's loads
                 uint4 jv;
-                const mlp_bf16x8 a = mlp_a_build(ain, ci, cj, tab, TRAIN ? &jv : nullptr);
+                const mlp_bf16x8 a = mlp_a_build(ain, ci, cj, tabi, TRAIN ? &jv : nullptr);
                 if (TRAIN && bvalid) {
                     const size_t off = grow * H1 + c * MLP_KC + ks + 8 * quad;
                     *(mlp_bf16x8 *)(T.Hb + off) = a;
@@ -525,24 +579,35 @@ void k_mlp_gemm(DevProblem P, const uint32_t *spins, MlpWs W, MlpTws T) {
 #define MLP_GW2 8
 typedef __attribute__((ext_vector_type(16))) float mlp_f32x16;
 
-template <int NT>
+// Operands of a row: z of its walker (ZLDS: the two blocks' z rows copied to
+// LDS once per tile, read at each K step; otherwise read from global memory
+// at each step) and the two W1 columns of its move, loaded two K steps ahead.
+// Every lane loads: a padding row reads its walker's z and W1 row 0 (walker
+// 0 for a block past the count), all in bounds, finite activations that reach
+// only that row's MFMA outputs, which are never stored.
+template <int NT, bool ZLDS>
 __global__ __launch_bounds__(MLP_GW2 * WAVE) __attribute__((amdgpu_waves_per_eu(2)))
 void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
+    // the tables in static LDS (addresses known when compiling: the table
+    // reads take their offset as an immediate), W2's buffers in the dynamic part
+    __shared__ float2 tab[MLP_NT];
+    __shared__ float2 tabi[MLP_NT + 2];
     extern __shared__ __align__(16) unsigned char lds[];
-    float2 *tab = (float2 *)lds;
     constexpr int H2 = NT * 32;
     constexpr int CH = H2 * (MLP_KC / 8);                 // uint4 per chunk
     constexpr int TPB = MLP_GW2 * WAVE;
     constexpr int PER = (CH + TPB - 1) / TPB;
     static_assert(PER <= 4, "W2 chunk prefetch holds four uint4 per thread");
-    uint16_t *Bs0 = (uint16_t *)(lds + MLP_NT * sizeof(float2));
+    uint16_t *Bs0 = (uint16_t *)lds;
     uint16_t *Bs1 = Bs0 + H2 * MLP_BSTR;
     for (int i = threadIdx.x; i < MLP_NT; i += blockDim.x) tab[i] = P.mlp_tab[i];
+    mlp_stage_tabi(P.mlp_tab, tabi);
     const int total = W.total[0];
     const int pairs = (total + 1) >> 1;
     const int tiles = (pairs + MLP_GW2 - 1) / MLP_GW2;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int H1 = P.mlp_h1, nchunks = H1 / MLP_KC;
+    int32_t *zs = (int32_t *)(Bs1 + H2 * MLP_BSTR) + (size_t)wave * 2 * H1;   // ZLDS: [2][H1]
     const int kh = lane >> 5, r32 = lane & 31;
     const int jq = threadIdx.x / (MLP_KC / 8), qq = threadIdx.x % (MLP_KC / 8);
     const int jstep = TPB / (MLP_KC / 8);
@@ -560,16 +625,27 @@ void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
         const int row = r0 + (r32 & 15);
         const bool rvalid = bvalid && row < cnt;
         int si = -1, sj = -1;
-        double ci = 0.0, cj = 0.0;
+        int ci = 0, cj = 0;
         if (rvalid && row > 0) {
             slot_sites(P, W.cand[(size_t)w * P.n_off + row - 1], &si, &sj);
             const uint32_t *sp = spins + w * P.words;
-            ci = ((sp[si >> 5] >> (si & 31)) & 1u) ? -2.0 : 2.0;
-            if (sj >= 0) cj = ((sp[sj >> 5] >> (sj & 31)) & 1u) ? -2.0 : 2.0;
+            ci = ((sp[si >> 5] >> (si & 31)) & 1u) ? -2 : 2;
+            if (sj >= 0) cj = ((sp[sj >> 5] >> (sj & 31)) & 1u) ? -2 : 2;
         }
-        const double *z1 = W.z1 + (size_t)w * H1;
-        const uint16_t *w1i = P.mlp_W1T + (size_t)(si < 0 ? 0 : si) * H1;
-        const uint16_t *w1j = P.mlp_W1T + (size_t)(sj < 0 ? 0 : sj) * H1;
+        const int32_t *w1i = P.mlp_W1I + (size_t)(si < 0 ? 0 : si) * H1;
+        const int32_t *w1j = P.mlp_W1I + (size_t)(sj < 0 ? 0 : sj) * H1;
+        if constexpr (ZLDS) {
+            // the z rows of the wave's two blocks (their walkers in lanes 0 and 16);
+            // the previous tile's reads of zs came earlier in this wave's order
+            const int wa = __shfl((int)w, 0, WAVE), wb = __shfl((int)w, 16, WAVE);
+            for (int i = lane * 4; i < 2 * H1; i += WAVE * 4) {
+                const int bh = i >= H1 ? 1 : 0;
+                *(int4 *)(zs + i) =
+                    *(const int4 *)(W.z1 + (size_t)(bh ? wb : wa) * H1 + (i - bh * H1));
+            }
+        }
+        const int32_t *zg = W.z1 + (size_t)w * H1;          // !ZLDS: z in global memory
+        const int32_t *zl = zs + (r32 >> 4) * H1;           // ZLDS: this lane's block's row
         mlp_f32x16 acc[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
@@ -584,10 +660,35 @@ void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
                     *(const uint4 *)(P.mlp_W2 + (size_t)j * H1 + q * 8);
             }
         }
-        MlpAIn ain;
-        mlp_a_load(ain, z1, w1i, w1j, 8 * kh, rvalid);
+        // the W1 columns of K steps 0 and 1 (H1 >= 64)
+        MlpWIn wa, wb;
+        mlp_w_load(wa, w1i, w1j, 8 * kh);
+        mlp_w_load(wb, w1i, w1j, 16 + 8 * kh);
         __syncthreads();
         uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
+        // one K step: every B fragment in flight while the activations are
+        // built, the W1 columns of the step two ahead requested, then the MFMAs
+        auto kstep = [&](const uint16_t *Bc, int k, MlpWIn &wv) {
+            mlp_bf16x8 bv[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                bv[t] = *(const mlp_bf16x8 *)(Bc + (t * 32 + r32) * MLP_BSTR + (k % MLP_KC) +
+                                              8 * kh);
+            int4 za, zb;
+            if constexpr (ZLDS) {
+                za = *(const int4 *)(zl + k + 8 * kh);
+                zb = *(const int4 *)(zl + k + 8 * kh + 4);
+            } else {
+                za = *(const int4 *)(zg + k + 8 * kh);
+                zb = *(const int4 *)(zg + k + 8 * kh + 4);
+            }
+            const mlp_bf16x8 a = mlp_a_build8(za, zb, wv.i0, wv.i1, wv.j0, wv.j1, ci, cj, tabi);
+            // H1 is a multiple of 64: step k + 32 exists for both halves or neither
+            if (k + 32 < H1) mlp_w_load(wv, w1i, w1j, k + 32 + 8 * kh);
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv[t], acc[t], 0, 0, 0);
+        };
         for (int c = 0; c < nchunks; ++c) {
             const uint16_t *Bc = (c & 1) ? Bs1 : Bs0;
             uint16_t *Bn = (c & 1) ? Bs0 : Bs1;
@@ -602,21 +703,9 @@ void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
                 if (PER > 3 && (int)threadIdx.x + 3 * TPB < CH)
                     pre3 = *(const uint4 *)(src + (size_t)3 * jstep * H1);
             }
-#pragma unroll
-            for (int ks = 0; ks < MLP_KC; ks += 16) {
-                // every B fragment of the step in flight while the activations
-                // are built, then the MFMAs back to back
-                mlp_bf16x8 bv[NT];
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    bv[t] = *(const mlp_bf16x8 *)(Bc + (t * 32 + r32) * MLP_BSTR + ks + 8 * kh);
-                const mlp_bf16x8 a = mlp_a_build(ain, ci, cj, tab);
-                const int knext = c * MLP_KC + ks + 16 + 8 * kh;
-                if (knext < H1) mlp_a_load(ain, z1, w1i, w1j, knext, rvalid);
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv[t], acc[t], 0, 0, 0);
-            }
+            static_assert(MLP_KC == 32, "two K steps per W2 chunk");
+            kstep(Bc, c * MLP_KC, wa);
+            kstep(Bc, c * MLP_KC + 16, wb);
             if (more) {
                 uint16_t *dst = Bn + jq * MLP_BSTR + qq * 8;
                 if ((int)threadIdx.x < CH) *(uint4 *)dst = pre0;
@@ -828,7 +917,7 @@ static void free_mlp_work(MlpWork *m) {
 // pieces in the order of MlpWs; returns the total bytes (offsets into off)
 static size_t mlp_work_layout(const DevProblem &P, int bpw, int64_t nw, size_t *off) {
     const size_t n = (size_t)nw, nb = n * (size_t)bpw;
-    const size_t sizes[14] = {n * P.mlp_h1 * 8, n * (size_t)(P.n_off > 0 ? P.n_off : 1) * 4,
+    const size_t sizes[14] = {n * P.mlp_h1 * 4, n * (size_t)(P.n_off > 0 ? P.n_off : 1) * 4,
                               n * 4, n * 4, n * 8, n * 8, n * 8, n * 4, n * 4, nb * 4,
                               (n / 1024 + 2) * 4, 16, nb * MLP_RB * 4, 40};
     size_t o = 0;
@@ -850,7 +939,7 @@ static int mlp_reserve(const gfmc_problem *p, int64_t nw) {
 static size_t mlp_walker_lds(const gfmc_problem *p) { return (size_t)MLP_SW * p->dp.rbm_wave_bytes; }
 
 static size_t mlp_gemm_lds(const gfmc_problem *p) {
-    return (size_t)MLP_NT * sizeof(float2) + 2 * (size_t)p->dp.mlp_h2 * MLP_BSTR * 2;
+    return (size_t)MLP_TABS_BYTES + 2 * (size_t)p->dp.mlp_h2 * MLP_BSTR * 2;
 }
 
 static unsigned mlp_walker_grid(const gfmc_problem *p, int64_t nw) {
@@ -867,15 +956,27 @@ static unsigned launch_mlp_gemm(const gfmc_problem *p, const uint32_t *spins, hi
     const size_t lds = mlp_gemm_lds(p);
     const MlpWs &W = p->mlp->ws;
     if (!TRAIN) {
-        // the propagation's and evaluations' rows: 32-row tiles
+        // the propagation's and evaluations' rows: 32-row tiles (tables in static
+        // LDS); the walkers' z rows in LDS as well when they fit beside W2's buffers
+        const size_t zbytes = (size_t)MLP_GW2 * 2 * p->dp.mlp_h1 * 4;
+        const bool zlds = lds + zbytes <= 160 * 1024;
+        const size_t lds32 = lds - MLP_TABS_BYTES + (zlds ? zbytes : 0);
         const unsigned grid = (unsigned)(p->n_cu * 2);
         const dim3 g(grid), b(MLP_GW2 * WAVE);
+#define MLP_G32(nt)                                                                          \
+    do {                                                                                     \
+        if (zlds)                                                                            \
+            hipLaunchKernelGGL((k_mlp_gemm32<nt, true>), g, b, lds32, st, p->dp, spins, W);  \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_mlp_gemm32<nt, false>), g, b, lds32, st, p->dp, spins, W); \
+    } while (0)
         switch (p->dp.mlp_h2) {
-        case 32: hipLaunchKernelGGL(k_mlp_gemm32<1>, g, b, lds, st, p->dp, spins, W); break;
-        case 64: hipLaunchKernelGGL(k_mlp_gemm32<2>, g, b, lds, st, p->dp, spins, W); break;
-        case 128: hipLaunchKernelGGL(k_mlp_gemm32<4>, g, b, lds, st, p->dp, spins, W); break;
-        default: hipLaunchKernelGGL(k_mlp_gemm32<8>, g, b, lds, st, p->dp, spins, W); break;
+        case 32: MLP_G32(1); break;
+        case 64: MLP_G32(2); break;
+        case 128: MLP_G32(4); break;
+        default: MLP_G32(8); break;
         }
+#undef MLP_G32
         return grid;
     }
     const unsigned grid = (unsigned)(p->n_cu * 4);
@@ -1046,7 +1147,7 @@ extern "C" int gfmc_mlp_set_work(gfmc_problem *p, void *work, int64_t bytes, int
     mlp_work_layout(p->dp, p->mlp->bpw, n_walkers, off);
     unsigned char *b = (unsigned char *)work;
     MlpWs &W = p->mlp->ws;
-    W.z1 = (double *)(b + off[0]);
+    W.z1 = (int32_t *)(b + off[0]);
     W.cand = (int32_t *)(b + off[1]);
     W.cnt = (int32_t *)(b + off[2]);
     W.active = (int32_t *)(b + off[3]);

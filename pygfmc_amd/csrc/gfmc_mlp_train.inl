@@ -238,9 +238,11 @@ __global__ __launch_bounds__(256) void k_mlp_greduce(int N, int h1, int h2, int 
     grad[idx] = s;
 }
 
-// FP32 parameters -> the problem's BF16 / FP64 / FP32 tables
+// FP32 parameters -> the problem's tables: W1 and b1 on the layer-1 grid
+// (BF16, magnitudes below 2^-9 flushed, above 64 clamped; W1 as integers x
+// 2^16), W2 on the BF16 grid (below 2^-24 flushed), b2, w3, b3 in FP32
 __global__ __launch_bounds__(256) void k_mlp_pack(int N, int h1, int h2, const float *flat,
-                                                  uint16_t *W1T, double *b1, uint16_t *W2,
+                                                  int32_t *W1I, double *b1, uint16_t *W2,
                                                   float *b2, float *w3, float *b3) {
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t nW1 = (int64_t)h1 * N, nW2 = (int64_t)h2 * h1;
@@ -252,12 +254,18 @@ __global__ __launch_bounds__(256) void k_mlp_pack(int N, int h1, int h2, const f
         if (fabsf(__uint_as_float(r << 16)) < 5.9604644775390625e-08f) r = 0;
         return (uint16_t)r;
     };
+    auto l1 = [](float v) -> float {
+        const uint32_t b = __float_as_uint(v);
+        const float r = __uint_as_float(((b + 0x7fffu + ((b >> 16) & 1u)) >> 16) << 16);
+        if (fabsf(r) < 0.001953125f) return 0.0f;
+        return fabsf(r) > 64.0f ? copysignf(64.0f, r) : r;
+    };
     const float v = flat[idx];
     if (idx < nW1) {
         const int64_t k = idx / N, n = idx % N;
-        W1T[n * h1 + k] = bf(v);
+        W1I[n * h1 + k] = (int32_t)((double)l1(v) * 65536.0);
     } else if (idx < nW1 + h1) {
-        b1[idx - nW1] = (double)__uint_as_float((uint32_t)bf(v) << 16);
+        b1[idx - nW1] = (double)l1(v);
     } else if (idx < nW1 + h1 + nW2) {
         W2[idx - nW1 - h1] = bf(v);
     } else if (idx < nW1 + h1 + nW2 + h2) {
@@ -364,7 +372,7 @@ int gfmc_mlp_set_params(gfmc_problem *p, const float *flat, void *stream) {
     const DevProblem &P = p->dp;
     const int64_t total = gfmc_mlp_n_params(p);
     hipLaunchKernelGGL(k_mlp_pack, dim3(grid1d(total, 256)), dim3(256), 0, (hipStream_t)stream,
-                       P.N, P.mlp_h1, P.mlp_h2, flat, (uint16_t *)P.mlp_W1T, (double *)P.mlp_b1,
+                       P.N, P.mlp_h1, P.mlp_h2, flat, (int32_t *)P.mlp_W1I, (double *)P.mlp_b1,
                        (uint16_t *)P.mlp_W2, (float *)P.mlp_b2, (float *)P.mlp_w3,
                        (float *)P.mlp_b3p);
     HIP_TRY(hipGetLastError());

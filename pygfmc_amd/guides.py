@@ -295,6 +295,16 @@ def bf16_flush(a):
     return np.where(np.abs(r) < np.float32(2.0 ** -24), np.float32(0.0), r).astype(np.float32)
 
 
+def mlp_l1_grid(a):
+    """The MLP's layer-1 grid (W1, b1): BF16 rounding (nearest even) with
+    magnitudes below 2^-9 flushed to 0 and above 64 clamped to +-64, so every
+    value is an integer multiple of 2^-16 of magnitude at most 2^6 and the
+    layer-1 sums of up to 510 sites are exact 32-bit integers at scale 2^16."""
+    r = bf16_round(a)
+    r = np.where(np.abs(r) < np.float32(2.0 ** -9), np.float32(0.0), r)
+    return np.clip(r, np.float32(-64.0), np.float32(64.0)).astype(np.float32)
+
+
 def mlp_tanh_table():
     """(T, D) of the tabulated tanh: T_j = tanh(j / 256) and D_j = T_{j+1} - T_j
     in FP32, j = 0..2047."""
@@ -311,15 +321,18 @@ class MLPSpinHalf:
         h_k = bf16(s(b1_k + sum_n W1_kn x_n)),
 
     ``s`` the tabulated tanh (2048 linear pieces on [0, 8), odd, saturating
-    at tanh(8)), ``bf16`` rounding to nearest even.  W1, b1 and W2 are held on
-    the BF16 grid (magnitudes below 2^-24 flushed to 0); b2, w3 and b3 are
-    FP32.  ``hidden = (h1, h2)``: h1 a multiple of 64, h2 one of 32, 64, 128,
-    256.
+    at tanh(8)), ``bf16`` rounding to nearest even.  W1 and b1 are held on the
+    layer-1 grid (:func:`mlp_l1_grid`: BF16, magnitudes below 2^-9 flushed to
+    0, above 64 clamped), W2 on the BF16 grid (magnitudes below 2^-24
+    flushed); b2, w3 and b3 are FP32.  ``hidden = (h1, h2)``: h1 a multiple of
+    64, h2 one of 32, 64, 128, 256; at most 510 sites.
     """
 
     def __init__(self, N, hidden=(512, 256), W1=None, b1=None, W2=None, b2=None, w3=None,
                  b3=0.0, seed=0, scale=1.0, out_scale=0.1, dtype=np.float32):
         self.N = int(N)
+        if not 1 <= self.N <= 510:
+            raise ValueError(f"the MLP guide takes 1 to 510 sites, got N={self.N}")
         h1, h2 = (int(hidden[0]), int(hidden[1]))
         if h1 <= 0 or h1 % 64 or h2 not in (32, 64, 128, 256):
             raise ValueError(f"hidden must be (multiple of 64, one of 32/64/128/256); got "
@@ -332,9 +345,9 @@ class MLPSpinHalf:
             W2 = rng.normal(scale=scale / np.sqrt(h1), size=(h2, h1))
         if w3 is None:
             w3 = rng.normal(scale=out_scale / np.sqrt(h2), size=h2)
-        self.W1 = bf16_flush(np.asarray(W1, dtype=np.float32).reshape(h1, self.N))
+        self.W1 = mlp_l1_grid(np.asarray(W1, dtype=np.float32).reshape(h1, self.N))
         self.W2 = bf16_flush(np.asarray(W2, dtype=np.float32).reshape(h2, h1))
-        self.b1 = bf16_flush(np.zeros(h1) if b1 is None else np.asarray(b1, dtype=np.float32))
+        self.b1 = mlp_l1_grid(np.zeros(h1) if b1 is None else np.asarray(b1, dtype=np.float32))
         self.b2 = np.zeros(h2, np.float32) if b2 is None else np.asarray(b2, np.float32).copy()
         self.w3 = np.asarray(w3, dtype=np.float32).reshape(h2).copy()
         self.b3 = np.float32(b3)

@@ -6,7 +6,7 @@ import pytest
 
 from helpers import basis, index_of, spinhalf_matrix
 from pygfmc_amd import MLPSpinHalf, SpinHalf, chain, kagome
-from pygfmc_amd.guides import bf16_flush, mlp_tanh_table
+from pygfmc_amd.guides import bf16_flush, mlp_l1_grid, mlp_tanh_table
 from pygfmc_amd.problem import lower
 
 
@@ -23,6 +23,54 @@ def test_tabulated_tanh(orc):
     T2, D2 = mlp_tanh_table()
     np.testing.assert_array_equal(T, T2)
     np.testing.assert_array_equal(D, D2)
+
+
+def test_layer1_grid_is_the_oracles_and_integral_at_scale_2_16(orc):
+    """W1 and b1 live on the layer-1 grid: BF16 values with magnitudes below
+    2^-9 flushed and above 64 clamped; every grid value times 2^16 is an
+    integer of magnitude at most 2^22 (so the layer-1 sums of up to 510 sites
+    are exact 32-bit integers)."""
+    edge = np.array([0.0, 2.0 ** -9, -(2.0 ** -9), 2.0 ** -9 * (1 - 2.0 ** -9),
+                     2.0 ** -9 * (1 - 2.0 ** -7), 2.0 ** -10, 2.0 ** -24, 1e-30, 63.9, 64.0,
+                     64.2, 64.4, -65.0, 1e6, -1e6, 0.3, -0.77], dtype=np.float32)
+    rng = np.random.default_rng(5)
+    v = np.concatenate([edge] + [rng.normal(scale=s, size=4000).astype(np.float32)
+                                 for s in (1e-3, 1e-2, 0.1, 1.0, 30.0, 300.0)])
+    g = mlp_l1_grid(v)
+    np.testing.assert_array_equal(g.view(np.uint32), orc.mlp_l1_grid(v).view(np.uint32))
+    np.testing.assert_array_equal(g[np.abs(g) > 0], bf16_flush(v)[np.abs(g) > 0].clip(-64, 64))
+    assert np.all((np.abs(g) >= 2.0 ** -9) | (g == 0)) and np.abs(g).max() == 64.0
+    q = g.astype(np.float64) * 65536.0
+    assert np.all(q == np.round(q)) and np.abs(q).max() <= 2.0 ** 22
+    assert g[1] == 2.0 ** -9 and g[2] == -(2.0 ** -9) and g[5] == 0.0 and g[10] == 64.0
+    # rounding to BF16 happens before the flush: 2^-9 (1 - 2^-9) rounds up to 2^-9
+    assert g[3] == 2.0 ** -9 and g[4] == 0.0
+
+
+def test_integer_route_of_the_activation_is_the_tabulated_tanh(orc):
+    """The GEMM kernels build s(z) from a = min(|z| 2^16, 2^19): table entry
+    j = a >> 8 of (T_j, D_j / 256) (entry 2048: the saturated value, slope 0)
+    and s = fma(a & 255, D_j / 256, T_j) with the sign of z.  Every a up to
+    and past saturation, both signs, against the oracle's FP32 route
+    (t = min(256 |z|, 2048 - 2^-13), j = floor(t), fma(fract t, D_j, T_j))."""
+    T, D = orc.mlp_table()
+    s_sat = np.float32(np.longdouble(np.float32(2047.9998779296875) - np.float32(2047.0))
+                       * np.longdouble(D[2047]) + np.longdouble(T[2047]))
+    Ti = np.append(T, s_sat).astype(np.float32)
+    Di = np.append(D * np.float32(1.0 / 256.0), np.float32(0.0)).astype(np.float32)
+    a = np.arange(0, 2 ** 19 + 4096, dtype=np.int64)
+    ac = np.minimum(a, 2 ** 19)
+    j = ac >> 8
+    f = (ac & 255).astype(np.longdouble)
+    # |T| < 1 and f D_j / 256 a multiple of 2^-32 T_j: the longdouble sum is exact,
+    # one rounding to FP32 as the fma
+    v = (f * Di[j].astype(np.longdouble) + Ti[j].astype(np.longdouble)).astype(np.float32)
+    z = (a.astype(np.float64) / 65536.0).astype(np.float32)
+    assert np.all(z.astype(np.float64) * 65536.0 == a)
+    ref = orc.mlp_tanh(z)
+    np.testing.assert_array_equal(v.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal((-v).view(np.uint32), orc.mlp_tanh(-z).view(np.uint32))
+    assert s_sat == orc.mlp_tanh(np.float32(8.0)) == orc.mlp_tanh(np.float32(100.0))
 
 
 def _bf16(a):
@@ -49,8 +97,8 @@ def numpy_log_psi(g, configs):
     (c mod 16 over c / 16, then a butterfly over the 16 columns)."""
     T, D = mlp_tanh_table()
     x = np.atleast_2d(np.asarray(configs, dtype=np.float64))
-    W1 = bf16_flush(g.W1).astype(np.float64)
-    b1 = bf16_flush(g.b1).astype(np.float64)
+    W1 = mlp_l1_grid(g.W1).astype(np.float64)
+    b1 = mlp_l1_grid(g.b1).astype(np.float64)
     W2 = bf16_flush(g.W2).astype(np.float64)
     z1 = x @ W1.T + b1
     h = _bf16(_tanh_tab(z1.astype(np.float32), T, D))
