@@ -36,7 +36,7 @@
 // MLP_NT (2048): linear pieces of the tabulated tanh on [0, 8)
 #define MLP_TSCALE 256.0f   // pieces per unit of |z|
 #define MLP_GW 8            // wavefronts (32-row blocks) per GEMM workgroup
-#define MLP_KC 32           // K chunk of W2 staged in LDS
+#define MLP_KC 32           // K chunk of W2 staged in LDS (one barrier per chunk)
 #define MLP_RB 16           // rows per block (one MFMA row tile)
 #define MLP_BSTR (MLP_KC + 8)
 #define MLP_ZSCALE 65536.0  // z1 holds z x 2^16 (exact: layer-1 grid)
@@ -660,20 +660,8 @@ void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
                     *(const uint4 *)(P.mlp_W2 + (size_t)j * H1 + q * 8);
             }
         }
-        // the W1 columns of K steps 0 and 1 (H1 >= 64)
-        MlpWIn wa, wb;
-        mlp_w_load(wa, w1i, w1j, 8 * kh);
-        mlp_w_load(wb, w1i, w1j, 16 + 8 * kh);
-        __syncthreads();
-        uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
-        // one K step: every B fragment in flight while the activations are
-        // built, the W1 columns of the step two ahead requested, then the MFMAs
-        auto kstep = [&](const uint16_t *Bc, int k, MlpWIn &wv) {
-            mlp_bf16x8 bv[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-                bv[t] = *(const mlp_bf16x8 *)(Bc + (t * 32 + r32) * MLP_BSTR + (k % MLP_KC) +
-                                              8 * kh);
+        // the operand of step k: z of the row's walker and its move's W1 columns
+        auto a_of = [&](int k, const MlpWIn &wv) {
             int4 za, zb;
             if constexpr (ZLDS) {
                 za = *(const int4 *)(zl + k + 8 * kh);
@@ -682,12 +670,40 @@ void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
                 za = *(const int4 *)(zg + k + 8 * kh);
                 zb = *(const int4 *)(zg + k + 8 * kh + 4);
             }
-            const mlp_bf16x8 a = mlp_a_build8(za, zb, wv.i0, wv.i1, wv.j0, wv.j1, ci, cj, tabi);
-            // H1 is a multiple of 64: step k + 32 exists for both halves or neither
-            if (k + 32 < H1) mlp_w_load(wv, w1i, w1j, k + 32 + 8 * kh);
+            return mlp_a_build8(za, zb, wv.i0, wv.i1, wv.j0, wv.j1, ci, cj, tabi);
+        };
+        // W1 columns of steps 0 and 1 (H1 >= 64), the operand of step 0, then
+        // the stage of step 0 refilled with step 2's columns
+        MlpWIn wa, wb;
+        mlp_w_load(wa, w1i, w1j, 8 * kh);
+        mlp_w_load(wb, w1i, w1j, 16 + 8 * kh);
+        __syncthreads();
+        mlp_bf16x8 a = a_of(0, wa);
+        mlp_w_load(wa, w1i, w1j, 32 + 8 * kh);
+        uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
+        // one K step, software-pipelined within the wavefront: the MFMAs of step
+        // k interleaved with the build of step k + 1's operand (from the stage
+        // wn, filled two steps before) and the refill of wn with step k + 3's
+        // columns; branch-free (past the last step the indices are clamped to
+        // the last one, in bounds, and what they produce is not used)
+        auto kstep = [&](const uint16_t *Bc, int k, MlpWIn &wn) {
+            mlp_bf16x8 bv[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                bv[t] = *(const mlp_bf16x8 *)(Bc + (t * 32 + r32) * MLP_BSTR + (k % MLP_KC) +
+                                              8 * kh);
+            const mlp_bf16x8 an = a_of(min(k + 16, H1 - 16), wn);
+            mlp_w_load(wn, w1i, w1j, min(k + 48, H1 - 16) + 8 * kh);
 #pragma unroll
             for (int t = 0; t < NT; ++t)
                 acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv[t], acc[t], 0, 0, 0);
+            a = an;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);    // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);    // LDS reads
+                __builtin_amdgcn_sched_group_barrier(0x002, 12, 0);   // VALU
+            }
         };
         for (int c = 0; c < nchunks; ++c) {
             const uint16_t *Bc = (c & 1) ? Bs1 : Bs0;
@@ -704,8 +720,8 @@ void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
                     pre3 = *(const uint4 *)(src + (size_t)3 * jstep * H1);
             }
             static_assert(MLP_KC == 32, "two K steps per W2 chunk");
-            kstep(Bc, c * MLP_KC, wa);
-            kstep(Bc, c * MLP_KC + 16, wb);
+            kstep(Bc, c * MLP_KC, wb);
+            kstep(Bc, c * MLP_KC + 16, wa);
             if (more) {
                 uint16_t *dst = Bn + jq * MLP_BSTR + qq * 8;
                 if ((int)threadIdx.x < CH) *(uint4 *)dst = pre0;
