@@ -598,7 +598,7 @@ def kagome_mlp_section(args):
             "unit": "TFLOP/s", "frac": flop / args.mlp_steps / prop_s / 2.5e15,
             "work": "layer-2 FLOPs 2 * 512 * 256 per network row over the 16-row blocks the "
                     "propagation processed (padding rows included)",
-            "kernel": "k_mlp_gemm32<8> (the propagation of one branching step, timed "
+            "kernel": "k_mlp_gemm32<8, true> (the propagation of one branching step, timed "
                       "with HIP events around it, includes its row selection and scans)",
             "kernel_avg_ms": prop_s * 1e3,
             "traffic": prof["traffic_bytes_per_launch"] if prof else None,

@@ -115,10 +115,12 @@ typedef struct gfmc_problem_desc {
      *   log psi = b3 + sum_c w3_c s(b2_c + sum_k W2_ck h_k),
      *   h_k = bf16(s(b1_k + sum_n W1_kn x_n)),
      * s the tabulated tanh (2048 linear pieces on [0, 8), odd, saturated).
-     * W1, b1 and W2 are rounded to the BF16 grid (magnitudes below 2^-24
-     * flushed to 0), so the layer-1 sums are exact in FP64; layer 2 runs on
-     * MFMA (BF16 inputs, FP32 accumulation).  h1 a multiple of 64, h2 one
-     * of 32, 64, 128, 256. */
+     * W1 and b1 are rounded to the layer-1 grid (BF16, nearest even,
+     * magnitudes below 2^-9 flushed to 0 and above 64 clamped to +-64), so
+     * the layer-1 sums of up to 510 sites are exact 32-bit integers at scale
+     * 2^16; W2 to the BF16 grid (magnitudes below 2^-24 flushed to 0); layer
+     * 2 runs on MFMA (BF16 inputs, FP32 accumulation).  h1 a multiple of
+     * 64, h2 one of 32, 64, 128, 256; n_sites at most 510. */
     int32_t mlp_h1, mlp_h2;
     const float *mlp_W1;           /* [h1*N] row-major                          */
     const float *mlp_b1;           /* [h1]                                      */
@@ -502,7 +504,8 @@ int gfmc_adam_step(int64_t n, float *param, const float *grad, float *m, float *
  * No counterpart in the reference (see the RBM section above).  Parameters
  * travel as one flat FP32 vector [W1 (h1 x n_sites, row-major), b1 (h1),
  * W2 (h2 x h1, row-major), b2 (h2), w3 (h2), b3] of gfmc_mlp_n_params
- * elements; the problem holds W1, b1, W2 on the BF16 grid.  The gradient
+ * elements; the problem holds W1, b1 on the layer-1 grid and W2 on the BF16
+ * grid (as gfmc_problem_create rounds them).  The gradient
  * call synchronises the caller's stream with the host once (it sizes the
  * candidate rows) and grows a workspace the problem owns. */
 
@@ -522,8 +525,9 @@ int gfmc_mlp_counters(const gfmc_problem *p, int64_t *out);
 int gfmc_mlp_set_hop_cap(gfmc_problem *p, int32_t hop_cap);
 int gfmc_mlp_unfinished(const gfmc_problem *p, int64_t *out);
 /* The MLP guide's propagation workspace is the caller's (no counterpart in
- * the reference, whose guide evaluation is XLA's): per walker the exact FP64
- * layer-1 pre-activations, the candidate slots and the hold state, per hop
+ * the reference, whose guide evaluation is XLA's): per walker the exact
+ * layer-1 pre-activations (32-bit integers at scale 2^16), the candidate
+ * slots and the hold state, per hop
  * the 16-row block map and one FP32 log psi per row.  gfmc_mlp_work_bytes:
  * the bytes for n_walkers (-1 if not an MLP problem or too large);
  * gfmc_mlp_set_work: attach a 256-byte aligned device buffer of at least
@@ -537,8 +541,8 @@ int gfmc_mlp_set_work(gfmc_problem *p, void *work, int64_t bytes, int64_t n_walk
 /* h1*N + h1 + h2*h1 + 2*h2 + 1 of an MLP problem, -1 otherwise. */
 int64_t gfmc_mlp_n_params(const gfmc_problem *p);
 
-/* The problem's parameters from a flat device vector (W1, b1, W2 rounded to
- * BF16, nearest even, magnitudes below 2^-24 flushed to 0). */
+/* The problem's parameters from a flat device vector (W1, b1 rounded to the
+ * layer-1 grid, W2 to the BF16 grid, as gfmc_problem_create rounds them). */
 int gfmc_mlp_set_params(gfmc_problem *p, const float *flat, void *stream);
 
 /* Gradient of L = (1/n) sum_i (E_i - mean E)^2 over n packed samples, as
