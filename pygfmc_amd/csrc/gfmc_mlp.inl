@@ -572,10 +572,12 @@ void k_mlp_gemm(DevProblem P, const uint32_t *spins, MlpWs W, MlpTws T) {
 // (half the LDS traffic per row of k_mlp_gemm); a K step's B fragments are
 // all in flight while its activations are built.  W2 is staged as in
 // k_mlp_gemm (double-buffered 32-wide K chunks, one barrier per chunk).
-// Epilogue: per lane the sum over the column tiles t (column t 32 + l % 32)
-// of w3 s(u + b2), then a butterfly over the 32 lanes of a row set: every
-// row's value is the same sequence of operations wherever the row sits, so
-// log psi(x') as a candidate row equals log psi(x') as a configuration row.
+// The accumulators start at b2, so u + b2 leaves the MFMAs.  Epilogue: per
+// lane the sum over the column tiles t (column t 32 + l % 32) of w3 s(u + b2)
+// (fma), then a fixed tree over the 32 lanes of a row set (a lane swap, then
+// DPP): every row's value is the same sequence of operations wherever the
+// row sits, so log psi(x') as a candidate row equals log psi(x') as a
+// configuration row.
 #define MLP_GW2 8
 typedef __attribute__((ext_vector_type(16))) float mlp_f32x16;
 
@@ -646,11 +648,14 @@ void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
         }
         const int32_t *zg = W.z1 + (size_t)w * H1;          // !ZLDS: z in global memory
         const int32_t *zl = zs + (r32 >> 4) * H1;           // ZLDS: this lane's block's row
+        // the accumulators start at the layer-2 bias of their column (t 32 + l % 32)
         mlp_f32x16 acc[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
+        for (int t = 0; t < NT; ++t) {
+            const float b2c = P.mlp_b2[t * 32 + r32];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+            for (int r = 0; r < 16; ++r) acc[t][r] = b2c;
+        }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int idx = threadIdx.x + u * TPB;
@@ -733,32 +738,49 @@ void k_mlp_gemm32(DevProblem P, const uint32_t *spins, MlpWs W) {
             }
             __syncthreads();
         }
-        // epilogue: rows 8 (j / 4) + 4 kh + j % 4 of the tile
+        // epilogue: D register r holds row 8 (r / 4) + 4 kh + r % 4 of the tile,
+        // column t 32 + l % 32; per lane the sum over t of w3 s(u) (one fma each)
         float part[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) part[r] = 0.0f;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int col = t * 32 + r32;
-            const float bb = P.mlp_b2[col], ww = P.mlp_w3[col];
+            const float ww = P.mlp_w3[t * 32 + r32];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) part[r] = part[r] + ww * mlp_tanh(tab, acc[t][r] + bb);
+            for (int r = 0; r < 16; ++r)
+                part[r] = __builtin_fmaf(ww, mlp_tanh(tab, acc[t][r]), part[r]);
         }
+        // the sum over the 32 lanes of each half: registers 2q and 2q + 1 paired
+        // by v_permlane16_swap (row 0 of each half then holds lane l + lane
+        // l + 16 of register 2q, row 1 of register 2q + 1), then a DPP tree over
+        // the 16 lanes of the row (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+        // row_mirror): the same tree for every row of every tile, all in registers
+        float red[8];
 #pragma unroll
-        for (int d = 16; d >= 1; d >>= 1)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) part[r] = part[r] + __shfl_xor(part[r], d, WAVE);
+        for (int q = 0; q < 8; ++q) {
+            const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(part[2 * q]),
+                                                             __float_as_uint(part[2 * q + 1]),
+                                                             false, false);
+            float v = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+            v = v + dpp_get<0xB1, 0xf>(v);
+            v = v + dpp_get<0x4E, 0xf>(v);
+            v = v + dpp_get<0x141, 0xf>(v);
+            v = v + dpp_get<0x140, 0xf>(v);
+            red[q] = v;
+        }
         // the two blocks' row offsets and counts (held by lanes 0 and 16)
         const int r0a = __shfl(r0, 0, WAVE), cnta = __shfl(cnt, 0, WAVE);
         const int r0b = __shfl(r0, 16, WAVE), cntb = __shfl(cnt, 16, WAVE);
-        if (r32 == 0) {
+        if ((r32 & 15) == 0) {
+            const int odd = r32 >> 4;    // lane 16 of the half holds the odd registers
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
+            for (int q = 0; q < 8; ++q) {
+                const int r = 2 * q + odd;
                 const int tr = 8 * (r >> 2) + 4 * kh + (r & 3);   // row of the tile
                 const int hb = tr >> 4, i = tr & 15;
                 const int bb = 2 * pb + hb;
                 if (bb < total && (hb ? r0b : r0a) + i < (hb ? cntb : cnta))
-                    W.logpsi[(size_t)bb * MLP_RB + i] = part[r] + P.mlp_b3p[0];
+                    W.logpsi[(size_t)bb * MLP_RB + i] = red[q] + P.mlp_b3p[0];
             }
         }
     }
