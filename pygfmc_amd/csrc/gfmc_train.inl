@@ -485,19 +485,23 @@ __device__ double sr_block_sum(double v, double *sh) {
     return t;
 }
 
-// y = (S + eps_rel diag S + eps_abs) x: a wavefront per row, lanes over the
-// columns (coalesced), a butterfly per row (a fixed order)
+// y = (S + eps_rel diag S + eps_abs) x with S symmetric: y_r = sum_c S[c][r]
+// x_c, so thread r reads row c of S at column r (coalesced, every load
+// independent of the running sum's); the threads in two halves over the
+// columns [0, ceil(m / 2)) and the rest, each summing its columns in
+// ascending order, the halves' sums (part, 2 m doubles) added in that order
 __device__ void sr_apply(int m, const double *S, double eps_rel, double eps_abs, const double *x,
-                         double *y) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    for (int r = wv; r < m; r += nwv) {
-        const double *row = S + (int64_t)r * m;
+                         double *y, double *part) {
+    const int hb = blockDim.x >> 1, half = threadIdx.x >= hb ? 1 : 0, lr = threadIdx.x - half * hb;
+    const int cm = (m + 1) >> 1, c0 = half ? cm : 0, c1 = half ? m : cm;
+    for (int r = lr; r < m; r += hb) {
         double acc = 0.0;
-        for (int c = lane; c < m; c += 64) acc += row[c] * x[c];
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, WAVE);
-        if (lane == 0) y[r] = acc + (eps_rel * row[r] + eps_abs) * x[r];
+        for (int c = c0; c < c1; ++c) acc += S[(int64_t)c * m + r] * x[c];
+        part[(int64_t)half * m + r] = acc;
     }
+    __syncthreads();
+    for (int r = threadIdx.x; r < m; r += blockDim.x)
+        y[r] = (part[r] + part[m + r]) + (eps_rel * S[(int64_t)r * m + r] + eps_abs) * x[r];
     __syncthreads();
 }
 
@@ -506,11 +510,11 @@ __device__ void sr_apply(int m, const double *S, double eps_rel, double eps_abs,
 __global__ __launch_bounds__(1024) void k_sr_cg(int m, const double *S, double eps_rel,
                                                double eps_abs, const float *g, int max_iter,
                                                double tol, double max_norm, int warm, double *x,
-                                               double *r, double *pv, double *Ap,
+                                               double *r, double *pv, double *Ap, double *part,
                                                float *delta) {
     __shared__ double sh[16];
     if (warm) {
-        sr_apply(m, S, eps_rel, eps_abs, x, Ap);
+        sr_apply(m, S, eps_rel, eps_abs, x, Ap, part);
         for (int q = threadIdx.x; q < m; q += blockDim.x) {
             r[q] = (double)g[q] - Ap[q];
             pv[q] = r[q];
@@ -528,7 +532,7 @@ __global__ __launch_bounds__(1024) void k_sr_cg(int m, const double *S, double e
     rr = sr_block_sum(rr, sh);
     const double stop = tol * tol * rr;
     for (int it = 0; it < max_iter && rr > stop && rr > 0.0; ++it) {
-        sr_apply(m, S, eps_rel, eps_abs, pv, Ap);
+        sr_apply(m, S, eps_rel, eps_abs, pv, Ap, part);
         double pAp = 0.0;
         for (int q = threadIdx.x; q < m; q += blockDim.x) pAp += pv[q] * Ap[q];
         pAp = sr_block_sum(pAp, sh);
@@ -550,7 +554,7 @@ __global__ __launch_bounds__(1024) void k_sr_cg(int m, const double *S, double e
     // sqrt(x^T S x), at most max_norm (0: no limit)
     double scale = 1.0;
     if (max_norm > 0.0) {
-        sr_apply(m, S, 0.0, 0.0, x, Ap);
+        sr_apply(m, S, 0.0, 0.0, x, Ap, part);
         double xSx = 0.0;
         for (int q = threadIdx.x; q < m; q += blockDim.x) xSx += x[q] * Ap[q];
         xSx = sr_block_sum(xSx, sh);
@@ -722,7 +726,7 @@ int gfmc_sr_solve(int64_t n, int32_t m, const float *O, const float *g, double e
     hipLaunchKernelGGL(k_sr_cov, dim3(tiles, tiles), dim3(SR_T * SR_T), 0, st, n, m, O, mean, S);
     hipLaunchKernelGGL(k_sr_cg, dim3(1), dim3(1024), 0, st, m, (const double *)S, eps_rel,
                        eps_abs, g, max_iter, tol, max_norm, warm_start ? 1 : 0, x, r, pv, Ap,
-                       delta);
+                       part, delta);
     HIP_TRY(hipGetLastError());
     return 0;
 }
