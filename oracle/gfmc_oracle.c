@@ -719,7 +719,10 @@ static void propagate_one(const gfmc_problem_desc *d, int64_t widx, uint32_t *sp
             draw2(seed, it, (uint32_t)widx, step, tag, &U1, &psi);
             int idx = -1;
             double tau_raw = INFINITY;
-            if (R > 0) {
+            /* a total that overflowed takes no hop (the kernels' hop_rate):
+             * the walker holds for the rest of the step, its log weight
+             * becomes non-finite; a zero hold time would never end the step */
+            if (R > 0 && isfinite(R)) {
                 double u = U1 * R, cap = nextafter(R, 0.0);
                 if (u > cap) u = cap;
                 idx = orc_sample_blocked(w, n_off, incl, u);

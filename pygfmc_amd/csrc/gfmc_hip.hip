@@ -320,6 +320,17 @@ __device__ __forceinline__ double below(double r) {
 }
 __device__ __forceinline__ float below(float r) { return __int_as_float(__float_as_int(r) - 1); }
 
+// a total rate that starts a hop: positive and finite.  A total that
+// overflowed (a guide ratio beyond the range of T) takes no hop: the walker
+// holds for the rest of the step, its log weight becomes non-finite (-e_L
+// = +inf times the remaining time) and the driver's population check
+// raises; a hop would have a zero hold time and the step would never end.
+// NaN fails the comparisons as well.
+template <typename T>
+__device__ __forceinline__ bool hop_rate(T R) {
+    return R > (T)0 && __builtin_isfinite(R);
+}
+
 template <typename T>
 __device__ __forceinline__ T uniform_for(gfmc_u4 r);
 template <>
@@ -1005,7 +1016,7 @@ __device__ __forceinline__ void propagate_walker(const DevProblem &P, const Tabl
             int idx = -1;
             double tau_raw = INFINITY;
             MovePrefetch m;
-            if (R > (T)0) {
+            if (hop_rate(R)) {
                 T u = uniform_for<T>(r) * R;
                 T cap = below(R);
                 if (u > cap) u = cap;
@@ -1481,7 +1492,7 @@ __global__ __launch_bounds__(RBM_WAVES * WAVE) void k_propagate_rbm(
             const double psi = gfmc_u53(r.z, r.w);
             int idx = -1;
             double tau_raw = INFINITY;
-            if (R > 0.0f) {
+            if (hop_rate(R)) {
                 float u = gfmc_u24(r.x) * R;
                 const float cap = below(R);
                 if (u > cap) u = cap;

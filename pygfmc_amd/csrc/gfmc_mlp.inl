@@ -834,7 +834,7 @@ __global__ __launch_bounds__(MLP_SW * WAVE) void k_mlp_select(
         const double psi = gfmc_u53(r.z, r.w);
         int idx = -1;
         double tau_raw = INFINITY;
-        if (R > 0.0f) {
+        if (hop_rate(R)) {
             float u = gfmc_u24(r.x) * R;
             const float cap = below(R);
             if (u > cap) u = cap;

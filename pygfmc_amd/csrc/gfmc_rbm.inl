@@ -330,7 +330,7 @@ __global__ __launch_bounds__(RBH_WAVES * WAVE) void k_propagate_rbh(
             const double psi = gfmc_u53(r.z, r.w);
             int idx = -1;
             double tau_raw = INFINITY;
-            if (R > 0.0f) {
+            if (hop_rate(R)) {
                 float u = gfmc_u24(r.x) * R;
                 const float cap = below(R);
                 if (u > cap) u = cap;
@@ -843,7 +843,7 @@ __global__ __launch_bounds__(1024) void k_propagate_rbt(
             const double psi = gfmc_u53(r.z, r.w);
             int idx = -1;
             double tau_raw = INFINITY;
-            if (R > 0.0f) {
+            if (hop_rate(R)) {
                 float u = gfmc_u24(r.x) * R;
                 const float cap = below(R);
                 if (u > cap) u = cap;

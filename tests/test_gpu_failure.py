@@ -1,8 +1,11 @@
 """Failure handling forced from the host: an extinct population is an error
 that names the step (the reference's
-tests/core/test_positivity_and_zero_weight.py:101), and a discrete sub-step
+tests/core/test_positivity_and_zero_weight.py:101), a discrete sub-step
 whose total weight lam - H_xx + R is zero keeps the walker (its weight
-becomes zero) instead of drawing a slot that has no weight."""
+becomes zero) instead of drawing a slot that has no weight, and a total
+rate that overflows (a guide ratio beyond the rate precision) ends the
+walker's step with a non-finite weight, which the run reports, instead of
+hopping with zero hold time forever."""
 import sys
 
 import numpy as np
@@ -93,3 +96,48 @@ def test_single_step_zero_total_with_slots_stays(orc):
     fin = np.isfinite(ref[1])
     np.testing.assert_array_equal(np.isfinite(got[1]), fin)
     np.testing.assert_allclose(got[1][fin], ref[1][fin], rtol=1e-12, atol=1e-12)
+
+
+def test_overflowing_total_rate_ends_the_step_like_the_oracle(orc):
+    """Jastrow beta = 200 on a ring (FP64): walkers with a move of ratio
+    e^800 have total rate inf, take no hop and end the step with log weight
+    +inf, as in the oracle; every walker's spins, log weight and evaluation
+    count equal the oracle's."""
+    from pygfmc_amd import JastrowSpinHalf
+    say("overflowing total rate, fp64")
+    L = chain(8)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    G = JastrowSpinHalf(8, L.edges, beta=200.0, dtype=np.float64)
+    p = Problem(H, G, _lib.GFMC_FP64)
+    spins = orc.init_sz(8, 64, 4, 7)
+    nw = spins.shape[0]
+    s = torch.as_tensor(spins.view(np.int32), device="cuda")
+    lw = torch.zeros(nw, dtype=torch.float64, device="cuda")
+    e = torch.zeros(nw, dtype=torch.float64, device="cuda")
+    ev = torch.zeros(nw, dtype=torch.int32, device="cuda")
+    p.propagate(s, lw, e, 0.2, 0.0, 5, 1, 0, ev)
+    torch.cuda.synchronize()
+    ref = orc.Problem(lower(H, G)).propagate(spins, np.zeros(nw), np.zeros(nw), 0.2, 0.0, 5, 1, 0)
+    np.testing.assert_array_equal(s.cpu().numpy().view(np.uint32), ref[0])
+    np.testing.assert_array_equal(ev.cpu().numpy(), ref[3])
+    got = lw.cpu().numpy()
+    np.testing.assert_array_equal(np.isinf(got), np.isinf(ref[1]))
+    assert np.isinf(got).any() and np.isfinite(got).any()
+    fin = np.isfinite(ref[1])
+    np.testing.assert_allclose(got[fin], ref[1][fin], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision,beta", [(np.float64, 200.0), (np.float32, 30.0)],
+                         ids=["fp64", "fp32"])
+def test_guide_ratio_overflow_is_an_error_not_a_hang(precision, beta):
+    """A guide whose ratios overflow the rate precision: the run raises
+    (the population check) instead of stepping forever."""
+    from pygfmc_amd import JastrowSpinHalf, prepare_walkers_sz, run
+    say("ratio overflow", np.dtype(precision).name)
+    L = chain(8)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=precision)
+    G = JastrowSpinHalf(8, L.edges, beta=beta, dtype=precision)
+    w = prepare_walkers_sz(256, 8, 1)
+    with pytest.raises(ValueError, match="population extinct|non-finite energy"):
+        run(w, 20, H, 0.2, G, [2], rng_key=0, warmup_steps=0, avg_log_weight=0.0,
+            verbosity=0)

@@ -597,3 +597,33 @@ def test_oracle_propagation_of_a_sample_equals_the_full_population(orc):
     other = prob.propagate_ids(ids + 1, spins[ids], lw0[ids], np.zeros(len(ids)), 0.3, 0.1, 77,
                                5, 0)
     assert not np.array_equal(other[1], part[1])
+
+
+@pytest.mark.timeout(60)
+def test_overflowing_total_rate_ends_the_step_without_a_hop(orc):
+    """A guide whose ratios overflow FP64 (Jastrow beta = 200 on a ring:
+    pair moves with both outer bonds aligned have ratio e^800) makes some
+    walkers' total rate inf: such a walker takes no hop (a zero hold time
+    would never end the step) and its log weight becomes +inf; the others
+    step (some end finite, some reach such a configuration by a hop)."""
+    from pygfmc_amd import JastrowSpinHalf, SpinHalf, chain
+    L = chain(8)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float64)
+    G = JastrowSpinHalf(8, L.edges, beta=200.0, dtype=np.float64)
+    spins = orc.init_sz(8, 64, 4, 7)
+    p = orc.Problem(lower(H, G))
+    out_s, lw, e, ev = p.propagate(spins, np.zeros(64), np.zeros(64), 0.2, 0.0, 5, 1, 0)
+    x = orc.unpack(spins, 8)
+    # a ring site pair (i, i+1) flipped with both outer bonds aligned changes
+    # log psi by 4 beta = 800: the total rate overflows
+    big = np.zeros(64, dtype=bool)
+    for w in range(64):
+        for i in range(8):
+            a, b, c, d = x[w, (i - 1) % 8], x[w, i], x[w, (i + 1) % 8], x[w, (i + 2) % 8]
+            if b != c and a * b > 0 and c * d > 0:
+                big[w] = True
+    assert big.any() and (~big).any()
+    np.testing.assert_array_equal(lw[big], np.inf)
+    np.testing.assert_array_equal(out_s[big], spins[big])      # no hop
+    np.testing.assert_array_equal(ev[big], 1)                  # one evaluation
+    assert np.isfinite(lw[~big]).any()
