@@ -331,6 +331,12 @@ def train_variance(hamiltonian, rbm, walkers, *, n_iter=200, steps_per_iter=4, s
     torch.cuda.synchronize()
     secs = time.time() - t0
     h = hist[:n_iter].cpu().numpy()
+    bad = np.flatnonzero(~np.isfinite(h).all(axis=1))
+    if bad.size:
+        raise ValueError(
+            f"[train] the local-energy statistics are not finite from iteration {int(bad[0])} "
+            "on: the guide diverged (a ratio overflowed the rate precision, or its samples "
+            "died out); lower the learning rate or the trust region (sr_max_norm)")
     history = dict(mean_e=h[:, 0].copy(), variance=h[:, 1].copy(), seconds=secs,
                    walker_steps=walkers.n_walkers * (warmup_steps + n_iter * steps_per_iter),
                    trainer=tr)

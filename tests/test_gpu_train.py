@@ -193,6 +193,22 @@ def test_training_shrinks_variance_and_beats_the_plain_error_bar():
     assert res["trained"][1] < 0.5 * res["rbm0"][1]
 
 
+def test_training_a_guide_whose_ratios_overflow_is_an_error():
+    """An RBM with weights of magnitude 50 on 4x4: its ratios overflow FP32,
+    the sampling walkers' weights become non-finite (their steps end without
+    a hop) and train_variance reports the divergence instead of returning a
+    guide trained on garbage (or stepping forever)."""
+    say("training a diverged guide")
+    L = square(4, 4)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    rng = np.random.default_rng(2)
+    rbm = RBMSpinHalf(16, alpha=1, W=50.0 * rng.choice([-1.0, 1.0], size=(16, 16)))
+    w = prepare_walkers_sz(256, 16, 3)
+    with pytest.raises(ValueError, match="not finite from iteration"):
+        train_variance(H, rbm, w, n_iter=3, steps_per_iter=2, lr=0.02, rng_key=5,
+                       warmup_steps=2)
+
+
 def test_tied_rbm_natural_gradient_pieces_match_numpy(orc):
     """The orbit-summed log-derivatives of a translation-invariant RBM
     against the oracle's FP64 theta, the orbit sum and update, and the
