@@ -730,7 +730,10 @@ def sigma_race(args):
             speedup=J["gpu_s"] / t_need, speedup_with_training=J["gpu_s"] / (t_need + train_s),
             energy_z=float(z),
             per_step_variance_ratio=(J["sigma"] ** 2 * J["steps"]) / (err ** 2 * rbm["steps"]),
-            per_step_cost_ratio=rbm["ms_per_step"] / J["ms_per_step"])
+            per_step_cost_ratio=rbm["ms_per_step"] / J["ms_per_step"],
+            # each sigma^2 from n jackknife blocks carries a relative error of
+            # about sqrt(2 / n); the speed-up is their ratio
+            speedup_rel_err=float(np.sqrt(2.0 / (J["steps"] // 200) + 2.0 / (rbm["steps"] // 200))))
     return out
 
 
