@@ -209,6 +209,33 @@ def test_training_a_guide_whose_ratios_overflow_is_an_error():
                        warmup_steps=2)
 
 
+def test_filter_support_stays_zero_through_natural_gradient_training():
+    """A tied RBM whose filters exist on a 3 x 3 window (from_jastrow with
+    a support): natural-gradient training moves the window's entries and
+    leaves every entry outside it at exactly 0 in every hidden unit."""
+    say("filter support training")
+    from pygfmc_amd import RBMSpinHalf
+    L = square(4, 4)
+    H = SpinHalf.create(L.edges, -1.0, -1.0, 1.0, dtype=np.float32)
+    x, y = np.arange(16) % 4, np.arange(16) // 4
+    dx, dy = np.minimum(x, 4 - x), np.minimum(y, 4 - y)
+    sup = np.maximum(dx, dy) <= 1
+    rbm0 = RBMSpinHalf.from_jastrow(L.edges, L.extras["translations"], 0.15, 2, seed=1,
+                                    scale=0.01, support=sup)
+    w = prepare_walkers_sz(1024, 16, 3)
+    G, hist = train_variance(H, rbm0, w, n_iter=40, steps_per_iter=2, lr=1.0, rng_key=3,
+                             optimizer="sr", sr_max_norm=0.05, warmup_steps=20)
+    assert np.all(np.isfinite(hist["variance"]))
+    T = L.extras["translations"]
+    moved = 0
+    for f in range(2):
+        for r in range(16):
+            row0, row = rbm0.W[f * 16 + r], G.W[f * 16 + r]
+            assert np.all(row[T[r][~sup]] == 0.0)
+            moved += int(np.any(row[T[r][sup]] != row0[T[r][sup]]))
+    assert moved == 32
+
+
 def test_tied_rbm_natural_gradient_pieces_match_numpy(orc):
     """The orbit-summed log-derivatives of a translation-invariant RBM
     against the oracle's FP64 theta, the orbit sum and update, and the
