@@ -740,11 +740,15 @@ __device__ int rbt_eval_all(const DevProblem &P, const Tables<float> &tb, const 
         v += dpp_get<0x112, 0xf>(v);
         v += dpp_get<0x114, 0xf>(v);
         v += dpp_get<0x118, 0xf>(v);
-        if ((lane & 15) == 15) {
-            const int a = L.ent[t + mine].x;
-            const float w = __expf(L.g[t + mine] + 0.69314718055994531f * v);
-            L.w[a & 0xffffff] = (a & RBH_POS_BARE) ? -w : w;
-        }
+        // the slot's log weight in place of its field term; the weights are
+        // made after the loop, 64 slots per instruction instead of 4
+        if ((lane & 15) == 15) L.g[t + mine] = L.g[t + mine] + 0.69314718055994531f * v;
+    }
+    wave_sync();
+    for (int k = lane; k < n_act; k += WAVE) {
+        const int a = L.ent[k].x;
+        const float w = __expf(L.g[k]);
+        L.w[a & 0xffffff] = (a & RBH_POS_BARE) ? -w : w;
     }
     int cnt = 0;
     if (P.has_jz) {
